@@ -1,0 +1,281 @@
+"""GraphKokkos's masked-product entry points and drivers, on the GPU.
+
+Names, argument meaning and error behaviour follow the reference:
+
+* ``mxmm_msa`` / ``mxmm_mca`` (include/matrix_la.h:87-123, 478-491, 562-574):
+  ``C<M> = A*B``; C takes M's pattern, C.Val gets the products, explicit
+  zeros kept.  Both names run the same HIP engine (the accumulator is picked
+  per row); ``isParallel`` is accepted for signature compatibility.
+* ``triangle_counting_masked_sandia(A, matrixMult, isParallel)``
+  (src/main.cpp:69-90): ``sum(C<L> = L*L)`` with L the strict lower part.
+* ``k_truss(A, k, matrixMult, isParallel)`` (src/main.cpp:93-141): repeats
+  ``S<C> = C*C``, keeps ``S >= k-2``, until nothing is removed; returns the
+  pattern (values 1).
+* ``betweenness_centrality_batch(isParallel, A, batchSize)``
+  (include/betweenness_centrality.h:406-506): sources ``0..batch-1``, result
+  as float32 like the reference; ``bc_batch_f64`` takes an explicit source
+  list and returns FP64.
+
+Every compute call goes through libgraphhip.so; a missing library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Callable
+
+import numpy as np
+
+from . import _lib
+from ._lib import GkCsr, GkOpts, GkTiming, check, default_opts, lib
+from .graph import SpMtx, _ptr, extract_lower_triangle
+
+
+class DeviceCsr:
+    """A CSR resident in HBM (gk_dcsr)."""
+
+    def __init__(self, ctx: "Context", a: SpMtx):
+        self.ctx = ctx
+        self.m, self.n, self.nz = a.m, a.n, a.nz
+        h = C.c_void_p()
+        csr = a.gk()
+        check(lib().gk_dcsr_upload(ctx.handle, C.byref(csr), C.byref(h)), "gk_dcsr_upload")
+        self.handle = h
+
+    def free(self):
+        if self.handle:
+            lib().gk_dcsr_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class DeviceBuffer:
+    def __init__(self, ctx: "Context", nbytes: int):
+        self.ctx, self.nbytes = ctx, nbytes
+        p = C.c_void_p()
+        check(lib().gk_dmalloc(ctx.handle, nbytes, C.byref(p)), "gk_dmalloc")
+        self.ptr = p
+
+    def to_host(self, dtype, count: int) -> np.ndarray:
+        out = np.empty(count, dtype)
+        if count:
+            check(lib().gk_memcpy_d2h(self.ctx.handle, out.ctypes.data_as(C.c_void_p), self.ptr,
+                                      out.nbytes), "gk_memcpy_d2h")
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().gk_dfree(self.ctx.handle, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """gk_ctx: one device, one stream (counterpart of Kokkos::initialize)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(lib().gk_init(device, C.byref(h)), "gk_init")
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            lib().gk_finalize(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def sync(self):
+        check(lib().gk_sync(self.handle), "gk_sync")
+
+    def upload(self, a: SpMtx) -> DeviceCsr:
+        return DeviceCsr(self, a)
+
+    def alloc(self, nbytes: int) -> DeviceBuffer:
+        return DeviceBuffer(self, nbytes)
+
+    def last_timing(self) -> dict:
+        t = GkTiming()
+        check(lib().gk_last_timing(self.handle, C.byref(t)), "gk_last_timing")
+        return {"total_ms": t.total_ms, "kernel_ms": t.kernel_ms, "select_ms": t.select_ms,
+                "launches": t.launches, "iterations": t.iterations,
+                "rows_per_acc": list(t.rows_per_acc), "bytes_alg": t.bytes_alg}
+
+    @staticmethod
+    def _opts(opts: GkOpts | dict | None) -> GkOpts:
+        if opts is None:
+            return default_opts()
+        if isinstance(opts, dict):
+            return default_opts(**opts)
+        return opts
+
+    # ---- products -------------------------------------------------------
+    def masked_spgemm(self, A: SpMtx, B: SpMtx, M: SpMtx, opts=None) -> np.ndarray:
+        out = np.zeros(M.nz, np.int32)
+        a, b, m = A.gk(), B.gk(), M.gk(with_values=False)
+        o = self._opts(opts)
+        check(lib().gk_masked_spgemm_i32(self.handle, C.byref(a), C.byref(b), C.byref(m), _ptr(out),
+                                         C.byref(o)), "gk_masked_spgemm_i32")
+        return out
+
+    def masked_spgemm_dev(self, A: DeviceCsr, B: DeviceCsr, M: DeviceCsr, out: DeviceBuffer, opts=None) -> None:
+        o = self._opts(opts)
+        check(lib().gk_masked_spgemm_dev(self.handle, A.handle, B.handle, M.handle, out.ptr, C.byref(o)),
+              "gk_masked_spgemm_dev")
+
+    def tc_count(self, L: SpMtx | DeviceCsr, opts=None) -> int:
+        o = self._opts(opts)
+        cnt = C.c_uint64(0)
+        if isinstance(L, DeviceCsr):
+            check(lib().gk_tc_count_dev(self.handle, L.handle, C.byref(cnt), C.byref(o)), "gk_tc_count_dev")
+        else:
+            csr = L.gk(with_values=False)
+            check(lib().gk_tc_count(self.handle, C.byref(csr), C.byref(cnt), C.byref(o)), "gk_tc_count")
+        return int(cnt.value)
+
+    def tc_rows(self, L: DeviceCsr, opts=None) -> np.ndarray:
+        o = self._opts(opts)
+        buf = self.alloc(8 * max(L.m, 1))
+        check(lib().gk_tc_rows_dev(self.handle, L.handle, buf.ptr, C.byref(o)), "gk_tc_rows_dev")
+        return buf.to_host(np.uint64, L.m)
+
+    def ktruss_support(self, A: SpMtx, opts=None) -> np.ndarray:
+        o = self._opts(opts)
+        out = np.zeros(A.nz, np.uint32)
+        csr = A.gk(with_values=False)
+        check(lib().gk_ktruss_support(self.handle, C.byref(csr), _ptr(out, C.c_uint32), C.byref(o)),
+              "gk_ktruss_support")
+        return out
+
+    def ktruss(self, A: SpMtx, k: int, opts=None) -> tuple[SpMtx, int]:
+        o = self._opts(opts)
+        rst = np.zeros(A.m + 1, np.int32)
+        col = np.zeros(max(A.nz, 1), np.int32)
+        nz = C.c_int64(0)
+        it = C.c_int64(0)
+        csr = A.gk(with_values=False)
+        check(lib().gk_ktruss(self.handle, C.byref(csr), k, _ptr(rst), _ptr(col), C.byref(nz), C.byref(it),
+                              C.byref(o)), "gk_ktruss")
+        res = SpMtx(A.m, A.n, rst, col[:nz.value].copy(), np.ones(nz.value, np.int32), A.matcode)
+        return res, int(it.value)
+
+    def truss_decompose(self, A: SpMtx | DeviceCsr, opts=None) -> tuple[np.ndarray, int]:
+        o = self._opts(opts)
+        kmax = C.c_int32(0)
+        if isinstance(A, DeviceCsr):
+            buf = self.alloc(4 * max(A.nz, 1))
+            check(lib().gk_truss_decompose_dev(self.handle, A.handle, buf.ptr, C.byref(kmax), C.byref(o)),
+                  "gk_truss_decompose_dev")
+            return buf.to_host(np.uint32, A.nz), int(kmax.value)
+        out = np.zeros(A.nz, np.uint32)
+        csr = A.gk(with_values=False)
+        check(lib().gk_truss_decompose(self.handle, C.byref(csr), _ptr(out, C.c_uint32), C.byref(kmax),
+                                       C.byref(o)), "gk_truss_decompose")
+        return out, int(kmax.value)
+
+    def bc_batch_f64(self, A: SpMtx, sources, opts=None) -> np.ndarray:
+        o = self._opts(opts)
+        src = np.ascontiguousarray(sources, np.int32)
+        out = np.zeros(A.m, np.float64)
+        csr = A.gk(with_values=False)
+        check(lib().gk_bc_batch_f64(self.handle, C.byref(csr), _ptr(src), len(src), _ptr(out, C.c_double),
+                                    C.byref(o)), "gk_bc_batch_f64")
+        return out
+
+    def bc_batch_dev(self, A: DeviceCsr, sources, out: DeviceBuffer, AT: DeviceCsr | None = None,
+                     opts=None) -> None:
+        o = self._opts(opts)
+        src = np.ascontiguousarray(sources, np.int32)
+        check(lib().gk_bc_batch_dev(self.handle, A.handle, AT.handle if AT else None, _ptr(src), len(src),
+                                    out.ptr, C.byref(o)), "gk_bc_batch_dev")
+
+
+_default: Context | None = None
+
+
+def default_context() -> Context:
+    global _default
+    if _default is None:
+        _default = Context(0)
+    return _default
+
+
+# ---- reference-named entry points ------------------------------------------
+
+def _product_into(A: SpMtx, B: SpMtx, M: SpMtx, C_: SpMtx) -> None:
+    if A.n != B.m:
+        raise ValueError("A.n must equal B.m")
+    vals = default_context().masked_spgemm(A, B, M)
+    # C.resizeRows(M.m); C.resizeVals(M.nz); C.n = M.n; copy M's pattern
+    C_.m, C_.n = M.m, M.n
+    C_.Rst = M.Rst.copy()
+    C_.Col = M.Col.copy()
+    C_.Val = vals
+    C_.matcode = M.matcode
+
+
+def mxmm_msa(isParallel: bool, A: SpMtx, B: SpMtx, M: SpMtx, C_: SpMtx) -> None:
+    """C<M> = A*B (include/matrix_la.h:562-574)."""
+    _product_into(A, B, M, C_)
+
+
+def mxmm_mca(isParallel: bool, A: SpMtx, B: SpMtx, M: SpMtx, C_: SpMtx) -> None:
+    """C<M> = A*B (include/matrix_la.h:478-491)."""
+    _product_into(A, B, M, C_)
+
+
+MxmOp = Callable[[bool, SpMtx, SpMtx, SpMtx, SpMtx], None]
+
+
+def triangle_counting_masked_sandia(A: SpMtx, matrixMult: MxmOp = mxmm_msa, isParallel: bool = True) -> int:
+    """sum(C<L> = L*L), L = extract_lower_triangle(A) (src/main.cpp:69-90)."""
+    L = extract_lower_triangle(A)
+    if matrixMult in (mxmm_msa, mxmm_mca):
+        return default_context().tc_count(L)
+    C_ = SpMtx()
+    matrixMult(isParallel, L, L, L, C_)
+    return int(np.asarray(C_.Val, np.int64).sum())
+
+
+def k_truss(A: SpMtx, k: int, matrixMult: MxmOp = mxmm_msa, isParallel: bool = True) -> SpMtx:
+    """k-truss pattern of the symmetric A (src/main.cpp:93-141)."""
+    if matrixMult in (mxmm_msa, mxmm_mca):
+        res, _ = default_context().ktruss(A, k)
+        return res
+    # An arbitrary product: the reference's loop, compaction on the host.
+    Cm = A
+    for _ in range(max(A.m, 1)):
+        T = SpMtx()
+        matrixMult(isParallel, Cm, Cm, Cm, T)
+        keep = np.asarray(T.Val) >= k - 2
+        rows = np.repeat(np.arange(Cm.m), np.diff(Cm.Rst))
+        rst = np.zeros(Cm.m + 1, np.int64)
+        np.cumsum(np.bincount(rows[keep], minlength=Cm.m), out=rst[1:])
+        nxt = SpMtx(Cm.m, Cm.n, rst.astype(np.int32), Cm.Col[keep], np.ones(int(keep.sum()), np.int32),
+                    Cm.matcode)
+        if nxt.nz == Cm.nz:
+            return nxt
+        Cm = nxt
+    return Cm
+
+
+def betweenness_centrality_batch(isParallel: bool, A: SpMtx, batchSize: int) -> np.ndarray:
+    """BC over sources 0..min(n, batchSize)-1, float32 result
+    (include/betweenness_centrality.h:406-506)."""
+    if A.m != A.n:
+        raise ValueError("non-square matrix; BC is only for square matrices")
+    n = min(A.n, int(batchSize))
+    return default_context().bc_batch_f64(A, np.arange(n, dtype=np.int32)).astype(np.float32)

@@ -1,0 +1,324 @@
+// Batched Brandes betweenness centrality on gfx950, sigma and delta in FP64.
+//
+// GraphKokkos's betweenness_centrality_batch (include/betweenness_centrality.h:406-506)
+// keeps a dense n x batch matrix of path counts (Numsp), the frontier of
+// every level as a sparse n x batch matrix (Sigmas[d]), advances the
+// frontier with a complement-masked product Front' = (A^T * Front) .* !Numsp
+// (bc_forward_step, :313-401) and accumulates dependencies backwards with
+// W = Nspinv .* Bcu on level d+1 and Bcu += (A * W) .* Numsp on level d
+// (bc_backward_step, :182-242); bc(v) = sum_j Bcu(v,j) - batch.
+//
+// Layout here, for a batch of up to 256 sources (one bit / one column each):
+//   S[v][256]  FP64   sigma(v,s) while going forward; overwritten by
+//                     W(v,s) = (1/sigma) * (1 + delta) once v's level for s
+//                     has been processed backwards
+//   vis[v][4]  u64    sources that have reached v
+//   F_d[v][4]  u64    sources for which v is at BFS level d (one per level)
+//   bc[v]      FP64   sum over sources of delta(v,s)
+// One wave per vertex; lane l owns sources 4l..4l+3, so a vertex's 256
+// sigmas (2 KiB) are read by one coalesced wave access, and a neighbour's
+// frontier bits cost one 8-byte word per 16 lanes.
+// Forward (pull over A^T): sigma(i,s) = sum over in-neighbours k with
+//   k in F_d for s, for s not yet in vis[i].
+// Backward (pull over A): for s with i in F_d,
+//   delta(i,s) = sigma(i,s) * sum over out-neighbours w in F_{d+1} for s of W(w,s).
+#include <cstring>
+
+#include "gk_device.h"
+
+using namespace gk;
+
+namespace gk {
+
+int exclusive_scan_i32(gk_ctx *ctx, const int32_t *in, int32_t *out, int64_t n);
+
+namespace {
+
+constexpr int kSrc = 256;
+
+__device__ __forceinline__ uint32_t nib_of(const uint64_t *row, int lane) {
+    return (uint32_t)(row[lane >> 4] >> ((lane & 15) * 4)) & 0xFu;
+}
+
+// Combine the lanes' nibbles of a wave into the 4 words of a row.
+__device__ __forceinline__ uint64_t gather_word(uint32_t nib, int lane) {
+    uint64_t w = (uint64_t)nib << ((lane & 15) * 4);
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) w |= __shfl_xor(w, o, 16);
+    return w;
+}
+
+__global__ __launch_bounds__(256) void k_bc_init(int64_t n, uint64_t *__restrict__ vis, uint64_t *__restrict__ f0,
+                                                 double *__restrict__ bc, int zero_bc) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < 4 * n; i += (int64_t)gridDim.x * blockDim.x) {
+        vis[i] = 0;
+        f0[i] = 0;
+        if (zero_bc && i < n) bc[i] = 0;
+    }
+}
+
+__global__ void k_bc_sources(const int32_t *__restrict__ src, int nsrc, uint64_t *__restrict__ vis,
+                             uint64_t *__restrict__ f0, double *__restrict__ S) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nsrc) {
+        int64_t v = src[s];
+        atomicOr((unsigned long long *)&vis[4 * v + (s >> 6)], 1ull << (s & 63));
+        atomicOr((unsigned long long *)&f0[4 * v + (s >> 6)], 1ull << (s & 63));
+        S[v * kSrc + s] = 1.0;
+    }
+}
+
+// Level d -> d+1.  cnt accumulates the number of (vertex, source) pairs
+// reached.
+__global__ __launch_bounds__(256) void k_bc_forward(int64_t n, const int32_t *__restrict__ trst,
+                                                    const int32_t *__restrict__ tcol,
+                                                    const uint64_t *__restrict__ fd, uint64_t *__restrict__ fn,
+                                                    uint64_t *__restrict__ vis, double *__restrict__ S,
+                                                    unsigned long long *__restrict__ cnt) {
+    const int lane = threadIdx.x & 63;
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    uint32_t reached = 0;
+    for (int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < n; i += waves) {
+        const uint32_t unv = ~nib_of(vis + 4 * i, lane) & 0xFu;
+        uint32_t got = 0;
+        if (__ballot(unv != 0)) {
+            double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+            const int32_t t1 = trst[i + 1];
+            for (int32_t t = trst[i]; t < t1; ++t) {
+                const int64_t k = tcol[t];
+                const uint32_t f = nib_of(fd + 4 * k, lane) & unv;
+                if (f) {
+                    const double *sk = S + k * kSrc + 4 * lane;
+                    if (f & 1) a0 += sk[0];
+                    if (f & 2) a1 += sk[1];
+                    if (f & 4) a2 += sk[2];
+                    if (f & 8) a3 += sk[3];
+                }
+            }
+            double *si = S + i * kSrc + 4 * lane;
+            if ((unv & 1) && a0 != 0) { si[0] = a0; got |= 1; }
+            if ((unv & 2) && a1 != 0) { si[1] = a1; got |= 2; }
+            if ((unv & 4) && a2 != 0) { si[2] = a2; got |= 4; }
+            if ((unv & 8) && a3 != 0) { si[3] = a3; got |= 8; }
+        }
+        const uint64_t w = gather_word(got, lane);
+        if ((lane & 15) == 0) {
+            fn[4 * i + (lane >> 4)] = w;
+            if (w) vis[4 * i + (lane >> 4)] |= w;
+        }
+        reached += __popc(got);
+    }
+    unsigned long long r = dev::wave_sum((unsigned long long)reached);
+    if (lane == 0 && r) atomicAdd(cnt, r);
+}
+
+// Deepest level: delta = 0, so W = 1/sigma.
+__global__ __launch_bounds__(256) void k_bc_last(int64_t n, const uint64_t *__restrict__ fd, double *__restrict__ S) {
+    const int lane = threadIdx.x & 63;
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < n; i += waves) {
+        const uint32_t nd = nib_of(fd + 4 * i, lane);
+        if (!nd) continue;
+        double *si = S + i * kSrc + 4 * lane;
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+            if (nd & (1u << b)) si[b] = (1.0 / si[b]) * 1.0;
+    }
+}
+
+// Level d (sources in F_d) from level d+1 (F_{d+1}, already holding W).
+__global__ __launch_bounds__(256) void k_bc_backward(int64_t n, const int32_t *__restrict__ rst,
+                                                     const int32_t *__restrict__ col,
+                                                     const uint64_t *__restrict__ fd,
+                                                     const uint64_t *__restrict__ fn, double *__restrict__ S,
+                                                     double *__restrict__ bc) {
+    const int lane = threadIdx.x & 63;
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < n; i += waves) {
+        const uint32_t nd = nib_of(fd + 4 * i, lane);
+        if (!__ballot(nd != 0)) continue;
+        double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+        if (nd) {
+            const int32_t t1 = rst[i + 1];
+            for (int32_t t = rst[i]; t < t1; ++t) {
+                const int64_t w = col[t];
+                const uint32_t f = nib_of(fn + 4 * w, lane) & nd;
+                if (f) {
+                    const double *sw = S + w * kSrc + 4 * lane;
+                    if (f & 1) a0 += sw[0];
+                    if (f & 2) a1 += sw[1];
+                    if (f & 4) a2 += sw[2];
+                    if (f & 8) a3 += sw[3];
+                }
+            }
+        }
+        double *si = S + i * kSrc + 4 * lane;
+        double dsum = 0;
+        const double acc[4] = {a0, a1, a2, a3};
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            if (nd & (1u << b)) {
+                const double sigma = si[b];
+                const double delta = acc[b] * sigma;
+                si[b] = (1.0 / sigma) * (1.0 + delta);
+                dsum += delta;
+            }
+        }
+        dsum = dev::wave_sum(dsum);
+        if (lane == 0) bc[i] += dsum;
+    }
+}
+
+// A^T of a pattern CSR (counting sort, as transpose in include/matrix_la.h:155-179).
+__global__ void k_tcount(int64_t nz, const int32_t *__restrict__ col, int32_t *__restrict__ cnt) {
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < nz; p += (int64_t)gridDim.x * blockDim.x)
+        atomicAdd(&cnt[col[p]], 1);
+}
+__global__ void k_tfill(int64_t m, const int32_t *__restrict__ rst, const int32_t *__restrict__ col,
+                        int32_t *__restrict__ cur, int32_t *__restrict__ tcol) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
+        for (int32_t p = rst[i]; p < rst[i + 1]; ++p) tcol[atomicAdd(&cur[col[p]], 1)] = (int32_t)i;
+}
+int grid_waves(gk_ctx *ctx, int64_t rows) {
+    int64_t g = (rows + 3) / 4;
+    int64_t cap = (int64_t)ctx->num_cus * 16;
+    return (int)std::max<int64_t>(1, std::min(g, cap));
+}
+
+struct Dev {
+    std::vector<void *> allocs;
+    ~Dev() { for (void *p : allocs) (void)hipFree(p); }
+    template <class T>
+    int alloc(T **p, size_t n) {
+        if (hipMalloc((void **)p, sizeof(T) * (n ? n : 1)) != hipSuccess) {
+            set_error("hipMalloc(%zu) in BC failed", sizeof(T) * n);
+            return GK_ENOMEM;
+        }
+        allocs.push_back(*p);
+        return GK_OK;
+    }
+};
+
+}  // namespace
+}  // namespace gk
+
+extern "C" int gk_bc_batch_dev(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *AT, const int32_t *sources, int64_t nsrc,
+                               double *d_bc, const gk_opts *opts) {
+    GK_CHECK_ARG(ctx && A && d_bc && (sources || nsrc == 0), "null argument");
+    GK_CHECK_ARG(A->m == A->n, "non-square matrix; BC is only for square matrices");
+    GK_CHECK_ARG(!AT || (AT->m == A->n && AT->nz == A->nz), "AT does not match A");
+    for (int64_t s = 0; s < nsrc; ++s)
+        GK_CHECK_ARG(sources[s] >= 0 && sources[s] < A->m, "source %lld out of range", (long long)sources[s]);
+    (void)opts;
+    const int64_t n = A->m;
+    Dev D;
+    const int32_t *trst = AT ? AT->rst : nullptr, *tcol = AT ? AT->col : nullptr;
+    Timer total(ctx);
+    total.start();
+    if (!AT) {
+        int32_t *cnt, *tr, *tc, *cur;
+        GK_TRY(D.alloc(&cnt, n + 1));
+        GK_TRY(D.alloc(&tr, n + 1));
+        GK_TRY(D.alloc(&tc, A->nz));
+        GK_TRY(D.alloc(&cur, n + 1));
+        GK_HIP(hipMemsetAsync(cnt, 0, 4 * (n + 1), ctx->stream));
+        int g = (int)std::max<int64_t>(1, std::min<int64_t>((A->nz + 255) / 256, ctx->num_cus * 16));
+        if (A->nz) k_tcount<<<g, 256, 0, ctx->stream>>>(A->nz, A->col, cnt);
+        GK_TRY(exclusive_scan_i32(ctx, cnt, tr, n + 1));
+        GK_HIP(hipMemcpyAsync(cur, tr, 4 * (n + 1), hipMemcpyDeviceToDevice, ctx->stream));
+        int gm = (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, ctx->num_cus * 16));
+        // Row order of A^T does not matter: forward sums add integers
+        // (exact in FP64 below 2^53).
+        k_tfill<<<gm, 256, 0, ctx->stream>>>(n, A->rst, A->col, cur, tc);
+        GK_TRY(launch_check("transpose"));
+        trst = tr;
+        tcol = tc;
+    }
+    double *S;
+    uint64_t *vis;
+    int32_t *dsrc;
+    unsigned long long *dcnt;
+    GK_TRY(D.alloc(&S, (size_t)n * 256));
+    GK_TRY(D.alloc(&vis, (size_t)n * 4));
+    GK_TRY(D.alloc(&dsrc, 256));
+    GK_TRY(D.alloc(&dcnt, 1));
+    std::vector<uint64_t *> F;  // per-level frontier bitmaps, grown on demand
+    unsigned long long *h;
+    GK_TRY(pinned(ctx, 16, (void **)&h));
+    const int gw = grid_waves(ctx, n);
+    const int gi = (int)std::max<int64_t>(1, std::min<int64_t>((4 * n + 255) / 256, ctx->num_cus * 16));
+    int64_t levels_total = 0;
+    double fwd_ms = 0, bwd_ms = 0;
+    for (int64_t b0 = 0; b0 < nsrc || (b0 == 0 && nsrc == 0); b0 += kSrc) {
+        const int ns = (int)std::min<int64_t>(kSrc, nsrc - b0);
+        if (F.empty()) {
+            uint64_t *f0;
+            GK_TRY(D.alloc(&f0, (size_t)n * 4));
+            F.push_back(f0);
+        }
+        k_bc_init<<<gi, 256, 0, ctx->stream>>>(n, vis, F[0], d_bc, b0 == 0);
+        GK_TRY(launch_check("k_bc_init"));
+        if (ns <= 0) break;
+        GK_HIP(hipMemcpyAsync(dsrc, sources + b0, 4 * ns, hipMemcpyHostToDevice, ctx->stream));
+        k_bc_sources<<<1, 256, 0, ctx->stream>>>(dsrc, ns, vis, F[0], S);
+        GK_TRY(launch_check("k_bc_sources"));
+        // forward
+        Timer tf(ctx);
+        tf.start();
+        int d = 0;
+        for (;;) {
+            if ((int)F.size() < d + 2) {
+                uint64_t *f;
+                GK_TRY(D.alloc(&f, (size_t)n * 4));
+                F.push_back(f);
+            }
+            GK_HIP(hipMemsetAsync(dcnt, 0, 8, ctx->stream));
+            k_bc_forward<<<gw, 256, 0, ctx->stream>>>(n, trst, tcol, F[d], F[d + 1], vis, S, dcnt);
+            GK_TRY(launch_check("k_bc_forward"));
+            GK_HIP(hipMemcpyAsync(h, dcnt, 8, hipMemcpyDeviceToHost, ctx->stream));
+            GK_HIP(hipStreamSynchronize(ctx->stream));
+            if (h[0] == 0) break;
+            ++d;
+        }
+        tf.stop();
+        fwd_ms += tf.ms();
+        // d = deepest non-empty level
+        Timer tb(ctx);
+        tb.start();
+        if (d >= 1) {
+            k_bc_last<<<gw, 256, 0, ctx->stream>>>(n, F[d], S);
+            GK_TRY(launch_check("k_bc_last"));
+            for (int l = d - 1; l >= 1; --l) {
+                k_bc_backward<<<gw, 256, 0, ctx->stream>>>(n, A->rst, A->col, F[l], F[l + 1], S, d_bc);
+                GK_TRY(launch_check("k_bc_backward"));
+            }
+        }
+        tb.stop();
+        bwd_ms += tb.ms();
+        levels_total += d;
+    }
+    total.stop();
+    ctx->last = gk_timing{};
+    ctx->last.total_ms = total.ms();
+    ctx->last.kernel_ms = fwd_ms + bwd_ms;
+    ctx->last.select_ms = fwd_ms;  // forward share (backward = kernel_ms - select_ms)
+    ctx->last.iterations = levels_total;
+    return GK_OK;
+}
+
+extern "C" int gk_bc_batch_f64(gk_ctx *ctx, const gk_csr *A, const int32_t *sources, int64_t nsrc, double *bc,
+                               const gk_opts *opts) {
+    GK_CHECK_ARG(ctx && A && bc, "null argument");
+    gk_dcsr d;
+    GK_TRY(upload_csr(ctx, A, &d));
+    double *db = nullptr;
+    if (hipMalloc(&db, 8 * (A->m ? A->m : 1)) != hipSuccess) { free_csr(&d); set_error("hipMalloc failed"); return GK_ENOMEM; }
+    int rc = gk_bc_batch_dev(ctx, &d, nullptr, sources, nsrc, db, opts);
+    if (rc == GK_OK && A->m && hipMemcpy(bc, db, 8 * A->m, hipMemcpyDeviceToHost) != hipSuccess) {
+        set_error("copy of bc failed");
+        rc = GK_EHIP;
+    }
+    (void)hipFree(db);
+    free_csr(&d);
+    return rc;
+}

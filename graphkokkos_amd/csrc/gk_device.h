@@ -1,0 +1,138 @@
+// Device-side internals of libgraphhip.so shared by the .hip translation
+// units: the context, device CSR handles, scratch buffers, HIP error
+// checks and small wave-level helpers for gfx950 (wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "gk_internal.h"
+
+#define GK_HIP(call)                                                                    \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            gk::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, \
+                          __LINE__);                                                    \
+            return GK_EHIP;                                                             \
+        }                                                                               \
+    } while (0)
+
+#define GK_TRY(expr)              \
+    do {                          \
+        int rc_ = (expr);         \
+        if (rc_ != GK_OK) return rc_; \
+    } while (0)
+
+namespace gk {
+
+// Named grow-only device buffers owned by a context.
+enum ScratchSlot {
+    SC_FEAT = 0,      // per-row features
+    SC_TASK,          // task list
+    SC_BINS,          // bin counters
+    SC_ACC,           // global accumulators (hash/dense)
+    SC_RED,           // reductions
+    SC_TMP0, SC_TMP1, SC_TMP2, SC_TMP3, SC_TMP4, SC_TMP5, SC_TMP6, SC_TMP7,
+    SC_CUB,           // rocprim temporary storage
+    SC_MODEL,         // selector network parameters
+    SC_NUM
+};
+
+struct Model;  // learned selector (selector.hip)
+
+}  // namespace gk
+
+struct gk_ctx {
+    int device = 0;
+    int num_cus = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[8] = {};
+    void *scratch[gk::SC_NUM] = {};
+    size_t scratch_bytes[gk::SC_NUM] = {};
+    void *pinned = nullptr;          // small pinned host staging buffer
+    int32_t *dense_map = nullptr;    // dense-global maps, all zero between launches
+    int64_t dense_blocks = 0, dense_ncols = 0;
+    gk_timing last{};
+    gk::Model *model = nullptr;
+};
+
+struct gk_dcsr {
+    gk_ctx *ctx = nullptr;
+    int64_t m = 0, n = 0, nz = 0;
+    int32_t *rst = nullptr;
+    int32_t *col = nullptr;
+    int32_t *val = nullptr;   // nullptr: pattern (all ones)
+    bool owns = true;
+};
+
+namespace gk {
+
+// Device scratch of at least `bytes` in slot `s` (contents undefined).
+int scratch(gk_ctx *ctx, int s, size_t bytes, void **out);
+// Pinned host staging of at least `bytes`.
+int pinned(gk_ctx *ctx, size_t bytes, void **out);
+// Non-owning device CSR view.
+gk_dcsr view(gk_ctx *ctx, int64_t m, int64_t n, int64_t nz, int32_t *rst, int32_t *col, int32_t *val);
+// Upload helpers used by the host-pointer entry points.
+int upload_csr(gk_ctx *ctx, const gk_csr *h, gk_dcsr *d);
+void free_csr(gk_dcsr *d);
+int check_csr(const gk_csr *c, const char *name);
+// Event timer on the context stream.
+struct Timer {
+    gk_ctx *ctx;
+    hipEvent_t a, b;
+    explicit Timer(gk_ctx *c);
+    ~Timer();
+    void start();
+    void stop();
+    double ms();  // synchronises on the stop event
+};
+// Resolve options (NULL -> defaults).
+gk_opts resolve_opts(const gk_opts *o);
+// Kernel-launch error check.
+int launch_check(const char *what);
+
+}  // namespace gk
+
+// ---- device helpers -------------------------------------------------------
+namespace gk {
+namespace dev {
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <class T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        T w = __shfl_xor(v, o, 64);
+        v = v > w ? v : w;
+    }
+    return v;
+}
+
+// Inclusive prefix sum across the wave.
+template <class T>
+__device__ __forceinline__ T wave_incl_scan(T v) {
+    const int l = __lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        T w = __shfl_up(v, o, 64);
+        if (l >= o) v += w;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint32_t hash32(uint32_t x) { return x * 2654435761u; }
+
+}  // namespace dev
+}  // namespace gk

@@ -1,0 +1,218 @@
+// Context, device CSR handles, scratch buffers and timing for libgraphhip.so.
+// The context is this library's counterpart of Kokkos::initialize
+// (src/main.cpp:358 in GraphKokkos): it binds one device and one stream.
+#include <cstring>
+
+#include "gk_device.h"
+
+using namespace gk;
+
+namespace gk {
+
+int scratch(gk_ctx *ctx, int s, size_t bytes, void **out) {
+    if (bytes == 0) bytes = 256;
+    if (ctx->scratch_bytes[s] < bytes) {
+        if (ctx->scratch[s]) {
+            GK_HIP(hipStreamSynchronize(ctx->stream));
+            GK_HIP(hipFree(ctx->scratch[s]));
+            ctx->scratch[s] = nullptr;
+            ctx->scratch_bytes[s] = 0;
+        }
+        size_t want = bytes + bytes / 4;  // headroom against regrowth
+        hipError_t e = hipMalloc(&ctx->scratch[s], want);
+        if (e != hipSuccess) {
+            set_error("hipMalloc(%zu) for scratch slot %d failed: %s", want, s, hipGetErrorString(e));
+            return GK_ENOMEM;
+        }
+        ctx->scratch_bytes[s] = want;
+    }
+    *out = ctx->scratch[s];
+    return GK_OK;
+}
+
+int pinned(gk_ctx *ctx, size_t bytes, void **out) {
+    static const size_t kPinned = 1 << 16;
+    if (bytes > kPinned) { set_error("pinned staging request too large"); return GK_EINVAL; }
+    if (!ctx->pinned) GK_HIP(hipHostMalloc(&ctx->pinned, kPinned, hipHostMallocDefault));
+    *out = ctx->pinned;
+    return GK_OK;
+}
+
+gk_dcsr view(gk_ctx *ctx, int64_t m, int64_t n, int64_t nz, int32_t *rst, int32_t *col, int32_t *val) {
+    gk_dcsr d;
+    d.ctx = ctx; d.m = m; d.n = n; d.nz = nz; d.rst = rst; d.col = col; d.val = val; d.owns = false;
+    return d;
+}
+
+int check_csr(const gk_csr *c, const char *name) {
+    if (!c) { set_error("%s is NULL", name); return GK_EINVAL; }
+    if (c->m < 0 || c->n < 0 || c->nz < 0) { set_error("%s has negative sizes", name); return GK_EINVAL; }
+    if (c->m >= INT32_MAX || c->n >= INT32_MAX || c->nz >= INT32_MAX) {
+        set_error("%s exceeds int32 CSR indexing", name);
+        return GK_ERANGE;
+    }
+    if (!c->rst || (c->nz > 0 && !c->col)) { set_error("%s has NULL arrays", name); return GK_EINVAL; }
+    if (c->rst[0] != 0 || c->rst[c->m] != c->nz) {
+        set_error("%s: Rst[0] must be 0 and Rst[m] must equal nz", name);
+        return GK_EINVAL;
+    }
+    return GK_OK;
+}
+
+int upload_csr(gk_ctx *ctx, const gk_csr *h, gk_dcsr *d) {
+    GK_TRY(check_csr(h, "csr"));
+    d->ctx = ctx; d->m = h->m; d->n = h->n; d->nz = h->nz; d->owns = true;
+    d->rst = d->col = d->val = nullptr;
+    GK_HIP(hipMalloc(&d->rst, sizeof(int32_t) * (h->m + 1)));
+    GK_HIP(hipMalloc(&d->col, sizeof(int32_t) * (h->nz ? h->nz : 1)));
+    GK_HIP(hipMemcpyAsync(d->rst, h->rst, sizeof(int32_t) * (h->m + 1), hipMemcpyHostToDevice, ctx->stream));
+    if (h->nz) GK_HIP(hipMemcpyAsync(d->col, h->col, sizeof(int32_t) * h->nz, hipMemcpyHostToDevice, ctx->stream));
+    if (h->val) {
+        GK_HIP(hipMalloc(&d->val, sizeof(int32_t) * (h->nz ? h->nz : 1)));
+        if (h->nz) GK_HIP(hipMemcpyAsync(d->val, h->val, sizeof(int32_t) * h->nz, hipMemcpyHostToDevice, ctx->stream));
+    }
+    GK_HIP(hipStreamSynchronize(ctx->stream));
+    return GK_OK;
+}
+
+void free_csr(gk_dcsr *d) {
+    if (!d || !d->owns) return;
+    if (d->rst) (void)hipFree(d->rst);
+    if (d->col) (void)hipFree(d->col);
+    if (d->val) (void)hipFree(d->val);
+    d->rst = d->col = d->val = nullptr;
+}
+
+Timer::Timer(gk_ctx *c) : ctx(c), a(nullptr), b(nullptr) {
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+}
+Timer::~Timer() {
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+}
+void Timer::start() { (void)hipEventRecord(a, ctx->stream); }
+void Timer::stop() { (void)hipEventRecord(b, ctx->stream); }
+double Timer::ms() {
+    float t = 0;
+    (void)hipEventSynchronize(b);
+    (void)hipEventElapsedTime(&t, a, b);
+    return t;
+}
+
+gk_opts resolve_opts(const gk_opts *o) {
+    gk_opts r;
+    gk_opts_default(&r);
+    if (o && o->struct_size >= (int32_t)sizeof(gk_opts)) r = *o;
+    return r;
+}
+
+int launch_check(const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error("launch of %s failed: %s", what, hipGetErrorString(e));
+        return GK_EHIP;
+    }
+    return GK_OK;
+}
+
+}  // namespace gk
+
+extern "C" void gk_opts_default(gk_opts *o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof *o);
+    o->struct_size = sizeof(gk_opts);
+    o->selector = GK_SEL_HEURISTIC;
+    o->forced_accum = GK_ACC_MSA_LDS;
+    o->mca_max_mask = 8;
+    o->lds_max_mask = 1536;
+    o->dense_min_mask = 16384;
+    o->split_flop = 1 << 18;
+    o->block_threads = 64;
+}
+
+extern "C" int gk_init(int device, gk_ctx **out) {
+    GK_CHECK_ARG(out, "out is NULL");
+    int count = 0;
+    GK_HIP(hipGetDeviceCount(&count));
+    GK_CHECK_ARG(device >= 0 && device < count, "device %d not present (%d devices)", device, count);
+    GK_HIP(hipSetDevice(device));
+    auto *ctx = new gk_ctx();
+    ctx->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) { delete ctx; set_error("hipGetDeviceProperties failed"); return GK_EHIP; }
+    ctx->num_cus = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        set_error("hipStreamCreate failed");
+        return GK_EHIP;
+    }
+    *out = ctx;
+    return GK_OK;
+}
+
+namespace gk { void model_free(Model *m); }
+
+extern "C" void gk_finalize(gk_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (int s = 0; s < SC_NUM; ++s)
+        if (ctx->scratch[s]) (void)hipFree(ctx->scratch[s]);
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    if (ctx->dense_map) (void)hipFree(ctx->dense_map);
+    if (ctx->model) model_free(ctx->model);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+extern "C" int gk_sync(gk_ctx *ctx) {
+    GK_CHECK_ARG(ctx, "ctx is NULL");
+    GK_HIP(hipStreamSynchronize(ctx->stream));
+    return GK_OK;
+}
+
+extern "C" int gk_last_timing(gk_ctx *ctx, gk_timing *out) {
+    GK_CHECK_ARG(ctx && out, "null argument");
+    *out = ctx->last;
+    return GK_OK;
+}
+
+extern "C" int gk_dcsr_upload(gk_ctx *ctx, const gk_csr *h, gk_dcsr **out) {
+    GK_CHECK_ARG(ctx && out, "null argument");
+    auto *d = new gk_dcsr();
+    int rc = upload_csr(ctx, h, d);
+    if (rc != GK_OK) { free_csr(d); delete d; return rc; }
+    *out = d;
+    return GK_OK;
+}
+
+extern "C" void gk_dcsr_free(gk_dcsr *d) {
+    if (!d) return;
+    free_csr(d);
+    delete d;
+}
+
+extern "C" int64_t gk_dcsr_nnz(const gk_dcsr *d) { return d ? d->nz : -1; }
+
+extern "C" int gk_dmalloc(gk_ctx *ctx, size_t bytes, void **dptr) {
+    GK_CHECK_ARG(ctx && dptr, "null argument");
+    GK_HIP(hipMalloc(dptr, bytes ? bytes : 1));
+    return GK_OK;
+}
+extern "C" void gk_dfree(gk_ctx *ctx, void *dptr) {
+    (void)ctx;
+    if (dptr) (void)hipFree(dptr);
+}
+extern "C" int gk_memcpy_d2h(gk_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    GK_CHECK_ARG(ctx, "ctx is NULL");
+    GK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    GK_HIP(hipStreamSynchronize(ctx->stream));
+    return GK_OK;
+}
+extern "C" int gk_memcpy_h2d(gk_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    GK_CHECK_ARG(ctx, "ctx is NULL");
+    GK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    GK_HIP(hipStreamSynchronize(ctx->stream));
+    return GK_OK;
+}

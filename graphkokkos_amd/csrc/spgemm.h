@@ -1,0 +1,75 @@
+// Shared declarations of the masked-product engine (spgemm.hip) used by the
+// k-truss, selector and batching translation units.
+#pragma once
+
+#include "gk_device.h"
+
+namespace gk {
+
+// Product modes.
+enum { MODE_TC = 0, MODE_VALUES = 1, MODE_SUPPORT = 2 };
+
+// Capacities (entries).  A task covers at most kMaxTaskA A-entries so its
+// B-row prefix fits LDS; MSA-LDS holds mask rows up to kLdsMaxMask in a
+// hash of kLdsHashCap slots; MCA-wave holds mask rows up to kMcaMaxMask.
+constexpr int kMaxTaskA = 256;
+constexpr int kLdsHashCap = 4096;
+constexpr int kLdsMaxMask = 1536;
+constexpr int kMcaMaxMask = 64;
+constexpr int kHashGlobalMaxMask = 1 << 20;
+constexpr int kFlopBins = 32;
+constexpr int kNumBins = GK_NUM_ACC * kFlopBins;
+// Persistent-grid sizes (workgroups of one wave per CU).
+constexpr int kLdsBlocksPerCu = 8;
+constexpr int kMcaBlocksPerCu = 8;
+constexpr int kGlobalBlocksPerCu = 4;
+
+struct RowStats {
+    int64_t *flop;   // sum over A row of |B_k|
+    int32_t *mlen;   // mask row length
+    int32_t *alen;   // A row length
+    int32_t *bmax;   // max |B_k|
+    int32_t *ntask;  // tasks of the row
+    int8_t *acc;     // chosen accumulator (-1: no work)
+};
+
+struct SelectParams {
+    int selector, forced;
+    int32_t mca_max_mask, lds_max_mask, dense_min_mask;
+    int32_t split_flop;
+};
+
+struct Task {
+    int32_t row, a0, a1, split;
+};
+
+struct Plan {
+    int32_t ntasks = 0;
+    int32_t acc_tasks[GK_NUM_ACC] = {};
+    int32_t acc_maxmask[GK_NUM_ACC] = {};
+    Task *tasks = nullptr;
+    int32_t *bin_off = nullptr;  // device, kNumBins + 1
+};
+
+struct ProductInfo {
+    double total_ms = 0, select_ms = 0, kernel_ms = 0;
+    int32_t ntasks = 0;
+    int32_t acc_tasks[GK_NUM_ACC] = {};
+};
+
+int row_stats(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcsr *M, RowStats *st);
+int plan_tasks(gk_ctx *ctx, const gk_dcsr *A, RowStats &st, const SelectParams &sp, Plan *plan);
+SelectParams select_params(const gk_opts &o);
+int zero_u32(gk_ctx *ctx, void *p, int64_t n);
+
+// C<M> = A*B in the given mode.  d_out: VALUES -> int32 C values (M.nz),
+// SUPPORT -> uint32 supports (M.nz), TC -> unused.  d_count / d_rowcount:
+// TC totals (zeroed by the caller).
+int masked_product(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcsr *M, int mode, int32_t *d_out,
+                   unsigned long long *d_count, unsigned long long *d_rowcount, const gk_opts &o,
+                   ProductInfo *info);
+
+// Learned selector (selector.hip): writes st.acc for every row.
+int model_select(gk_ctx *ctx, RowStats &st, int64_t m, int64_t ncols);
+
+}  // namespace gk

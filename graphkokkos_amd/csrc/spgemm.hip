@@ -1,0 +1,853 @@
+// Masked sparse-sparse product C<M> = A*B on gfx950, and the triangle-count
+// and edge-support products built on it.
+//
+// GraphKokkos computes the same product with a dense per-thread accumulator
+// of B.n entries (MSA, include/matrix_la.h:576-635) or a mask-indexed
+// accumulator with a merge walk (MCA, include/matrix_la.h:493-528).  Here:
+//
+//  1. k_row_stats: one wave per row records mask length, A-row length,
+//     flop = sum_k |B_k| and max |B_k|.
+//  2. Every row gets an accumulator (fixed thresholds, the learned cost
+//     model, or a forced choice) and is cut into tasks of at most
+//     kMaxTaskA A-entries and about opts.split_flop streamed B entries.
+//  3. Tasks are counting-sorted into bins (accumulator, log2 flop
+//     descending) so heavy tasks start first.
+//  4. One persistent kernel per accumulator drains its bins:
+//       MSA-LDS      the mask row hashed into LDS (key -> mask position),
+//                    dense per-position counters in LDS;
+//       MCA-wave     the sorted mask row in LDS, each lane binary-searches
+//                    its streamed B entry, hits counted by ballot/popcount;
+//                    16-lane tiles, four mask rows per wave;
+//       hash-global  the same table as MSA-LDS but in HBM scratch, for mask
+//                    rows too long for LDS;
+//       dense-global a column-indexed map of the mask row in HBM scratch
+//                    (B.n entries per wave), for the longest mask rows.
+//     Each task streams the concatenation of its B rows with all 64 lanes:
+//     lane g takes streamed entries g, g+64, ..., locating its B row in an
+//     LDS prefix of B-row lengths, so reads of B are coalesced whatever the
+//     row lengths.
+//
+// Modes: TC counts hits (C<L>=L*L, sum of C; src/main.cpp:69-90), VALUES
+// writes C's values at M's pattern (mxmm_msa / mxmm_mca semantics), SUPPORT
+// turns every hit (i,k,j) of C<L>=L*L into +1 on the three edges of the
+// triangle (edge supports of the symmetric graph, S<A>=A*A restricted to L).
+#include "gk_device.h"
+#include "spgemm.h"
+
+using namespace gk;
+
+namespace gk {
+
+// ---------------------------------------------------------------------------
+// features and task construction
+
+__global__ __launch_bounds__(256) void k_row_stats(int64_t m, const int32_t *__restrict__ arst,
+                                                   const int32_t *__restrict__ acol,
+                                                   const int32_t *__restrict__ brst,
+                                                   const int32_t *__restrict__ mrst, RowStats st) {
+    const int lane = threadIdx.x & 63;
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < m; i += waves) {
+        const int32_t a0 = arst[i], a1 = arst[i + 1];
+        int64_t flop = 0;
+        int32_t bmax = 0;
+        for (int32_t p = a0 + lane; p < a1; p += 64) {
+            int32_t k = acol[p];
+            int32_t len = brst[k + 1] - brst[k];
+            flop += len;
+            bmax = max(bmax, len);
+        }
+        flop = dev::wave_sum(flop);
+        bmax = dev::wave_max(bmax);
+        if (lane == 0) {
+            st.mlen[i] = mrst[i + 1] - mrst[i];
+            st.alen[i] = a1 - a0;
+            st.flop[i] = flop;
+            st.bmax[i] = bmax;
+        }
+    }
+}
+
+__device__ __forceinline__ int heuristic_accum(const SelectParams &sp, int32_t mlen) {
+    if (mlen <= sp.mca_max_mask) return GK_ACC_MCA_WAVE;
+    if (mlen <= sp.lds_max_mask) return GK_ACC_MSA_LDS;
+    if (mlen >= sp.dense_min_mask) return GK_ACC_DENSE_GLOBAL;
+    return GK_ACC_HASH_GLOBAL;
+}
+
+// Capacity rules shared by the host and the kernels.
+__host__ __device__ __forceinline__ int lds_fits(int32_t mlen) { return mlen <= kLdsMaxMask; }
+__host__ __device__ __forceinline__ int mca_fits(int32_t mlen) { return mlen <= kMcaMaxMask; }
+
+__device__ __forceinline__ int flop_bin(int64_t flop) {
+    int lg = flop > 0 ? 63 - __clzll((unsigned long long)flop) : 0;
+    return lg > kFlopBins - 1 ? kFlopBins - 1 : lg;
+}
+
+__device__ __forceinline__ int32_t task_count(const SelectParams &sp, int32_t alen, int64_t flop) {
+    int64_t by_a = (alen + kMaxTaskA - 1) / kMaxTaskA;
+    int64_t by_f = (flop + sp.split_flop - 1) / sp.split_flop;
+    int64_t t = by_a > by_f ? by_a : by_f;
+    if (t > alen) t = alen;
+    return (int32_t)(t < 1 ? 1 : t);
+}
+
+// Pick the accumulator for every row (or take the learned choice already in
+// st.acc), count its tasks and add them to the bin histogram.
+__global__ __launch_bounds__(256) void k_select_bin(int64_t m, RowStats st, SelectParams sp,
+                                                    int32_t *__restrict__ bin_count,
+                                                    int32_t *__restrict__ acc_maxmask) {
+    __shared__ int32_t h[kNumBins];
+    for (int b = threadIdx.x; b < kNumBins; b += blockDim.x) h[b] = 0;
+    __syncthreads();
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+        int32_t mlen = st.mlen[i];
+        int64_t flop = st.flop[i];
+        int acc = -1;
+        if (mlen > 0 && flop > 0) {
+            if (sp.selector == GK_SEL_FORCED) acc = sp.forced;
+            else if (sp.selector == GK_SEL_LEARNED) acc = st.acc[i];
+            else acc = heuristic_accum(sp, mlen);
+            // Capacity overrides: a choice that cannot hold the mask row
+            // falls to the next accumulator that can.
+            if (acc == GK_ACC_MCA_WAVE && !mca_fits(mlen)) acc = GK_ACC_MSA_LDS;
+            if (acc == GK_ACC_MSA_LDS && !lds_fits(mlen)) acc = GK_ACC_HASH_GLOBAL;
+            if (acc == GK_ACC_HASH_GLOBAL && mlen > kHashGlobalMaxMask) acc = GK_ACC_DENSE_GLOBAL;
+        }
+        st.acc[i] = (int8_t)acc;
+        if (acc >= 0) {
+            int32_t nt = task_count(sp, st.alen[i], flop);
+            st.ntask[i] = nt;
+            atomicAdd(&h[acc * kFlopBins + (kFlopBins - 1 - flop_bin(flop))], nt);
+            atomicMax(&acc_maxmask[acc], mlen);
+        } else {
+            st.ntask[i] = 0;
+        }
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < kNumBins; b += blockDim.x)
+        if (h[b]) atomicAdd(&bin_count[b], h[b]);
+}
+
+// Exclusive scan of the bin histogram (one block), also per-accumulator
+// ranges and per-accumulator maximum mask length.
+__global__ void k_bin_scan(const int32_t *__restrict__ bin_count, int32_t *__restrict__ bin_off,
+                           int32_t *__restrict__ bin_cur) {
+    if (threadIdx.x == 0) {
+        int32_t s = 0;
+        for (int b = 0; b < kNumBins; ++b) {
+            bin_off[b] = s;
+            bin_cur[b] = s;
+            s += bin_count[b];
+        }
+        bin_off[kNumBins] = s;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_scatter_tasks(int64_t m, RowStats st, const int32_t *__restrict__ arst,
+                                                       int32_t *__restrict__ bin_cur, Task *__restrict__ tasks) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+        int acc = st.acc[i];
+        if (acc < 0) continue;
+        int32_t nt = st.ntask[i];
+        int b = acc * kFlopBins + (kFlopBins - 1 - flop_bin(st.flop[i]));
+        int32_t pos = atomicAdd(&bin_cur[b], nt);
+        int32_t a0 = arst[i], alen = arst[i + 1] - a0;
+        for (int32_t t = 0; t < nt; ++t) {
+            Task tk;
+            tk.row = (int32_t)i;
+            tk.a0 = a0 + (int32_t)((int64_t)alen * t / nt);
+            tk.a1 = a0 + (int32_t)((int64_t)alen * (t + 1) / nt);
+            tk.split = nt > 1;
+            tasks[pos + t] = tk;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// task engine
+
+struct Prob {
+    const int32_t *__restrict__ arst, *__restrict__ acol, *__restrict__ aval;
+    const int32_t *__restrict__ brst, *__restrict__ bcol, *__restrict__ bval;
+    const int32_t *__restrict__ mrst, *__restrict__ mcol;
+    int32_t *__restrict__ cval;                 // VALUES: C values; SUPPORT: uint32 supports
+    unsigned long long *__restrict__ count;     // TC: total
+    unsigned long long *__restrict__ rowcount;  // TC: optional per-row counts
+    const Task *__restrict__ tasks;
+    const int32_t *__restrict__ bin_off;
+};
+
+// LDS prefix of B-row lengths for A entries [a0, a1) of one task (a wave or
+// a tile of `G` lanes).  pre[x] = streamed entries before A entry a0+x;
+// bst[x] = index in B.col of streamed entry 0 of A entry a0+x minus pre[x].
+// Returns the number of streamed entries.
+template <int G>
+__device__ __forceinline__ int32_t build_prefix(const Prob &P, int32_t a0, int32_t a1, int32_t *pre, int32_t *bst,
+                                                int gl) {
+    int32_t total = 0;
+    for (int32_t base = a0; base < a1; base += G) {
+        int32_t p = base + gl;
+        int32_t len = 0, bs = 0;
+        if (p < a1) {
+            int32_t k = P.acol[p];
+            bs = P.brst[k];
+            len = P.brst[k + 1] - bs;
+        }
+        int32_t incl = len;
+#pragma unroll
+        for (int o = 1; o < G; o <<= 1) {
+            int32_t w = __shfl_up(incl, o, G);
+            if (gl >= o) incl += w;
+        }
+        if (p < a1) {
+            pre[p - a0] = total + incl - len;
+            bst[p - a0] = bs - (total + incl - len);
+        }
+        total += __shfl(incl, G - 1, G);
+    }
+    if (gl == 0) pre[a1 - a0] = total;
+    return total;
+}
+
+// Streamed-entry cursor: advance kk until pre[kk] <= g < pre[kk+1].
+__device__ __forceinline__ int32_t advance(const int32_t *pre, int32_t kk, int32_t g) {
+    while (pre[kk + 1] <= g) ++kk;
+    return kk;
+}
+
+// Hit handling common to every accumulator.  q = mask position (0-based in
+// the row), kk = A entry offset in the task, t = index of the streamed entry
+// in B.col.
+template <int MODE>
+struct Hit;
+
+template <>
+struct Hit<MODE_TC> {
+    __device__ static void on(const Prob &, int32_t *, int32_t *, int32_t, int32_t, int32_t, int32_t, uint32_t &cnt) {
+        ++cnt;
+    }
+};
+template <>
+struct Hit<MODE_VALUES> {
+    __device__ static void on(const Prob &P, int32_t *acc, int32_t *, int32_t q, int32_t a0, int32_t kk, int32_t t,
+                              uint32_t &) {
+        uint32_t av = P.aval ? (uint32_t)P.aval[a0 + kk] : 1u;
+        uint32_t bv = P.bval ? (uint32_t)P.bval[t] : 1u;
+        atomicAdd((uint32_t *)&acc[q], av * bv);
+    }
+};
+template <>
+struct Hit<MODE_SUPPORT> {
+    __device__ static void on(const Prob &P, int32_t *acc, int32_t *acnt, int32_t q, int32_t, int32_t kk, int32_t t,
+                              uint32_t &) {
+        atomicAdd((uint32_t *)&acc[q], 1u);          // edge (i, j)
+        atomicAdd((uint32_t *)&acnt[kk], 1u);        // edge (i, k)
+        atomicAdd((uint32_t *)&P.cval[t], 1u);       // edge (k, j)
+    }
+};
+
+// Write back the per-task accumulators.
+template <int MODE, int G>
+__device__ __forceinline__ void flush(const Prob &P, const Task &tk, int32_t mb, int32_t mlen, const int32_t *acc,
+                                      const int32_t *acnt, int gl) {
+    if (MODE == MODE_VALUES) {
+        for (int32_t q = gl; q < mlen; q += G) {
+            if (tk.split) atomicAdd((uint32_t *)&P.cval[mb + q], (uint32_t)acc[q]);
+            else P.cval[mb + q] = acc[q];
+        }
+    } else if (MODE == MODE_SUPPORT) {
+        for (int32_t q = gl; q < mlen; q += G)
+            if (acc[q]) atomicAdd((uint32_t *)&P.cval[mb + q], (uint32_t)acc[q]);
+        for (int32_t x = gl; x < tk.a1 - tk.a0; x += G)
+            if (acnt[x]) atomicAdd((uint32_t *)&P.cval[tk.a0 + x], (uint32_t)acnt[x]);
+    }
+}
+
+template <int MODE>
+__device__ __forceinline__ void tc_finish(const Prob &P, const Task &tk, uint32_t cnt_task, int gl, int G) {
+    (void)gl; (void)G;
+    if (MODE == MODE_TC && P.rowcount && cnt_task) atomicAdd(&P.rowcount[tk.row], (unsigned long long)cnt_task);
+}
+
+// ---- MSA-LDS --------------------------------------------------------------
+// One wave per task; LDS: key[cap] (mask column or -1), pos[cap] (mask
+// position), acc[mlen], pre/bst[kMaxTaskA+1].
+template <int MODE>
+__global__ __launch_bounds__(64) void k_msa_lds(Prob P, int acc_id) {
+    constexpr int HC = kLdsHashCap;
+    __shared__ int32_t hkey[HC];
+    __shared__ int32_t hpos[MODE == MODE_TC ? 1 : HC];
+    __shared__ int32_t acc[MODE == MODE_TC ? 1 : kLdsMaxMask];
+    __shared__ int32_t acnt[MODE == MODE_SUPPORT ? kMaxTaskA : 1];
+    __shared__ int32_t pre[kMaxTaskA + 1];
+    __shared__ int32_t bst[kMaxTaskA];
+    const int lane = threadIdx.x;
+    const int32_t lo = P.bin_off[acc_id * kFlopBins], hi = P.bin_off[(acc_id + 1) * kFlopBins];
+    uint32_t cnt = 0;
+    for (int32_t ti = lo + blockIdx.x; ti < hi; ti += gridDim.x) {
+        const Task tk = P.tasks[ti];
+        const int32_t mb = P.mrst[tk.row], mlen = P.mrst[tk.row + 1] - mb;
+        int lg = 6;
+        while ((1 << lg) < 4 * mlen && (1 << lg) < HC) ++lg;
+        const uint32_t cap = 1u << lg, cmask = cap - 1;
+        for (uint32_t s = lane; s < cap; s += 64) hkey[s] = -1;
+        if (MODE != MODE_TC)
+            for (int32_t q = lane; q < mlen; q += 64) acc[q] = 0;
+        if (MODE == MODE_SUPPORT)
+            for (int32_t x = lane; x < tk.a1 - tk.a0; x += 64) acnt[x] = 0;
+        __syncthreads();
+        for (int32_t q = lane; q < mlen; q += 64) {
+            int32_t c = P.mcol[mb + q];
+            uint32_t s = dev::hash32((uint32_t)c) >> (32 - lg);
+            while (atomicCAS(&hkey[s], -1, c) != -1) s = (s + 1) & cmask;
+            if (MODE != MODE_TC) hpos[s] = q;
+        }
+        const int32_t total = build_prefix<64>(P, tk.a0, tk.a1, pre, bst, lane);
+        __syncthreads();
+        uint32_t cnt_task = 0;
+        int32_t kk = 0;
+        for (int32_t g = lane; g < total; g += 64) {
+            kk = advance(pre, kk, g);
+            const int32_t t = bst[kk] + g;
+            const int32_t b = P.bcol[t];
+            uint32_t s = dev::hash32((uint32_t)b) >> (32 - lg);
+            for (;;) {
+                int32_t key = hkey[s];
+                if (key == b) {
+                    Hit<MODE>::on(P, acc, acnt, MODE == MODE_TC ? 0 : hpos[s], tk.a0, kk, t, cnt_task);
+                    break;
+                }
+                if (key < 0) break;
+                s = (s + 1) & cmask;
+            }
+        }
+        __syncthreads();
+        flush<MODE, 64>(P, tk, mb, mlen, acc, acnt, lane);
+        if (MODE == MODE_TC) {
+            if (P.rowcount) {
+                uint32_t r = dev::wave_sum(cnt_task);
+                if (lane == 0 && r) atomicAdd(&P.rowcount[tk.row], (unsigned long long)r);
+            }
+            cnt += cnt_task;
+        }
+        __syncthreads();
+    }
+    if (MODE == MODE_TC) {
+        unsigned long long s = dev::wave_sum((unsigned long long)cnt);
+        if (lane == 0 && s) atomicAdd(P.count, s);
+    }
+}
+
+// ---- MCA-wave -------------------------------------------------------------
+// Tiles of G lanes, 64/G tasks per wave.  The sorted mask row sits in LDS;
+// every lane binary-searches its streamed B entry in it.  In TC mode the
+// tile's hits are counted with one ballot + popcount per step.
+template <int MODE, int G>
+__global__ __launch_bounds__(64) void k_mca_wave(Prob P, int acc_id) {
+    constexpr int T = 64 / G;
+    __shared__ int32_t mk[T][kMcaMaxMask];
+    __shared__ int32_t acc[T][MODE == MODE_TC ? 1 : kMcaMaxMask];
+    __shared__ int32_t acnt[T][MODE == MODE_SUPPORT ? kMaxTaskA : 1];
+    __shared__ int32_t pre[T][kMaxTaskA + 1];
+    __shared__ int32_t bst[T][kMaxTaskA];
+    const int lane = threadIdx.x, tile = lane / G, gl = lane % G;
+    const int32_t lo = P.bin_off[acc_id * kFlopBins], hi = P.bin_off[(acc_id + 1) * kFlopBins];
+    uint32_t cnt = 0;
+    // All tiles of a wave run the same number of rounds so that the wave
+    // stays converged at the barriers.
+    const int32_t ntask = hi - lo;
+    const int32_t rounds = (ntask + (int32_t)gridDim.x * T - 1) / ((int32_t)gridDim.x * T);
+    for (int32_t r = 0; r < rounds; ++r) {
+        const int32_t ti = lo + (r * (int32_t)gridDim.x + (int32_t)blockIdx.x) * T + tile;
+        const bool live = ti < hi;
+        Task tk{0, 0, 0, 0};
+        int32_t mb = 0, mlen = 0;
+        if (live) {
+            tk = P.tasks[ti];
+            mb = P.mrst[tk.row];
+            mlen = P.mrst[tk.row + 1] - mb;
+        }
+        for (int32_t q = gl; q < mlen; q += G) {
+            mk[tile][q] = P.mcol[mb + q];
+            if (MODE != MODE_TC) acc[tile][q] = 0;
+        }
+        if (MODE == MODE_SUPPORT)
+            for (int32_t x = gl; x < tk.a1 - tk.a0; x += G) acnt[tile][x] = 0;
+        int32_t total = 0;
+        if (live) total = build_prefix<G>(P, tk.a0, tk.a1, pre[tile], bst[tile], gl);
+        __syncthreads();
+        uint32_t cnt_task = 0;
+        if (live && mlen > 0) {
+            const int32_t mlo = mk[tile][0], mhi = mk[tile][mlen - 1];
+            int32_t kk = 0;
+            for (int32_t g = gl; g < total; g += G) {
+                kk = advance(pre[tile], kk, g);
+                const int32_t t = bst[tile][kk] + g;
+                const int32_t b = P.bcol[t];
+                int32_t q = -1;
+                if (b >= mlo && b <= mhi) {
+                    int32_t l = 0, h = mlen;  // lower_bound in mk[tile][0, mlen)
+                    while (l < h) {
+                        int32_t mid = (l + h) >> 1;
+                        if (mk[tile][mid] < b) l = mid + 1; else h = mid;
+                    }
+                    if (l < mlen && mk[tile][l] == b) q = l;
+                }
+                if (MODE == MODE_TC) {
+                    cnt_task += q >= 0;
+                } else if (q >= 0) {
+                    Hit<MODE>::on(P, acc[tile], acnt[tile], q, tk.a0, kk, t, cnt_task);
+                }
+            }
+        }
+        __syncthreads();
+        if (live) {
+            flush<MODE, G>(P, tk, mb, mlen, acc[tile], acnt[tile], gl);
+            if (MODE == MODE_TC) {
+                if (P.rowcount) {
+                    uint32_t s = cnt_task;
+#pragma unroll
+                    for (int o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, G);
+                    if (gl == 0 && s) atomicAdd(&P.rowcount[tk.row], (unsigned long long)s);
+                }
+                cnt += cnt_task;
+            }
+        }
+        __syncthreads();
+    }
+    if (MODE == MODE_TC) {
+        // popcount of the per-lane hit counts: the wave total.
+        unsigned long long s = dev::wave_sum((unsigned long long)cnt);
+        if (lane == 0 && s) atomicAdd(P.count, s);
+    }
+}
+
+// ---- hash-global ----------------------------------------------------------
+// One wave per task, its table (key, position) in HBM scratch of `cap`
+// slots per wave; mask-position accumulators in HBM too.
+template <int MODE>
+__global__ __launch_bounds__(64) void k_hash_global(Prob P, int acc_id, int32_t *__restrict__ gkey,
+                                                    int32_t *__restrict__ gpos, int32_t *__restrict__ gacc,
+                                                    int lgcap) {
+    __shared__ int32_t pre[kMaxTaskA + 1];
+    __shared__ int32_t bst[kMaxTaskA];
+    __shared__ int32_t acnt[MODE == MODE_SUPPORT ? kMaxTaskA : 1];
+    const int lane = threadIdx.x;
+    const uint32_t cap = 1u << lgcap;
+    int32_t *key = gkey + (size_t)blockIdx.x * cap;
+    int32_t *pos = gpos + (size_t)blockIdx.x * cap;
+    int32_t *acc = gacc + (size_t)blockIdx.x * (cap >> 1);
+    const int32_t lo = P.bin_off[acc_id * kFlopBins], hi = P.bin_off[(acc_id + 1) * kFlopBins];
+    uint32_t cnt = 0;
+    for (int32_t ti = lo + blockIdx.x; ti < hi; ti += gridDim.x) {
+        const Task tk = P.tasks[ti];
+        const int32_t mb = P.mrst[tk.row], mlen = P.mrst[tk.row + 1] - mb;
+        int lg = 6;
+        while ((1 << lg) < 2 * mlen && lg < lgcap) ++lg;
+        const uint32_t c = 1u << lg, cmask = c - 1;
+        for (uint32_t s = lane; s < c; s += 64) key[s] = -1;
+        if (MODE != MODE_TC)
+            for (int32_t q = lane; q < mlen; q += 64) acc[q] = 0;
+        if (MODE == MODE_SUPPORT)
+            for (int32_t x = lane; x < tk.a1 - tk.a0; x += 64) acnt[x] = 0;
+        __threadfence();
+        __syncthreads();
+        for (int32_t q = lane; q < mlen; q += 64) {
+            int32_t col = P.mcol[mb + q];
+            uint32_t s = dev::hash32((uint32_t)col) >> (32 - lg);
+            while (atomicCAS(&key[s], -1, col) != -1) s = (s + 1) & cmask;
+            if (MODE != MODE_TC) pos[s] = q;
+        }
+        const int32_t total = build_prefix<64>(P, tk.a0, tk.a1, pre, bst, lane);
+        __threadfence();
+        __syncthreads();
+        uint32_t cnt_task = 0;
+        int32_t kk = 0;
+        for (int32_t g = lane; g < total; g += 64) {
+            kk = advance(pre, kk, g);
+            const int32_t t = bst[kk] + g;
+            const int32_t b = P.bcol[t];
+            uint32_t s = dev::hash32((uint32_t)b) >> (32 - lg);
+            for (;;) {
+                int32_t k2 = __builtin_nontemporal_load(&key[s]);
+                if (k2 == b) {
+                    Hit<MODE>::on(P, acc, acnt, MODE == MODE_TC ? 0 : pos[s], tk.a0, kk, t, cnt_task);
+                    break;
+                }
+                if (k2 < 0) break;
+                s = (s + 1) & cmask;
+            }
+        }
+        __threadfence();
+        __syncthreads();
+        flush<MODE, 64>(P, tk, mb, mlen, acc, acnt, lane);
+        if (MODE == MODE_TC) {
+            if (P.rowcount) {
+                uint32_t r = dev::wave_sum(cnt_task);
+                if (lane == 0 && r) atomicAdd(&P.rowcount[tk.row], (unsigned long long)r);
+            }
+            cnt += cnt_task;
+        }
+        __threadfence();
+        __syncthreads();
+    }
+    if (MODE == MODE_TC) {
+        unsigned long long s = dev::wave_sum((unsigned long long)cnt);
+        if (lane == 0 && s) atomicAdd(P.count, s);
+    }
+}
+
+// ---- dense-global ---------------------------------------------------------
+// One wave per task; a map of B.n entries per wave in HBM scratch holds
+// mask position + 1 at each mask column (0 elsewhere).  Only the mask
+// columns are set and cleared, so a row costs O(mask + streamed entries).
+template <int MODE>
+__global__ __launch_bounds__(64) void k_dense_global(Prob P, int acc_id, int32_t *__restrict__ gmap,
+                                                     int32_t *__restrict__ gacc, int64_t ncols,
+                                                     int32_t maxmask) {
+    __shared__ int32_t pre[kMaxTaskA + 1];
+    __shared__ int32_t bst[kMaxTaskA];
+    __shared__ int32_t acnt[MODE == MODE_SUPPORT ? kMaxTaskA : 1];
+    const int lane = threadIdx.x;
+    int32_t *map = gmap + (size_t)blockIdx.x * ncols;
+    int32_t *acc = gacc + (size_t)blockIdx.x * maxmask;
+    const int32_t lo = P.bin_off[acc_id * kFlopBins], hi = P.bin_off[(acc_id + 1) * kFlopBins];
+    uint32_t cnt = 0;
+    for (int32_t ti = lo + blockIdx.x; ti < hi; ti += gridDim.x) {
+        const Task tk = P.tasks[ti];
+        const int32_t mb = P.mrst[tk.row], mlen = P.mrst[tk.row + 1] - mb;
+        for (int32_t q = lane; q < mlen; q += 64) {
+            map[P.mcol[mb + q]] = q + 1;
+            if (MODE != MODE_TC) acc[q] = 0;
+        }
+        if (MODE == MODE_SUPPORT)
+            for (int32_t x = lane; x < tk.a1 - tk.a0; x += 64) acnt[x] = 0;
+        const int32_t total = build_prefix<64>(P, tk.a0, tk.a1, pre, bst, lane);
+        __threadfence();
+        __syncthreads();
+        uint32_t cnt_task = 0;
+        int32_t kk = 0;
+        for (int32_t g = lane; g < total; g += 64) {
+            kk = advance(pre, kk, g);
+            const int32_t t = bst[kk] + g;
+            const int32_t b = P.bcol[t];
+            const int32_t q1 = map[b];
+            if (q1) Hit<MODE>::on(P, acc, acnt, q1 - 1, tk.a0, kk, t, cnt_task);
+        }
+        __threadfence();
+        __syncthreads();
+        flush<MODE, 64>(P, tk, mb, mlen, acc, acnt, lane);
+        for (int32_t q = lane; q < mlen; q += 64) map[P.mcol[mb + q]] = 0;
+        if (MODE == MODE_TC) {
+            if (P.rowcount) {
+                uint32_t r = dev::wave_sum(cnt_task);
+                if (lane == 0 && r) atomicAdd(&P.rowcount[tk.row], (unsigned long long)r);
+            }
+            cnt += cnt_task;
+        }
+        __threadfence();
+        __syncthreads();
+    }
+    if (MODE == MODE_TC) {
+        unsigned long long s = dev::wave_sum((unsigned long long)cnt);
+        if (lane == 0 && s) atomicAdd(P.count, s);
+    }
+}
+
+__global__ void k_zero_u32(uint32_t *p, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// host driver
+
+static int grid_for(gk_ctx *ctx, int64_t work, int per_cu) {
+    int64_t g = (int64_t)ctx->num_cus * per_cu;
+    if (work < g) g = work;
+    return (int)(g < 1 ? 1 : g);
+}
+
+int zero_u32(gk_ctx *ctx, void *p, int64_t n) {
+    if (n <= 0) return GK_OK;
+    int blocks = grid_for(ctx, (n + 255) / 256, 16);
+    k_zero_u32<<<blocks, 256, 0, ctx->stream>>>((uint32_t *)p, n);
+    return launch_check("k_zero_u32");
+}
+
+// Features only (for the selector's training and inference).
+int row_stats(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcsr *M, RowStats *st) {
+    const int64_t m = A->m;
+    void *p;
+    size_t need = (size_t)m * (4 + 4 + 8 + 4 + 4 + 1) + 64;
+    GK_TRY(scratch(ctx, SC_FEAT, need, &p));
+    char *c = (char *)p;
+    st->flop = (int64_t *)c; c += 8 * (size_t)m;
+    st->mlen = (int32_t *)c; c += 4 * (size_t)m;
+    st->alen = (int32_t *)c; c += 4 * (size_t)m;
+    st->bmax = (int32_t *)c; c += 4 * (size_t)m;
+    st->ntask = (int32_t *)c; c += 4 * (size_t)m;
+    st->acc = (int8_t *)c;
+    if (m == 0) return GK_OK;
+    int blocks = grid_for(ctx, (m + 3) / 4, 32);
+    k_row_stats<<<blocks, 256, 0, ctx->stream>>>(m, A->rst, A->col, B->rst, M->rst, *st);
+    return launch_check("k_row_stats");
+}
+
+int plan_tasks(gk_ctx *ctx, const gk_dcsr *A, RowStats &st, const SelectParams &sp, Plan *plan) {
+    const int64_t m = A->m;
+    void *p;
+    const size_t words = 3 * kNumBins + 1 + GK_NUM_ACC;
+    GK_TRY(scratch(ctx, SC_BINS, sizeof(int32_t) * words, &p));
+    int32_t *bin_count = (int32_t *)p, *bin_off = bin_count + kNumBins, *bin_cur = bin_off + kNumBins + 1;
+    int32_t *acc_maxmask = bin_cur + kNumBins;
+    GK_HIP(hipMemsetAsync(bin_count, 0, sizeof(int32_t) * words, ctx->stream));
+    int blocks = grid_for(ctx, (m + 255) / 256, 8);
+    if (m > 0) {
+        k_select_bin<<<blocks, 256, 0, ctx->stream>>>(m, st, sp, bin_count, acc_maxmask);
+        GK_TRY(launch_check("k_select_bin"));
+    }
+    k_bin_scan<<<1, 64, 0, ctx->stream>>>(bin_count, bin_off, bin_cur);
+    GK_TRY(launch_check("k_bin_scan"));
+    // Task count and per-accumulator mask sizes come back to the host: the
+    // hash-global and dense-global kernels size their scratch from them.
+    int32_t *h;
+    GK_TRY(pinned(ctx, sizeof(int32_t) * (kNumBins + 1 + GK_NUM_ACC), (void **)&h));
+    GK_HIP(hipMemcpyAsync(h, bin_off, sizeof(int32_t) * (kNumBins + 1), hipMemcpyDeviceToHost, ctx->stream));
+    GK_HIP(hipMemcpyAsync(h + kNumBins + 1, acc_maxmask, sizeof(int32_t) * GK_NUM_ACC, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    GK_HIP(hipStreamSynchronize(ctx->stream));
+    plan->ntasks = h[kNumBins];
+    for (int a = 0; a < GK_NUM_ACC; ++a) {
+        plan->acc_tasks[a] = h[(a + 1) * kFlopBins] - h[a * kFlopBins];
+        plan->acc_maxmask[a] = h[kNumBins + 1 + a];
+    }
+    void *tp;
+    GK_TRY(scratch(ctx, SC_TASK, sizeof(Task) * (plan->ntasks + 1), &tp));
+    plan->tasks = (Task *)tp;
+    plan->bin_off = bin_off;
+    if (m > 0) {
+        k_scatter_tasks<<<blocks, 256, 0, ctx->stream>>>(m, st, A->rst, bin_cur, plan->tasks);
+        GK_TRY(launch_check("k_scatter_tasks"));
+    }
+    return GK_OK;
+}
+
+// Column-indexed maps for dense-global: `blocks` maps of `ncols` entries,
+// kept all-zero between launches (each task clears what it set).
+static int dense_maps(gk_ctx *ctx, int64_t blocks, int64_t ncols, int32_t **out) {
+    if (ctx->dense_map && ctx->dense_blocks >= blocks && ctx->dense_ncols == ncols) {
+        *out = ctx->dense_map;
+        return GK_OK;
+    }
+    if (ctx->dense_map) {
+        GK_HIP(hipStreamSynchronize(ctx->stream));
+        GK_HIP(hipFree(ctx->dense_map));
+        ctx->dense_map = nullptr;
+    }
+    size_t bytes = (size_t)blocks * (size_t)ncols * 4;
+    if (hipMalloc(&ctx->dense_map, bytes ? bytes : 4) != hipSuccess) {
+        ctx->dense_map = nullptr;
+        set_error("hipMalloc(%zu) for dense-global maps failed", bytes);
+        return GK_ENOMEM;
+    }
+    ctx->dense_blocks = blocks;
+    ctx->dense_ncols = ncols;
+    GK_TRY(zero_u32(ctx, ctx->dense_map, blocks * ncols));
+    *out = ctx->dense_map;
+    return GK_OK;
+}
+
+template <int MODE>
+static int run_mode(gk_ctx *ctx, const Prob &P, const Plan &plan, int64_t bn, const gk_opts &o, double *kernel_ms) {
+    Timer t(ctx);
+    t.start();
+    if (plan.acc_tasks[GK_ACC_MSA_LDS]) {
+        int g = grid_for(ctx, plan.acc_tasks[GK_ACC_MSA_LDS], kLdsBlocksPerCu);
+        k_msa_lds<MODE><<<g, 64, 0, ctx->stream>>>(P, GK_ACC_MSA_LDS);
+        GK_TRY(launch_check("k_msa_lds"));
+    }
+    if (plan.acc_tasks[GK_ACC_MCA_WAVE]) {
+        int g = grid_for(ctx, (plan.acc_tasks[GK_ACC_MCA_WAVE] + 3) / 4, kMcaBlocksPerCu);
+        k_mca_wave<MODE, 16><<<g, 64, 0, ctx->stream>>>(P, GK_ACC_MCA_WAVE);
+        GK_TRY(launch_check("k_mca_wave"));
+    }
+    if (plan.acc_tasks[GK_ACC_HASH_GLOBAL]) {
+        int lg = 6;
+        while ((1 << lg) < 2 * plan.acc_maxmask[GK_ACC_HASH_GLOBAL]) ++lg;
+        int g = grid_for(ctx, plan.acc_tasks[GK_ACC_HASH_GLOBAL], kGlobalBlocksPerCu);
+        void *sp;
+        size_t cap = (size_t)1 << lg;
+        // Bound the tables to about 4 GiB.
+        while (g > 1 && (size_t)g * cap * 10 > ((size_t)4 << 30)) g >>= 1;
+        GK_TRY(scratch(ctx, SC_ACC, (size_t)g * cap * 4 * 2 + (size_t)g * (cap / 2) * 4, &sp));
+        int32_t *key = (int32_t *)sp, *pos = key + (size_t)g * cap, *acc = pos + (size_t)g * cap;
+        k_hash_global<MODE><<<g, 64, 0, ctx->stream>>>(P, GK_ACC_HASH_GLOBAL, key, pos, acc, lg);
+        GK_TRY(launch_check("k_hash_global"));
+    }
+    if (plan.acc_tasks[GK_ACC_DENSE_GLOBAL]) {
+        int g = grid_for(ctx, plan.acc_tasks[GK_ACC_DENSE_GLOBAL], kGlobalBlocksPerCu);
+        // Bound the map memory to about 4 GiB.
+        while (g > 1 && (size_t)g * (size_t)bn * 4 > ((size_t)4 << 30)) g >>= 1;
+        int32_t mm = plan.acc_maxmask[GK_ACC_DENSE_GLOBAL];
+        int32_t *map;
+        GK_TRY(dense_maps(ctx, g, bn, &map));
+        void *sp;
+        GK_TRY(scratch(ctx, SC_TMP7, (size_t)g * (size_t)mm * 4 + 4, &sp));
+        k_dense_global<MODE><<<g, 64, 0, ctx->stream>>>(P, GK_ACC_DENSE_GLOBAL, map, (int32_t *)sp, bn, mm);
+        GK_TRY(launch_check("k_dense_global"));
+    }
+    t.stop();
+    *kernel_ms = t.ms();
+    (void)o;
+    return GK_OK;
+}
+
+SelectParams select_params(const gk_opts &o) {
+    SelectParams sp;
+    sp.selector = o.selector;
+    sp.forced = o.forced_accum;
+    sp.mca_max_mask = o.mca_max_mask;
+    sp.lds_max_mask = o.lds_max_mask;
+    sp.dense_min_mask = o.dense_min_mask;
+    sp.split_flop = o.split_flop > 0 ? o.split_flop : (1 << 18);
+    return sp;
+}
+
+int masked_product(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcsr *M, int mode, int32_t *d_out,
+                   unsigned long long *d_count, unsigned long long *d_rowcount, const gk_opts &o, ProductInfo *info) {
+    if (A->n != B->m || A->m != M->m || B->n != M->n) {
+        set_error("shape mismatch: A %lldx%lld, B %lldx%lld, M %lldx%lld", (long long)A->m, (long long)A->n,
+                  (long long)B->m, (long long)B->n, (long long)M->m, (long long)M->n);
+        return GK_EINVAL;
+    }
+    Timer total(ctx), sel(ctx);
+    total.start();
+    sel.start();
+    RowStats st;
+    GK_TRY(row_stats(ctx, A, B, M, &st));
+    SelectParams sp = select_params(o);
+    if (sp.selector == GK_SEL_LEARNED) GK_TRY(model_select(ctx, st, A->m, B->n));
+    Plan plan;
+    GK_TRY(plan_tasks(ctx, A, st, sp, &plan));
+    sel.stop();
+    if (mode == MODE_VALUES || mode == MODE_SUPPORT) GK_TRY(zero_u32(ctx, d_out, M->nz));
+    Prob P;
+    P.arst = A->rst; P.acol = A->col; P.aval = A->val;
+    P.brst = B->rst; P.bcol = B->col; P.bval = B->val;
+    P.mrst = M->rst; P.mcol = M->col;
+    P.cval = d_out; P.count = d_count; P.rowcount = d_rowcount;
+    P.tasks = plan.tasks; P.bin_off = plan.bin_off;
+    double kms = 0;
+    if (mode == MODE_TC) GK_TRY(run_mode<MODE_TC>(ctx, P, plan, B->n, o, &kms));
+    else if (mode == MODE_VALUES) GK_TRY(run_mode<MODE_VALUES>(ctx, P, plan, B->n, o, &kms));
+    else GK_TRY(run_mode<MODE_SUPPORT>(ctx, P, plan, B->n, o, &kms));
+    total.stop();
+    if (info) {
+        info->total_ms = total.ms();
+        info->select_ms = sel.ms();
+        info->kernel_ms = kms;
+        info->ntasks = plan.ntasks;
+        for (int a = 0; a < GK_NUM_ACC; ++a) info->acc_tasks[a] = plan.acc_tasks[a];
+    }
+    return GK_OK;
+}
+
+}  // namespace gk
+
+// ---------------------------------------------------------------------------
+// C ABI
+
+static void fill_timing(gk_ctx *ctx, const ProductInfo &pi, double bytes_alg) {
+    gk_timing &t = ctx->last;
+    t = gk_timing{};
+    t.total_ms = pi.total_ms;
+    t.kernel_ms = pi.kernel_ms;
+    t.select_ms = pi.select_ms;
+    t.launches = 1;
+    for (int a = 0; a < GK_NUM_ACC; ++a) t.rows_per_acc[a] = pi.acc_tasks[a];
+    t.bytes_alg = bytes_alg;
+}
+
+extern "C" int gk_masked_spgemm_dev(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcsr *M,
+                                    int32_t *d_cval, const gk_opts *opts) {
+    GK_CHECK_ARG(ctx && A && B && M && (d_cval || M->nz == 0), "null argument");
+    gk_opts o = resolve_opts(opts);
+    ProductInfo pi;
+    GK_TRY(masked_product(ctx, A, B, M, MODE_VALUES, d_cval, nullptr, nullptr, o, &pi));
+    fill_timing(ctx, pi, 0);
+    return GK_OK;
+}
+
+extern "C" int gk_masked_spgemm_i32(gk_ctx *ctx, const gk_csr *A, const gk_csr *B, const gk_csr *M, int32_t *c_val,
+                                    const gk_opts *opts) {
+    GK_CHECK_ARG(ctx && M && (c_val || M->nz == 0), "null argument");
+    GK_TRY(check_csr(A, "A"));
+    GK_TRY(check_csr(B, "B"));
+    GK_TRY(check_csr(M, "M"));
+    gk_dcsr dA, dB, dM;
+    GK_TRY(upload_csr(ctx, A, &dA));
+    int rc = upload_csr(ctx, B, &dB);
+    if (rc) { free_csr(&dA); return rc; }
+    rc = upload_csr(ctx, M, &dM);
+    if (rc) { free_csr(&dA); free_csr(&dB); return rc; }
+    int32_t *d_c = nullptr;
+    if (hipMalloc(&d_c, sizeof(int32_t) * (M->nz ? M->nz : 1)) != hipSuccess) {
+        free_csr(&dA); free_csr(&dB); free_csr(&dM);
+        set_error("hipMalloc for C failed");
+        return GK_ENOMEM;
+    }
+    rc = gk_masked_spgemm_dev(ctx, &dA, &dB, &dM, d_c, opts);
+    if (rc == GK_OK && M->nz) {
+        if (hipMemcpy(c_val, d_c, sizeof(int32_t) * M->nz, hipMemcpyDeviceToHost) != hipSuccess) {
+            set_error("copy of C to host failed");
+            rc = GK_EHIP;
+        }
+    }
+    (void)hipFree(d_c);
+    free_csr(&dA); free_csr(&dB); free_csr(&dM);
+    return rc;
+}
+
+extern "C" int gk_tc_count_dev(gk_ctx *ctx, const gk_dcsr *L, uint64_t *count, const gk_opts *opts) {
+    GK_CHECK_ARG(ctx && L && count, "null argument");
+    GK_CHECK_ARG(L->m == L->n, "L must be square");
+    gk_opts o = resolve_opts(opts);
+    void *p;
+    GK_TRY(scratch(ctx, SC_RED, 64, &p));
+    unsigned long long *d_count = (unsigned long long *)p;
+    GK_HIP(hipMemsetAsync(d_count, 0, 8, ctx->stream));
+    ProductInfo pi;
+    GK_TRY(masked_product(ctx, L, L, L, MODE_TC, nullptr, d_count, nullptr, o, &pi));
+    unsigned long long *h;
+    GK_TRY(pinned(ctx, 8, (void **)&h));
+    GK_HIP(hipMemcpyAsync(h, d_count, 8, hipMemcpyDeviceToHost, ctx->stream));
+    GK_HIP(hipStreamSynchronize(ctx->stream));
+    *count = *h;
+    fill_timing(ctx, pi, 0);
+    return GK_OK;
+}
+
+extern "C" int gk_tc_rows_dev(gk_ctx *ctx, const gk_dcsr *L, uint64_t *d_rowcount, const gk_opts *opts) {
+    GK_CHECK_ARG(ctx && L && d_rowcount, "null argument");
+    gk_opts o = resolve_opts(opts);
+    void *p;
+    GK_TRY(scratch(ctx, SC_RED, 64, &p));
+    unsigned long long *d_count = (unsigned long long *)p;
+    GK_HIP(hipMemsetAsync(d_count, 0, 8, ctx->stream));
+    GK_HIP(hipMemsetAsync(d_rowcount, 0, 8 * (size_t)L->m, ctx->stream));
+    ProductInfo pi;
+    GK_TRY(masked_product(ctx, L, L, L, MODE_TC, nullptr, d_count, (unsigned long long *)d_rowcount, o, &pi));
+    fill_timing(ctx, pi, 0);
+    return GK_OK;
+}
+
+extern "C" int gk_tc_count(gk_ctx *ctx, const gk_csr *L, uint64_t *count, const gk_opts *opts) {
+    GK_CHECK_ARG(ctx && count, "null argument");
+    gk_dcsr d;
+    GK_TRY(upload_csr(ctx, L, &d));
+    int rc = gk_tc_count_dev(ctx, &d, count, opts);
+    free_csr(&d);
+    return rc;
+}

@@ -1,0 +1,224 @@
+"""Host-side mirror of GraphKokkos's containers and graph operations.
+
+``SpMtx`` mirrors ``spMtx<ValT>`` (include/matrix.h:33-208): CSR arrays
+``Rst`` (m+1), ``Col`` (nz), ``Val`` (nz) plus ``m``, ``n``, ``nz`` and the
+Matrix Market typecode ``matcode``.  Graph construction (generators,
+cleaning, relabelling, the Matrix Market reader) runs in libgraphhip.so on
+host threads; the structural helpers below restate the reference's
+``extract_lower_triangle`` / ``build_symm_from_lower`` /
+``build_adjacency_matrix`` / ``transpose`` with numpy.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import GkCsr, check, lib
+
+
+def _ptr(a: np.ndarray | None, ct=C.c_int32):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+@dataclass
+class SpMtx:
+    """CSR matrix with the reference's member names (include/matrix.h:36-42)."""
+
+    m: int = 0
+    n: int = 0
+    Rst: np.ndarray = field(default_factory=lambda: np.zeros(1, np.int32))
+    Col: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    Val: np.ndarray | None = None
+    matcode: str = "MCPG"
+
+    def __post_init__(self):
+        self.Rst = np.ascontiguousarray(self.Rst, dtype=np.int32)
+        self.Col = np.ascontiguousarray(self.Col, dtype=np.int32)
+        if self.Val is not None:
+            self.Val = np.ascontiguousarray(self.Val)
+        if len(self.Rst) != self.m + 1:
+            raise ValueError(f"Rst has {len(self.Rst)} entries, expected m+1 = {self.m + 1}")
+        if int(self.Rst[-1]) != len(self.Col):
+            raise ValueError("Rst[m] must equal len(Col)")
+
+    @property
+    def nz(self) -> int:
+        return int(self.Rst[-1])
+
+    def row(self, i: int) -> np.ndarray:
+        return self.Col[self.Rst[i]:self.Rst[i + 1]]
+
+    def values_i32(self) -> np.ndarray | None:
+        """int32 values, or None for a pattern matrix whose values are all 1."""
+        if self.Val is None:
+            return None
+        v = np.ascontiguousarray(self.Val, dtype=np.int32)
+        if v.size and np.all(v == 1):
+            return None
+        return v
+
+    def gk(self, with_values: bool = True) -> GkCsr:
+        """A gk_csr view; the arrays stay owned by this object."""
+        v = self.values_i32() if with_values else None
+        self._keep = v  # keep the int32 copy alive while the struct is used
+        return GkCsr(self.m, self.n, self.nz, _ptr(self.Rst), _ptr(self.Col), _ptr(v))
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        val = np.ones(self.nz, np.int64) if self.Val is None else self.Val
+        return sp.csr_matrix((val, self.Col, self.Rst), shape=(self.m, self.n))
+
+    def __eq__(self, other) -> bool:  # operator== (include/matrix.h:165-175)
+        if not isinstance(other, SpMtx):
+            return NotImplemented
+        va = np.ones(self.nz) if self.Val is None else self.Val
+        vb = np.ones(other.nz) if other.Val is None else other.Val
+        return (self.m == other.m and self.n == other.n and self.nz == other.nz
+                and np.array_equal(self.Rst, other.Rst) and np.array_equal(self.Col, other.Col)
+                and np.array_equal(va, vb))
+
+    @classmethod
+    def from_file(cls, path: str, fmt: str = "mtx") -> "SpMtx":
+        """spMtx(filename, format) for the Matrix Market format
+        (include/matrix.h:44-61, read_mtx_to_crs :259-395)."""
+        if fmt != "mtx":
+            raise ValueError(f"format {fmt!r} is not supported (only 'mtx')")
+        h = C.c_void_p()
+        check(lib().gk_mtx_read(str(path).encode(), C.byref(h)), "gk_mtx_read")
+        try:
+            csr = GkCsr()
+            val = _lib.f64p()
+            code = C.create_string_buffer(4)
+            check(lib().gk_mtx_view(h, C.byref(csr), C.byref(val), code), "gk_mtx_view")
+            m, nz = csr.m, csr.nz
+            rst = np.ctypeslib.as_array(csr.rst, (m + 1,)).copy()
+            col = np.ctypeslib.as_array(csr.col, (nz,)).copy() if nz else np.zeros(0, np.int32)
+            v = np.ctypeslib.as_array(val, (nz,)).copy() if nz else np.zeros(0)
+            matcode = code.raw.decode()
+            if matcode[2] in "IP":
+                v = v.astype(np.int32)
+            return cls(int(m), int(csr.n), rst, col, v, matcode)
+        finally:
+            lib().gk_mtx_free(h)
+
+
+def build_adjacency_matrix(g: SpMtx) -> SpMtx:
+    """Pattern with every value 1 (include/matrix.h:643-660)."""
+    return SpMtx(g.m, g.n, g.Rst.copy(), g.Col.copy(), np.ones(g.nz, np.int32), g.matcode[:2] + "I" + g.matcode[3])
+
+
+def extract_lower_triangle(g: SpMtx) -> SpMtx:
+    """Strict-lower prefix of every row; assumes sorted rows
+    (include/matrix.h:698-726)."""
+    keep = np.zeros(g.m + 1, np.int64)
+    cols, vals = [], []
+    for i in range(g.m):
+        r = g.row(i)
+        k = int(np.searchsorted(r, i, side="left")) if r.size else 0
+        # the reference walks while Col < i; on a sorted row that is the prefix
+        keep[i + 1] = keep[i] + k
+        cols.append(r[:k])
+        if g.Val is not None:
+            vals.append(g.Val[g.Rst[i]:g.Rst[i] + k])
+    col = np.concatenate(cols) if cols else np.zeros(0, np.int32)
+    val = np.concatenate(vals) if vals else None
+    return SpMtx(g.m, g.n, keep.astype(np.int32), col, val, g.matcode)
+
+
+def transpose(a: SpMtx) -> SpMtx:
+    """Counting-sort transpose, rows come out sorted (include/matrix_la.h:155-179)."""
+    cnt = np.bincount(a.Col, minlength=a.n).astype(np.int64)
+    rst = np.zeros(a.n + 1, np.int64)
+    np.cumsum(cnt, out=rst[1:])
+    rows = np.repeat(np.arange(a.m, dtype=np.int32), np.diff(a.Rst))
+    order = np.lexsort((rows, a.Col))
+    val = None if a.Val is None else a.Val[order]
+    return SpMtx(a.n, a.m, rst.astype(np.int32), rows[order], val, a.matcode)
+
+
+def build_symm_from_lower(low: SpMtx) -> SpMtx:
+    """L merged with L^T row by row (include/matrix.h:662-696)."""
+    up = transpose(low)
+    rows = np.concatenate([np.repeat(np.arange(low.m), np.diff(low.Rst)), np.repeat(np.arange(up.m), np.diff(up.Rst))])
+    cols = np.concatenate([low.Col, up.Col])
+    vals = None
+    if low.Val is not None:
+        vals = np.concatenate([low.Val, up.Val])
+    order = np.lexsort((cols, rows))
+    rst = np.zeros(low.m + 1, np.int64)
+    np.cumsum(np.bincount(rows, minlength=low.m), out=rst[1:])
+    return SpMtx(low.m, low.n, rst.astype(np.int32), cols[order].astype(np.int32),
+                 None if vals is None else vals[order], low.matcode)
+
+
+class HostGraph:
+    """A cleaned undirected simple graph built by libgraphhip.so: symmetric
+    pattern ``A`` (sorted rows) and its strict lower part ``L``."""
+
+    def __init__(self, handle: C.c_void_p, desc: str):
+        self._h = handle
+        self.desc = desc
+        A, L = GkCsr(), GkCsr()
+        check(lib().gk_hgraph_view(self._h, C.byref(A), C.byref(L)), "gk_hgraph_view")
+        self.n = int(A.m)
+        self.A = SpMtx(self.n, self.n, np.ctypeslib.as_array(A.rst, (self.n + 1,)).copy(),
+                       np.ctypeslib.as_array(A.col, (A.nz,)).copy() if A.nz else np.zeros(0, np.int32))
+        self.L = SpMtx(self.n, self.n, np.ctypeslib.as_array(L.rst, (self.n + 1,)).copy(),
+                       np.ctypeslib.as_array(L.col, (L.nz,)).copy() if L.nz else np.zeros(0, np.int32))
+        self.perm = np.zeros(self.n, np.int32)
+        check(lib().gk_hgraph_perm(self._h, _ptr(self.perm)), "gk_hgraph_perm")
+        lib().gk_hgraph_free(self._h)
+        self._h = None
+
+    @property
+    def edges(self) -> int:
+        return self.L.nz
+
+    @classmethod
+    def rmat(cls, scale: int, edge_factor: int = 16, a: float = 0.57, b: float = 0.19, c: float = 0.19,
+             seed: int = 1, relabel: bool = True) -> "HostGraph":
+        h = C.c_void_p()
+        check(lib().gk_hgraph_rmat(scale, edge_factor, a, b, c, seed, int(relabel), C.byref(h)), "gk_hgraph_rmat")
+        return cls(h, f"rmat-s{scale}-ef{edge_factor}-seed{seed}")
+
+    @classmethod
+    def er(cls, n: int, mean_degree: float, seed: int = 1, relabel: bool = True) -> "HostGraph":
+        h = C.c_void_p()
+        check(lib().gk_hgraph_er(n, mean_degree, seed, int(relabel), C.byref(h)), "gk_hgraph_er")
+        return cls(h, f"er-n{n}-d{mean_degree}-seed{seed}")
+
+    @classmethod
+    def chung_lu(cls, n: int, exponent: float, mean_degree: float = 16.0, seed: int = 1,
+                 relabel: bool = True) -> "HostGraph":
+        h = C.c_void_p()
+        check(lib().gk_hgraph_chung_lu(n, exponent, mean_degree, seed, int(relabel), C.byref(h)),
+              "gk_hgraph_chung_lu")
+        return cls(h, f"chunglu-n{n}-g{exponent}-d{mean_degree}-seed{seed}")
+
+    @classmethod
+    def grid(cls, rows: int, cols: int | None = None, diagonal: bool = True, relabel: bool = True) -> "HostGraph":
+        h = C.c_void_p()
+        cols = rows if cols is None else cols
+        check(lib().gk_hgraph_grid(rows, cols, int(diagonal), int(relabel), C.byref(h)), "gk_hgraph_grid")
+        return cls(h, f"grid-{rows}x{cols}{'-diag' if diagonal else ''}")
+
+    @classmethod
+    def from_edges(cls, n: int, u, v, relabel: bool = False) -> "HostGraph":
+        u = np.ascontiguousarray(u, np.int32)
+        v = np.ascontiguousarray(v, np.int32)
+        h = C.c_void_p()
+        check(lib().gk_hgraph_from_edges(n, len(u), _ptr(u), _ptr(v), int(relabel), C.byref(h)),
+              "gk_hgraph_from_edges")
+        return cls(h, f"edges-n{n}")
+
+    @classmethod
+    def from_spmtx(cls, a: SpMtx, relabel: bool = False) -> "HostGraph":
+        h = C.c_void_p()
+        csr = a.gk(with_values=False)
+        check(lib().gk_hgraph_from_csr(C.byref(csr), int(relabel), C.byref(h)), "gk_hgraph_from_csr")
+        return cls(h, "csr")

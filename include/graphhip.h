@@ -1,0 +1,211 @@
+/*
+ * graphhip.h -- C ABI of libgraphhip.so: masked sparse-sparse products and
+ * the triangle-counting, k-truss and batched Brandes betweenness drivers on
+ * one AMD Instinct MI355X (gfx950).
+ *
+ * Every entry point takes plain pointers and sizes, returns 0 on success and
+ * a negative GK_E* code on failure; gk_last_error() then names the failing
+ * check (thread-local string).  Host-pointer entry points copy their inputs
+ * to HBM, run, and copy results back; the *_dev entry points work on graphs
+ * already resident in HBM (gk_dcsr) and are the ones that are timed.
+ *
+ * Reference interfaces each entry point replaces (paths relative to the
+ * GraphKokkos source tree):
+ *   gk_masked_spgemm_i32   mxmm_msa / mxmm_mca       include/matrix_la.h:87-123, 478-491, 562-574
+ *   gk_tc_count            triangle_counting_masked_sandia   src/main.cpp:69-90
+ *   gk_ktruss_support      "mxm" action  S<A>=A*A            src/main.cpp:228-233
+ *   gk_ktruss              k_truss                           src/main.cpp:93-141
+ *   gk_truss_decompose     k_truss looped over k=3..kmax     src/main.cpp:93-141
+ *   gk_bc_batch_f64        betweenness_centrality_batch      include/betweenness_centrality.h:406-506
+ *   gk_mtx_read            spMtx(filename,"mtx") / read_mtx_to_crs  include/matrix.h:44-61, 259-395
+ *   gk_hgraph_*            build_adjacency_matrix / extract_lower_triangle /
+ *                          build_symm_from_lower            include/matrix.h:643-726
+ */
+#ifndef GRAPHHIP_H
+#define GRAPHHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GK_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------ */
+#define GK_OK          0
+#define GK_EINVAL     -1   /* bad argument (null pointer, shape mismatch, unsorted row) */
+#define GK_ENOMEM     -2   /* device or host allocation failed */
+#define GK_EHIP       -3   /* a HIP runtime call failed */
+#define GK_EIO        -4   /* file could not be opened or parsed */
+#define GK_ERANGE     -5   /* size beyond what int32 CSR indices can address */
+#define GK_ESTATE     -6   /* context / handle misuse */
+
+const char *gk_last_error(void);
+int gk_abi_version(void);
+
+/* ---- CSR views (borrowed, host memory) ---------------------------------
+ * Same arrays as spMtx<ValT> (include/matrix.h:36-42): Rst[m+1], Col[nz],
+ * Val[nz].  val == NULL means a pattern matrix with every value 1 (what
+ * build_adjacency_matrix produces, include/matrix.h:643-660).              */
+typedef struct gk_csr {
+    int64_t m, n, nz;
+    const int32_t *rst;
+    const int32_t *col;
+    const int32_t *val;
+} gk_csr;
+
+/* ---- context ------------------------------------------------------------
+ * Counterpart of Kokkos::initialize (src/main.cpp:358): binds a device,
+ * creates the stream every kernel of this context runs on and owns all
+ * device buffers.  One host thread per context; calls are stream-ordered
+ * and not re-entrant for one context.                                      */
+typedef struct gk_ctx gk_ctx;
+int  gk_init(int device, gk_ctx **out);
+void gk_finalize(gk_ctx *ctx);
+int  gk_sync(gk_ctx *ctx);
+/* Host thread count used by graph construction (0 = hardware threads). */
+void gk_set_host_threads(int nthreads);
+
+/* ---- accumulator selection / options ------------------------------------ */
+enum gk_accum {
+    GK_ACC_AUTO        = -1, /* decided per row by the selector                */
+    GK_ACC_MSA_LDS     = 0,  /* mask row hashed into LDS, dense slot counters  */
+    GK_ACC_MCA_WAVE    = 1,  /* sorted-list intersection, ballot/popcount      */
+    GK_ACC_HASH_GLOBAL = 2,  /* open-addressing table in HBM scratch           */
+    GK_ACC_DENSE_GLOBAL= 3,  /* dense column-indexed marks in HBM scratch      */
+    GK_NUM_ACC         = 4
+};
+enum gk_selector {
+    GK_SEL_HEURISTIC = 0,   /* fixed thresholds (the fallback)            */
+    GK_SEL_LEARNED   = 1,   /* MLP cost model, needs gk_model_load/train  */
+    GK_SEL_FORCED    = 2    /* every row uses opts.forced_accum           */
+};
+
+typedef struct gk_opts {
+    int32_t struct_size;    /* sizeof(gk_opts); 0 or a NULL opts means defaults */
+    int32_t selector;       /* enum gk_selector                           */
+    int32_t forced_accum;   /* enum gk_accum, used with GK_SEL_FORCED     */
+    int32_t mca_max_mask;   /* heuristic: mask rows up to this use MCA    */
+    int32_t lds_max_mask;   /* heuristic: mask rows up to this use MSA-LDS*/
+    int32_t dense_min_mask; /* heuristic: mask rows from this use dense-global */
+    int32_t split_flop;     /* rows above this many B entries are split   */
+    int32_t block_threads;  /* workgroup size for the row kernels         */
+    int32_t verbose;
+    int32_t reserved[7];
+} gk_opts;
+void gk_opts_default(gk_opts *o);
+
+/* Per-call timings (device time from HIP events on the context stream). */
+typedef struct gk_timing {
+    double total_ms;        /* whole entry point, device side             */
+    double kernel_ms;       /* dominant kernel(s)                         */
+    double select_ms;       /* feature + selector kernels                 */
+    int64_t launches;       /* dominant-kernel launches                   */
+    int64_t iterations;     /* k-truss rounds / BC levels                 */
+    int64_t rows_per_acc[GK_NUM_ACC];
+    double  bytes_alg;      /* algorithmic HBM bytes of the dominant kernel(s) */
+} gk_timing;
+int gk_last_timing(gk_ctx *ctx, gk_timing *out);
+
+/* ---- host graphs (construction and cleaning, host threads) ---------------
+ * A gk_hgraph holds a cleaned undirected simple graph: symmetric pattern A
+ * (rows sorted, no self loops, no duplicates) and its strictly-lower part L
+ * (extract_lower_triangle, include/matrix.h:698-726).  With relabel=1 the
+ * vertices are renumbered by (degree descending, old id ascending).        */
+typedef struct gk_hgraph gk_hgraph;
+int  gk_hgraph_rmat(int scale, int edge_factor, double a, double b, double c,
+                    uint64_t seed, int relabel, gk_hgraph **out);
+int  gk_hgraph_er(int64_t n, double mean_degree, uint64_t seed, int relabel,
+                  gk_hgraph **out);
+int  gk_hgraph_chung_lu(int64_t n, double exponent, double mean_degree,
+                        uint64_t seed, int relabel, gk_hgraph **out);
+int  gk_hgraph_grid(int64_t rows, int64_t cols, int diagonal, int relabel,
+                    gk_hgraph **out);
+int  gk_hgraph_from_edges(int64_t n, int64_t ne, const int32_t *u,
+                          const int32_t *v, int relabel, gk_hgraph **out);
+/* Pattern of an arbitrary CSR (e.g. from gk_mtx_read): keeps rows as the
+ * reference does -- lower part mirrored (build_symm_from_lower(
+ * extract_lower_triangle(A)), src/main.cpp:214-215) but sorted and deduped. */
+int  gk_hgraph_from_csr(const gk_csr *A, int relabel, gk_hgraph **out);
+int  gk_hgraph_view(const gk_hgraph *g, gk_csr *A, gk_csr *L);
+/* new id of old vertex v (relabel permutation); NULL-safe when relabel=0 */
+int  gk_hgraph_perm(const gk_hgraph *g, int32_t *new_of_old);
+void gk_hgraph_free(gk_hgraph *g);
+
+/* ---- Matrix Market reader (reference semantics) -------------------------
+ * include/matrix.h:259-395: 1-based coordinates, self loops dropped,
+ * "symmetric" entries mirrored, column order within a row = file order,
+ * duplicates kept; complex/array and non-square rejected.  Values come back
+ * as double (pattern files give 1.0).                                       */
+typedef struct gk_mtx gk_mtx;
+int  gk_mtx_read(const char *path, gk_mtx **out);
+int  gk_mtx_view(const gk_mtx *mt, gk_csr *pattern, const double **val,
+                 char matcode[4]);
+void gk_mtx_free(gk_mtx *mt);
+
+/* ---- device-resident CSR ----------------------------------------------- */
+typedef struct gk_dcsr gk_dcsr;
+int  gk_dcsr_upload(gk_ctx *ctx, const gk_csr *h, gk_dcsr **out);
+void gk_dcsr_free(gk_dcsr *d);
+int64_t gk_dcsr_nnz(const gk_dcsr *d);
+
+/* ---- masked sparse-sparse product C<M> = A*B ----------------------------
+ * Output pattern := pattern of M (include/matrix_la.h:564-568); c_val[p]
+ * for p in [0, M.nz) is sum_k A(i,k)*B(k,j) at M's entry p = (i,j);
+ * explicit zeros kept.  Requires A.n == B.m, A.m == M.m, B.n == M.n and
+ * sorted, duplicate-free columns in M.  int32 wrap-around like the
+ * reference's int arithmetic.                                               */
+int gk_masked_spgemm_i32(gk_ctx *ctx, const gk_csr *A, const gk_csr *B,
+                         const gk_csr *M, int32_t *c_val, const gk_opts *o);
+int gk_masked_spgemm_dev(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B,
+                         const gk_dcsr *M, int32_t *d_cval, const gk_opts *o);
+
+/* ---- triangle counting C<L> = L*L, sum of C -----------------------------
+ * src/main.cpp:69-90; L strictly lower, sorted columns.                    */
+int gk_tc_count(gk_ctx *ctx, const gk_csr *L, uint64_t *count, const gk_opts *o);
+int gk_tc_count_dev(gk_ctx *ctx, const gk_dcsr *L, uint64_t *count, const gk_opts *o);
+/* per-row triangle counts (sum over the mask row), length L.m */
+int gk_tc_rows_dev(gk_ctx *ctx, const gk_dcsr *L, uint64_t *d_rowcount, const gk_opts *o);
+
+/* ---- supports S<A> = A*A at A's pattern (symmetric pattern A) ----------- */
+int gk_ktruss_support(gk_ctx *ctx, const gk_csr *A, uint32_t *support, const gk_opts *o);
+
+/* ---- single-k truss, reference semantics (src/main.cpp:93-141) ----------
+ * Repeats S<C>=C*C, keeps entries with S >= k-2, until no entry is removed.
+ * Output: pattern CSR of the k-truss (rst_out[m+1], col_out[<=A.nz]).      */
+int gk_ktruss(gk_ctx *ctx, const gk_csr *A, int k, int32_t *rst_out,
+              int32_t *col_out, int64_t *nz_out, int64_t *iterations,
+              const gk_opts *o);
+
+/* ---- full truss decomposition -------------------------------------------
+ * truss[p] for every entry p of the symmetric pattern A: the largest k such
+ * that the edge lies in the k-truss (2 for edges in no triangle).           */
+int gk_truss_decompose(gk_ctx *ctx, const gk_csr *A, uint32_t *truss,
+                       int32_t *kmax, const gk_opts *o);
+int gk_truss_decompose_dev(gk_ctx *ctx, const gk_dcsr *A, uint32_t *d_truss,
+                           int32_t *kmax, const gk_opts *o);
+
+/* ---- batched Brandes betweenness, FP64 ----------------------------------
+ * include/betweenness_centrality.h:406-506 with an explicit source list:
+ * bc[v] = sum over sources s of delta_s(v) (unnormalised, delta_s(s) = 0),
+ * sigma and delta kept in FP64.  Sources are processed in batches of up to
+ * 256 columns.  A is the (possibly directed) adjacency pattern; forward
+ * pulls over A^T, backward over A, as the reference does.                   */
+int gk_bc_batch_f64(gk_ctx *ctx, const gk_csr *A, const int32_t *sources,
+                    int64_t nsrc, double *bc, const gk_opts *o);
+int gk_bc_batch_dev(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *AT,
+                    const int32_t *sources, int64_t nsrc, double *d_bc,
+                    const gk_opts *o);
+
+/* ---- device buffers for the *_dev entry points ------------------------- */
+int  gk_dmalloc(gk_ctx *ctx, size_t bytes, void **dptr);
+void gk_dfree(gk_ctx *ctx, void *dptr);
+int  gk_memcpy_d2h(gk_ctx *ctx, void *dst, const void *src, size_t bytes);
+int  gk_memcpy_h2d(gk_ctx *ctx, void *dst, const void *src, size_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GRAPHHIP_H */

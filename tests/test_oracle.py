@@ -1,0 +1,196 @@
+"""Pins the CPU oracle (oracle/oracle.cpp) before it is trusted as the
+checker: golden fixtures written by networkx/scipy, closed forms, brute force
+on tiny graphs, and cross-checks between the oracle's independent
+restatements.  Each check is chosen so a plausible slip in the restatement
+(a dropped term, a wrong index, a transposed operand, a missing mask) fails.
+"""
+from math import comb
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN_NAMES, load_golden
+from graphkokkos_amd import HostGraph, SpMtx
+from oracle import oracle as O
+
+
+def complete(n):
+    u, v = zip(*[(i, j) for i in range(n) for j in range(i)])
+    return HostGraph.from_edges(n, u, v)
+
+
+def random_spmtx(rng, m, n, density, lo=-3, hi=4, sorted_unique=True):
+    rows, cols, vals = [], [], []
+    rst = [0]
+    for i in range(m):
+        k = rng.binomial(n, density)
+        c = np.sort(rng.choice(n, size=k, replace=False)) if k else np.zeros(0, int)
+        cols.append(c)
+        vals.append(rng.integers(lo, hi, size=k))
+        rst.append(rst[-1] + k)
+    col = np.concatenate(cols) if cols else np.zeros(0)
+    val = np.concatenate(vals) if vals else np.zeros(0)
+    return SpMtx(m, n, np.array(rst, np.int32), col.astype(np.int32), val.astype(np.int32))
+
+
+# ---- golden fixtures (networkx / scipy) -------------------------------------
+
+@pytest.mark.parametrize("name", GOLDEN_NAMES)
+def test_oracle_matches_golden(name):
+    A, z = load_golden(name)
+    G = HostGraph.from_spmtx(A)
+    assert np.array_equal(G.A.Rst, A.Rst) and np.array_equal(G.A.Col, A.Col)
+    for acc in (O.MSA, O.MCA):
+        for par in (True, False):
+            assert O.tc_sandia(G.L, acc, par) == int(z["triangles"])
+            assert np.array_equal(O.support(A, acc, par), z["support"])
+    tr, kmax, _ = O.truss_decompose(A)
+    assert np.array_equal(tr, z["truss"])
+    tp, kp = O.truss_peel(A)
+    assert np.array_equal(tp, z["truss"]) and kp == kmax
+    bc = O.bc_batch(A, z["sources"])
+    np.testing.assert_allclose(bc, z["bc"], rtol=1e-12, atol=1e-12)
+
+
+# ---- closed forms -----------------------------------------------------------
+
+@pytest.mark.parametrize("n", [3, 5, 8, 13])
+def test_complete_graph_closed_forms(n):
+    G = complete(n)
+    assert O.tc_sandia(G.L) == comb(n, 3)
+    assert set(O.support(G.A).tolist()) == {n - 2}
+    tr, kmax, _ = O.truss_decompose(G.A)
+    assert kmax == n and set(tr.tolist()) == {n}
+
+
+@pytest.mark.parametrize("r", [2, 6, 9])
+def test_triangulated_grid_closed_forms(r):
+    G = HostGraph.grid(r)
+    assert G.edges == 3 * r * r - 4 * r + 1
+    assert O.tc_sandia(G.L) == 2 * (r - 1) ** 2
+    s = O.support(G.A)
+    # 4(r-1) perimeter edges with support 1 (counted twice in A), the rest 2
+    assert np.count_nonzero(s == 1) == 2 * 4 * (r - 1)
+    assert np.count_nonzero(s == 2) == 2 * (G.edges - 4 * (r - 1))
+    tr, kmax, _ = O.truss_decompose(G.A)
+    assert kmax == 3 and set(tr.tolist()) == {3}
+    keep, _ = O.ktruss(G.A, 4)
+    assert not keep.any()  # empty 4-truss
+
+
+def test_bipartite_has_no_triangles():
+    u, v = zip(*[(i, 10 + j) for i in range(10) for j in range(7)])
+    G = HostGraph.from_edges(17, u, v)
+    assert O.tc_sandia(G.L) == 0
+    assert not O.support(G.A).any()
+    tr, kmax, _ = O.truss_decompose(G.A)
+    assert kmax == 2 and set(tr.tolist()) == {2}
+
+
+def test_path_and_star_bc():
+    # path 0-1-...-8, all sources: bc(v) = 2 * v * (n-1-v)
+    n = 9
+    G = HostGraph.from_edges(n, list(range(n - 1)), list(range(1, n)))
+    bc = O.bc_batch(G.A, np.arange(n))
+    assert np.allclose(bc, [2 * v * (n - 1 - v) for v in range(n)])
+    # star with centre 0 and 7 leaves, all sources: centre 7*6, leaves 0
+    G = HostGraph.from_edges(8, [0] * 7, list(range(1, 8)))
+    bc = O.bc_batch(G.A, np.arange(8))
+    assert bc[0] == 42 and not bc[1:].any()
+
+
+# ---- brute force ------------------------------------------------------------
+
+@pytest.mark.parametrize("seed", range(4))
+def test_masked_product_brute_force(seed):
+    rng = np.random.default_rng(seed)
+    m, k, n = 23, 17, 19
+    A = random_spmtx(rng, m, k, 0.3)
+    B = random_spmtx(rng, k, n, 0.3)
+    M = random_spmtx(rng, m, n, 0.4)
+    want = O.dense_masked_product(A, B, M)
+    for acc in (O.MSA, O.MCA):
+        for par in (True, False):
+            assert np.array_equal(O.mxmm(A, B, M, acc, par), want)
+
+
+def test_int32_wraparound_matches_dense():
+    rng = np.random.default_rng(5)
+    A = random_spmtx(rng, 9, 9, 0.8, lo=2**20, hi=2**21)
+    B = random_spmtx(rng, 9, 9, 0.8, lo=2**20, hi=2**21)
+    M = random_spmtx(rng, 9, 9, 0.9, lo=1, hi=2)
+    assert np.array_equal(O.mxmm(A, B, M, O.MCA), O.dense_masked_product(A, B, M))
+
+
+def test_reference_sequential_msa_stale_read():
+    """_mxmm_msa_sequential returns a previous row's accumulator value at a
+    mask column that gets no contribution (include/matrix_la.h:726-729);
+    the parallel MSA and the MCA return 0 there."""
+    # row 0: C(0,1) = A(0,0)*B(0,1) = 5; row 1: mask has column 1, no product.
+    A = SpMtx(2, 1, np.array([0, 1, 1], np.int32), np.array([0], np.int32), np.array([5], np.int32))
+    B = SpMtx(1, 2, np.array([0, 1], np.int32), np.array([1], np.int32), np.array([1], np.int32))
+    M = SpMtx(2, 2, np.array([0, 1, 2], np.int32), np.array([1, 1], np.int32))
+    assert O.mxmm(A, B, M, O.MSA, True).tolist() == [5, 0]
+    assert O.mxmm(A, B, M, O.MCA, False).tolist() == [5, 0]
+    assert O.mxmm(A, B, M, O.MSA, False).tolist() == [5, 0]
+    assert O.mxmm(A, B, M, O.MSA_STALE, False).tolist() == [5, 5]
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_truss_and_bc_brute_force(seed):
+    G = HostGraph.er(24, 7.0, seed=seed)
+    want = O.brute_truss(G.A)
+    assert np.array_equal(O.truss_decompose(G.A)[0], want)
+    assert np.array_equal(O.truss_peel(G.A)[0], want)
+    src = np.arange(0, 24, 3)
+    np.testing.assert_allclose(O.bc_batch(G.A, src), O.brandes_bc(G.A, src), rtol=1e-12, atol=1e-12)
+
+
+def test_bc_directed_brute_force():
+    rng = np.random.default_rng(9)
+    A = random_spmtx(rng, 30, 30, 0.12, lo=1, hi=2)
+    # drop self loops, keep the pattern
+    keep = np.repeat(np.arange(30), np.diff(A.Rst)) != A.Col
+    rows = np.repeat(np.arange(30), np.diff(A.Rst))[keep]
+    rst = np.zeros(31, np.int32)
+    np.cumsum(np.bincount(rows, minlength=30), out=rst[1:])
+    D = SpMtx(30, 30, rst, A.Col[keep])
+    src = np.arange(0, 30, 2)
+    np.testing.assert_allclose(O.bc_batch(D, src), O.brandes_bc(D, src), rtol=1e-12, atol=1e-12)
+    # batches of 4 columns give the same sums as one batch
+    np.testing.assert_allclose(O.bc_batch(D, src, batch=4), O.bc_batch(D, src), rtol=1e-12, atol=1e-12)
+
+
+def test_bc_fp32_mode_close_to_fp64():
+    G = HostGraph.rmat(9, 8, seed=2)
+    src = np.arange(0, 64)
+    b64 = O.bc_batch(G.A, src, fp64=True)
+    b32 = O.bc_batch(G.A, src, fp64=False)
+    np.testing.assert_allclose(b32, b64, rtol=2e-3, atol=1e-2)
+
+
+# ---- invariants at moderate size -------------------------------------------
+
+def test_invariants_rmat12():
+    G = HostGraph.rmat(12, 16, seed=4)
+    T = O.tc_sandia(G.L)
+    s = O.support(G.A)
+    assert int(s.sum()) == 6 * T  # every triangle adds 1 to 3 edges, each stored twice
+    tr, kmax, _ = O.truss_decompose(G.A)
+    assert np.all(tr <= s.astype(np.int64) + 2)
+    tp, kp = O.truss_peel(G.A)
+    assert np.array_equal(tr, tp) and kp == kmax
+    # BC: sum_v delta_s(v) = sum over reachable t != s of (dist(s,t) - 1)
+    import scipy.sparse.csgraph as cg
+    src = np.array([0, 5, 77, 1000], np.int32)
+    bc = O.bc_batch(G.A, src)
+    d = cg.shortest_path(G.A.to_scipy(), unweighted=True, indices=src)
+    fin = np.isfinite(d) & (d > 0)
+    assert np.isclose(bc.sum(), (d[fin] - 1).sum())
+
+
+def test_relabel_invariance_of_triangles():
+    G1 = HostGraph.rmat(11, 8, seed=6, relabel=True)
+    G2 = HostGraph.rmat(11, 8, seed=6, relabel=False)
+    assert G1.edges == G2.edges
+    assert O.tc_sandia(G1.L) == O.tc_sandia(G2.L)
