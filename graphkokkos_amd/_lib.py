@@ -41,7 +41,9 @@ class GkOpts(C.Structure):
 class GkTiming(C.Structure):
     _fields_ = [("total_ms", C.c_double), ("kernel_ms", C.c_double), ("select_ms", C.c_double),
                 ("launches", C.c_int64), ("iterations", C.c_int64), ("rows_per_acc", C.c_int64 * NUM_ACC),
-                ("bytes_alg", C.c_double)]
+                ("acc_ms", C.c_double * NUM_ACC), ("acc_bytes", C.c_double * NUM_ACC),
+                ("acc_launches", C.c_int64 * NUM_ACC), ("bytes_alg", C.c_double), ("flop", C.c_double),
+                ("hits", C.c_double)]
 
 
 class GraphHipError(RuntimeError):

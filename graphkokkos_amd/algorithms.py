@@ -112,7 +112,9 @@ class Context:
         check(lib().gk_last_timing(self.handle, C.byref(t)), "gk_last_timing")
         return {"total_ms": t.total_ms, "kernel_ms": t.kernel_ms, "select_ms": t.select_ms,
                 "launches": t.launches, "iterations": t.iterations,
-                "rows_per_acc": list(t.rows_per_acc), "bytes_alg": t.bytes_alg}
+                "rows_per_acc": list(t.rows_per_acc), "acc_ms": list(t.acc_ms), "acc_bytes": list(t.acc_bytes),
+                "acc_launches": list(t.acc_launches), "bytes_alg": t.bytes_alg, "flop": t.flop,
+                "hits": t.hits}
 
     @staticmethod
     def _opts(opts: GkOpts | dict | None) -> GkOpts:

@@ -314,7 +314,7 @@ extern "C" int gk_bc_batch_f64(gk_ctx *ctx, const gk_csr *A, const int32_t *sour
     double *db = nullptr;
     if (hipMalloc(&db, 8 * (A->m ? A->m : 1)) != hipSuccess) { free_csr(&d); set_error("hipMalloc failed"); return GK_ENOMEM; }
     int rc = gk_bc_batch_dev(ctx, &d, nullptr, sources, nsrc, db, opts);
-    if (rc == GK_OK && A->m && hipMemcpy(bc, db, 8 * A->m, hipMemcpyDeviceToHost) != hipSuccess) {
+    if (rc == GK_OK && A->m && d2h(ctx, bc, db, 8 * A->m) != hipSuccess) {
         set_error("copy of bc failed");
         rc = GK_EHIP;
     }

@@ -74,6 +74,10 @@ namespace gk {
 int scratch(gk_ctx *ctx, int s, size_t bytes, void **out);
 // Pinned host staging of at least `bytes`.
 int pinned(gk_ctx *ctx, size_t bytes, void **out);
+// Stream-ordered device-to-host copy on the context stream, then wait.
+// (A plain hipMemcpy runs on the null stream, which does not wait for the
+// context's non-blocking stream.)
+hipError_t d2h(gk_ctx *ctx, void *dst, const void *src, size_t bytes);
 // Non-owning device CSR view.
 gk_dcsr view(gk_ctx *ctx, int64_t m, int64_t n, int64_t nz, int32_t *rst, int32_t *col, int32_t *val);
 // Upload helpers used by the host-pointer entry points.

@@ -44,6 +44,13 @@ gk_dcsr view(gk_ctx *ctx, int64_t m, int64_t n, int64_t nz, int32_t *rst, int32_
     return d;
 }
 
+hipError_t d2h(gk_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (bytes == 0) return hipSuccess;
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(ctx->stream);
+}
+
 int check_csr(const gk_csr *c, const char *name) {
     if (!c) { set_error("%s is NULL", name); return GK_EINVAL; }
     if (c->m < 0 || c->n < 0 || c->nz < 0) { set_error("%s has negative sizes", name); return GK_EINVAL; }

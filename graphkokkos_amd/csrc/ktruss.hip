@@ -210,6 +210,7 @@ int truss_rounds(gk_ctx *ctx, TrussBufs &B, Lgraph &cur, Lgraph &nxt, int fixed_
         gk_dcsr L = view(ctx, cur.m, cur.m, cur.nz, cur.rst, cur.col, nullptr);
         ProductInfo pi;
         GK_TRY(masked_product(ctx, &L, &L, &L, MODE_SUPPORT, (int32_t *)sup, nullptr, nullptr, o, &pi));
+        record_timing(ctx, pi, false);
         sms += pi.kernel_ms;
         uint32_t thresh;
         if (fixed_k > 0) {
@@ -265,6 +266,7 @@ extern "C" int gk_truss_decompose_dev(gk_ctx *ctx, const gk_dcsr *A, uint32_t *d
     GK_CHECK_ARG(ctx && A && (d_truss || A->nz == 0), "null argument");
     GK_CHECK_ARG(A->m == A->n, "A must be square");
     gk_opts o = resolve_opts(opts);
+    ctx->last = gk_timing{};
     Timer total(ctx);
     total.start();
     TrussBufs B(ctx);
@@ -288,11 +290,8 @@ extern "C" int gk_truss_decompose_dev(gk_ctx *ctx, const gk_dcsr *A, uint32_t *d
         GK_TRY(launch_check("k_truss_to_sym"));
     }
     total.stop();
-    ctx->last = gk_timing{};
     ctx->last.total_ms = total.ms();
-    ctx->last.kernel_ms = sms;
     ctx->last.iterations = rounds;
-    ctx->last.launches = rounds;
     if (kmax) *kmax = kb;
     return GK_OK;
 }
@@ -304,7 +303,7 @@ extern "C" int gk_truss_decompose(gk_ctx *ctx, const gk_csr *A, uint32_t *truss,
     uint32_t *dt = nullptr;
     if (hipMalloc(&dt, 4 * (A->nz ? A->nz : 1)) != hipSuccess) { free_csr(&d); set_error("hipMalloc failed"); return GK_ENOMEM; }
     int rc = gk_truss_decompose_dev(ctx, &d, dt, kmax, opts);
-    if (rc == GK_OK && A->nz && hipMemcpy(truss, dt, 4 * A->nz, hipMemcpyDeviceToHost) != hipSuccess) {
+    if (rc == GK_OK && A->nz && d2h(ctx, truss, dt, 4 * A->nz) != hipSuccess) {
         set_error("copy of truss numbers failed");
         rc = GK_EHIP;
     }
@@ -390,11 +389,11 @@ extern "C" int gk_ktruss(gk_ctx *ctx, const gk_csr *A, int k, int32_t *rst_out, 
             if ((rc = B.alloc(&tl, cur.nz))) break;
             if ((rc = zero_u32(ctx, tl, cur.nz))) break;
             Timer t(ctx);
+            ctx->last = gk_timing{};
             t.start();
             // Removed edges are labelled 1; alive ones stay 0.
             if ((rc = truss_rounds(ctx, B, cur, nxt, k, tl, o, &rounds, nullptr, nullptr))) break;
             t.stop();
-            ctx->last = gk_timing{};
             ctx->last.total_ms = t.ms();
             ctx->last.iterations = rounds;
             if ((rc = B.alloc(&cnt, A->m + 1)) || (rc = B.alloc(&nrst, A->m + 1)) || (rc = B.alloc(&keep, A->nz)) ||
@@ -405,10 +404,10 @@ extern "C" int gk_ktruss(gk_ctx *ctx, const gk_csr *A, int k, int32_t *rst_out, 
             if ((rc = exclusive_scan_i32(ctx, cnt, nrst, A->m + 1))) break;
             k_keep_sym_scatter<<<waves_grid(ctx, A->m), 256, 0, ctx->stream>>>(A->m, d.rst, d.col, keep, nrst, ncol);
             if ((rc = launch_check("k_keep_sym_scatter"))) break;
-            if (hipMemcpy(rst_out, nrst, 4 * (A->m + 1), hipMemcpyDeviceToHost) != hipSuccess) { rc = GK_EHIP; set_error("copy failed"); break; }
+            if (d2h(ctx, rst_out, nrst, 4 * (A->m + 1)) != hipSuccess) { rc = GK_EHIP; set_error("copy failed"); break; }
             *nz_out = rst_out[A->m];
             if (col_out && *nz_out &&
-                hipMemcpy(col_out, ncol, 4 * (size_t)*nz_out, hipMemcpyDeviceToHost) != hipSuccess) {
+                d2h(ctx, col_out, ncol, 4 * (size_t)*nz_out) != hipSuccess) {
                 rc = GK_EHIP; set_error("copy failed"); break;
             }
             if (iterations) *iterations = rounds;
@@ -440,11 +439,9 @@ extern "C" int gk_ktruss_support(gk_ctx *ctx, const gk_csr *A, uint32_t *support
             if (A->nz) {
                 k_truss_to_sym<<<waves_grid(ctx, A->m), 256, 0, ctx->stream>>>(A->m, d.rst, d.col, lrst_keep, sup, dout);
                 if ((rc = launch_check("k_truss_to_sym"))) break;
-                if (hipMemcpy(support, dout, 4 * A->nz, hipMemcpyDeviceToHost) != hipSuccess) { rc = GK_EHIP; set_error("copy failed"); break; }
+                if (d2h(ctx, support, dout, 4 * A->nz) != hipSuccess) { rc = GK_EHIP; set_error("copy failed"); break; }
             }
-            ctx->last = gk_timing{};
-            ctx->last.total_ms = pi.total_ms;
-            ctx->last.kernel_ms = pi.kernel_ms;
+            record_timing(ctx, pi, true);
         } while (0);
     }
     free_csr(&d);

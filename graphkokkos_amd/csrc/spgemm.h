@@ -48,14 +48,28 @@ struct Plan {
     int32_t acc_tasks[GK_NUM_ACC] = {};
     int32_t acc_maxmask[GK_NUM_ACC] = {};
     Task *tasks = nullptr;
-    int32_t *bin_off = nullptr;  // device, kNumBins + 1
+    int32_t *bin_off = nullptr;             // device, kNumBins + 1
+    unsigned long long *stats = nullptr;    // device, GK_NUM_ACC * ST_NUM
+    unsigned long long *hits = nullptr;     // device, GK_NUM_ACC
 };
+
+// Per-accumulator work counters gathered while binning (device, u64):
+// [acc][0] tasks, [acc][1] sum of mask lengths, [acc][2] sum of A-row
+// lengths, [acc][3] streamed B entries; then hits per accumulator.
+enum { ST_TASKS = 0, ST_MASK, ST_AROW, ST_FLOP, ST_NUM };
 
 struct ProductInfo {
     double total_ms = 0, select_ms = 0, kernel_ms = 0;
     int32_t ntasks = 0;
     int32_t acc_tasks[GK_NUM_ACC] = {};
+    double acc_ms[GK_NUM_ACC] = {};
+    double acc_bytes[GK_NUM_ACC] = {};
+    double flop = 0, hits = 0;
 };
+
+// Adds one product's figures to the context's gk_timing (k-truss rounds
+// and repeated products accumulate); reset=true starts a new record.
+void record_timing(gk_ctx *ctx, const ProductInfo &pi, bool reset);
 
 int row_stats(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcsr *M, RowStats *st);
 int plan_tasks(gk_ctx *ctx, const gk_dcsr *A, RowStats &st, const SelectParams &sp, Plan *plan);

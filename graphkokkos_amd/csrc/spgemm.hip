@@ -96,9 +96,12 @@ __device__ __forceinline__ int32_t task_count(const SelectParams &sp, int32_t al
 // st.acc), count its tasks and add them to the bin histogram.
 __global__ __launch_bounds__(256) void k_select_bin(int64_t m, RowStats st, SelectParams sp,
                                                     int32_t *__restrict__ bin_count,
-                                                    int32_t *__restrict__ acc_maxmask) {
+                                                    int32_t *__restrict__ acc_maxmask,
+                                                    unsigned long long *__restrict__ stats) {
     __shared__ int32_t h[kNumBins];
+    __shared__ unsigned long long ss[GK_NUM_ACC * ST_NUM];
     for (int b = threadIdx.x; b < kNumBins; b += blockDim.x) h[b] = 0;
+    for (int b = threadIdx.x; b < GK_NUM_ACC * ST_NUM; b += blockDim.x) ss[b] = 0;
     __syncthreads();
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
         int32_t mlen = st.mlen[i];
@@ -120,6 +123,10 @@ __global__ __launch_bounds__(256) void k_select_bin(int64_t m, RowStats st, Sele
             st.ntask[i] = nt;
             atomicAdd(&h[acc * kFlopBins + (kFlopBins - 1 - flop_bin(flop))], nt);
             atomicMax(&acc_maxmask[acc], mlen);
+            atomicAdd(&ss[acc * ST_NUM + ST_TASKS], (unsigned long long)nt);
+            atomicAdd(&ss[acc * ST_NUM + ST_MASK], (unsigned long long)mlen);
+            atomicAdd(&ss[acc * ST_NUM + ST_AROW], (unsigned long long)st.alen[i]);
+            atomicAdd(&ss[acc * ST_NUM + ST_FLOP], (unsigned long long)flop);
         } else {
             st.ntask[i] = 0;
         }
@@ -127,6 +134,8 @@ __global__ __launch_bounds__(256) void k_select_bin(int64_t m, RowStats st, Sele
     __syncthreads();
     for (int b = threadIdx.x; b < kNumBins; b += blockDim.x)
         if (h[b]) atomicAdd(&bin_count[b], h[b]);
+    for (int b = threadIdx.x; b < GK_NUM_ACC * ST_NUM; b += blockDim.x)
+        if (ss[b]) atomicAdd(&stats[b], ss[b]);
 }
 
 // Exclusive scan of the bin histogram (one block), also per-accumulator
@@ -174,6 +183,7 @@ struct Prob {
     int32_t *__restrict__ cval;                 // VALUES: C values; SUPPORT: uint32 supports
     unsigned long long *__restrict__ count;     // TC: total
     unsigned long long *__restrict__ rowcount;  // TC: optional per-row counts
+    unsigned long long *__restrict__ hits;      // hits per accumulator
     const Task *__restrict__ tasks;
     const int32_t *__restrict__ bin_off;
 };
@@ -231,7 +241,8 @@ struct Hit<MODE_TC> {
 template <>
 struct Hit<MODE_VALUES> {
     __device__ static void on(const Prob &P, int32_t *acc, int32_t *, int32_t q, int32_t a0, int32_t kk, int32_t t,
-                              uint32_t &) {
+                              uint32_t &cnt) {
+        ++cnt;
         uint32_t av = P.aval ? (uint32_t)P.aval[a0 + kk] : 1u;
         uint32_t bv = P.bval ? (uint32_t)P.bval[t] : 1u;
         atomicAdd((uint32_t *)&acc[q], av * bv);
@@ -240,7 +251,8 @@ struct Hit<MODE_VALUES> {
 template <>
 struct Hit<MODE_SUPPORT> {
     __device__ static void on(const Prob &P, int32_t *acc, int32_t *acnt, int32_t q, int32_t, int32_t kk, int32_t t,
-                              uint32_t &) {
+                              uint32_t &cnt) {
+        ++cnt;
         atomicAdd((uint32_t *)&acc[q], 1u);          // edge (i, j)
         atomicAdd((uint32_t *)&acnt[kk], 1u);        // edge (i, k)
         atomicAdd((uint32_t *)&P.cval[t], 1u);       // edge (k, j)
@@ -324,18 +336,19 @@ __global__ __launch_bounds__(64) void k_msa_lds(Prob P, int acc_id) {
         }
         __syncthreads();
         flush<MODE, 64>(P, tk, mb, mlen, acc, acnt, lane);
-        if (MODE == MODE_TC) {
-            if (P.rowcount) {
-                uint32_t r = dev::wave_sum(cnt_task);
-                if (lane == 0 && r) atomicAdd(&P.rowcount[tk.row], (unsigned long long)r);
-            }
-            cnt += cnt_task;
+        if (MODE == MODE_TC && P.rowcount) {
+            uint32_t r = dev::wave_sum(cnt_task);
+            if (lane == 0 && r) atomicAdd(&P.rowcount[tk.row], (unsigned long long)r);
         }
+        cnt += cnt_task;
         __syncthreads();
     }
-    if (MODE == MODE_TC) {
+    {
         unsigned long long s = dev::wave_sum((unsigned long long)cnt);
-        if (lane == 0 && s) atomicAdd(P.count, s);
+        if (lane == 0 && s) {
+            if (MODE == MODE_TC) atomicAdd(P.count, s);
+            atomicAdd(&P.hits[acc_id], s);
+        }
     }
 }
 
@@ -404,22 +417,22 @@ __global__ __launch_bounds__(64) void k_mca_wave(Prob P, int acc_id) {
         __syncthreads();
         if (live) {
             flush<MODE, G>(P, tk, mb, mlen, acc[tile], acnt[tile], gl);
-            if (MODE == MODE_TC) {
-                if (P.rowcount) {
-                    uint32_t s = cnt_task;
+            if (MODE == MODE_TC && P.rowcount) {
+                uint32_t s = cnt_task;
 #pragma unroll
-                    for (int o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, G);
-                    if (gl == 0 && s) atomicAdd(&P.rowcount[tk.row], (unsigned long long)s);
-                }
-                cnt += cnt_task;
+                for (int o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, G);
+                if (gl == 0 && s) atomicAdd(&P.rowcount[tk.row], (unsigned long long)s);
             }
+            cnt += cnt_task;
         }
         __syncthreads();
     }
-    if (MODE == MODE_TC) {
-        // popcount of the per-lane hit counts: the wave total.
+    {
         unsigned long long s = dev::wave_sum((unsigned long long)cnt);
-        if (lane == 0 && s) atomicAdd(P.count, s);
+        if (lane == 0 && s) {
+            if (MODE == MODE_TC) atomicAdd(P.count, s);
+            atomicAdd(&P.hits[acc_id], s);
+        }
     }
 }
 
@@ -482,19 +495,20 @@ __global__ __launch_bounds__(64) void k_hash_global(Prob P, int acc_id, int32_t 
         __threadfence();
         __syncthreads();
         flush<MODE, 64>(P, tk, mb, mlen, acc, acnt, lane);
-        if (MODE == MODE_TC) {
-            if (P.rowcount) {
-                uint32_t r = dev::wave_sum(cnt_task);
-                if (lane == 0 && r) atomicAdd(&P.rowcount[tk.row], (unsigned long long)r);
-            }
-            cnt += cnt_task;
+        if (MODE == MODE_TC && P.rowcount) {
+            uint32_t r = dev::wave_sum(cnt_task);
+            if (lane == 0 && r) atomicAdd(&P.rowcount[tk.row], (unsigned long long)r);
         }
+        cnt += cnt_task;
         __threadfence();
         __syncthreads();
     }
-    if (MODE == MODE_TC) {
+    {
         unsigned long long s = dev::wave_sum((unsigned long long)cnt);
-        if (lane == 0 && s) atomicAdd(P.count, s);
+        if (lane == 0 && s) {
+            if (MODE == MODE_TC) atomicAdd(P.count, s);
+            atomicAdd(&P.hits[acc_id], s);
+        }
     }
 }
 
@@ -539,19 +553,20 @@ __global__ __launch_bounds__(64) void k_dense_global(Prob P, int acc_id, int32_t
         __syncthreads();
         flush<MODE, 64>(P, tk, mb, mlen, acc, acnt, lane);
         for (int32_t q = lane; q < mlen; q += 64) map[P.mcol[mb + q]] = 0;
-        if (MODE == MODE_TC) {
-            if (P.rowcount) {
-                uint32_t r = dev::wave_sum(cnt_task);
-                if (lane == 0 && r) atomicAdd(&P.rowcount[tk.row], (unsigned long long)r);
-            }
-            cnt += cnt_task;
+        if (MODE == MODE_TC && P.rowcount) {
+            uint32_t r = dev::wave_sum(cnt_task);
+            if (lane == 0 && r) atomicAdd(&P.rowcount[tk.row], (unsigned long long)r);
         }
+        cnt += cnt_task;
         __threadfence();
         __syncthreads();
     }
-    if (MODE == MODE_TC) {
+    {
         unsigned long long s = dev::wave_sum((unsigned long long)cnt);
-        if (lane == 0 && s) atomicAdd(P.count, s);
+        if (lane == 0 && s) {
+            if (MODE == MODE_TC) atomicAdd(P.count, s);
+            atomicAdd(&P.hits[acc_id], s);
+        }
     }
 }
 
@@ -598,14 +613,21 @@ int row_stats(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcsr *M,
 int plan_tasks(gk_ctx *ctx, const gk_dcsr *A, RowStats &st, const SelectParams &sp, Plan *plan) {
     const int64_t m = A->m;
     void *p;
+    // Layout: u64 stats[GK_NUM_ACC * ST_NUM], u64 hits[GK_NUM_ACC], then the
+    // int32 bin counters, offsets, cursors and per-accumulator max mask.
+    const size_t u64s = GK_NUM_ACC * ST_NUM + GK_NUM_ACC;
     const size_t words = 3 * kNumBins + 1 + GK_NUM_ACC;
-    GK_TRY(scratch(ctx, SC_BINS, sizeof(int32_t) * words, &p));
-    int32_t *bin_count = (int32_t *)p, *bin_off = bin_count + kNumBins, *bin_cur = bin_off + kNumBins + 1;
+    const size_t bytes = 8 * u64s + sizeof(int32_t) * words;
+    GK_TRY(scratch(ctx, SC_BINS, bytes, &p));
+    plan->stats = (unsigned long long *)p;
+    plan->hits = plan->stats + GK_NUM_ACC * ST_NUM;
+    int32_t *bin_count = (int32_t *)(plan->hits + GK_NUM_ACC), *bin_off = bin_count + kNumBins;
+    int32_t *bin_cur = bin_off + kNumBins + 1;
     int32_t *acc_maxmask = bin_cur + kNumBins;
-    GK_HIP(hipMemsetAsync(bin_count, 0, sizeof(int32_t) * words, ctx->stream));
+    GK_HIP(hipMemsetAsync(p, 0, bytes, ctx->stream));
     int blocks = grid_for(ctx, (m + 255) / 256, 8);
     if (m > 0) {
-        k_select_bin<<<blocks, 256, 0, ctx->stream>>>(m, st, sp, bin_count, acc_maxmask);
+        k_select_bin<<<blocks, 256, 0, ctx->stream>>>(m, st, sp, bin_count, acc_maxmask, plan->stats);
         GK_TRY(launch_check("k_select_bin"));
     }
     k_bin_scan<<<1, 64, 0, ctx->stream>>>(bin_count, bin_off, bin_cur);
@@ -660,17 +682,23 @@ static int dense_maps(gk_ctx *ctx, int64_t blocks, int64_t ncols, int32_t **out)
 }
 
 template <int MODE>
-static int run_mode(gk_ctx *ctx, const Prob &P, const Plan &plan, int64_t bn, const gk_opts &o, double *kernel_ms) {
-    Timer t(ctx);
-    t.start();
+static int run_mode(gk_ctx *ctx, const Prob &P, const Plan &plan, int64_t bn, double *acc_ms) {
+    Timer tm[GK_NUM_ACC] = {Timer(ctx), Timer(ctx), Timer(ctx), Timer(ctx)};
+    bool ran[GK_NUM_ACC] = {};
     if (plan.acc_tasks[GK_ACC_MSA_LDS]) {
         int g = grid_for(ctx, plan.acc_tasks[GK_ACC_MSA_LDS], kLdsBlocksPerCu);
+        tm[GK_ACC_MSA_LDS].start();
         k_msa_lds<MODE><<<g, 64, 0, ctx->stream>>>(P, GK_ACC_MSA_LDS);
+        tm[GK_ACC_MSA_LDS].stop();
+        ran[GK_ACC_MSA_LDS] = true;
         GK_TRY(launch_check("k_msa_lds"));
     }
     if (plan.acc_tasks[GK_ACC_MCA_WAVE]) {
         int g = grid_for(ctx, (plan.acc_tasks[GK_ACC_MCA_WAVE] + 3) / 4, kMcaBlocksPerCu);
+        tm[GK_ACC_MCA_WAVE].start();
         k_mca_wave<MODE, 16><<<g, 64, 0, ctx->stream>>>(P, GK_ACC_MCA_WAVE);
+        tm[GK_ACC_MCA_WAVE].stop();
+        ran[GK_ACC_MCA_WAVE] = true;
         GK_TRY(launch_check("k_mca_wave"));
     }
     if (plan.acc_tasks[GK_ACC_HASH_GLOBAL]) {
@@ -683,7 +711,10 @@ static int run_mode(gk_ctx *ctx, const Prob &P, const Plan &plan, int64_t bn, co
         while (g > 1 && (size_t)g * cap * 10 > ((size_t)4 << 30)) g >>= 1;
         GK_TRY(scratch(ctx, SC_ACC, (size_t)g * cap * 4 * 2 + (size_t)g * (cap / 2) * 4, &sp));
         int32_t *key = (int32_t *)sp, *pos = key + (size_t)g * cap, *acc = pos + (size_t)g * cap;
+        tm[GK_ACC_HASH_GLOBAL].start();
         k_hash_global<MODE><<<g, 64, 0, ctx->stream>>>(P, GK_ACC_HASH_GLOBAL, key, pos, acc, lg);
+        tm[GK_ACC_HASH_GLOBAL].stop();
+        ran[GK_ACC_HASH_GLOBAL] = true;
         GK_TRY(launch_check("k_hash_global"));
     }
     if (plan.acc_tasks[GK_ACC_DENSE_GLOBAL]) {
@@ -695,12 +726,13 @@ static int run_mode(gk_ctx *ctx, const Prob &P, const Plan &plan, int64_t bn, co
         GK_TRY(dense_maps(ctx, g, bn, &map));
         void *sp;
         GK_TRY(scratch(ctx, SC_TMP7, (size_t)g * (size_t)mm * 4 + 4, &sp));
+        tm[GK_ACC_DENSE_GLOBAL].start();
         k_dense_global<MODE><<<g, 64, 0, ctx->stream>>>(P, GK_ACC_DENSE_GLOBAL, map, (int32_t *)sp, bn, mm);
+        tm[GK_ACC_DENSE_GLOBAL].stop();
+        ran[GK_ACC_DENSE_GLOBAL] = true;
         GK_TRY(launch_check("k_dense_global"));
     }
-    t.stop();
-    *kernel_ms = t.ms();
-    (void)o;
+    for (int a = 0; a < GK_NUM_ACC; ++a) acc_ms[a] = ran[a] ? tm[a].ms() : 0.0;
     return GK_OK;
 }
 
@@ -737,21 +769,59 @@ int masked_product(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcs
     P.arst = A->rst; P.acol = A->col; P.aval = A->val;
     P.brst = B->rst; P.bcol = B->col; P.bval = B->val;
     P.mrst = M->rst; P.mcol = M->col;
-    P.cval = d_out; P.count = d_count; P.rowcount = d_rowcount;
+    P.cval = d_out; P.count = d_count; P.rowcount = d_rowcount; P.hits = plan.hits;
     P.tasks = plan.tasks; P.bin_off = plan.bin_off;
-    double kms = 0;
-    if (mode == MODE_TC) GK_TRY(run_mode<MODE_TC>(ctx, P, plan, B->n, o, &kms));
-    else if (mode == MODE_VALUES) GK_TRY(run_mode<MODE_VALUES>(ctx, P, plan, B->n, o, &kms));
-    else GK_TRY(run_mode<MODE_SUPPORT>(ctx, P, plan, B->n, o, &kms));
+    double acc_ms[GK_NUM_ACC] = {};
+    if (mode == MODE_TC) GK_TRY(run_mode<MODE_TC>(ctx, P, plan, B->n, acc_ms));
+    else if (mode == MODE_VALUES) GK_TRY(run_mode<MODE_VALUES>(ctx, P, plan, B->n, acc_ms));
+    else GK_TRY(run_mode<MODE_SUPPORT>(ctx, P, plan, B->n, acc_ms));
     total.stop();
     if (info) {
+        unsigned long long hs[GK_NUM_ACC * (ST_NUM + 1)];
+        GK_HIP(d2h(ctx, hs, plan.stats, sizeof hs));
+        const bool same_am = A->col == M->col && A->rst == M->rst;
+        *info = ProductInfo{};
         info->total_ms = total.ms();
         info->select_ms = sel.ms();
-        info->kernel_ms = kms;
         info->ntasks = plan.ntasks;
-        for (int a = 0; a < GK_NUM_ACC; ++a) info->acc_tasks[a] = plan.acc_tasks[a];
+        for (int a = 0; a < GK_NUM_ACC; ++a) {
+            const unsigned long long *x = hs + a * ST_NUM;
+            const double hits = (double)hs[GK_NUM_ACC * ST_NUM + a];
+            // Algorithmic HBM bytes: mask row pointers and columns, A columns
+            // unless A is the mask, a B row-pointer pair per A entry, the
+            // streamed B columns, and the outputs (C values once; supports:
+            // three 4-byte increments per triangle).
+            double b = 8.0 * x[ST_TASKS] + 4.0 * x[ST_MASK] + (same_am ? 0.0 : 4.0 * x[ST_AROW]) +
+                       8.0 * x[ST_AROW] + 4.0 * x[ST_FLOP];
+            if (mode == MODE_VALUES) b += 4.0 * x[ST_MASK];
+            if (mode == MODE_SUPPORT) b += 12.0 * hits;
+            info->acc_tasks[a] = plan.acc_tasks[a];
+            info->acc_ms[a] = acc_ms[a];
+            info->acc_bytes[a] = b;
+            info->kernel_ms += acc_ms[a];
+            info->flop += (double)x[ST_FLOP];
+            info->hits += hits;
+        }
     }
     return GK_OK;
+}
+
+void record_timing(gk_ctx *ctx, const ProductInfo &pi, bool reset) {
+    gk_timing &t = ctx->last;
+    if (reset) t = gk_timing{};
+    t.total_ms += pi.total_ms;
+    t.kernel_ms += pi.kernel_ms;
+    t.select_ms += pi.select_ms;
+    t.launches += 1;
+    for (int a = 0; a < GK_NUM_ACC; ++a) {
+        t.rows_per_acc[a] += pi.acc_tasks[a];
+        t.acc_ms[a] += pi.acc_ms[a];
+        t.acc_bytes[a] += pi.acc_bytes[a];
+        t.acc_launches[a] += pi.acc_tasks[a] ? 1 : 0;
+        t.bytes_alg += pi.acc_bytes[a];
+    }
+    t.flop += pi.flop;
+    t.hits += pi.hits;
 }
 
 }  // namespace gk
@@ -759,24 +829,13 @@ int masked_product(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcs
 // ---------------------------------------------------------------------------
 // C ABI
 
-static void fill_timing(gk_ctx *ctx, const ProductInfo &pi, double bytes_alg) {
-    gk_timing &t = ctx->last;
-    t = gk_timing{};
-    t.total_ms = pi.total_ms;
-    t.kernel_ms = pi.kernel_ms;
-    t.select_ms = pi.select_ms;
-    t.launches = 1;
-    for (int a = 0; a < GK_NUM_ACC; ++a) t.rows_per_acc[a] = pi.acc_tasks[a];
-    t.bytes_alg = bytes_alg;
-}
-
 extern "C" int gk_masked_spgemm_dev(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcsr *M,
                                     int32_t *d_cval, const gk_opts *opts) {
     GK_CHECK_ARG(ctx && A && B && M && (d_cval || M->nz == 0), "null argument");
     gk_opts o = resolve_opts(opts);
     ProductInfo pi;
     GK_TRY(masked_product(ctx, A, B, M, MODE_VALUES, d_cval, nullptr, nullptr, o, &pi));
-    fill_timing(ctx, pi, 0);
+    record_timing(ctx, pi, true);
     return GK_OK;
 }
 
@@ -800,7 +859,7 @@ extern "C" int gk_masked_spgemm_i32(gk_ctx *ctx, const gk_csr *A, const gk_csr *
     }
     rc = gk_masked_spgemm_dev(ctx, &dA, &dB, &dM, d_c, opts);
     if (rc == GK_OK && M->nz) {
-        if (hipMemcpy(c_val, d_c, sizeof(int32_t) * M->nz, hipMemcpyDeviceToHost) != hipSuccess) {
+        if (d2h(ctx, c_val, d_c, sizeof(int32_t) * M->nz) != hipSuccess) {
             set_error("copy of C to host failed");
             rc = GK_EHIP;
         }
@@ -825,7 +884,7 @@ extern "C" int gk_tc_count_dev(gk_ctx *ctx, const gk_dcsr *L, uint64_t *count, c
     GK_HIP(hipMemcpyAsync(h, d_count, 8, hipMemcpyDeviceToHost, ctx->stream));
     GK_HIP(hipStreamSynchronize(ctx->stream));
     *count = *h;
-    fill_timing(ctx, pi, 0);
+    record_timing(ctx, pi, true);
     return GK_OK;
 }
 
@@ -839,7 +898,7 @@ extern "C" int gk_tc_rows_dev(gk_ctx *ctx, const gk_dcsr *L, uint64_t *d_rowcoun
     GK_HIP(hipMemsetAsync(d_rowcount, 0, 8 * (size_t)L->m, ctx->stream));
     ProductInfo pi;
     GK_TRY(masked_product(ctx, L, L, L, MODE_TC, nullptr, d_count, (unsigned long long *)d_rowcount, o, &pi));
-    fill_timing(ctx, pi, 0);
+    record_timing(ctx, pi, true);
     return GK_OK;
 }
 

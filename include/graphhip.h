@@ -97,15 +97,22 @@ typedef struct gk_opts {
 } gk_opts;
 void gk_opts_default(gk_opts *o);
 
-/* Per-call timings (device time from HIP events on the context stream). */
+/* Per-call timings (device time from HIP events on the context stream) and
+ * algorithmic HBM bytes of the product kernels, summed over the launches of
+ * the call (k-truss: every round; BC: every level).                         */
 typedef struct gk_timing {
-    double total_ms;        /* whole entry point, device side             */
-    double kernel_ms;       /* dominant kernel(s)                         */
-    double select_ms;       /* feature + selector kernels                 */
-    int64_t launches;       /* dominant-kernel launches                   */
-    int64_t iterations;     /* k-truss rounds / BC levels                 */
-    int64_t rows_per_acc[GK_NUM_ACC];
-    double  bytes_alg;      /* algorithmic HBM bytes of the dominant kernel(s) */
+    double  total_ms;                 /* whole entry point, device side          */
+    double  kernel_ms;                /* accumulator kernels, all launches       */
+    double  select_ms;                /* features + selection + binning          */
+    int64_t launches;                 /* product launches                        */
+    int64_t iterations;               /* k-truss rounds / BC levels              */
+    int64_t rows_per_acc[GK_NUM_ACC]; /* tasks per accumulator                   */
+    double  acc_ms[GK_NUM_ACC];       /* device time per accumulator kernel      */
+    double  acc_bytes[GK_NUM_ACC];    /* algorithmic bytes per accumulator kernel*/
+    int64_t acc_launches[GK_NUM_ACC]; /* launches per accumulator kernel         */
+    double  bytes_alg;                /* algorithmic bytes, all product kernels  */
+    double  flop;                     /* streamed B entries (sum_k |B_k|)         */
+    double  hits;                     /* mask hits (triangles for TC)            */
 } gk_timing;
 int gk_last_timing(gk_ctx *ctx, gk_timing *out);
 
