@@ -45,7 +45,7 @@ def log(*a):
 
 
 def dominant(t: dict) -> dict:
-    from graphkokkos_amd import ACC_NAMES
+    from graphkokkos_amd._lib import KERNEL_NAMES as ACC_NAMES
     a = int(np.argmax(t["acc_ms"]))
     launches = max(int(t["acc_launches"][a]), 1)
     ms = t["acc_ms"][a]

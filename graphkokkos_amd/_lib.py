@@ -16,7 +16,9 @@ GK_EINVAL, GK_ENOMEM, GK_EHIP, GK_EIO, GK_ERANGE, GK_ESTATE = -1, -2, -3, -4, -5
 
 ACC_AUTO, ACC_MSA_LDS, ACC_MCA_WAVE, ACC_HASH_GLOBAL, ACC_DENSE_GLOBAL = -1, 0, 1, 2, 3
 NUM_ACC = 4
+NUM_KERNELS = 5
 ACC_NAMES = ["msa-lds", "mca-wave", "hash-global", "dense-global"]
+KERNEL_NAMES = ["msa-lds", "mca-wave", "hash-global", "dense-global", "msa-lds-long"]
 SEL_HEURISTIC, SEL_LEARNED, SEL_FORCED = 0, 1, 2
 
 i32p = C.POINTER(C.c_int32)
@@ -41,8 +43,8 @@ class GkOpts(C.Structure):
 class GkTiming(C.Structure):
     _fields_ = [("total_ms", C.c_double), ("kernel_ms", C.c_double), ("select_ms", C.c_double),
                 ("launches", C.c_int64), ("iterations", C.c_int64), ("rows_per_acc", C.c_int64 * NUM_ACC),
-                ("acc_ms", C.c_double * NUM_ACC), ("acc_bytes", C.c_double * NUM_ACC),
-                ("acc_launches", C.c_int64 * NUM_ACC), ("bytes_alg", C.c_double), ("flop", C.c_double),
+                ("acc_ms", C.c_double * NUM_KERNELS), ("acc_bytes", C.c_double * NUM_KERNELS),
+                ("acc_launches", C.c_int64 * NUM_KERNELS), ("bytes_alg", C.c_double), ("flop", C.c_double),
                 ("hits", C.c_double)]
 
 

@@ -2,16 +2,21 @@
 //
 // GraphKokkos's k_truss (src/main.cpp:93-141) repeats
 //     Tmp<C> = C*C;  keep entries with Tmp >= k-2 (values reset to 1)
-// until an iteration removes nothing, with a serial host compaction.  Here
-// the graph lives in HBM as its strictly-lower part L (every undirected
-// edge once, rows sorted), and one round is
-//     1. supports: the masked product C<L> = L*L in SUPPORT mode, which adds
-//        every triangle (j<k<i) to its three edges -- the values of
-//        S<A> = A*A at A's pattern, computed on the oriented graph;
-//     2. threshold: k = max(k, min support + 3) (decomposition) or the
-//        fixed k (single k-truss);
-//     3. prune + compact on the device: per-row keep counts, exclusive scan,
-//        order-preserving scatter; removed edges get truss number k-1.
+// until an iteration removes nothing, recomputing every support each time
+// and compacting on one host thread.  Here the graph stays in HBM and the
+// supports are kept up to date instead of recomputed:
+//     0. S<L> = L*L in SUPPORT mode: every triangle (j<k<i) adds 1 to its
+//        three edges -- S<A> = A*A at A's pattern, on the oriented graph;
+//     per round (same rounds as the reference's iterations):
+//     1. frontier F = alive edges with S < k-2 (labelled k-1); if F is empty
+//        the decomposition moves to k = min S + 3;
+//     2. dS = the masked product of F (one row per edge, at its endpoint
+//        with the longer row) with the alive adjacency, in PEEL mode: each
+//        triangle destroyed this round takes one support off each of its
+//        surviving edges;
+//     3. F is removed; the adjacency is compacted on the device (per-row
+//        keep counts, exclusive scan, order-preserving scatter) once a
+//        quarter of its entries are dead.
 // Rounds repeat until no edge is left (decomposition) or none is removed
 // (single k).  Truss numbers come back per entry of the symmetric pattern.
 #include <cstring>
@@ -60,54 +65,6 @@ __global__ void k_lower_copy(int64_t m, const int32_t *__restrict__ rst, const i
         for (int32_t x = lane; x < n; x += 64) {
             lcol[lrst[i] + x] = col[rst[i] + x];
             lid[lrst[i] + x] = lrst[i] + x;
-        }
-    }
-}
-
-// Minimum support over the current edges.
-__global__ void k_min_u32(int64_t n, const uint32_t *__restrict__ v, uint32_t *__restrict__ out) {
-    uint32_t mn = 0xffffffffu;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        mn = min(mn, v[i]);
-    mn = dev::wave_max(~mn);
-    if ((threadIdx.x & 63) == 0) atomicMax(out, mn);  // out holds ~min
-}
-
-__global__ void k_keep_count(int64_t m, const int32_t *__restrict__ rst, const uint32_t *__restrict__ sup,
-                             uint32_t thresh, int32_t *__restrict__ cnt) {
-    const int lane = threadIdx.x & 63;
-    const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
-    for (int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < m; i += waves) {
-        int32_t c = 0;
-        for (int32_t p = rst[i] + lane; p < rst[i + 1]; p += 64) c += sup[p] >= thresh;
-        c = dev::wave_sum(c);
-        if (lane == 0) cnt[i] = c;
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) cnt[m] = 0;
-}
-
-// Order-preserving compaction of one round; removed edges get `label`.
-__global__ void k_keep_scatter(int64_t m, const int32_t *__restrict__ rst, const int32_t *__restrict__ col,
-                               const int32_t *__restrict__ id, const uint32_t *__restrict__ sup, uint32_t thresh,
-                               const int32_t *__restrict__ nrst, int32_t *__restrict__ ncol,
-                               int32_t *__restrict__ nid, uint32_t *__restrict__ truss, uint32_t label) {
-    const int lane = threadIdx.x & 63;
-    const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
-    for (int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < m; i += waves) {
-        int32_t out = nrst[i];
-        for (int32_t base = rst[i]; base < rst[i + 1]; base += 64) {
-            int32_t p = base + lane;
-            bool valid = p < rst[i + 1];
-            bool keep = valid && sup[p] >= thresh;
-            uint64_t bal = __ballot(keep);
-            int32_t before = __popcll(bal & ((1ull << lane) - 1));
-            if (keep) {
-                ncol[out + before] = col[p];
-                nid[out + before] = id[p];
-            } else if (valid && truss) {
-                truss[id[p]] = label;
-            }
-            out += __popcll(bal);
         }
     }
 }
@@ -189,72 +146,300 @@ int build_lower(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, Lgraph *L) {
     return launch_check("k_lower_copy");
 }
 
-// The round loop.  fixed_k > 0: single k-truss (stop when nothing is
-// removed); fixed_k == 0: decomposition (stop when no edge is left).
-// truss_l (per L0 edge) gets labels; alive graph left in *cur.
-int truss_rounds(gk_ctx *ctx, TrussBufs &B, Lgraph &cur, Lgraph &nxt, int fixed_k, uint32_t *truss_l,
-                 const gk_opts &o, int64_t *rounds, int32_t *kmax, double *support_ms) {
-    uint32_t *sup, *dmin;
-    int32_t *cnt;
-    GK_TRY(B.alloc(&sup, cur.nz));
-    GK_TRY(B.alloc(&cnt, cur.m + 1));
+// ---- incremental peeling ------------------------------------------------
+// Edge ids are positions in L0 (the strict lower part of the input).  The
+// alive adjacency Aal is the symmetric pattern with the edge id of every
+// entry; it is compacted when enough of its entries are dead.
+
+__global__ void k_edge_ends(int64_t m, const int32_t *__restrict__ lrst, const int32_t *__restrict__ lcol,
+                            int32_t *__restrict__ eu, int32_t *__restrict__ ev) {
+    const int lane = threadIdx.x & 63;
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < m; i += waves)
+        for (int32_t p = lrst[i] + lane; p < lrst[i + 1]; p += 64) {
+            eu[p] = (int32_t)i;
+            ev[p] = lcol[p];
+        }
+}
+
+// Edge id of every entry of the symmetric pattern.
+__global__ void k_sym_eid(int64_t m, const int32_t *__restrict__ rst, const int32_t *__restrict__ col,
+                          const int32_t *__restrict__ lrst, int32_t *__restrict__ eid) {
+    const int lane = threadIdx.x & 63;
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < m; i += waves) {
+        for (int32_t p = rst[i] + lane; p < rst[i + 1]; p += 64) {
+            int32_t j = col[p];
+            if (j < i) {
+                eid[p] = lrst[i] + (p - rst[i]);
+            } else {
+                int32_t l = rst[j], h = rst[j + 1];
+                while (l < h) {
+                    int32_t mid = (l + h) >> 1;
+                    if (col[mid] < (int32_t)i) l = mid + 1; else h = mid;
+                }
+                eid[p] = lrst[j] + (l - rst[j]);
+            }
+        }
+    }
+}
+
+__global__ void k_iota(int64_t n, int32_t *__restrict__ a) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        a[i] = (int32_t)i;
+}
+
+// Frontier of a round: alive edges with support below `thresh` become
+// frontier edges (state 1) labelled `label`, appended to F.
+__global__ void k_frontier(int64_t na, const int32_t *__restrict__ alist, uint8_t *__restrict__ st,
+                           const int32_t *__restrict__ sup, int32_t thresh, uint32_t *__restrict__ truss,
+                           uint32_t label, int32_t *__restrict__ F, int32_t *__restrict__ nF) {
+    const int lane = threadIdx.x & 63;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t base = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63); base < na; base += stride) {
+        int64_t x = base + lane;
+        bool take = false;
+        int32_t e = 0;
+        if (x < na) {
+            e = alist[x];
+            take = st[e] == 0 && sup[e] < thresh;
+        }
+        uint64_t bal = __ballot(take);
+        if (!bal) continue;
+        int32_t first = 0;
+        if (lane == 0) first = atomicAdd(nF, __popcll(bal));
+        first = __shfl(first, 0, 64);
+        if (take) {
+            F[first + __popcll(bal & ((1ull << lane) - 1))] = e;
+            st[e] = 1;
+            truss[e] = label;
+        }
+    }
+}
+
+__global__ void k_min_alive(int64_t na, const int32_t *__restrict__ alist, const uint8_t *__restrict__ st,
+                            const int32_t *__restrict__ sup, uint32_t *__restrict__ out) {
+    uint32_t mn = 0xffffffffu;
+    for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < na; x += (int64_t)gridDim.x * blockDim.x) {
+        int32_t e = alist[x];
+        if (st[e] == 0) mn = min(mn, (uint32_t)sup[e]);
+    }
+    mn = dev::wave_max(~mn);
+    if ((threadIdx.x & 63) == 0 && mn) atomicMax(out, mn);
+}
+
+// Frontier matrix: every frontier edge goes to the row of its endpoint with
+// the longer current adjacency row (ties: larger vertex id), so each edge is
+// processed once and the shorter row is the one streamed.
+__global__ void k_fcount(int32_t nF, const int32_t *__restrict__ F, const int32_t *__restrict__ eu,
+                         const int32_t *__restrict__ ev, const int32_t *__restrict__ rst,
+                         int32_t *__restrict__ fy, int32_t *__restrict__ fcnt) {
+    for (int32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < nF; f += gridDim.x * blockDim.x) {
+        int32_t e = F[f], u = eu[e], v = ev[e];
+        int32_t du = rst[u + 1] - rst[u], dv = rst[v + 1] - rst[v];
+        int32_t y = (du > dv || (du == dv && u > v)) ? u : v;
+        fy[f] = y;
+        atomicAdd(&fcnt[y], 1);
+    }
+}
+
+__global__ void k_fscatter(int32_t nF, const int32_t *__restrict__ F, const int32_t *__restrict__ eu,
+                           const int32_t *__restrict__ ev, const int32_t *__restrict__ fy,
+                           int32_t *__restrict__ fcur, int32_t *__restrict__ fcol, int32_t *__restrict__ fval) {
+    for (int32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < nF; f += gridDim.x * blockDim.x) {
+        int32_t e = F[f], y = fy[f];
+        int32_t x = eu[e] == y ? ev[e] : eu[e];
+        int32_t pos = atomicAdd(&fcur[y], 1);
+        fcol[pos] = x;
+        fval[pos] = e;
+    }
+}
+
+__global__ void k_mark_dead(int32_t nF, const int32_t *__restrict__ F, uint8_t *__restrict__ st) {
+    for (int32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < nF; f += gridDim.x * blockDim.x) st[F[f]] = 2;
+}
+
+// Compaction of the alive adjacency (rows stay sorted).
+__global__ void k_alive_count(int64_t m, const int32_t *__restrict__ rst, const int32_t *__restrict__ eid,
+                              const uint8_t *__restrict__ st, int32_t *__restrict__ cnt) {
+    const int lane = threadIdx.x & 63;
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < m; i += waves) {
+        int32_t c = 0;
+        for (int32_t p = rst[i] + lane; p < rst[i + 1]; p += 64) c += st[eid[p]] != 2;
+        c = dev::wave_sum(c);
+        if (lane == 0) cnt[i] = c;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) cnt[m] = 0;
+}
+
+__global__ void k_alive_scatter(int64_t m, const int32_t *__restrict__ rst, const int32_t *__restrict__ col,
+                                const int32_t *__restrict__ eid, const uint8_t *__restrict__ st,
+                                const int32_t *__restrict__ nrst, int32_t *__restrict__ ncol,
+                                int32_t *__restrict__ neid) {
+    const int lane = threadIdx.x & 63;
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < m; i += waves) {
+        int32_t out = nrst[i];
+        for (int32_t base = rst[i]; base < rst[i + 1]; base += 64) {
+            int32_t p = base + lane;
+            bool k = p < rst[i + 1] && st[eid[p]] != 2;
+            uint64_t bal = __ballot(k);
+            if (k) {
+                int32_t o = out + __popcll(bal & ((1ull << lane) - 1));
+                ncol[o] = col[p];
+                neid[o] = eid[p];
+            }
+            out += __popcll(bal);
+        }
+    }
+}
+
+__global__ void k_alist_compact(int64_t na, const int32_t *__restrict__ alist, const uint8_t *__restrict__ st,
+                                int32_t *__restrict__ out, int32_t *__restrict__ nout) {
+    const int lane = threadIdx.x & 63;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t base = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63); base < na; base += stride) {
+        int64_t x = base + lane;
+        int32_t e = x < na ? alist[x] : 0;
+        bool k = x < na && st[e] == 0;
+        uint64_t bal = __ballot(k);
+        if (!bal) continue;
+        int32_t first = 0;
+        if (lane == 0) first = atomicAdd(nout, __popcll(bal));
+        first = __shfl(first, 0, 64);
+        if (k) out[first + __popcll(bal & ((1ull << lane) - 1))] = e;
+    }
+}
+
+int grid_items(gk_ctx *ctx, int64_t n) {
+    return (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, (int64_t)ctx->num_cus * 16));
+}
+
+// Truss numbers of every L0 edge by incremental peeling.  Round:
+//   F = alive edges with support < k-2 (labelled k-1);  none -> k = min + 3;
+//   dS = PEEL product of F (rows: one endpoint) with the alive adjacency,
+//        which takes one support off each surviving edge of every triangle
+//        destroyed this round;
+//   F -> removed; compact the adjacency once a quarter of it is dead.
+// fixed_k > 0 runs the single-k semantics instead (k_truss, src/main.cpp:93-141):
+// the threshold stays k-2, removed edges are labelled 1, and the run stops at
+// the first round that removes nothing; the returned count then includes that
+// last round, as the reference's iteration count does.
+int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, uint32_t *tl, const gk_opts &o,
+               int64_t *rounds_out, int32_t *kmax_out, int fixed_k = 0) {
+    const int64_t m = A->m, nL = L0.nz;
+    int32_t *sup, *eu, *ev, *alist, *alist2, *F, *fy, *fcnt, *frst, *fcur, *fcol, *fval, *cnt;
+    int32_t *arst[2], *acol[2], *aeid[2];
+    uint8_t *st;
+    uint32_t *dmin;
+    int32_t *dn;
+    GK_TRY(B.alloc(&sup, nL));
+    GK_TRY(B.alloc(&eu, nL));
+    GK_TRY(B.alloc(&ev, nL));
+    GK_TRY(B.alloc(&alist, nL));
+    GK_TRY(B.alloc(&alist2, nL));
+    GK_TRY(B.alloc(&F, nL));
+    GK_TRY(B.alloc(&fy, nL));
+    GK_TRY(B.alloc(&fcol, nL));
+    GK_TRY(B.alloc(&fval, nL));
+    GK_TRY(B.alloc(&st, nL));
+    GK_TRY(B.alloc(&fcnt, m + 1));
+    GK_TRY(B.alloc(&frst, m + 1));
+    GK_TRY(B.alloc(&fcur, m + 1));
+    GK_TRY(B.alloc(&cnt, m + 1));
     GK_TRY(B.alloc(&dmin, 1));
-    uint32_t *h;
-    GK_TRY(pinned(ctx, 16, (void **)&h));
-    int k = fixed_k > 0 ? fixed_k : 3;
-    int32_t kbest = cur.nz ? 2 : 0;
-    int64_t r = 0;
-    double sms = 0;
-    while (cur.nz > 0) {
-        ++r;
-        gk_dcsr L = view(ctx, cur.m, cur.m, cur.nz, cur.rst, cur.col, nullptr);
+    GK_TRY(B.alloc(&dn, 2));
+    for (int b = 0; b < 2; ++b) {
+        GK_TRY(B.alloc(&arst[b], m + 1));
+        GK_TRY(B.alloc(&acol[b], A->nz));
+        GK_TRY(B.alloc(&aeid[b], A->nz));
+    }
+    const int gw = waves_grid(ctx, m);
+    // supports of the whole graph: C<L0> = L0*L0 in SUPPORT mode
+    {
+        gk_dcsr Ld = view(ctx, m, m, nL, L0.rst, L0.col, nullptr);
         ProductInfo pi;
-        GK_TRY(masked_product(ctx, &L, &L, &L, MODE_SUPPORT, (int32_t *)sup, nullptr, nullptr, o, &pi));
+        GK_TRY(masked_product(ctx, &Ld, &Ld, &Ld, MODE_SUPPORT, sup, nullptr, nullptr, o, &pi));
         record_timing(ctx, pi, false);
-        sms += pi.kernel_ms;
-        uint32_t thresh;
-        if (fixed_k > 0) {
-            thresh = (uint32_t)std::max(0, fixed_k - 2);
-        } else {
+    }
+    k_edge_ends<<<gw, 256, 0, ctx->stream>>>(m, L0.rst, L0.col, eu, ev);
+    GK_HIP(hipMemcpyAsync(arst[0], A->rst, 4 * (m + 1), hipMemcpyDeviceToDevice, ctx->stream));
+    GK_HIP(hipMemcpyAsync(acol[0], A->col, 4 * (size_t)A->nz, hipMemcpyDeviceToDevice, ctx->stream));
+    k_sym_eid<<<gw, 256, 0, ctx->stream>>>(m, A->rst, A->col, L0.rst, aeid[0]);
+    k_iota<<<grid_items(ctx, nL), 256, 0, ctx->stream>>>(nL, alist);
+    GK_HIP(hipMemsetAsync(st, 0, nL, ctx->stream));
+    GK_TRY(launch_check("truss_peel setup"));
+    int cur = 0;
+    int64_t aal_nz = A->nz;      // entries in the current adjacency
+    int64_t na = nL;             // entries in the alive list
+    int64_t alive = nL, dead_pending = 0;
+    int64_t rounds = 0;
+    int k = fixed_k > 0 ? fixed_k : 3;
+    int32_t kbest = nL ? 2 : 0;
+    int32_t *h;
+    GK_TRY(pinned(ctx, 16, (void **)&h));
+    while (alive > 0) {
+        const uint32_t label = fixed_k > 0 ? 1u : (uint32_t)(k - 1);
+        GK_HIP(hipMemsetAsync(dn, 0, 4, ctx->stream));
+        k_frontier<<<grid_items(ctx, na), 256, 0, ctx->stream>>>(na, alist, st, sup, k - 2, tl, label, F, dn);
+        GK_TRY(launch_check("k_frontier"));
+        GK_HIP(hipMemcpyAsync(h, dn, 4, hipMemcpyDeviceToHost, ctx->stream));
+        GK_HIP(hipStreamSynchronize(ctx->stream));
+        const int32_t nF = h[0];
+        if (nF == 0 && fixed_k > 0) break;
+        if (nF == 0) {
             GK_HIP(hipMemsetAsync(dmin, 0, 4, ctx->stream));
-            int g = (int)std::max<int64_t>(1, std::min<int64_t>((cur.nz + 255) / 256, ctx->num_cus * 8));
-            k_min_u32<<<g, 256, 0, ctx->stream>>>(cur.nz, sup, dmin);
-            GK_TRY(launch_check("k_min_u32"));
+            k_min_alive<<<grid_items(ctx, na), 256, 0, ctx->stream>>>(na, alist, st, sup, dmin);
+            GK_TRY(launch_check("k_min_alive"));
             GK_HIP(hipMemcpyAsync(h, dmin, 4, hipMemcpyDeviceToHost, ctx->stream));
             GK_HIP(hipStreamSynchronize(ctx->stream));
-            uint32_t minsup = ~h[0];
-            k = std::max<int64_t>(k, (int64_t)minsup + 3);
-            thresh = (uint32_t)(k - 2);
+            const uint32_t mn = ~(uint32_t)h[0];
+            k = (int)std::max<int64_t>(k + 1, (int64_t)mn + 3);
+            continue;
         }
-        const int g = waves_grid(ctx, cur.m);
-        k_keep_count<<<g, 256, 0, ctx->stream>>>(cur.m, cur.rst, sup, thresh, cnt);
-        GK_TRY(launch_check("k_keep_count"));
-        if (!nxt.rst) GK_TRY(B.alloc(&nxt.rst, cur.m + 1));
-        GK_TRY(exclusive_scan_i32(ctx, cnt, nxt.rst, cur.m + 1));
-        int32_t nz;
-        GK_HIP(hipMemcpyAsync(&h[1], nxt.rst + cur.m, 4, hipMemcpyDeviceToHost, ctx->stream));
-        GK_HIP(hipStreamSynchronize(ctx->stream));
-        nz = (int32_t)h[1];
-        if (nz == cur.nz) {
-            // nothing removed: only possible for a fixed k (the
-            // decomposition threshold always removes the minimum).
-            break;
+        ++rounds;
+        kbest = k - 1;
+        // frontier matrix
+        GK_HIP(hipMemsetAsync(fcnt, 0, 4 * (m + 1), ctx->stream));
+        k_fcount<<<grid_items(ctx, nF), 256, 0, ctx->stream>>>(nF, F, eu, ev, arst[cur], fy, fcnt);
+        GK_TRY(exclusive_scan_i32(ctx, fcnt, frst, m + 1));
+        GK_HIP(hipMemcpyAsync(fcur, frst, 4 * (m + 1), hipMemcpyDeviceToDevice, ctx->stream));
+        k_fscatter<<<grid_items(ctx, nF), 256, 0, ctx->stream>>>(nF, F, eu, ev, fy, fcur, fcol, fval);
+        GK_TRY(launch_check("frontier matrix"));
+        // support update
+        gk_dcsr Fm = view(ctx, m, m, nF, frst, fcol, fval);
+        gk_dcsr Aal = view(ctx, m, m, aal_nz, arst[cur], acol[cur], aeid[cur]);
+        PeelArgs pa{aeid[cur], st, sup};
+        ProductInfo pi;
+        GK_TRY(masked_product(ctx, &Fm, &Aal, &Aal, MODE_PEEL, nullptr, nullptr, nullptr, o, &pi, &pa));
+        record_timing(ctx, pi, false);
+        k_mark_dead<<<grid_items(ctx, nF), 256, 0, ctx->stream>>>(nF, F, st);
+        GK_TRY(launch_check("k_mark_dead"));
+        alive -= nF;
+        dead_pending += nF;
+        // compact once a quarter of the adjacency entries are dead
+        if (alive > 0 && 2 * dead_pending * 4 > aal_nz) {
+            const int nx = cur ^ 1;
+            k_alive_count<<<gw, 256, 0, ctx->stream>>>(m, arst[cur], aeid[cur], st, cnt);
+            GK_TRY(exclusive_scan_i32(ctx, cnt, arst[nx], m + 1));
+            k_alive_scatter<<<gw, 256, 0, ctx->stream>>>(m, arst[cur], acol[cur], aeid[cur], st, arst[nx], acol[nx],
+                                                         aeid[nx]);
+            GK_HIP(hipMemsetAsync(dn + 1, 0, 4, ctx->stream));
+            k_alist_compact<<<grid_items(ctx, na), 256, 0, ctx->stream>>>(na, alist, st, alist2, dn + 1);
+            GK_TRY(launch_check("compaction"));
+            GK_HIP(hipMemcpyAsync(h, arst[nx] + m, 4, hipMemcpyDeviceToHost, ctx->stream));
+            GK_HIP(hipMemcpyAsync(h + 1, dn + 1, 4, hipMemcpyDeviceToHost, ctx->stream));
+            GK_HIP(hipStreamSynchronize(ctx->stream));
+            aal_nz = h[0];
+            na = h[1];
+            std::swap(alist, alist2);
+            cur = nx;
+            dead_pending = 0;
         }
-        if (!nxt.col) {
-            GK_TRY(B.alloc(&nxt.col, cur.nz));
-            GK_TRY(B.alloc(&nxt.id, cur.nz));
-        }
-        k_keep_scatter<<<g, 256, 0, ctx->stream>>>(cur.m, cur.rst, cur.col, cur.id, sup, thresh, nxt.rst, nxt.col,
-                                                   nxt.id, truss_l, (uint32_t)(k - 1));
-        GK_TRY(launch_check("k_keep_scatter"));
-        if (fixed_k == 0) kbest = k - 1;
-        nxt.m = cur.m;
-        nxt.nz = nz;
-        std::swap(cur, nxt);
     }
-    if (rounds) *rounds = r;
-    if (kmax) *kmax = kbest;
-    if (support_ms) *support_ms = sms;
+    if (rounds_out) *rounds_out = fixed_k > 0 ? rounds + 1 : rounds;
+    if (kmax_out) *kmax_out = kbest;
     return GK_OK;
 }
 
@@ -270,7 +455,7 @@ extern "C" int gk_truss_decompose_dev(gk_ctx *ctx, const gk_dcsr *A, uint32_t *d
     Timer total(ctx);
     total.start();
     TrussBufs B(ctx);
-    Lgraph cur, nxt;
+    Lgraph cur;
     GK_TRY(build_lower(ctx, A, B, &cur));
     const int64_t nl = cur.nz;
     int32_t *lrst0 = cur.rst;
@@ -283,8 +468,7 @@ extern "C" int gk_truss_decompose_dev(gk_ctx *ctx, const gk_dcsr *A, uint32_t *d
     GK_TRY(B.alloc(&tl, nl));
     int64_t rounds = 0;
     int32_t kb = 0;
-    double sms = 0;
-    GK_TRY(truss_rounds(ctx, B, cur, nxt, 0, tl, o, &rounds, &kb, &sms));
+    GK_TRY(truss_peel(ctx, A, B, cur, tl, o, &rounds, &kb));
     if (A->nz) {
         k_truss_to_sym<<<waves_grid(ctx, A->m), 256, 0, ctx->stream>>>(A->m, A->rst, A->col, lrst_keep, tl, d_truss);
         GK_TRY(launch_check("k_truss_to_sym"));
@@ -375,7 +559,7 @@ extern "C" int gk_ktruss(gk_ctx *ctx, const gk_csr *A, int k, int32_t *rst_out, 
     int rc = GK_OK;
     {
         TrussBufs B(ctx);
-        Lgraph cur, nxt;
+        Lgraph cur;
         int64_t rounds = 0;
         uint32_t *tl = nullptr;
         int32_t *lrst_keep = nullptr, *cnt = nullptr, *nrst = nullptr, *ncol = nullptr;
@@ -392,7 +576,7 @@ extern "C" int gk_ktruss(gk_ctx *ctx, const gk_csr *A, int k, int32_t *rst_out, 
             ctx->last = gk_timing{};
             t.start();
             // Removed edges are labelled 1; alive ones stay 0.
-            if ((rc = truss_rounds(ctx, B, cur, nxt, k, tl, o, &rounds, nullptr, nullptr))) break;
+            if ((rc = truss_peel(ctx, &d, B, cur, tl, o, &rounds, nullptr, k))) break;
             t.stop();
             ctx->last.total_ms = t.ms();
             ctx->last.iterations = rounds;

@@ -7,7 +7,7 @@
 namespace gk {
 
 // Product modes.
-enum { MODE_TC = 0, MODE_VALUES = 1, MODE_SUPPORT = 2 };
+enum { MODE_TC = 0, MODE_VALUES = 1, MODE_SUPPORT = 2, MODE_PEEL = 3 };
 
 // Capacities (entries).  A task covers at most kMaxTaskA A-entries so its
 // B-row prefix fits LDS; MSA-LDS holds mask rows up to kLdsMaxMask in a
@@ -17,9 +17,17 @@ constexpr int kLdsHashCap = 4096;
 constexpr int kLdsMaxMask = 1536;
 constexpr int kMcaMaxMask = 64;
 constexpr int kHashGlobalMaxMask = 1 << 20;
+// MSA-LDS runs as two kernels: mask rows up to kLdsSmallMask in a table of
+// kLdsSmallCap slots (small LDS footprint, high occupancy) and longer rows in
+// the kLdsHashCap table.  The long-row kernel is an internal class of its own.
+constexpr int kLdsSmallMask = 128;
+constexpr int kLdsSmallCap = 512;
+constexpr int kNumClass = GK_NUM_KERNELS;
+constexpr int kClassMsaLarge = GK_NUM_ACC;
 constexpr int kFlopBins = 32;
-constexpr int kNumBins = GK_NUM_ACC * kFlopBins;
+constexpr int kNumBins = kNumClass * kFlopBins;
 // Persistent-grid sizes (workgroups of one wave per CU).
+constexpr int kLdsSmallBlocksPerCu = 24;
 constexpr int kLdsBlocksPerCu = 8;
 constexpr int kMcaBlocksPerCu = 8;
 constexpr int kGlobalBlocksPerCu = 4;
@@ -30,7 +38,7 @@ struct RowStats {
     int32_t *alen;   // A row length
     int32_t *bmax;   // max |B_k|
     int32_t *ntask;  // tasks of the row
-    int8_t *acc;     // chosen accumulator (-1: no work)
+    int8_t *acc;     // accumulator wanted (learned selector) / kernel class chosen (-1: no work)
 };
 
 struct SelectParams {
@@ -45,12 +53,12 @@ struct Task {
 
 struct Plan {
     int32_t ntasks = 0;
-    int32_t acc_tasks[GK_NUM_ACC] = {};
-    int32_t acc_maxmask[GK_NUM_ACC] = {};
+    int32_t acc_tasks[kNumClass] = {};      // per kernel class
+    int32_t acc_maxmask[kNumClass] = {};
     Task *tasks = nullptr;
     int32_t *bin_off = nullptr;             // device, kNumBins + 1
-    unsigned long long *stats = nullptr;    // device, GK_NUM_ACC * ST_NUM
-    unsigned long long *hits = nullptr;     // device, GK_NUM_ACC
+    unsigned long long *stats = nullptr;    // device, kNumClass * ST_NUM
+    unsigned long long *hits = nullptr;     // device, kNumClass
 };
 
 // Per-accumulator work counters gathered while binning (device, u64):
@@ -61,9 +69,10 @@ enum { ST_TASKS = 0, ST_MASK, ST_AROW, ST_FLOP, ST_NUM };
 struct ProductInfo {
     double total_ms = 0, select_ms = 0, kernel_ms = 0;
     int32_t ntasks = 0;
-    int32_t acc_tasks[GK_NUM_ACC] = {};
-    double acc_ms[GK_NUM_ACC] = {};
-    double acc_bytes[GK_NUM_ACC] = {};
+    int32_t acc_tasks[GK_NUM_ACC] = {};   // per accumulator
+    double acc_ms[kNumClass] = {};         // per kernel class
+    double acc_bytes[kNumClass] = {};
+    int32_t class_tasks[kNumClass] = {};
     double flop = 0, hits = 0;
 };
 
@@ -76,12 +85,20 @@ int plan_tasks(gk_ctx *ctx, const gk_dcsr *A, RowStats &st, const SelectParams &
 SelectParams select_params(const gk_opts &o);
 int zero_u32(gk_ctx *ctx, void *p, int64_t n);
 
+// PEEL mode: edge ids of M's entries (M.val carries them too), states and
+// supports indexed by edge id.
+struct PeelArgs {
+    const int32_t *mval;
+    const uint8_t *state;
+    int32_t *sup;
+};
+
 // C<M> = A*B in the given mode.  d_out: VALUES -> int32 C values (M.nz),
 // SUPPORT -> uint32 supports (M.nz), TC -> unused.  d_count / d_rowcount:
 // TC totals (zeroed by the caller).
 int masked_product(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcsr *M, int mode, int32_t *d_out,
                    unsigned long long *d_count, unsigned long long *d_rowcount, const gk_opts &o,
-                   ProductInfo *info);
+                   ProductInfo *info, const PeelArgs *peel = nullptr);
 
 // Learned selector (selector.hip): writes st.acc for every row.
 int model_select(gk_ctx *ctx, RowStats &st, int64_t m, int64_t ncols);

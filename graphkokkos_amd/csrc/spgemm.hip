@@ -99,9 +99,9 @@ __global__ __launch_bounds__(256) void k_select_bin(int64_t m, RowStats st, Sele
                                                     int32_t *__restrict__ acc_maxmask,
                                                     unsigned long long *__restrict__ stats) {
     __shared__ int32_t h[kNumBins];
-    __shared__ unsigned long long ss[GK_NUM_ACC * ST_NUM];
+    __shared__ unsigned long long ss[kNumClass * ST_NUM];
     for (int b = threadIdx.x; b < kNumBins; b += blockDim.x) h[b] = 0;
-    for (int b = threadIdx.x; b < GK_NUM_ACC * ST_NUM; b += blockDim.x) ss[b] = 0;
+    for (int b = threadIdx.x; b < kNumClass * ST_NUM; b += blockDim.x) ss[b] = 0;
     __syncthreads();
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
         int32_t mlen = st.mlen[i];
@@ -116,8 +116,9 @@ __global__ __launch_bounds__(256) void k_select_bin(int64_t m, RowStats st, Sele
             if (acc == GK_ACC_MCA_WAVE && !mca_fits(mlen)) acc = GK_ACC_MSA_LDS;
             if (acc == GK_ACC_MSA_LDS && !lds_fits(mlen)) acc = GK_ACC_HASH_GLOBAL;
             if (acc == GK_ACC_HASH_GLOBAL && mlen > kHashGlobalMaxMask) acc = GK_ACC_DENSE_GLOBAL;
+            if (acc == GK_ACC_MSA_LDS && mlen > kLdsSmallMask) acc = kClassMsaLarge;
         }
-        st.acc[i] = (int8_t)acc;
+        st.acc[i] = (int8_t)acc;  // kernel class from here on
         if (acc >= 0) {
             int32_t nt = task_count(sp, st.alen[i], flop);
             st.ntask[i] = nt;
@@ -134,7 +135,7 @@ __global__ __launch_bounds__(256) void k_select_bin(int64_t m, RowStats st, Sele
     __syncthreads();
     for (int b = threadIdx.x; b < kNumBins; b += blockDim.x)
         if (h[b]) atomicAdd(&bin_count[b], h[b]);
-    for (int b = threadIdx.x; b < GK_NUM_ACC * ST_NUM; b += blockDim.x)
+    for (int b = threadIdx.x; b < kNumClass * ST_NUM; b += blockDim.x)
         if (ss[b]) atomicAdd(&stats[b], ss[b]);
 }
 
@@ -153,23 +154,44 @@ __global__ void k_bin_scan(const int32_t *__restrict__ bin_count, int32_t *__res
     }
 }
 
+// Tasks of every row written into their bins.  A block first counts its
+// rows' tasks per bin in LDS, reserves each bin's span with one global atomic,
+// then writes -- so the ~100 bin cursors see one atomic per block, not per row.
 __global__ __launch_bounds__(256) void k_scatter_tasks(int64_t m, RowStats st, const int32_t *__restrict__ arst,
                                                        int32_t *__restrict__ bin_cur, Task *__restrict__ tasks) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
-        int acc = st.acc[i];
-        if (acc < 0) continue;
-        int32_t nt = st.ntask[i];
-        int b = acc * kFlopBins + (kFlopBins - 1 - flop_bin(st.flop[i]));
-        int32_t pos = atomicAdd(&bin_cur[b], nt);
-        int32_t a0 = arst[i], alen = arst[i + 1] - a0;
-        for (int32_t t = 0; t < nt; ++t) {
-            Task tk;
-            tk.row = (int32_t)i;
-            tk.a0 = a0 + (int32_t)((int64_t)alen * t / nt);
-            tk.a1 = a0 + (int32_t)((int64_t)alen * (t + 1) / nt);
-            tk.split = nt > 1;
-            tasks[pos + t] = tk;
+    __shared__ int32_t lcnt[kNumBins];
+    __shared__ int32_t lbase[kNumBins];
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < m; base += (int64_t)gridDim.x * blockDim.x) {
+        for (int b = threadIdx.x; b < kNumBins; b += blockDim.x) lcnt[b] = 0;
+        __syncthreads();
+        const int64_t i = base + threadIdx.x;
+        int cls = -1, bin = 0;
+        int32_t nt = 0, off = 0;
+        if (i < m) {
+            cls = st.acc[i];
+            if (cls >= 0) {
+                nt = st.ntask[i];
+                bin = cls * kFlopBins + (kFlopBins - 1 - flop_bin(st.flop[i]));
+                off = atomicAdd(&lcnt[bin], nt);
+            }
         }
+        __syncthreads();
+        for (int b = threadIdx.x; b < kNumBins; b += blockDim.x)
+            if (lcnt[b]) lbase[b] = atomicAdd(&bin_cur[b], lcnt[b]);
+        __syncthreads();
+        if (cls >= 0) {
+            const int32_t pos = lbase[bin] + off;
+            const int32_t a0 = arst[i], alen = arst[i + 1] - a0;
+            for (int32_t t = 0; t < nt; ++t) {
+                Task tk;
+                tk.row = (int32_t)i;
+                tk.a0 = a0 + (int32_t)((int64_t)alen * t / nt);
+                tk.a1 = a0 + (int32_t)((int64_t)alen * (t + 1) / nt);
+                tk.split = nt > 1;
+                tasks[pos + t] = tk;
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -184,6 +206,9 @@ struct Prob {
     unsigned long long *__restrict__ count;     // TC: total
     unsigned long long *__restrict__ rowcount;  // TC: optional per-row counts
     unsigned long long *__restrict__ hits;      // hits per accumulator
+    const int32_t *__restrict__ mval;           // PEEL: edge id per mask entry
+    const uint8_t *__restrict__ estate;         // PEEL: edge states
+    int32_t *__restrict__ esup;                 // PEEL: edge supports
     const Task *__restrict__ tasks;
     const int32_t *__restrict__ bin_off;
 };
@@ -220,28 +245,28 @@ __device__ __forceinline__ int32_t build_prefix(const Prob &P, int32_t a0, int32
     return total;
 }
 
-// Streamed-entry cursor: advance kk until pre[kk] <= g < pre[kk+1].
-__device__ __forceinline__ int32_t advance(const int32_t *pre, int32_t kk, int32_t g) {
-    while (pre[kk + 1] <= g) ++kk;
-    return kk;
-}
-
 // Hit handling common to every accumulator.  q = mask position (0-based in
 // the row), kk = A entry offset in the task, t = index of the streamed entry
 // in B.col.
 template <int MODE>
 struct Hit;
+// Modes with per-mask-position accumulators.
+template <int MODE>
+struct HasAcc {
+    static constexpr bool v = MODE == MODE_VALUES || MODE == MODE_SUPPORT;
+};
 
 template <>
 struct Hit<MODE_TC> {
-    __device__ static void on(const Prob &, int32_t *, int32_t *, int32_t, int32_t, int32_t, int32_t, uint32_t &cnt) {
+    __device__ static void on(const Prob &, int32_t *, int32_t *, int32_t, int32_t, int32_t, int32_t, int32_t,
+                              uint32_t &cnt) {
         ++cnt;
     }
 };
 template <>
 struct Hit<MODE_VALUES> {
-    __device__ static void on(const Prob &P, int32_t *acc, int32_t *, int32_t q, int32_t a0, int32_t kk, int32_t t,
-                              uint32_t &cnt) {
+    __device__ static void on(const Prob &P, int32_t *acc, int32_t *, int32_t q, int32_t, int32_t a0, int32_t kk,
+                              int32_t t, uint32_t &cnt) {
         ++cnt;
         uint32_t av = P.aval ? (uint32_t)P.aval[a0 + kk] : 1u;
         uint32_t bv = P.bval ? (uint32_t)P.bval[t] : 1u;
@@ -250,12 +275,33 @@ struct Hit<MODE_VALUES> {
 };
 template <>
 struct Hit<MODE_SUPPORT> {
-    __device__ static void on(const Prob &P, int32_t *acc, int32_t *acnt, int32_t q, int32_t, int32_t kk, int32_t t,
-                              uint32_t &cnt) {
+    __device__ static void on(const Prob &P, int32_t *acc, int32_t *acnt, int32_t q, int32_t, int32_t, int32_t kk,
+                              int32_t t, uint32_t &cnt) {
         ++cnt;
         atomicAdd((uint32_t *)&acc[q], 1u);          // edge (i, j)
         atomicAdd((uint32_t *)&acnt[kk], 1u);        // edge (i, k)
         atomicAdd((uint32_t *)&P.cval[t], 1u);       // edge (k, j)
+    }
+};
+// Support update of one truss-peeling round: A holds the round's frontier
+// edges (row y, column x, value = edge id e), M and B the current adjacency
+// with edge ids as values.  A hit w closes the triangle (y, x, w); it is
+// destroyed this round, and its owner -- the frontier edge with the smallest
+// id among its frontier edges -- takes one support off each of its edges that
+// stays alive.  Edge states: 0 alive, 1 frontier, 2 removed earlier.
+template <>
+struct Hit<MODE_PEEL> {
+    __device__ static void on(const Prob &P, int32_t *, int32_t *, int32_t q, int32_t mb, int32_t a0, int32_t kk,
+                              int32_t t, uint32_t &cnt) {
+        const int32_t e = P.aval[a0 + kk];
+        const int32_t e1 = P.mval[mb + q];
+        const int32_t e2 = P.bval[t];
+        const uint8_t s1 = P.estate[e1], s2 = P.estate[e2];
+        if (s1 == 2 || s2 == 2) return;
+        if ((s1 == 1 && e1 < e) || (s2 == 1 && e2 < e)) return;
+        ++cnt;
+        if (s1 == 0) atomicSub(&P.esup[e1], 1);
+        if (s2 == 0) atomicSub(&P.esup[e2], 1);
     }
 };
 
@@ -282,29 +328,83 @@ __device__ __forceinline__ void tc_finish(const Prob &P, const Task &tk, uint32_
     if (MODE == MODE_TC && P.rowcount && cnt_task) atomicAdd(&P.rowcount[tk.row], (unsigned long long)cnt_task);
 }
 
-// ---- MSA-LDS --------------------------------------------------------------
-// One wave per task; LDS: key[cap] (mask column or -1), pos[cap] (mask
-// position), acc[mlen], pre/bst[kMaxTaskA+1].
+// Stream the concatenated B rows of a task with a tile of G lanes, U
+// entries per lane per step: the cursor (A entry kk, its B-row base) lives
+// in registers and is advanced through the LDS prefix only when an entry
+// crosses into the next B row; the U loads of B columns are issued before
+// any of them is probed.  probe(b) returns the mask position of column b or
+// -1.  Returns the number of hits.
+template <int MODE, int G, int U, class Probe>
+__device__ __forceinline__ uint32_t stream_entries(const Prob &P, const Task &tk, int32_t mb, const int32_t *pre,
+                                                   const int32_t *bst, int32_t total, int gl, int32_t *acc,
+                                                   int32_t *acnt, Probe probe) {
+    uint32_t cnt = 0;
+    if (total <= 0) return 0;
+    int32_t kk = 0, nxt = pre[1], base = bst[0];
+    for (int32_t g0 = gl; g0 < total; g0 += G * U) {
+        int32_t t[U], k[U], b[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int32_t g = g0 + G * u;
+            t[u] = -1;
+            k[u] = 0;
+            if (g < total) {
+                while (g >= nxt) {
+                    ++kk;
+                    nxt = pre[kk + 1];
+                    base = bst[kk];
+                }
+                t[u] = base + g;
+                k[u] = kk;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) b[u] = t[u] >= 0 ? P.bcol[t[u]] : -1;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (t[u] >= 0) {
+                const int32_t q = probe(b[u]);
+                if (q >= 0) Hit<MODE>::on(P, acc, acnt, q, mb, tk.a0, k[u], t[u], cnt);
+            }
+        }
+    }
+    return cnt;
+}
+
 template <int MODE>
-__global__ __launch_bounds__(64) void k_msa_lds(Prob P, int acc_id) {
-    constexpr int HC = kLdsHashCap;
+__device__ __forceinline__ void finish_kernel(const Prob &P, uint32_t cnt, int cls, int lane) {
+    unsigned long long s = dev::wave_sum((unsigned long long)cnt);
+    if (lane == 0 && s) {
+        if (MODE == MODE_TC) atomicAdd(P.count, s);
+        atomicAdd(&P.hits[cls], s);
+    }
+}
+
+// ---- MSA-LDS --------------------------------------------------------------
+// One wave per task; LDS: key[HC] (mask column or -1), pos[HC] (mask
+// position), acc[MAXM] (per-position counters), pre/bst[kMaxTaskA+1].  Two
+// instantiations: a small table for short mask rows (high occupancy) and a
+// large one.  A task's table has the smallest power-of-two capacity >= 4 x
+// its mask length (at most HC), so probes rarely walk past one slot.
+template <int MODE, int HC, int MAXM>
+__global__ __launch_bounds__(64) void k_msa_lds(Prob P, int cls) {
     __shared__ int32_t hkey[HC];
     __shared__ int32_t hpos[MODE == MODE_TC ? 1 : HC];
-    __shared__ int32_t acc[MODE == MODE_TC ? 1 : kLdsMaxMask];
+    __shared__ int32_t acc[HasAcc<MODE>::v ? MAXM : 1];
     __shared__ int32_t acnt[MODE == MODE_SUPPORT ? kMaxTaskA : 1];
     __shared__ int32_t pre[kMaxTaskA + 1];
     __shared__ int32_t bst[kMaxTaskA];
     const int lane = threadIdx.x;
-    const int32_t lo = P.bin_off[acc_id * kFlopBins], hi = P.bin_off[(acc_id + 1) * kFlopBins];
+    const int32_t lo = P.bin_off[cls * kFlopBins], hi = P.bin_off[(cls + 1) * kFlopBins];
     uint32_t cnt = 0;
     for (int32_t ti = lo + blockIdx.x; ti < hi; ti += gridDim.x) {
         const Task tk = P.tasks[ti];
         const int32_t mb = P.mrst[tk.row], mlen = P.mrst[tk.row + 1] - mb;
         int lg = 6;
         while ((1 << lg) < 4 * mlen && (1 << lg) < HC) ++lg;
-        const uint32_t cap = 1u << lg, cmask = cap - 1;
-        for (uint32_t s = lane; s < cap; s += 64) hkey[s] = -1;
-        if (MODE != MODE_TC)
+        const uint32_t cmask = (1u << lg) - 1;
+        for (uint32_t s = lane; s <= cmask; s += 64) hkey[s] = -1;
+        if (HasAcc<MODE>::v)
             for (int32_t q = lane; q < mlen; q += 64) acc[q] = 0;
         if (MODE == MODE_SUPPORT)
             for (int32_t x = lane; x < tk.a1 - tk.a0; x += 64) acnt[x] = 0;
@@ -317,55 +417,44 @@ __global__ __launch_bounds__(64) void k_msa_lds(Prob P, int acc_id) {
         }
         const int32_t total = build_prefix<64>(P, tk.a0, tk.a1, pre, bst, lane);
         __syncthreads();
-        uint32_t cnt_task = 0;
-        int32_t kk = 0;
-        for (int32_t g = lane; g < total; g += 64) {
-            kk = advance(pre, kk, g);
-            const int32_t t = bst[kk] + g;
-            const int32_t b = P.bcol[t];
+        const uint32_t c = stream_entries<MODE, 64, 4>(P, tk, mb, pre, bst, total, lane, acc, acnt,
+                                                       [&](int32_t b) -> int32_t {
             uint32_t s = dev::hash32((uint32_t)b) >> (32 - lg);
             for (;;) {
-                int32_t key = hkey[s];
-                if (key == b) {
-                    Hit<MODE>::on(P, acc, acnt, MODE == MODE_TC ? 0 : hpos[s], tk.a0, kk, t, cnt_task);
-                    break;
-                }
-                if (key < 0) break;
+                const int32_t key = hkey[s];
+                if (key == b) return MODE == MODE_TC ? 0 : hpos[s];
+                if (key < 0) return -1;
                 s = (s + 1) & cmask;
             }
-        }
+        });
         __syncthreads();
         flush<MODE, 64>(P, tk, mb, mlen, acc, acnt, lane);
         if (MODE == MODE_TC && P.rowcount) {
-            uint32_t r = dev::wave_sum(cnt_task);
+            uint32_t r = dev::wave_sum(c);
             if (lane == 0 && r) atomicAdd(&P.rowcount[tk.row], (unsigned long long)r);
         }
-        cnt += cnt_task;
+        cnt += c;
         __syncthreads();
     }
-    {
-        unsigned long long s = dev::wave_sum((unsigned long long)cnt);
-        if (lane == 0 && s) {
-            if (MODE == MODE_TC) atomicAdd(P.count, s);
-            atomicAdd(&P.hits[acc_id], s);
-        }
-    }
+    finish_kernel<MODE>(P, cnt, cls, lane);
 }
 
 // ---- MCA-wave -------------------------------------------------------------
 // Tiles of G lanes, 64/G tasks per wave.  The sorted mask row sits in LDS;
-// every lane binary-searches its streamed B entry in it.  In TC mode the
-// tile's hits are counted with one ballot + popcount per step.
+// every lane binary-searches its streamed B entry in it (after a range
+// check against the row's first and last column).  In TC mode the tile's
+// hits are counted from the wave's ballot of found entries with a popcount
+// of the tile's bits.
 template <int MODE, int G>
-__global__ __launch_bounds__(64) void k_mca_wave(Prob P, int acc_id) {
+__global__ __launch_bounds__(64) void k_mca_wave(Prob P, int cls) {
     constexpr int T = 64 / G;
     __shared__ int32_t mk[T][kMcaMaxMask];
-    __shared__ int32_t acc[T][MODE == MODE_TC ? 1 : kMcaMaxMask];
+    __shared__ int32_t acc[T][HasAcc<MODE>::v ? kMcaMaxMask : 1];
     __shared__ int32_t acnt[T][MODE == MODE_SUPPORT ? kMaxTaskA : 1];
     __shared__ int32_t pre[T][kMaxTaskA + 1];
     __shared__ int32_t bst[T][kMaxTaskA];
     const int lane = threadIdx.x, tile = lane / G, gl = lane % G;
-    const int32_t lo = P.bin_off[acc_id * kFlopBins], hi = P.bin_off[(acc_id + 1) * kFlopBins];
+    const int32_t lo = P.bin_off[cls * kFlopBins], hi = P.bin_off[(cls + 1) * kFlopBins];
     uint32_t cnt = 0;
     // All tiles of a wave run the same number of rounds so that the wave
     // stays converged at the barriers.
@@ -383,7 +472,7 @@ __global__ __launch_bounds__(64) void k_mca_wave(Prob P, int acc_id) {
         }
         for (int32_t q = gl; q < mlen; q += G) {
             mk[tile][q] = P.mcol[mb + q];
-            if (MODE != MODE_TC) acc[tile][q] = 0;
+            if (HasAcc<MODE>::v) acc[tile][q] = 0;
         }
         if (MODE == MODE_SUPPORT)
             for (int32_t x = gl; x < tk.a1 - tk.a0; x += G) acnt[tile][x] = 0;
@@ -391,26 +480,32 @@ __global__ __launch_bounds__(64) void k_mca_wave(Prob P, int acc_id) {
         if (live) total = build_prefix<G>(P, tk.a0, tk.a1, pre[tile], bst[tile], gl);
         __syncthreads();
         uint32_t cnt_task = 0;
-        if (live && mlen > 0) {
-            const int32_t mlo = mk[tile][0], mhi = mk[tile][mlen - 1];
-            int32_t kk = 0;
+        if (live && mlen > 0 && total > 0) {
+            const int32_t *m = mk[tile];
+            const int32_t mlo = m[0], mhi = m[mlen - 1];
+            int32_t kk = 0, nxt = pre[tile][1], base = bst[tile][0];
             for (int32_t g = gl; g < total; g += G) {
-                kk = advance(pre[tile], kk, g);
-                const int32_t t = bst[tile][kk] + g;
+                while (g >= nxt) {
+                    ++kk;
+                    nxt = pre[tile][kk + 1];
+                    base = bst[tile][kk];
+                }
+                const int32_t t = base + g;
                 const int32_t b = P.bcol[t];
                 int32_t q = -1;
                 if (b >= mlo && b <= mhi) {
-                    int32_t l = 0, h = mlen;  // lower_bound in mk[tile][0, mlen)
+                    int32_t l = 0, h = mlen;  // lower_bound in m[0, mlen)
                     while (l < h) {
                         int32_t mid = (l + h) >> 1;
-                        if (mk[tile][mid] < b) l = mid + 1; else h = mid;
+                        if (m[mid] < b) l = mid + 1; else h = mid;
                     }
-                    if (l < mlen && mk[tile][l] == b) q = l;
+                    if (m[l] == b) q = l;
                 }
                 if (MODE == MODE_TC) {
-                    cnt_task += q >= 0;
+                    const uint64_t found = __ballot(q >= 0);
+                    if (gl == 0) cnt_task += __popcll((found >> (tile * G)) & ((G == 64) ? ~0ull : ((1ull << G) - 1)));
                 } else if (q >= 0) {
-                    Hit<MODE>::on(P, acc[tile], acnt[tile], q, tk.a0, kk, t, cnt_task);
+                    Hit<MODE>::on(P, acc[tile], acnt[tile], q, mb, tk.a0, kk, t, cnt_task);
                 }
             }
         }
@@ -427,20 +522,14 @@ __global__ __launch_bounds__(64) void k_mca_wave(Prob P, int acc_id) {
         }
         __syncthreads();
     }
-    {
-        unsigned long long s = dev::wave_sum((unsigned long long)cnt);
-        if (lane == 0 && s) {
-            if (MODE == MODE_TC) atomicAdd(P.count, s);
-            atomicAdd(&P.hits[acc_id], s);
-        }
-    }
+    finish_kernel<MODE>(P, cnt, cls, lane);
 }
 
 // ---- hash-global ----------------------------------------------------------
 // One wave per task, its table (key, position) in HBM scratch of `cap`
 // slots per wave; mask-position accumulators in HBM too.
 template <int MODE>
-__global__ __launch_bounds__(64) void k_hash_global(Prob P, int acc_id, int32_t *__restrict__ gkey,
+__global__ __launch_bounds__(64) void k_hash_global(Prob P, int cls, int32_t *__restrict__ gkey,
                                                     int32_t *__restrict__ gpos, int32_t *__restrict__ gacc,
                                                     int lgcap) {
     __shared__ int32_t pre[kMaxTaskA + 1];
@@ -451,16 +540,16 @@ __global__ __launch_bounds__(64) void k_hash_global(Prob P, int acc_id, int32_t 
     int32_t *key = gkey + (size_t)blockIdx.x * cap;
     int32_t *pos = gpos + (size_t)blockIdx.x * cap;
     int32_t *acc = gacc + (size_t)blockIdx.x * (cap >> 1);
-    const int32_t lo = P.bin_off[acc_id * kFlopBins], hi = P.bin_off[(acc_id + 1) * kFlopBins];
+    const int32_t lo = P.bin_off[cls * kFlopBins], hi = P.bin_off[(cls + 1) * kFlopBins];
     uint32_t cnt = 0;
     for (int32_t ti = lo + blockIdx.x; ti < hi; ti += gridDim.x) {
         const Task tk = P.tasks[ti];
         const int32_t mb = P.mrst[tk.row], mlen = P.mrst[tk.row + 1] - mb;
         int lg = 6;
         while ((1 << lg) < 2 * mlen && lg < lgcap) ++lg;
-        const uint32_t c = 1u << lg, cmask = c - 1;
-        for (uint32_t s = lane; s < c; s += 64) key[s] = -1;
-        if (MODE != MODE_TC)
+        const uint32_t cmask = (1u << lg) - 1;
+        for (uint32_t s = lane; s <= cmask; s += 64) key[s] = -1;
+        if (HasAcc<MODE>::v)
             for (int32_t q = lane; q < mlen; q += 64) acc[q] = 0;
         if (MODE == MODE_SUPPORT)
             for (int32_t x = lane; x < tk.a1 - tk.a0; x += 64) acnt[x] = 0;
@@ -475,41 +564,28 @@ __global__ __launch_bounds__(64) void k_hash_global(Prob P, int acc_id, int32_t 
         const int32_t total = build_prefix<64>(P, tk.a0, tk.a1, pre, bst, lane);
         __threadfence();
         __syncthreads();
-        uint32_t cnt_task = 0;
-        int32_t kk = 0;
-        for (int32_t g = lane; g < total; g += 64) {
-            kk = advance(pre, kk, g);
-            const int32_t t = bst[kk] + g;
-            const int32_t b = P.bcol[t];
+        const uint32_t c = stream_entries<MODE, 64, 4>(P, tk, mb, pre, bst, total, lane, acc, acnt,
+                                                       [&](int32_t b) -> int32_t {
             uint32_t s = dev::hash32((uint32_t)b) >> (32 - lg);
             for (;;) {
-                int32_t k2 = __builtin_nontemporal_load(&key[s]);
-                if (k2 == b) {
-                    Hit<MODE>::on(P, acc, acnt, MODE == MODE_TC ? 0 : pos[s], tk.a0, kk, t, cnt_task);
-                    break;
-                }
-                if (k2 < 0) break;
+                const int32_t k2 = key[s];
+                if (k2 == b) return MODE == MODE_TC ? 0 : pos[s];
+                if (k2 < 0) return -1;
                 s = (s + 1) & cmask;
             }
-        }
+        });
         __threadfence();
         __syncthreads();
         flush<MODE, 64>(P, tk, mb, mlen, acc, acnt, lane);
         if (MODE == MODE_TC && P.rowcount) {
-            uint32_t r = dev::wave_sum(cnt_task);
+            uint32_t r = dev::wave_sum(c);
             if (lane == 0 && r) atomicAdd(&P.rowcount[tk.row], (unsigned long long)r);
         }
-        cnt += cnt_task;
+        cnt += c;
         __threadfence();
         __syncthreads();
     }
-    {
-        unsigned long long s = dev::wave_sum((unsigned long long)cnt);
-        if (lane == 0 && s) {
-            if (MODE == MODE_TC) atomicAdd(P.count, s);
-            atomicAdd(&P.hits[acc_id], s);
-        }
-    }
+    finish_kernel<MODE>(P, cnt, cls, lane);
 }
 
 // ---- dense-global ---------------------------------------------------------
@@ -517,7 +593,7 @@ __global__ __launch_bounds__(64) void k_hash_global(Prob P, int acc_id, int32_t 
 // mask position + 1 at each mask column (0 elsewhere).  Only the mask
 // columns are set and cleared, so a row costs O(mask + streamed entries).
 template <int MODE>
-__global__ __launch_bounds__(64) void k_dense_global(Prob P, int acc_id, int32_t *__restrict__ gmap,
+__global__ __launch_bounds__(64) void k_dense_global(Prob P, int cls, int32_t *__restrict__ gmap,
                                                      int32_t *__restrict__ gacc, int64_t ncols,
                                                      int32_t maxmask) {
     __shared__ int32_t pre[kMaxTaskA + 1];
@@ -526,48 +602,35 @@ __global__ __launch_bounds__(64) void k_dense_global(Prob P, int acc_id, int32_t
     const int lane = threadIdx.x;
     int32_t *map = gmap + (size_t)blockIdx.x * ncols;
     int32_t *acc = gacc + (size_t)blockIdx.x * maxmask;
-    const int32_t lo = P.bin_off[acc_id * kFlopBins], hi = P.bin_off[(acc_id + 1) * kFlopBins];
+    const int32_t lo = P.bin_off[cls * kFlopBins], hi = P.bin_off[(cls + 1) * kFlopBins];
     uint32_t cnt = 0;
     for (int32_t ti = lo + blockIdx.x; ti < hi; ti += gridDim.x) {
         const Task tk = P.tasks[ti];
         const int32_t mb = P.mrst[tk.row], mlen = P.mrst[tk.row + 1] - mb;
         for (int32_t q = lane; q < mlen; q += 64) {
             map[P.mcol[mb + q]] = q + 1;
-            if (MODE != MODE_TC) acc[q] = 0;
+            if (HasAcc<MODE>::v) acc[q] = 0;
         }
         if (MODE == MODE_SUPPORT)
             for (int32_t x = lane; x < tk.a1 - tk.a0; x += 64) acnt[x] = 0;
         const int32_t total = build_prefix<64>(P, tk.a0, tk.a1, pre, bst, lane);
         __threadfence();
         __syncthreads();
-        uint32_t cnt_task = 0;
-        int32_t kk = 0;
-        for (int32_t g = lane; g < total; g += 64) {
-            kk = advance(pre, kk, g);
-            const int32_t t = bst[kk] + g;
-            const int32_t b = P.bcol[t];
-            const int32_t q1 = map[b];
-            if (q1) Hit<MODE>::on(P, acc, acnt, q1 - 1, tk.a0, kk, t, cnt_task);
-        }
+        const uint32_t c = stream_entries<MODE, 64, 4>(P, tk, mb, pre, bst, total, lane, acc, acnt,
+                                                       [&](int32_t b) -> int32_t { return map[b] - 1; });
         __threadfence();
         __syncthreads();
         flush<MODE, 64>(P, tk, mb, mlen, acc, acnt, lane);
         for (int32_t q = lane; q < mlen; q += 64) map[P.mcol[mb + q]] = 0;
         if (MODE == MODE_TC && P.rowcount) {
-            uint32_t r = dev::wave_sum(cnt_task);
+            uint32_t r = dev::wave_sum(c);
             if (lane == 0 && r) atomicAdd(&P.rowcount[tk.row], (unsigned long long)r);
         }
-        cnt += cnt_task;
+        cnt += c;
         __threadfence();
         __syncthreads();
     }
-    {
-        unsigned long long s = dev::wave_sum((unsigned long long)cnt);
-        if (lane == 0 && s) {
-            if (MODE == MODE_TC) atomicAdd(P.count, s);
-            atomicAdd(&P.hits[acc_id], s);
-        }
-    }
+    finish_kernel<MODE>(P, cnt, cls, lane);
 }
 
 __global__ void k_zero_u32(uint32_t *p, int64_t n) {
@@ -615,13 +678,13 @@ int plan_tasks(gk_ctx *ctx, const gk_dcsr *A, RowStats &st, const SelectParams &
     void *p;
     // Layout: u64 stats[GK_NUM_ACC * ST_NUM], u64 hits[GK_NUM_ACC], then the
     // int32 bin counters, offsets, cursors and per-accumulator max mask.
-    const size_t u64s = GK_NUM_ACC * ST_NUM + GK_NUM_ACC;
-    const size_t words = 3 * kNumBins + 1 + GK_NUM_ACC;
+    const size_t u64s = kNumClass * ST_NUM + kNumClass;
+    const size_t words = 3 * kNumBins + 1 + kNumClass;
     const size_t bytes = 8 * u64s + sizeof(int32_t) * words;
     GK_TRY(scratch(ctx, SC_BINS, bytes, &p));
     plan->stats = (unsigned long long *)p;
-    plan->hits = plan->stats + GK_NUM_ACC * ST_NUM;
-    int32_t *bin_count = (int32_t *)(plan->hits + GK_NUM_ACC), *bin_off = bin_count + kNumBins;
+    plan->hits = plan->stats + kNumClass * ST_NUM;
+    int32_t *bin_count = (int32_t *)(plan->hits + kNumClass), *bin_off = bin_count + kNumBins;
     int32_t *bin_cur = bin_off + kNumBins + 1;
     int32_t *acc_maxmask = bin_cur + kNumBins;
     GK_HIP(hipMemsetAsync(p, 0, bytes, ctx->stream));
@@ -635,13 +698,13 @@ int plan_tasks(gk_ctx *ctx, const gk_dcsr *A, RowStats &st, const SelectParams &
     // Task count and per-accumulator mask sizes come back to the host: the
     // hash-global and dense-global kernels size their scratch from them.
     int32_t *h;
-    GK_TRY(pinned(ctx, sizeof(int32_t) * (kNumBins + 1 + GK_NUM_ACC), (void **)&h));
+    GK_TRY(pinned(ctx, sizeof(int32_t) * (kNumBins + 1 + kNumClass), (void **)&h));
     GK_HIP(hipMemcpyAsync(h, bin_off, sizeof(int32_t) * (kNumBins + 1), hipMemcpyDeviceToHost, ctx->stream));
-    GK_HIP(hipMemcpyAsync(h + kNumBins + 1, acc_maxmask, sizeof(int32_t) * GK_NUM_ACC, hipMemcpyDeviceToHost,
+    GK_HIP(hipMemcpyAsync(h + kNumBins + 1, acc_maxmask, sizeof(int32_t) * kNumClass, hipMemcpyDeviceToHost,
                           ctx->stream));
     GK_HIP(hipStreamSynchronize(ctx->stream));
     plan->ntasks = h[kNumBins];
-    for (int a = 0; a < GK_NUM_ACC; ++a) {
+    for (int a = 0; a < kNumClass; ++a) {
         plan->acc_tasks[a] = h[(a + 1) * kFlopBins] - h[a * kFlopBins];
         plan->acc_maxmask[a] = h[kNumBins + 1 + a];
     }
@@ -683,15 +746,23 @@ static int dense_maps(gk_ctx *ctx, int64_t blocks, int64_t ncols, int32_t **out)
 
 template <int MODE>
 static int run_mode(gk_ctx *ctx, const Prob &P, const Plan &plan, int64_t bn, double *acc_ms) {
-    Timer tm[GK_NUM_ACC] = {Timer(ctx), Timer(ctx), Timer(ctx), Timer(ctx)};
-    bool ran[GK_NUM_ACC] = {};
+    Timer tm[kNumClass] = {Timer(ctx), Timer(ctx), Timer(ctx), Timer(ctx), Timer(ctx)};
+    bool ran[kNumClass] = {};
+    if (plan.acc_tasks[kClassMsaLarge]) {
+        int g = grid_for(ctx, plan.acc_tasks[kClassMsaLarge], kLdsBlocksPerCu);
+        tm[kClassMsaLarge].start();
+        k_msa_lds<MODE, kLdsHashCap, kLdsMaxMask><<<g, 64, 0, ctx->stream>>>(P, kClassMsaLarge);
+        tm[kClassMsaLarge].stop();
+        ran[kClassMsaLarge] = true;
+        GK_TRY(launch_check("k_msa_lds<large>"));
+    }
     if (plan.acc_tasks[GK_ACC_MSA_LDS]) {
-        int g = grid_for(ctx, plan.acc_tasks[GK_ACC_MSA_LDS], kLdsBlocksPerCu);
+        int g = grid_for(ctx, plan.acc_tasks[GK_ACC_MSA_LDS], kLdsSmallBlocksPerCu);
         tm[GK_ACC_MSA_LDS].start();
-        k_msa_lds<MODE><<<g, 64, 0, ctx->stream>>>(P, GK_ACC_MSA_LDS);
+        k_msa_lds<MODE, kLdsSmallCap, kLdsSmallMask><<<g, 64, 0, ctx->stream>>>(P, GK_ACC_MSA_LDS);
         tm[GK_ACC_MSA_LDS].stop();
         ran[GK_ACC_MSA_LDS] = true;
-        GK_TRY(launch_check("k_msa_lds"));
+        GK_TRY(launch_check("k_msa_lds<small>"));
     }
     if (plan.acc_tasks[GK_ACC_MCA_WAVE]) {
         int g = grid_for(ctx, (plan.acc_tasks[GK_ACC_MCA_WAVE] + 3) / 4, kMcaBlocksPerCu);
@@ -732,7 +803,7 @@ static int run_mode(gk_ctx *ctx, const Prob &P, const Plan &plan, int64_t bn, do
         ran[GK_ACC_DENSE_GLOBAL] = true;
         GK_TRY(launch_check("k_dense_global"));
     }
-    for (int a = 0; a < GK_NUM_ACC; ++a) acc_ms[a] = ran[a] ? tm[a].ms() : 0.0;
+    for (int a = 0; a < kNumClass; ++a) acc_ms[a] = ran[a] ? tm[a].ms() : 0.0;
     return GK_OK;
 }
 
@@ -748,7 +819,8 @@ SelectParams select_params(const gk_opts &o) {
 }
 
 int masked_product(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcsr *M, int mode, int32_t *d_out,
-                   unsigned long long *d_count, unsigned long long *d_rowcount, const gk_opts &o, ProductInfo *info) {
+                   unsigned long long *d_count, unsigned long long *d_rowcount, const gk_opts &o, ProductInfo *info,
+                   const PeelArgs *peel) {
     if (A->n != B->m || A->m != M->m || B->n != M->n) {
         set_error("shape mismatch: A %lldx%lld, B %lldx%lld, M %lldx%lld", (long long)A->m, (long long)A->n,
                   (long long)B->m, (long long)B->n, (long long)M->m, (long long)M->n);
@@ -770,23 +842,32 @@ int masked_product(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcs
     P.brst = B->rst; P.bcol = B->col; P.bval = B->val;
     P.mrst = M->rst; P.mcol = M->col;
     P.cval = d_out; P.count = d_count; P.rowcount = d_rowcount; P.hits = plan.hits;
+    P.mval = peel ? peel->mval : nullptr;
+    P.estate = peel ? peel->state : nullptr;
+    P.esup = peel ? peel->sup : nullptr;
+    if (mode == MODE_PEEL && !peel) {
+        set_error("peel mode needs edge states and supports");
+        return GK_EINVAL;
+    }
     P.tasks = plan.tasks; P.bin_off = plan.bin_off;
-    double acc_ms[GK_NUM_ACC] = {};
+    double acc_ms[kNumClass] = {};
     if (mode == MODE_TC) GK_TRY(run_mode<MODE_TC>(ctx, P, plan, B->n, acc_ms));
     else if (mode == MODE_VALUES) GK_TRY(run_mode<MODE_VALUES>(ctx, P, plan, B->n, acc_ms));
-    else GK_TRY(run_mode<MODE_SUPPORT>(ctx, P, plan, B->n, acc_ms));
+    else if (mode == MODE_SUPPORT) GK_TRY(run_mode<MODE_SUPPORT>(ctx, P, plan, B->n, acc_ms));
+    else GK_TRY(run_mode<MODE_PEEL>(ctx, P, plan, B->n, acc_ms));
     total.stop();
     if (info) {
-        unsigned long long hs[GK_NUM_ACC * (ST_NUM + 1)];
+        unsigned long long hs[kNumClass * (ST_NUM + 1)];
         GK_HIP(d2h(ctx, hs, plan.stats, sizeof hs));
         const bool same_am = A->col == M->col && A->rst == M->rst;
         *info = ProductInfo{};
         info->total_ms = total.ms();
         info->select_ms = sel.ms();
         info->ntasks = plan.ntasks;
-        for (int a = 0; a < GK_NUM_ACC; ++a) {
-            const unsigned long long *x = hs + a * ST_NUM;
-            const double hits = (double)hs[GK_NUM_ACC * ST_NUM + a];
+        for (int c = 0; c < kNumClass; ++c) {
+            const int a = c == kClassMsaLarge ? (int)GK_ACC_MSA_LDS : c;  // reported accumulator
+            const unsigned long long *x = hs + c * ST_NUM;
+            const double hits = (double)hs[kNumClass * ST_NUM + c];
             // Algorithmic HBM bytes: mask row pointers and columns, A columns
             // unless A is the mask, a B row-pointer pair per A entry, the
             // streamed B columns, and the outputs (C values once; supports:
@@ -795,10 +876,11 @@ int masked_product(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcs
                        8.0 * x[ST_AROW] + 4.0 * x[ST_FLOP];
             if (mode == MODE_VALUES) b += 4.0 * x[ST_MASK];
             if (mode == MODE_SUPPORT) b += 12.0 * hits;
-            info->acc_tasks[a] = plan.acc_tasks[a];
-            info->acc_ms[a] = acc_ms[a];
-            info->acc_bytes[a] = b;
-            info->kernel_ms += acc_ms[a];
+            info->acc_tasks[a] += plan.acc_tasks[c];
+            info->class_tasks[c] = plan.acc_tasks[c];
+            info->acc_ms[c] = acc_ms[c];
+            info->acc_bytes[c] = b;
+            info->kernel_ms += acc_ms[c];
             info->flop += (double)x[ST_FLOP];
             info->hits += hits;
         }
@@ -813,12 +895,12 @@ void record_timing(gk_ctx *ctx, const ProductInfo &pi, bool reset) {
     t.kernel_ms += pi.kernel_ms;
     t.select_ms += pi.select_ms;
     t.launches += 1;
-    for (int a = 0; a < GK_NUM_ACC; ++a) {
-        t.rows_per_acc[a] += pi.acc_tasks[a];
-        t.acc_ms[a] += pi.acc_ms[a];
-        t.acc_bytes[a] += pi.acc_bytes[a];
-        t.acc_launches[a] += pi.acc_tasks[a] ? 1 : 0;
-        t.bytes_alg += pi.acc_bytes[a];
+    for (int a = 0; a < GK_NUM_ACC; ++a) t.rows_per_acc[a] += pi.acc_tasks[a];
+    for (int c = 0; c < kNumClass; ++c) {
+        t.acc_ms[c] += pi.acc_ms[c];
+        t.acc_bytes[c] += pi.acc_bytes[c];
+        t.acc_launches[c] += pi.class_tasks[c] ? 1 : 0;
+        t.bytes_alg += pi.acc_bytes[c];
     }
     t.flop += pi.flop;
     t.hits += pi.hits;

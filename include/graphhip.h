@@ -77,6 +77,9 @@ enum gk_accum {
     GK_ACC_DENSE_GLOBAL= 3,  /* dense column-indexed marks in HBM scratch      */
     GK_NUM_ACC         = 4
 };
+/* Accumulator kernels reported in gk_timing: the four accumulators, and
+ * index 4 = the MSA-LDS kernel for long mask rows (larger LDS table). */
+#define GK_NUM_KERNELS 5
 enum gk_selector {
     GK_SEL_HEURISTIC = 0,   /* fixed thresholds (the fallback)            */
     GK_SEL_LEARNED   = 1,   /* MLP cost model, needs gk_model_load/train  */
@@ -106,10 +109,10 @@ typedef struct gk_timing {
     double  select_ms;                /* features + selection + binning          */
     int64_t launches;                 /* product launches                        */
     int64_t iterations;               /* k-truss rounds / BC levels              */
-    int64_t rows_per_acc[GK_NUM_ACC]; /* tasks per accumulator                   */
-    double  acc_ms[GK_NUM_ACC];       /* device time per accumulator kernel      */
-    double  acc_bytes[GK_NUM_ACC];    /* algorithmic bytes per accumulator kernel*/
-    int64_t acc_launches[GK_NUM_ACC]; /* launches per accumulator kernel         */
+    int64_t rows_per_acc[GK_NUM_ACC];     /* tasks per accumulator               */
+    double  acc_ms[GK_NUM_KERNELS];       /* device time per accumulator kernel  */
+    double  acc_bytes[GK_NUM_KERNELS];    /* algorithmic bytes per kernel        */
+    int64_t acc_launches[GK_NUM_KERNELS]; /* launches per kernel                 */
     double  bytes_alg;                /* algorithmic bytes, all product kernels  */
     double  flop;                     /* streamed B entries (sum_k |B_k|)         */
     double  hits;                     /* mask hits (triangles for TC)            */

@@ -187,7 +187,7 @@ def test_ktruss_single_k(gpu, k):
     assert k_truss(G.A, k, mxmm_msa, True) == res
 
 
-@pytest.mark.parametrize("scale,seed", [(10, 1), (12, 2), (14, 1)])
+@pytest.mark.parametrize("scale,seed", [(10, 1), (12, 2), (14, 1), (16, 1)])
 def test_truss_decomposition_rmat(gpu, scale, seed):
     say("truss", scale)
     G = HostGraph.rmat(scale, 16, seed=seed)
