@@ -16,9 +16,9 @@ GK_EINVAL, GK_ENOMEM, GK_EHIP, GK_EIO, GK_ERANGE, GK_ESTATE = -1, -2, -3, -4, -5
 
 ACC_AUTO, ACC_MSA_LDS, ACC_MCA_WAVE, ACC_HASH_GLOBAL, ACC_DENSE_GLOBAL = -1, 0, 1, 2, 3
 NUM_ACC = 4
-NUM_KERNELS = 5
+NUM_KERNELS = 6
 ACC_NAMES = ["msa-lds", "mca-wave", "hash-global", "dense-global"]
-KERNEL_NAMES = ["msa-lds", "mca-wave", "hash-global", "dense-global", "msa-lds-long"]
+KERNEL_NAMES = ["msa-lds", "mca-wave", "hash-global", "dense-global", "msa-lds-long", "peel-stream"]
 SEL_HEURISTIC, SEL_LEARNED, SEL_FORCED = 0, 1, 2
 
 i32p = C.POINTER(C.c_int32)

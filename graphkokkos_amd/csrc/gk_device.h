@@ -49,13 +49,18 @@ struct gk_ctx {
     int device = 0;
     int num_cus = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev[8] = {};
+    // Auxiliary streams (default priority) that run the accumulator kernels
+    // of one product side by side; forked from and joined into `stream`.
+    static constexpr int kAux = 4;
+    hipStream_t aux[kAux] = {};
+    hipEvent_t fork = nullptr, join[kAux] = {};
     void *scratch[gk::SC_NUM] = {};
     size_t scratch_bytes[gk::SC_NUM] = {};
     void *pinned = nullptr;          // small pinned host staging buffer
     int32_t *dense_map = nullptr;    // dense-global maps, all zero between launches
     int64_t dense_blocks = 0, dense_ncols = 0;
     gk_timing last{};
+    std::vector<hipEvent_t> events;  // reusable timing events
     gk::Model *model = nullptr;
 };
 
@@ -87,8 +92,9 @@ int check_csr(const gk_csr *c, const char *name);
 // Event timer on the context stream.
 struct Timer {
     gk_ctx *ctx;
+    hipStream_t s;
     hipEvent_t a, b;
-    explicit Timer(gk_ctx *c);
+    explicit Timer(gk_ctx *c, hipStream_t stream = nullptr);
     ~Timer();
     void start();
     void stop();

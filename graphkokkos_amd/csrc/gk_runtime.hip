@@ -90,7 +90,7 @@ void free_csr(gk_dcsr *d) {
     d->rst = d->col = d->val = nullptr;
 }
 
-Timer::Timer(gk_ctx *c) : ctx(c), a(nullptr), b(nullptr) {
+Timer::Timer(gk_ctx *c, hipStream_t stream) : ctx(c), s(stream ? stream : c->stream), a(nullptr), b(nullptr) {
     (void)hipEventCreate(&a);
     (void)hipEventCreate(&b);
 }
@@ -98,8 +98,8 @@ Timer::~Timer() {
     if (a) (void)hipEventDestroy(a);
     if (b) (void)hipEventDestroy(b);
 }
-void Timer::start() { (void)hipEventRecord(a, ctx->stream); }
-void Timer::stop() { (void)hipEventRecord(b, ctx->stream); }
+void Timer::start() { (void)hipEventRecord(a, s); }
+void Timer::stop() { (void)hipEventRecord(b, s); }
 double Timer::ms() {
     float t = 0;
     (void)hipEventSynchronize(b);
@@ -149,9 +149,14 @@ extern "C" int gk_init(int device, gk_ctx **out) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) { delete ctx; set_error("hipGetDeviceProperties failed"); return GK_EHIP; }
     ctx->num_cus = prop.multiProcessorCount;
-    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    bool ok = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming) == hipSuccess;
+    for (int i = 0; ok && i < gk_ctx::kAux; ++i)
+        ok = hipStreamCreateWithFlags(&ctx->aux[i], hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&ctx->join[i], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
         delete ctx;
-        set_error("hipStreamCreate failed");
+        set_error("stream or event creation failed");
         return GK_EHIP;
     }
     *out = ctx;
@@ -169,6 +174,13 @@ extern "C" void gk_finalize(gk_ctx *ctx) {
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     if (ctx->dense_map) (void)hipFree(ctx->dense_map);
     if (ctx->model) model_free(ctx->model);
+    for (int i = 0; i < gk_ctx::kAux; ++i) {
+        if (ctx->aux[i]) (void)hipStreamSynchronize(ctx->aux[i]);
+        if (ctx->aux[i]) (void)hipStreamDestroy(ctx->aux[i]);
+        if (ctx->join[i]) (void)hipEventDestroy(ctx->join[i]);
+    }
+    if (ctx->fork) (void)hipEventDestroy(ctx->fork);
+    for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

@@ -39,6 +39,16 @@ int exclusive_scan_i32(gk_ctx *ctx, const int32_t *in, int32_t *out, int64_t n) 
     return GK_OK;
 }
 
+int exclusive_scan_i64(gk_ctx *ctx, const int64_t *in, int64_t *out, int64_t n) {
+    size_t tmp = 0;
+    GK_HIP(rocprim::exclusive_scan(nullptr, tmp, in, out, (int64_t)0, (size_t)n, rocprim::plus<int64_t>(),
+                                   ctx->stream));
+    void *t;
+    GK_TRY(scratch(ctx, SC_CUB, tmp, &t));
+    GK_HIP(rocprim::exclusive_scan(t, tmp, in, out, (int64_t)0, (size_t)n, rocprim::plus<int64_t>(), ctx->stream));
+    return GK_OK;
+}
+
 namespace {
 
 // Length of the strictly-lower prefix of every row of a sorted symmetric
@@ -228,31 +238,157 @@ __global__ void k_min_alive(int64_t na, const int32_t *__restrict__ alist, const
     if ((threadIdx.x & 63) == 0 && mn) atomicMax(out, mn);
 }
 
-// Frontier matrix: every frontier edge goes to the row of its endpoint with
-// the longer current adjacency row (ties: larger vertex id), so each edge is
-// processed once and the shorter row is the one streamed.
-__global__ void k_fcount(int32_t nF, const int32_t *__restrict__ F, const int32_t *__restrict__ eu,
-                         const int32_t *__restrict__ ev, const int32_t *__restrict__ rst,
-                         int32_t *__restrict__ fy, int32_t *__restrict__ fcnt) {
-    for (int32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < nF; f += gridDim.x * blockDim.x) {
-        int32_t e = F[f], u = eu[e], v = ev[e];
-        int32_t du = rst[u + 1] - rst[u], dv = rst[v + 1] - rst[v];
-        int32_t y = (du > dv || (du == dv && u > v)) ? u : v;
-        fy[f] = y;
-        atomicAdd(&fcnt[y], 1);
+// ---- static adjacency hash for the peeling rounds ------------------------
+// Vertex v's adjacency gets an open-addressing table of cap(v) = the
+// smallest power of two >= 2 deg(v) slots, (neighbour, edge id) per slot,
+// built once per decomposition.  The adjacency only loses entries while
+// peeling, so the tables stay valid: a removed edge is still found and then
+// rejected by its state.
+
+__global__ void k_hash_caps(int64_t m, const int32_t *__restrict__ rst, int64_t *__restrict__ cap) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+        int32_t d = rst[i + 1] - rst[i];
+        int32_t c = 0;
+        if (d > 0) {
+            c = 2;
+            while (c < 2 * d) c <<= 1;
+        }
+        cap[i] = c;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) cap[m] = 0;
+}
+
+__global__ void k_hash_fill(int64_t m, const int32_t *__restrict__ rst, const int32_t *__restrict__ col,
+                            const int32_t *__restrict__ eid, const int64_t *__restrict__ hoff,
+                            int2 *__restrict__ tab) {
+    const int lane = threadIdx.x & 63;
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < m; i += waves) {
+        const int64_t h0 = hoff[i];
+        const uint32_t c = (uint32_t)(hoff[i + 1] - h0);
+        if (!c) continue;
+        const int lg = 31 - __clz((int)c);
+        for (int32_t p = rst[i] + lane; p < rst[i + 1]; p += 64) {
+            const int32_t w = col[p];
+            uint32_t s = dev::hash32((uint32_t)w) >> (32 - lg);
+            while (atomicCAS(&tab[h0 + s].x, -1, w) != -1) s = (s + 1) & (c - 1);
+            tab[h0 + s].y = eid[p];
+        }
     }
 }
 
-__global__ void k_fscatter(int32_t nF, const int32_t *__restrict__ F, const int32_t *__restrict__ eu,
-                           const int32_t *__restrict__ ev, const int32_t *__restrict__ fy,
-                           int32_t *__restrict__ fcur, int32_t *__restrict__ fcol, int32_t *__restrict__ fval) {
-    for (int32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < nF; f += gridDim.x * blockDim.x) {
-        int32_t e = F[f], y = fy[f];
-        int32_t x = eu[e] == y ? ev[e] : eu[e];
-        int32_t pos = atomicAdd(&fcur[y], 1);
-        fcol[pos] = x;
-        fval[pos] = e;
+__device__ __forceinline__ int32_t adj_probe(const int64_t *__restrict__ hoff, const int2 *__restrict__ tab,
+                                             int32_t y, int32_t w) {
+    const int64_t h0 = hoff[y];
+    const uint32_t c = (uint32_t)(hoff[y + 1] - h0);
+    const int lg = 31 - __clz((int)c);
+    uint32_t s = dev::hash32((uint32_t)w) >> (32 - lg);
+    for (;;) {
+        const int2 kv = tab[h0 + s];
+        if (kv.x == w) return kv.y;
+        if (kv.x < 0) return -1;
+        s = (s + 1) & (c - 1);
     }
+}
+
+// Per frontier edge f = (y, x): y = endpoint with the longer current row
+// (probed through its table), x = the other (its alive row is streamed);
+// len[f] = current row length of x.
+__global__ void k_fprep(int32_t nF, const int32_t *__restrict__ F, const int32_t *__restrict__ eu,
+                        const int32_t *__restrict__ ev, const int32_t *__restrict__ rst, int32_t *__restrict__ fy,
+                        int32_t *__restrict__ fx, int32_t *__restrict__ len) {
+    for (int32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < nF; f += gridDim.x * blockDim.x) {
+        int32_t e = F[f], u = eu[e], v = ev[e];
+        int32_t du = rst[u + 1] - rst[u], dv = rst[v + 1] - rst[v];
+        bool uy = du > dv || (du == dv && u > v);
+        int32_t y = uy ? u : v, x = uy ? v : u;
+        fy[f] = y;
+        fx[f] = x;
+        len[f] = uy ? dv : du;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) len[nF] = 0;
+}
+
+// Support update of one round, as the masked product dS<A> = F * A with the
+// adjacency tables as the mask accumulator: the entries w of every frontier
+// edge's streamed row are flattened over all lanes (pre = exclusive scan of
+// len); w closes the triangle (y, x, w), destroyed this round; its owner --
+// the frontier edge with the smallest id among its frontier edges -- takes
+// one support off each of its edges that stays alive.  Edge states: 0 alive,
+// 1 frontier, 2 removed earlier.
+template <int U>
+__global__ __launch_bounds__(256) void k_peel_stream(int32_t nF, const int32_t *__restrict__ pre,
+                                                     const int32_t *__restrict__ F, const int32_t *__restrict__ fy,
+                                                     const int32_t *__restrict__ fx, const int32_t *__restrict__ rst,
+                                                     const int32_t *__restrict__ col, const int32_t *__restrict__ eid,
+                                                     const int64_t *__restrict__ hoff, const int2 *__restrict__ tab,
+                                                     const uint8_t *__restrict__ st, int32_t *__restrict__ sup,
+                                                     unsigned long long *__restrict__ destroyed) {
+    constexpr int CH = 64 * U;
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    const int32_t total = pre[nF];
+    uint32_t cnt = 0;
+    for (int64_t g0 = wave * CH; g0 < total; g0 += nwaves * CH) {
+        // frontier edges covering items [g0, g0 + CH): f_lo .. f_hi
+        const int32_t glast = (int32_t)min<int64_t>(g0 + CH, total) - 1;
+        int32_t lo = 0, hi = nF - 1;
+        while (lo < hi) {  // largest f with pre[f] <= g0
+            int32_t mid = (lo + hi + 1) >> 1;
+            if (pre[mid] <= g0) lo = mid; else hi = mid - 1;
+        }
+        const int32_t f_lo = lo;
+        hi = nF - 1;
+        while (lo < hi) {  // largest f with pre[f] <= glast
+            int32_t mid = (lo + hi + 1) >> 1;
+            if (pre[mid] <= glast) lo = mid; else hi = mid - 1;
+        }
+        const int32_t f_hi = lo;
+        int32_t t[U], fe[U], yy[U], w[U], e2[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int32_t g = (int32_t)g0 + 64 * u + lane;
+            t[u] = -1;
+            if (g < total) {
+                int32_t l = f_lo, h = f_hi;
+                while (l < h) {
+                    int32_t mid = (l + h + 1) >> 1;
+                    if (pre[mid] <= g) l = mid; else h = mid - 1;
+                }
+                fe[u] = F[l];
+                yy[u] = fy[l];
+                t[u] = rst[fx[l]] + (g - pre[l]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (t[u] >= 0) {
+                w[u] = col[t[u]];
+                e2[u] = eid[t[u]];
+            }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (t[u] < 0) continue;
+            const int32_t e = fe[u], b = e2[u];
+            const uint8_t s2 = st[b];
+            if (s2 == 2 || (s2 == 1 && b < e)) continue;
+            const int32_t a = adj_probe(hoff, tab, yy[u], w[u]);
+            if (a < 0) continue;
+            const uint8_t s1 = st[a];
+            if (s1 == 2 || (s1 == 1 && a < e)) continue;
+            ++cnt;
+            if (s1 == 0) atomicSub(&sup[a], 1);
+            if (s2 == 0) atomicSub(&sup[b], 1);
+        }
+    }
+    unsigned long long s = dev::wave_sum((unsigned long long)cnt);
+    if (lane == 0 && s) atomicAdd(destroyed, s);
+}
+
+// Adds a round's streamed-entry count (pre[nF]) to the running total.
+__global__ void k_count_items(const int32_t *__restrict__ total, unsigned long long *__restrict__ ctr) {
+    if (threadIdx.x == 0) ctr[0] += (unsigned long long)*total;
 }
 
 __global__ void k_mark_dead(int32_t nF, const int32_t *__restrict__ F, uint8_t *__restrict__ st) {
@@ -318,9 +454,8 @@ int grid_items(gk_ctx *ctx, int64_t n) {
 
 // Truss numbers of every L0 edge by incremental peeling.  Round:
 //   F = alive edges with support < k-2 (labelled k-1);  none -> k = min + 3;
-//   dS = PEEL product of F (rows: one endpoint) with the alive adjacency,
-//        which takes one support off each surviving edge of every triangle
-//        destroyed this round;
+//   dS = F * A masked by A (k_peel_stream): one support off each surviving
+//        edge of every triangle destroyed this round;
 //   F -> removed; compact the adjacency once a quarter of it is dead.
 // fixed_k > 0 runs the single-k semantics instead (k_truss, src/main.cpp:93-141):
 // the threshold stays k-2, removed edges are labelled 1, and the run stops at
@@ -329,11 +464,14 @@ int grid_items(gk_ctx *ctx, int64_t n) {
 int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, uint32_t *tl, const gk_opts &o,
                int64_t *rounds_out, int32_t *kmax_out, int fixed_k = 0) {
     const int64_t m = A->m, nL = L0.nz;
-    int32_t *sup, *eu, *ev, *alist, *alist2, *F, *fy, *fcnt, *frst, *fcur, *fcol, *fval, *cnt;
+    int32_t *sup, *eu, *ev, *alist, *alist2, *F, *fy, *fx, *flen, *fpre, *cnt;
     int32_t *arst[2], *acol[2], *aeid[2];
+    int64_t *caps, *hoff;
+    int2 *tab;
     uint8_t *st;
     uint32_t *dmin;
     int32_t *dn;
+    unsigned long long *dctr;  // [0] items streamed, [1] triangles destroyed
     GK_TRY(B.alloc(&sup, nL));
     GK_TRY(B.alloc(&eu, nL));
     GK_TRY(B.alloc(&ev, nL));
@@ -341,15 +479,16 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
     GK_TRY(B.alloc(&alist2, nL));
     GK_TRY(B.alloc(&F, nL));
     GK_TRY(B.alloc(&fy, nL));
-    GK_TRY(B.alloc(&fcol, nL));
-    GK_TRY(B.alloc(&fval, nL));
+    GK_TRY(B.alloc(&fx, nL));
+    GK_TRY(B.alloc(&flen, nL + 1));
+    GK_TRY(B.alloc(&fpre, nL + 1));
     GK_TRY(B.alloc(&st, nL));
-    GK_TRY(B.alloc(&fcnt, m + 1));
-    GK_TRY(B.alloc(&frst, m + 1));
-    GK_TRY(B.alloc(&fcur, m + 1));
     GK_TRY(B.alloc(&cnt, m + 1));
+    GK_TRY(B.alloc(&caps, m + 1));
+    GK_TRY(B.alloc(&hoff, m + 1));
     GK_TRY(B.alloc(&dmin, 1));
     GK_TRY(B.alloc(&dn, 2));
+    GK_TRY(B.alloc(&dctr, 2));
     for (int b = 0; b < 2; ++b) {
         GK_TRY(B.alloc(&arst[b], m + 1));
         GK_TRY(B.alloc(&acol[b], A->nz));
@@ -369,6 +508,18 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
     k_sym_eid<<<gw, 256, 0, ctx->stream>>>(m, A->rst, A->col, L0.rst, aeid[0]);
     k_iota<<<grid_items(ctx, nL), 256, 0, ctx->stream>>>(nL, alist);
     GK_HIP(hipMemsetAsync(st, 0, nL, ctx->stream));
+    GK_HIP(hipMemsetAsync(dctr, 0, 16, ctx->stream));
+    // adjacency tables
+    k_hash_caps<<<grid_items(ctx, m), 256, 0, ctx->stream>>>(m, A->rst, caps);
+    GK_TRY(exclusive_scan_i64(ctx, caps, hoff, m + 1));
+    int64_t *h64;
+    GK_TRY(pinned(ctx, 64, (void **)&h64));
+    GK_HIP(hipMemcpyAsync(h64, hoff + m, 8, hipMemcpyDeviceToHost, ctx->stream));
+    GK_HIP(hipStreamSynchronize(ctx->stream));
+    const int64_t tab_n = h64[0];
+    GK_TRY(B.alloc(&tab, tab_n));
+    GK_HIP(hipMemsetAsync(tab, 0xff, sizeof(int2) * (size_t)std::max<int64_t>(tab_n, 1), ctx->stream));
+    k_hash_fill<<<gw, 256, 0, ctx->stream>>>(m, A->rst, A->col, aeid[0], hoff, tab);
     GK_TRY(launch_check("truss_peel setup"));
     int cur = 0;
     int64_t aal_nz = A->nz;      // entries in the current adjacency
@@ -377,8 +528,10 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
     int64_t rounds = 0;
     int k = fixed_k > 0 ? fixed_k : 3;
     int32_t kbest = nL ? 2 : 0;
-    int32_t *h;
-    GK_TRY(pinned(ctx, 16, (void **)&h));
+    int32_t *h = (int32_t *)(h64 + 2);
+    std::vector<hipEvent_t> &ev_pool = ctx->events;
+    size_t ev_used = 0;
+    const int gs = (int)std::max<int64_t>(1, (int64_t)ctx->num_cus * 8);
     while (alive > 0) {
         const uint32_t label = fixed_k > 0 ? 1u : (uint32_t)(k - 1);
         GK_HIP(hipMemsetAsync(dn, 0, 4, ctx->stream));
@@ -400,20 +553,23 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
         }
         ++rounds;
         kbest = k - 1;
-        // frontier matrix
-        GK_HIP(hipMemsetAsync(fcnt, 0, 4 * (m + 1), ctx->stream));
-        k_fcount<<<grid_items(ctx, nF), 256, 0, ctx->stream>>>(nF, F, eu, ev, arst[cur], fy, fcnt);
-        GK_TRY(exclusive_scan_i32(ctx, fcnt, frst, m + 1));
-        GK_HIP(hipMemcpyAsync(fcur, frst, 4 * (m + 1), hipMemcpyDeviceToDevice, ctx->stream));
-        k_fscatter<<<grid_items(ctx, nF), 256, 0, ctx->stream>>>(nF, F, eu, ev, fy, fcur, fcol, fval);
-        GK_TRY(launch_check("frontier matrix"));
         // support update
-        gk_dcsr Fm = view(ctx, m, m, nF, frst, fcol, fval);
-        gk_dcsr Aal = view(ctx, m, m, aal_nz, arst[cur], acol[cur], aeid[cur]);
-        PeelArgs pa{aeid[cur], st, sup};
-        ProductInfo pi;
-        GK_TRY(masked_product(ctx, &Fm, &Aal, &Aal, MODE_PEEL, nullptr, nullptr, nullptr, o, &pi, &pa));
-        record_timing(ctx, pi, false);
+        k_fprep<<<grid_items(ctx, nF), 256, 0, ctx->stream>>>(nF, F, eu, ev, arst[cur], fy, fx, flen);
+        GK_TRY(exclusive_scan_i32(ctx, flen, fpre, nF + 1));
+        if (ev_used + 2 > ev_pool.size()) {
+            hipEvent_t a, b2;
+            GK_HIP(hipEventCreate(&a));
+            GK_HIP(hipEventCreate(&b2));
+            ev_pool.push_back(a);
+            ev_pool.push_back(b2);
+        }
+        GK_HIP(hipEventRecord(ev_pool[ev_used], ctx->stream));
+        k_peel_stream<4><<<gs, 256, 0, ctx->stream>>>(nF, fpre, F, fy, fx, arst[cur], acol[cur], aeid[cur], hoff,
+                                                      tab, st, sup, dctr + 1);
+        GK_HIP(hipEventRecord(ev_pool[ev_used + 1], ctx->stream));
+        ev_used += 2;
+        GK_TRY(launch_check("k_peel_stream"));
+        k_count_items<<<1, 64, 0, ctx->stream>>>(fpre + nF, dctr);
         k_mark_dead<<<grid_items(ctx, nF), 256, 0, ctx->stream>>>(nF, F, st);
         GK_TRY(launch_check("k_mark_dead"));
         alive -= nF;
@@ -438,6 +594,23 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
             dead_pending = 0;
         }
     }
+    // peel-stream figures: device time over all rounds, algorithmic bytes
+    // (per streamed entry: column and edge id 8 B, a table probe 8 B, two
+    // states 2 B; per destroyed triangle: two 4-byte support updates)
+    unsigned long long hc[2];
+    GK_HIP(d2h(ctx, hc, dctr, sizeof hc));
+    double sms = 0;
+    for (size_t i = 0; i < ev_used; i += 2) {
+        float t = 0;
+        GK_HIP(hipEventElapsedTime(&t, ev_pool[i], ev_pool[i + 1]));
+        sms += t;
+    }
+    gk_timing &tt = ctx->last;
+    tt.acc_ms[GK_KERNEL_PEEL] += sms;
+    tt.acc_bytes[GK_KERNEL_PEEL] += 18.0 * (double)hc[0] + 8.0 * (double)hc[1];
+    tt.acc_launches[GK_KERNEL_PEEL] += rounds;
+    tt.kernel_ms += sms;
+    tt.bytes_alg += 18.0 * (double)hc[0] + 8.0 * (double)hc[1];
     if (rounds_out) *rounds_out = fixed_k > 0 ? rounds + 1 : rounds;
     if (kmax_out) *kmax_out = kbest;
     return GK_OK;

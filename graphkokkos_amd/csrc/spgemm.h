@@ -7,7 +7,7 @@
 namespace gk {
 
 // Product modes.
-enum { MODE_TC = 0, MODE_VALUES = 1, MODE_SUPPORT = 2, MODE_PEEL = 3 };
+enum { MODE_TC = 0, MODE_VALUES = 1, MODE_SUPPORT = 2 };
 
 // Capacities (entries).  A task covers at most kMaxTaskA A-entries so its
 // B-row prefix fits LDS; MSA-LDS holds mask rows up to kLdsMaxMask in a
@@ -22,7 +22,7 @@ constexpr int kHashGlobalMaxMask = 1 << 20;
 // the kLdsHashCap table.  The long-row kernel is an internal class of its own.
 constexpr int kLdsSmallMask = 128;
 constexpr int kLdsSmallCap = 512;
-constexpr int kNumClass = GK_NUM_KERNELS;
+constexpr int kNumClass = GK_KERNEL_MSA_LONG + 1;
 constexpr int kClassMsaLarge = GK_NUM_ACC;
 constexpr int kFlopBins = 32;
 constexpr int kNumBins = kNumClass * kFlopBins;
@@ -85,20 +85,12 @@ int plan_tasks(gk_ctx *ctx, const gk_dcsr *A, RowStats &st, const SelectParams &
 SelectParams select_params(const gk_opts &o);
 int zero_u32(gk_ctx *ctx, void *p, int64_t n);
 
-// PEEL mode: edge ids of M's entries (M.val carries them too), states and
-// supports indexed by edge id.
-struct PeelArgs {
-    const int32_t *mval;
-    const uint8_t *state;
-    int32_t *sup;
-};
-
 // C<M> = A*B in the given mode.  d_out: VALUES -> int32 C values (M.nz),
 // SUPPORT -> uint32 supports (M.nz), TC -> unused.  d_count / d_rowcount:
 // TC totals (zeroed by the caller).
 int masked_product(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcsr *M, int mode, int32_t *d_out,
                    unsigned long long *d_count, unsigned long long *d_rowcount, const gk_opts &o,
-                   ProductInfo *info, const PeelArgs *peel = nullptr);
+                   ProductInfo *info);
 
 // Learned selector (selector.hip): writes st.acc for every row.
 int model_select(gk_ctx *ctx, RowStats &st, int64_t m, int64_t ncols);

@@ -93,48 +93,76 @@ __device__ __forceinline__ int32_t task_count(const SelectParams &sp, int32_t al
 }
 
 // Pick the accumulator for every row (or take the learned choice already in
-// st.acc), count its tasks and add them to the bin histogram.
+// st.acc), count its tasks and add them to the bin histogram and the
+// per-class work counters.  Rows of a wave usually fall into one or two bins:
+// the wave reduces each distinct bin with ballots and shuffles, and only
+// that bin's leader lane touches the block's LDS counters.
 __global__ __launch_bounds__(256) void k_select_bin(int64_t m, RowStats st, SelectParams sp,
                                                     int32_t *__restrict__ bin_count,
                                                     int32_t *__restrict__ acc_maxmask,
                                                     unsigned long long *__restrict__ stats) {
     __shared__ int32_t h[kNumBins];
+    __shared__ int32_t mx[kNumClass];
     __shared__ unsigned long long ss[kNumClass * ST_NUM];
     for (int b = threadIdx.x; b < kNumBins; b += blockDim.x) h[b] = 0;
     for (int b = threadIdx.x; b < kNumClass * ST_NUM; b += blockDim.x) ss[b] = 0;
+    for (int b = threadIdx.x; b < kNumClass; b += blockDim.x) mx[b] = 0;
     __syncthreads();
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
-        int32_t mlen = st.mlen[i];
-        int64_t flop = st.flop[i];
-        int acc = -1;
-        if (mlen > 0 && flop > 0) {
-            if (sp.selector == GK_SEL_FORCED) acc = sp.forced;
-            else if (sp.selector == GK_SEL_LEARNED) acc = st.acc[i];
-            else acc = heuristic_accum(sp, mlen);
-            // Capacity overrides: a choice that cannot hold the mask row
-            // falls to the next accumulator that can.
-            if (acc == GK_ACC_MCA_WAVE && !mca_fits(mlen)) acc = GK_ACC_MSA_LDS;
-            if (acc == GK_ACC_MSA_LDS && !lds_fits(mlen)) acc = GK_ACC_HASH_GLOBAL;
-            if (acc == GK_ACC_HASH_GLOBAL && mlen > kHashGlobalMaxMask) acc = GK_ACC_DENSE_GLOBAL;
-            if (acc == GK_ACC_MSA_LDS && mlen > kLdsSmallMask) acc = kClassMsaLarge;
-        }
-        st.acc[i] = (int8_t)acc;  // kernel class from here on
-        if (acc >= 0) {
-            int32_t nt = task_count(sp, st.alen[i], flop);
+    const int lane = threadIdx.x & 63;
+    for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < m; base += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = base + threadIdx.x;
+        int acc = -1, bin = 0;
+        int32_t mlen = 0, alen = 0, nt = 0;
+        int64_t flop = 0;
+        if (i < m) {
+            mlen = st.mlen[i];
+            flop = st.flop[i];
+            alen = st.alen[i];
+            if (mlen > 0 && flop > 0) {
+                if (sp.selector == GK_SEL_FORCED) acc = sp.forced;
+                else if (sp.selector == GK_SEL_LEARNED) acc = st.acc[i];
+                else acc = heuristic_accum(sp, mlen);
+                // Capacity overrides: a choice that cannot hold the mask row
+                // falls to the next accumulator that can.
+                if (acc == GK_ACC_MCA_WAVE && !mca_fits(mlen)) acc = GK_ACC_MSA_LDS;
+                if (acc == GK_ACC_MSA_LDS && !lds_fits(mlen)) acc = GK_ACC_HASH_GLOBAL;
+                if (acc == GK_ACC_HASH_GLOBAL && mlen > kHashGlobalMaxMask) acc = GK_ACC_DENSE_GLOBAL;
+                if (acc == GK_ACC_MSA_LDS && mlen > kLdsSmallMask) acc = kClassMsaLarge;
+                nt = task_count(sp, alen, flop);
+                bin = acc * kFlopBins + (kFlopBins - 1 - flop_bin(flop));
+            }
+            st.acc[i] = (int8_t)acc;  // kernel class from here on
             st.ntask[i] = nt;
-            atomicAdd(&h[acc * kFlopBins + (kFlopBins - 1 - flop_bin(flop))], nt);
-            atomicMax(&acc_maxmask[acc], mlen);
-            atomicAdd(&ss[acc * ST_NUM + ST_TASKS], (unsigned long long)nt);
-            atomicAdd(&ss[acc * ST_NUM + ST_MASK], (unsigned long long)mlen);
-            atomicAdd(&ss[acc * ST_NUM + ST_AROW], (unsigned long long)st.alen[i]);
-            atomicAdd(&ss[acc * ST_NUM + ST_FLOP], (unsigned long long)flop);
-        } else {
-            st.ntask[i] = 0;
+        }
+        bool pend = acc >= 0;
+        for (;;) {
+            const uint64_t act = __ballot(pend);
+            if (!act) break;
+            const int leader = __ffsll((unsigned long long)act) - 1;
+            const int lbin = __shfl(bin, leader, 64);
+            const bool mine = pend && bin == lbin;
+            const int32_t s_nt = dev::wave_sum(mine ? nt : 0);
+            const unsigned long long s_m = dev::wave_sum((unsigned long long)(mine ? mlen : 0));
+            const unsigned long long s_a = dev::wave_sum((unsigned long long)(mine ? alen : 0));
+            const unsigned long long s_f = dev::wave_sum((unsigned long long)(mine ? flop : 0));
+            const int32_t s_x = dev::wave_max(mine ? mlen : 0);
+            if (lane == leader) {
+                const int c = lbin / kFlopBins;
+                atomicAdd(&h[lbin], s_nt);
+                atomicMax(&mx[c], s_x);
+                atomicAdd(&ss[c * ST_NUM + ST_TASKS], (unsigned long long)s_nt);
+                atomicAdd(&ss[c * ST_NUM + ST_MASK], s_m);
+                atomicAdd(&ss[c * ST_NUM + ST_AROW], s_a);
+                atomicAdd(&ss[c * ST_NUM + ST_FLOP], s_f);
+            }
+            pend = pend && !mine;
         }
     }
     __syncthreads();
     for (int b = threadIdx.x; b < kNumBins; b += blockDim.x)
         if (h[b]) atomicAdd(&bin_count[b], h[b]);
+    for (int b = threadIdx.x; b < kNumClass; b += blockDim.x)
+        if (mx[b]) atomicMax(&acc_maxmask[b], mx[b]);
     for (int b = threadIdx.x; b < kNumClass * ST_NUM; b += blockDim.x)
         if (ss[b]) atomicAdd(&stats[b], ss[b]);
 }
@@ -206,9 +234,6 @@ struct Prob {
     unsigned long long *__restrict__ count;     // TC: total
     unsigned long long *__restrict__ rowcount;  // TC: optional per-row counts
     unsigned long long *__restrict__ hits;      // hits per accumulator
-    const int32_t *__restrict__ mval;           // PEEL: edge id per mask entry
-    const uint8_t *__restrict__ estate;         // PEEL: edge states
-    int32_t *__restrict__ esup;                 // PEEL: edge supports
     const Task *__restrict__ tasks;
     const int32_t *__restrict__ bin_off;
 };
@@ -281,27 +306,6 @@ struct Hit<MODE_SUPPORT> {
         atomicAdd((uint32_t *)&acc[q], 1u);          // edge (i, j)
         atomicAdd((uint32_t *)&acnt[kk], 1u);        // edge (i, k)
         atomicAdd((uint32_t *)&P.cval[t], 1u);       // edge (k, j)
-    }
-};
-// Support update of one truss-peeling round: A holds the round's frontier
-// edges (row y, column x, value = edge id e), M and B the current adjacency
-// with edge ids as values.  A hit w closes the triangle (y, x, w); it is
-// destroyed this round, and its owner -- the frontier edge with the smallest
-// id among its frontier edges -- takes one support off each of its edges that
-// stays alive.  Edge states: 0 alive, 1 frontier, 2 removed earlier.
-template <>
-struct Hit<MODE_PEEL> {
-    __device__ static void on(const Prob &P, int32_t *, int32_t *, int32_t q, int32_t mb, int32_t a0, int32_t kk,
-                              int32_t t, uint32_t &cnt) {
-        const int32_t e = P.aval[a0 + kk];
-        const int32_t e1 = P.mval[mb + q];
-        const int32_t e2 = P.bval[t];
-        const uint8_t s1 = P.estate[e1], s2 = P.estate[e2];
-        if (s1 == 2 || s2 == 2) return;
-        if ((s1 == 1 && e1 < e) || (s2 == 1 && e2 < e)) return;
-        ++cnt;
-        if (s1 == 0) atomicSub(&P.esup[e1], 1);
-        if (s2 == 0) atomicSub(&P.esup[e2], 1);
     }
 };
 
@@ -744,14 +748,56 @@ static int dense_maps(gk_ctx *ctx, int64_t blocks, int64_t ncols, int32_t **out)
     return GK_OK;
 }
 
+// Launch the accumulator kernels of one product.  Each class runs on its own
+// auxiliary stream (forked from the context stream after the plan, joined
+// back before the caller continues), so the classes share the GPU instead
+// of running one after the other.
 template <int MODE>
 static int run_mode(gk_ctx *ctx, const Prob &P, const Plan &plan, int64_t bn, double *acc_ms) {
-    Timer tm[kNumClass] = {Timer(ctx), Timer(ctx), Timer(ctx), Timer(ctx), Timer(ctx)};
+    static_assert(kNumClass <= gk_ctx::kAux + 1, "one stream per kernel class");
+    hipStream_t sc[kNumClass];
+    // the context stream takes dense-global, whose scratch setup runs there
+    sc[GK_ACC_DENSE_GLOBAL] = ctx->stream;
+    sc[kClassMsaLarge] = ctx->aux[0];
+    sc[GK_ACC_MSA_LDS] = ctx->aux[1];
+    sc[GK_ACC_MCA_WAVE] = ctx->aux[2];
+    sc[GK_ACC_HASH_GLOBAL] = ctx->aux[3];
+    Timer tm[kNumClass] = {Timer(ctx, sc[0]), Timer(ctx, sc[1]), Timer(ctx, sc[2]), Timer(ctx, sc[3]),
+                           Timer(ctx, sc[4])};
     bool ran[kNumClass] = {};
+    // scratch for the global accumulators is sized and bound on the context
+    // stream before the fork
+    int32_t *hkey = nullptr, *hpos = nullptr, *hacc = nullptr, *dmap = nullptr, *dacc = nullptr;
+    int hlg = 6, hg = 0, dg = 0;
+    int32_t dmm = 0;
+    if (plan.acc_tasks[GK_ACC_HASH_GLOBAL]) {
+        while ((1 << hlg) < 2 * plan.acc_maxmask[GK_ACC_HASH_GLOBAL]) ++hlg;
+        hg = grid_for(ctx, plan.acc_tasks[GK_ACC_HASH_GLOBAL], kGlobalBlocksPerCu);
+        size_t cap = (size_t)1 << hlg;
+        // Bound the tables to about 4 GiB.
+        while (hg > 1 && (size_t)hg * cap * 10 > ((size_t)4 << 30)) hg >>= 1;
+        void *sp;
+        GK_TRY(scratch(ctx, SC_ACC, (size_t)hg * cap * 4 * 2 + (size_t)hg * (cap / 2) * 4, &sp));
+        hkey = (int32_t *)sp;
+        hpos = hkey + (size_t)hg * cap;
+        hacc = hpos + (size_t)hg * cap;
+    }
+    if (plan.acc_tasks[GK_ACC_DENSE_GLOBAL]) {
+        dg = grid_for(ctx, plan.acc_tasks[GK_ACC_DENSE_GLOBAL], kGlobalBlocksPerCu);
+        // Bound the map memory to about 4 GiB.
+        while (dg > 1 && (size_t)dg * (size_t)bn * 4 > ((size_t)4 << 30)) dg >>= 1;
+        dmm = plan.acc_maxmask[GK_ACC_DENSE_GLOBAL];
+        GK_TRY(dense_maps(ctx, dg, bn, &dmap));
+        void *sp;
+        GK_TRY(scratch(ctx, SC_TMP7, (size_t)dg * (size_t)dmm * 4 + 4, &sp));
+        dacc = (int32_t *)sp;
+    }
+    GK_HIP(hipEventRecord(ctx->fork, ctx->stream));
+    for (int i = 0; i < gk_ctx::kAux; ++i) GK_HIP(hipStreamWaitEvent(ctx->aux[i], ctx->fork, 0));
     if (plan.acc_tasks[kClassMsaLarge]) {
         int g = grid_for(ctx, plan.acc_tasks[kClassMsaLarge], kLdsBlocksPerCu);
         tm[kClassMsaLarge].start();
-        k_msa_lds<MODE, kLdsHashCap, kLdsMaxMask><<<g, 64, 0, ctx->stream>>>(P, kClassMsaLarge);
+        k_msa_lds<MODE, kLdsHashCap, kLdsMaxMask><<<g, 64, 0, sc[kClassMsaLarge]>>>(P, kClassMsaLarge);
         tm[kClassMsaLarge].stop();
         ran[kClassMsaLarge] = true;
         GK_TRY(launch_check("k_msa_lds<large>"));
@@ -759,7 +805,7 @@ static int run_mode(gk_ctx *ctx, const Prob &P, const Plan &plan, int64_t bn, do
     if (plan.acc_tasks[GK_ACC_MSA_LDS]) {
         int g = grid_for(ctx, plan.acc_tasks[GK_ACC_MSA_LDS], kLdsSmallBlocksPerCu);
         tm[GK_ACC_MSA_LDS].start();
-        k_msa_lds<MODE, kLdsSmallCap, kLdsSmallMask><<<g, 64, 0, ctx->stream>>>(P, GK_ACC_MSA_LDS);
+        k_msa_lds<MODE, kLdsSmallCap, kLdsSmallMask><<<g, 64, 0, sc[GK_ACC_MSA_LDS]>>>(P, GK_ACC_MSA_LDS);
         tm[GK_ACC_MSA_LDS].stop();
         ran[GK_ACC_MSA_LDS] = true;
         GK_TRY(launch_check("k_msa_lds<small>"));
@@ -767,41 +813,28 @@ static int run_mode(gk_ctx *ctx, const Prob &P, const Plan &plan, int64_t bn, do
     if (plan.acc_tasks[GK_ACC_MCA_WAVE]) {
         int g = grid_for(ctx, (plan.acc_tasks[GK_ACC_MCA_WAVE] + 3) / 4, kMcaBlocksPerCu);
         tm[GK_ACC_MCA_WAVE].start();
-        k_mca_wave<MODE, 16><<<g, 64, 0, ctx->stream>>>(P, GK_ACC_MCA_WAVE);
+        k_mca_wave<MODE, 16><<<g, 64, 0, sc[GK_ACC_MCA_WAVE]>>>(P, GK_ACC_MCA_WAVE);
         tm[GK_ACC_MCA_WAVE].stop();
         ran[GK_ACC_MCA_WAVE] = true;
         GK_TRY(launch_check("k_mca_wave"));
     }
     if (plan.acc_tasks[GK_ACC_HASH_GLOBAL]) {
-        int lg = 6;
-        while ((1 << lg) < 2 * plan.acc_maxmask[GK_ACC_HASH_GLOBAL]) ++lg;
-        int g = grid_for(ctx, plan.acc_tasks[GK_ACC_HASH_GLOBAL], kGlobalBlocksPerCu);
-        void *sp;
-        size_t cap = (size_t)1 << lg;
-        // Bound the tables to about 4 GiB.
-        while (g > 1 && (size_t)g * cap * 10 > ((size_t)4 << 30)) g >>= 1;
-        GK_TRY(scratch(ctx, SC_ACC, (size_t)g * cap * 4 * 2 + (size_t)g * (cap / 2) * 4, &sp));
-        int32_t *key = (int32_t *)sp, *pos = key + (size_t)g * cap, *acc = pos + (size_t)g * cap;
         tm[GK_ACC_HASH_GLOBAL].start();
-        k_hash_global<MODE><<<g, 64, 0, ctx->stream>>>(P, GK_ACC_HASH_GLOBAL, key, pos, acc, lg);
+        k_hash_global<MODE><<<hg, 64, 0, sc[GK_ACC_HASH_GLOBAL]>>>(P, GK_ACC_HASH_GLOBAL, hkey, hpos, hacc, hlg);
         tm[GK_ACC_HASH_GLOBAL].stop();
         ran[GK_ACC_HASH_GLOBAL] = true;
         GK_TRY(launch_check("k_hash_global"));
     }
     if (plan.acc_tasks[GK_ACC_DENSE_GLOBAL]) {
-        int g = grid_for(ctx, plan.acc_tasks[GK_ACC_DENSE_GLOBAL], kGlobalBlocksPerCu);
-        // Bound the map memory to about 4 GiB.
-        while (g > 1 && (size_t)g * (size_t)bn * 4 > ((size_t)4 << 30)) g >>= 1;
-        int32_t mm = plan.acc_maxmask[GK_ACC_DENSE_GLOBAL];
-        int32_t *map;
-        GK_TRY(dense_maps(ctx, g, bn, &map));
-        void *sp;
-        GK_TRY(scratch(ctx, SC_TMP7, (size_t)g * (size_t)mm * 4 + 4, &sp));
         tm[GK_ACC_DENSE_GLOBAL].start();
-        k_dense_global<MODE><<<g, 64, 0, ctx->stream>>>(P, GK_ACC_DENSE_GLOBAL, map, (int32_t *)sp, bn, mm);
+        k_dense_global<MODE><<<dg, 64, 0, sc[GK_ACC_DENSE_GLOBAL]>>>(P, GK_ACC_DENSE_GLOBAL, dmap, dacc, bn, dmm);
         tm[GK_ACC_DENSE_GLOBAL].stop();
         ran[GK_ACC_DENSE_GLOBAL] = true;
         GK_TRY(launch_check("k_dense_global"));
+    }
+    for (int i = 0; i < gk_ctx::kAux; ++i) {
+        GK_HIP(hipEventRecord(ctx->join[i], ctx->aux[i]));
+        GK_HIP(hipStreamWaitEvent(ctx->stream, ctx->join[i], 0));
     }
     for (int a = 0; a < kNumClass; ++a) acc_ms[a] = ran[a] ? tm[a].ms() : 0.0;
     return GK_OK;
@@ -819,8 +852,8 @@ SelectParams select_params(const gk_opts &o) {
 }
 
 int masked_product(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcsr *M, int mode, int32_t *d_out,
-                   unsigned long long *d_count, unsigned long long *d_rowcount, const gk_opts &o, ProductInfo *info,
-                   const PeelArgs *peel) {
+                   unsigned long long *d_count, unsigned long long *d_rowcount, const gk_opts &o,
+                   ProductInfo *info) {
     if (A->n != B->m || A->m != M->m || B->n != M->n) {
         set_error("shape mismatch: A %lldx%lld, B %lldx%lld, M %lldx%lld", (long long)A->m, (long long)A->n,
                   (long long)B->m, (long long)B->n, (long long)M->m, (long long)M->n);
@@ -842,19 +875,11 @@ int masked_product(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcs
     P.brst = B->rst; P.bcol = B->col; P.bval = B->val;
     P.mrst = M->rst; P.mcol = M->col;
     P.cval = d_out; P.count = d_count; P.rowcount = d_rowcount; P.hits = plan.hits;
-    P.mval = peel ? peel->mval : nullptr;
-    P.estate = peel ? peel->state : nullptr;
-    P.esup = peel ? peel->sup : nullptr;
-    if (mode == MODE_PEEL && !peel) {
-        set_error("peel mode needs edge states and supports");
-        return GK_EINVAL;
-    }
     P.tasks = plan.tasks; P.bin_off = plan.bin_off;
     double acc_ms[kNumClass] = {};
     if (mode == MODE_TC) GK_TRY(run_mode<MODE_TC>(ctx, P, plan, B->n, acc_ms));
     else if (mode == MODE_VALUES) GK_TRY(run_mode<MODE_VALUES>(ctx, P, plan, B->n, acc_ms));
-    else if (mode == MODE_SUPPORT) GK_TRY(run_mode<MODE_SUPPORT>(ctx, P, plan, B->n, acc_ms));
-    else GK_TRY(run_mode<MODE_PEEL>(ctx, P, plan, B->n, acc_ms));
+    else GK_TRY(run_mode<MODE_SUPPORT>(ctx, P, plan, B->n, acc_ms));
     total.stop();
     if (info) {
         unsigned long long hs[kNumClass * (ST_NUM + 1)];

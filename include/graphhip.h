@@ -77,9 +77,11 @@ enum gk_accum {
     GK_ACC_DENSE_GLOBAL= 3,  /* dense column-indexed marks in HBM scratch      */
     GK_NUM_ACC         = 4
 };
-/* Accumulator kernels reported in gk_timing: the four accumulators, and
- * index 4 = the MSA-LDS kernel for long mask rows (larger LDS table). */
-#define GK_NUM_KERNELS 5
+/* Kernels reported in gk_timing: the four accumulators (0-3), the MSA-LDS
+ * kernel for long mask rows (4) and the truss-peeling support update (5). */
+#define GK_KERNEL_MSA_LONG 4
+#define GK_KERNEL_PEEL     5
+#define GK_NUM_KERNELS     6
 enum gk_selector {
     GK_SEL_HEURISTIC = 0,   /* fixed thresholds (the fallback)            */
     GK_SEL_LEARNED   = 1,   /* MLP cost model, needs gk_model_load/train  */
