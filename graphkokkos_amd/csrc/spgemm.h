@@ -22,6 +22,13 @@ constexpr int kHashGlobalMaxMask = 1 << 20;
 // the kLdsHashCap table.  The long-row kernel is an internal class of its own.
 constexpr int kLdsSmallMask = 128;
 constexpr int kLdsSmallCap = 512;
+// Triangle counting: MSA-LDS keeps mask columns below kWindow in an LDS
+// bitmap and the rest in a hash of kLdsSmallCap / kWindowHashLong slots.
+constexpr int kWindow = 32768;
+constexpr int kWindowHashLong = 2048;
+// long mask rows: one workgroup of kWindowWavesLong waves per task
+constexpr int kWindowWavesLong = 4;
+constexpr int kWindowBlocksLong = 8;  // per CU
 constexpr int kNumClass = GK_KERNEL_MSA_LONG + 1;
 constexpr int kClassMsaLarge = GK_NUM_ACC;
 constexpr int kFlopBins = 32;
