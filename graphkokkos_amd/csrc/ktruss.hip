@@ -199,11 +199,11 @@ __global__ void k_iota(int64_t n, int32_t *__restrict__ a) {
         a[i] = (int32_t)i;
 }
 
-// Frontier of a round: alive edges with support below `thresh` become
-// frontier edges (state 1) labelled `label`, appended to F.
-__global__ void k_frontier(int64_t na, const int32_t *__restrict__ alist, uint8_t *__restrict__ st,
-                           const int32_t *__restrict__ sup, int32_t thresh, uint32_t *__restrict__ truss,
-                           uint32_t label, int32_t *__restrict__ F, int32_t *__restrict__ nF) {
+// Frontier at a new threshold: alive edges with support below `thresh`,
+// appended to F (their states and labels are set by the round prologue).
+__global__ void k_frontier(int64_t na, const int32_t *__restrict__ alist, const uint8_t *__restrict__ st,
+                           const int32_t *__restrict__ sup, int32_t thresh, int32_t *__restrict__ F,
+                           int32_t *__restrict__ nF) {
     const int lane = threadIdx.x & 63;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t base = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63); base < na; base += stride) {
@@ -219,11 +219,7 @@ __global__ void k_frontier(int64_t na, const int32_t *__restrict__ alist, uint8_
         int32_t first = 0;
         if (lane == 0) first = atomicAdd(nF, __popcll(bal));
         first = __shfl(first, 0, 64);
-        if (take) {
-            F[first + __popcll(bal & ((1ull << lane) - 1))] = e;
-            st[e] = 1;
-            truss[e] = label;
-        }
+        if (take) F[first + __popcll(bal & ((1ull << lane) - 1))] = e;
     }
 }
 
@@ -291,22 +287,69 @@ __device__ __forceinline__ int32_t adj_probe(const int64_t *__restrict__ hoff, c
     }
 }
 
+// Round prologue.  The previous round's frontier Fp becomes removed
+// (state 2); the new frontier F becomes state 1 with truss label `label`.
 // Per frontier edge f = (y, x): y = endpoint with the longer current row
 // (probed through its table), x = the other (its alive row is streamed);
 // len[f] = current row length of x.
-__global__ void k_fprep(int32_t nF, const int32_t *__restrict__ F, const int32_t *__restrict__ eu,
-                        const int32_t *__restrict__ ev, const int32_t *__restrict__ rst, int32_t *__restrict__ fy,
-                        int32_t *__restrict__ fx, int32_t *__restrict__ len) {
-    for (int32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < nF; f += gridDim.x * blockDim.x) {
-        int32_t e = F[f], u = eu[e], v = ev[e];
-        int32_t du = rst[u + 1] - rst[u], dv = rst[v + 1] - rst[v];
-        bool uy = du > dv || (du == dv && u > v);
-        int32_t y = uy ? u : v, x = uy ? v : u;
-        fy[f] = y;
-        fx[f] = x;
-        len[f] = uy ? dv : du;
-    }
+__device__ __forceinline__ int32_t prep_edge(int32_t f, const int32_t *__restrict__ F, uint8_t *__restrict__ st,
+                                             uint32_t *__restrict__ truss, uint32_t label,
+                                             const int32_t *__restrict__ eu, const int32_t *__restrict__ ev,
+                                             const int32_t *__restrict__ rst, int32_t *__restrict__ fy,
+                                             int32_t *__restrict__ fx) {
+    const int32_t e = F[f], u = eu[e], v = ev[e];
+    st[e] = 1;
+    truss[e] = label;
+    const int32_t du = rst[u + 1] - rst[u], dv = rst[v + 1] - rst[v];
+    const bool uy = du > dv || (du == dv && u > v);
+    fy[f] = uy ? u : v;
+    fx[f] = uy ? v : u;
+    return uy ? dv : du;
+}
+
+__global__ void k_round_prologue(int32_t nP, const int32_t *__restrict__ Fp, int32_t nF,
+                                 const int32_t *__restrict__ F, uint8_t *__restrict__ st,
+                                 uint32_t *__restrict__ truss, uint32_t label, const int32_t *__restrict__ eu,
+                                 const int32_t *__restrict__ ev, const int32_t *__restrict__ rst,
+                                 int32_t *__restrict__ fy, int32_t *__restrict__ fx, int32_t *__restrict__ len) {
+    const int32_t stride = gridDim.x * blockDim.x;
+    for (int32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < nP; f += stride) st[Fp[f]] = 2;
+    for (int32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < nF; f += stride)
+        len[f] = prep_edge(f, F, st, truss, label, eu, ev, rst, fy, fx);
     if (blockIdx.x == 0 && threadIdx.x == 0) len[nF] = 0;
+}
+
+// The same for frontiers of at most kSmallRound edges in one block, with
+// the exclusive scan of len written straight to pre.
+constexpr int kSmallRoundThreads = 1024;
+constexpr int kSmallRoundPer = 4;
+constexpr int kSmallRound = kSmallRoundThreads * kSmallRoundPer;
+__global__ __launch_bounds__(kSmallRoundThreads) void k_round_small(
+    int32_t nP, const int32_t *__restrict__ Fp, int32_t nF, const int32_t *__restrict__ F, uint8_t *__restrict__ st,
+    uint32_t *__restrict__ truss, uint32_t label, const int32_t *__restrict__ eu, const int32_t *__restrict__ ev,
+    const int32_t *__restrict__ rst, int32_t *__restrict__ fy, int32_t *__restrict__ fx, int32_t *__restrict__ pre) {
+    __shared__ int32_t wsum[kSmallRoundThreads / 64];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    for (int32_t f = tid; f < nP; f += kSmallRoundThreads) st[Fp[f]] = 2;
+    int32_t l[kSmallRoundPer], run = 0;
+#pragma unroll
+    for (int j = 0; j < kSmallRoundPer; ++j) {
+        const int32_t f = tid * kSmallRoundPer + j;
+        l[j] = f < nF ? prep_edge(f, F, st, truss, label, eu, ev, rst, fy, fx) : 0;
+        run += l[j];
+    }
+    const int32_t incl = dev::wave_incl_scan(run);
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    int32_t before = 0;
+    for (int x = 0; x < w; ++x) before += wsum[x];
+    int32_t ex = before + incl - run;
+#pragma unroll
+    for (int j = 0; j < kSmallRoundPer; ++j) {
+        const int32_t f = tid * kSmallRoundPer + j;
+        if (f <= nF) pre[f] = ex;
+        ex += l[j];
+    }
 }
 
 // Support update of one round, as the masked product dS<A> = F * A with the
@@ -323,7 +366,8 @@ __global__ __launch_bounds__(256) void k_peel_stream(int32_t nF, const int32_t *
                                                      const int32_t *__restrict__ col, const int32_t *__restrict__ eid,
                                                      const int64_t *__restrict__ hoff, const int2 *__restrict__ tab,
                                                      const uint8_t *__restrict__ st, int32_t *__restrict__ sup,
-                                                     unsigned long long *__restrict__ destroyed) {
+                                                     int32_t thresh, int32_t *__restrict__ Fn,
+                                                     int32_t *__restrict__ nFn, unsigned long long *__restrict__ ctr) {
     constexpr int CH = 64 * U;
     const int lane = threadIdx.x & 63;
     const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -378,17 +422,16 @@ __global__ __launch_bounds__(256) void k_peel_stream(int32_t nF, const int32_t *
             const uint8_t s1 = st[a];
             if (s1 == 2 || (s1 == 1 && a < e)) continue;
             ++cnt;
-            if (s1 == 0) atomicSub(&sup[a], 1);
-            if (s2 == 0) atomicSub(&sup[b], 1);
+            // an alive edge whose support falls below the threshold right
+            // now (old value == thresh; supports drop by one at a time)
+            // joins the next round's frontier
+            if (s1 == 0 && atomicSub(&sup[a], 1) == thresh) Fn[atomicAdd(nFn, 1)] = a;
+            if (s2 == 0 && atomicSub(&sup[b], 1) == thresh) Fn[atomicAdd(nFn, 1)] = b;
         }
     }
     unsigned long long s = dev::wave_sum((unsigned long long)cnt);
-    if (lane == 0 && s) atomicAdd(destroyed, s);
-}
-
-// Adds a round's streamed-entry count (pre[nF]) to the running total.
-__global__ void k_count_items(const int32_t *__restrict__ total, unsigned long long *__restrict__ ctr) {
-    if (threadIdx.x == 0) ctr[0] += (unsigned long long)*total;
+    if (lane == 0 && s) atomicAdd(&ctr[1], s);
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctr[0], (unsigned long long)total);
 }
 
 __global__ void k_mark_dead(int32_t nF, const int32_t *__restrict__ F, uint8_t *__restrict__ st) {
@@ -464,7 +507,7 @@ int grid_items(gk_ctx *ctx, int64_t n) {
 int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, uint32_t *tl, const gk_opts &o,
                int64_t *rounds_out, int32_t *kmax_out, int fixed_k = 0) {
     const int64_t m = A->m, nL = L0.nz;
-    int32_t *sup, *eu, *ev, *alist, *alist2, *F, *fy, *fx, *flen, *fpre, *cnt;
+    int32_t *sup, *eu, *ev, *alist, *alist2, *F, *Fn, *fy, *fx, *flen, *fpre, *cnt;
     int32_t *arst[2], *acol[2], *aeid[2];
     int64_t *caps, *hoff;
     int2 *tab;
@@ -478,6 +521,7 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
     GK_TRY(B.alloc(&alist, nL));
     GK_TRY(B.alloc(&alist2, nL));
     GK_TRY(B.alloc(&F, nL));
+    GK_TRY(B.alloc(&Fn, nL));
     GK_TRY(B.alloc(&fy, nL));
     GK_TRY(B.alloc(&fx, nL));
     GK_TRY(B.alloc(&flen, nL + 1));
@@ -532,16 +576,23 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
     std::vector<hipEvent_t> &ev_pool = ctx->events;
     size_t ev_used = 0;
     const int gs = (int)std::max<int64_t>(1, (int64_t)ctx->num_cus * 8);
+    int32_t *Fp = fy;  // placeholder, nP == 0 before the first round
+    int32_t nP = 0;    // previous frontier, still to be marked removed
+    int32_t nF = 0;    // current frontier (in F)
+    bool scan_needed = true;  // the frontier comes from a full scan
     while (alive > 0) {
-        const uint32_t label = fixed_k > 0 ? 1u : (uint32_t)(k - 1);
-        GK_HIP(hipMemsetAsync(dn, 0, 4, ctx->stream));
-        k_frontier<<<grid_items(ctx, na), 256, 0, ctx->stream>>>(na, alist, st, sup, k - 2, tl, label, F, dn);
-        GK_TRY(launch_check("k_frontier"));
-        GK_HIP(hipMemcpyAsync(h, dn, 4, hipMemcpyDeviceToHost, ctx->stream));
-        GK_HIP(hipStreamSynchronize(ctx->stream));
-        const int32_t nF = h[0];
-        if (nF == 0 && fixed_k > 0) break;
+        if (scan_needed) {
+            // New threshold: the frontier is every alive edge below it.
+            GK_HIP(hipMemsetAsync(dn, 0, 4, ctx->stream));
+            k_frontier<<<grid_items(ctx, na), 256, 0, ctx->stream>>>(na, alist, st, sup, k - 2, F, dn);
+            GK_TRY(launch_check("k_frontier"));
+            GK_HIP(hipMemcpyAsync(h, dn, 4, hipMemcpyDeviceToHost, ctx->stream));
+            GK_HIP(hipStreamSynchronize(ctx->stream));
+            nF = h[0];
+            scan_needed = false;
+        }
         if (nF == 0) {
+            if (fixed_k > 0) break;
             GK_HIP(hipMemsetAsync(dmin, 0, 4, ctx->stream));
             k_min_alive<<<grid_items(ctx, na), 256, 0, ctx->stream>>>(na, alist, st, sup, dmin);
             GK_TRY(launch_check("k_min_alive"));
@@ -549,13 +600,25 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
             GK_HIP(hipStreamSynchronize(ctx->stream));
             const uint32_t mn = ~(uint32_t)h[0];
             k = (int)std::max<int64_t>(k + 1, (int64_t)mn + 3);
+            scan_needed = true;
             continue;
         }
         ++rounds;
         kbest = k - 1;
-        // support update
-        k_fprep<<<grid_items(ctx, nF), 256, 0, ctx->stream>>>(nF, F, eu, ev, arst[cur], fy, fx, flen);
-        GK_TRY(exclusive_scan_i32(ctx, flen, fpre, nF + 1));
+        const uint32_t label = fixed_k > 0 ? 1u : (uint32_t)(k - 1);
+        // prologue: previous frontier removed, this one marked and prepared
+        if (nF < kSmallRound) {
+            k_round_small<<<1, kSmallRoundThreads, 0, ctx->stream>>>(nP, Fp, nF, F, st, tl, label, eu, ev, arst[cur],
+                                                                     fy, fx, fpre);
+        } else {
+            k_round_prologue<<<grid_items(ctx, std::max(nF, nP)), 256, 0, ctx->stream>>>(
+                nP, Fp, nF, F, st, tl, label, eu, ev, arst[cur], fy, fx, flen);
+            GK_TRY(exclusive_scan_i32(ctx, flen, fpre, nF + 1));
+        }
+        GK_TRY(launch_check("round prologue"));
+        alive -= nF;
+        dead_pending += nP;
+        // support update; edges falling below the threshold go to Fn
         if (ev_used + 2 > ev_pool.size()) {
             hipEvent_t a, b2;
             GK_HIP(hipEventCreate(&a));
@@ -563,19 +626,24 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
             ev_pool.push_back(a);
             ev_pool.push_back(b2);
         }
+        GK_HIP(hipMemsetAsync(dn, 0, 4, ctx->stream));
         GK_HIP(hipEventRecord(ev_pool[ev_used], ctx->stream));
         k_peel_stream<4><<<gs, 256, 0, ctx->stream>>>(nF, fpre, F, fy, fx, arst[cur], acol[cur], aeid[cur], hoff,
-                                                      tab, st, sup, dctr + 1);
+                                                      tab, st, sup, k - 2, Fn, dn, dctr);
         GK_HIP(hipEventRecord(ev_pool[ev_used + 1], ctx->stream));
         ev_used += 2;
         GK_TRY(launch_check("k_peel_stream"));
-        k_count_items<<<1, 64, 0, ctx->stream>>>(fpre + nF, dctr);
-        k_mark_dead<<<grid_items(ctx, nF), 256, 0, ctx->stream>>>(nF, F, st);
-        GK_TRY(launch_check("k_mark_dead"));
-        alive -= nF;
-        dead_pending += nF;
+        GK_HIP(hipMemcpyAsync(h, dn, 4, hipMemcpyDeviceToHost, ctx->stream));
+        GK_HIP(hipStreamSynchronize(ctx->stream));
+        // this round's frontier is removed by the next prologue
+        std::swap(F, Fn);
+        Fp = Fn;  // (the old F)
+        nP = nF;
+        nF = h[0];
         // compact once a quarter of the adjacency entries are dead
-        if (alive > 0 && 2 * dead_pending * 4 > aal_nz) {
+        if (alive > 0 && 2 * (dead_pending + nP) * 4 > aal_nz) {
+            k_mark_dead<<<grid_items(ctx, nP), 256, 0, ctx->stream>>>(nP, Fp, st);
+            nP = 0;
             const int nx = cur ^ 1;
             k_alive_count<<<gw, 256, 0, ctx->stream>>>(m, arst[cur], aeid[cur], st, cnt);
             GK_TRY(exclusive_scan_i32(ctx, cnt, arst[nx], m + 1));

@@ -484,11 +484,21 @@ __device__ __forceinline__ int32_t build_prefix_block(const Prob &P, int32_t a0,
     return total;
 }
 
-template <int NW, int HC, int MAXM>
-__global__ __launch_bounds__(64 * NW) void k_tc_window(Prob P, int cls) {
+// TC and SUPPORT modes.  SUPPORT also needs the mask position of a hit:
+// the sorted mask row is copied to LDS and a hit below kWindow finds its
+// position by binary search there (hits are a few percent of the probes);
+// the hash for the columns above the window stores positions directly.
+template <int MODE, int NW, int HC, int MAXM>
+__global__ __launch_bounds__(64 * NW) void k_msa_window(Prob P, int cls) {
+    static_assert(MODE == MODE_TC || MODE == MODE_SUPPORT, "window kernel modes");
     constexpr int NT = 64 * NW;
+    constexpr bool SUP = MODE == MODE_SUPPORT;
     __shared__ uint32_t bm[kWindow / 32];
     __shared__ int32_t hkey[HC];
+    __shared__ int32_t hpos[SUP ? HC : 1];
+    __shared__ int32_t mk[SUP ? MAXM : 1];
+    __shared__ int32_t acc[SUP ? MAXM : 1];
+    __shared__ int32_t acnt[SUP ? kMaxTaskA : 1];
     __shared__ int32_t pre[kMaxTaskA + 1];
     __shared__ int32_t bst[kMaxTaskA];
     __shared__ int32_t wsum[NW];
@@ -511,42 +521,58 @@ __global__ __launch_bounds__(64 * NW) void k_tc_window(Prob P, int cls) {
         const uint32_t cmask = (1u << lg) - 1;
         if (nhigh)
             for (uint32_t s = tid; s <= cmask; s += NT) hkey[s] = -1;
+        if (SUP) {
+            for (int32_t q = tid; q < mlen; q += NT) acc[q] = 0;
+            for (int32_t x = tid; x < tk.a1 - tk.a0; x += NT) acnt[x] = 0;
+        }
         __syncthreads();
         for (int32_t q = tid; q < nlow; q += NT) {
             const int32_t c = P.mcol[mb + q];
             atomicOr(&bm[c >> 5], 1u << (c & 31));
+            if (SUP) mk[q] = c;
         }
         for (int32_t q = nlow + tid; q < mlen; q += NT) {
             const int32_t c = P.mcol[mb + q];
             uint32_t s = dev::hash32((uint32_t)c) >> (32 - lg);
             while (atomicCAS(&hkey[s], -1, c) != -1) s = (s + 1) & cmask;
+            if (SUP) hpos[s] = q;
         }
         int32_t total;
         if constexpr (NW == 1) total = build_prefix<64>(P, tk.a0, tk.a1, pre, bst, tid);
         else total = build_prefix_block<NT>(P, tk.a0, tk.a1, pre, bst, wsum);
         __syncthreads();
-        const uint32_t c = stream_entries<MODE_TC, NT, 4>(P, tk, mb, pre, bst, total, tid, nullptr, nullptr,
-                                                          [&](int32_t b) -> int32_t {
-            if (b < kWindow) return ((bm[b >> 5] >> (b & 31)) & 1u) ? 0 : -1;
+        const uint32_t c = stream_entries<MODE, NT, 4>(P, tk, mb, pre, bst, total, tid, acc, acnt,
+                                                       [&](int32_t b) -> int32_t {
+            if (b < kWindow) {
+                if (!((bm[b >> 5] >> (b & 31)) & 1u)) return -1;
+                if (!SUP) return 0;
+                int32_t x = 0, y = nlow;  // position of b in mk[0, nlow)
+                while (x < y) {
+                    int32_t mid = (x + y) >> 1;
+                    if (mk[mid] < b) x = mid + 1; else y = mid;
+                }
+                return x;
+            }
             if (!nhigh) return -1;
             uint32_t s = dev::hash32((uint32_t)b) >> (32 - lg);
             for (;;) {
                 const int32_t key = hkey[s];
-                if (key == b) return 0;
+                if (key == b) return SUP ? hpos[s] : 0;
                 if (key < 0) return -1;
                 s = (s + 1) & cmask;
             }
         });
         __syncthreads();
         for (int32_t q = tid; q < nlow; q += NT) bm[P.mcol[mb + q] >> 5] = 0;
-        if (P.rowcount) {
+        if (SUP) flush<MODE, NT>(P, tk, mb, mlen, acc, acnt, tid);
+        if (MODE == MODE_TC && P.rowcount) {
             uint32_t r = dev::wave_sum(c);
             if ((tid & 63) == 0 && r) atomicAdd(&P.rowcount[tk.row], (unsigned long long)r);
         }
         cnt += c;
         __syncthreads();
     }
-    finish_kernel<MODE_TC>(P, cnt, cls, tid & 63);
+    finish_kernel<MODE>(P, cnt, cls, tid & 63);
 }
 
 // ---- MCA-wave -------------------------------------------------------------
@@ -903,8 +929,8 @@ static int run_mode(gk_ctx *ctx, const Prob &P, const Plan &plan, int64_t bn, do
     if (plan.acc_tasks[kClassMsaLarge]) {
         int g = grid_for(ctx, plan.acc_tasks[kClassMsaLarge], kLdsBlocksPerCu);
         tm[kClassMsaLarge].start();
-        if (MODE == MODE_TC)
-            k_tc_window<kWindowWavesLong, kWindowHashLong, kLdsMaxMask>
+        if (MODE == MODE_TC || MODE == MODE_SUPPORT)
+            k_msa_window<MODE == MODE_VALUES ? MODE_TC : MODE, kWindowWavesLong, kWindowHashLong, kLdsMaxMask>
                 <<<grid_for(ctx, plan.acc_tasks[kClassMsaLarge], kWindowBlocksLong), 64 * kWindowWavesLong, 0,
                    sc[kClassMsaLarge]>>>(P, kClassMsaLarge);
         else
@@ -916,8 +942,9 @@ static int run_mode(gk_ctx *ctx, const Prob &P, const Plan &plan, int64_t bn, do
     if (plan.acc_tasks[GK_ACC_MSA_LDS]) {
         int g = grid_for(ctx, plan.acc_tasks[GK_ACC_MSA_LDS], kLdsSmallBlocksPerCu);
         tm[GK_ACC_MSA_LDS].start();
-        if (MODE == MODE_TC)
-            k_tc_window<1, kLdsSmallCap, kLdsSmallMask><<<g, 64, 0, sc[GK_ACC_MSA_LDS]>>>(P, GK_ACC_MSA_LDS);
+        if (MODE == MODE_TC || MODE == MODE_SUPPORT)
+            k_msa_window<MODE == MODE_VALUES ? MODE_TC : MODE, 1, kLdsSmallCap, kLdsSmallMask>
+                <<<g, 64, 0, sc[GK_ACC_MSA_LDS]>>>(P, GK_ACC_MSA_LDS);
         else
             k_msa_lds<MODE, kLdsSmallCap, kLdsSmallMask><<<g, 64, 0, sc[GK_ACC_MSA_LDS]>>>(P, GK_ACC_MSA_LDS);
         tm[GK_ACC_MSA_LDS].stop();
