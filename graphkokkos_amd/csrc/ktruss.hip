@@ -200,19 +200,25 @@ __global__ void k_iota(int64_t n, int32_t *__restrict__ a) {
 }
 
 // Frontier at a new threshold: alive edges with support below `thresh`,
-// appended to F (their states and labels are set by the round prologue).
+// appended to F (their states and labels are set by the round prologue),
+// and the minimum support over the alive edges (as its complement in *mn).
 __global__ void k_frontier(int64_t na, const int32_t *__restrict__ alist, const uint8_t *__restrict__ st,
                            const int32_t *__restrict__ sup, int32_t thresh, int32_t *__restrict__ F,
-                           int32_t *__restrict__ nF) {
+                           int32_t *__restrict__ nF, uint32_t *__restrict__ mn) {
     const int lane = threadIdx.x & 63;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    uint32_t lo = 0xffffffffu;
     for (int64_t base = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63); base < na; base += stride) {
         int64_t x = base + lane;
         bool take = false;
         int32_t e = 0;
         if (x < na) {
             e = alist[x];
-            take = st[e] == 0 && sup[e] < thresh;
+            if (st[e] == 0) {
+                const int32_t v = sup[e];
+                lo = min(lo, (uint32_t)v);
+                take = v < thresh;
+            }
         }
         uint64_t bal = __ballot(take);
         if (!bal) continue;
@@ -221,18 +227,10 @@ __global__ void k_frontier(int64_t na, const int32_t *__restrict__ alist, const 
         first = __shfl(first, 0, 64);
         if (take) F[first + __popcll(bal & ((1ull << lane) - 1))] = e;
     }
+    lo = dev::wave_max(~lo);
+    if (lane == 0 && lo) atomicMax(mn, lo);
 }
 
-__global__ void k_min_alive(int64_t na, const int32_t *__restrict__ alist, const uint8_t *__restrict__ st,
-                            const int32_t *__restrict__ sup, uint32_t *__restrict__ out) {
-    uint32_t mn = 0xffffffffu;
-    for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < na; x += (int64_t)gridDim.x * blockDim.x) {
-        int32_t e = alist[x];
-        if (st[e] == 0) mn = min(mn, (uint32_t)sup[e]);
-    }
-    mn = dev::wave_max(~mn);
-    if ((threadIdx.x & 63) == 0 && mn) atomicMax(out, mn);
-}
 
 // ---- static adjacency hash for the peeling rounds ------------------------
 // Vertex v's adjacency gets an open-addressing table of cap(v) = the
@@ -359,8 +357,20 @@ __global__ __launch_bounds__(kSmallRoundThreads) void k_round_small(
 // the frontier edge with the smallest id among its frontier edges -- takes
 // one support off each of its edges that stays alive.  Edge states: 0 alive,
 // 1 frontier, 2 removed earlier.
+// Control block of device-driven rounds (frontiers small enough for the
+// one-block prologue): the kernels of a batch of rounds read the sizes here
+// instead of from the host.  F lives in buf[cur], the previous frontier and
+// this round's crossings in buf[cur ^ 1].
+struct RoundCtl {
+    int32_t nP, nF, nFn, cur;
+    int32_t stop;     // set when a round cannot run on the device path
+    int32_t rounds;   // rounds run by the batch
+    int32_t removed;  // frontier edges of those rounds
+    int32_t pad;
+};
+
 template <int U>
-__global__ __launch_bounds__(256) void k_peel_stream(int32_t nF, const int32_t *__restrict__ pre,
+__device__ __forceinline__ void k_peel_body(int32_t nF, const int32_t *__restrict__ pre,
                                                      const int32_t *__restrict__ F, const int32_t *__restrict__ fy,
                                                      const int32_t *__restrict__ fx, const int32_t *__restrict__ rst,
                                                      const int32_t *__restrict__ col, const int32_t *__restrict__ eid,
@@ -432,6 +442,85 @@ __global__ __launch_bounds__(256) void k_peel_stream(int32_t nF, const int32_t *
     unsigned long long s = dev::wave_sum((unsigned long long)cnt);
     if (lane == 0 && s) atomicAdd(&ctr[1], s);
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctr[0], (unsigned long long)total);
+}
+
+template <int U>
+__global__ __launch_bounds__(256) void k_peel_stream(int32_t nF, const int32_t *__restrict__ pre,
+                                                     const int32_t *__restrict__ F, const int32_t *__restrict__ fy,
+                                                     const int32_t *__restrict__ fx, const int32_t *__restrict__ rst,
+                                                     const int32_t *__restrict__ col, const int32_t *__restrict__ eid,
+                                                     const int64_t *__restrict__ hoff, const int2 *__restrict__ tab,
+                                                     const uint8_t *__restrict__ st, int32_t *__restrict__ sup,
+                                                     int32_t thresh, int32_t *__restrict__ Fn,
+                                                     int32_t *__restrict__ nFn, unsigned long long *__restrict__ ctr) {
+    k_peel_body<U>(nF, pre, F, fy, fx, rst, col, eid, hoff, tab, st, sup, thresh, Fn, nFn, ctr);
+}
+
+// Device-driven round: prologue + scan in one block (as k_round_small),
+// sizes from the control block.  A frontier that is empty or too big for
+// one block stops the batch; the remaining kernels of the batch return.
+__global__ __launch_bounds__(kSmallRoundThreads) void k_dround_prologue(
+    RoundCtl *__restrict__ ctl, int32_t *__restrict__ buf0, int32_t *__restrict__ buf1, uint8_t *__restrict__ st,
+    uint32_t *__restrict__ truss, uint32_t label, const int32_t *__restrict__ eu, const int32_t *__restrict__ ev,
+    const int32_t *__restrict__ rst, int32_t *__restrict__ fy, int32_t *__restrict__ fx, int32_t *__restrict__ pre) {
+    __shared__ int32_t wsum[kSmallRoundThreads / 64];
+    if (ctl->stop) return;
+    const int32_t nF = ctl->nF, nP = ctl->nP;
+    if (nF == 0 || nF >= kSmallRound) {
+        if (threadIdx.x == 0) ctl->stop = 1;
+        return;
+    }
+    const int32_t *F = ctl->cur ? buf1 : buf0;
+    const int32_t *Fp = ctl->cur ? buf0 : buf1;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    for (int32_t f = tid; f < nP; f += kSmallRoundThreads) st[Fp[f]] = 2;
+    int32_t l[kSmallRoundPer], run = 0;
+#pragma unroll
+    for (int j = 0; j < kSmallRoundPer; ++j) {
+        const int32_t f = tid * kSmallRoundPer + j;
+        l[j] = f < nF ? prep_edge(f, F, st, truss, label, eu, ev, rst, fy, fx) : 0;
+        run += l[j];
+    }
+    const int32_t incl = dev::wave_incl_scan(run);
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    int32_t before = 0;
+    for (int x = 0; x < w; ++x) before += wsum[x];
+    int32_t ex = before + incl - run;
+#pragma unroll
+    for (int j = 0; j < kSmallRoundPer; ++j) {
+        const int32_t f = tid * kSmallRoundPer + j;
+        if (f <= nF) pre[f] = ex;
+        ex += l[j];
+    }
+    if (tid == 0) ctl->nFn = 0;
+}
+
+template <int U>
+__global__ __launch_bounds__(256) void k_dround_stream(const RoundCtl *__restrict__ ctl, int32_t *__restrict__ buf0,
+                                                       int32_t *__restrict__ buf1, const int32_t *__restrict__ pre,
+                                                       const int32_t *__restrict__ fy, const int32_t *__restrict__ fx,
+                                                       const int32_t *__restrict__ rst,
+                                                       const int32_t *__restrict__ col,
+                                                       const int32_t *__restrict__ eid,
+                                                       const int64_t *__restrict__ hoff, const int2 *__restrict__ tab,
+                                                       const uint8_t *__restrict__ st, int32_t *__restrict__ sup,
+                                                       int32_t thresh, int32_t *__restrict__ nFn,
+                                                       unsigned long long *__restrict__ ctr) {
+    if (ctl->stop) return;
+    const int32_t cur = ctl->cur;
+    // the crossings go to buf[cur ^ 1], counted in ctl->nFn
+    k_peel_body<U>(ctl->nF, pre, cur ? buf1 : buf0, fy, fx, rst, col, eid, hoff, tab, st, sup, thresh,
+                   cur ? buf0 : buf1, nFn, ctr);
+}
+
+__global__ void k_dround_advance(RoundCtl *__restrict__ ctl) {
+    if (threadIdx.x != 0 || ctl->stop) return;
+    ctl->rounds += 1;
+    ctl->removed += ctl->nF;
+    ctl->nP = ctl->nF;
+    ctl->nF = ctl->nFn;
+    ctl->cur ^= 1;
 }
 
 __global__ void k_mark_dead(int32_t nF, const int32_t *__restrict__ F, uint8_t *__restrict__ st) {
@@ -515,6 +604,7 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
     uint32_t *dmin;
     int32_t *dn;
     unsigned long long *dctr;  // [0] items streamed, [1] triangles destroyed
+    RoundCtl *dctl;
     GK_TRY(B.alloc(&sup, nL));
     GK_TRY(B.alloc(&eu, nL));
     GK_TRY(B.alloc(&ev, nL));
@@ -533,6 +623,7 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
     GK_TRY(B.alloc(&dmin, 1));
     GK_TRY(B.alloc(&dn, 2));
     GK_TRY(B.alloc(&dctr, 2));
+    GK_TRY(B.alloc(&dctl, 1));
     for (int b = 0; b < 2; ++b) {
         GK_TRY(B.alloc(&arst[b], m + 1));
         GK_TRY(B.alloc(&acol[b], A->nz));
@@ -557,7 +648,7 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
     k_hash_caps<<<grid_items(ctx, m), 256, 0, ctx->stream>>>(m, A->rst, caps);
     GK_TRY(exclusive_scan_i64(ctx, caps, hoff, m + 1));
     int64_t *h64;
-    GK_TRY(pinned(ctx, 64, (void **)&h64));
+    GK_TRY(pinned(ctx, 128, (void **)&h64));
     GK_HIP(hipMemcpyAsync(h64, hoff + m, 8, hipMemcpyDeviceToHost, ctx->stream));
     GK_HIP(hipStreamSynchronize(ctx->stream));
     const int64_t tab_n = h64[0];
@@ -565,7 +656,7 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
     GK_HIP(hipMemsetAsync(tab, 0xff, sizeof(int2) * (size_t)std::max<int64_t>(tab_n, 1), ctx->stream));
     k_hash_fill<<<gw, 256, 0, ctx->stream>>>(m, A->rst, A->col, aeid[0], hoff, tab);
     GK_TRY(launch_check("truss_peel setup"));
-    int cur = 0;
+    int cur = 0;                 // adjacency buffer in use
     int64_t aal_nz = A->nz;      // entries in the current adjacency
     int64_t na = nL;             // entries in the alive list
     int64_t alive = nL, dead_pending = 0;
@@ -573,76 +664,130 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
     int k = fixed_k > 0 ? fixed_k : 3;
     int32_t kbest = nL ? 2 : 0;
     int32_t *h = (int32_t *)(h64 + 2);
+    RoundCtl *hctl = (RoundCtl *)(h64 + 4);
     std::vector<hipEvent_t> &ev_pool = ctx->events;
     size_t ev_used = 0;
+    auto ev_pair = [&](size_t *at) -> int {
+        while (ev_used + 2 > ev_pool.size()) {
+            hipEvent_t e;
+            GK_HIP(hipEventCreate(&e));
+            ev_pool.push_back(e);
+        }
+        *at = ev_used;
+        ev_used += 2;
+        return GK_OK;
+    };
     const int gs = (int)std::max<int64_t>(1, (int64_t)ctx->num_cus * 8);
-    int32_t *Fp = fy;  // placeholder, nP == 0 before the first round
-    int32_t nP = 0;    // previous frontier, still to be marked removed
-    int32_t nF = 0;    // current frontier (in F)
-    bool scan_needed = true;  // the frontier comes from a full scan
+    int32_t *fb[2] = {F, Fn};    // frontier buffers: F in fb[fc], previous / crossings in fb[fc ^ 1]
+    int fc = 0;
+    int32_t nP = 0;              // previous frontier, still to be marked removed
+    int32_t nF = 0;              // current frontier
+    bool scan_needed = true;     // the frontier comes from a full scan
+    bool started = false;
+    constexpr int kBatch = 16;   // device-driven rounds per host check
     while (alive > 0) {
         if (scan_needed) {
-            // New threshold: the frontier is every alive edge below it.
+            // A level is finished (or the run starts): every alive edge has
+            // support >= k-2.  The next frontier is the alive edges at the
+            // minimum support s, at threshold k' - 2 with k' = max(k+1, s+3)
+            // (the start uses k itself).  One pass collects the edges below
+            // k'-2 for k' = k (+1) and the minimum; a second pass is needed
+            // only when that finds nothing.
+            const int kt = started ? k + 1 : k;
             GK_HIP(hipMemsetAsync(dn, 0, 4, ctx->stream));
-            k_frontier<<<grid_items(ctx, na), 256, 0, ctx->stream>>>(na, alist, st, sup, k - 2, F, dn);
+            GK_HIP(hipMemsetAsync(dmin, 0, 4, ctx->stream));
+            k_frontier<<<grid_items(ctx, na), 256, 0, ctx->stream>>>(na, alist, st, sup, kt - 2, fb[fc], dn, dmin);
             GK_TRY(launch_check("k_frontier"));
             GK_HIP(hipMemcpyAsync(h, dn, 4, hipMemcpyDeviceToHost, ctx->stream));
+            GK_HIP(hipMemcpyAsync(h + 1, dmin, 4, hipMemcpyDeviceToHost, ctx->stream));
             GK_HIP(hipStreamSynchronize(ctx->stream));
             nF = h[0];
+            if (fixed_k > 0 || nF > 0) {
+                k = kt;
+            } else {
+                const uint32_t mn = ~(uint32_t)h[1];
+                k = (int)std::max<int64_t>(kt + 1, (int64_t)mn + 3);
+                GK_HIP(hipMemsetAsync(dn, 0, 4, ctx->stream));
+                k_frontier<<<grid_items(ctx, na), 256, 0, ctx->stream>>>(na, alist, st, sup, k - 2, fb[fc], dn, dmin);
+                GK_TRY(launch_check("k_frontier"));
+                GK_HIP(hipMemcpyAsync(h, dn, 4, hipMemcpyDeviceToHost, ctx->stream));
+                GK_HIP(hipStreamSynchronize(ctx->stream));
+                nF = h[0];
+            }
+            started = true;
             scan_needed = false;
         }
         if (nF == 0) {
             if (fixed_k > 0) break;
-            GK_HIP(hipMemsetAsync(dmin, 0, 4, ctx->stream));
-            k_min_alive<<<grid_items(ctx, na), 256, 0, ctx->stream>>>(na, alist, st, sup, dmin);
-            GK_TRY(launch_check("k_min_alive"));
-            GK_HIP(hipMemcpyAsync(h, dmin, 4, hipMemcpyDeviceToHost, ctx->stream));
-            GK_HIP(hipStreamSynchronize(ctx->stream));
-            const uint32_t mn = ~(uint32_t)h[0];
-            k = (int)std::max<int64_t>(k + 1, (int64_t)mn + 3);
             scan_needed = true;
             continue;
         }
-        ++rounds;
-        kbest = k - 1;
         const uint32_t label = fixed_k > 0 ? 1u : (uint32_t)(k - 1);
-        // prologue: previous frontier removed, this one marked and prepared
-        if (nF < kSmallRound) {
-            k_round_small<<<1, kSmallRoundThreads, 0, ctx->stream>>>(nP, Fp, nF, F, st, tl, label, eu, ev, arst[cur],
-                                                                     fy, fx, fpre);
-        } else {
+        kbest = k - 1;
+        int64_t marked = 0;  // edges marked removed by the prologues below
+        if (nF >= kSmallRound) {
+            // one large round driven from the host
+            ++rounds;
             k_round_prologue<<<grid_items(ctx, std::max(nF, nP)), 256, 0, ctx->stream>>>(
-                nP, Fp, nF, F, st, tl, label, eu, ev, arst[cur], fy, fx, flen);
+                nP, fb[fc ^ 1], nF, fb[fc], st, tl, label, eu, ev, arst[cur], fy, fx, flen);
             GK_TRY(exclusive_scan_i32(ctx, flen, fpre, nF + 1));
+            GK_TRY(launch_check("round prologue"));
+            size_t e0;
+            GK_TRY(ev_pair(&e0));
+            GK_HIP(hipMemsetAsync(dn, 0, 4, ctx->stream));
+            GK_HIP(hipEventRecord(ev_pool[e0], ctx->stream));
+            k_peel_stream<4><<<gs, 256, 0, ctx->stream>>>(nF, fpre, fb[fc], fy, fx, arst[cur], acol[cur], aeid[cur],
+                                                          hoff, tab, st, sup, k - 2, fb[fc ^ 1], dn, dctr);
+            GK_HIP(hipEventRecord(ev_pool[e0 + 1], ctx->stream));
+            GK_TRY(launch_check("k_peel_stream"));
+            GK_HIP(hipMemcpyAsync(h, dn, 4, hipMemcpyDeviceToHost, ctx->stream));
+            GK_HIP(hipStreamSynchronize(ctx->stream));
+            marked = nP;
+            alive -= nF;
+            nP = nF;
+            nF = h[0];
+            fc ^= 1;
+        } else {
+            // a batch of small rounds driven from the device
+            *hctl = RoundCtl{nP, nF, 0, fc, 0, 0, 0, 0};
+            GK_HIP(hipMemcpyAsync(dctl, hctl, sizeof(RoundCtl), hipMemcpyHostToDevice, ctx->stream));
+            const int gsd = std::min(gs, 1024);
+            for (int r = 0; r < kBatch; ++r) {
+                k_dround_prologue<<<1, kSmallRoundThreads, 0, ctx->stream>>>(dctl, fb[0], fb[1], st, tl, label, eu,
+                                                                            ev, arst[cur], fy, fx, fpre);
+                size_t e0;
+                GK_TRY(ev_pair(&e0));
+                GK_HIP(hipEventRecord(ev_pool[e0], ctx->stream));
+                k_dround_stream<4><<<gsd, 256, 0, ctx->stream>>>(dctl, fb[0], fb[1], fpre, fy, fx, arst[cur],
+                                                                 acol[cur], aeid[cur], hoff, tab, st, sup, k - 2,
+                                                                 &dctl->nFn, dctr);
+                GK_HIP(hipEventRecord(ev_pool[e0 + 1], ctx->stream));
+                k_dround_advance<<<1, 64, 0, ctx->stream>>>(dctl);
+            }
+            GK_TRY(launch_check("device rounds"));
+            GK_HIP(hipMemcpyAsync(hctl, dctl, sizeof(RoundCtl), hipMemcpyDeviceToHost, ctx->stream));
+            GK_HIP(hipStreamSynchronize(ctx->stream));
+            rounds += hctl->rounds;
+            alive -= hctl->removed;
+            marked = (int64_t)nP + hctl->removed - hctl->nP;
+            nP = hctl->nP;
+            nF = hctl->nF;
+            fc = hctl->cur;
         }
-        GK_TRY(launch_check("round prologue"));
-        alive -= nF;
-        dead_pending += nP;
-        // support update; edges falling below the threshold go to Fn
-        if (ev_used + 2 > ev_pool.size()) {
-            hipEvent_t a, b2;
-            GK_HIP(hipEventCreate(&a));
-            GK_HIP(hipEventCreate(&b2));
-            ev_pool.push_back(a);
-            ev_pool.push_back(b2);
+        dead_pending += marked;
+        // the alive list is compacted once half of it is gone
+        if (alive > 0 && 2 * (alive - nF) < na && !(2 * (dead_pending + nP) * 4 > aal_nz)) {
+            GK_HIP(hipMemsetAsync(dn + 1, 0, 4, ctx->stream));
+            k_alist_compact<<<grid_items(ctx, na), 256, 0, ctx->stream>>>(na, alist, st, alist2, dn + 1);
+            GK_TRY(launch_check("k_alist_compact"));
+            GK_HIP(hipMemcpyAsync(h + 1, dn + 1, 4, hipMemcpyDeviceToHost, ctx->stream));
+            GK_HIP(hipStreamSynchronize(ctx->stream));
+            na = h[1];
+            std::swap(alist, alist2);
         }
-        GK_HIP(hipMemsetAsync(dn, 0, 4, ctx->stream));
-        GK_HIP(hipEventRecord(ev_pool[ev_used], ctx->stream));
-        k_peel_stream<4><<<gs, 256, 0, ctx->stream>>>(nF, fpre, F, fy, fx, arst[cur], acol[cur], aeid[cur], hoff,
-                                                      tab, st, sup, k - 2, Fn, dn, dctr);
-        GK_HIP(hipEventRecord(ev_pool[ev_used + 1], ctx->stream));
-        ev_used += 2;
-        GK_TRY(launch_check("k_peel_stream"));
-        GK_HIP(hipMemcpyAsync(h, dn, 4, hipMemcpyDeviceToHost, ctx->stream));
-        GK_HIP(hipStreamSynchronize(ctx->stream));
-        // this round's frontier is removed by the next prologue
-        std::swap(F, Fn);
-        Fp = Fn;  // (the old F)
-        nP = nF;
-        nF = h[0];
         // compact once a quarter of the adjacency entries are dead
         if (alive > 0 && 2 * (dead_pending + nP) * 4 > aal_nz) {
-            k_mark_dead<<<grid_items(ctx, nP), 256, 0, ctx->stream>>>(nP, Fp, st);
+            k_mark_dead<<<grid_items(ctx, nP), 256, 0, ctx->stream>>>(nP, fb[fc ^ 1], st);
             nP = 0;
             const int nx = cur ^ 1;
             k_alive_count<<<gw, 256, 0, ctx->stream>>>(m, arst[cur], aeid[cur], st, cnt);
