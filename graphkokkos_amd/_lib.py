@@ -37,7 +37,8 @@ class GkOpts(C.Structure):
     _fields_ = [("struct_size", C.c_int32), ("selector", C.c_int32), ("forced_accum", C.c_int32),
                 ("mca_max_mask", C.c_int32), ("lds_max_mask", C.c_int32), ("dense_min_mask", C.c_int32),
                 ("split_flop", C.c_int32), ("block_threads", C.c_int32), ("verbose", C.c_int32),
-                ("reserved", C.c_int32 * 7)]
+                ("small_max_mask", C.c_int32), ("blocks_small", C.c_int32), ("blocks_long", C.c_int32),
+                ("blocks_mca", C.c_int32), ("blocks_global", C.c_int32), ("reserved", C.c_int32 * 2)]
 
 
 class GkTiming(C.Structure):
@@ -94,6 +95,17 @@ _SIGS = {
     "gk_bc_batch_f64": (C.c_int, [C.c_void_p, C.POINTER(GkCsr), i32p, C.c_int64, f64p, C.POINTER(GkOpts)]),
     "gk_bc_batch_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, i32p, C.c_int64, C.c_void_p,
                                   C.POINTER(GkOpts)]),
+    "gk_model_num_params": (C.c_int, []),
+    "gk_model_set": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_int]),
+    "gk_model_get": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_int]),
+    "gk_model_clear": (C.c_int, [C.c_void_p]),
+    "gk_model_forward": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_int64, C.POINTER(C.c_float)]),
+    "gk_model_grad": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_uint8),
+                                C.c_int64, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "gk_model_train": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_uint8),
+                                 C.c_int64, C.c_int, C.c_int, C.c_float, C.c_uint64, C.POINTER(C.c_float)]),
+    "gk_selector_samples": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                      C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_uint8)]),
     "gk_dmalloc": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
     "gk_dfree": (None, [C.c_void_p, C.c_void_p]),
     "gk_memcpy_d2h": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),

@@ -136,6 +136,11 @@ extern "C" void gk_opts_default(gk_opts *o) {
     o->dense_min_mask = 16384;
     o->split_flop = 1 << 18;
     o->block_threads = 64;
+    o->small_max_mask = 128;
+    o->blocks_small = 24;
+    o->blocks_long = 8;
+    o->blocks_mca = 8;
+    o->blocks_global = 4;
 }
 
 extern "C" int gk_init(int device, gk_ctx **out) {

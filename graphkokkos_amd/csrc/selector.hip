@@ -1,22 +1,505 @@
-// Learned accumulator selector (placeholder until the cost model lands).
+// Learned accumulator selector: a 16 -> 64 -> 64 -> 4 FP32 MLP that
+// predicts, for every row of a masked product, the cost of each accumulator
+// (MSA-LDS, MCA-wave, hash-global, dense-global) in log2 CU-cycles; the row
+// takes the accumulator with the smallest prediction.  Rows the chosen
+// accumulator cannot hold fall back by the same capacity rules as the
+// fixed-threshold heuristic, which stays the default selector.
+//
+// Training data come from the product itself: for every accumulator the
+// product runs with that accumulator forced and every task records its
+// clock64 duration; a row's label is the sum over its tasks divided by the
+// number of tasks the kernel keeps resident per CU (cycles of CU time).
+//
+// Kernels, all hand-written:
+//   k_features      16 features per row from the row statistics
+//   k_mlp_select    features -> MLP forward -> argmin, weights in LDS
+//   k_mlp_forward   MLP forward over a feature matrix (evaluation, tests)
+//   k_mlp_grad      masked mean-squared-error loss and its gradient,
+//                   per-block LDS accumulation then one atomic per weight
+//   k_adam          Adam update
+//   k_row_cycles    per-row label aggregation from per-task durations
+#include <cmath>
+#include <cstring>
+#include <vector>
+
 #include "gk_device.h"
 #include "spgemm.h"
 
+using namespace gk;
+
 namespace gk {
 
+constexpr int kIn = 16, kH = 64, kOut = GK_NUM_ACC;
+// parameter layout (floats)
+constexpr int oW1 = 0, oB1 = oW1 + kIn * kH, oW2 = oB1 + kH, oB2 = oW2 + kH * kH, oW3 = oB2 + kH,
+              oB3 = oW3 + kH * kOut, oMean = oB3 + kOut, oStd = oMean + kIn, kParams = oStd + kIn;
+constexpr int kTrainable = oMean;  // mean/std are fixed from the training set
+
 struct Model {
-    int dummy;
+    float *params = nullptr;  // device, kParams
+    bool ready = false;
 };
 
-void model_free(Model *m) { delete m; }
+void model_free(Model *m) {
+    if (!m) return;
+    if (m->params) (void)hipFree(m->params);
+    delete m;
+}
 
-int model_select(gk_ctx *ctx, RowStats &st, int64_t m, int64_t ncols) {
-    (void)st; (void)m; (void)ncols;
+static int model_get(gk_ctx *ctx, Model **out) {
     if (!ctx->model) {
-        set_error("learned selector requested but no model is loaded");
-        return GK_ESTATE;
+        ctx->model = new Model();
+        if (hipMalloc(&ctx->model->params, sizeof(float) * kParams) != hipSuccess) {
+            set_error("hipMalloc for the selector model failed");
+            return GK_ENOMEM;
+        }
     }
+    *out = ctx->model;
     return GK_OK;
 }
 
+// ---- features ----------------------------------------------------------------
+// log2-scaled sizes plus a one-hot of the mask length's range:
+//  0 log2(1+mask)   1 log2(1+|A row|)   2 log2(1+flop)   3 log2(1+max |B_k|)
+//  4 log2(1+flop/|A row|)  5 log2(1+flop/mask)  6 log2(1+mask)-log2(1+|A row|)
+//  7 1 if A's pattern is the mask's (C<L> = L*L style products)
+//  8..15 mask length in (0,8], (8,32], (32,64], (64,128], (128,512],
+//        (512,1536], (1536,16384], above
+__device__ __forceinline__ void row_features(const RowStats &st, int64_t i, int same_am, float *f) {
+    const float ml = (float)st.mlen[i], al = (float)st.alen[i], fl = (float)st.flop[i], bm = (float)st.bmax[i];
+    f[0] = log2f(1.f + ml);
+    f[1] = log2f(1.f + al);
+    f[2] = log2f(1.f + fl);
+    f[3] = log2f(1.f + bm);
+    f[4] = log2f(1.f + fl / fmaxf(al, 1.f));
+    f[5] = log2f(1.f + fl / fmaxf(ml, 1.f));
+    f[6] = f[0] - f[1];
+    f[7] = (float)same_am;
+    const int32_t m = st.mlen[i];
+    const int32_t lim[7] = {8, 32, 64, 128, 512, 1536, 16384};
+    int b = 7;
+    for (int x = 0; x < 7; ++x)
+        if (m <= lim[x]) { b = x; break; }
+#pragma unroll
+    for (int x = 0; x < 8; ++x) f[8 + x] = x == b ? 1.f : 0.f;
+}
+
+__global__ void k_features(int64_t m, RowStats st, int same_am, float *__restrict__ X) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+        float f[kIn];
+        row_features(st, i, same_am, f);
+#pragma unroll
+        for (int x = 0; x < kIn; ++x) X[i * kIn + x] = f[x];
+    }
+}
+
+// Forward pass of one sample with the weights in LDS; h1/h2 keep the
+// post-activation hidden layers for the backward pass.
+__device__ __forceinline__ void mlp_forward(const float *__restrict__ w, const float *xin, float *h1, float *h2,
+                                            float *y) {
+    float x[kIn];
+#pragma unroll
+    for (int a = 0; a < kIn; ++a) x[a] = (xin[a] - w[oMean + a]) / w[oStd + a];
+    for (int j = 0; j < kH; ++j) {
+        float s = w[oB1 + j];
+#pragma unroll
+        for (int a = 0; a < kIn; ++a) s += x[a] * w[oW1 + a * kH + j];
+        h1[j] = s > 0.f ? s : 0.f;
+    }
+    for (int j = 0; j < kH; ++j) {
+        float s = w[oB2 + j];
+        for (int a = 0; a < kH; ++a) s += h1[a] * w[oW2 + a * kH + j];
+        h2[j] = s > 0.f ? s : 0.f;
+    }
+#pragma unroll
+    for (int o = 0; o < kOut; ++o) {
+        float s = w[oB3 + o];
+        for (int a = 0; a < kH; ++a) s += h2[a] * w[oW3 + a * kOut + o];
+        y[o] = s;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_mlp_select(int64_t m, RowStats st, int same_am,
+                                                    const float *__restrict__ params) {
+    __shared__ float w[kParams];
+    for (int x = threadIdx.x; x < kParams; x += blockDim.x) w[x] = params[x];
+    __syncthreads();
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+        float f[kIn], h1[kH], h2[kH], y[kOut];
+        row_features(st, i, same_am, f);
+        mlp_forward(w, f, h1, h2, y);
+        int best = 0;
+#pragma unroll
+        for (int o = 1; o < kOut; ++o)
+            if (y[o] < y[best]) best = o;
+        st.acc[i] = (int8_t)best;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_mlp_forward(int64_t n, const float *__restrict__ X,
+                                                     const float *__restrict__ params, float *__restrict__ Y) {
+    __shared__ float w[kParams];
+    for (int x = threadIdx.x; x < kParams; x += blockDim.x) w[x] = params[x];
+    __syncthreads();
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        float h1[kH], h2[kH], y[kOut];
+        mlp_forward(w, X + i * kIn, h1, h2, y);
+#pragma unroll
+        for (int o = 0; o < kOut; ++o) Y[i * kOut + o] = y[o];
+    }
+}
+
+// Loss = sum over samples and valid outputs of (y - t)^2 / count; gradient
+// of the trainable parameters accumulated into g (zeroed by the caller).
+__global__ __launch_bounds__(256) void k_mlp_grad(int64_t n, const float *__restrict__ X, const float *__restrict__ T,
+                                                  const uint8_t *__restrict__ V, const int64_t *__restrict__ idx,
+                                                  float inv_count, const float *__restrict__ params,
+                                                  float *__restrict__ g, float *__restrict__ loss) {
+    __shared__ float w[kParams];
+    __shared__ float gl[kTrainable];
+    for (int x = threadIdx.x; x < kParams; x += blockDim.x) w[x] = params[x];
+    for (int x = threadIdx.x; x < kTrainable; x += blockDim.x) gl[x] = 0.f;
+    __syncthreads();
+    float lsum = 0.f;
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = idx ? idx[r] : r;
+        float x[kIn], h1[kH], h2[kH], y[kOut], dy[kOut];
+        mlp_forward(w, X + i * kIn, h1, h2, y);
+#pragma unroll
+        for (int a = 0; a < kIn; ++a) x[a] = (X[i * kIn + a] - w[oMean + a]) / w[oStd + a];
+#pragma unroll
+        for (int o = 0; o < kOut; ++o) {
+            const bool v = V[i * kOut + o] != 0;
+            const float d = v ? y[o] - T[i * kOut + o] : 0.f;
+            lsum += d * d * inv_count;
+            dy[o] = 2.f * d * inv_count;
+            atomicAdd(&gl[oB3 + o], dy[o]);
+        }
+        // layer 3 and back into h2
+        float dh2[kH];
+        for (int a = 0; a < kH; ++a) {
+            float s = 0.f;
+#pragma unroll
+            for (int o = 0; o < kOut; ++o) {
+                if (dy[o] != 0.f) atomicAdd(&gl[oW3 + a * kOut + o], h2[a] * dy[o]);
+                s += dy[o] * w[oW3 + a * kOut + o];
+            }
+            dh2[a] = h2[a] > 0.f ? s : 0.f;
+        }
+        // layer 2 and back into h1
+        float dh1[kH];
+        for (int a = 0; a < kH; ++a) dh1[a] = 0.f;
+        for (int j = 0; j < kH; ++j) {
+            if (dh2[j] == 0.f) continue;
+            atomicAdd(&gl[oB2 + j], dh2[j]);
+            for (int a = 0; a < kH; ++a) {
+                if (h1[a] != 0.f) atomicAdd(&gl[oW2 + a * kH + j], h1[a] * dh2[j]);
+                dh1[a] += dh2[j] * w[oW2 + a * kH + j];
+            }
+        }
+        // layer 1
+        for (int j = 0; j < kH; ++j) {
+            const float d = h1[j] > 0.f ? dh1[j] : 0.f;
+            if (d == 0.f) continue;
+            atomicAdd(&gl[oB1 + j], d);
+#pragma unroll
+            for (int a = 0; a < kIn; ++a)
+                if (x[a] != 0.f) atomicAdd(&gl[oW1 + a * kH + j], x[a] * d);
+        }
+    }
+    __syncthreads();
+    for (int x = threadIdx.x; x < kTrainable; x += blockDim.x)
+        if (gl[x] != 0.f) atomicAdd(&g[x], gl[x]);
+    lsum = dev::wave_sum(lsum);
+    if ((threadIdx.x & 63) == 0 && lsum != 0.f) atomicAdd(loss, lsum);
+}
+
+__global__ void k_adam(int n, float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m1,
+                       float *__restrict__ m2, float lr, float b1, float b2, float eps, float c1, float c2) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float gi = g[i];
+        const float a = b1 * m1[i] + (1.f - b1) * gi;
+        const float b = b2 * m2[i] + (1.f - b2) * gi * gi;
+        m1[i] = a;
+        m2[i] = b;
+        p[i] -= lr * (a / c1) / (sqrtf(b / c2) + eps);
+    }
+}
+
+// Label aggregation: CU-cycles of every row from the durations of its tasks.
+__global__ void k_row_cycles(int32_t lo, int32_t hi, const Task *__restrict__ tasks,
+                             const long long *__restrict__ cyc, float per_cu, double *__restrict__ row) {
+    for (int32_t t = lo + blockIdx.x * blockDim.x + threadIdx.x; t < hi; t += gridDim.x * blockDim.x)
+        atomicAdd(&row[tasks[t].row], (double)cyc[t] / per_cu);
+}
+
+// ---- engine hook -----------------------------------------------------------
+
+int model_select(gk_ctx *ctx, RowStats &st, int64_t m, int64_t ncols, int same_am) {
+    (void)ncols;
+    if (!ctx->model || !ctx->model->ready) {
+        set_error("learned selector requested but no model is loaded (gk_model_set / gk_model_train)");
+        return GK_ESTATE;
+    }
+    if (m == 0) return GK_OK;
+    const int g = (int)std::max<int64_t>(1, std::min<int64_t>((m + 255) / 256, (int64_t)ctx->num_cus * 4));
+    k_mlp_select<<<g, 256, 0, ctx->stream>>>(m, st, same_am, ctx->model->params);
+    return launch_check("k_mlp_select");
+}
+
 }  // namespace gk
+
+// ---- C ABI -------------------------------------------------------------------
+
+extern "C" int gk_model_num_params(void) { return gk::kParams; }
+
+extern "C" int gk_model_set(gk_ctx *ctx, const float *params, int n) {
+    GK_CHECK_ARG(ctx && params, "null argument");
+    GK_CHECK_ARG(n == gk::kParams, "model has %d parameters, got %d", gk::kParams, n);
+    for (int a = 0; a < gk::kIn; ++a)
+        GK_CHECK_ARG(params[gk::oStd + a] > 0.f, "feature scale %d must be positive", a);
+    Model *md;
+    GK_TRY(model_get(ctx, &md));
+    GK_HIP(hipMemcpyAsync(md->params, params, sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
+    GK_HIP(hipStreamSynchronize(ctx->stream));
+    md->ready = true;
+    return GK_OK;
+}
+
+extern "C" int gk_model_get(gk_ctx *ctx, float *params, int n) {
+    GK_CHECK_ARG(ctx && params, "null argument");
+    GK_CHECK_ARG(n == gk::kParams, "model has %d parameters, got %d", gk::kParams, n);
+    GK_CHECK_ARG(ctx->model && ctx->model->ready, "no model loaded");
+    GK_HIP(d2h(ctx, params, ctx->model->params, sizeof(float) * n));
+    return GK_OK;
+}
+
+extern "C" int gk_model_clear(gk_ctx *ctx) {
+    GK_CHECK_ARG(ctx, "null argument");
+    if (ctx->model) ctx->model->ready = false;
+    return GK_OK;
+}
+
+namespace {
+struct DevBuf {
+    std::vector<void *> p;
+    ~DevBuf() { for (void *x : p) (void)hipFree(x); }
+    template <class T>
+    int alloc(T **out, size_t n) {
+        if (hipMalloc((void **)out, sizeof(T) * (n ? n : 1)) != hipSuccess) {
+            set_error("hipMalloc(%zu) failed", sizeof(T) * n);
+            return GK_ENOMEM;
+        }
+        p.push_back(*out);
+        return GK_OK;
+    }
+};
+}  // namespace
+
+extern "C" int gk_model_forward(gk_ctx *ctx, const float *X, int64_t n, float *Y) {
+    GK_CHECK_ARG(ctx && (n == 0 || (X && Y)), "null argument");
+    GK_CHECK_ARG(ctx->model && ctx->model->ready, "no model loaded");
+    DevBuf b;
+    float *dX, *dY;
+    GK_TRY(b.alloc(&dX, n * gk::kIn));
+    GK_TRY(b.alloc(&dY, n * gk::kOut));
+    GK_HIP(hipMemcpyAsync(dX, X, sizeof(float) * n * gk::kIn, hipMemcpyHostToDevice, ctx->stream));
+    const int g = (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, (int64_t)ctx->num_cus * 4));
+    if (n) k_mlp_forward<<<g, 256, 0, ctx->stream>>>(n, dX, ctx->model->params, dY);
+    GK_TRY(launch_check("k_mlp_forward"));
+    GK_HIP(d2h(ctx, Y, dY, sizeof(float) * n * gk::kOut));
+    return GK_OK;
+}
+
+// Loss and gradient of the current model on (X, T, valid): for gradient
+// checks against an independent implementation.
+extern "C" int gk_model_grad(gk_ctx *ctx, const float *X, const float *T, const uint8_t *valid, int64_t n, float *grad,
+                             float *loss) {
+    GK_CHECK_ARG(ctx && X && T && valid && grad && loss && n > 0, "null argument");
+    GK_CHECK_ARG(ctx->model && ctx->model->ready, "no model loaded");
+    DevBuf b;
+    float *dX, *dT, *dg, *dl;
+    uint8_t *dV;
+    GK_TRY(b.alloc(&dX, n * gk::kIn));
+    GK_TRY(b.alloc(&dT, n * gk::kOut));
+    GK_TRY(b.alloc(&dV, n * gk::kOut));
+    GK_TRY(b.alloc(&dg, gk::kTrainable));
+    GK_TRY(b.alloc(&dl, 1));
+    GK_HIP(hipMemcpyAsync(dX, X, sizeof(float) * n * gk::kIn, hipMemcpyHostToDevice, ctx->stream));
+    GK_HIP(hipMemcpyAsync(dT, T, sizeof(float) * n * gk::kOut, hipMemcpyHostToDevice, ctx->stream));
+    GK_HIP(hipMemcpyAsync(dV, valid, n * gk::kOut, hipMemcpyHostToDevice, ctx->stream));
+    GK_HIP(hipMemsetAsync(dg, 0, sizeof(float) * gk::kTrainable, ctx->stream));
+    GK_HIP(hipMemsetAsync(dl, 0, sizeof(float), ctx->stream));
+    int64_t cnt = 0;
+    for (int64_t i = 0; i < n * gk::kOut; ++i) cnt += valid[i] != 0;
+    const float inv = cnt ? 1.f / (float)cnt : 0.f;
+    const int g = (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, (int64_t)ctx->num_cus * 2));
+    k_mlp_grad<<<g, 256, 0, ctx->stream>>>(n, dX, dT, dV, nullptr, inv, ctx->model->params, dg, dl);
+    GK_TRY(launch_check("k_mlp_grad"));
+    GK_HIP(d2h(ctx, grad, dg, sizeof(float) * gk::kTrainable));
+    GK_HIP(d2h(ctx, loss, dl, sizeof(float)));
+    return GK_OK;
+}
+
+// Train from scratch (He-normal initialisation from `seed`) with Adam on
+// mini-batches; feature mean/std are taken from X.  Returns the last epoch's
+// mean loss.
+extern "C" int gk_model_train(gk_ctx *ctx, const float *X, const float *T, const uint8_t *valid, int64_t n,
+                              int epochs, int batch, float lr, uint64_t seed, float *final_loss) {
+    GK_CHECK_ARG(ctx && X && T && valid && n > 0 && epochs > 0 && batch > 0, "bad arguments");
+    std::vector<float> p(gk::kParams, 0.f);
+    // feature standardisation
+    for (int a = 0; a < gk::kIn; ++a) {
+        double s = 0, s2 = 0;
+        for (int64_t i = 0; i < n; ++i) {
+            s += X[i * gk::kIn + a];
+            s2 += (double)X[i * gk::kIn + a] * X[i * gk::kIn + a];
+        }
+        const double mean = s / n, var = std::max(0.0, s2 / n - mean * mean);
+        p[gk::oMean + a] = (float)mean;
+        p[gk::oStd + a] = var > 1e-12 ? (float)std::sqrt(var) : 1.f;
+    }
+    // He-normal weights from a counter-based generator; biases 0
+    auto normal = [&](uint64_t k) {
+        const double u1 = std::max(u01(rng(seed, 7, 2 * k)), 1e-300), u2 = u01(rng(seed, 7, 2 * k + 1));
+        return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+    };
+    uint64_t ctr = 0;
+    for (int i = 0; i < gk::kIn * gk::kH; ++i) p[gk::oW1 + i] = (float)(normal(ctr++) * std::sqrt(2.0 / gk::kIn));
+    for (int i = 0; i < gk::kH * gk::kH; ++i) p[gk::oW2 + i] = (float)(normal(ctr++) * std::sqrt(2.0 / gk::kH));
+    for (int i = 0; i < gk::kH * gk::kOut; ++i) p[gk::oW3 + i] = (float)(normal(ctr++) * std::sqrt(1.0 / gk::kH));
+    // output biases start at the mean label of each output
+    for (int o = 0; o < gk::kOut; ++o) {
+        double s = 0;
+        int64_t c = 0;
+        for (int64_t i = 0; i < n; ++i)
+            if (valid[i * gk::kOut + o]) { s += T[i * gk::kOut + o]; ++c; }
+        p[gk::oB3 + o] = c ? (float)(s / c) : 0.f;
+    }
+    GK_TRY(gk_model_set(ctx, p.data(), gk::kParams));
+    float *params = ctx->model->params;
+    DevBuf b;
+    float *dX, *dT, *dg, *dl, *m1, *m2;
+    uint8_t *dV;
+    int64_t *didx;
+    GK_TRY(b.alloc(&dX, n * gk::kIn));
+    GK_TRY(b.alloc(&dT, n * gk::kOut));
+    GK_TRY(b.alloc(&dV, n * gk::kOut));
+    GK_TRY(b.alloc(&dg, gk::kTrainable));
+    GK_TRY(b.alloc(&dl, 1));
+    GK_TRY(b.alloc(&m1, gk::kTrainable));
+    GK_TRY(b.alloc(&m2, gk::kTrainable));
+    GK_TRY(b.alloc(&didx, n));
+    GK_HIP(hipMemcpyAsync(dX, X, sizeof(float) * n * gk::kIn, hipMemcpyHostToDevice, ctx->stream));
+    GK_HIP(hipMemcpyAsync(dT, T, sizeof(float) * n * gk::kOut, hipMemcpyHostToDevice, ctx->stream));
+    GK_HIP(hipMemcpyAsync(dV, valid, n * gk::kOut, hipMemcpyHostToDevice, ctx->stream));
+    GK_HIP(hipMemsetAsync(m1, 0, sizeof(float) * gk::kTrainable, ctx->stream));
+    GK_HIP(hipMemsetAsync(m2, 0, sizeof(float) * gk::kTrainable, ctx->stream));
+    std::vector<int64_t> perm(n);
+    std::vector<float> hl(1);
+    float last = 0.f;
+    int64_t step = 0;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    for (int ep = 0; ep < epochs; ++ep) {
+        // shuffle (Fisher-Yates from the counter-based generator)
+        for (int64_t i = 0; i < n; ++i) perm[i] = i;
+        for (int64_t i = n - 1; i > 0; --i) {
+            const int64_t j = (int64_t)(rng(seed, 11 + ep, i) % (uint64_t)(i + 1));
+            std::swap(perm[i], perm[j]);
+        }
+        GK_HIP(hipMemcpyAsync(didx, perm.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, ctx->stream));
+        GK_HIP(hipMemsetAsync(dl, 0, sizeof(float), ctx->stream));
+        for (int64_t b0 = 0; b0 < n; b0 += batch) {
+            const int64_t nb = std::min<int64_t>(batch, n - b0);
+            int64_t cnt = 0;
+            for (int64_t r = b0; r < b0 + nb; ++r)
+                for (int o = 0; o < gk::kOut; ++o) cnt += valid[perm[r] * gk::kOut + o] != 0;
+            if (!cnt) continue;
+            GK_HIP(hipMemsetAsync(dg, 0, sizeof(float) * gk::kTrainable, ctx->stream));
+            const int g = (int)std::max<int64_t>(1, std::min<int64_t>((nb + 255) / 256, (int64_t)ctx->num_cus));
+            // the loss of the epoch is accumulated with weight nb/n per batch
+            k_mlp_grad<<<g, 256, 0, ctx->stream>>>(nb, dX, dT, dV, didx + b0, 1.f / (float)cnt, params, dg, dl);
+            ++step;
+            const float c1 = 1.f - std::pow(b1, (float)step), c2 = 1.f - std::pow(b2, (float)step);
+            k_adam<<<(gk::kTrainable + 255) / 256, 256, 0, ctx->stream>>>(gk::kTrainable, params, dg, m1, m2, lr, b1,
+                                                                         b2, eps, c1, c2);
+        }
+        GK_TRY(launch_check("model training"));
+        GK_HIP(d2h(ctx, hl.data(), dl, sizeof(float)));
+        last = hl[0] / (float)((n + batch - 1) / batch);
+    }
+    if (final_loss) *final_loss = last;
+    return GK_OK;
+}
+
+// Features (row statistics) and cost labels of every row of C<M> = A*B for
+// every accumulator.  X: m x 16; T: m x 4 log2 CU-cycles; valid: m x 4
+// (0 where the row has no work or the accumulator cannot hold its mask).
+extern "C" int gk_selector_samples(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcsr *M, int mode,
+                                   float *X, float *T, uint8_t *valid) {
+    GK_CHECK_ARG(ctx && A && B && M && X && T && valid, "null argument");
+    GK_CHECK_ARG(mode == 0 || mode == 1, "mode must be 0 (triangle counting) or 1 (values)");
+    const int64_t m = A->m;
+    const int same_am = A->rst == M->rst && A->col == M->col;
+    DevBuf b;
+    float *dX;
+    double *rowc;
+    int8_t *dacc;
+    long long *cyc = nullptr;
+    GK_TRY(b.alloc(&dX, m * gk::kIn));
+    GK_TRY(b.alloc(&rowc, m));
+    GK_TRY(b.alloc(&dacc, m));
+    // features from the row statistics
+    RowStats st;
+    GK_TRY(row_stats(ctx, A, B, M, &st));
+    if (m) {
+        const int g = (int)std::max<int64_t>(1, std::min<int64_t>((m + 255) / 256, (int64_t)ctx->num_cus * 4));
+        k_features<<<g, 256, 0, ctx->stream>>>(m, st, same_am, dX);
+        GK_TRY(launch_check("k_features"));
+    }
+    GK_HIP(d2h(ctx, X, dX, sizeof(float) * m * gk::kIn));
+    int32_t *d_out = nullptr;
+    unsigned long long *d_count = nullptr;
+    GK_TRY(b.alloc(&d_out, std::max<int64_t>(M->nz, 1)));
+    GK_TRY(b.alloc(&d_count, 1));
+    std::vector<double> hrow(m);
+    std::vector<int8_t> hacc(m);
+    // resident tasks per CU of each kernel class (its persistent grid)
+    const float per_cu[kNumClass] = {(float)kLdsSmallBlocksPerCu, (float)kMcaBlocksPerCu * 4.f,
+                                     (float)kGlobalBlocksPerCu, (float)kGlobalBlocksPerCu,
+                                     (float)kWindowBlocksLong};
+    for (int a = 0; a < GK_NUM_ACC; ++a) {
+        gk_opts o;
+        gk_opts_default(&o);
+        o.selector = GK_SEL_FORCED;
+        o.forced_accum = a;
+        // task durations: one slot per task (tasks <= A entries + rows)
+        if (!cyc) GK_TRY(b.alloc(&cyc, A->nz + m + 1));
+        GK_HIP(hipMemsetAsync(d_count, 0, 8, ctx->stream));
+        Plan plan;
+        ProductInfo pi;
+        GK_TRY(masked_product(ctx, A, B, M, mode == 0 ? MODE_TC : MODE_VALUES, d_out, d_count, nullptr, o, &pi, cyc,
+                              &plan));
+        // which class every row really ran in (capacity fallbacks)
+        GK_HIP(d2h(ctx, hacc.data(), st.acc, m));
+        GK_HIP(hipMemsetAsync(rowc, 0, sizeof(double) * m, ctx->stream));
+        int32_t off[kNumBins + 1];
+        GK_HIP(d2h(ctx, off, plan.bin_off, sizeof off));
+        for (int c = 0; c < kNumClass; ++c) {
+            const int32_t lo = off[c * kFlopBins], hi = off[(c + 1) * kFlopBins];
+            if (hi > lo)
+                k_row_cycles<<<std::min(1024, (hi - lo + 255) / 256), 256, 0, ctx->stream>>>(lo, hi, plan.tasks, cyc,
+                                                                                          per_cu[c], rowc);
+        }
+        GK_TRY(launch_check("k_row_cycles"));
+        GK_HIP(d2h(ctx, hrow.data(), rowc, sizeof(double) * m));
+        for (int64_t i = 0; i < m; ++i) {
+            int c = hacc[i];
+            const int ran = c == kClassMsaLarge ? (int)GK_ACC_MSA_LDS : c;
+            const bool ok = c >= 0 && ran == a && hrow[i] > 0;
+            valid[i * GK_NUM_ACC + a] = ok;
+            T[i * GK_NUM_ACC + a] = ok ? (float)std::log2(hrow[i]) : 0.f;
+        }
+    }
+    return GK_OK;
+}

@@ -52,6 +52,7 @@ struct SelectParams {
     int selector, forced;
     int32_t mca_max_mask, lds_max_mask, dense_min_mask;
     int32_t split_flop;
+    int32_t small_max_mask;
 };
 
 struct Task {
@@ -97,9 +98,9 @@ int zero_u32(gk_ctx *ctx, void *p, int64_t n);
 // TC totals (zeroed by the caller).
 int masked_product(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcsr *M, int mode, int32_t *d_out,
                    unsigned long long *d_count, unsigned long long *d_rowcount, const gk_opts &o,
-                   ProductInfo *info);
+                   ProductInfo *info, long long *task_cycles = nullptr, Plan *plan_out = nullptr);
 
 // Learned selector (selector.hip): writes st.acc for every row.
-int model_select(gk_ctx *ctx, RowStats &st, int64_t m, int64_t ncols);
+int model_select(gk_ctx *ctx, RowStats &st, int64_t m, int64_t ncols, int same_am);
 
 }  // namespace gk

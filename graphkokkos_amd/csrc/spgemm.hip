@@ -127,7 +127,7 @@ __global__ __launch_bounds__(256) void k_select_bin(int64_t m, RowStats st, Sele
                 if (acc == GK_ACC_MCA_WAVE && !mca_fits(mlen)) acc = GK_ACC_MSA_LDS;
                 if (acc == GK_ACC_MSA_LDS && !lds_fits(mlen)) acc = GK_ACC_HASH_GLOBAL;
                 if (acc == GK_ACC_HASH_GLOBAL && mlen > kHashGlobalMaxMask) acc = GK_ACC_DENSE_GLOBAL;
-                if (acc == GK_ACC_MSA_LDS && mlen > kLdsSmallMask) acc = kClassMsaLarge;
+                if (acc == GK_ACC_MSA_LDS && mlen > sp.small_max_mask) acc = kClassMsaLarge;
                 nt = task_count(sp, alen, flop);
                 bin = acc * kFlopBins + (kFlopBins - 1 - flop_bin(flop));
             }
@@ -234,6 +234,7 @@ struct Prob {
     unsigned long long *__restrict__ count;     // TC: total
     unsigned long long *__restrict__ rowcount;  // TC: optional per-row counts
     unsigned long long *__restrict__ hits;      // hits per accumulator
+    long long *__restrict__ task_cycles;        // optional: clock64 duration of every task
     const Task *__restrict__ tasks;
     const int32_t *__restrict__ bin_off;
 };
@@ -403,6 +404,7 @@ __global__ __launch_bounds__(64) void k_msa_lds(Prob P, int cls) {
     uint32_t cnt = 0;
     for (int32_t ti = lo + blockIdx.x; ti < hi; ti += gridDim.x) {
         const Task tk = P.tasks[ti];
+        const long long tstart = P.task_cycles ? clock64() : 0;
         const int32_t mb = P.mrst[tk.row], mlen = P.mrst[tk.row + 1] - mb;
         int lg = 6;
         while ((1 << lg) < 4 * mlen && (1 << lg) < HC) ++lg;
@@ -438,6 +440,7 @@ __global__ __launch_bounds__(64) void k_msa_lds(Prob P, int cls) {
             if (lane == 0 && r) atomicAdd(&P.rowcount[tk.row], (unsigned long long)r);
         }
         cnt += c;
+        if (P.task_cycles && threadIdx.x == 0) P.task_cycles[ti] = clock64() - tstart;
         __syncthreads();
     }
     finish_kernel<MODE>(P, cnt, cls, lane);
@@ -508,6 +511,7 @@ __global__ __launch_bounds__(64 * NW) void k_msa_window(Prob P, int cls) {
     uint32_t cnt = 0;
     for (int32_t ti = lo + blockIdx.x; ti < hi; ti += gridDim.x) {
         const Task tk = P.tasks[ti];
+        const long long tstart = P.task_cycles ? clock64() : 0;
         const int32_t mb = P.mrst[tk.row], mlen = P.mrst[tk.row + 1] - mb;
         // mask columns are sorted: the ones >= kWindow form the row's tail
         int32_t l = 0, h = mlen;
@@ -570,6 +574,7 @@ __global__ __launch_bounds__(64 * NW) void k_msa_window(Prob P, int cls) {
             if ((tid & 63) == 0 && r) atomicAdd(&P.rowcount[tk.row], (unsigned long long)r);
         }
         cnt += c;
+        if (P.task_cycles && threadIdx.x == 0) P.task_cycles[ti] = clock64() - tstart;
         __syncthreads();
     }
     finish_kernel<MODE>(P, cnt, cls, tid & 63);
@@ -601,6 +606,7 @@ __global__ __launch_bounds__(64) void k_mca_wave(Prob P, int cls) {
         const bool live = ti < hi;
         Task tk{0, 0, 0, 0};
         int32_t mb = 0, mlen = 0;
+        const long long tstart = P.task_cycles ? clock64() : 0;
         if (live) {
             tk = P.tasks[ti];
             mb = P.mrst[tk.row];
@@ -655,6 +661,7 @@ __global__ __launch_bounds__(64) void k_mca_wave(Prob P, int cls) {
                 if (gl == 0 && s) atomicAdd(&P.rowcount[tk.row], (unsigned long long)s);
             }
             cnt += cnt_task;
+            if (P.task_cycles && gl == 0) P.task_cycles[ti] = clock64() - tstart;
         }
         __syncthreads();
     }
@@ -680,6 +687,7 @@ __global__ __launch_bounds__(64) void k_hash_global(Prob P, int cls, int32_t *__
     uint32_t cnt = 0;
     for (int32_t ti = lo + blockIdx.x; ti < hi; ti += gridDim.x) {
         const Task tk = P.tasks[ti];
+        const long long tstart = P.task_cycles ? clock64() : 0;
         const int32_t mb = P.mrst[tk.row], mlen = P.mrst[tk.row + 1] - mb;
         int lg = 6;
         while ((1 << lg) < 2 * mlen && lg < lgcap) ++lg;
@@ -718,6 +726,7 @@ __global__ __launch_bounds__(64) void k_hash_global(Prob P, int cls, int32_t *__
             if (lane == 0 && r) atomicAdd(&P.rowcount[tk.row], (unsigned long long)r);
         }
         cnt += c;
+        if (P.task_cycles && threadIdx.x == 0) P.task_cycles[ti] = clock64() - tstart;
         __threadfence();
         __syncthreads();
     }
@@ -742,6 +751,7 @@ __global__ __launch_bounds__(64) void k_dense_global(Prob P, int cls, int32_t *_
     uint32_t cnt = 0;
     for (int32_t ti = lo + blockIdx.x; ti < hi; ti += gridDim.x) {
         const Task tk = P.tasks[ti];
+        const long long tstart = P.task_cycles ? clock64() : 0;
         const int32_t mb = P.mrst[tk.row], mlen = P.mrst[tk.row + 1] - mb;
         for (int32_t q = lane; q < mlen; q += 64) {
             map[P.mcol[mb + q]] = q + 1;
@@ -763,6 +773,7 @@ __global__ __launch_bounds__(64) void k_dense_global(Prob P, int cls, int32_t *_
             if (lane == 0 && r) atomicAdd(&P.rowcount[tk.row], (unsigned long long)r);
         }
         cnt += c;
+        if (P.task_cycles && threadIdx.x == 0) P.task_cycles[ti] = clock64() - tstart;
         __threadfence();
         __syncthreads();
     }
@@ -885,7 +896,10 @@ static int dense_maps(gk_ctx *ctx, int64_t blocks, int64_t ncols, int32_t **out)
 // back before the caller continues), so the classes share the GPU instead
 // of running one after the other.
 template <int MODE>
-static int run_mode(gk_ctx *ctx, const Prob &P, const Plan &plan, int64_t bn, double *acc_ms) {
+static int run_mode(gk_ctx *ctx, const Prob &P, const Plan &plan, int64_t bn, const gk_opts &o, double *acc_ms) {
+    auto per_cu = [](int32_t v, int dflt) { return v > 0 ? std::min<int32_t>(v, 64) : dflt; };
+    const int b_small = per_cu(o.blocks_small, kLdsSmallBlocksPerCu), b_long = per_cu(o.blocks_long, kWindowBlocksLong);
+    const int b_mca = per_cu(o.blocks_mca, kMcaBlocksPerCu), b_glob = per_cu(o.blocks_global, kGlobalBlocksPerCu);
     static_assert(kNumClass <= gk_ctx::kAux + 1, "one stream per kernel class");
     hipStream_t sc[kNumClass];
     // the context stream takes dense-global, whose scratch setup runs there
@@ -904,7 +918,7 @@ static int run_mode(gk_ctx *ctx, const Prob &P, const Plan &plan, int64_t bn, do
     int32_t dmm = 0;
     if (plan.acc_tasks[GK_ACC_HASH_GLOBAL]) {
         while ((1 << hlg) < 2 * plan.acc_maxmask[GK_ACC_HASH_GLOBAL]) ++hlg;
-        hg = grid_for(ctx, plan.acc_tasks[GK_ACC_HASH_GLOBAL], kGlobalBlocksPerCu);
+        hg = grid_for(ctx, plan.acc_tasks[GK_ACC_HASH_GLOBAL], b_glob);
         size_t cap = (size_t)1 << hlg;
         // Bound the tables to about 4 GiB.
         while (hg > 1 && (size_t)hg * cap * 10 > ((size_t)4 << 30)) hg >>= 1;
@@ -915,7 +929,7 @@ static int run_mode(gk_ctx *ctx, const Prob &P, const Plan &plan, int64_t bn, do
         hacc = hpos + (size_t)hg * cap;
     }
     if (plan.acc_tasks[GK_ACC_DENSE_GLOBAL]) {
-        dg = grid_for(ctx, plan.acc_tasks[GK_ACC_DENSE_GLOBAL], kGlobalBlocksPerCu);
+        dg = grid_for(ctx, plan.acc_tasks[GK_ACC_DENSE_GLOBAL], b_glob);
         // Bound the map memory to about 4 GiB.
         while (dg > 1 && (size_t)dg * (size_t)bn * 4 > ((size_t)4 << 30)) dg >>= 1;
         dmm = plan.acc_maxmask[GK_ACC_DENSE_GLOBAL];
@@ -927,11 +941,11 @@ static int run_mode(gk_ctx *ctx, const Prob &P, const Plan &plan, int64_t bn, do
     GK_HIP(hipEventRecord(ctx->fork, ctx->stream));
     for (int i = 0; i < gk_ctx::kAux; ++i) GK_HIP(hipStreamWaitEvent(ctx->aux[i], ctx->fork, 0));
     if (plan.acc_tasks[kClassMsaLarge]) {
-        int g = grid_for(ctx, plan.acc_tasks[kClassMsaLarge], kLdsBlocksPerCu);
+        int g = grid_for(ctx, plan.acc_tasks[kClassMsaLarge], std::min(b_long, kLdsBlocksPerCu));
         tm[kClassMsaLarge].start();
         if (MODE == MODE_TC || MODE == MODE_SUPPORT)
             k_msa_window<MODE == MODE_VALUES ? MODE_TC : MODE, kWindowWavesLong, kWindowHashLong, kLdsMaxMask>
-                <<<grid_for(ctx, plan.acc_tasks[kClassMsaLarge], kWindowBlocksLong), 64 * kWindowWavesLong, 0,
+                <<<grid_for(ctx, plan.acc_tasks[kClassMsaLarge], b_long), 64 * kWindowWavesLong, 0,
                    sc[kClassMsaLarge]>>>(P, kClassMsaLarge);
         else
             k_msa_lds<MODE, kLdsHashCap, kLdsMaxMask><<<g, 64, 0, sc[kClassMsaLarge]>>>(P, kClassMsaLarge);
@@ -940,7 +954,7 @@ static int run_mode(gk_ctx *ctx, const Prob &P, const Plan &plan, int64_t bn, do
         GK_TRY(launch_check("k_msa_lds<large>"));
     }
     if (plan.acc_tasks[GK_ACC_MSA_LDS]) {
-        int g = grid_for(ctx, plan.acc_tasks[GK_ACC_MSA_LDS], kLdsSmallBlocksPerCu);
+        int g = grid_for(ctx, plan.acc_tasks[GK_ACC_MSA_LDS], b_small);
         tm[GK_ACC_MSA_LDS].start();
         if (MODE == MODE_TC || MODE == MODE_SUPPORT)
             k_msa_window<MODE == MODE_VALUES ? MODE_TC : MODE, 1, kLdsSmallCap, kLdsSmallMask>
@@ -952,7 +966,7 @@ static int run_mode(gk_ctx *ctx, const Prob &P, const Plan &plan, int64_t bn, do
         GK_TRY(launch_check("k_msa_lds<small>"));
     }
     if (plan.acc_tasks[GK_ACC_MCA_WAVE]) {
-        int g = grid_for(ctx, (plan.acc_tasks[GK_ACC_MCA_WAVE] + 3) / 4, kMcaBlocksPerCu);
+        int g = grid_for(ctx, (plan.acc_tasks[GK_ACC_MCA_WAVE] + 3) / 4, b_mca);
         tm[GK_ACC_MCA_WAVE].start();
         k_mca_wave<MODE, 16><<<g, 64, 0, sc[GK_ACC_MCA_WAVE]>>>(P, GK_ACC_MCA_WAVE);
         tm[GK_ACC_MCA_WAVE].stop();
@@ -989,12 +1003,14 @@ SelectParams select_params(const gk_opts &o) {
     sp.lds_max_mask = o.lds_max_mask;
     sp.dense_min_mask = o.dense_min_mask;
     sp.split_flop = o.split_flop > 0 ? o.split_flop : (1 << 18);
+    sp.small_max_mask = std::max(1, std::min<int32_t>(o.small_max_mask > 0 ? o.small_max_mask : kLdsSmallMask,
+                                                      kLdsSmallMask));
     return sp;
 }
 
 int masked_product(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcsr *M, int mode, int32_t *d_out,
                    unsigned long long *d_count, unsigned long long *d_rowcount, const gk_opts &o,
-                   ProductInfo *info) {
+                   ProductInfo *info, long long *task_cycles, Plan *plan_out) {
     if (A->n != B->m || A->m != M->m || B->n != M->n) {
         set_error("shape mismatch: A %lldx%lld, B %lldx%lld, M %lldx%lld", (long long)A->m, (long long)A->n,
                   (long long)B->m, (long long)B->n, (long long)M->m, (long long)M->n);
@@ -1006,7 +1022,8 @@ int masked_product(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcs
     RowStats st;
     GK_TRY(row_stats(ctx, A, B, M, &st));
     SelectParams sp = select_params(o);
-    if (sp.selector == GK_SEL_LEARNED) GK_TRY(model_select(ctx, st, A->m, B->n));
+    if (sp.selector == GK_SEL_LEARNED)
+        GK_TRY(model_select(ctx, st, A->m, B->n, A->rst == M->rst && A->col == M->col));
     Plan plan;
     GK_TRY(plan_tasks(ctx, A, st, sp, &plan));
     sel.stop();
@@ -1016,12 +1033,14 @@ int masked_product(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcs
     P.brst = B->rst; P.bcol = B->col; P.bval = B->val;
     P.mrst = M->rst; P.mcol = M->col;
     P.cval = d_out; P.count = d_count; P.rowcount = d_rowcount; P.hits = plan.hits;
+    P.task_cycles = task_cycles;
     P.tasks = plan.tasks; P.bin_off = plan.bin_off;
     double acc_ms[kNumClass] = {};
-    if (mode == MODE_TC) GK_TRY(run_mode<MODE_TC>(ctx, P, plan, B->n, acc_ms));
-    else if (mode == MODE_VALUES) GK_TRY(run_mode<MODE_VALUES>(ctx, P, plan, B->n, acc_ms));
-    else GK_TRY(run_mode<MODE_SUPPORT>(ctx, P, plan, B->n, acc_ms));
+    if (mode == MODE_TC) GK_TRY(run_mode<MODE_TC>(ctx, P, plan, B->n, o, acc_ms));
+    else if (mode == MODE_VALUES) GK_TRY(run_mode<MODE_VALUES>(ctx, P, plan, B->n, o, acc_ms));
+    else GK_TRY(run_mode<MODE_SUPPORT>(ctx, P, plan, B->n, o, acc_ms));
     total.stop();
+    if (plan_out) *plan_out = plan;
     if (info) {
         unsigned long long hs[kNumClass * (ST_NUM + 1)];
         GK_HIP(d2h(ctx, hs, plan.stats, sizeof hs));

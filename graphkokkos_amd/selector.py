@@ -1,0 +1,162 @@
+"""Learned accumulator selector and autotuner (host-side orchestration).
+
+The network, its training and its inference run in libgraphhip.so
+(selector.hip): a 16 -> 64 -> 64 -> 4 FP32 MLP over per-row features that
+predicts each accumulator's cost of a row in log2 CU-cycles.  This module
+collects training samples from the product's own task timers, trains, saves
+and loads the parameters, and runs the autotuner over the product's global
+knobs (bin thresholds, tile widths, persistent grid sizes).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import itertools
+import time
+from pathlib import Path
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, default_opts, lib
+from .algorithms import Context, DeviceCsr
+from .graph import SpMtx
+
+N_IN, N_HIDDEN, N_OUT = 16, 64, 4
+
+
+def num_params() -> int:
+    return int(lib().gk_model_num_params())
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, np.float32)
+
+
+def samples(ctx: Context, A: DeviceCsr, B: DeviceCsr | None = None, M: DeviceCsr | None = None,
+            mode: int = 0) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """Features X (m x 16), labels T (m x 4, log2 CU-cycles) and validity
+    (m x 4) of every row of C<M> = A*B; mode 0 = triangle counting."""
+    B = B or A
+    M = M or A
+    m = A.m
+    X = np.zeros((m, N_IN), np.float32)
+    T = np.zeros((m, N_OUT), np.float32)
+    V = np.zeros((m, N_OUT), np.uint8)
+    check(lib().gk_selector_samples(ctx.handle, A.handle, B.handle, M.handle, mode,
+                                    X.ctypes.data_as(C.POINTER(C.c_float)), T.ctypes.data_as(C.POINTER(C.c_float)),
+                                    V.ctypes.data_as(C.POINTER(C.c_uint8))), "gk_selector_samples")
+    keep = V.any(axis=1)
+    return X[keep], T[keep], V[keep]
+
+
+def train(ctx: Context, X, T, V, epochs: int = 40, batch: int = 4096, lr: float = 2e-3, seed: int = 1) -> float:
+    X, T = _f32(X), _f32(T)
+    V = np.ascontiguousarray(V, np.uint8)
+    loss = C.c_float(0)
+    check(lib().gk_model_train(ctx.handle, X.ctypes.data_as(C.POINTER(C.c_float)),
+                               T.ctypes.data_as(C.POINTER(C.c_float)), V.ctypes.data_as(C.POINTER(C.c_uint8)),
+                               len(X), epochs, batch, lr, seed, C.byref(loss)), "gk_model_train")
+    return float(loss.value)
+
+
+def get_params(ctx: Context) -> np.ndarray:
+    p = np.zeros(num_params(), np.float32)
+    check(lib().gk_model_get(ctx.handle, p.ctypes.data_as(C.POINTER(C.c_float)), len(p)), "gk_model_get")
+    return p
+
+
+def set_params(ctx: Context, p) -> None:
+    p = _f32(p)
+    check(lib().gk_model_set(ctx.handle, p.ctypes.data_as(C.POINTER(C.c_float)), len(p)), "gk_model_set")
+
+
+def forward(ctx: Context, X) -> np.ndarray:
+    X = _f32(X)
+    Y = np.zeros((len(X), N_OUT), np.float32)
+    check(lib().gk_model_forward(ctx.handle, X.ctypes.data_as(C.POINTER(C.c_float)), len(X),
+                                 Y.ctypes.data_as(C.POINTER(C.c_float))), "gk_model_forward")
+    return Y
+
+
+def grad(ctx: Context, X, T, V) -> tuple[np.ndarray, float]:
+    X, T = _f32(X), _f32(T)
+    V = np.ascontiguousarray(V, np.uint8)
+    g = np.zeros(num_params() - 2 * N_IN, np.float32)
+    loss = C.c_float(0)
+    check(lib().gk_model_grad(ctx.handle, X.ctypes.data_as(C.POINTER(C.c_float)),
+                              T.ctypes.data_as(C.POINTER(C.c_float)), V.ctypes.data_as(C.POINTER(C.c_uint8)),
+                              len(X), g.ctypes.data_as(C.POINTER(C.c_float)), C.byref(loss)), "gk_model_grad")
+    return g, float(loss.value)
+
+
+def save(path, params) -> None:
+    np.save(Path(path), _f32(params), allow_pickle=False)
+
+
+def load(path) -> np.ndarray:
+    return np.load(Path(path), allow_pickle=False)
+
+
+def train_from_graphs(ctx: Context, graphs, epochs: int = 40, seed: int = 1, mode: int = 0) -> dict:
+    """Sample every graph's L (triangle counting), train, return a report."""
+    Xs, Ts, Vs = [], [], []
+    t0 = time.perf_counter()
+    for g in graphs:
+        d = ctx.upload(g.L if mode == 0 else g.A)
+        X, T, V = samples(ctx, d, mode=mode)
+        Xs.append(X)
+        Ts.append(T)
+        Vs.append(V)
+        d.free()
+    X, T, V = np.concatenate(Xs), np.concatenate(Ts), np.concatenate(Vs)
+    t1 = time.perf_counter()
+    loss = train(ctx, X, T, V, epochs=epochs, seed=seed)
+    t2 = time.perf_counter()
+    return {"samples": int(len(X)), "sample_s": t1 - t0, "train_s": t2 - t1, "final_loss": loss}
+
+
+# ---- autotuner ----------------------------------------------------------------
+
+SEARCH_SPACE = {
+    "mca_max_mask": [0, 4, 8, 16, 32],
+    "small_max_mask": [32, 64, 96, 128],
+    "blocks_small": [12, 16, 24, 32],
+    "blocks_long": [4, 6, 8],
+    "blocks_mca": [4, 8, 16],
+    "split_flop": [1 << 16, 1 << 18, 1 << 20],
+}
+
+
+def time_tc(ctx: Context, L: DeviceCsr, opts, reps: int = 3) -> float:
+    ctx.tc_count(L, opts)  # warm-up
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.tc_count(L, opts)
+    ctx.sync()
+    return (time.perf_counter() - t0) / reps
+
+
+def autotune_tc(ctx: Context, L: DeviceCsr, base: dict | None = None, rounds: int = 2, reps: int = 3,
+                space: dict | None = None) -> tuple[dict, float, list]:
+    """Coordinate descent over the product's global knobs, timing triangle
+    counting on L; returns (best options, best seconds, trial log)."""
+    space = space or SEARCH_SPACE
+    best = dict(base or {})
+    best_t = time_tc(ctx, L, default_opts(**best), reps)
+    log = [(dict(best), best_t)]
+    for _ in range(rounds):
+        improved = False
+        for key, values in space.items():
+            for v in values:
+                if best.get(key, getattr(default_opts(), key)) == v:
+                    continue
+                cand = dict(best)
+                cand[key] = v
+                t = time_tc(ctx, L, default_opts(**cand), reps)
+                log.append((dict(cand), t))
+                if t < best_t * 0.98:
+                    best, best_t, improved = cand, t, True
+        if not improved:
+            break
+    return best, best_t, log

@@ -98,7 +98,12 @@ typedef struct gk_opts {
     int32_t split_flop;     /* rows above this many B entries are split   */
     int32_t block_threads;  /* workgroup size for the row kernels         */
     int32_t verbose;
-    int32_t reserved[7];
+    int32_t small_max_mask; /* MSA-LDS rows up to this (<= 128) use the short-row kernel */
+    int32_t blocks_small;   /* persistent workgroups per CU, short-row MSA-LDS kernel */
+    int32_t blocks_long;    /* ... long-row MSA-LDS kernel                           */
+    int32_t blocks_mca;     /* ... MCA-wave kernel                                   */
+    int32_t blocks_global;  /* ... hash-global and dense-global kernels              */
+    int32_t reserved[2];
 } gk_opts;
 void gk_opts_default(gk_opts *o);
 
@@ -210,6 +215,33 @@ int gk_bc_batch_f64(gk_ctx *ctx, const gk_csr *A, const int32_t *sources,
 int gk_bc_batch_dev(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *AT,
                     const int32_t *sources, int64_t nsrc, double *d_bc,
                     const gk_opts *o);
+
+/* ---- learned accumulator selector ----------------------------------------
+ * A 16 -> 64 -> 64 -> 4 FP32 MLP predicting each accumulator's cost (log2
+ * CU-cycles) of a row from 16 row features; GK_SEL_LEARNED picks the
+ * cheapest prediction per row.  No counterpart in GraphKokkos (its products
+ * take the accumulator from the caller: mxmm_msa vs mxmm_mca).
+ * Parameter vector: W1[16x64] b1[64] W2[64x64] b2[64] W3[64x4] b3[4]
+ * feature mean[16] feature std[16] (gk_model_num_params() floats).        */
+int gk_model_num_params(void);
+int gk_model_set(gk_ctx *ctx, const float *params, int n);
+int gk_model_get(gk_ctx *ctx, float *params, int n);
+int gk_model_clear(gk_ctx *ctx);
+/* Y[n x 4] = model(X[n x 16]) */
+int gk_model_forward(gk_ctx *ctx, const float *X, int64_t n, float *Y);
+/* loss = mean over valid outputs of (Y - T)^2; grad of W1..b3 (5508 floats) */
+int gk_model_grad(gk_ctx *ctx, const float *X, const float *T, const uint8_t *valid,
+                  int64_t n, float *grad, float *loss);
+/* Adam on mini-batches from a seeded initialisation; loads the result. */
+int gk_model_train(gk_ctx *ctx, const float *X, const float *T, const uint8_t *valid,
+                   int64_t n, int epochs, int batch, float lr, uint64_t seed,
+                   float *final_loss);
+/* Training samples of C<M> = A*B (mode 0: triangle counting, 1: values):
+ * X[m x 16] features, T[m x 4] measured log2 CU-cycles per accumulator,
+ * valid[m x 4]. */
+int gk_selector_samples(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B,
+                        const gk_dcsr *M, int mode, float *X, float *T,
+                        uint8_t *valid);
 
 /* ---- device buffers for the *_dev entry points ------------------------- */
 int  gk_dmalloc(gk_ctx *ctx, size_t bytes, void **dptr);

@@ -1,0 +1,87 @@
+"""The learned accumulator selector on the GPU: the network's forward pass
+and gradients against the FP64 oracle (oracle/mlp.py, pinned by finite
+differences), training, the timing samples the product produces, and the
+learned selector's results (bit-exact counts) and timing beside the
+fixed-threshold heuristic."""
+import numpy as np
+import pytest
+
+from graphkokkos_amd import SEL_LEARNED, HostGraph, selector
+from oracle import mlp
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def test_forward_matches_fp64_oracle(gpu):
+    rng = np.random.default_rng(1)
+    p = mlp.random_params(rng)
+    selector.set_params(gpu, p)
+    X = rng.normal(0, 2, (1000, 16))
+    want, _ = mlp.forward(p, X)
+    got = selector.forward(gpu, X)
+    np.testing.assert_allclose(got, want, rtol=1e-4, atol=1e-4 * np.abs(want).max())
+
+
+def test_gradient_matches_fp64_oracle(gpu):
+    """FP32 gradients accumulated with atomics: |g - g64| <= 1e-3 max|g64|."""
+    rng = np.random.default_rng(2)
+    p = mlp.random_params(rng)
+    selector.set_params(gpu, p)
+    X = rng.normal(0, 2, (3000, 16))
+    T = rng.normal(0, 1, (3000, 4))
+    V = (rng.random((3000, 4)) < 0.8).astype(np.uint8)
+    g, loss = selector.grad(gpu, X, T, V)
+    l64, g64 = mlp.loss_grad(p, X, T, V)
+    assert abs(loss - l64) <= 1e-4 * abs(l64)
+    err = np.abs(g - g64).max()
+    assert err <= 1e-3 * np.abs(g64).max(), err
+
+
+def test_training_fits_a_known_cost_function(gpu):
+    rng = np.random.default_rng(3)
+    X = rng.normal(0, 1, (20000, 16)).astype(np.float32)
+    W = rng.normal(0, 1, (16, 4))
+    T = np.tanh(X @ W) + 0.5 * X[:, :4] ** 2
+    V = np.ones_like(T, np.uint8)
+    loss0 = float(((T - T.mean(0)) ** 2).mean())
+    loss = selector.train(gpu, X, T, V, epochs=30, batch=1024, lr=3e-3, seed=4)
+    assert loss < 0.1 * loss0, (loss, loss0)
+    pred = selector.forward(gpu, X[:2000])
+    assert np.corrcoef(pred.ravel(), T[:2000].ravel())[0, 1] > 0.95
+
+
+def test_samples_learned_selection_and_gain(gpu):
+    train_graphs = [HostGraph.rmat(14, 16, seed=5), HostGraph.grid(128), HostGraph.chung_lu(20000, 2.1, 16, seed=5)]
+    rep = selector.train_from_graphs(gpu, train_graphs, epochs=20, seed=1)
+    assert rep["samples"] > 10000
+    print("selector training:", rep, flush=True)
+    G = HostGraph.rmat(16, 16, seed=9)
+    d = gpu.upload(G.L)
+    want = O.tc_sandia(G.L)
+    assert gpu.tc_count(d, {"selector": SEL_LEARNED}) == want
+    assert gpu.tc_count(d) == want
+    t_h = selector.time_tc(gpu, d, None, reps=5)
+    t_l = selector.time_tc(gpu, d, {"selector": SEL_LEARNED}, reps=5)
+    print(f"R-MAT s16 triangle counting: heuristic {t_h * 1e3:.3f} ms, learned {t_l * 1e3:.3f} ms, "
+          f"learned/heuristic speedup {t_h / t_l:.3f}", flush=True)
+    assert t_l < 3.0 * t_h
+
+
+def test_sample_labels_follow_capacities(gpu):
+    G = HostGraph.rmat(12, 16, seed=6)
+    d = gpu.upload(G.L)
+    X, T, V = selector.samples(gpu, d)
+    mlen = np.round(2 ** X[:, 0] - 1).astype(int)
+    assert not V[mlen > 64, 1].any()      # MCA-wave holds mask rows up to 64
+    assert V[:, 0].all()                  # MSA-LDS holds every row here
+    assert np.isfinite(T[V.astype(bool)]).all()
+
+
+def test_autotuner_returns_correct_configuration(gpu):
+    G = HostGraph.rmat(13, 16, seed=2)
+    d = gpu.upload(G.L)
+    space = {"mca_max_mask": [0, 16], "small_max_mask": [64, 128]}
+    best, t, log = selector.autotune_tc(gpu, d, rounds=1, reps=2, space=space)
+    assert len(log) >= 3 and t > 0
+    assert gpu.tc_count(d, best) == O.tc_sandia(G.L)
