@@ -188,16 +188,38 @@ def main() -> int:
                         (plain["timing"]["total_ms"] / 1e3) / HBM_PEAK_GBS}}
         log(f"tc plain: {plain['wall_s'] * 1e3:.2f} ms, T={plain['count']}, dominant {pd['kernel']} "
             f"{pd['achieved_gbs']:.0f} GB/s")
-        try:
-            learned = run_tc(ctx, G22.L, max(a.steps, 3), a.warmup, {"selector": SEL_LEARNED})
-            assert learned["count"] == plain["count"], "learned selector changed the triangle count"
-            ld = dominant(learned["timing"])
-            tc["learned"] = {"value": G22.edges / learned["wall_s"], "ms": learned["wall_s"] * 1e3,
-                             "roofline": roofline_obj(ld), "speedup_vs_plain": plain["wall_s"] / learned["wall_s"]}
-        except Exception as e:  # no trained model in this build
-            tc["learned"] = {"value": None, "note": f"learned selector unavailable: {str(e)[:160]}"}
-
-    bc = None
+        # learned selector: trained on timing samples the product generates
+        # on other seeded graphs, then the global knobs autotuned on a
+        # separate R-MAT scale-20 graph; both outside the timed region
+        from graphkokkos_amd import selector as SEL
+        t0 = time.perf_counter()
+        train_graphs = [HostGraph.rmat(18, 16, seed=a.seed + 1, **RMAT), HostGraph.grid(512),
+                        HostGraph.chung_lu(1 << 18, 2.0, 16.0, seed=a.seed + 1)]
+        rep = SEL.train_from_graphs(ctx, train_graphs, epochs=30, seed=a.seed)
+        del train_graphs
+        log(f"selector trained on {rep['samples']} rows, loss {rep['final_loss']:.3f}, "
+            f"{time.perf_counter() - t0:.1f}s")
+        Gt = HostGraph.rmat(20, 16, seed=a.seed + 2, **RMAT)
+        dtune = ctx.upload(Gt.L)
+        tuned_h, _, tlog_h = SEL.autotune_tc(ctx, dtune, rounds=2, reps=3)
+        tuned_l, _, tlog_l = SEL.autotune_tc(ctx, dtune, base={"selector": SEL_LEARNED}, rounds=2, reps=3)
+        dtune.free()
+        del Gt
+        log(f"autotuned: heuristic {tuned_h}, learned {tuned_l} ({len(tlog_h) + len(tlog_l)} trials)")
+        res = {}
+        for name, opts in (("tuned_heuristic", tuned_h), ("learned", {"selector": SEL_LEARNED}),
+                           ("learned_tuned", tuned_l)):
+            r = run_tc(ctx, G22.L, max(a.steps, 3), a.warmup, opts)
+            assert r["count"] == plain["count"], f"{name} changed the triangle count"
+            d = dominant(r["timing"])
+            res[name] = {"value": G22.edges / r["wall_s"], "ms": r["wall_s"] * 1e3, "opts": opts,
+                         "speedup_vs_plain": plain["wall_s"] / r["wall_s"], "roofline": roofline_obj(d),
+                         "frac_all_kernels": (r["timing"]["bytes_alg"] / 1e9) / (r["wall_s"]) / HBM_PEAK_GBS,
+                         "rows_per_acc": r["timing"]["rows_per_acc"]}
+            log(f"tc {name}: {r['wall_s'] * 1e3:.2f} ms ({plain['wall_s'] / r['wall_s']:.3f}x plain)")
+        tc["plain"]["rows_per_acc"] = plain["timing"]["rows_per_acc"]
+        tc.update(res)
+        tc["selector_training"] = rep
     sources = None
     if "bc" in legs:
         rng = np.random.default_rng(a.seed)
