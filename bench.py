@@ -220,6 +220,7 @@ def main() -> int:
         tc["plain"]["rows_per_acc"] = plain["timing"]["rows_per_acc"]
         tc.update(res)
         tc["selector_training"] = rep
+    bc = None
     sources = None
     if "bc" in legs:
         rng = np.random.default_rng(a.seed)
