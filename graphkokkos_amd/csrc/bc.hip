@@ -35,6 +35,9 @@ int exclusive_scan_i32(gk_ctx *ctx, const int32_t *in, int32_t *out, int64_t n);
 namespace {
 
 constexpr int kSrc = 256;
+// vertices with this many neighbours get a workgroup of kHeavyWaves waves
+constexpr int kHeavy = 1024;
+constexpr int kHeavyWaves = 16;
 
 __device__ __forceinline__ uint32_t nib_of(const uint64_t *row, int lane) {
     return (uint32_t)(row[lane >> 4] >> ((lane & 15) * 4)) & 0xFu;
@@ -68,8 +71,80 @@ __global__ void k_bc_sources(const int32_t *__restrict__ src, int nsrc, uint64_t
     }
 }
 
-// Level d -> d+1.  cnt accumulates the number of (vertex, source) pairs
-// reached.
+// Sum of S[w][4 lane + b] over the neighbours w = col[t], t in [t0, t1) with
+// stride `step`, whose frontier bits (fbits) intersect `mask` at b.  U
+// neighbours per step: their ids, then their frontier words, then the
+// matching S values are loaded together, so a long list costs about
+// deg / (step U) memory round trips.  Neighbours are visited in increasing
+// t, so for one (vertex, source) the additions run in list order.
+template <int U>
+__device__ __forceinline__ void pull_sum(int32_t t0, int32_t t1, int32_t step, const int32_t *__restrict__ col,
+                                         const uint64_t *__restrict__ fbits, uint32_t mask,
+                                         const double *__restrict__ S, int lane, double *a) {
+    if (!mask) return;
+    const int word = lane >> 4, sh = (lane & 15) * 4;
+    for (int32_t t = t0; t < t1; t += step * U) {
+        int64_t w[U];
+        uint32_t f[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int32_t tt = t + u * step;
+            w[u] = tt < t1 ? (int64_t)col[tt] : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) f[u] = w[u] >= 0 ? ((uint32_t)(fbits[4 * w[u] + word] >> sh) & mask) : 0u;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (!f[u]) continue;
+            const double *sw = S + w[u] * kSrc + 4 * lane;
+            if (f[u] & 1) a[0] += sw[0];
+            if (f[u] & 2) a[1] += sw[1];
+            if (f[u] & 4) a[2] += sw[2];
+            if (f[u] & 8) a[3] += sw[3];
+        }
+    }
+}
+
+// Forward finish for vertex i: sigma of newly reached sources, frontier and
+// visited bits.  Returns the number of (vertex, source) pairs reached.
+__device__ __forceinline__ uint32_t forward_finish(int64_t i, uint32_t unv, const double *a, uint64_t *fn,
+                                                   uint64_t *vis, double *S, int lane) {
+    uint32_t got = 0;
+    double *si = S + i * kSrc + 4 * lane;
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+        if ((unv & (1u << b)) && a[b] != 0) {
+            si[b] = a[b];
+            got |= 1u << b;
+        }
+    const uint64_t w = gather_word(got, lane);
+    if ((lane & 15) == 0) {
+        fn[4 * i + (lane >> 4)] = w;
+        if (w) vis[4 * i + (lane >> 4)] |= w;
+    }
+    return __popc(got);
+}
+
+// Backward finish for vertex i at level d: delta = sigma * sum, W = (1/sigma)
+// * (1 + delta) in place of sigma; returns the wave's sum of delta.
+__device__ __forceinline__ double backward_finish(int64_t i, uint32_t nd, const double *a, double *S, int lane) {
+    double *si = S + i * kSrc + 4 * lane;
+    double dsum = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        if (nd & (1u << b)) {
+            const double sigma = si[b];
+            const double delta = a[b] * sigma;
+            si[b] = (1.0 / sigma) * (1.0 + delta);
+            dsum += delta;
+        }
+    }
+    return dev::wave_sum(dsum);
+}
+
+// Level d -> d+1 for vertices with fewer than kHeavy in-neighbours, one wave
+// per vertex (heavier ones: k_bc_forward_heavy).  cnt accumulates the number
+// of (vertex, source) pairs reached.
 __global__ __launch_bounds__(256) void k_bc_forward(int64_t n, const int32_t *__restrict__ trst,
                                                     const int32_t *__restrict__ tcol,
                                                     const uint64_t *__restrict__ fd, uint64_t *__restrict__ fn,
@@ -79,37 +154,47 @@ __global__ __launch_bounds__(256) void k_bc_forward(int64_t n, const int32_t *__
     const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
     uint32_t reached = 0;
     for (int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < n; i += waves) {
+        const int32_t t0 = trst[i], t1 = trst[i + 1];
+        if (t1 - t0 >= kHeavy) continue;
         const uint32_t unv = ~nib_of(vis + 4 * i, lane) & 0xFu;
-        uint32_t got = 0;
-        if (__ballot(unv != 0)) {
-            double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-            const int32_t t1 = trst[i + 1];
-            for (int32_t t = trst[i]; t < t1; ++t) {
-                const int64_t k = tcol[t];
-                const uint32_t f = nib_of(fd + 4 * k, lane) & unv;
-                if (f) {
-                    const double *sk = S + k * kSrc + 4 * lane;
-                    if (f & 1) a0 += sk[0];
-                    if (f & 2) a1 += sk[1];
-                    if (f & 4) a2 += sk[2];
-                    if (f & 8) a3 += sk[3];
-                }
-            }
-            double *si = S + i * kSrc + 4 * lane;
-            if ((unv & 1) && a0 != 0) { si[0] = a0; got |= 1; }
-            if ((unv & 2) && a1 != 0) { si[1] = a1; got |= 2; }
-            if ((unv & 4) && a2 != 0) { si[2] = a2; got |= 4; }
-            if ((unv & 8) && a3 != 0) { si[3] = a3; got |= 8; }
-        }
-        const uint64_t w = gather_word(got, lane);
-        if ((lane & 15) == 0) {
-            fn[4 * i + (lane >> 4)] = w;
-            if (w) vis[4 * i + (lane >> 4)] |= w;
-        }
-        reached += __popc(got);
+        double a[4] = {0, 0, 0, 0};
+        if (__ballot(unv != 0)) pull_sum<8>(t0, t1, 1, tcol, fd, unv, S, lane, a);
+        reached += forward_finish(i, unv, a, fn, vis, S, lane);
     }
     unsigned long long r = dev::wave_sum((unsigned long long)reached);
     if (lane == 0 && r) atomicAdd(cnt, r);
+}
+
+// The same for the heavy vertices, one workgroup of kHeavyWaves waves each:
+// wave w takes in-neighbours t0 + w, t0 + w + kHeavyWaves, ...; the partial
+// sums are added in wave order through LDS.
+__global__ __launch_bounds__(64 * kHeavyWaves) void k_bc_forward_heavy(
+    int32_t nh, const int32_t *__restrict__ heavy, const int32_t *__restrict__ trst, const int32_t *__restrict__ tcol,
+    const uint64_t *__restrict__ fd, uint64_t *__restrict__ fn, uint64_t *__restrict__ vis, double *__restrict__ S,
+    unsigned long long *__restrict__ cnt) {
+    __shared__ double part[kHeavyWaves][kSrc];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int32_t h = blockIdx.x; h < nh; h += gridDim.x) {
+        const int64_t i = heavy[h];
+        const uint32_t unv = ~nib_of(vis + 4 * i, lane) & 0xFu;
+        double a[4] = {0, 0, 0, 0};
+        if (__ballot(unv != 0)) pull_sum<8>(trst[i] + wv, trst[i + 1], kHeavyWaves, tcol, fd, unv, S, lane, a);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) part[wv][4 * lane + b] = a[b];
+        __syncthreads();
+        if (wv == 0) {
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                double s = 0;
+                for (int x = 0; x < kHeavyWaves; ++x) s += part[x][4 * lane + b];
+                a[b] = s;
+            }
+            const uint32_t got = forward_finish(i, unv, a, fn, vis, S, lane);
+            const unsigned long long r = dev::wave_sum((unsigned long long)got);
+            if (lane == 0 && r) atomicAdd(cnt, r);
+        }
+        __syncthreads();
+    }
 }
 
 // Deepest level: delta = 0, so W = 1/sigma.
@@ -126,7 +211,8 @@ __global__ __launch_bounds__(256) void k_bc_last(int64_t n, const uint64_t *__re
     }
 }
 
-// Level d (sources in F_d) from level d+1 (F_{d+1}, already holding W).
+// Level d (sources in F_d) from level d+1 (F_{d+1}, already holding W), for
+// vertices with fewer than kHeavy out-neighbours.
 __global__ __launch_bounds__(256) void k_bc_backward(int64_t n, const int32_t *__restrict__ rst,
                                                      const int32_t *__restrict__ col,
                                                      const uint64_t *__restrict__ fd,
@@ -137,36 +223,49 @@ __global__ __launch_bounds__(256) void k_bc_backward(int64_t n, const int32_t *_
     for (int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < n; i += waves) {
         const uint32_t nd = nib_of(fd + 4 * i, lane);
         if (!__ballot(nd != 0)) continue;
-        double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-        if (nd) {
-            const int32_t t1 = rst[i + 1];
-            for (int32_t t = rst[i]; t < t1; ++t) {
-                const int64_t w = col[t];
-                const uint32_t f = nib_of(fn + 4 * w, lane) & nd;
-                if (f) {
-                    const double *sw = S + w * kSrc + 4 * lane;
-                    if (f & 1) a0 += sw[0];
-                    if (f & 2) a1 += sw[1];
-                    if (f & 4) a2 += sw[2];
-                    if (f & 8) a3 += sw[3];
-                }
-            }
-        }
-        double *si = S + i * kSrc + 4 * lane;
-        double dsum = 0;
-        const double acc[4] = {a0, a1, a2, a3};
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            if (nd & (1u << b)) {
-                const double sigma = si[b];
-                const double delta = acc[b] * sigma;
-                si[b] = (1.0 / sigma) * (1.0 + delta);
-                dsum += delta;
-            }
-        }
-        dsum = dev::wave_sum(dsum);
+        const int32_t t0 = rst[i], t1 = rst[i + 1];
+        if (t1 - t0 >= kHeavy) continue;
+        double a[4] = {0, 0, 0, 0};
+        pull_sum<8>(t0, t1, 1, col, fn, nd, S, lane, a);
+        const double dsum = backward_finish(i, nd, a, S, lane);
         if (lane == 0) bc[i] += dsum;
     }
+}
+
+__global__ __launch_bounds__(64 * kHeavyWaves) void k_bc_backward_heavy(
+    int32_t nh, const int32_t *__restrict__ heavy, const int32_t *__restrict__ rst, const int32_t *__restrict__ col,
+    const uint64_t *__restrict__ fd, const uint64_t *__restrict__ fn, double *__restrict__ S,
+    double *__restrict__ bc) {
+    __shared__ double part[kHeavyWaves][kSrc];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int32_t h = blockIdx.x; h < nh; h += gridDim.x) {
+        const int64_t i = heavy[h];
+        const uint32_t nd = nib_of(fd + 4 * i, lane);
+        if (!__syncthreads_or(nd != 0)) continue;
+        double a[4] = {0, 0, 0, 0};
+        pull_sum<8>(rst[i] + wv, rst[i + 1], kHeavyWaves, col, fn, nd, S, lane, a);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) part[wv][4 * lane + b] = a[b];
+        __syncthreads();
+        if (wv == 0) {
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                double s = 0;
+                for (int x = 0; x < kHeavyWaves; ++x) s += part[x][4 * lane + b];
+                a[b] = s;
+            }
+            const double dsum = backward_finish(i, nd, a, S, lane);
+            if (lane == 0) bc[i] += dsum;
+        }
+        __syncthreads();
+    }
+}
+
+// Vertices with at least kHeavy entries in `rst` rows.
+__global__ void k_heavy_list(int64_t n, const int32_t *__restrict__ rst, int32_t *__restrict__ out,
+                             int32_t *__restrict__ nout) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        if (rst[i + 1] - rst[i] >= kHeavy) out[atomicAdd(nout, 1)] = (int32_t)i;
 }
 
 // A^T of a pattern CSR (counting sort, as transpose in include/matrix_la.h:155-179).
@@ -242,6 +341,20 @@ extern "C" int gk_bc_batch_dev(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *AT,
     GK_TRY(D.alloc(&vis, (size_t)n * 4));
     GK_TRY(D.alloc(&dsrc, 256));
     GK_TRY(D.alloc(&dcnt, 1));
+    // heavy vertices: in-degree (forward) and out-degree (backward) lists
+    int32_t *hin, *hout, *nh;
+    GK_TRY(D.alloc(&hin, n));
+    GK_TRY(D.alloc(&hout, n));
+    GK_TRY(D.alloc(&nh, 2));
+    GK_HIP(hipMemsetAsync(nh, 0, 8, ctx->stream));
+    {
+        const int g = (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, ctx->num_cus * 16));
+        k_heavy_list<<<g, 256, 0, ctx->stream>>>(n, trst, hin, nh);
+        k_heavy_list<<<g, 256, 0, ctx->stream>>>(n, A->rst, hout, nh + 1);
+        GK_TRY(launch_check("k_heavy_list"));
+    }
+    int32_t nheavy[2];
+    GK_HIP(d2h(ctx, nheavy, nh, 8));
     std::vector<uint64_t *> F;  // per-level frontier bitmaps, grown on demand
     unsigned long long *h;
     GK_TRY(pinned(ctx, 16, (void **)&h));
@@ -274,6 +387,9 @@ extern "C" int gk_bc_batch_dev(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *AT,
             }
             GK_HIP(hipMemsetAsync(dcnt, 0, 8, ctx->stream));
             k_bc_forward<<<gw, 256, 0, ctx->stream>>>(n, trst, tcol, F[d], F[d + 1], vis, S, dcnt);
+            if (nheavy[0])
+                k_bc_forward_heavy<<<std::min(nheavy[0], ctx->num_cus * 2), 64 * kHeavyWaves, 0, ctx->stream>>>(
+                    nheavy[0], hin, trst, tcol, F[d], F[d + 1], vis, S, dcnt);
             GK_TRY(launch_check("k_bc_forward"));
             GK_HIP(hipMemcpyAsync(h, dcnt, 8, hipMemcpyDeviceToHost, ctx->stream));
             GK_HIP(hipStreamSynchronize(ctx->stream));
@@ -290,6 +406,9 @@ extern "C" int gk_bc_batch_dev(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *AT,
             GK_TRY(launch_check("k_bc_last"));
             for (int l = d - 1; l >= 1; --l) {
                 k_bc_backward<<<gw, 256, 0, ctx->stream>>>(n, A->rst, A->col, F[l], F[l + 1], S, d_bc);
+                if (nheavy[1])
+                    k_bc_backward_heavy<<<std::min(nheavy[1], ctx->num_cus * 2), 64 * kHeavyWaves, 0,
+                                          ctx->stream>>>(nheavy[1], hout, A->rst, A->col, F[l], F[l + 1], S, d_bc);
                 GK_TRY(launch_check("k_bc_backward"));
             }
         }
