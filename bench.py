@@ -129,6 +129,19 @@ def bc_bytes(n: int, nnz: int, levels: int, nsrc: int) -> float:
     return 2 * levels * (4 * (n + 1) + 4 * nnz) + 24.0 * nsrc * nnz + 24.0 * nsrc * n
 
 
+def calibrate_copy(nbytes: int = 256 << 20) -> None:
+    """One device-to-device copy of nbytes (read nbytes, write nbytes): the
+    reference kernel for converting FETCH_SIZE / WRITE_SIZE counter readings
+    into bytes."""
+    import torch
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"calibration copy: {nbytes} bytes read and written")
+
+
 def cpu_threads() -> int:
     try:
         return len(os.sched_getaffinity(0))
@@ -147,6 +160,11 @@ def main() -> int:
     ap.add_argument("--bc-batches", type=int, default=4)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tc-variants", default="all", choices=["all", "plain"],
+                    help="plain: only the fixed-heuristic triangle count (no selector training or autotuning)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="end with a 256 MiB device-to-device copy, the known-size kernel that scales "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); this build has no multi-GPU path.",
@@ -176,7 +194,7 @@ def main() -> int:
         t0 = time.perf_counter()
         G22 = HostGraph.rmat(a.tc_scale, 16, seed=a.seed, **RMAT)
         log(f"R-MAT s{a.tc_scale}: n={G22.n} |E|={G22.edges} built in {time.perf_counter() - t0:.1f}s")
-        plain = run_tc(ctx, G22.L, max(a.steps, 3), a.warmup)
+        plain = run_tc(ctx, G22.L, a.steps, a.warmup)
         pd = dominant(plain["timing"])
         tc = {"workload": f"triangle counting C<L>=L*L, R-MAT scale {a.tc_scale} ef16", "edges": G22.edges,
               "triangles": plain["count"],
@@ -188,6 +206,8 @@ def main() -> int:
                         (plain["timing"]["total_ms"] / 1e3) / HBM_PEAK_GBS}}
         log(f"tc plain: {plain['wall_s'] * 1e3:.2f} ms, T={plain['count']}, dominant {pd['kernel']} "
             f"{pd['achieved_gbs']:.0f} GB/s")
+        tc["plain"]["rows_per_acc"] = plain["timing"]["rows_per_acc"]
+    if tc is not None and a.tc_variants == "all":
         # learned selector: trained on timing samples the product generates
         # on other seeded graphs, then the global knobs autotuned on a
         # separate R-MAT scale-20 graph; both outside the timed region
@@ -209,7 +229,7 @@ def main() -> int:
         res = {}
         for name, opts in (("tuned_heuristic", tuned_h), ("learned", {"selector": SEL_LEARNED}),
                            ("learned_tuned", tuned_l)):
-            r = run_tc(ctx, G22.L, max(a.steps, 3), a.warmup, opts)
+            r = run_tc(ctx, G22.L, a.steps, a.warmup, opts)
             assert r["count"] == plain["count"], f"{name} changed the triangle count"
             d = dominant(r["timing"])
             res[name] = {"value": G22.edges / r["wall_s"], "ms": r["wall_s"] * 1e3, "opts": opts,
@@ -217,7 +237,6 @@ def main() -> int:
                          "frac_all_kernels": (r["timing"]["bytes_alg"] / 1e9) / (r["wall_s"]) / HBM_PEAK_GBS,
                          "rows_per_acc": r["timing"]["rows_per_acc"]}
             log(f"tc {name}: {r['wall_s'] * 1e3:.2f} ms ({plain['wall_s'] / r['wall_s']:.3f}x plain)")
-        tc["plain"]["rows_per_acc"] = plain["timing"]["rows_per_acc"]
         tc.update(res)
         tc["selector_training"] = rep
     bc = None
@@ -267,6 +286,9 @@ def main() -> int:
                                   "sample": f"the first batch of 256 sources on the same scale-{a.ktruss_scale} "
                                             f"graph, FP64 restatement, in {dt:.2f}s"}
             log(f"cpu baseline bc: {bc['cpu_baseline']['value']:.3g} source-edges/s")
+
+    if a.calibrate_copy:
+        calibrate_copy()
 
     if kt is not None:
         value = E20 / kt["wall_s"]

@@ -217,10 +217,32 @@ def default_context() -> Context:
 
 # ---- reference-named entry points ------------------------------------------
 
+def _canonical_mask(M: SpMtx):
+    """(M', pos): M' holds M's pattern with every row sorted and duplicate
+    free, pos[p] is the entry of M' that entry p of M names; (M, None) when
+    M is already canonical.  The reference's MSA reads its accumulator once
+    per mask entry (include/matrix_la.h:612-631), so rows in file order and
+    repeated columns are legal masks and every copy gets the column's value."""
+    rows = np.repeat(np.arange(M.m, dtype=np.int64), np.diff(M.Rst))
+    same = rows[1:] == rows[:-1]
+    if M.nz < 2 or bool(np.all(M.Col[1:][same] > M.Col[:-1][same])):
+        return M, None
+    key = rows * max(M.n, 1) + M.Col.astype(np.int64)
+    uniq, pos = np.unique(key, return_inverse=True)
+    urow = uniq // max(M.n, 1)
+    rst = np.zeros(M.m + 1, np.int64)
+    np.cumsum(np.bincount(urow, minlength=M.m), out=rst[1:])
+    Ms = SpMtx(M.m, M.n, rst.astype(np.int32), (uniq % max(M.n, 1)).astype(np.int32), None, M.matcode)
+    return Ms, pos
+
+
 def _product_into(A: SpMtx, B: SpMtx, M: SpMtx, C_: SpMtx) -> None:
     if A.n != B.m:
         raise ValueError("A.n must equal B.m")
-    vals = default_context().masked_spgemm(A, B, M)
+    Ms, pos = _canonical_mask(M)
+    vals = default_context().masked_spgemm(A, B, Ms)
+    if pos is not None:
+        vals = vals[pos]
     # C.resizeRows(M.m); C.resizeVals(M.nz); C.n = M.n; copy M's pattern
     C_.m, C_.n = M.m, M.n
     C_.Rst = M.Rst.copy()

@@ -1,5 +1,7 @@
 """Build libgraphhip.so in-tree: hipcc for the gfx950 kernels, g++ for the
-host-side graph construction, one shared library next to this file.
+host-side graph construction, one shared library next to this file.  Then
+g++ builds tests/cpp/dropin_check, the C++ program that drives the drop-in
+headers (include/graphkokkos/*.h) against that library.
 
     python -m graphkokkos_amd.build            # build (always recompiles)
     python -m graphkokkos_amd.build --check    # report whether it is stale
@@ -20,6 +22,8 @@ CSRC = PKG / "csrc"
 INCLUDE = ROOT / "include"
 OUT = PKG / "libgraphhip.so"
 BUILD_DIR = PKG / "_build"
+DROPIN_SRC = ROOT / "tests" / "cpp" / "dropin_check.cpp"
+DROPIN_BIN = ROOT / "tests" / "cpp" / "_build" / "dropin_check"
 
 HIP_SOURCES = ["gk_runtime.hip", "spgemm.hip", "ktruss.hip", "bc.hip", "selector.hip"]
 HOST_SOURCES = ["host_graph.cpp"]
@@ -54,7 +58,7 @@ def _run(cmd: list[str]) -> None:
 
 
 def sources() -> list[Path]:
-    return [CSRC / s for s in HIP_SOURCES + HOST_SOURCES] + sorted(CSRC.glob("*.h")) + sorted(INCLUDE.glob("*.h"))
+    return [CSRC / s for s in HIP_SOURCES + HOST_SOURCES] + sorted(CSRC.glob("*.h")) + sorted(INCLUDE.glob("**/*.h"))
 
 
 def is_stale() -> bool:
@@ -87,7 +91,16 @@ def build(verbose: bool = False, jobs: int | None = None) -> Path:
     tmp = OUT.with_suffix(".so.tmp")
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs), "-pthread"])
     tmp.replace(OUT)
+    build_dropin_check()
     return OUT
+
+
+def build_dropin_check() -> Path:
+    """The drop-in headers compiled as a client of libgraphhip.so would."""
+    DROPIN_BIN.parent.mkdir(parents=True, exist_ok=True)
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", "-Wno-sign-compare", f"-I{INCLUDE}", str(DROPIN_SRC),
+          "-o", str(DROPIN_BIN), f"-L{PKG}", "-lgraphhip", "-Wl,-rpath,$ORIGIN/../../../graphkokkos_amd"])
+    return DROPIN_BIN
 
 
 def main(argv: list[str] | None = None) -> int:

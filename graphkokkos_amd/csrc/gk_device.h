@@ -71,6 +71,11 @@ struct gk_dcsr {
     int32_t *col = nullptr;
     int32_t *val = nullptr;   // nullptr: pattern (all ones)
     bool owns = true;
+    // Structure recorded when the host arrays were checked at upload
+    // (internal views are built sorted by construction):
+    bool sorted = true;       // every row strictly increasing
+    bool lower = false;       // every column below its row
+    bool symmetric = false;   // square, pattern symmetric, no self loops
 };
 
 namespace gk {
@@ -89,6 +94,12 @@ gk_dcsr view(gk_ctx *ctx, int64_t m, int64_t n, int64_t nz, int32_t *rst, int32_
 int upload_csr(gk_ctx *ctx, const gk_csr *h, gk_dcsr *d);
 void free_csr(gk_dcsr *d);
 int check_csr(const gk_csr *c, const char *name);
+// check_csr plus a scan of the arrays: row pointers non-decreasing, columns
+// in [0, n); fills d's sorted / lower / symmetric flags.
+int inspect_csr(const gk_csr *c, const char *name, gk_dcsr *d);
+// Entry-point preconditions on device handles.
+int require_sorted(const gk_dcsr *d, const char *name);
+int require_symmetric(const gk_dcsr *d, const char *name);
 // Event timer on the context stream.
 struct Timer {
     gk_ctx *ctx;

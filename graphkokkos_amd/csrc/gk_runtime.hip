@@ -66,8 +66,76 @@ int check_csr(const gk_csr *c, const char *name) {
     return GK_OK;
 }
 
+int inspect_csr(const gk_csr *c, const char *name, gk_dcsr *d) {
+    GK_TRY(check_csr(c, name));
+    const int64_t m = c->m, n = c->n;
+    const int32_t *rst = c->rst, *col = c->col;
+    std::atomic<int64_t> bad_row{-1}, bad_col{-1};
+    std::atomic<bool> unsorted{false}, upper{false};
+    // pass 1: row pointers, column range, row order, triangle side
+    par_dynamic(m, 4096, [&](int64_t i) {
+        const int32_t b = rst[i], e = rst[i + 1];
+        if (e < b || b < 0 || e > c->nz) { bad_row.store(i); return; }
+        int32_t prev = -1;
+        bool srt = true, low = true;
+        for (int32_t p = b; p < e; ++p) {
+            const int32_t j = col[p];
+            if (j < 0 || j >= n) { bad_col.store(p); return; }
+            srt &= j > prev;
+            low &= j < i;
+            prev = j;
+        }
+        if (!srt) unsorted.store(true, std::memory_order_relaxed);
+        if (!low) upper.store(true, std::memory_order_relaxed);
+    });
+    if (bad_row.load() >= 0) {
+        set_error("%s: row pointers of row %lld are decreasing or outside [0, nz]", name, (long long)bad_row.load());
+        return GK_EINVAL;
+    }
+    if (bad_col.load() >= 0) {
+        set_error("%s: column index at entry %lld is outside [0, %lld)", name, (long long)bad_col.load(),
+                  (long long)n);
+        return GK_EINVAL;
+    }
+    d->sorted = !unsorted.load();
+    d->lower = !upper.load();
+    d->symmetric = false;
+    if (d->sorted && m == n) {
+        // pass 2: every (i, j) has its mirror (j, i), no diagonal entries
+        std::atomic<bool> asym{false};
+        par_dynamic(m, 4096, [&](int64_t i) {
+            if (asym.load(std::memory_order_relaxed)) return;
+            for (int32_t p = rst[i]; p < rst[i + 1]; ++p) {
+                const int32_t j = col[p];
+                if (j == i || !std::binary_search(col + rst[j], col + rst[j + 1], (int32_t)i)) {
+                    asym.store(true, std::memory_order_relaxed);
+                    return;
+                }
+            }
+        });
+        d->symmetric = !asym.load();
+    }
+    return GK_OK;
+}
+
+int require_sorted(const gk_dcsr *d, const char *name) {
+    if (!d->sorted) {
+        set_error("%s must have strictly increasing column indices in every row (sorted, no duplicates)", name);
+        return GK_EINVAL;
+    }
+    return GK_OK;
+}
+
+int require_symmetric(const gk_dcsr *d, const char *name) {
+    if (!d->symmetric) {
+        set_error("%s must be the sorted, symmetric pattern of an undirected graph without self loops", name);
+        return GK_EINVAL;
+    }
+    return GK_OK;
+}
+
 int upload_csr(gk_ctx *ctx, const gk_csr *h, gk_dcsr *d) {
-    GK_TRY(check_csr(h, "csr"));
+    GK_TRY(inspect_csr(h, "csr", d));
     d->ctx = ctx; d->m = h->m; d->n = h->n; d->nz = h->nz; d->owns = true;
     d->rst = d->col = d->val = nullptr;
     GK_HIP(hipMalloc(&d->rst, sizeof(int32_t) * (h->m + 1)));

@@ -836,6 +836,7 @@ extern "C" int gk_truss_decompose_dev(gk_ctx *ctx, const gk_dcsr *A, uint32_t *d
                                       const gk_opts *opts) {
     GK_CHECK_ARG(ctx && A && (d_truss || A->nz == 0), "null argument");
     GK_CHECK_ARG(A->m == A->n, "A must be square");
+    GK_TRY(require_symmetric(A, "A"));
     gk_opts o = resolve_opts(opts);
     ctx->last = gk_timing{};
     Timer total(ctx);
@@ -942,7 +943,8 @@ extern "C" int gk_ktruss(gk_ctx *ctx, const gk_csr *A, int k, int32_t *rst_out, 
     gk_opts o = resolve_opts(opts);
     gk_dcsr d;
     GK_TRY(upload_csr(ctx, A, &d));
-    int rc = GK_OK;
+    int rc = require_symmetric(&d, "A");
+    if (rc) { free_csr(&d); return rc; }
     {
         TrussBufs B(ctx);
         Lgraph cur;
@@ -993,7 +995,8 @@ extern "C" int gk_ktruss_support(gk_ctx *ctx, const gk_csr *A, uint32_t *support
     gk_opts o = resolve_opts(opts);
     gk_dcsr d;
     GK_TRY(upload_csr(ctx, A, &d));
-    int rc = GK_OK;
+    int rc = require_symmetric(&d, "A");
+    if (rc) { free_csr(&d); return rc; }
     {
         TrussBufs B(ctx);
         Lgraph L;

@@ -439,6 +439,7 @@ extern "C" int gk_selector_samples(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr 
                                    float *X, float *T, uint8_t *valid) {
     GK_CHECK_ARG(ctx && A && B && M && X && T && valid, "null argument");
     GK_CHECK_ARG(mode == 0 || mode == 1, "mode must be 0 (triangle counting) or 1 (values)");
+    GK_TRY(require_sorted(M, "M"));
     const int64_t m = A->m;
     const int same_am = A->rst == M->rst && A->col == M->col;
     DevBuf b;

@@ -1099,6 +1099,7 @@ void record_timing(gk_ctx *ctx, const ProductInfo &pi, bool reset) {
 extern "C" int gk_masked_spgemm_dev(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcsr *M,
                                     int32_t *d_cval, const gk_opts *opts) {
     GK_CHECK_ARG(ctx && A && B && M && (d_cval || M->nz == 0), "null argument");
+    GK_TRY(require_sorted(M, "M"));
     gk_opts o = resolve_opts(opts);
     ProductInfo pi;
     GK_TRY(masked_product(ctx, A, B, M, MODE_VALUES, d_cval, nullptr, nullptr, o, &pi));
@@ -1139,6 +1140,7 @@ extern "C" int gk_masked_spgemm_i32(gk_ctx *ctx, const gk_csr *A, const gk_csr *
 extern "C" int gk_tc_count_dev(gk_ctx *ctx, const gk_dcsr *L, uint64_t *count, const gk_opts *opts) {
     GK_CHECK_ARG(ctx && L && count, "null argument");
     GK_CHECK_ARG(L->m == L->n, "L must be square");
+    GK_TRY(require_sorted(L, "L"));
     gk_opts o = resolve_opts(opts);
     void *p;
     GK_TRY(scratch(ctx, SC_RED, 64, &p));
@@ -1157,6 +1159,8 @@ extern "C" int gk_tc_count_dev(gk_ctx *ctx, const gk_dcsr *L, uint64_t *count, c
 
 extern "C" int gk_tc_rows_dev(gk_ctx *ctx, const gk_dcsr *L, uint64_t *d_rowcount, const gk_opts *opts) {
     GK_CHECK_ARG(ctx && L && d_rowcount, "null argument");
+    GK_CHECK_ARG(L->m == L->n, "L must be square");
+    GK_TRY(require_sorted(L, "L"));
     gk_opts o = resolve_opts(opts);
     void *p;
     GK_TRY(scratch(ctx, SC_RED, 64, &p));

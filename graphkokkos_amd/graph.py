@@ -112,22 +112,28 @@ def build_adjacency_matrix(g: SpMtx) -> SpMtx:
     return SpMtx(g.m, g.n, g.Rst.copy(), g.Col.copy(), np.ones(g.nz, np.int32), g.matcode[:2] + "I" + g.matcode[3])
 
 
+def _rows_sorted(a: SpMtx) -> bool:
+    rows = np.repeat(np.arange(a.m, dtype=np.int64), np.diff(a.Rst))
+    same = rows[1:] == rows[:-1]
+    return bool(np.all(a.Col[1:][same] > a.Col[:-1][same]))
+
+
 def extract_lower_triangle(g: SpMtx) -> SpMtx:
-    """Strict-lower prefix of every row; assumes sorted rows
-    (include/matrix.h:698-726)."""
-    keep = np.zeros(g.m + 1, np.int64)
-    cols, vals = [], []
-    for i in range(g.m):
-        r = g.row(i)
-        k = int(np.searchsorted(r, i, side="left")) if r.size else 0
-        # the reference walks while Col < i; on a sorted row that is the prefix
-        keep[i + 1] = keep[i] + k
-        cols.append(r[:k])
-        if g.Val is not None:
-            vals.append(g.Val[g.Rst[i]:g.Rst[i] + k])
-    col = np.concatenate(cols) if cols else np.zeros(0, np.int32)
-    val = np.concatenate(vals) if vals else None
-    return SpMtx(g.m, g.n, keep.astype(np.int32), col, val, g.matcode)
+    """Each row's leading run of columns below the diagonal; on sorted rows
+    that is the strictly lower part (include/matrix.h:698-726)."""
+    rst = np.asarray(g.Rst, np.int64)
+    rows = np.repeat(np.arange(g.m, dtype=np.int64), np.diff(rst))
+    stop = np.where(g.Col >= rows, np.arange(g.nz, dtype=np.int64), np.iinfo(np.int64).max)
+    first = rst[1:].copy()
+    nonempty = np.flatnonzero(np.diff(rst) > 0)
+    if nonempty.size:
+        first[nonempty] = np.minimum(first[nonempty], np.minimum.reduceat(stop, rst[nonempty]))
+    lens = first - rst[:-1]
+    keep_rst = np.zeros(g.m + 1, np.int64)
+    np.cumsum(lens, out=keep_rst[1:])
+    sel = np.arange(g.nz, dtype=np.int64) < np.repeat(first, np.diff(rst))
+    val = None if g.Val is None else g.Val[sel]
+    return SpMtx(g.m, g.n, keep_rst.astype(np.int32), g.Col[sel], val, g.matcode)
 
 
 def transpose(a: SpMtx) -> SpMtx:
@@ -142,18 +148,38 @@ def transpose(a: SpMtx) -> SpMtx:
 
 
 def build_symm_from_lower(low: SpMtx) -> SpMtx:
-    """L merged with L^T row by row (include/matrix.h:662-696)."""
+    """Row-wise merge of L and L^T (include/matrix.h:662-696): an entry of L
+    is taken while its column is below the next column of L^T's row."""
     up = transpose(low)
-    rows = np.concatenate([np.repeat(np.arange(low.m), np.diff(low.Rst)), np.repeat(np.arange(up.m), np.diff(up.Rst))])
-    cols = np.concatenate([low.Col, up.Col])
-    vals = None
-    if low.Val is not None:
-        vals = np.concatenate([low.Val, up.Val])
-    order = np.lexsort((cols, rows))
-    rst = np.zeros(low.m + 1, np.int64)
-    np.cumsum(np.bincount(rows, minlength=low.m), out=rst[1:])
-    return SpMtx(low.m, low.n, rst.astype(np.int32), cols[order].astype(np.int32),
-                 None if vals is None else vals[order], low.matcode)
+    lv = low.Val if low.Val is not None else np.ones(low.nz, np.int32)
+    uv = up.Val if up.Val is not None else np.ones(up.nz, np.int32)
+    if _rows_sorted(low):
+        rows = np.concatenate([np.repeat(np.arange(up.m), np.diff(up.Rst)), np.repeat(np.arange(low.m), np.diff(low.Rst))])
+        cols = np.concatenate([up.Col, low.Col])
+        vals = np.concatenate([uv, lv])
+        order = np.lexsort((cols, rows))   # stable: on equal columns L^T's entry first, as the merge takes
+        rst = np.zeros(low.m + 1, np.int64)
+        np.cumsum(np.bincount(rows, minlength=low.m), out=rst[1:])
+        cols, vals = cols[order], vals[order]
+    else:
+        rst = np.zeros(low.m + 1, np.int64)
+        cols = np.zeros(2 * low.nz, np.int32)
+        vals = np.zeros(2 * low.nz, lv.dtype)
+        w = 0
+        for i in range(low.m):
+            a, ae = int(low.Rst[i]), int(low.Rst[i + 1])
+            b, be = int(up.Rst[i]), int(up.Rst[i + 1])
+            while a < ae or b < be:
+                if b >= be or (a < ae and low.Col[a] < up.Col[b]):
+                    cols[w], vals[w] = low.Col[a], lv[a]
+                    a += 1
+                else:
+                    cols[w], vals[w] = up.Col[b], uv[b]
+                    b += 1
+                w += 1
+            rst[i + 1] = w
+    return SpMtx(low.m, low.n, rst.astype(np.int32), cols.astype(np.int32),
+                 None if low.Val is None else vals, low.matcode)
 
 
 class HostGraph:

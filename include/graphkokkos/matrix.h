@@ -1,0 +1,295 @@
+// graphkokkos/matrix.h -- drop-in for GraphKokkos's include/matrix.h: the CSR
+// container spMtx<ValT>, the Matrix Market loader and the graph
+// pre-processing helpers, on host memory, with no Kokkos dependency.
+//
+// Same member names and meaning as the reference (include/matrix.h:33-208):
+// m, n, nz, matcode and the arrays Rst[m+1], Col[nz], Val[nz].  The arrays
+// are gk::View<T>: reference-counted host buffers with Kokkos::View's
+// shallow-copy semantics, element access by () and [], data(), size() and
+// extent(0).  Products run on the GPU through libgraphhip (graphhip.h); the
+// container itself never touches device memory.
+#pragma once
+
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <memory>
+#include <string>
+#include <type_traits>
+#include <utility>
+#include <vector>
+
+#include "runtime.h"
+
+#ifndef MM_TYPECODE_DEFINED
+#define MM_TYPECODE_DEFINED
+typedef char MM_typecode[4];
+#endif
+
+namespace gk {
+
+template <class T>
+class View {
+public:
+    View() = default;
+    // zero-initialised, like View<T*>(label, n)
+    View(const std::string &, size_t n) : n_(n), p_(n ? std::shared_ptr<T[]>(new T[n]()) : nullptr) {}
+    T &operator()(size_t i) const { return p_[i]; }
+    T &operator[](size_t i) const { return p_[i]; }
+    T *data() const { return p_.get(); }
+    size_t size() const { return n_; }
+    size_t extent(int) const { return n_; }
+
+private:
+    size_t n_ = 0;
+    std::shared_ptr<T[]> p_;
+};
+
+// Kokkos::resize: a new buffer holding the first min(old, n) elements, the
+// rest zero.  Other views of the old buffer keep it.
+template <class T>
+void resize(View<T> &v, size_t n) {
+    if (n == v.size()) return;
+    View<T> w("", n);
+    std::copy(v.data(), v.data() + std::min(n, v.size()), w.data());
+    v = w;
+}
+
+// Kokkos::deep_copy between views of equal extent.
+template <class T>
+void deep_copy(const View<T> &dst, const View<T> &src) {
+    if (dst.size() != src.size()) throw error(GK_EINVAL, "deep_copy: extents differ");
+    std::copy(src.data(), src.data() + src.size(), dst.data());
+}
+
+}  // namespace gk
+
+template <typename ValT>
+class spMtx {
+public:
+    size_t m = 0;
+    size_t n = 0;
+    size_t nz = 0;
+    MM_typecode matcode = {'M', 'C', 'I', 'G'};
+    gk::View<int> Rst;
+    gk::View<int> Col;
+    gk::View<ValT> Val;
+
+    // include/matrix.h:44-61.  "mtx" is read by libgraphhip's Matrix Market
+    // reader (1-based coordinates, self loops dropped, symmetric entries
+    // mirrored, rows in file order, duplicates kept).
+    spMtx(const char *filename, const std::string &format) {
+        if (format == "mtx") {
+            if (read_mtx_to_crs(filename)) {
+                std::cout << "Can't read MTX from file\n";
+                throw "Can't read MTX from file";
+            }
+        } else {
+            std::cout << "Can't read " << format << " from file\n";
+            throw "Unsupported matrix file format (graphkokkos_amd reads mtx)";
+        }
+    }
+
+    spMtx() {}
+    spMtx(size_t _m, size_t _n) : m(_m), n(_n), Rst("", _m + 1) {}
+    spMtx(size_t _m, size_t _n, size_t _nz) : m(_m), n(_n), nz(_nz), Rst("", _m + 1), Col("", _nz), Val("", _nz) {}
+
+    // copies share the arrays (include/matrix.h:72-75)
+    spMtx(const spMtx &) = default;
+    spMtx &operator=(const spMtx &) = default;
+    spMtx(spMtx &&o) noexcept { *this = std::move(o); }
+    spMtx &operator=(spMtx &&o) noexcept {
+        if (this != &o) {
+            m = o.m; n = o.n; nz = o.nz;
+            std::memcpy(matcode, o.matcode, sizeof(MM_typecode));
+            std::swap(Rst, o.Rst);
+            std::swap(Col, o.Col);
+            std::swap(Val, o.Val);
+        }
+        return *this;
+    }
+
+    void resizeRows(size_t newM) {
+        m = newM;
+        gk::resize(Rst, newM + 1);
+    }
+    void resizeVals(size_t newNz) {
+        gk::resize(Col, newNz);
+        gk::resize(Val, newNz);
+        nz = newNz;
+    }
+
+    // pattern of src, values left zero (include/matrix.h:98-108)
+    template <typename ValT2>
+    void deep_copy_pattern(const spMtx<ValT2> &src) {
+        m = src.m; n = src.n; nz = src.nz;
+        Rst = gk::View<int>("", src.Rst.size());
+        gk::deep_copy(Rst, src.Rst);
+        Col = gk::View<int>("", src.Col.size());
+        gk::deep_copy(Col, src.Col);
+        Val = gk::View<ValT>("", src.nz);
+    }
+
+    void deep_copy(const spMtx &src) {
+        m = src.m; n = src.n; nz = src.nz;
+        std::memcpy(matcode, src.matcode, sizeof(MM_typecode));
+        Rst = gk::View<int>("", src.Rst.size());
+        Col = gk::View<int>("", src.Col.size());
+        Val = gk::View<ValT>("", src.Val.size());
+        gk::deep_copy(Rst, src.Rst);
+        gk::deep_copy(Col, src.Col);
+        gk::deep_copy(Val, src.Val);
+    }
+
+    // rows [begin, end) as a matrix of their own (include/matrix.h:151-163)
+    spMtx extractRows(size_t begin, size_t end) const {
+        const int b = Rst(begin), e = Rst(end);
+        spMtx r(end - begin, n, size_t(e - b));
+        for (size_t i = 0; i <= end - begin; ++i) r.Rst(i) = Rst(i + begin) - b;
+        std::copy(Col.data() + b, Col.data() + e, r.Col.data());
+        std::copy(Val.data() + b, Val.data() + e, r.Val.data());
+        std::memcpy(r.matcode, matcode, sizeof(MM_typecode));
+        return r;
+    }
+
+    bool operator==(const spMtx &o) const {
+        if (m != o.m || n != o.n || nz != o.nz) return false;
+        return std::equal(Rst.data(), Rst.data() + m + 1, o.Rst.data()) &&
+               std::equal(Col.data(), Col.data() + nz, o.Col.data()) &&
+               std::equal(Val.data(), Val.data() + nz, o.Val.data());
+    }
+
+    // 1-based "row col [value]" lines (include/matrix.h:177-189)
+    void print_crs() const {
+        std::cout << m << ' ' << nz << '\n';
+        for (size_t i = 0; i < m; ++i)
+            for (int j = Rst(i); j < Rst(i + 1); ++j) {
+                std::cout << i + 1 << ' ' << Col(j) + 1;
+                if (Val.size() > 0) std::cout << ' ' << Val(j);
+                std::cout << '\n';
+            }
+    }
+
+    // general coordinate Matrix Market file (include/matrix.h:225-252)
+    int write_crs_to_mtx(const char *filename) const {
+        std::ofstream f(filename);
+        if (!f) return -1;
+        const bool pattern = matcode[2] == 'P';
+        f << "%%MatrixMarket matrix coordinate " << (pattern ? "pattern" : "integer") << " general\n";
+        f << m << ' ' << n << ' ' << nz << '\n';
+        for (size_t i = 0; i < m; ++i)
+            for (int j = Rst(i); j < Rst(i + 1); ++j) {
+                f << i + 1 << ' ' << Col(j) + 1;
+                if (!pattern) f << ' ' << Val(j);
+                f << '\n';
+            }
+        return f ? 0 : -1;
+    }
+
+    // Borrowed C-ABI view of the arrays; values only when ValT is int32 and
+    // with_values is set (otherwise a pattern: every value 1).
+    gk_csr csr(bool with_values = true) const {
+        gk_csr c;
+        c.m = (int64_t)m; c.n = (int64_t)n; c.nz = (int64_t)nz;
+        c.rst = Rst.data();
+        c.col = Col.data();
+        c.val = nullptr;
+        if constexpr (std::is_same<ValT, int>::value)
+            if (with_values && Val.size() >= nz) c.val = Val.data();
+        return c;
+    }
+
+private:
+    int read_mtx_to_crs(const char *filename) {
+        gk_mtx *mt = nullptr;
+        if (gk_mtx_read(filename, &mt) != GK_OK) return -1;
+        gk_csr p;
+        const double *v = nullptr;
+        char code[4];
+        if (gk_mtx_view(mt, &p, &v, code) != GK_OK) { gk_mtx_free(mt); return -1; }
+        m = (size_t)p.m; n = (size_t)p.n; nz = (size_t)p.nz;
+        std::memcpy(matcode, code, 4);
+        Rst = gk::View<int>("", m + 1);
+        Col = gk::View<int>("", nz);
+        Val = gk::View<ValT>("", nz);
+        std::copy(p.rst, p.rst + m + 1, Rst.data());
+        std::copy(p.col, p.col + nz, Col.data());
+        for (size_t j = 0; j < nz; ++j) Val(j) = v ? static_cast<ValT>(v[j]) : ValT(1);
+        gk_mtx_free(mt);
+        return 0;
+    }
+};
+
+// Counting-sort transpose; rows of the result come out sorted
+// (include/matrix_la.h:155-179).
+template <typename T>
+spMtx<T> transpose(const spMtx<T> &A) {
+    spMtx<T> R(A.n, A.m, A.nz);
+    std::memcpy(R.matcode, A.matcode, sizeof(MM_typecode));
+    for (size_t p = 0; p < A.nz; ++p) ++R.Rst(A.Col(p) + 1);
+    for (size_t i = 0; i < A.n; ++i) R.Rst(i + 1) += R.Rst(i);
+    std::vector<int> next(R.Rst.data(), R.Rst.data() + A.n);
+    for (size_t i = 0; i < A.m; ++i)
+        for (int p = A.Rst(i); p < A.Rst(i + 1); ++p) {
+            const int q = next[A.Col(p)]++;
+            R.Col(q) = (int)i;
+            R.Val(q) = A.Val(p);
+        }
+    return R;
+}
+
+// Same pattern, every value 1, integer matcode (include/matrix.h:643-660).
+template <typename T>
+spMtx<int> build_adjacency_matrix(const spMtx<T> &G) {
+    spMtx<int> R(G.m, G.n, G.nz);
+    std::copy(G.Rst.data(), G.Rst.data() + G.m + 1, R.Rst.data());
+    std::copy(G.Col.data(), G.Col.data() + G.nz, R.Col.data());
+    std::fill(R.Val.data(), R.Val.data() + G.nz, 1);
+    std::memcpy(R.matcode, G.matcode, sizeof(MM_typecode));
+    R.matcode[2] = 'I';
+    return R;
+}
+
+// Each row's leading run of columns below the diagonal; on sorted rows that
+// is the strictly lower part (include/matrix.h:698-726).
+template <typename T>
+spMtx<T> extract_lower_triangle(const spMtx<T> &G) {
+    spMtx<T> R(G.m, G.n);
+    std::memcpy(R.matcode, G.matcode, sizeof(MM_typecode));
+    for (size_t i = 0; i < G.m; ++i) {
+        int p = G.Rst(i);
+        while (p < G.Rst(i + 1) && (size_t)G.Col(p) < i) ++p;
+        R.Rst(i + 1) = R.Rst(i) + (p - G.Rst(i));
+    }
+    R.nz = (size_t)R.Rst(G.m);
+    R.Col = gk::View<int>("", R.nz);
+    R.Val = gk::View<T>("", R.nz);
+    for (size_t i = 0; i < G.m; ++i) {
+        const int len = R.Rst(i + 1) - R.Rst(i);
+        std::copy(G.Col.data() + G.Rst(i), G.Col.data() + G.Rst(i) + len, R.Col.data() + R.Rst(i));
+        std::copy(G.Val.data() + G.Rst(i), G.Val.data() + G.Rst(i) + len, R.Val.data() + R.Rst(i));
+    }
+    return R;
+}
+
+// Row-wise merge of the sorted rows of L and L^T (include/matrix.h:662-696).
+template <typename T>
+spMtx<T> build_symm_from_lower(const spMtx<T> &Low) {
+    spMtx<T> R(Low.m, Low.n, 2 * Low.nz);
+    std::memcpy(R.matcode, Low.matcode, sizeof(MM_typecode));
+    const spMtx<T> U = transpose(Low);
+    size_t w = 0;
+    for (size_t i = 0; i < Low.m; ++i) {
+        int a = Low.Rst(i), b = U.Rst(i);
+        const int ae = Low.Rst(i + 1), be = U.Rst(i + 1);
+        while (a < ae || b < be) {
+            const bool take_low = b >= be || (a < ae && Low.Col(a) < U.Col(b));
+            if (take_low) { R.Col(w) = Low.Col(a); R.Val(w++) = Low.Val(a++); }
+            else          { R.Col(w) = U.Col(b);   R.Val(w++) = U.Val(b++); }
+        }
+        R.Rst(i + 1) = (int)w;
+    }
+    return R;
+}

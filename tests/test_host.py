@@ -144,3 +144,30 @@ def test_structural_helpers_match_scipy():
     B = build_symm_from_lower(L)
     assert np.array_equal(B.Rst, S.indptr) and np.array_equal(B.Col, S.indices)
     del rng
+
+
+def test_structural_helpers_on_unsorted_rows():
+    # extract_lower_triangle keeps each row's leading run of columns below
+    # the diagonal (include/matrix.h:698-726), not every such column
+    A = SpMtx(4, 4, np.array([0, 0, 3, 7, 9], np.int32),
+              np.array([0, 3, 0, 1, 0, 3, 1, 3, 0], np.int32), np.arange(1, 10, dtype=np.int32))
+    L = extract_lower_triangle(A)
+    assert L.Rst.tolist() == [0, 0, 1, 3, 3]
+    assert L.Col.tolist() == [0, 1, 0] and L.Val.tolist() == [1, 4, 5]
+    # build_symm_from_lower merges each row of L with the same row of L^T
+    # (include/matrix.h:662-696); an unsorted L row stays in its order
+    L = SpMtx(4, 4, np.array([0, 0, 1, 3, 5], np.int32), np.array([0, 1, 0, 2, 0], np.int32),
+              np.ones(5, np.int32))
+    S = build_symm_from_lower(L)
+    assert S.Rst.tolist() == [0, 3, 5, 8, 10]
+    assert S.Col.tolist() == [1, 2, 3, 0, 2, 1, 0, 3, 2, 0]
+
+
+def test_dropin_headers_compile():
+    """The C++ drop-in headers (include/graphkokkos) compile warning-free as
+    a client program would include them."""
+    import subprocess
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Wextra", "-Werror", "-Wno-sign-compare",
+                        f"-I{ROOT / 'include'}", str(ROOT / "tests" / "cpp" / "dropin_check.cpp")],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
