@@ -262,6 +262,23 @@ __global__ __launch_bounds__(64 * kHeavyWaves) void k_bc_backward_heavy(
 }
 
 // Vertices with at least kHeavy entries in `rst` rows.
+// A source row that lists a neighbour v c > 1 times (a Matrix Market file
+// may repeat an edge) puts c copies of (v, s) into the reference's first
+// frontier (transpose(A.extractRows), include/betweenness_centrality.h:447),
+// and bc_backward_step (:236-239) adds (A*W)*Numsp once per copy, so v's
+// dependency on s counts c times.  sigma(v,s) = c there, so
+// delta = c * W(v,s) - 1 and the extra copies add (c - 1) * delta.
+// t holds (source column, v, c) triples.
+__global__ void k_bc_dup_fix(int32_t np, const int32_t *__restrict__ t, const double *__restrict__ S,
+                             double *__restrict__ bc) {
+    const int32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < np) {
+        const int32_t j = t[3 * p], v = t[3 * p + 1], c = t[3 * p + 2];
+        const double delta = (double)c * S[(int64_t)v * kSrc + j] - 1.0;
+        atomicAdd(&bc[v], (double)(c - 1) * delta);
+    }
+}
+
 __global__ void k_heavy_list(int64_t n, const int32_t *__restrict__ rst, int32_t *__restrict__ out,
                              int32_t *__restrict__ nout) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -298,6 +315,37 @@ struct Dev {
     }
 };
 
+// Repeated neighbours in the rows of the sources, per batch of kSrc
+// sources: (column in the batch, neighbour, multiplicity) for every
+// multiplicity above 1.  Only patterns whose rows are not strictly
+// increasing can repeat a column.
+int repeated_neighbours(gk_ctx *ctx, const gk_dcsr *A, const int32_t *sources, int64_t nsrc,
+                        std::vector<std::vector<int32_t>> *per_batch) {
+    per_batch->assign((size_t)((nsrc + kSrc - 1) / kSrc), {});
+    if (A->sorted || nsrc == 0) return GK_OK;
+    std::vector<int32_t> rst(A->m + 1), row;
+    GK_HIP(d2h(ctx, rst.data(), A->rst, 4 * (size_t)(A->m + 1)));
+    for (int64_t s = 0; s < nsrc; ++s) {
+        const int32_t v0 = sources[s], b = rst[v0], e = rst[v0 + 1];
+        if (e - b < 2) continue;
+        row.resize(e - b);
+        GK_HIP(d2h(ctx, row.data(), A->col + b, 4 * (size_t)(e - b)));
+        std::sort(row.begin(), row.end());
+        for (size_t q = 0; q < row.size();) {
+            size_t r = q;
+            while (r < row.size() && row[r] == row[q]) ++r;
+            if (r - q > 1 && row[q] != v0) {
+                auto &t = (*per_batch)[s / kSrc];
+                t.push_back((int32_t)(s % kSrc));
+                t.push_back(row[q]);
+                t.push_back((int32_t)(r - q));
+            }
+            q = r;
+        }
+    }
+    return GK_OK;
+}
+
 }  // namespace
 }  // namespace gk
 
@@ -312,9 +360,15 @@ extern "C" int gk_bc_batch_dev(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *AT,
     const int64_t n = A->m;
     Dev D;
     const int32_t *trst = AT ? AT->rst : nullptr, *tcol = AT ? AT->col : nullptr;
+    // a symmetric pattern is its own transpose (row order inside A^T does
+    // not matter to the forward sums)
+    if (!AT && A->symmetric) {
+        trst = A->rst;
+        tcol = A->col;
+    }
     Timer total(ctx);
     total.start();
-    if (!AT) {
+    if (!trst) {
         int32_t *cnt, *tr, *tc, *cur;
         GK_TRY(D.alloc(&cnt, n + 1));
         GK_TRY(D.alloc(&tr, n + 1));
@@ -355,6 +409,12 @@ extern "C" int gk_bc_batch_dev(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *AT,
     }
     int32_t nheavy[2];
     GK_HIP(d2h(ctx, nheavy, nh, 8));
+    std::vector<std::vector<int32_t>> dups;
+    GK_TRY(repeated_neighbours(ctx, A, sources, nsrc, &dups));
+    int32_t *ddup = nullptr;
+    size_t dup_max = 0;
+    for (auto &t : dups) dup_max = std::max(dup_max, t.size());
+    if (dup_max) GK_TRY(D.alloc(&ddup, dup_max));
     std::vector<uint64_t *> F;  // per-level frontier bitmaps, grown on demand
     unsigned long long *h;
     GK_TRY(pinned(ctx, 16, (void **)&h));
@@ -410,6 +470,13 @@ extern "C" int gk_bc_batch_dev(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *AT,
                     k_bc_backward_heavy<<<std::min(nheavy[1], ctx->num_cus * 2), 64 * kHeavyWaves, 0,
                                           ctx->stream>>>(nheavy[1], hout, A->rst, A->col, F[l], F[l + 1], S, d_bc);
                 GK_TRY(launch_check("k_bc_backward"));
+            }
+            const auto &t = dups[b0 / kSrc];
+            if (!t.empty()) {
+                const int32_t np = (int32_t)(t.size() / 3);
+                GK_HIP(hipMemcpyAsync(ddup, t.data(), 4 * t.size(), hipMemcpyHostToDevice, ctx->stream));
+                k_bc_dup_fix<<<(np + 255) / 256, 256, 0, ctx->stream>>>(np, ddup, S, d_bc);
+                GK_TRY(launch_check("k_bc_dup_fix"));
             }
         }
         tb.stop();

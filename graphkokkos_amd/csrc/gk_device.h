@@ -155,5 +155,62 @@ __device__ __forceinline__ T wave_incl_scan(T v) {
 
 __device__ __forceinline__ uint32_t hash32(uint32_t x) { return x * 2654435761u; }
 
+// Block-wide append of the values whose pred is set to out[], one atomic
+// per call on *counter (instead of one per wave: appends from every wave
+// of a large grid to one counter serialise in L2).  Every thread of the
+// block must call it; lds needs NT / 64 + 1 ints.
+template <int NT>
+__device__ __forceinline__ void block_append(bool pred, int32_t v, int32_t *__restrict__ out,
+                                             int32_t *__restrict__ counter, int32_t *lds) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t bal = __ballot(pred);
+    if (lane == 0) lds[w] = __popcll(bal);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int32_t tot = 0;
+#pragma unroll
+        for (int x = 0; x < NT / 64; ++x) {
+            const int32_t c = lds[x];
+            lds[x] = tot;
+            tot += c;
+        }
+        lds[NT / 64] = tot ? atomicAdd(counter, tot) : 0;
+    }
+    __syncthreads();
+    if (pred) out[lds[NT / 64] + lds[w] + __popcll(bal & ((1ull << lane) - 1))] = v;
+    __syncthreads();
+}
+
+// The same for IT candidates per thread (one atomic per NT * IT of them);
+// a thread's kept values land next to each other.
+template <int NT, int IT>
+__device__ __forceinline__ void block_append_n(const bool (&pred)[IT], const int32_t (&v)[IT],
+                                               int32_t *__restrict__ out, int32_t *__restrict__ counter,
+                                               int32_t *lds) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < IT; ++i) c += pred[i];
+    const int32_t incl = wave_incl_scan(c);
+    if (lane == 63) lds[w] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int32_t tot = 0;
+#pragma unroll
+        for (int x = 0; x < NT / 64; ++x) {
+            const int32_t t = lds[x];
+            lds[x] = tot;
+            tot += t;
+        }
+        lds[NT / 64] = tot ? atomicAdd(counter, tot) : 0;
+    }
+    __syncthreads();
+    int32_t o = lds[NT / 64] + lds[w] + incl - c;
+#pragma unroll
+    for (int i = 0; i < IT; ++i)
+        if (pred[i]) out[o++] = v[i];
+    __syncthreads();
+}
+
 }  // namespace dev
 }  // namespace gk

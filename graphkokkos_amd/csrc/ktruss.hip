@@ -202,35 +202,55 @@ __global__ void k_iota(int64_t n, int32_t *__restrict__ a) {
 // Frontier at a new threshold: alive edges with support below `thresh`,
 // appended to F (their states and labels are set by the round prologue),
 // and the minimum support over the alive edges (as its complement in *mn).
-__global__ void k_frontier(int64_t na, const int32_t *__restrict__ alist, const uint8_t *__restrict__ st,
-                           const int32_t *__restrict__ sup, int32_t thresh, int32_t *__restrict__ F,
-                           int32_t *__restrict__ nF, uint32_t *__restrict__ mn) {
-    const int lane = threadIdx.x & 63;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+// With `keep`, the alive edges at or above the threshold whose support is
+// below keep_hi are appended there too (the near list of the next levels).
+constexpr int kScanIt = 4;   // entries per thread per pass of the list scans
+
+__global__ __launch_bounds__(256) void k_frontier(int64_t na, const int32_t *__restrict__ alist,
+                                                  const uint8_t *__restrict__ st, const int32_t *__restrict__ sup,
+                                                  int32_t thresh, int32_t *__restrict__ F, int32_t *__restrict__ nF,
+                                                  uint32_t *__restrict__ mn, int32_t *__restrict__ keep,
+                                                  int32_t *__restrict__ nkeep, int32_t keep_hi) {
+    __shared__ int32_t lds[256 / 64 + 1];
+    __shared__ uint32_t wlo[256 / 64];
+    constexpr int CH = 256 * kScanIt;
+    const int64_t stride = (int64_t)gridDim.x * CH;
     uint32_t lo = 0xffffffffu;
-    for (int64_t base = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63); base < na; base += stride) {
-        int64_t x = base + lane;
-        bool take = false;
-        int32_t e = 0;
-        if (x < na) {
-            e = alist[x];
-            if (st[e] == 0) {
-                const int32_t v = sup[e];
+    // block-uniform trip count: every thread takes part in the appends
+    for (int64_t base = (int64_t)blockIdx.x * CH; base < na; base += stride) {
+        bool take[kScanIt], kp[kScanIt];
+        int32_t e[kScanIt];
+        int any_take = 0, any_keep = 0;
+#pragma unroll
+        for (int i = 0; i < kScanIt; ++i) {
+            const int64_t x = base + i * 256 + threadIdx.x;
+            take[i] = kp[i] = false;
+            e[i] = x < na ? alist[x] : 0;
+        }
+#pragma unroll
+        for (int i = 0; i < kScanIt; ++i) {
+            const int64_t x = base + i * 256 + threadIdx.x;
+            if (x < na && st[e[i]] == 0) {
+                const int32_t v = sup[e[i]];
                 lo = min(lo, (uint32_t)v);
-                take = v < thresh;
+                take[i] = v < thresh;
+                kp[i] = keep && !take[i] && v < keep_hi;
+                any_take |= take[i];
+                any_keep |= kp[i];
             }
         }
-        uint64_t bal = __ballot(take);
-        if (!bal) continue;
-        int32_t first = 0;
-        if (lane == 0) first = atomicAdd(nF, __popcll(bal));
-        first = __shfl(first, 0, 64);
-        if (take) F[first + __popcll(bal & ((1ull << lane) - 1))] = e;
+        if (__syncthreads_or(any_take)) dev::block_append_n<256, kScanIt>(take, e, F, nF, lds);
+        if (keep && __syncthreads_or(any_keep)) dev::block_append_n<256, kScanIt>(kp, e, keep, nkeep, lds);
     }
     lo = dev::wave_max(~lo);
-    if (lane == 0 && lo) atomicMax(mn, lo);
+    if ((threadIdx.x & 63) == 0) wlo[threadIdx.x >> 6] = lo;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t b = 0;
+        for (int w = 0; w < 256 / 64; ++w) b = max(b, wlo[w]);
+        if (b) atomicMax(mn, b);
+    }
 }
-
 
 // ---- static adjacency hash for the peeling rounds ------------------------
 // Vertex v's adjacency gets an open-addressing table of cap(v) = the
@@ -317,38 +337,11 @@ __global__ void k_round_prologue(int32_t nP, const int32_t *__restrict__ Fp, int
     if (blockIdx.x == 0 && threadIdx.x == 0) len[nF] = 0;
 }
 
-// The same for frontiers of at most kSmallRound edges in one block, with
-// the exclusive scan of len written straight to pre.
+// Frontiers of at most kSmallRound edges get their prologue and the scan of
+// len in one block (k_dround_prologue).
 constexpr int kSmallRoundThreads = 1024;
 constexpr int kSmallRoundPer = 4;
 constexpr int kSmallRound = kSmallRoundThreads * kSmallRoundPer;
-__global__ __launch_bounds__(kSmallRoundThreads) void k_round_small(
-    int32_t nP, const int32_t *__restrict__ Fp, int32_t nF, const int32_t *__restrict__ F, uint8_t *__restrict__ st,
-    uint32_t *__restrict__ truss, uint32_t label, const int32_t *__restrict__ eu, const int32_t *__restrict__ ev,
-    const int32_t *__restrict__ rst, int32_t *__restrict__ fy, int32_t *__restrict__ fx, int32_t *__restrict__ pre) {
-    __shared__ int32_t wsum[kSmallRoundThreads / 64];
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    for (int32_t f = tid; f < nP; f += kSmallRoundThreads) st[Fp[f]] = 2;
-    int32_t l[kSmallRoundPer], run = 0;
-#pragma unroll
-    for (int j = 0; j < kSmallRoundPer; ++j) {
-        const int32_t f = tid * kSmallRoundPer + j;
-        l[j] = f < nF ? prep_edge(f, F, st, truss, label, eu, ev, rst, fy, fx) : 0;
-        run += l[j];
-    }
-    const int32_t incl = dev::wave_incl_scan(run);
-    if (lane == 63) wsum[w] = incl;
-    __syncthreads();
-    int32_t before = 0;
-    for (int x = 0; x < w; ++x) before += wsum[x];
-    int32_t ex = before + incl - run;
-#pragma unroll
-    for (int j = 0; j < kSmallRoundPer; ++j) {
-        const int32_t f = tid * kSmallRoundPer + j;
-        if (f <= nF) pre[f] = ex;
-        ex += l[j];
-    }
-}
 
 // Support update of one round, as the masked product dS<A> = F * A with the
 // adjacency tables as the mask accumulator: the entries w of every frontier
@@ -369,6 +362,34 @@ struct RoundCtl {
     int32_t pad;
 };
 
+// Near list: the alive edges whose support is below `hi` (plus removed
+// ones, skipped when it is scanned).  The peeling kernels append an edge
+// whose support drops from hi to hi - 1; hi = INT32_MAX disables it.
+struct NearList {
+    int32_t *list;
+    int32_t *n;
+    int32_t hi;
+};
+
+// Appends the non-negative entries of every lane's v to out, one atomic on
+// *counter per wave (none when the wave has nothing).
+template <int N>
+__device__ __forceinline__ void wave_append(const int32_t (&v)[N], int32_t *__restrict__ out,
+                                            int32_t *__restrict__ counter) {
+    int32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) c += v[i] >= 0;
+    if (!__ballot(c > 0)) return;
+    const int32_t incl = dev::wave_incl_scan(c);
+    int32_t base = 0;
+    if ((threadIdx.x & 63) == 63) base = atomicAdd(counter, incl);
+    base = __shfl(base, 63, 64);
+    int32_t o = base + incl - c;
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+        if (v[i] >= 0) out[o++] = v[i];
+}
+
 template <int U>
 __device__ __forceinline__ void k_peel_body(int32_t nF, const int32_t *__restrict__ pre,
                                                      const int32_t *__restrict__ F, const int32_t *__restrict__ fy,
@@ -377,7 +398,8 @@ __device__ __forceinline__ void k_peel_body(int32_t nF, const int32_t *__restric
                                                      const int64_t *__restrict__ hoff, const int2 *__restrict__ tab,
                                                      const uint8_t *__restrict__ st, int32_t *__restrict__ sup,
                                                      int32_t thresh, int32_t *__restrict__ Fn,
-                                                     int32_t *__restrict__ nFn, unsigned long long *__restrict__ ctr) {
+                                                     int32_t *__restrict__ nFn, const NearList nl,
+                                                     unsigned long long *__restrict__ ctr) {
     constexpr int CH = 64 * U;
     const int lane = threadIdx.x & 63;
     const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -421,6 +443,10 @@ __device__ __forceinline__ void k_peel_body(int32_t nF, const int32_t *__restric
                 w[u] = col[t[u]];
                 e2[u] = eid[t[u]];
             }
+        // edges leaving the threshold / entering the near list (-1: none)
+        int32_t xf[2 * U], xn[2 * U];
+#pragma unroll
+        for (int u = 0; u < 2 * U; ++u) xf[u] = xn[u] = -1;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             if (t[u] < 0) continue;
@@ -434,10 +460,21 @@ __device__ __forceinline__ void k_peel_body(int32_t nF, const int32_t *__restric
             ++cnt;
             // an alive edge whose support falls below the threshold right
             // now (old value == thresh; supports drop by one at a time)
-            // joins the next round's frontier
-            if (s1 == 0 && atomicSub(&sup[a], 1) == thresh) Fn[atomicAdd(nFn, 1)] = a;
-            if (s2 == 0 && atomicSub(&sup[b], 1) == thresh) Fn[atomicAdd(nFn, 1)] = b;
+            // joins the next round's frontier; one falling below the near
+            // list's bound joins the near list
+            if (s1 == 0) {
+                const int32_t old = atomicSub(&sup[a], 1);
+                if (old == thresh) xf[2 * u] = a;
+                else if (old == nl.hi) xn[2 * u] = a;
+            }
+            if (s2 == 0) {
+                const int32_t old = atomicSub(&sup[b], 1);
+                if (old == thresh) xf[2 * u + 1] = b;
+                else if (old == nl.hi) xn[2 * u + 1] = b;
+            }
         }
+        wave_append(xf, Fn, nFn);
+        wave_append(xn, nl.list, nl.n);
     }
     unsigned long long s = dev::wave_sum((unsigned long long)cnt);
     if (lane == 0 && s) atomicAdd(&ctr[1], s);
@@ -452,11 +489,12 @@ __global__ __launch_bounds__(256) void k_peel_stream(int32_t nF, const int32_t *
                                                      const int64_t *__restrict__ hoff, const int2 *__restrict__ tab,
                                                      const uint8_t *__restrict__ st, int32_t *__restrict__ sup,
                                                      int32_t thresh, int32_t *__restrict__ Fn,
-                                                     int32_t *__restrict__ nFn, unsigned long long *__restrict__ ctr) {
-    k_peel_body<U>(nF, pre, F, fy, fx, rst, col, eid, hoff, tab, st, sup, thresh, Fn, nFn, ctr);
+                                                     int32_t *__restrict__ nFn, const NearList nl,
+                                                     unsigned long long *__restrict__ ctr) {
+    k_peel_body<U>(nF, pre, F, fy, fx, rst, col, eid, hoff, tab, st, sup, thresh, Fn, nFn, nl, ctr);
 }
 
-// Device-driven round: prologue + scan in one block (as k_round_small),
+// Device-driven round: prologue + scan in one block,
 // sizes from the control block.  A frontier that is empty or too big for
 // one block stops the batch; the remaining kernels of the batch return.
 __global__ __launch_bounds__(kSmallRoundThreads) void k_dround_prologue(
@@ -506,12 +544,12 @@ __global__ __launch_bounds__(256) void k_dround_stream(const RoundCtl *__restric
                                                        const int64_t *__restrict__ hoff, const int2 *__restrict__ tab,
                                                        const uint8_t *__restrict__ st, int32_t *__restrict__ sup,
                                                        int32_t thresh, int32_t *__restrict__ nFn,
-                                                       unsigned long long *__restrict__ ctr) {
+                                                       const NearList nl, unsigned long long *__restrict__ ctr) {
     if (ctl->stop) return;
     const int32_t cur = ctl->cur;
     // the crossings go to buf[cur ^ 1], counted in ctl->nFn
     k_peel_body<U>(ctl->nF, pre, cur ? buf1 : buf0, fy, fx, rst, col, eid, hoff, tab, st, sup, thresh,
-                   cur ? buf0 : buf1, nFn, ctr);
+                   cur ? buf0 : buf1, nFn, nl, ctr);
 }
 
 __global__ void k_dround_advance(RoundCtl *__restrict__ ctl) {
@@ -563,21 +601,30 @@ __global__ void k_alive_scatter(int64_t m, const int32_t *__restrict__ rst, cons
     }
 }
 
-__global__ void k_alist_compact(int64_t na, const int32_t *__restrict__ alist, const uint8_t *__restrict__ st,
-                                int32_t *__restrict__ out, int32_t *__restrict__ nout) {
-    const int lane = threadIdx.x & 63;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t base = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63); base < na; base += stride) {
-        int64_t x = base + lane;
-        int32_t e = x < na ? alist[x] : 0;
-        bool k = x < na && st[e] == 0;
-        uint64_t bal = __ballot(k);
-        if (!bal) continue;
-        int32_t first = 0;
-        if (lane == 0) first = atomicAdd(nout, __popcll(bal));
-        first = __shfl(first, 0, 64);
-        if (k) out[first + __popcll(bal & ((1ull << lane) - 1))] = e;
+__global__ __launch_bounds__(256) void k_alist_compact(int64_t na, const int32_t *__restrict__ alist,
+                                                       const uint8_t *__restrict__ st, int32_t *__restrict__ out,
+                                                       int32_t *__restrict__ nout) {
+    __shared__ int32_t lds[256 / 64 + 1];
+    constexpr int CH = 256 * kScanIt;
+    const int64_t stride = (int64_t)gridDim.x * CH;
+    for (int64_t base = (int64_t)blockIdx.x * CH; base < na; base += stride) {
+        bool k[kScanIt];
+        int32_t e[kScanIt];
+#pragma unroll
+        for (int i = 0; i < kScanIt; ++i) {
+            const int64_t x = base + i * 256 + threadIdx.x;
+            e[i] = x < na ? alist[x] : 0;
+        }
+#pragma unroll
+        for (int i = 0; i < kScanIt; ++i) k[i] = base + i * 256 + threadIdx.x < na && st[e[i]] == 0;
+        dev::block_append_n<256, kScanIt>(k, e, out, nout, lds);
     }
+}
+
+// Blocks for a scan of n list entries (kScanIt per thread).
+int grid_scan(gk_ctx *ctx, int64_t n) {
+    return (int)std::max<int64_t>(1, std::min<int64_t>((n + 256 * kScanIt - 1) / (256 * kScanIt),
+                                                       (int64_t)ctx->num_cus * 8));
 }
 
 int grid_items(gk_ctx *ctx, int64_t n) {
@@ -624,6 +671,10 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
     GK_TRY(B.alloc(&dn, 2));
     GK_TRY(B.alloc(&dctr, 2));
     GK_TRY(B.alloc(&dctl, 1));
+    int32_t *NL, *NL2, *dnl;
+    GK_TRY(B.alloc(&NL, nL));
+    GK_TRY(B.alloc(&NL2, nL));
+    GK_TRY(B.alloc(&dnl, 1));
     for (int b = 0; b < 2; ++b) {
         GK_TRY(B.alloc(&arst[b], m + 1));
         GK_TRY(B.alloc(&acol[b], A->nz));
@@ -682,37 +733,68 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
     int fc = 0;
     int32_t nP = 0;              // previous frontier, still to be marked removed
     int32_t nF = 0;              // current frontier
-    bool scan_needed = true;     // the frontier comes from a full scan
+    bool scan_needed = true;     // the frontier comes from a scan
     bool started = false;
     constexpr int kBatch = 16;   // device-driven rounds per host check
+    // Near list (NearList): alive edges with support below nl_hi, built by a
+    // full scan of the alive list and kept complete by the peeling kernels.
+    // While a level's threshold stays below nl_hi its frontier comes from
+    // the near list, compacted as it is scanned.  The window nl_hi - thresh
+    // adapts so the list holds about an eighth of the alive list.
+    bool nl_valid = false;
+    int32_t nl_hi = INT32_MAX;
+    int64_t nNL = 0, nl_window = 8;
+    auto near_list = [&]() { return NearList{NL, dnl, nl_valid ? nl_hi : INT32_MAX}; };
+    // one scan at threshold thr: nF, the minimum support over the scanned
+    // edges (h[1], complemented) and the near list's new length
+    auto scan = [&](bool full, int32_t thr) -> int {
+        GK_HIP(hipMemsetAsync(dn, 0, 4, ctx->stream));
+        GK_HIP(hipMemsetAsync(dmin, 0, 4, ctx->stream));
+        GK_HIP(hipMemsetAsync(dnl, 0, 4, ctx->stream));
+        if (full) {
+            const int32_t hi = fixed_k > 0 ? thr : (int32_t)std::min<int64_t>((int64_t)thr + nl_window, INT32_MAX - 1);
+            k_frontier<<<grid_scan(ctx, na), 256, 0, ctx->stream>>>(na, alist, st, sup, thr, fb[fc], dn, dmin,
+                                                                     fixed_k > 0 ? nullptr : NL, dnl, hi);
+            nl_hi = hi;
+        } else {
+            k_frontier<<<grid_scan(ctx, nNL), 256, 0, ctx->stream>>>(nNL, NL, st, sup, thr, fb[fc], dn, dmin, NL2,
+                                                                      dnl, INT32_MAX);
+            std::swap(NL, NL2);
+        }
+        GK_TRY(launch_check("k_frontier"));
+        GK_HIP(hipMemcpyAsync(h, dn, 4, hipMemcpyDeviceToHost, ctx->stream));
+        GK_HIP(hipMemcpyAsync(h + 1, dmin, 4, hipMemcpyDeviceToHost, ctx->stream));
+        GK_HIP(hipMemcpyAsync(h + 2, dnl, 4, hipMemcpyDeviceToHost, ctx->stream));
+        GK_HIP(hipStreamSynchronize(ctx->stream));
+        nF = h[0];
+        nNL = h[2];
+        if (full && fixed_k == 0) {
+            nl_valid = true;
+            if (nNL * 16 < na) nl_window = std::min<int64_t>(nl_window * 2, 1 << 16);
+            else if (nNL * 4 > na) nl_window = std::max<int64_t>(nl_window / 2, 2);
+        }
+        return GK_OK;
+    };
     while (alive > 0) {
         if (scan_needed) {
             // A level is finished (or the run starts): every alive edge has
             // support >= k-2.  The next frontier is the alive edges at the
             // minimum support s, at threshold k' - 2 with k' = max(k+1, s+3)
-            // (the start uses k itself).  One pass collects the edges below
-            // k'-2 for k' = k (+1) and the minimum; a second pass is needed
-            // only when that finds nothing.
+            // (the start uses k itself).  One scan collects the edges below
+            // k'-2 for k' = k (+1) and the minimum; another is needed only
+            // when that finds nothing.
             const int kt = started ? k + 1 : k;
-            GK_HIP(hipMemsetAsync(dn, 0, 4, ctx->stream));
-            GK_HIP(hipMemsetAsync(dmin, 0, 4, ctx->stream));
-            k_frontier<<<grid_items(ctx, na), 256, 0, ctx->stream>>>(na, alist, st, sup, kt - 2, fb[fc], dn, dmin);
-            GK_TRY(launch_check("k_frontier"));
-            GK_HIP(hipMemcpyAsync(h, dn, 4, hipMemcpyDeviceToHost, ctx->stream));
-            GK_HIP(hipMemcpyAsync(h + 1, dmin, 4, hipMemcpyDeviceToHost, ctx->stream));
-            GK_HIP(hipStreamSynchronize(ctx->stream));
-            nF = h[0];
+            bool full = !(nl_valid && kt - 2 < nl_hi);
+            GK_TRY(scan(full, kt - 2));
             if (fixed_k > 0 || nF > 0) {
                 k = kt;
             } else {
+                // an empty near list leaves the minimum to a full scan
+                if (!full && nNL == 0) GK_TRY(scan(true, kt - 2));
                 const uint32_t mn = ~(uint32_t)h[1];
                 k = (int)std::max<int64_t>(kt + 1, (int64_t)mn + 3);
-                GK_HIP(hipMemsetAsync(dn, 0, 4, ctx->stream));
-                k_frontier<<<grid_items(ctx, na), 256, 0, ctx->stream>>>(na, alist, st, sup, k - 2, fb[fc], dn, dmin);
-                GK_TRY(launch_check("k_frontier"));
-                GK_HIP(hipMemcpyAsync(h, dn, 4, hipMemcpyDeviceToHost, ctx->stream));
-                GK_HIP(hipStreamSynchronize(ctx->stream));
-                nF = h[0];
+                full = !(nl_valid && k - 2 < nl_hi);
+                GK_TRY(scan(full, k - 2));
             }
             started = true;
             scan_needed = false;
@@ -737,11 +819,14 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
             GK_HIP(hipMemsetAsync(dn, 0, 4, ctx->stream));
             GK_HIP(hipEventRecord(ev_pool[e0], ctx->stream));
             k_peel_stream<4><<<gs, 256, 0, ctx->stream>>>(nF, fpre, fb[fc], fy, fx, arst[cur], acol[cur], aeid[cur],
-                                                          hoff, tab, st, sup, k - 2, fb[fc ^ 1], dn, dctr);
+                                                          hoff, tab, st, sup, k - 2, fb[fc ^ 1], dn, near_list(),
+                                                          dctr);
             GK_HIP(hipEventRecord(ev_pool[e0 + 1], ctx->stream));
             GK_TRY(launch_check("k_peel_stream"));
             GK_HIP(hipMemcpyAsync(h, dn, 4, hipMemcpyDeviceToHost, ctx->stream));
+            GK_HIP(hipMemcpyAsync(h + 2, dnl, 4, hipMemcpyDeviceToHost, ctx->stream));
             GK_HIP(hipStreamSynchronize(ctx->stream));
+            nNL = h[2];
             marked = nP;
             alive -= nF;
             nP = nF;
@@ -760,13 +845,15 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
                 GK_HIP(hipEventRecord(ev_pool[e0], ctx->stream));
                 k_dround_stream<4><<<gsd, 256, 0, ctx->stream>>>(dctl, fb[0], fb[1], fpre, fy, fx, arst[cur],
                                                                  acol[cur], aeid[cur], hoff, tab, st, sup, k - 2,
-                                                                 &dctl->nFn, dctr);
+                                                                 &dctl->nFn, near_list(), dctr);
                 GK_HIP(hipEventRecord(ev_pool[e0 + 1], ctx->stream));
                 k_dround_advance<<<1, 64, 0, ctx->stream>>>(dctl);
             }
             GK_TRY(launch_check("device rounds"));
             GK_HIP(hipMemcpyAsync(hctl, dctl, sizeof(RoundCtl), hipMemcpyDeviceToHost, ctx->stream));
+            GK_HIP(hipMemcpyAsync(h + 2, dnl, 4, hipMemcpyDeviceToHost, ctx->stream));
             GK_HIP(hipStreamSynchronize(ctx->stream));
+            nNL = h[2];
             rounds += hctl->rounds;
             alive -= hctl->removed;
             marked = (int64_t)nP + hctl->removed - hctl->nP;
@@ -778,7 +865,7 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
         // the alive list is compacted once half of it is gone
         if (alive > 0 && 2 * (alive - nF) < na && !(2 * (dead_pending + nP) * 4 > aal_nz)) {
             GK_HIP(hipMemsetAsync(dn + 1, 0, 4, ctx->stream));
-            k_alist_compact<<<grid_items(ctx, na), 256, 0, ctx->stream>>>(na, alist, st, alist2, dn + 1);
+            k_alist_compact<<<grid_scan(ctx, na), 256, 0, ctx->stream>>>(na, alist, st, alist2, dn + 1);
             GK_TRY(launch_check("k_alist_compact"));
             GK_HIP(hipMemcpyAsync(h + 1, dn + 1, 4, hipMemcpyDeviceToHost, ctx->stream));
             GK_HIP(hipStreamSynchronize(ctx->stream));
@@ -795,7 +882,7 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
             k_alive_scatter<<<gw, 256, 0, ctx->stream>>>(m, arst[cur], acol[cur], aeid[cur], st, arst[nx], acol[nx],
                                                          aeid[nx]);
             GK_HIP(hipMemsetAsync(dn + 1, 0, 4, ctx->stream));
-            k_alist_compact<<<grid_items(ctx, na), 256, 0, ctx->stream>>>(na, alist, st, alist2, dn + 1);
+            k_alist_compact<<<grid_scan(ctx, na), 256, 0, ctx->stream>>>(na, alist, st, alist2, dn + 1);
             GK_TRY(launch_check("compaction"));
             GK_HIP(hipMemcpyAsync(h, arst[nx] + m, 4, hipMemcpyDeviceToHost, ctx->stream));
             GK_HIP(hipMemcpyAsync(h + 1, dn + 1, 4, hipMemcpyDeviceToHost, ctx->stream));

@@ -251,3 +251,25 @@ def test_bc_edge_cases(gpu):
     G = HostGraph.from_edges(9, list(range(8)), list(range(1, 9)))  # path
     got = gpu.bc_batch_f64(G.A, np.arange(9))
     assert np.allclose(got, [2 * v * (8 - v) for v in range(9)], rtol=1e-12)
+
+
+def test_bc_repeated_edges(gpu):
+    say("bc repeated edges")
+    # path 0 - 1 - 2 with the edge 0-1 listed twice (see the oracle's test):
+    # the reference counts vertex 1's dependency on source 0 once per copy
+    A = SpMtx(3, 3, np.array([0, 2, 5, 6], np.int32), np.array([1, 1, 0, 0, 2, 1], np.int32))
+    assert gpu.bc_batch_f64(A, [0]).tolist() == [0.0, 2.0, 0.0]
+    assert gpu.bc_batch_f64(A, [2]).tolist() == [0.0, 1.0, 0.0]
+    # a larger pattern with unsorted rows and ~10% repeated entries, two
+    # batches of sources
+    rng = np.random.default_rng(7)
+    G = HostGraph.rmat(10, 8, seed=7)
+    rows = np.repeat(np.arange(G.n), np.diff(G.A.Rst))
+    e = np.stack([rows, G.A.Col], 1)
+    e = np.concatenate([e, e[rng.choice(len(e), len(e) // 10, replace=False)]])
+    e = e[rng.permutation(len(e))]
+    e = e[np.argsort(e[:, 0], kind="stable")]
+    rst = np.concatenate([[0], np.cumsum(np.bincount(e[:, 0], minlength=G.n))]).astype(np.int32)
+    D = SpMtx(G.n, G.n, rst, e[:, 1].astype(np.int32))
+    src = rng.choice(np.flatnonzero(np.diff(rst) > 0), size=300, replace=False).astype(np.int32)
+    assert_bc_close(gpu.bc_batch_f64(D, src), O.bc_batch(D, src))

@@ -99,6 +99,21 @@ def test_path_and_star_bc():
     assert bc[0] == 42 and not bc[1:].any()
 
 
+def test_bc_repeated_edge_counts_per_copy():
+    # Path 0 - 1 - 2 with the edge 0-1 listed twice, source 0.  sigma(0,1) = 2
+    # and sigma(0,2) = 2 (each copy is a path), so delta_0(1) = 2/2 * (1 + 0) = 1.
+    # The reference's first frontier holds (1, source 0) twice and
+    # bc_backward_step adds (A*W)*Numsp once per copy
+    # (include/betweenness_centrality.h:236-239, 447), so vertex 1 gets 2.
+    A = SpMtx(3, 3, np.array([0, 2, 5, 6], np.int32), np.array([1, 1, 0, 0, 2, 1], np.int32))
+    assert O.bc_batch(A, [0]).tolist() == [0.0, 2.0, 0.0]
+    # textbook Brandes on the multigraph counts the dependency once
+    assert O.brandes_bc(A, [0]).tolist() == [0.0, 1.0, 0.0]
+    # from source 2 the repeated edge is one hop further out: no per-copy
+    # effect, sigma(2,0) = 2 and delta_2(1) = 1
+    assert O.bc_batch(A, [2]).tolist() == [0.0, 1.0, 0.0]
+
+
 # ---- brute force ------------------------------------------------------------
 
 @pytest.mark.parametrize("seed", range(4))
