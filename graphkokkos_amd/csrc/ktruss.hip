@@ -309,12 +309,15 @@ __device__ __forceinline__ int32_t adj_probe(const int64_t *__restrict__ hoff, c
 // (state 2); the new frontier F becomes state 1 with truss label `label`.
 // Per frontier edge f = (y, x): y = endpoint with the longer current row
 // (probed through its table), x = the other (its alive row is streamed);
-// len[f] = current row length of x.
+// len[f] = current row length of x, or 0 when f's support is 0: the
+// support of a frontier edge counts its triangles that are still intact
+// (every decrement of earlier rounds has landed), so such an edge closes
+// no triangle this round and its row need not be streamed.
 __device__ __forceinline__ int32_t prep_edge(int32_t f, const int32_t *__restrict__ F, uint8_t *__restrict__ st,
                                              uint32_t *__restrict__ truss, uint32_t label,
                                              const int32_t *__restrict__ eu, const int32_t *__restrict__ ev,
-                                             const int32_t *__restrict__ rst, int32_t *__restrict__ fy,
-                                             int32_t *__restrict__ fx) {
+                                             const int32_t *__restrict__ sup, const int32_t *__restrict__ rst,
+                                             int32_t *__restrict__ fy, int32_t *__restrict__ fx) {
     const int32_t e = F[f], u = eu[e], v = ev[e];
     st[e] = 1;
     truss[e] = label;
@@ -322,18 +325,19 @@ __device__ __forceinline__ int32_t prep_edge(int32_t f, const int32_t *__restric
     const bool uy = du > dv || (du == dv && u > v);
     fy[f] = uy ? u : v;
     fx[f] = uy ? v : u;
-    return uy ? dv : du;
+    return sup[e] > 0 ? (uy ? dv : du) : 0;
 }
 
 __global__ void k_round_prologue(int32_t nP, const int32_t *__restrict__ Fp, int32_t nF,
                                  const int32_t *__restrict__ F, uint8_t *__restrict__ st,
                                  uint32_t *__restrict__ truss, uint32_t label, const int32_t *__restrict__ eu,
-                                 const int32_t *__restrict__ ev, const int32_t *__restrict__ rst,
-                                 int32_t *__restrict__ fy, int32_t *__restrict__ fx, int32_t *__restrict__ len) {
+                                 const int32_t *__restrict__ ev, const int32_t *__restrict__ sup,
+                                 const int32_t *__restrict__ rst, int32_t *__restrict__ fy, int32_t *__restrict__ fx,
+                                 int32_t *__restrict__ len) {
     const int32_t stride = gridDim.x * blockDim.x;
     for (int32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < nP; f += stride) st[Fp[f]] = 2;
     for (int32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < nF; f += stride)
-        len[f] = prep_edge(f, F, st, truss, label, eu, ev, rst, fy, fx);
+        len[f] = prep_edge(f, F, st, truss, label, eu, ev, sup, rst, fy, fx);
     if (blockIdx.x == 0 && threadIdx.x == 0) len[nF] = 0;
 }
 
@@ -500,7 +504,8 @@ __global__ __launch_bounds__(256) void k_peel_stream(int32_t nF, const int32_t *
 __global__ __launch_bounds__(kSmallRoundThreads) void k_dround_prologue(
     RoundCtl *__restrict__ ctl, int32_t *__restrict__ buf0, int32_t *__restrict__ buf1, uint8_t *__restrict__ st,
     uint32_t *__restrict__ truss, uint32_t label, const int32_t *__restrict__ eu, const int32_t *__restrict__ ev,
-    const int32_t *__restrict__ rst, int32_t *__restrict__ fy, int32_t *__restrict__ fx, int32_t *__restrict__ pre) {
+    const int32_t *__restrict__ sup, const int32_t *__restrict__ rst, int32_t *__restrict__ fy,
+    int32_t *__restrict__ fx, int32_t *__restrict__ pre) {
     __shared__ int32_t wsum[kSmallRoundThreads / 64];
     if (ctl->stop) return;
     const int32_t nF = ctl->nF, nP = ctl->nP;
@@ -516,7 +521,7 @@ __global__ __launch_bounds__(kSmallRoundThreads) void k_dround_prologue(
 #pragma unroll
     for (int j = 0; j < kSmallRoundPer; ++j) {
         const int32_t f = tid * kSmallRoundPer + j;
-        l[j] = f < nF ? prep_edge(f, F, st, truss, label, eu, ev, rst, fy, fx) : 0;
+        l[j] = f < nF ? prep_edge(f, F, st, truss, label, eu, ev, sup, rst, fy, fx) : 0;
         run += l[j];
     }
     const int32_t incl = dev::wave_incl_scan(run);
@@ -811,7 +816,7 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
             // one large round driven from the host
             ++rounds;
             k_round_prologue<<<grid_items(ctx, std::max(nF, nP)), 256, 0, ctx->stream>>>(
-                nP, fb[fc ^ 1], nF, fb[fc], st, tl, label, eu, ev, arst[cur], fy, fx, flen);
+                nP, fb[fc ^ 1], nF, fb[fc], st, tl, label, eu, ev, sup, arst[cur], fy, fx, flen);
             GK_TRY(exclusive_scan_i32(ctx, flen, fpre, nF + 1));
             GK_TRY(launch_check("round prologue"));
             size_t e0;
@@ -839,7 +844,7 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
             const int gsd = std::min(gs, 1024);
             for (int r = 0; r < kBatch; ++r) {
                 k_dround_prologue<<<1, kSmallRoundThreads, 0, ctx->stream>>>(dctl, fb[0], fb[1], st, tl, label, eu,
-                                                                            ev, arst[cur], fy, fx, fpre);
+                                                                            ev, sup, arst[cur], fy, fx, fpre);
                 size_t e0;
                 GK_TRY(ev_pair(&e0));
                 GK_HIP(hipEventRecord(ev_pool[e0], ctx->stream));
