@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark of the masked-product path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--legs ktruss,tc,bc] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--legs ktruss,tc,bc,batch] [--no-cpu]
 
 Headline (`value`): full truss decomposition (k = 3..max) of the seeded
 R-MAT scale-20 graph (edge factor 16, a/b/c/d = .57/.19/.19/.05,
@@ -14,7 +14,10 @@ Also reported, each in its own object:
       edges/s, fixed-threshold selector and learned selector when a trained
       model is present;
   bc  batched Brandes BC, 4 batches of 256 seeded sources on the scale-20
-      graph, FP64, source-edges/s.
+      graph, FP64, source-edges/s;
+  batch  triangle count + 4-truss of each of 4096 seeded uneven graphs in one
+      hipGraph-captured step, edges/s, knobs autotuned (switched off: see
+      run_batch).
 `roofline` is for the dominant kernel of the headline run (the accumulator
 kernel with the most device time): algorithmic bytes per launch over its
 average launch duration (HIP events on the library's stream).
@@ -129,6 +132,74 @@ def bc_bytes(n: int, nnz: int, levels: int, nsrc: int) -> float:
     return 2 * levels * (4 * (n + 1) + 4 * nnz) + 24.0 * nsrc * nnz + 24.0 * nsrc * n
 
 
+def run_batch(ctx, a) -> dict:
+    """Triangle count + 4-truss of every graph of a seeded batch of uneven
+    graphs (vertex counts log-uniform in [1024, 65536], half Erdos-Renyi with
+    mean degree U[8,32], half Chung-Lu with exponent U[2.1,2.8] and mean
+    degree 16), as one hipGraph-captured step; knobs autotuned."""
+    # The full-size run (4096 graphs, a 4.4 GB union CSR) faulted the GPU once
+    # and the cause is not established; the leg stays switched off until it is.
+    raise RuntimeError("the batching leg is switched off: its full-size run faulted the GPU and the cause "
+                       "is not established (the batching tests at small sizes run)")
+    from graphkokkos_amd import HostGraph
+    from graphkokkos_amd import selector as SEL
+    from graphkokkos_amd.graph import batch_config
+    t0 = time.perf_counter()
+    cfg = batch_config(a.batch_graphs, seed=a.seed)
+    U = HostGraph.batch(**cfg)
+    E = U.edges
+    log(f"batch: {a.batch_graphs} graphs, {U.n} vertices, {E} edges built in {time.perf_counter() - t0:.1f}s")
+    dA = ctx.upload(U.A)
+    voff = U.voff
+    eager = ctx.batch_tc_ktruss(dA, voff, k=4, use_graph=False, reps=2)
+    plain = ctx.batch_tc_ktruss(dA, voff, k=4, use_graph=True, reps=max(a.steps, 3))
+    assert np.array_equal(eager["triangles"], plain["triangles"]) and \
+        np.array_equal(eager["truss_edges"], plain["truss_edges"]), "graph replay differs from the eager run"
+    log(f"batch: eager {eager['ms']:.2f} ms, graph {plain['ms']:.2f} ms per step")
+    t0 = time.perf_counter()
+    tuned, _, tlog = SEL.autotune_batch(ctx, dA, voff, k=4, rounds=1, reps=2)
+    best = ctx.batch_tc_ktruss(dA, voff, k=4, use_graph=True, reps=max(a.steps, 3), opts=tuned)
+    assert np.array_equal(best["triangles"], plain["triangles"]) and \
+        np.array_equal(best["truss_edges"], plain["truss_edges"]), "tuned options changed the results"
+    log(f"batch: autotuned {tuned} ({len(tlog)} trials, {time.perf_counter() - t0:.1f}s): {best['ms']:.2f} ms")
+    dA.free()
+    return {"workload": f"{a.batch_graphs} graphs (log-uniform n in [1024, 65536], half ER d~U[8,32], half "
+                        "Chung-Lu exponent~U[2.1,2.8] d=16): triangle count + 4-truss per graph, one "
+                        "hipGraph-captured step",
+            "graphs": a.batch_graphs, "vertices": int(U.n), "edges": int(E),
+            "value": E / (best["ms"] / 1e3), "unit": "edges/s", "ms": best["ms"], "ms_eager": eager["ms"],
+            "ms_graph_default": plain["ms"], "tuned_opts": tuned,
+            "triangles_total": int(plain["triangles"].sum()),
+            "truss4_edges_total": int(plain["truss_edges"].sum())}
+
+
+def batch_cpu_baseline(a, threads: int) -> dict:
+    """The oracle's triangle count and 4-truss loop on the first graphs of the
+    same batch until about 10 s of work."""
+    from graphkokkos_amd import HostGraph
+    from graphkokkos_amd.graph import batch_config
+    from oracle import oracle as O
+    cfg = batch_config(a.batch_graphs, seed=a.seed)
+    edges, dt, ng = 0, 0.0, 0
+    for g in range(a.batch_graphs):
+        if cfg["kind"][g] == 0:
+            G = HostGraph.er(int(cfg["n"][g]), float(cfg["mean_degree"][g]), seed=int(cfg["seed"][g]))
+        else:
+            G = HostGraph.chung_lu(int(cfg["n"][g]), float(cfg["exponent"][g]), float(cfg["mean_degree"][g]),
+                                   seed=int(cfg["seed"][g]))
+        t0 = time.perf_counter()
+        O.tc_sandia(G.L)
+        O.ktruss(G.A, 4)
+        dt += time.perf_counter() - t0
+        edges += G.edges
+        ng += 1
+        if dt > 10.0:
+            break
+    return {"value": edges / dt, "unit": "edges/s", "cores": threads, "kind": "port",
+            "sample": f"the first {ng} graphs of the same batch ({edges} edges): triangle count (parallel MSA) "
+                      f"and the reference's 4-truss loop, {dt:.1f}s"}
+
+
 def calibrate_copy(nbytes: int = 256 << 20) -> None:
     """One device-to-device copy of nbytes (read nbytes, write nbytes): the
     reference kernel for converting FETCH_SIZE / WRITE_SIZE counter readings
@@ -162,6 +233,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tc-variants", default="all", choices=["all", "plain"],
                     help="plain: only the fixed-heuristic triangle count (no selector training or autotuning)")
+    ap.add_argument("--batch-graphs", type=int, default=4096)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="end with a 256 MiB device-to-device copy, the known-size kernel that scales "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes")
@@ -256,6 +328,10 @@ def main() -> int:
               "forward_ms": r["timing"]["select_ms"], "backward_ms": r["timing"]["kernel_ms"] - r["timing"]["select_ms"]}
         log(f"bc: {r['wall_s'] * 1e3:.1f} ms for {nsrc} sources, {bc['value']:.3g} source-edges/s")
 
+    batch = None
+    if "batch" in legs:
+        batch = run_batch(ctx, a)
+
     cpu = None
     if not a.no_cpu:
         from oracle import oracle as O
@@ -278,6 +354,9 @@ def main() -> int:
                                   "sample": f"the same triangle count (R-MAT scale {a.tc_scale}) by the parallel "
                                             f"MSA restatement in {dt:.2f}s"}
             log(f"cpu baseline tc: {tc['cpu_baseline']['value']:.3g} edges/s")
+        if batch is not None:
+            batch["cpu_baseline"] = batch_cpu_baseline(a, th)
+            log(f"cpu baseline batch: {batch['cpu_baseline']['value']:.3g} edges/s")
         if bc is not None:
             t0 = time.perf_counter()
             O.bc_batch(G20.A, sources[:256])
@@ -311,6 +390,8 @@ def main() -> int:
         result["tc"] = tc
     if bc is not None:
         result["bc"] = bc
+    if batch is not None:
+        result["batch"] = batch
     ctx.close()
     print(json.dumps(result), flush=True)
     return 0

@@ -38,7 +38,8 @@ class GkOpts(C.Structure):
                 ("mca_max_mask", C.c_int32), ("lds_max_mask", C.c_int32), ("dense_min_mask", C.c_int32),
                 ("split_flop", C.c_int32), ("block_threads", C.c_int32), ("verbose", C.c_int32),
                 ("small_max_mask", C.c_int32), ("blocks_small", C.c_int32), ("blocks_long", C.c_int32),
-                ("blocks_mca", C.c_int32), ("blocks_global", C.c_int32), ("reserved", C.c_int32 * 2)]
+                ("blocks_mca", C.c_int32), ("blocks_global", C.c_int32), ("batch_blocks", C.c_int32),
+                ("batch_order", C.c_int32)]
 
 
 class GkTiming(C.Structure):
@@ -70,6 +71,7 @@ _SIGS = {
     "gk_hgraph_chung_lu": (C.c_int, [C.c_int64, C.c_double, C.c_double, C.c_uint64, C.c_int,
                                      C.POINTER(C.c_void_p)]),
     "gk_hgraph_grid": (C.c_int, [C.c_int64, C.c_int64, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+    "gk_hgraph_batch": (C.c_int, [C.c_int64, i32p, i64p, f64p, f64p, u64p, C.c_int, C.POINTER(C.c_void_p), i64p]),
     "gk_hgraph_from_edges": (C.c_int, [C.c_int64, C.c_int64, i32p, i32p, C.c_int, C.POINTER(C.c_void_p)]),
     "gk_hgraph_from_csr": (C.c_int, [C.POINTER(GkCsr), C.c_int, C.POINTER(C.c_void_p)]),
     "gk_hgraph_view": (C.c_int, [C.c_void_p, C.POINTER(GkCsr), C.POINTER(GkCsr)]),
@@ -95,6 +97,8 @@ _SIGS = {
     "gk_bc_batch_f64": (C.c_int, [C.c_void_p, C.POINTER(GkCsr), i32p, C.c_int64, f64p, C.POINTER(GkOpts)]),
     "gk_bc_batch_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, i32p, C.c_int64, C.c_void_p,
                                   C.POINTER(GkOpts)]),
+    "gk_batch_tc_ktruss_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, i64p, C.c_int, C.c_int, C.c_int, u64p,
+                                         i64p, C.c_void_p, f64p, u64p, C.POINTER(GkOpts)]),
     "gk_model_num_params": (C.c_int, []),
     "gk_model_set": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_int]),
     "gk_model_get": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_int]),

@@ -197,6 +197,30 @@ class Context:
                                     C.byref(o)), "gk_bc_batch_f64")
         return out
 
+    def batch_tc_ktruss(self, A: DeviceCsr, voff, k: int = 4, use_graph: bool = True, reps: int = 1,
+                        keep: bool = False, digests: bool = False, opts=None) -> dict:
+        """Triangle count and k-truss of every graph of a block-diagonal union
+        (gk_batch_tc_ktruss_dev)."""
+        o = self._opts(opts)
+        voff = np.ascontiguousarray(voff, np.int64)
+        ng = len(voff) - 1
+        tri = np.zeros(ng, np.uint64)
+        te = np.zeros(ng, np.int64)
+        ms = C.c_double(0)
+        dig = np.zeros(reps, np.uint64) if digests else None
+        kb = self.alloc(max(A.nz, 1)) if keep else None
+        check(lib().gk_batch_tc_ktruss_dev(self.handle, A.handle, ng, _ptr(voff, C.c_int64), k, int(use_graph), reps,
+                                           _ptr(tri, C.c_uint64), _ptr(te, C.c_int64), kb.ptr if kb else None,
+                                           C.byref(ms), _ptr(dig, C.c_uint64) if dig is not None else None,
+                                           C.byref(o)), "gk_batch_tc_ktruss_dev")
+        out = {"triangles": tri, "truss_edges": te, "ms": ms.value}
+        if dig is not None:
+            out["digests"] = dig
+        if kb is not None:
+            out["keep"] = kb.to_host(np.uint8, A.nz).astype(bool)
+            kb.free()
+        return out
+
     def bc_batch_dev(self, A: DeviceCsr, sources, out: DeviceBuffer, AT: DeviceCsr | None = None,
                      opts=None) -> None:
         o = self._opts(opts)

@@ -92,6 +92,7 @@ hipError_t d2h(gk_ctx *ctx, void *dst, const void *src, size_t bytes);
 gk_dcsr view(gk_ctx *ctx, int64_t m, int64_t n, int64_t nz, int32_t *rst, int32_t *col, int32_t *val);
 // Upload helpers used by the host-pointer entry points.
 int upload_csr(gk_ctx *ctx, const gk_csr *h, gk_dcsr *d);
+hipError_t h2d_chunked(gk_ctx *ctx, void *dst, const void *src, size_t bytes);
 void free_csr(gk_dcsr *d);
 int check_csr(const gk_csr *c, const char *name);
 // check_csr plus a scan of the arrays: row pointers non-decreasing, columns

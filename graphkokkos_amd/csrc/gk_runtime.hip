@@ -44,11 +44,19 @@ gk_dcsr view(gk_ctx *ctx, int64_t m, int64_t n, int64_t nz, int32_t *rst, int32_
     return d;
 }
 
+// Device-to-host copy on the context stream, in pieces of at most 256 MiB
+// (see h2d_chunked).
 hipError_t d2h(gk_ctx *ctx, void *dst, const void *src, size_t bytes) {
-    if (bytes == 0) return hipSuccess;
-    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream);
-    if (e != hipSuccess) return e;
-    return hipStreamSynchronize(ctx->stream);
+    constexpr size_t kChunk = size_t(256) << 20;
+    for (size_t off = 0; off < bytes; off += kChunk) {
+        const size_t n = std::min(kChunk, bytes - off);
+        hipError_t e = hipMemcpyAsync((char *)dst + off, (const char *)src + off, n, hipMemcpyDeviceToHost,
+                                      ctx->stream);
+        if (e != hipSuccess) return e;
+        e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 int check_csr(const gk_csr *c, const char *name) {
@@ -134,19 +142,35 @@ int require_symmetric(const gk_dcsr *d, const char *name) {
     return GK_OK;
 }
 
+// Host-to-device copy of a host array in pieces of at most kCopyChunk
+// bytes, each a plain pageable copy of the size the rest of the library
+// issues, so multi-GB arrays take the same runtime path as small ones.
+constexpr size_t kCopyChunk = size_t(256) << 20;
+
+hipError_t h2d_chunked(gk_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    for (size_t off = 0; off < bytes; off += kCopyChunk) {
+        const size_t n = std::min(kCopyChunk, bytes - off);
+        hipError_t e = hipMemcpyAsync((char *)dst + off, (const char *)src + off, n, hipMemcpyHostToDevice,
+                                      ctx->stream);
+        if (e != hipSuccess) return e;
+        e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 int upload_csr(gk_ctx *ctx, const gk_csr *h, gk_dcsr *d) {
     GK_TRY(inspect_csr(h, "csr", d));
     d->ctx = ctx; d->m = h->m; d->n = h->n; d->nz = h->nz; d->owns = true;
     d->rst = d->col = d->val = nullptr;
     GK_HIP(hipMalloc(&d->rst, sizeof(int32_t) * (h->m + 1)));
     GK_HIP(hipMalloc(&d->col, sizeof(int32_t) * (h->nz ? h->nz : 1)));
-    GK_HIP(hipMemcpyAsync(d->rst, h->rst, sizeof(int32_t) * (h->m + 1), hipMemcpyHostToDevice, ctx->stream));
-    if (h->nz) GK_HIP(hipMemcpyAsync(d->col, h->col, sizeof(int32_t) * h->nz, hipMemcpyHostToDevice, ctx->stream));
+    GK_HIP(h2d_chunked(ctx, d->rst, h->rst, sizeof(int32_t) * (h->m + 1)));
+    GK_HIP(h2d_chunked(ctx, d->col, h->col, sizeof(int32_t) * h->nz));
     if (h->val) {
         GK_HIP(hipMalloc(&d->val, sizeof(int32_t) * (h->nz ? h->nz : 1)));
-        if (h->nz) GK_HIP(hipMemcpyAsync(d->val, h->val, sizeof(int32_t) * h->nz, hipMemcpyHostToDevice, ctx->stream));
+        GK_HIP(h2d_chunked(ctx, d->val, h->val, sizeof(int32_t) * h->nz));
     }
-    GK_HIP(hipStreamSynchronize(ctx->stream));
     return GK_OK;
 }
 
@@ -209,6 +233,8 @@ extern "C" void gk_opts_default(gk_opts *o) {
     o->blocks_long = 8;
     o->blocks_mca = 8;
     o->blocks_global = 4;
+    o->batch_blocks = 8;
+    o->batch_order = 0;
 }
 
 extern "C" int gk_init(int device, gk_ctx **out) {
@@ -298,13 +324,11 @@ extern "C" void gk_dfree(gk_ctx *ctx, void *dptr) {
 }
 extern "C" int gk_memcpy_d2h(gk_ctx *ctx, void *dst, const void *src, size_t bytes) {
     GK_CHECK_ARG(ctx, "ctx is NULL");
-    GK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
-    GK_HIP(hipStreamSynchronize(ctx->stream));
+    GK_HIP(d2h(ctx, dst, src, bytes));
     return GK_OK;
 }
 extern "C" int gk_memcpy_h2d(gk_ctx *ctx, void *dst, const void *src, size_t bytes) {
     GK_CHECK_ARG(ctx, "ctx is NULL");
-    GK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
-    GK_HIP(hipStreamSynchronize(ctx->stream));
+    GK_HIP(h2d_chunked(ctx, dst, src, bytes));
     return GK_OK;
 }

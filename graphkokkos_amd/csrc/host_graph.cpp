@@ -34,7 +34,11 @@ void set_error(const char *fmt, ...) {
     g_err = buf;
 }
 void clear_error() { g_err.clear(); }
+// Set on the workers of a batch build: each builds whole graphs on its own.
+static thread_local bool t_serial = false;
+
 int host_threads() {
+    if (t_serial) return 1;
     if (g_host_threads > 0) return g_host_threads;
     unsigned h = std::thread::hardware_concurrency();
     return h ? (int)std::min(h, 64u) : 1;
@@ -229,6 +233,83 @@ extern "C" int gk_hgraph_chung_lu(int64_t n, double exponent, double mean_degree
         v = std::min<int64_t>(draw(rng(seed, (uint64_t)e, 1)), n - 1);
         return true;
     }, relabel, out);
+}
+
+extern "C" int gk_hgraph_batch(int64_t ngraphs, const int32_t *kind, const int64_t *n, const double *exponent,
+                               const double *mean_degree, const uint64_t *seed, int relabel, gk_hgraph **out,
+                               int64_t *voff) {
+    GK_CHECK_ARG(out && voff && kind && n && mean_degree && seed, "null argument");
+    GK_CHECK_ARG(ngraphs >= 1, "ngraphs must be positive");
+    for (int64_t g = 0; g < ngraphs; ++g) {
+        GK_CHECK_ARG(kind[g] == 0 || kind[g] == 1, "graph %lld: kind must be 0 (ER) or 1 (Chung-Lu)", (long long)g);
+        GK_CHECK_ARG(n[g] >= 2 && mean_degree[g] > 0, "graph %lld: needs n >= 2 and a positive degree", (long long)g);
+        GK_CHECK_ARG(kind[g] == 0 || (exponent && exponent[g] > 1.0), "graph %lld: exponent must be > 1",
+                     (long long)g);
+    }
+    voff[0] = 0;
+    for (int64_t g = 0; g < ngraphs; ++g) voff[g + 1] = voff[g] + n[g];
+    GK_CHECK_ARG(voff[ngraphs] < INT32_MAX, "total vertex count exceeds int32 CSR");
+    // every worker builds whole graphs with the single-threaded pipeline
+    std::vector<gk_hgraph *> parts(ngraphs, nullptr);
+    std::vector<int> rcs(ngraphs, GK_OK);
+    std::vector<std::string> errs(ngraphs);
+    {
+        std::atomic<int64_t> next{0};
+        std::vector<std::thread> th;
+        const int nt = (int)std::max<int64_t>(1, std::min<int64_t>(host_threads(), ngraphs));
+        for (int t = 0; t < nt; ++t)
+            th.emplace_back([&] {
+                t_serial = true;
+                for (;;) {
+                    const int64_t g = next.fetch_add(1);
+                    if (g >= ngraphs) break;
+                    rcs[g] = kind[g] == 0 ? gk_hgraph_er(n[g], mean_degree[g], seed[g], relabel, &parts[g])
+                                          : gk_hgraph_chung_lu(n[g], exponent[g], mean_degree[g], seed[g], relabel,
+                                                               &parts[g]);
+                    if (rcs[g] != GK_OK) errs[g] = gk_last_error();
+                }
+                t_serial = false;
+            });
+        for (auto &x : th) x.join();
+    }
+    auto release = [&] { for (auto *q : parts) delete q; };
+    for (int64_t g = 0; g < ngraphs; ++g)
+        if (rcs[g] != GK_OK) {
+            set_error("graph %lld: %s", (long long)g, errs[g].c_str());
+            release();
+            return rcs[g];
+        }
+    // block-diagonal union: graph g on vertices [voff[g], voff[g+1])
+    int64_t nnzA = 0, nnzL = 0;
+    for (auto *q : parts) { nnzA += (int64_t)q->acol.size(); nnzL += (int64_t)q->lcol.size(); }
+    if (nnzA >= INT32_MAX) { release(); set_error("union nnz %lld exceeds int32 CSR", (long long)nnzA); return GK_ERANGE; }
+    auto u = std::make_unique<gk_hgraph>();
+    const int64_t N = voff[ngraphs];
+    u->n = N;
+    u->arst.assign(N + 1, 0);
+    u->lrst.assign(N + 1, 0);
+    u->acol.resize(nnzA);
+    u->lcol.resize(nnzL);
+    u->perm.resize(N);
+    std::vector<int64_t> aoff(ngraphs + 1, 0), loff(ngraphs + 1, 0);
+    for (int64_t g = 0; g < ngraphs; ++g) {
+        aoff[g + 1] = aoff[g] + (int64_t)parts[g]->acol.size();
+        loff[g + 1] = loff[g] + (int64_t)parts[g]->lcol.size();
+    }
+    par_dynamic(ngraphs, 16, [&](int64_t g) {
+        const gk_hgraph *q = parts[g];
+        const int32_t v0 = (int32_t)voff[g];
+        for (int64_t i = 0; i < q->n; ++i) {
+            u->arst[v0 + i + 1] = (int32_t)(aoff[g] + q->arst[i + 1]);
+            u->lrst[v0 + i + 1] = (int32_t)(loff[g] + q->lrst[i + 1]);
+            u->perm[v0 + i] = v0 + q->perm[i];
+        }
+        for (size_t p = 0; p < q->acol.size(); ++p) u->acol[aoff[g] + p] = v0 + q->acol[p];
+        for (size_t p = 0; p < q->lcol.size(); ++p) u->lcol[loff[g] + p] = v0 + q->lcol[p];
+    });
+    release();
+    *out = u.release();
+    return GK_OK;
 }
 
 extern "C" int gk_hgraph_grid(int64_t rows, int64_t cols, int diagonal, int relabel, gk_hgraph **out) {

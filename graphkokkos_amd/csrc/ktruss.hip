@@ -25,6 +25,7 @@
 
 #include "gk_device.h"
 #include "spgemm.h"
+#include "truss.h"
 
 using namespace gk;
 
@@ -922,6 +923,35 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
 }
 
 }  // namespace
+}  // namespace gk
+
+namespace gk {
+
+int edge_index(gk_ctx *ctx, const gk_dcsr *A, EdgeIndex *X) {
+    const int64_t m = A->m;
+    int32_t *len, *lid;
+    GK_TRY(X->alloc(&len, m + 1));
+    GK_TRY(X->alloc(&X->lrst, m + 1));
+    const int g = (int)std::max<int64_t>(1, std::min<int64_t>((m + 255) / 256, ctx->num_cus * 16));
+    k_lower_len<<<g, 256, 0, ctx->stream>>>(m, A->rst, A->col, len);
+    GK_TRY(launch_check("k_lower_len"));
+    GK_TRY(exclusive_scan_i32(ctx, len, X->lrst, m + 1));
+    int32_t nz;
+    GK_HIP(d2h(ctx, &nz, X->lrst + m, 4));
+    X->m = m;
+    X->nL = nz;
+    GK_TRY(X->alloc(&X->lcol, nz));
+    GK_TRY(X->alloc(&lid, nz));
+    GK_TRY(X->alloc(&X->eu, nz));
+    GK_TRY(X->alloc(&X->ev, nz));
+    GK_TRY(X->alloc(&X->aeid, A->nz));
+    const int gw = waves_grid(ctx, m);
+    k_lower_copy<<<gw, 256, 0, ctx->stream>>>(m, A->rst, A->col, X->lrst, X->lcol, lid);
+    k_edge_ends<<<gw, 256, 0, ctx->stream>>>(m, X->lrst, X->lcol, X->eu, X->ev);
+    k_sym_eid<<<gw, 256, 0, ctx->stream>>>(m, A->rst, A->col, X->lrst, X->aeid);
+    return launch_check("edge_index");
+}
+
 }  // namespace gk
 
 extern "C" int gk_truss_decompose_dev(gk_ctx *ctx, const gk_dcsr *A, uint32_t *d_truss, int32_t *kmax,

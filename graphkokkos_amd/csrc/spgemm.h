@@ -74,6 +74,31 @@ struct Plan {
 // lengths, [acc][3] streamed B entries; then hits per accumulator.
 enum { ST_TASKS = 0, ST_MASK, ST_AROW, ST_FLOP, ST_NUM };
 
+struct Prob {
+    const int32_t *__restrict__ arst, *__restrict__ acol, *__restrict__ aval;
+    const int32_t *__restrict__ brst, *__restrict__ bcol, *__restrict__ bval;
+    const int32_t *__restrict__ mrst, *__restrict__ mcol;
+    int32_t *__restrict__ cval;                 // VALUES: C values; SUPPORT: uint32 supports
+    unsigned long long *__restrict__ count;     // TC: total
+    unsigned long long *__restrict__ rowcount;  // TC: optional per-row counts
+    unsigned long long *__restrict__ hits;      // hits per accumulator
+    long long *__restrict__ task_cycles;        // optional: clock64 duration of every task
+    const Task *__restrict__ tasks;
+    const int32_t *__restrict__ bin_off;
+};
+
+// Launch shape of one product's class kernels and the scratch of the
+// global accumulators (setup_classes), so launch_classes only launches.
+struct ClassLaunch {
+    int grid[kNumClass] = {};
+    int grid_lds_large = 0;
+    int64_t bn = 0;
+    int32_t *hkey = nullptr, *hpos = nullptr, *hacc = nullptr;
+    int hlg = 6;
+    int32_t *dmap = nullptr, *dacc = nullptr;
+    int32_t dmm = 0;
+};
+
 struct ProductInfo {
     double total_ms = 0, select_ms = 0, kernel_ms = 0;
     int32_t ntasks = 0;
@@ -99,6 +124,16 @@ int zero_u32(gk_ctx *ctx, void *p, int64_t n);
 int masked_product(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcsr *M, int mode, int32_t *d_out,
                    unsigned long long *d_count, unsigned long long *d_rowcount, const gk_opts &o,
                    ProductInfo *info, long long *task_cycles = nullptr, Plan *plan_out = nullptr);
+
+int setup_classes(gk_ctx *ctx, const Plan &plan, int64_t bn, const gk_opts &o, ClassLaunch *L);
+// Plan of C<M> = A*B writing d_out, for products launched repeatedly with
+// launch_classes (the plan and the scratch live in the context's scratch
+// slots until the next product on the context).
+int prepare_product(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcsr *M, int32_t *d_out,
+                    const gk_opts &o, Prob *P, ClassLaunch *L);
+// Kernel launches only (capturable in a hipGraph); timers optional.
+template <int MODE>
+int launch_classes(gk_ctx *ctx, const Prob &P, const ClassLaunch &L, Timer *tm);
 
 // Learned selector (selector.hip): writes st.acc for every row.
 int model_select(gk_ctx *ctx, RowStats &st, int64_t m, int64_t ncols, int same_am);

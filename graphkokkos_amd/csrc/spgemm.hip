@@ -226,19 +226,6 @@ __global__ __launch_bounds__(256) void k_scatter_tasks(int64_t m, RowStats st, c
 // ---------------------------------------------------------------------------
 // task engine
 
-struct Prob {
-    const int32_t *__restrict__ arst, *__restrict__ acol, *__restrict__ aval;
-    const int32_t *__restrict__ brst, *__restrict__ bcol, *__restrict__ bval;
-    const int32_t *__restrict__ mrst, *__restrict__ mcol;
-    int32_t *__restrict__ cval;                 // VALUES: C values; SUPPORT: uint32 supports
-    unsigned long long *__restrict__ count;     // TC: total
-    unsigned long long *__restrict__ rowcount;  // TC: optional per-row counts
-    unsigned long long *__restrict__ hits;      // hits per accumulator
-    long long *__restrict__ task_cycles;        // optional: clock64 duration of every task
-    const Task *__restrict__ tasks;
-    const int32_t *__restrict__ bin_off;
-};
-
 // LDS prefix of B-row lengths for A entries [a0, a1) of one task (a wave or
 // a tile of `G` lanes).  pre[x] = streamed entries before A entry a0+x;
 // bst[x] = index in B.col of streamed entry 0 of A entry a0+x minus pre[x].
@@ -891,107 +878,123 @@ static int dense_maps(gk_ctx *ctx, int64_t blocks, int64_t ncols, int32_t **out)
     return GK_OK;
 }
 
-// Launch the accumulator kernels of one product.  Each class runs on its own
-// auxiliary stream (forked from the context stream after the plan, joined
-// back before the caller continues), so the classes share the GPU instead
-// of running one after the other.
-template <int MODE>
-static int run_mode(gk_ctx *ctx, const Prob &P, const Plan &plan, int64_t bn, const gk_opts &o, double *acc_ms) {
+// Scratch of the global accumulators and the grid of every class, sized
+// on the context stream before any launch (launch_classes itself only
+// launches, so it can be captured in a hipGraph).
+int setup_classes(gk_ctx *ctx, const Plan &plan, int64_t bn, const gk_opts &o, ClassLaunch *L) {
     auto per_cu = [](int32_t v, int dflt) { return v > 0 ? std::min<int32_t>(v, 64) : dflt; };
     const int b_small = per_cu(o.blocks_small, kLdsSmallBlocksPerCu), b_long = per_cu(o.blocks_long, kWindowBlocksLong);
     const int b_mca = per_cu(o.blocks_mca, kMcaBlocksPerCu), b_glob = per_cu(o.blocks_global, kGlobalBlocksPerCu);
+    *L = ClassLaunch{};
+    L->bn = bn;
+    L->grid[kClassMsaLarge] = plan.acc_tasks[kClassMsaLarge] ? grid_for(ctx, plan.acc_tasks[kClassMsaLarge], b_long) : 0;
+    L->grid_lds_large = plan.acc_tasks[kClassMsaLarge]
+                            ? grid_for(ctx, plan.acc_tasks[kClassMsaLarge], std::min(b_long, kLdsBlocksPerCu))
+                            : 0;
+    L->grid[GK_ACC_MSA_LDS] = plan.acc_tasks[GK_ACC_MSA_LDS] ? grid_for(ctx, plan.acc_tasks[GK_ACC_MSA_LDS], b_small) : 0;
+    L->grid[GK_ACC_MCA_WAVE] =
+        plan.acc_tasks[GK_ACC_MCA_WAVE] ? grid_for(ctx, (plan.acc_tasks[GK_ACC_MCA_WAVE] + 3) / 4, b_mca) : 0;
+    if (plan.acc_tasks[GK_ACC_HASH_GLOBAL]) {
+        while ((1 << L->hlg) < 2 * plan.acc_maxmask[GK_ACC_HASH_GLOBAL]) ++L->hlg;
+        int hg = grid_for(ctx, plan.acc_tasks[GK_ACC_HASH_GLOBAL], b_glob);
+        size_t cap = (size_t)1 << L->hlg;
+        // Bound the tables to about 4 GiB.
+        while (hg > 1 && (size_t)hg * cap * 10 > ((size_t)4 << 30)) hg >>= 1;
+        void *sp;
+        GK_TRY(scratch(ctx, SC_ACC, (size_t)hg * cap * 4 * 2 + (size_t)hg * (cap / 2) * 4, &sp));
+        L->hkey = (int32_t *)sp;
+        L->hpos = L->hkey + (size_t)hg * cap;
+        L->hacc = L->hpos + (size_t)hg * cap;
+        L->grid[GK_ACC_HASH_GLOBAL] = hg;
+    }
+    if (plan.acc_tasks[GK_ACC_DENSE_GLOBAL]) {
+        int dg = grid_for(ctx, plan.acc_tasks[GK_ACC_DENSE_GLOBAL], b_glob);
+        // Bound the map memory to about 4 GiB.
+        while (dg > 1 && (size_t)dg * (size_t)bn * 4 > ((size_t)4 << 30)) dg >>= 1;
+        L->dmm = plan.acc_maxmask[GK_ACC_DENSE_GLOBAL];
+        GK_TRY(dense_maps(ctx, dg, bn, &L->dmap));
+        void *sp;
+        GK_TRY(scratch(ctx, SC_TMP7, (size_t)dg * (size_t)L->dmm * 4 + 4, &sp));
+        L->dacc = (int32_t *)sp;
+        L->grid[GK_ACC_DENSE_GLOBAL] = dg;
+    }
+    return GK_OK;
+}
+
+// Launch the accumulator kernels of one product.  Each class runs on its own
+// auxiliary stream (forked from the context stream, joined back before the
+// caller continues), so the classes share the GPU instead of running one
+// after the other.  With tm, each class is bracketed by its timer.
+template <int MODE>
+int launch_classes(gk_ctx *ctx, const Prob &P, const ClassLaunch &L, Timer *tm) {
     static_assert(kNumClass <= gk_ctx::kAux + 1, "one stream per kernel class");
     hipStream_t sc[kNumClass];
-    // the context stream takes dense-global, whose scratch setup runs there
     sc[GK_ACC_DENSE_GLOBAL] = ctx->stream;
     sc[kClassMsaLarge] = ctx->aux[0];
     sc[GK_ACC_MSA_LDS] = ctx->aux[1];
     sc[GK_ACC_MCA_WAVE] = ctx->aux[2];
     sc[GK_ACC_HASH_GLOBAL] = ctx->aux[3];
-    Timer tm[kNumClass] = {Timer(ctx, sc[0]), Timer(ctx, sc[1]), Timer(ctx, sc[2]), Timer(ctx, sc[3]),
-                           Timer(ctx, sc[4])};
-    bool ran[kNumClass] = {};
-    // scratch for the global accumulators is sized and bound on the context
-    // stream before the fork
-    int32_t *hkey = nullptr, *hpos = nullptr, *hacc = nullptr, *dmap = nullptr, *dacc = nullptr;
-    int hlg = 6, hg = 0, dg = 0;
-    int32_t dmm = 0;
-    if (plan.acc_tasks[GK_ACC_HASH_GLOBAL]) {
-        while ((1 << hlg) < 2 * plan.acc_maxmask[GK_ACC_HASH_GLOBAL]) ++hlg;
-        hg = grid_for(ctx, plan.acc_tasks[GK_ACC_HASH_GLOBAL], b_glob);
-        size_t cap = (size_t)1 << hlg;
-        // Bound the tables to about 4 GiB.
-        while (hg > 1 && (size_t)hg * cap * 10 > ((size_t)4 << 30)) hg >>= 1;
-        void *sp;
-        GK_TRY(scratch(ctx, SC_ACC, (size_t)hg * cap * 4 * 2 + (size_t)hg * (cap / 2) * 4, &sp));
-        hkey = (int32_t *)sp;
-        hpos = hkey + (size_t)hg * cap;
-        hacc = hpos + (size_t)hg * cap;
-    }
-    if (plan.acc_tasks[GK_ACC_DENSE_GLOBAL]) {
-        dg = grid_for(ctx, plan.acc_tasks[GK_ACC_DENSE_GLOBAL], b_glob);
-        // Bound the map memory to about 4 GiB.
-        while (dg > 1 && (size_t)dg * (size_t)bn * 4 > ((size_t)4 << 30)) dg >>= 1;
-        dmm = plan.acc_maxmask[GK_ACC_DENSE_GLOBAL];
-        GK_TRY(dense_maps(ctx, dg, bn, &dmap));
-        void *sp;
-        GK_TRY(scratch(ctx, SC_TMP7, (size_t)dg * (size_t)dmm * 4 + 4, &sp));
-        dacc = (int32_t *)sp;
-    }
     GK_HIP(hipEventRecord(ctx->fork, ctx->stream));
     for (int i = 0; i < gk_ctx::kAux; ++i) GK_HIP(hipStreamWaitEvent(ctx->aux[i], ctx->fork, 0));
-    if (plan.acc_tasks[kClassMsaLarge]) {
-        int g = grid_for(ctx, plan.acc_tasks[kClassMsaLarge], std::min(b_long, kLdsBlocksPerCu));
-        tm[kClassMsaLarge].start();
+    if (L.grid[kClassMsaLarge]) {
+        if (tm) tm[kClassMsaLarge].start();
         if (MODE == MODE_TC || MODE == MODE_SUPPORT)
             k_msa_window<MODE == MODE_VALUES ? MODE_TC : MODE, kWindowWavesLong, kWindowHashLong, kLdsMaxMask>
-                <<<grid_for(ctx, plan.acc_tasks[kClassMsaLarge], b_long), 64 * kWindowWavesLong, 0,
-                   sc[kClassMsaLarge]>>>(P, kClassMsaLarge);
+                <<<L.grid[kClassMsaLarge], 64 * kWindowWavesLong, 0, sc[kClassMsaLarge]>>>(P, kClassMsaLarge);
         else
-            k_msa_lds<MODE, kLdsHashCap, kLdsMaxMask><<<g, 64, 0, sc[kClassMsaLarge]>>>(P, kClassMsaLarge);
-        tm[kClassMsaLarge].stop();
-        ran[kClassMsaLarge] = true;
+            k_msa_lds<MODE, kLdsHashCap, kLdsMaxMask><<<L.grid_lds_large, 64, 0, sc[kClassMsaLarge]>>>(P,
+                                                                                                       kClassMsaLarge);
+        if (tm) tm[kClassMsaLarge].stop();
         GK_TRY(launch_check("k_msa_lds<large>"));
     }
-    if (plan.acc_tasks[GK_ACC_MSA_LDS]) {
-        int g = grid_for(ctx, plan.acc_tasks[GK_ACC_MSA_LDS], b_small);
-        tm[GK_ACC_MSA_LDS].start();
+    if (L.grid[GK_ACC_MSA_LDS]) {
+        if (tm) tm[GK_ACC_MSA_LDS].start();
         if (MODE == MODE_TC || MODE == MODE_SUPPORT)
             k_msa_window<MODE == MODE_VALUES ? MODE_TC : MODE, 1, kLdsSmallCap, kLdsSmallMask>
-                <<<g, 64, 0, sc[GK_ACC_MSA_LDS]>>>(P, GK_ACC_MSA_LDS);
+                <<<L.grid[GK_ACC_MSA_LDS], 64, 0, sc[GK_ACC_MSA_LDS]>>>(P, GK_ACC_MSA_LDS);
         else
-            k_msa_lds<MODE, kLdsSmallCap, kLdsSmallMask><<<g, 64, 0, sc[GK_ACC_MSA_LDS]>>>(P, GK_ACC_MSA_LDS);
-        tm[GK_ACC_MSA_LDS].stop();
-        ran[GK_ACC_MSA_LDS] = true;
+            k_msa_lds<MODE, kLdsSmallCap, kLdsSmallMask><<<L.grid[GK_ACC_MSA_LDS], 64, 0, sc[GK_ACC_MSA_LDS]>>>(
+                P, GK_ACC_MSA_LDS);
+        if (tm) tm[GK_ACC_MSA_LDS].stop();
         GK_TRY(launch_check("k_msa_lds<small>"));
     }
-    if (plan.acc_tasks[GK_ACC_MCA_WAVE]) {
-        int g = grid_for(ctx, (plan.acc_tasks[GK_ACC_MCA_WAVE] + 3) / 4, b_mca);
-        tm[GK_ACC_MCA_WAVE].start();
-        k_mca_wave<MODE, 16><<<g, 64, 0, sc[GK_ACC_MCA_WAVE]>>>(P, GK_ACC_MCA_WAVE);
-        tm[GK_ACC_MCA_WAVE].stop();
-        ran[GK_ACC_MCA_WAVE] = true;
+    if (L.grid[GK_ACC_MCA_WAVE]) {
+        if (tm) tm[GK_ACC_MCA_WAVE].start();
+        k_mca_wave<MODE, 16><<<L.grid[GK_ACC_MCA_WAVE], 64, 0, sc[GK_ACC_MCA_WAVE]>>>(P, GK_ACC_MCA_WAVE);
+        if (tm) tm[GK_ACC_MCA_WAVE].stop();
         GK_TRY(launch_check("k_mca_wave"));
     }
-    if (plan.acc_tasks[GK_ACC_HASH_GLOBAL]) {
-        tm[GK_ACC_HASH_GLOBAL].start();
-        k_hash_global<MODE><<<hg, 64, 0, sc[GK_ACC_HASH_GLOBAL]>>>(P, GK_ACC_HASH_GLOBAL, hkey, hpos, hacc, hlg);
-        tm[GK_ACC_HASH_GLOBAL].stop();
-        ran[GK_ACC_HASH_GLOBAL] = true;
+    if (L.grid[GK_ACC_HASH_GLOBAL]) {
+        if (tm) tm[GK_ACC_HASH_GLOBAL].start();
+        k_hash_global<MODE><<<L.grid[GK_ACC_HASH_GLOBAL], 64, 0, sc[GK_ACC_HASH_GLOBAL]>>>(
+            P, GK_ACC_HASH_GLOBAL, L.hkey, L.hpos, L.hacc, L.hlg);
+        if (tm) tm[GK_ACC_HASH_GLOBAL].stop();
         GK_TRY(launch_check("k_hash_global"));
     }
-    if (plan.acc_tasks[GK_ACC_DENSE_GLOBAL]) {
-        tm[GK_ACC_DENSE_GLOBAL].start();
-        k_dense_global<MODE><<<dg, 64, 0, sc[GK_ACC_DENSE_GLOBAL]>>>(P, GK_ACC_DENSE_GLOBAL, dmap, dacc, bn, dmm);
-        tm[GK_ACC_DENSE_GLOBAL].stop();
-        ran[GK_ACC_DENSE_GLOBAL] = true;
+    if (L.grid[GK_ACC_DENSE_GLOBAL]) {
+        if (tm) tm[GK_ACC_DENSE_GLOBAL].start();
+        k_dense_global<MODE><<<L.grid[GK_ACC_DENSE_GLOBAL], 64, 0, sc[GK_ACC_DENSE_GLOBAL]>>>(
+            P, GK_ACC_DENSE_GLOBAL, L.dmap, L.dacc, L.bn, L.dmm);
+        if (tm) tm[GK_ACC_DENSE_GLOBAL].stop();
         GK_TRY(launch_check("k_dense_global"));
     }
     for (int i = 0; i < gk_ctx::kAux; ++i) {
         GK_HIP(hipEventRecord(ctx->join[i], ctx->aux[i]));
         GK_HIP(hipStreamWaitEvent(ctx->stream, ctx->join[i], 0));
     }
-    for (int a = 0; a < kNumClass; ++a) acc_ms[a] = ran[a] ? tm[a].ms() : 0.0;
+    return GK_OK;
+}
+template int launch_classes<MODE_SUPPORT>(gk_ctx *, const Prob &, const ClassLaunch &, Timer *);
+
+template <int MODE>
+static int run_mode(gk_ctx *ctx, const Prob &P, const Plan &plan, int64_t bn, const gk_opts &o, double *acc_ms) {
+    ClassLaunch L;
+    GK_TRY(setup_classes(ctx, plan, bn, o, &L));
+    hipStream_t sc[kNumClass] = {ctx->aux[1], ctx->aux[2], ctx->aux[3], ctx->stream, ctx->aux[0]};
+    Timer tm[kNumClass] = {Timer(ctx, sc[0]), Timer(ctx, sc[1]), Timer(ctx, sc[2]), Timer(ctx, sc[3]),
+                           Timer(ctx, sc[4])};
+    GK_TRY(launch_classes<MODE>(ctx, P, L, tm));
+    for (int a = 0; a < kNumClass; ++a) acc_ms[a] = L.grid[a] ? tm[a].ms() : 0.0;
     return GK_OK;
 }
 
@@ -1071,6 +1074,28 @@ int masked_product(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcs
         }
     }
     return GK_OK;
+}
+
+int prepare_product(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcsr *M, int32_t *d_out,
+                    const gk_opts &o, Prob *P, ClassLaunch *L) {
+    if (A->n != B->m || A->m != M->m || B->n != M->n) {
+        set_error("shape mismatch in a prepared product");
+        return GK_EINVAL;
+    }
+    RowStats st;
+    GK_TRY(row_stats(ctx, A, B, M, &st));
+    SelectParams sp = select_params(o);
+    if (sp.selector == GK_SEL_LEARNED)
+        GK_TRY(model_select(ctx, st, A->m, B->n, A->rst == M->rst && A->col == M->col));
+    Plan plan;
+    GK_TRY(plan_tasks(ctx, A, st, sp, &plan));
+    *P = Prob{};
+    P->arst = A->rst; P->acol = A->col; P->aval = A->val;
+    P->brst = B->rst; P->bcol = B->col; P->bval = B->val;
+    P->mrst = M->rst; P->mcol = M->col;
+    P->cval = d_out; P->hits = plan.hits;
+    P->tasks = plan.tasks; P->bin_off = plan.bin_off;
+    return setup_classes(ctx, plan, B->n, o, L);
 }
 
 void record_timing(gk_ctx *ctx, const ProductInfo &pi, bool reset) {

@@ -227,6 +227,26 @@ class HostGraph:
         return cls(h, f"chunglu-n{n}-g{exponent}-d{mean_degree}-seed{seed}")
 
     @classmethod
+    def batch(cls, kind, n, exponent, mean_degree, seed, relabel: bool = True) -> "HostGraph":
+        """Block-diagonal union of many ER (kind 0) / Chung-Lu (kind 1)
+        graphs; ``.voff`` holds the vertex offsets (graph g on
+        [voff[g], voff[g+1]))."""
+        kind = np.ascontiguousarray(kind, np.int32)
+        ng = len(kind)
+        n = np.ascontiguousarray(n, np.int64)
+        exponent = np.ascontiguousarray(exponent, np.float64)
+        mean_degree = np.ascontiguousarray(mean_degree, np.float64)
+        seed = np.ascontiguousarray(seed, np.uint64)
+        voff = np.zeros(ng + 1, np.int64)
+        h = C.c_void_p()
+        check(lib().gk_hgraph_batch(ng, _ptr(kind), _ptr(n, C.c_int64), _ptr(exponent, C.c_double),
+                                    _ptr(mean_degree, C.c_double), _ptr(seed, C.c_uint64), int(relabel), C.byref(h),
+                                    _ptr(voff, C.c_int64)), "gk_hgraph_batch")
+        g = cls(h, f"batch-{ng}")
+        g.voff = voff
+        return g
+
+    @classmethod
     def grid(cls, rows: int, cols: int | None = None, diagonal: bool = True, relabel: bool = True) -> "HostGraph":
         h = C.c_void_p()
         cols = rows if cols is None else cols
@@ -248,3 +268,19 @@ class HostGraph:
         csr = a.gk(with_values=False)
         check(lib().gk_hgraph_from_csr(C.byref(csr), int(relabel), C.byref(h)), "gk_hgraph_from_csr")
         return cls(h, "csr")
+
+
+def batch_config(ngraphs: int = 4096, seed: int = 1, n_min: int = 1024, n_max: int = 65536,
+                 cl_mean_degree: float = 16.0) -> dict:
+    """Seeded description of a batch of uneven graphs: vertex counts
+    log-uniform in [n_min, n_max]; even-numbered graphs Erdos-Renyi with
+    mean degree uniform in [8, 32], odd-numbered Chung-Lu with exponent
+    uniform in [2.1, 2.8] and mean degree ``cl_mean_degree``."""
+    rng = np.random.default_rng(seed)
+    n = np.exp(rng.uniform(np.log(n_min), np.log(n_max), ngraphs)).astype(np.int64)
+    kind = (np.arange(ngraphs) % 2).astype(np.int32)
+    exponent = np.where(kind == 1, rng.uniform(2.1, 2.8, ngraphs), 0.0)
+    mean_degree = np.where(kind == 0, rng.uniform(8.0, 32.0, ngraphs), cl_mean_degree)
+    seeds = rng.integers(1, 2 ** 62, ngraphs).astype(np.uint64)
+    return {"kind": kind, "n": n, "exponent": exponent, "mean_degree": mean_degree, "seed": seeds}
+

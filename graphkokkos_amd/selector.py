@@ -137,13 +137,11 @@ def time_tc(ctx: Context, L: DeviceCsr, opts, reps: int = 3) -> float:
     return (time.perf_counter() - t0) / reps
 
 
-def autotune_tc(ctx: Context, L: DeviceCsr, base: dict | None = None, rounds: int = 2, reps: int = 3,
-                space: dict | None = None) -> tuple[dict, float, list]:
-    """Coordinate descent over the product's global knobs, timing triangle
-    counting on L; returns (best options, best seconds, trial log)."""
-    space = space or SEARCH_SPACE
+def coordinate_descent(measure, space: dict, base: dict | None = None, rounds: int = 2) -> tuple[dict, float, list]:
+    """Minimise measure(options dict) one knob at a time over `space`;
+    returns (best options, best value, trial log)."""
     best = dict(base or {})
-    best_t = time_tc(ctx, L, default_opts(**best), reps)
+    best_t = measure(best)
     log = [(dict(best), best_t)]
     for _ in range(rounds):
         improved = False
@@ -153,10 +151,39 @@ def autotune_tc(ctx: Context, L: DeviceCsr, base: dict | None = None, rounds: in
                     continue
                 cand = dict(best)
                 cand[key] = v
-                t = time_tc(ctx, L, default_opts(**cand), reps)
+                t = measure(cand)
                 log.append((dict(cand), t))
                 if t < best_t * 0.98:
                     best, best_t, improved = cand, t, True
         if not improved:
             break
     return best, best_t, log
+
+
+def autotune_tc(ctx: Context, L: DeviceCsr, base: dict | None = None, rounds: int = 2, reps: int = 3,
+                space: dict | None = None) -> tuple[dict, float, list]:
+    """Coordinate descent over the product's global knobs, timing triangle
+    counting on L; returns (best options, best seconds, trial log)."""
+    return coordinate_descent(lambda o: time_tc(ctx, L, default_opts(**o), reps), space or SEARCH_SPACE, base,
+                              rounds)
+
+
+BATCH_SPACE = {
+    "batch_blocks": [2, 4, 8, 16],
+    "batch_order": [0, 1],
+    "small_max_mask": [32, 64, 128],
+    "mca_max_mask": [0, 8, 16],
+    "blocks_small": [12, 24, 32],
+    "split_flop": [1 << 16, 1 << 18],
+}
+
+
+def autotune_batch(ctx: Context, A: DeviceCsr, voff, k: int = 4, base: dict | None = None, rounds: int = 2,
+                   reps: int = 2, space: dict | None = None) -> tuple[dict, float, list]:
+    """Coordinate descent over the batching knobs (peeling workgroups per CU,
+    graph order) and the support product's bin thresholds and tile/grid
+    sizes, timing the captured batch step; returns (best options, best ms,
+    trial log)."""
+    def measure(o):
+        return ctx.batch_tc_ktruss(A, voff, k=k, use_graph=True, reps=reps, opts=default_opts(**o))["ms"]
+    return coordinate_descent(measure, space or BATCH_SPACE, base, rounds)

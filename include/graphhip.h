@@ -84,7 +84,7 @@ enum gk_accum {
 #define GK_NUM_KERNELS     6
 enum gk_selector {
     GK_SEL_HEURISTIC = 0,   /* fixed thresholds (the fallback)            */
-    GK_SEL_LEARNED   = 1,   /* MLP cost model, needs gk_model_load/train  */
+    GK_SEL_LEARNED   = 1,   /* MLP cost model, needs gk_model_set/train   */
     GK_SEL_FORCED    = 2    /* every row uses opts.forced_accum           */
 };
 
@@ -103,7 +103,8 @@ typedef struct gk_opts {
     int32_t blocks_long;    /* ... long-row MSA-LDS kernel                           */
     int32_t blocks_mca;     /* ... MCA-wave kernel                                   */
     int32_t blocks_global;  /* ... hash-global and dense-global kernels              */
-    int32_t reserved[2];
+    int32_t batch_blocks;   /* batching: peeling workgroups per CU                   */
+    int32_t batch_order;    /* batching: 0 = graphs largest first, 1 = given order   */
 } gk_opts;
 void gk_opts_default(gk_opts *o);
 
@@ -138,6 +139,16 @@ int  gk_hgraph_er(int64_t n, double mean_degree, uint64_t seed, int relabel,
                   gk_hgraph **out);
 int  gk_hgraph_chung_lu(int64_t n, double exponent, double mean_degree,
                         uint64_t seed, int relabel, gk_hgraph **out);
+/* Many graphs as one block-diagonal graph: graph g is Erdos-Renyi (kind 0,
+ * mean degree mean_degree[g]) or Chung-Lu (kind 1, exponent[g] and
+ * mean_degree[g]) on n[g] vertices with seed[g], built and cleaned like
+ * gk_hgraph_er / gk_hgraph_chung_lu (graphs in parallel on host threads),
+ * and placed on vertices [voff[g], voff[g+1]) of the union; voff gets
+ * ngraphs + 1 entries.                                                      */
+int  gk_hgraph_batch(int64_t ngraphs, const int32_t *kind, const int64_t *n,
+                     const double *exponent, const double *mean_degree,
+                     const uint64_t *seed, int relabel, gk_hgraph **out,
+                     int64_t *voff);
 int  gk_hgraph_grid(int64_t rows, int64_t cols, int diagonal, int relabel,
                     gk_hgraph **out);
 int  gk_hgraph_from_edges(int64_t n, int64_t ne, const int32_t *u,
@@ -215,6 +226,27 @@ int gk_bc_batch_f64(gk_ctx *ctx, const gk_csr *A, const int32_t *sources,
 int gk_bc_batch_dev(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *AT,
                     const int32_t *sources, int64_t nsrc, double *d_bc,
                     const gk_opts *o);
+
+/* ---- batching: many independent graphs in one launch sequence -----------
+ * A is the block-diagonal union of ngraphs simple undirected graphs (graph g
+ * on vertices [voff[g], voff[g+1]), voff[0] = 0, voff[ngraphs] = A.m; an
+ * edge joining two graphs is rejected).  One step computes, for every
+ * graph, its triangle count (src/main.cpp:69-90) and its k-truss
+ * (src/main.cpp:93-141): one support product over the union, a per-graph
+ * reduction, and one persistent kernel that peels the graphs largest first,
+ * one workgroup per graph.  use_graph = 1 captures the step once as a
+ * hipGraph and replays it `reps` times; 0 launches it eagerly each time.
+ * Outputs (after the last repetition): triangles[g], truss_edges[g] (edges
+ * left in graph g's k-truss) and, when d_keep is not NULL, d_keep[p] = 1 for
+ * the entries p of A in the k-truss.  ms_per_rep (optional) gets the mean
+ * device time of one step; rep_digest (optional, reps entries) a hash of
+ * every repetition's per-graph results.  No counterpart in GraphKokkos,
+ * which handles one graph per run.                                          */
+int gk_batch_tc_ktruss_dev(gk_ctx *ctx, const gk_dcsr *A, int64_t ngraphs,
+                           const int64_t *voff, int k, int use_graph, int reps,
+                           uint64_t *triangles, int64_t *truss_edges,
+                           uint8_t *d_keep, double *ms_per_rep,
+                           uint64_t *rep_digest, const gk_opts *o);
 
 /* ---- learned accumulator selector ----------------------------------------
  * A 16 -> 64 -> 64 -> 4 FP32 MLP predicting each accumulator's cost (log2

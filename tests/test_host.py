@@ -171,3 +171,35 @@ def test_dropin_headers_compile():
                         f"-I{ROOT / 'include'}", str(ROOT / "tests" / "cpp" / "dropin_check.cpp")],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_batch_union_is_block_diagonal_copy_of_each_graph():
+    from graphkokkos_amd.graph import batch_config
+    cfg = batch_config(12, seed=3, n_min=50, n_max=400)
+    U = HostGraph.batch(**cfg)
+    voff = U.voff
+    assert voff[0] == 0 and voff[-1] == U.n and np.all(np.diff(voff) == cfg["n"])
+    for g in range(12):
+        if cfg["kind"][g] == 0:
+            G = HostGraph.er(int(cfg["n"][g]), float(cfg["mean_degree"][g]), seed=int(cfg["seed"][g]))
+        else:
+            G = HostGraph.chung_lu(int(cfg["n"][g]), float(cfg["exponent"][g]), float(cfg["mean_degree"][g]),
+                                   seed=int(cfg["seed"][g]))
+        a, b = int(voff[g]), int(voff[g + 1])
+        rst = U.A.Rst[a:b + 1] - U.A.Rst[a]
+        assert np.array_equal(rst, G.A.Rst)
+        assert np.array_equal(U.A.Col[U.A.Rst[a]:U.A.Rst[b]] - a, G.A.Col)
+        assert np.array_equal(U.L.Col[U.L.Rst[a]:U.L.Rst[b]] - a, G.L.Col)
+
+
+def test_batch_config_follows_the_specification():
+    from graphkokkos_amd.graph import batch_config
+    cfg = batch_config(4096, seed=1)
+    n, kind = cfg["n"], cfg["kind"]
+    assert n.min() >= 1024 and n.max() <= 65536
+    assert abs(np.median(np.log(n)) - 0.5 * (np.log(1024) + np.log(65536))) < 0.1   # log-uniform
+    assert (kind == 0).sum() == (kind == 1).sum() == 2048
+    d = cfg["mean_degree"][kind == 0]
+    assert d.min() >= 8 and d.max() <= 32
+    x = cfg["exponent"][kind == 1]
+    assert x.min() >= 2.1 and x.max() <= 2.8
