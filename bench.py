@@ -58,9 +58,25 @@ def dominant(t: dict) -> dict:
             "bytes_per_launch": byts / launches, "achieved_gbs": gbs}
 
 
-def roofline_obj(d: dict) -> dict:
+def measured_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/hbm_traffic.txt),
+    or None when that pair was not measured."""
+    path = ROOT / "profiles" / "hbm_traffic.txt"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        f = line.split()
+        if len(f) == 4 and not line.startswith("#") and f[0] == kernel and f[1] == workload:
+            return float(f[2]) / float(f[3])
+    return None
+
+
+def roofline_obj(d: dict, workload: str | None = None) -> dict:
+    traffic = measured_traffic(d["kernel"], workload) if workload else None
     return {"bound": "hbm", "achieved": round(d["achieved_gbs"], 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(d["achieved_gbs"] / HBM_PEAK_GBS, 5), "traffic": None,
+            "frac": round(d["achieved_gbs"] / HBM_PEAK_GBS, 5),
+            "traffic": round(traffic) if traffic is not None else None,
             "kernel": d["kernel"], "avg_launch_ms": round(d["avg_launch_ms"], 6),
             "bytes_per_launch": round(d["bytes_per_launch"]), "launches": d["launches"]}
 
@@ -200,16 +216,15 @@ def batch_cpu_baseline(a, threads: int) -> dict:
                       f"and the reference's 4-truss loop, {dt:.1f}s"}
 
 
-def calibrate_copy(nbytes: int = 256 << 20) -> None:
+def calibrate_copy(ctx, nbytes: int = 256 << 20) -> None:
     """One device-to-device copy of nbytes (read nbytes, write nbytes): the
     reference kernel for converting FETCH_SIZE / WRITE_SIZE counter readings
     into bytes."""
-    import torch
-    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
-    dst = torch.empty_like(src)
-    torch.cuda.synchronize()
-    dst.copy_(src)
-    torch.cuda.synchronize()
+    from graphkokkos_amd import _lib
+    src, dst = ctx.alloc(nbytes), ctx.alloc(nbytes)
+    _lib.check(_lib.lib().gk_memcpy_d2d(ctx.handle, dst.ptr, src.ptr, nbytes), "gk_memcpy_d2d")
+    src.free()
+    dst.free()
     log(f"calibration copy: {nbytes} bytes read and written")
 
 
@@ -274,6 +289,7 @@ def main() -> int:
                         "kernel_ms": plain["timing"]["kernel_ms"], "select_ms": plain["timing"]["select_ms"],
                         "roofline": roofline_obj(pd),
                         "bytes_alg_all": plain["timing"]["bytes_alg"],
+                        "traffic_all_kernels": measured_traffic("tc-product", f"tc-rmat{a.tc_scale}"),
                         "frac_all_kernels": (plain["timing"]["bytes_alg"] / 1e9) /
                         (plain["timing"]["total_ms"] / 1e3) / HBM_PEAK_GBS}}
         log(f"tc plain: {plain['wall_s'] * 1e3:.2f} ms, T={plain['count']}, dominant {pd['kernel']} "
@@ -367,7 +383,7 @@ def main() -> int:
             log(f"cpu baseline bc: {bc['cpu_baseline']['value']:.3g} source-edges/s")
 
     if a.calibrate_copy:
-        calibrate_copy()
+        calibrate_copy(ctx)
 
     if kt is not None:
         value = E20 / kt["wall_s"]
@@ -378,7 +394,7 @@ def main() -> int:
             "config": {"workload": f"k-truss full decomposition (k=3..max), R-MAT scale {a.ktruss_scale} ef16, "
                                    "a/b/c=.57/.19/.19, seed 1, degree-relabelled int32 CSR",
                        "vertices": G20.n, "edges": E20, "kmax": kt["kmax"], "rounds": kt["rounds"]},
-            "roofline": roofline_obj(kt["dominant"]),
+            "roofline": roofline_obj(kt["dominant"], f"ktruss-rmat{a.ktruss_scale}"),
             "cpu_baseline": cpu,
             "ktruss": {"dev_ms": kt["dev_ms"], "kernel_ms": kt["timing"]["kernel_ms"],
                        "select_ms": kt["timing"]["select_ms"], "bytes_alg": kt["timing"]["bytes_alg"],

@@ -114,6 +114,7 @@ _SIGS = {
     "gk_dfree": (None, [C.c_void_p, C.c_void_p]),
     "gk_memcpy_d2h": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
     "gk_memcpy_h2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
+    "gk_memcpy_d2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
 }
 
 _lib: C.CDLL | None = None

@@ -327,6 +327,12 @@ extern "C" int gk_memcpy_d2h(gk_ctx *ctx, void *dst, const void *src, size_t byt
     GK_HIP(d2h(ctx, dst, src, bytes));
     return GK_OK;
 }
+extern "C" int gk_memcpy_d2d(gk_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    GK_CHECK_ARG(ctx, "ctx is NULL");
+    GK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    GK_HIP(hipStreamSynchronize(ctx->stream));
+    return GK_OK;
+}
 extern "C" int gk_memcpy_h2d(gk_ctx *ctx, void *dst, const void *src, size_t bytes) {
     GK_CHECK_ARG(ctx, "ctx is NULL");
     GK_HIP(h2d_chunked(ctx, dst, src, bytes));

@@ -280,6 +280,7 @@ int  gk_dmalloc(gk_ctx *ctx, size_t bytes, void **dptr);
 void gk_dfree(gk_ctx *ctx, void *dptr);
 int  gk_memcpy_d2h(gk_ctx *ctx, void *dst, const void *src, size_t bytes);
 int  gk_memcpy_h2d(gk_ctx *ctx, void *dst, const void *src, size_t bytes);
+int  gk_memcpy_d2d(gk_ctx *ctx, void *dst, const void *src, size_t bytes);
 
 #ifdef __cplusplus
 }
