@@ -62,6 +62,10 @@ struct gk_ctx {
     gk_timing last{};
     std::vector<hipEvent_t> events;  // reusable timing events
     gk::Model *model = nullptr;
+    // Work buffers of the k-truss runs, kept between calls (grow-only, freed
+    // by gk_finalize): the n-th allocation of a run reuses the n-th buffer.
+    std::vector<std::pair<void *, size_t>> truss_cache;
+    bool truss_cache_busy = false;
 };
 
 struct gk_dcsr {

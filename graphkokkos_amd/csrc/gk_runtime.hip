@@ -272,6 +272,8 @@ extern "C" void gk_finalize(gk_ctx *ctx) {
         if (ctx->scratch[s]) (void)hipFree(ctx->scratch[s]);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     if (ctx->dense_map) (void)hipFree(ctx->dense_map);
+    for (auto &b : ctx->truss_cache)
+        if (b.first) (void)hipFree(b.first);
     if (ctx->model) model_free(ctx->model);
     for (int i = 0; i < gk_ctx::kAux; ++i) {
         if (ctx->aux[i]) (void)hipStreamSynchronize(ctx->aux[i]);

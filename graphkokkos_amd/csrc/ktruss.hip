@@ -106,6 +106,13 @@ __global__ void k_truss_to_sym(int64_t m, const int32_t *__restrict__ rst, const
     }
 }
 
+// out[p] = v[idx[p]]: per-entry values from per-edge values.
+__global__ void k_gather_u32(int64_t n, const int32_t *__restrict__ idx, const uint32_t *__restrict__ v,
+                             uint32_t *__restrict__ out) {
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x)
+        out[p] = v[idx[p]];
+}
+
 int waves_grid(gk_ctx *ctx, int64_t rows) {
     int64_t g = (rows + 3) / 4;
     int64_t cap = (int64_t)ctx->num_cus * 16;
@@ -117,21 +124,50 @@ struct Lgraph {
     int32_t *rst = nullptr, *col = nullptr, *id = nullptr;
 };
 
-// Device buffers of a truss run: two ping-pong L graphs plus supports.
+// Device buffers of a truss run.  The first run on a context at a time
+// draws them from the context's grow-only cache (the n-th allocation reuses
+// the n-th cached buffer when it is large enough), so repeated runs on one
+// graph allocate nothing; a nested run allocates and frees its own.
 struct TrussBufs {
     gk_ctx *ctx;
-    std::vector<void *> allocs;
-    explicit TrussBufs(gk_ctx *c) : ctx(c) {}
+    bool cached;
+    size_t next = 0;
+    std::vector<void *> own;
+    explicit TrussBufs(gk_ctx *c) : ctx(c), cached(!c->truss_cache_busy) {
+        if (cached) ctx->truss_cache_busy = true;
+    }
     ~TrussBufs() {
-        for (void *p : allocs) (void)hipFree(p);
+        if (cached) ctx->truss_cache_busy = false;
+        for (void *p : own) (void)hipFree(p);
     }
     template <class T>
     int alloc(T **p, size_t n) {
-        if (hipMalloc((void **)p, sizeof(T) * (n ? n : 1)) != hipSuccess) {
-            set_error("hipMalloc(%zu) in truss run failed", sizeof(T) * n);
+        const size_t bytes = sizeof(T) * (n ? n : 1);
+        if (cached) {
+            auto &cache = ctx->truss_cache;
+            if (next == cache.size()) cache.emplace_back(nullptr, 0);
+            auto &slot = cache[next++];
+            if (slot.second < bytes) {
+                if (slot.first) {
+                    (void)hipStreamSynchronize(ctx->stream);
+                    (void)hipFree(slot.first);
+                    slot = {nullptr, 0};
+                }
+                if (hipMalloc(&slot.first, bytes) != hipSuccess) {
+                    slot = {nullptr, 0};
+                    set_error("hipMalloc(%zu) in truss run failed", bytes);
+                    return GK_ENOMEM;
+                }
+                slot.second = bytes;
+            }
+            *p = (T *)slot.first;
+            return GK_OK;
+        }
+        if (hipMalloc((void **)p, bytes) != hipSuccess) {
+            set_error("hipMalloc(%zu) in truss run failed", bytes);
             return GK_ENOMEM;
         }
-        allocs.push_back(*p);
+        own.push_back(*p);
         return GK_OK;
     }
 };
@@ -329,18 +365,6 @@ __device__ __forceinline__ int32_t prep_edge(int32_t f, const int32_t *__restric
     return sup[e] > 0 ? (uy ? dv : du) : 0;
 }
 
-__global__ void k_round_prologue(int32_t nP, const int32_t *__restrict__ Fp, int32_t nF,
-                                 const int32_t *__restrict__ F, uint8_t *__restrict__ st,
-                                 uint32_t *__restrict__ truss, uint32_t label, const int32_t *__restrict__ eu,
-                                 const int32_t *__restrict__ ev, const int32_t *__restrict__ sup,
-                                 const int32_t *__restrict__ rst, int32_t *__restrict__ fy, int32_t *__restrict__ fx,
-                                 int32_t *__restrict__ len) {
-    const int32_t stride = gridDim.x * blockDim.x;
-    for (int32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < nP; f += stride) st[Fp[f]] = 2;
-    for (int32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < nF; f += stride)
-        len[f] = prep_edge(f, F, st, truss, label, eu, ev, sup, rst, fy, fx);
-    if (blockIdx.x == 0 && threadIdx.x == 0) len[nF] = 0;
-}
 
 // Frontiers of at most kSmallRound edges get their prologue and the scan of
 // len in one block (k_dround_prologue).
@@ -486,18 +510,6 @@ __device__ __forceinline__ void k_peel_body(int32_t nF, const int32_t *__restric
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctr[0], (unsigned long long)total);
 }
 
-template <int U>
-__global__ __launch_bounds__(256) void k_peel_stream(int32_t nF, const int32_t *__restrict__ pre,
-                                                     const int32_t *__restrict__ F, const int32_t *__restrict__ fy,
-                                                     const int32_t *__restrict__ fx, const int32_t *__restrict__ rst,
-                                                     const int32_t *__restrict__ col, const int32_t *__restrict__ eid,
-                                                     const int64_t *__restrict__ hoff, const int2 *__restrict__ tab,
-                                                     const uint8_t *__restrict__ st, int32_t *__restrict__ sup,
-                                                     int32_t thresh, int32_t *__restrict__ Fn,
-                                                     int32_t *__restrict__ nFn, const NearList nl,
-                                                     unsigned long long *__restrict__ ctr) {
-    k_peel_body<U>(nF, pre, F, fy, fx, rst, col, eid, hoff, tab, st, sup, thresh, Fn, nFn, nl, ctr);
-}
 
 // Device-driven round: prologue + scan in one block,
 // sizes from the control block.  A frontier that is empty or too big for
@@ -558,6 +570,10 @@ __global__ __launch_bounds__(256) void k_dround_stream(const RoundCtl *__restric
                    cur ? buf0 : buf1, nFn, nl, ctr);
 }
 
+// One thread moves the control block to the next round.  (A device-scope
+// fence in every block of the stream kernel, to let its last block do this,
+// costs more than this launch: on gfx950 such a fence writes back the XCD's
+// L2.)
 __global__ void k_dround_advance(RoundCtl *__restrict__ ctl) {
     if (threadIdx.x != 0 || ctl->stop) return;
     ctl->rounds += 1;
@@ -566,6 +582,89 @@ __global__ void k_dround_advance(RoundCtl *__restrict__ ctl) {
     ctl->nF = ctl->nFn;
     ctl->cur ^= 1;
 }
+
+// Device-driven rounds for frontiers of at least kSmallRound edges, in
+// three kernels: k_big_prologue (every block owns a contiguous chunk of the
+// frontier: marks, streamed lengths, the chunk's sum), k_big_scan_top (one
+// block: exclusive scan of the chunk sums) and k_big_scan_apply (every
+// chunk's lengths scanned from its offset into pre).  A frontier below
+// kSmallRound (or empty) stops the batch, which the one-block path of
+// k_dround_prologue then continues.
+constexpr int kBigBlocksMax = 1024;
+
+__global__ __launch_bounds__(256) void k_big_prologue(
+    RoundCtl *__restrict__ ctl, int32_t *__restrict__ buf0, int32_t *__restrict__ buf1, uint8_t *__restrict__ st,
+    uint32_t *__restrict__ truss, uint32_t label, const int32_t *__restrict__ eu, const int32_t *__restrict__ ev,
+    const int32_t *__restrict__ sup, const int32_t *__restrict__ rst, int32_t *__restrict__ fy,
+    int32_t *__restrict__ fx, int32_t *__restrict__ len, int32_t *__restrict__ bsum) {
+    __shared__ int32_t ws[256 / 64];
+    if (ctl->stop) return;
+    const int32_t nF = ctl->nF, nP = ctl->nP;
+    if (nF < kSmallRound) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) ctl->stop = 1;
+        return;
+    }
+    const int32_t *F = ctl->cur ? buf1 : buf0;
+    const int32_t *Fp = ctl->cur ? buf0 : buf1;
+    const int32_t stride = gridDim.x * blockDim.x;
+    for (int32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < nP; f += stride) st[Fp[f]] = 2;
+    const int32_t chunk = (nF + gridDim.x - 1) / gridDim.x;
+    const int32_t f0 = min(nF, (int32_t)blockIdx.x * chunk), f1 = min(nF, f0 + chunk);
+    int32_t sum = 0;
+    for (int32_t f = f0 + threadIdx.x; f < f1; f += blockDim.x) {
+        const int32_t l = prep_edge(f, F, st, truss, label, eu, ev, sup, rst, fy, fx);
+        len[f] = l;
+        sum += l;
+    }
+    sum = dev::wave_sum(sum);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) bsum[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
+}
+
+__global__ __launch_bounds__(kBigBlocksMax) void k_big_scan_top(RoundCtl *__restrict__ ctl, int32_t *__restrict__ bsum,
+                                                                int nb) {
+    __shared__ int32_t ws[kBigBlocksMax / 64];
+    if (ctl->stop) return;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int32_t v = tid < nb ? bsum[tid] : 0;
+    const int32_t incl = dev::wave_incl_scan(v);
+    if (lane == 63) ws[w] = incl;
+    __syncthreads();
+    int32_t before = 0;
+    for (int x = 0; x < w; ++x) before += ws[x];
+    if (tid < nb) bsum[tid] = before + incl - v;
+    if (tid == 0) ctl->nFn = 0;
+}
+
+__global__ __launch_bounds__(256) void k_big_scan_apply(const RoundCtl *__restrict__ ctl,
+                                                        const int32_t *__restrict__ len,
+                                                        const int32_t *__restrict__ boff, int32_t *__restrict__ pre) {
+    __shared__ int32_t ws[256 / 64];
+    if (ctl->stop) return;
+    const int32_t nF = ctl->nF;
+    const int32_t chunk = (nF + gridDim.x - 1) / gridDim.x;
+    const int32_t f0 = min(nF, (int32_t)blockIdx.x * chunk), f1 = min(nF, f0 + chunk);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int32_t carry = boff[blockIdx.x];
+    for (int32_t c = f0; c < f1; c += blockDim.x) {
+        const int32_t f = c + threadIdx.x;
+        const int32_t l = f < f1 ? len[f] : 0;
+        const int32_t incl = dev::wave_incl_scan(l);
+        if (lane == 63) ws[w] = incl;
+        __syncthreads();
+        int32_t before = 0, tot = 0;
+        for (int x = 0; x < 4; ++x) {
+            before += x < w ? ws[x] : 0;
+            tot += ws[x];
+        }
+        if (f < f1) pre[f] = carry + before + incl - l;
+        carry += tot;
+        __syncthreads();
+    }
+    if (f1 == nF && f0 < f1 && threadIdx.x == 0) pre[nF] = carry;
+}
+
 
 __global__ void k_mark_dead(int32_t nF, const int32_t *__restrict__ F, uint8_t *__restrict__ st) {
     for (int32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < nF; f += gridDim.x * blockDim.x) st[F[f]] = 2;
@@ -639,15 +738,18 @@ int grid_items(gk_ctx *ctx, int64_t n) {
 
 // Truss numbers of every L0 edge by incremental peeling.  Round:
 //   F = alive edges with support < k-2 (labelled k-1);  none -> k = min + 3;
-//   dS = F * A masked by A (k_peel_stream): one support off each surviving
-//        edge of every triangle destroyed this round;
+//   dS = F * A masked by A (k_dround_stream): one support off each
+//        surviving edge of every triangle destroyed this round;
 //   F -> removed; compact the adjacency once a quarter of it is dead.
+// Rounds run in device-driven batches (sizes in a RoundCtl block): frontiers
+// below kSmallRound get the one-block prologue, larger ones the chunked
+// prologue + scan; the host checks in once per batch.
 // fixed_k > 0 runs the single-k semantics instead (k_truss, src/main.cpp:93-141):
 // the threshold stays k-2, removed edges are labelled 1, and the run stops at
 // the first round that removes nothing; the returned count then includes that
 // last round, as the reference's iteration count does.
 int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, uint32_t *tl, const gk_opts &o,
-               int64_t *rounds_out, int32_t *kmax_out, int fixed_k = 0) {
+               int64_t *rounds_out, int32_t *kmax_out, int fixed_k = 0, int32_t **sym_eid_out = nullptr) {
     const int64_t m = A->m, nL = L0.nz;
     int32_t *sup, *eu, *ev, *alist, *alist2, *F, *Fn, *fy, *fx, *flen, *fpre, *cnt;
     int32_t *arst[2], *acol[2], *aeid[2];
@@ -698,6 +800,12 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
     GK_HIP(hipMemcpyAsync(arst[0], A->rst, 4 * (m + 1), hipMemcpyDeviceToDevice, ctx->stream));
     GK_HIP(hipMemcpyAsync(acol[0], A->col, 4 * (size_t)A->nz, hipMemcpyDeviceToDevice, ctx->stream));
     k_sym_eid<<<gw, 256, 0, ctx->stream>>>(m, A->rst, A->col, L0.rst, aeid[0]);
+    if (sym_eid_out) {
+        // the edge id of every entry of A, kept for the final per-entry output
+        // (aeid[0] itself is reused by the adjacency compaction)
+        GK_TRY(B.alloc(sym_eid_out, A->nz));
+        GK_HIP(hipMemcpyAsync(*sym_eid_out, aeid[0], 4 * (size_t)A->nz, hipMemcpyDeviceToDevice, ctx->stream));
+    }
     k_iota<<<grid_items(ctx, nL), 256, 0, ctx->stream>>>(nL, alist);
     GK_HIP(hipMemsetAsync(st, 0, nL, ctx->stream));
     GK_HIP(hipMemsetAsync(dctr, 0, 16, ctx->stream));
@@ -741,7 +849,11 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
     int32_t nF = 0;              // current frontier
     bool scan_needed = true;     // the frontier comes from a scan
     bool started = false;
-    constexpr int kBatch = 16;   // device-driven rounds per host check
+    constexpr int kBatch = 16;   // device-driven small rounds per host check
+    constexpr int kBigBatch = 4; // device-driven large rounds per host check
+    const int nbig = (int)std::min<int64_t>(kBigBlocksMax, (int64_t)ctx->num_cus * 2);
+    int32_t *bsum;
+    GK_TRY(B.alloc(&bsum, kBigBlocksMax));
     // Near list (NearList): alive edges with support below nl_hi, built by a
     // full scan of the alive list and kept complete by the peeling kernels.
     // While a level's threshold stays below nl_hi its frontier comes from
@@ -814,30 +926,34 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
         kbest = k - 1;
         int64_t marked = 0;  // edges marked removed by the prologues below
         if (nF >= kSmallRound) {
-            // one large round driven from the host
-            ++rounds;
-            k_round_prologue<<<grid_items(ctx, std::max(nF, nP)), 256, 0, ctx->stream>>>(
-                nP, fb[fc ^ 1], nF, fb[fc], st, tl, label, eu, ev, sup, arst[cur], fy, fx, flen);
-            GK_TRY(exclusive_scan_i32(ctx, flen, fpre, nF + 1));
-            GK_TRY(launch_check("round prologue"));
-            size_t e0;
-            GK_TRY(ev_pair(&e0));
-            GK_HIP(hipMemsetAsync(dn, 0, 4, ctx->stream));
-            GK_HIP(hipEventRecord(ev_pool[e0], ctx->stream));
-            k_peel_stream<4><<<gs, 256, 0, ctx->stream>>>(nF, fpre, fb[fc], fy, fx, arst[cur], acol[cur], aeid[cur],
-                                                          hoff, tab, st, sup, k - 2, fb[fc ^ 1], dn, near_list(),
-                                                          dctr);
-            GK_HIP(hipEventRecord(ev_pool[e0 + 1], ctx->stream));
-            GK_TRY(launch_check("k_peel_stream"));
-            GK_HIP(hipMemcpyAsync(h, dn, 4, hipMemcpyDeviceToHost, ctx->stream));
+            // a batch of large rounds driven from the device
+            *hctl = RoundCtl{nP, nF, 0, fc, 0, 0, 0, 0};
+            GK_HIP(hipMemcpyAsync(dctl, hctl, sizeof(RoundCtl), hipMemcpyHostToDevice, ctx->stream));
+            for (int r = 0; r < kBigBatch; ++r) {
+                k_big_prologue<<<nbig, 256, 0, ctx->stream>>>(dctl, fb[0], fb[1], st, tl, label, eu, ev, sup,
+                                                              arst[cur], fy, fx, flen, bsum);
+                k_big_scan_top<<<1, kBigBlocksMax, 0, ctx->stream>>>(dctl, bsum, nbig);
+                k_big_scan_apply<<<nbig, 256, 0, ctx->stream>>>(dctl, flen, bsum, fpre);
+                size_t e0;
+                GK_TRY(ev_pair(&e0));
+                GK_HIP(hipEventRecord(ev_pool[e0], ctx->stream));
+                k_dround_stream<4><<<gs, 256, 0, ctx->stream>>>(dctl, fb[0], fb[1], fpre, fy, fx, arst[cur],
+                                                                acol[cur], aeid[cur], hoff, tab, st, sup, k - 2,
+                                                                &dctl->nFn, near_list(), dctr);
+                GK_HIP(hipEventRecord(ev_pool[e0 + 1], ctx->stream));
+                k_dround_advance<<<1, 64, 0, ctx->stream>>>(dctl);
+            }
+            GK_TRY(launch_check("device large rounds"));
+            GK_HIP(hipMemcpyAsync(hctl, dctl, sizeof(RoundCtl), hipMemcpyDeviceToHost, ctx->stream));
             GK_HIP(hipMemcpyAsync(h + 2, dnl, 4, hipMemcpyDeviceToHost, ctx->stream));
             GK_HIP(hipStreamSynchronize(ctx->stream));
             nNL = h[2];
-            marked = nP;
-            alive -= nF;
-            nP = nF;
-            nF = h[0];
-            fc ^= 1;
+            rounds += hctl->rounds;
+            alive -= hctl->removed;
+            marked = (int64_t)nP + hctl->removed - hctl->nP;
+            nP = hctl->nP;
+            nF = hctl->nF;
+            fc = hctl->cur;
         } else {
             // a batch of small rounds driven from the device
             *hctl = RoundCtl{nP, nF, 0, fc, 0, 0, 0, 0};
@@ -967,20 +1083,15 @@ extern "C" int gk_truss_decompose_dev(gk_ctx *ctx, const gk_dcsr *A, uint32_t *d
     Lgraph cur;
     GK_TRY(build_lower(ctx, A, B, &cur));
     const int64_t nl = cur.nz;
-    int32_t *lrst0 = cur.rst;
-    // Keep L0's row pointers: the final mapping to symmetric entries needs
-    // them, while `cur` is replaced every round.
-    int32_t *lrst_keep;
-    GK_TRY(B.alloc(&lrst_keep, A->m + 1));
-    GK_HIP(hipMemcpyAsync(lrst_keep, lrst0, 4 * (A->m + 1), hipMemcpyDeviceToDevice, ctx->stream));
     uint32_t *tl;
     GK_TRY(B.alloc(&tl, nl));
     int64_t rounds = 0;
     int32_t kb = 0;
-    GK_TRY(truss_peel(ctx, A, B, cur, tl, o, &rounds, &kb));
+    int32_t *sym_eid = nullptr;
+    GK_TRY(truss_peel(ctx, A, B, cur, tl, o, &rounds, &kb, 0, &sym_eid));
     if (A->nz) {
-        k_truss_to_sym<<<waves_grid(ctx, A->m), 256, 0, ctx->stream>>>(A->m, A->rst, A->col, lrst_keep, tl, d_truss);
-        GK_TRY(launch_check("k_truss_to_sym"));
+        k_gather_u32<<<grid_items(ctx, A->nz), 256, 0, ctx->stream>>>(A->nz, sym_eid, tl, d_truss);
+        GK_TRY(launch_check("k_gather_u32"));
     }
     total.stop();
     ctx->last.total_ms = total.ms();

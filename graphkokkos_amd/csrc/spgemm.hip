@@ -684,7 +684,7 @@ __global__ __launch_bounds__(64) void k_hash_global(Prob P, int cls, int32_t *__
             for (int32_t q = lane; q < mlen; q += 64) acc[q] = 0;
         if (MODE == MODE_SUPPORT)
             for (int32_t x = lane; x < tk.a1 - tk.a0; x += 64) acnt[x] = 0;
-        __threadfence();
+        __threadfence_block();
         __syncthreads();
         for (int32_t q = lane; q < mlen; q += 64) {
             int32_t col = P.mcol[mb + q];
@@ -693,7 +693,7 @@ __global__ __launch_bounds__(64) void k_hash_global(Prob P, int cls, int32_t *__
             if (MODE != MODE_TC) pos[s] = q;
         }
         const int32_t total = build_prefix<64>(P, tk.a0, tk.a1, pre, bst, lane);
-        __threadfence();
+        __threadfence_block();
         __syncthreads();
         const uint32_t c = stream_entries<MODE, 64, 4>(P, tk, mb, pre, bst, total, lane, acc, acnt,
                                                        [&](int32_t b) -> int32_t {
@@ -705,7 +705,7 @@ __global__ __launch_bounds__(64) void k_hash_global(Prob P, int cls, int32_t *__
                 s = (s + 1) & cmask;
             }
         });
-        __threadfence();
+        __threadfence_block();
         __syncthreads();
         flush<MODE, 64>(P, tk, mb, mlen, acc, acnt, lane);
         if (MODE == MODE_TC && P.rowcount) {
@@ -714,7 +714,7 @@ __global__ __launch_bounds__(64) void k_hash_global(Prob P, int cls, int32_t *__
         }
         cnt += c;
         if (P.task_cycles && threadIdx.x == 0) P.task_cycles[ti] = clock64() - tstart;
-        __threadfence();
+        __threadfence_block();
         __syncthreads();
     }
     finish_kernel<MODE>(P, cnt, cls, lane);
@@ -747,11 +747,11 @@ __global__ __launch_bounds__(64) void k_dense_global(Prob P, int cls, int32_t *_
         if (MODE == MODE_SUPPORT)
             for (int32_t x = lane; x < tk.a1 - tk.a0; x += 64) acnt[x] = 0;
         const int32_t total = build_prefix<64>(P, tk.a0, tk.a1, pre, bst, lane);
-        __threadfence();
+        __threadfence_block();
         __syncthreads();
         const uint32_t c = stream_entries<MODE, 64, 4>(P, tk, mb, pre, bst, total, lane, acc, acnt,
                                                        [&](int32_t b) -> int32_t { return map[b] - 1; });
-        __threadfence();
+        __threadfence_block();
         __syncthreads();
         flush<MODE, 64>(P, tk, mb, mlen, acc, acnt, lane);
         for (int32_t q = lane; q < mlen; q += 64) map[P.mcol[mb + q]] = 0;
@@ -761,7 +761,7 @@ __global__ __launch_bounds__(64) void k_dense_global(Prob P, int cls, int32_t *_
         }
         cnt += c;
         if (P.task_cycles && threadIdx.x == 0) P.task_cycles[ti] = clock64() - tstart;
-        __threadfence();
+        __threadfence_block();
         __syncthreads();
     }
     finish_kernel<MODE>(P, cnt, cls, lane);
