@@ -366,6 +366,9 @@ __device__ __forceinline__ int32_t prep_edge(int32_t f, const int32_t *__restric
 }
 
 
+// streamed entries per lane in flight in the peeling kernels
+constexpr int kPeelU = 4;
+
 // Frontiers of at most kSmallRound edges get their prologue and the scan of
 // len in one block (k_dround_prologue).
 constexpr int kSmallRoundThreads = 1024;
@@ -937,7 +940,7 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
                 size_t e0;
                 GK_TRY(ev_pair(&e0));
                 GK_HIP(hipEventRecord(ev_pool[e0], ctx->stream));
-                k_dround_stream<4><<<gs, 256, 0, ctx->stream>>>(dctl, fb[0], fb[1], fpre, fy, fx, arst[cur],
+                k_dround_stream<kPeelU><<<gs, 256, 0, ctx->stream>>>(dctl, fb[0], fb[1], fpre, fy, fx, arst[cur],
                                                                 acol[cur], aeid[cur], hoff, tab, st, sup, k - 2,
                                                                 &dctl->nFn, near_list(), dctr);
                 GK_HIP(hipEventRecord(ev_pool[e0 + 1], ctx->stream));
@@ -965,7 +968,7 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
                 size_t e0;
                 GK_TRY(ev_pair(&e0));
                 GK_HIP(hipEventRecord(ev_pool[e0], ctx->stream));
-                k_dround_stream<4><<<gsd, 256, 0, ctx->stream>>>(dctl, fb[0], fb[1], fpre, fy, fx, arst[cur],
+                k_dround_stream<kPeelU><<<gsd, 256, 0, ctx->stream>>>(dctl, fb[0], fb[1], fpre, fy, fx, arst[cur],
                                                                  acol[cur], aeid[cur], hoff, tab, st, sup, k - 2,
                                                                  &dctl->nFn, near_list(), dctr);
                 GK_HIP(hipEventRecord(ev_pool[e0 + 1], ctx->stream));

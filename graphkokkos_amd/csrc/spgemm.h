@@ -29,6 +29,10 @@ constexpr int kWindowHashLong = 2048;
 // long mask rows: one workgroup of kWindowWavesLong waves per task
 constexpr int kWindowWavesLong = 4;
 constexpr int kWindowBlocksLong = 8;  // per CU
+// streamed entries per lane in flight in the window kernels
+constexpr int kWindowU = 16;
+// ... in the MCA-wave kernel
+constexpr int kMcaU = 8;
 constexpr int kNumClass = GK_KERNEL_MSA_LONG + 1;
 constexpr int kClassMsaLarge = GK_NUM_ACC;
 constexpr int kFlopBins = 32;

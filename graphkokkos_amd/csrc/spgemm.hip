@@ -532,7 +532,7 @@ __global__ __launch_bounds__(64 * NW) void k_msa_window(Prob P, int cls) {
         if constexpr (NW == 1) total = build_prefix<64>(P, tk.a0, tk.a1, pre, bst, tid);
         else total = build_prefix_block<NT>(P, tk.a0, tk.a1, pre, bst, wsum);
         __syncthreads();
-        const uint32_t c = stream_entries<MODE, NT, 4>(P, tk, mb, pre, bst, total, tid, acc, acnt,
+        const uint32_t c = stream_entries<MODE, NT, kWindowU>(P, tk, mb, pre, bst, total, tid, acc, acnt,
                                                        [&](int32_t b) -> int32_t {
             if (b < kWindow) {
                 if (!((bm[b >> 5] >> (b & 31)) & 1u)) return -1;
@@ -613,28 +613,44 @@ __global__ __launch_bounds__(64) void k_mca_wave(Prob P, int cls) {
             const int32_t *m = mk[tile];
             const int32_t mlo = m[0], mhi = m[mlen - 1];
             int32_t kk = 0, nxt = pre[tile][1], base = bst[tile][0];
-            for (int32_t g = gl; g < total; g += G) {
-                while (g >= nxt) {
-                    ++kk;
-                    nxt = pre[tile][kk + 1];
-                    base = bst[tile][kk];
-                }
-                const int32_t t = base + g;
-                const int32_t b = P.bcol[t];
-                int32_t q = -1;
-                if (b >= mlo && b <= mhi) {
-                    int32_t l = 0, h = mlen;  // lower_bound in m[0, mlen)
-                    while (l < h) {
-                        int32_t mid = (l + h) >> 1;
-                        if (m[mid] < b) l = mid + 1; else h = mid;
+            // kMcaU streamed entries per lane in flight
+            for (int32_t g0 = gl; g0 < total; g0 += G * kMcaU) {
+                int32_t t[kMcaU], kx[kMcaU], b[kMcaU];
+#pragma unroll
+                for (int u = 0; u < kMcaU; ++u) {
+                    const int32_t g = g0 + G * u;
+                    t[u] = -1;
+                    kx[u] = 0;
+                    if (g < total) {
+                        while (g >= nxt) {
+                            ++kk;
+                            nxt = pre[tile][kk + 1];
+                            base = bst[tile][kk];
+                        }
+                        t[u] = base + g;
+                        kx[u] = kk;
                     }
-                    if (m[l] == b) q = l;
                 }
-                if (MODE == MODE_TC) {
-                    const uint64_t found = __ballot(q >= 0);
-                    if (gl == 0) cnt_task += __popcll((found >> (tile * G)) & ((G == 64) ? ~0ull : ((1ull << G) - 1)));
-                } else if (q >= 0) {
-                    Hit<MODE>::on(P, acc[tile], acnt[tile], q, mb, tk.a0, kk, t, cnt_task);
+#pragma unroll
+                for (int u = 0; u < kMcaU; ++u) b[u] = t[u] >= 0 ? P.bcol[t[u]] : -1;
+#pragma unroll
+                for (int u = 0; u < kMcaU; ++u) {
+                    int32_t q = -1;
+                    if (t[u] >= 0 && b[u] >= mlo && b[u] <= mhi) {
+                        int32_t l = 0, h = mlen;  // lower_bound in m[0, mlen)
+                        while (l < h) {
+                            int32_t mid = (l + h) >> 1;
+                            if (m[mid] < b[u]) l = mid + 1; else h = mid;
+                        }
+                        if (m[l] == b[u]) q = l;
+                    }
+                    if (MODE == MODE_TC) {
+                        const uint64_t found = __ballot(q >= 0);
+                        if (gl == 0)
+                            cnt_task += __popcll((found >> (tile * G)) & ((G == 64) ? ~0ull : ((1ull << G) - 1)));
+                    } else if (q >= 0) {
+                        Hit<MODE>::on(P, acc[tile], acnt[tile], q, mb, tk.a0, kx[u], t[u], cnt_task);
+                    }
                 }
             }
         }
