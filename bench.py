@@ -148,6 +148,61 @@ def bc_bytes(n: int, nnz: int, levels: int, nsrc: int) -> float:
     return 2 * levels * (4 * (n + 1) + 4 * nnz) + 24.0 * nsrc * nnz + 24.0 * nsrc * n
 
 
+def run_learned_graphs(ctx, tuned_h: dict, tuned_l: dict) -> dict:
+    """The learned selector (already trained) against the fixed-threshold
+    heuristic on the other two graphs of the learned-selector evaluation:
+    a triangulated 2048 x 2048 grid (one diagonal per cell) and a Chung-Lu
+    graph with exponent 2.0, 4M vertices, mean degree 16.  Triangle counting
+    in four variants and the full truss decomposition with each selector;
+    counts and truss numbers must agree."""
+    from graphkokkos_amd import HostGraph, SEL_LEARNED
+    from graphkokkos_amd import _lib
+    import ctypes as C
+    out = {}
+    for name, make in (("grid-2048x2048-diag", lambda: HostGraph.grid(2048)),
+                       ("chung-lu-4M-exp2.0-d16", lambda: HostGraph.chung_lu(1 << 22, 2.0, 16.0, seed=1))):
+        t0 = time.perf_counter()
+        G = make()
+        log(f"{name}: n={G.n} |E|={G.edges} built in {time.perf_counter() - t0:.1f}s")
+        res = {"edges": G.edges}
+        base = None
+        for var, opts in (("plain", None), ("tuned_heuristic", tuned_h), ("learned", {"selector": SEL_LEARNED}),
+                          ("learned_tuned", tuned_l)):
+            r = run_tc(ctx, G.L, 5, 1, opts)
+            if base is None:
+                base = r
+            assert r["count"] == base["count"], f"{name}: {var} changed the triangle count"
+            res[f"tc_{var}_ms"] = r["wall_s"] * 1e3
+        res["triangles"] = base["count"]
+        res["tc_learned_speedup"] = res["tc_plain_ms"] / res["tc_learned_ms"]
+        res["tc_learned_tuned_speedup"] = res["tc_plain_ms"] / res["tc_learned_tuned_ms"]
+        dA = ctx.upload(G.A)
+        buf = ctx.alloc(4 * max(G.A.nz, 1))
+        kmax = C.c_int32(0)
+        truss = {}
+        for var, opts in (("heuristic", None), ("learned", {"selector": SEL_LEARNED})):
+            o = _lib.default_opts(**(opts or {}))
+            times = []
+            for _ in range(3):
+                t1 = time.perf_counter()
+                _lib.check(_lib.lib().gk_truss_decompose_dev(ctx.handle, dA.handle, buf.ptr, C.byref(kmax),
+                                                             C.byref(o)), "gk_truss_decompose_dev")
+                ctx.sync()
+                times.append(time.perf_counter() - t1)
+            truss[var] = buf.to_host(np.uint32, G.A.nz)
+            res[f"ktruss_{var}_ms"] = min(times) * 1e3
+        assert np.array_equal(truss["heuristic"], truss["learned"]), f"{name}: selectors disagree on truss numbers"
+        res["kmax"] = int(kmax.value)
+        res["ktruss_learned_speedup"] = res["ktruss_heuristic_ms"] / res["ktruss_learned_ms"]
+        buf.free()
+        dA.free()
+        log(f"{name}: tc plain {res['tc_plain_ms']:.2f} ms, learned {res['tc_learned_ms']:.2f}, "
+            f"learned+tuned {res['tc_learned_tuned_ms']:.2f}; k-truss heuristic {res['ktruss_heuristic_ms']:.1f} ms, "
+            f"learned {res['ktruss_learned_ms']:.1f} (kmax {res['kmax']})")
+        out[name] = res
+    return out
+
+
 def run_batch(ctx, a) -> dict:
     """Triangle count + 4-truss of every graph of a seeded batch of uneven
     graphs (vertex counts log-uniform in [1024, 65536], half Erdos-Renyi with
@@ -249,6 +304,8 @@ def main() -> int:
     ap.add_argument("--tc-variants", default="all", choices=["all", "plain"],
                     help="plain: only the fixed-heuristic triangle count (no selector training or autotuning)")
     ap.add_argument("--batch-graphs", type=int, default=4096)
+    ap.add_argument("--no-learned-graphs", action="store_true",
+                    help="skip the grid and Chung-Lu graphs of the learned-selector evaluation")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="end with a 256 MiB device-to-device copy, the known-size kernel that scales "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes")
@@ -327,6 +384,8 @@ def main() -> int:
             log(f"tc {name}: {r['wall_s'] * 1e3:.2f} ms ({plain['wall_s'] / r['wall_s']:.3f}x plain)")
         tc.update(res)
         tc["selector_training"] = rep
+        if not a.no_learned_graphs:
+            tc["other_graphs"] = run_learned_graphs(ctx, tuned_h, tuned_l)
     bc = None
     sources = None
     if "bc" in legs:

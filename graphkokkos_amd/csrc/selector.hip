@@ -1,23 +1,25 @@
 // Learned accumulator selector: a 16 -> 64 -> 64 -> 4 FP32 MLP that
 // predicts, for every row of a masked product, the cost of each accumulator
-// (MSA-LDS, MCA-wave, hash-global, dense-global) in log2 CU-cycles; the row
+// (MSA-LDS, MCA-wave, hash-global, dense-global) in log2 ns of device time; the row
 // takes the accumulator with the smallest prediction.  Rows the chosen
 // accumulator cannot hold fall back by the same capacity rules as the
 // fixed-threshold heuristic, which stays the default selector.
 //
 // Training data come from the product itself: for every accumulator the
 // product runs with that accumulator forced and every task records its
-// clock64 duration; a row's label is the sum over its tasks divided by the
-// number of tasks the kernel keeps resident per CU (cycles of CU time).
+// clock64 duration; a row's label (log2 ns) is its share of its class
+// kernel's measured time, split in proportion to the rows' task durations.
 //
 // Kernels, all hand-written:
 //   k_features      16 features per row from the row statistics
-//   k_mlp_select    features -> MLP forward -> argmin, weights in LDS
-//   k_mlp_forward   MLP forward over a feature matrix (evaluation, tests)
+//   k_mlp_select    features -> MLP forward -> argmin, weights in LDS,
+//                   hidden layers on FP32 MFMA, 32 rows per wave
+//   k_mlp_forward   the same forward over a feature matrix (evaluation)
 //   k_mlp_grad      masked mean-squared-error loss and its gradient,
 //                   per-block LDS accumulation then one atomic per weight
 //   k_adam          Adam update
-//   k_row_cycles    per-row label aggregation from per-task durations
+//   k_row_cycles    per-row sums of task durations (labels: shares of the
+//                   class kernel's measured time)
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -119,33 +121,141 @@ __device__ __forceinline__ void mlp_forward(const float *__restrict__ w, const f
     }
 }
 
-__global__ __launch_bounds__(256) void k_mlp_select(int64_t m, RowStats st, int same_am,
-                                                    const float *__restrict__ params) {
+// ---- inference on matrix cores --------------------------------------------
+// The two hidden layers are dense products, [32 x 16] x [16 x 64] and
+// [32 x 64] x [64 x 64] per tile of 32 samples, run on FP32 MFMA
+// (v_mfma_f32_32x32x2f32: A 32x2, B 2x32, D 32x32 in 16 registers; lane l
+// supplies A[l % 32][l / 32] and B[l / 32][l % 32], and holds
+// D[8 (v / 4) + 4 (l / 32) + v % 4][l % 32] in register v).  One wave per
+// tile; the tile's activations pass through LDS.
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+constexpr int kTile = 32;
+constexpr int kMlpWaves = 4;
+
+struct TileLds {
+    float x[kTile][kIn + 1];   // normalised features
+    float h[kTile][kH + 1];    // hidden activations (layer 1, then layer 2)
+};
+
+__device__ __forceinline__ int d_row(int v, int lane) { return 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3); }
+
+// Orders one wave's LDS writes before its following reads by other lanes
+// (a wave's LDS accesses complete in order; this keeps the compiler from
+// moving them across).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// y[k] of sample (lane % 32) for k = 0..3 (lanes >= 32 return garbage).
+__device__ __forceinline__ void mlp_tile(const float *__restrict__ w, TileLds &t, float *y) {
+    const int lane = threadIdx.x & 63, r = lane & 31, hi = lane >> 5;
+    floatx16 acc[2] = {floatx16{}, floatx16{}};
+    // layer 1: K = 16 in 8 steps of 2, both 32-column halves per step
+#pragma unroll 2
+    for (int s = 0; s < kIn / 2; ++s) {
+        const int k = 2 * s + hi;
+        const float a = t.x[r][k];
+        acc[0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, w[oW1 + k * kH + r], acc[0], 0, 0, 0);
+        acc[1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, w[oW1 + k * kH + 32 + r], acc[1], 0, 0, 0);
+    }
+    wave_sync();
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const float z = acc[c][v] + w[oB1 + 32 * c + r];
+            t.h[d_row(v, lane)][32 * c + r] = z > 0.f ? z : 0.f;
+        }
+    wave_sync();
+    // layer 2: K = 64 in 32 steps of 2
+    acc[0] = floatx16{};
+    acc[1] = floatx16{};
+#pragma unroll 2
+    for (int s = 0; s < kH / 2; ++s) {
+        const int k = 2 * s + hi;
+        const float a = t.h[r][k];
+        acc[0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, w[oW2 + k * kH + r], acc[0], 0, 0, 0);
+        acc[1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, w[oW2 + k * kH + 32 + r], acc[1], 0, 0, 0);
+    }
+    wave_sync();
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const float z = acc[c][v] + w[oB2 + 32 * c + r];
+            t.h[d_row(v, lane)][32 * c + r] = z > 0.f ? z : 0.f;
+        }
+    wave_sync();
+    // output layer on the vector units: lane r < 32 takes sample r
+#pragma unroll
+    for (int o = 0; o < kOut; ++o) y[o] = w[oB3 + o];
+#pragma unroll 4
+    for (int j = 0; j < kH; ++j) {
+        const float h = t.h[r][j];
+#pragma unroll
+        for (int o = 0; o < kOut; ++o) y[o] += h * w[oW3 + j * kOut + o];
+    }
+    wave_sync();
+}
+
+__global__ __launch_bounds__(64 * kMlpWaves) void k_mlp_select(int64_t m, RowStats st, int same_am,
+                                                               const float *__restrict__ params) {
     __shared__ float w[kParams];
+    __shared__ TileLds tl[kMlpWaves];
     for (int x = threadIdx.x; x < kParams; x += blockDim.x) w[x] = params[x];
     __syncthreads();
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
-        float f[kIn], h1[kH], h2[kH], y[kOut];
-        row_features(st, i, same_am, f);
-        mlp_forward(w, f, h1, h2, y);
-        int best = 0;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    TileLds &t = tl[wv];
+    const int64_t ntiles = (m + kTile - 1) / kTile;
+    for (int64_t tile = (int64_t)blockIdx.x * kMlpWaves + wv; tile < ntiles; tile += (int64_t)gridDim.x * kMlpWaves) {
+        const int64_t i = tile * kTile + (lane & 31);
+        if (lane < kTile) {
+            float f[kIn];
+            if (i < m) row_features(st, i, same_am, f);
+            else
+                for (int a = 0; a < kIn; ++a) f[a] = w[oMean + a];
 #pragma unroll
-        for (int o = 1; o < kOut; ++o)
-            if (y[o] < y[best]) best = o;
-        st.acc[i] = (int8_t)best;
+            for (int a = 0; a < kIn; ++a) t.x[lane][a] = (f[a] - w[oMean + a]) / w[oStd + a];
+        }
+        wave_sync();
+        float y[kOut];
+        mlp_tile(w, t, y);
+        if (lane < kTile && i < m) {
+            // cheapest predicted accumulator among those that can hold the row
+            const int32_t mlen = st.mlen[i];
+            const bool can[kOut] = {mlen <= kLdsMaxMask, mlen <= kMcaMaxMask, mlen <= kHashGlobalMaxMask, true};
+            int best = GK_ACC_DENSE_GLOBAL;
+#pragma unroll
+            for (int o = 0; o < kOut; ++o)
+                if (can[o] && y[o] < y[best]) best = o;
+            st.acc[i] = (int8_t)best;
+        }
     }
 }
 
-__global__ __launch_bounds__(256) void k_mlp_forward(int64_t n, const float *__restrict__ X,
-                                                     const float *__restrict__ params, float *__restrict__ Y) {
+__global__ __launch_bounds__(64 * kMlpWaves) void k_mlp_forward(int64_t n, const float *__restrict__ X,
+                                                                const float *__restrict__ params,
+                                                                float *__restrict__ Y) {
     __shared__ float w[kParams];
+    __shared__ TileLds tl[kMlpWaves];
     for (int x = threadIdx.x; x < kParams; x += blockDim.x) w[x] = params[x];
     __syncthreads();
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        float h1[kH], h2[kH], y[kOut];
-        mlp_forward(w, X + i * kIn, h1, h2, y);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    TileLds &t = tl[wv];
+    const int64_t ntiles = (n + kTile - 1) / kTile;
+    for (int64_t tile = (int64_t)blockIdx.x * kMlpWaves + wv; tile < ntiles; tile += (int64_t)gridDim.x * kMlpWaves) {
+        const int64_t i = tile * kTile + (lane & 31);
+        if (lane < kTile)
 #pragma unroll
-        for (int o = 0; o < kOut; ++o) Y[i * kOut + o] = y[o];
+            for (int a = 0; a < kIn; ++a)
+                t.x[lane][a] = i < n ? (X[i * kIn + a] - w[oMean + a]) / w[oStd + a] : 0.f;
+        wave_sync();
+        float y[kOut];
+        mlp_tile(w, t, y);
+        if (lane < kTile && i < n)
+#pragma unroll
+            for (int o = 0; o < kOut; ++o) Y[i * kOut + o] = y[o];
     }
 }
 
@@ -226,7 +336,7 @@ __global__ void k_adam(int n, float *__restrict__ p, const float *__restrict__ g
     }
 }
 
-// Label aggregation: CU-cycles of every row from the durations of its tasks.
+// Label aggregation: clock64 cycles of every row summed over its tasks.
 __global__ void k_row_cycles(int32_t lo, int32_t hi, const Task *__restrict__ tasks,
                              const long long *__restrict__ cyc, float per_cu, double *__restrict__ row) {
     for (int32_t t = lo + blockIdx.x * blockDim.x + threadIdx.x; t < hi; t += gridDim.x * blockDim.x)
@@ -433,7 +543,7 @@ extern "C" int gk_model_train(gk_ctx *ctx, const float *X, const float *T, const
 }
 
 // Features (row statistics) and cost labels of every row of C<M> = A*B for
-// every accumulator.  X: m x 16; T: m x 4 log2 CU-cycles; valid: m x 4
+// every accumulator.  X: m x 16; T: m x 4 log2 ns of device time; valid: m x 4
 // (0 where the row has no work or the accumulator cannot hold its mask).
 extern "C" int gk_selector_samples(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcsr *M, int mode,
                                    float *X, float *T, uint8_t *valid) {
@@ -465,10 +575,11 @@ extern "C" int gk_selector_samples(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr 
     GK_TRY(b.alloc(&d_count, 1));
     std::vector<double> hrow(m);
     std::vector<int8_t> hacc(m);
-    // resident tasks per CU of each kernel class (its persistent grid)
-    const float per_cu[kNumClass] = {(float)kLdsSmallBlocksPerCu, (float)kMcaBlocksPerCu * 4.f,
-                                     (float)kGlobalBlocksPerCu, (float)kGlobalBlocksPerCu,
-                                     (float)kWindowBlocksLong};
+    // A row's label is its share of the device time of the class kernel it
+    // ran in: the kernel's measured time split over its rows in proportion
+    // to their tasks' clock64 durations.  (Durations alone over-count
+    // kernels whose tasks wait on each other, such as the four tiles of an
+    // MCA wave.)
     for (int a = 0; a < GK_NUM_ACC; ++a) {
         gk_opts o;
         gk_opts_default(&o);
@@ -490,16 +601,23 @@ extern "C" int gk_selector_samples(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr 
             const int32_t lo = off[c * kFlopBins], hi = off[(c + 1) * kFlopBins];
             if (hi > lo)
                 k_row_cycles<<<std::min(1024, (hi - lo + 255) / 256), 256, 0, ctx->stream>>>(lo, hi, plan.tasks, cyc,
-                                                                                          per_cu[c], rowc);
+                                                                                          1.0f, rowc);
         }
         GK_TRY(launch_check("k_row_cycles"));
         GK_HIP(d2h(ctx, hrow.data(), rowc, sizeof(double) * m));
+        double cls_cycles[kNumClass] = {};
+        for (int64_t i = 0; i < m; ++i)
+            if (hacc[i] >= 0) cls_cycles[hacc[i]] += hrow[i];
+        double ns_per_cycle[kNumClass] = {};
+        for (int c = 0; c < kNumClass; ++c)
+            ns_per_cycle[c] = cls_cycles[c] > 0 ? pi.acc_ms[c] * 1e6 / cls_cycles[c] : 0.0;
         for (int64_t i = 0; i < m; ++i) {
             int c = hacc[i];
             const int ran = c == kClassMsaLarge ? (int)GK_ACC_MSA_LDS : c;
-            const bool ok = c >= 0 && ran == a && hrow[i] > 0;
+            const double ns = c >= 0 ? hrow[i] * ns_per_cycle[c] : 0.0;
+            const bool ok = c >= 0 && ran == a && ns > 0;
             valid[i * GK_NUM_ACC + a] = ok;
-            T[i * GK_NUM_ACC + a] = ok ? (float)std::log2(hrow[i]) : 0.f;
+            T[i * GK_NUM_ACC + a] = ok ? (float)std::log2(ns) : 0.f;
         }
     }
     return GK_OK;

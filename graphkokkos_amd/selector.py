@@ -2,7 +2,7 @@
 
 The network, its training and its inference run in libgraphhip.so
 (selector.hip): a 16 -> 64 -> 64 -> 4 FP32 MLP over per-row features that
-predicts each accumulator's cost of a row in log2 CU-cycles.  This module
+predicts each accumulator's cost of a row in log2 ns of device time.  This module
 collects training samples from the product's own task timers, trains, saves
 and loads the parameters, and runs the autotuner over the product's global
 knobs (bin thresholds, tile widths, persistent grid sizes).
@@ -34,7 +34,7 @@ def _f32(a):
 
 def samples(ctx: Context, A: DeviceCsr, B: DeviceCsr | None = None, M: DeviceCsr | None = None,
             mode: int = 0) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
-    """Features X (m x 16), labels T (m x 4, log2 CU-cycles) and validity
+    """Features X (m x 16), labels T (m x 4, log2 ns) and validity
     (m x 4) of every row of C<M> = A*B; mode 0 = triangle counting."""
     B = B or A
     M = M or A

@@ -250,7 +250,7 @@ int gk_batch_tc_ktruss_dev(gk_ctx *ctx, const gk_dcsr *A, int64_t ngraphs,
 
 /* ---- learned accumulator selector ----------------------------------------
  * A 16 -> 64 -> 64 -> 4 FP32 MLP predicting each accumulator's cost (log2
- * CU-cycles) of a row from 16 row features; GK_SEL_LEARNED picks the
+ * ns of device time) of a row from 16 row features; GK_SEL_LEARNED picks the
  * cheapest prediction per row.  No counterpart in GraphKokkos (its products
  * take the accumulator from the caller: mxmm_msa vs mxmm_mca).
  * Parameter vector: W1[16x64] b1[64] W2[64x64] b2[64] W3[64x4] b3[4]
@@ -269,8 +269,8 @@ int gk_model_train(gk_ctx *ctx, const float *X, const float *T, const uint8_t *v
                    int64_t n, int epochs, int batch, float lr, uint64_t seed,
                    float *final_loss);
 /* Training samples of C<M> = A*B (mode 0: triangle counting, 1: values):
- * X[m x 16] features, T[m x 4] measured log2 CU-cycles per accumulator,
- * valid[m x 4]. */
+ * X[m x 16] features, T[m x 4] measured log2 ns per accumulator (the row's
+ * share of its kernel's time), valid[m x 4]. */
 int gk_selector_samples(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B,
                         const gk_dcsr *M, int mode, float *X, float *T,
                         uint8_t *valid);
