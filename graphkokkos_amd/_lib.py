@@ -39,7 +39,7 @@ class GkOpts(C.Structure):
                 ("split_flop", C.c_int32), ("block_threads", C.c_int32), ("verbose", C.c_int32),
                 ("small_max_mask", C.c_int32), ("blocks_small", C.c_int32), ("blocks_long", C.c_int32),
                 ("blocks_mca", C.c_int32), ("blocks_global", C.c_int32), ("batch_blocks", C.c_int32),
-                ("batch_order", C.c_int32)]
+                ("batch_order", C.c_int32), ("hub_window", C.c_int32)]
 
 
 class GkTiming(C.Structure):
@@ -109,7 +109,8 @@ _SIGS = {
     "gk_model_train": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_uint8),
                                  C.c_int64, C.c_int, C.c_int, C.c_float, C.c_uint64, C.POINTER(C.c_float)]),
     "gk_selector_samples": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
-                                      C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_uint8)]),
+                                      C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_uint8),
+                                      C.c_void_p]),
     "gk_dmalloc": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
     "gk_dfree": (None, [C.c_void_p, C.c_void_p]),
     "gk_memcpy_d2h": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),

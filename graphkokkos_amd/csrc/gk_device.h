@@ -38,6 +38,7 @@ enum ScratchSlot {
     SC_TMP0, SC_TMP1, SC_TMP2, SC_TMP3, SC_TMP4, SC_TMP5, SC_TMP6, SC_TMP7,
     SC_CUB,           // rocprim temporary storage
     SC_MODEL,         // selector network parameters
+    SC_HUB,           // triangle counting: bitmaps of the hub rows of B
     SC_NUM
 };
 

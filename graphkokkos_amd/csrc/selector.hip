@@ -546,8 +546,9 @@ extern "C" int gk_model_train(gk_ctx *ctx, const float *X, const float *T, const
 // every accumulator.  X: m x 16; T: m x 4 log2 ns of device time; valid: m x 4
 // (0 where the row has no work or the accumulator cannot hold its mask).
 extern "C" int gk_selector_samples(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcsr *M, int mode,
-                                   float *X, float *T, uint8_t *valid) {
+                                   float *X, float *T, uint8_t *valid, const gk_opts *opts) {
     GK_CHECK_ARG(ctx && A && B && M && X && T && valid, "null argument");
+    const gk_opts base = resolve_opts(opts);
     GK_CHECK_ARG(mode == 0 || mode == 1, "mode must be 0 (triangle counting) or 1 (values)");
     GK_TRY(require_sorted(M, "M"));
     const int64_t m = A->m;
@@ -562,7 +563,7 @@ extern "C" int gk_selector_samples(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr 
     GK_TRY(b.alloc(&dacc, m));
     // features from the row statistics
     RowStats st;
-    GK_TRY(row_stats(ctx, A, B, M, &st));
+    GK_TRY(row_stats(ctx, A, B, M, &st, hub_rows(B, mode == 0 ? MODE_TC : MODE_VALUES, base)));
     if (m) {
         const int g = (int)std::max<int64_t>(1, std::min<int64_t>((m + 255) / 256, (int64_t)ctx->num_cus * 4));
         k_features<<<g, 256, 0, ctx->stream>>>(m, st, same_am, dX);
@@ -581,8 +582,7 @@ extern "C" int gk_selector_samples(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr 
     // kernels whose tasks wait on each other, such as the four tiles of an
     // MCA wave.)
     for (int a = 0; a < GK_NUM_ACC; ++a) {
-        gk_opts o;
-        gk_opts_default(&o);
+        gk_opts o = base;
         o.selector = GK_SEL_FORCED;
         o.forced_accum = a;
         // task durations: one slot per task (tasks <= A entries + rows)

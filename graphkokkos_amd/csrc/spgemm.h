@@ -44,7 +44,9 @@ constexpr int kMcaBlocksPerCu = 8;
 constexpr int kGlobalBlocksPerCu = 4;
 
 struct RowStats {
-    int64_t *flop;   // sum over A row of |B_k|
+    int64_t *flop;   // words the window kernels read per A row: sum of |B_k|, with
+                     // hub rows counted by their bitmap words where that is fewer
+    int64_t *sflop;  // sum over A row of |B_k| (what the other kernels stream)
     int32_t *mlen;   // mask row length
     int32_t *alen;   // A row length
     int32_t *bmax;   // max |B_k|
@@ -57,6 +59,7 @@ struct SelectParams {
     int32_t mca_max_mask, lds_max_mask, dense_min_mask;
     int32_t split_flop;
     int32_t small_max_mask;
+    int32_t hub;     // hub bitmaps in use (window classes read rows by the cheaper form)
 };
 
 struct Task {
@@ -89,7 +92,16 @@ struct Prob {
     long long *__restrict__ task_cycles;        // optional: clock64 duration of every task
     const Task *__restrict__ tasks;
     const int32_t *__restrict__ bin_off;
+    // Triangle counting with hub bitmaps: row k < hub_n of B as hub_w
+    // words at hub + k * hub_w (bit c set iff column c is in the row).
+    const uint32_t *__restrict__ hub;
+    int32_t hub_n, hub_w;
 };
+
+// Whether A entry k of a window task reads B row k as bitmap words.
+__host__ __device__ __forceinline__ bool hub_row(int32_t hub_n, int32_t k, int32_t len) {
+    return k < hub_n && ((k + 31) >> 5) < len;
+}
 
 // Launch shape of one product's class kernels and the scratch of the
 // global accumulators (setup_classes), so launch_classes only launches.
@@ -117,7 +129,14 @@ struct ProductInfo {
 // and repeated products accumulate); reset=true starts a new record.
 void record_timing(gk_ctx *ctx, const ProductInfo &pi, bool reset);
 
-int row_stats(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcsr *M, RowStats *st);
+// hub_n > 0: rows of B below hub_n are read as bitmaps by the window kernels
+// (RowStats.flop counts their words).
+int row_stats(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcsr *M, RowStats *st, int32_t hub_n = 0);
+// Bitmaps of rows [0, hub_n) of B (B strictly lower: every column below its
+// row) into the context's hub scratch; fills P's hub fields.
+int build_hub(gk_ctx *ctx, const gk_dcsr *B, int32_t hub_n, Prob *P);
+// Hub rows for C<M> = A*B in the given mode under the options (0: none).
+int32_t hub_rows(const gk_dcsr *B, int mode, const gk_opts &o);
 int plan_tasks(gk_ctx *ctx, const gk_dcsr *A, RowStats &st, const SelectParams &sp, Plan *plan);
 SelectParams select_params(const gk_opts &o);
 int zero_u32(gk_ctx *ctx, void *p, int64_t n);

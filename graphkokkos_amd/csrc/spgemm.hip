@@ -44,25 +44,29 @@ namespace gk {
 __global__ __launch_bounds__(256) void k_row_stats(int64_t m, const int32_t *__restrict__ arst,
                                                    const int32_t *__restrict__ acol,
                                                    const int32_t *__restrict__ brst,
-                                                   const int32_t *__restrict__ mrst, RowStats st) {
+                                                   const int32_t *__restrict__ mrst, RowStats st,
+                                                   int32_t hub_n) {
     const int lane = threadIdx.x & 63;
     const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
     for (int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < m; i += waves) {
         const int32_t a0 = arst[i], a1 = arst[i + 1];
-        int64_t flop = 0;
+        int64_t flop = 0, sflop = 0;
         int32_t bmax = 0;
         for (int32_t p = a0 + lane; p < a1; p += 64) {
             int32_t k = acol[p];
             int32_t len = brst[k + 1] - brst[k];
-            flop += len;
+            sflop += len;
+            flop += hub_row(hub_n, k, len) ? (k + 31) >> 5 : len;
             bmax = max(bmax, len);
         }
         flop = dev::wave_sum(flop);
+        sflop = dev::wave_sum(sflop);
         bmax = dev::wave_max(bmax);
         if (lane == 0) {
             st.mlen[i] = mrst[i + 1] - mrst[i];
             st.alen[i] = a1 - a0;
             st.flop[i] = flop;
+            st.sflop[i] = sflop;
             st.bmax[i] = bmax;
         }
     }
@@ -82,6 +86,12 @@ __host__ __device__ __forceinline__ int mca_fits(int32_t mlen) { return mlen <= 
 __device__ __forceinline__ int flop_bin(int64_t flop) {
     int lg = flop > 0 ? 63 - __clzll((unsigned long long)flop) : 0;
     return lg > kFlopBins - 1 ? kFlopBins - 1 : lg;
+}
+
+// Work estimate of a row in kernel class cls: the window kernels read hub
+// rows as bitmaps, the others stream every B entry.
+__device__ __forceinline__ int64_t class_flop(const SelectParams &sp, int cls, int64_t flop, int64_t sflop) {
+    return sp.hub && (cls == GK_ACC_MSA_LDS || cls == kClassMsaLarge) ? flop : sflop;
 }
 
 __device__ __forceinline__ int32_t task_count(const SelectParams &sp, int32_t alen, int64_t flop) {
@@ -116,7 +126,7 @@ __global__ __launch_bounds__(256) void k_select_bin(int64_t m, RowStats st, Sele
         int64_t flop = 0;
         if (i < m) {
             mlen = st.mlen[i];
-            flop = st.flop[i];
+            flop = st.sflop[i];
             alen = st.alen[i];
             if (mlen > 0 && flop > 0) {
                 if (sp.selector == GK_SEL_FORCED) acc = sp.forced;
@@ -128,6 +138,7 @@ __global__ __launch_bounds__(256) void k_select_bin(int64_t m, RowStats st, Sele
                 if (acc == GK_ACC_MSA_LDS && !lds_fits(mlen)) acc = GK_ACC_HASH_GLOBAL;
                 if (acc == GK_ACC_HASH_GLOBAL && mlen > kHashGlobalMaxMask) acc = GK_ACC_DENSE_GLOBAL;
                 if (acc == GK_ACC_MSA_LDS && mlen > sp.small_max_mask) acc = kClassMsaLarge;
+                flop = class_flop(sp, acc, st.flop[i], flop);
                 nt = task_count(sp, alen, flop);
                 bin = acc * kFlopBins + (kFlopBins - 1 - flop_bin(flop));
             }
@@ -185,7 +196,8 @@ __global__ void k_bin_scan(const int32_t *__restrict__ bin_count, int32_t *__res
 // Tasks of every row written into their bins.  A block first counts its
 // rows' tasks per bin in LDS, reserves each bin's span with one global atomic,
 // then writes -- so the ~100 bin cursors see one atomic per block, not per row.
-__global__ __launch_bounds__(256) void k_scatter_tasks(int64_t m, RowStats st, const int32_t *__restrict__ arst,
+__global__ __launch_bounds__(256) void k_scatter_tasks(int64_t m, RowStats st, SelectParams sp,
+                                                       const int32_t *__restrict__ arst,
                                                        int32_t *__restrict__ bin_cur, Task *__restrict__ tasks) {
     __shared__ int32_t lcnt[kNumBins];
     __shared__ int32_t lbase[kNumBins];
@@ -199,7 +211,7 @@ __global__ __launch_bounds__(256) void k_scatter_tasks(int64_t m, RowStats st, c
             cls = st.acc[i];
             if (cls >= 0) {
                 nt = st.ntask[i];
-                bin = cls * kFlopBins + (kFlopBins - 1 - flop_bin(st.flop[i]));
+                bin = cls * kFlopBins + (kFlopBins - 1 - flop_bin(class_flop(sp, cls, st.flop[i], st.sflop[i])));
                 off = atomicAdd(&lcnt[bin], nt);
             }
         }
@@ -256,6 +268,52 @@ __device__ __forceinline__ int32_t build_prefix(const Prob &P, int32_t a0, int32
     }
     if (gl == 0) pre[a1 - a0] = total;
     return total;
+}
+
+// The same with hub bitmaps: A entries whose B row is read as bitmap words
+// (hub_row) get length 0 in pre/bst and their word count in hpre/hbst
+// (hbst[x] = word offset of the row in P.hub minus hpre[x]); the others
+// the reverse.  Returns (streamed entries, bitmap words).
+template <int G>
+__device__ __forceinline__ int2 build_prefix_hub(const Prob &P, int32_t a0, int32_t a1, int32_t *pre, int32_t *bst,
+                                                 int32_t *hpre, int32_t *hbst, int gl) {
+    int32_t total = 0, htotal = 0;
+    for (int32_t base = a0; base < a1; base += G) {
+        const int32_t p = base + gl;
+        int32_t len = 0, bs = 0, hlen = 0, hbs = 0;
+        if (p < a1) {
+            const int32_t k = P.acol[p];
+            bs = P.brst[k];
+            len = P.brst[k + 1] - bs;
+            if (hub_row(P.hub_n, k, len)) {
+                hlen = (k + 31) >> 5;
+                hbs = k * P.hub_w;
+                len = 0;
+            }
+        }
+        int32_t incl = len, hincl = hlen;
+#pragma unroll
+        for (int o = 1; o < G; o <<= 1) {
+            const int32_t w = __shfl_up(incl, o, G), hw = __shfl_up(hincl, o, G);
+            if (gl >= o) {
+                incl += w;
+                hincl += hw;
+            }
+        }
+        if (p < a1) {
+            pre[p - a0] = total + incl - len;
+            bst[p - a0] = bs - (total + incl - len);
+            hpre[p - a0] = htotal + hincl - hlen;
+            hbst[p - a0] = hbs - (htotal + hincl - hlen);
+        }
+        total += __shfl(incl, G - 1, G);
+        htotal += __shfl(hincl, G - 1, G);
+    }
+    if (gl == 0) {
+        pre[a1 - a0] = total;
+        hpre[a1 - a0] = htotal;
+    }
+    return make_int2(total, htotal);
 }
 
 // Hit handling common to every accumulator.  q = mask position (0-based in
@@ -359,6 +417,44 @@ __device__ __forceinline__ uint32_t stream_entries(const Prob &P, const Task &tk
                 if (q >= 0) Hit<MODE>::on(P, acc, acnt, q, mb, tk.a0, k[u], t[u], cnt);
             }
         }
+    }
+    return cnt;
+}
+
+// Triangle counting over the hub rows of a task: the concatenated bitmap
+// words of its hub B rows (hpre / hbst from build_prefix_hub), U words per
+// lane in flight; word w of row k holds columns [32w, 32w + 32), all below
+// k, and is ANDed with word w of the mask row's LDS bitmap.  Consecutive
+// lanes read consecutive words of a row.  Returns the number of hits.
+template <int G, int U>
+__device__ __forceinline__ uint32_t stream_hub(const Prob &P, const int32_t *hpre, const int32_t *hbst,
+                                               int32_t total, int gl, const uint32_t *bm) {
+    uint32_t cnt = 0;
+    if (total <= 0) return 0;
+    int32_t kk = 0, cur = 0, nxt = hpre[1], base = hbst[0];
+    for (int32_t g0 = gl; g0 < total; g0 += G * U) {
+        int32_t t[U], w[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int32_t g = g0 + G * u;
+            t[u] = -1;
+            w[u] = 0;
+            if (g < total) {
+                while (g >= nxt) {
+                    ++kk;
+                    cur = nxt;
+                    nxt = hpre[kk + 1];
+                    base = hbst[kk];
+                }
+                t[u] = base + g;
+                w[u] = g - cur;
+            }
+        }
+        uint32_t v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = t[u] >= 0 ? P.hub[t[u]] : 0u;
+#pragma unroll
+        for (int u = 0; u < U; ++u) cnt += __popc(v[u] & bm[w[u]]);
     }
     return cnt;
 }
@@ -474,6 +570,55 @@ __device__ __forceinline__ int32_t build_prefix_block(const Prob &P, int32_t a0,
     return total;
 }
 
+// build_prefix_block with hub bitmaps (see build_prefix_hub); wsum holds
+// 2 * NT/64 wave totals.
+template <int NT>
+__device__ __forceinline__ int2 build_prefix_block_hub(const Prob &P, int32_t a0, int32_t a1, int32_t *pre,
+                                                       int32_t *bst, int32_t *hpre, int32_t *hbst, int32_t *wsum) {
+    static_assert(NT >= kMaxTaskA, "one A entry per thread");
+    constexpr int NW = NT / 64;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int32_t p = a0 + tid;
+    int32_t len = 0, bs = 0, hlen = 0, hbs = 0;
+    if (p < a1) {
+        const int32_t k = P.acol[p];
+        bs = P.brst[k];
+        len = P.brst[k + 1] - bs;
+        if (hub_row(P.hub_n, k, len)) {
+            hlen = (k + 31) >> 5;
+            hbs = k * P.hub_w;
+            len = 0;
+        }
+    }
+    const int32_t incl = dev::wave_incl_scan(len), hincl = dev::wave_incl_scan(hlen);
+    if (lane == 63) {
+        wsum[w] = incl;
+        wsum[NW + w] = hincl;
+    }
+    __syncthreads();
+    int32_t before = 0, total = 0, hbefore = 0, htotal = 0;
+#pragma unroll
+    for (int x = 0; x < NW; ++x) {
+        const int32_t v = wsum[x], hv = wsum[NW + x];
+        before += x < w ? v : 0;
+        total += v;
+        hbefore += x < w ? hv : 0;
+        htotal += hv;
+    }
+    if (p < a1) {
+        const int32_t ex = before + incl - len, hex = hbefore + hincl - hlen;
+        pre[p - a0] = ex;
+        bst[p - a0] = bs - ex;
+        hpre[p - a0] = hex;
+        hbst[p - a0] = hbs - hex;
+    }
+    if (tid == 0) {
+        pre[a1 - a0] = total;
+        hpre[a1 - a0] = htotal;
+    }
+    return make_int2(total, htotal);
+}
+
 // TC and SUPPORT modes.  SUPPORT also needs the mask position of a hit:
 // the sorted mask row is copied to LDS and a hit below kWindow finds its
 // position by binary search there (hits are a few percent of the probes);
@@ -491,8 +636,12 @@ __global__ __launch_bounds__(64 * NW) void k_msa_window(Prob P, int cls) {
     __shared__ int32_t acnt[SUP ? kMaxTaskA : 1];
     __shared__ int32_t pre[kMaxTaskA + 1];
     __shared__ int32_t bst[kMaxTaskA];
-    __shared__ int32_t wsum[NW];
+    // TC: prefix of the hub rows' bitmap words
+    __shared__ int32_t hpre[SUP ? 1 : kMaxTaskA + 1];
+    __shared__ int32_t hbst[SUP ? 1 : kMaxTaskA];
+    __shared__ int32_t wsum[2 * NW];
     const int tid = threadIdx.x;
+    const bool hub = !SUP && P.hub != nullptr;
     for (int w = tid; w < kWindow / 32; w += NT) bm[w] = 0;
     const int32_t lo = P.bin_off[cls * kFlopBins], hi = P.bin_off[(cls + 1) * kFlopBins];
     uint32_t cnt = 0;
@@ -528,11 +677,23 @@ __global__ __launch_bounds__(64 * NW) void k_msa_window(Prob P, int cls) {
             while (atomicCAS(&hkey[s], -1, c) != -1) s = (s + 1) & cmask;
             if (SUP) hpos[s] = q;
         }
-        int32_t total;
-        if constexpr (NW == 1) total = build_prefix<64>(P, tk.a0, tk.a1, pre, bst, tid);
-        else total = build_prefix_block<NT>(P, tk.a0, tk.a1, pre, bst, wsum);
+        int32_t total, htotal = 0;
+        if (hub) {
+            int2 tt;
+            if constexpr (NW == 1) tt = build_prefix_hub<64>(P, tk.a0, tk.a1, pre, bst, hpre, hbst, tid);
+            else tt = build_prefix_block_hub<NT>(P, tk.a0, tk.a1, pre, bst, hpre, hbst, wsum);
+            total = tt.x;
+            htotal = tt.y;
+        } else if constexpr (NW == 1) {
+            total = build_prefix<64>(P, tk.a0, tk.a1, pre, bst, tid);
+        } else {
+            total = build_prefix_block<NT>(P, tk.a0, tk.a1, pre, bst, wsum);
+        }
         __syncthreads();
-        const uint32_t c = stream_entries<MODE, NT, kWindowU>(P, tk, mb, pre, bst, total, tid, acc, acnt,
+        uint32_t c = 0;
+        if constexpr (!SUP)
+            if (htotal) c = stream_hub<NT, kWindowU>(P, hpre, hbst, htotal, tid, bm);
+        c += stream_entries<MODE, NT, kWindowU>(P, tk, mb, pre, bst, total, tid, acc, acnt,
                                                        [&](int32_t b) -> int32_t {
             if (b < kWindow) {
                 if (!((bm[b >> 5] >> (b & 31)) & 1u)) return -1;
@@ -783,6 +944,28 @@ __global__ __launch_bounds__(64) void k_dense_global(Prob P, int cls, int32_t *_
     finish_kernel<MODE>(P, cnt, cls, lane);
 }
 
+// Bitmaps of the hub rows of B: rows k < hub_n read as words (hub_row) get
+// words [0, ceil(k / 32)) of their bitmap, built in LDS by one workgroup.
+// B is strictly lower, so every column of row k is below k.
+__global__ __launch_bounds__(256) void k_hub_build(int32_t hub_n, int32_t hub_w, const int32_t *__restrict__ brst,
+                                                   const int32_t *__restrict__ bcol, uint32_t *__restrict__ hub) {
+    __shared__ uint32_t words[kWindow / 32];
+    for (int32_t k = blockIdx.x; k < hub_n; k += gridDim.x) {
+        const int32_t b0 = brst[k], b1 = brst[k + 1];
+        if (!hub_row(hub_n, k, b1 - b0)) continue;
+        const int32_t wk = (k + 31) >> 5;
+        for (int32_t w = threadIdx.x; w < wk; w += blockDim.x) words[w] = 0;
+        __syncthreads();
+        for (int32_t p = b0 + threadIdx.x; p < b1; p += blockDim.x) {
+            const int32_t c = bcol[p];
+            atomicOr(&words[c >> 5], 1u << (c & 31));
+        }
+        __syncthreads();
+        for (int32_t w = threadIdx.x; w < wk; w += blockDim.x) hub[(int64_t)k * hub_w + w] = words[w];
+        __syncthreads();
+    }
+}
+
 __global__ void k_zero_u32(uint32_t *p, int64_t n) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         p[i] = 0;
@@ -804,14 +987,36 @@ int zero_u32(gk_ctx *ctx, void *p, int64_t n) {
     return launch_check("k_zero_u32");
 }
 
+int32_t hub_rows(const gk_dcsr *B, int mode, const gk_opts &o) {
+    if (mode != MODE_TC || o.hub_window <= 0 || !B->lower || !B->sorted) return 0;
+    return (int32_t)std::min<int64_t>(std::min<int64_t>(o.hub_window, kWindow), B->m);
+}
+
+int build_hub(gk_ctx *ctx, const gk_dcsr *B, int32_t hub_n, Prob *P) {
+    P->hub = nullptr;
+    P->hub_n = P->hub_w = 0;
+    if (hub_n <= 0) return GK_OK;
+    const int32_t hub_w = (hub_n + 31) >> 5;
+    void *p;
+    GK_TRY(scratch(ctx, SC_HUB, (size_t)hub_n * hub_w * 4, &p));
+    const int blocks = grid_for(ctx, hub_n, 8);
+    k_hub_build<<<blocks, 256, 0, ctx->stream>>>(hub_n, hub_w, B->rst, B->col, (uint32_t *)p);
+    GK_TRY(launch_check("k_hub_build"));
+    P->hub = (const uint32_t *)p;
+    P->hub_n = hub_n;
+    P->hub_w = hub_w;
+    return GK_OK;
+}
+
 // Features only (for the selector's training and inference).
-int row_stats(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcsr *M, RowStats *st) {
+int row_stats(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcsr *M, RowStats *st, int32_t hub_n) {
     const int64_t m = A->m;
     void *p;
-    size_t need = (size_t)m * (4 + 4 + 8 + 4 + 4 + 1) + 64;
+    size_t need = (size_t)m * (8 + 8 + 4 + 4 + 4 + 4 + 1) + 64;
     GK_TRY(scratch(ctx, SC_FEAT, need, &p));
     char *c = (char *)p;
     st->flop = (int64_t *)c; c += 8 * (size_t)m;
+    st->sflop = (int64_t *)c; c += 8 * (size_t)m;
     st->mlen = (int32_t *)c; c += 4 * (size_t)m;
     st->alen = (int32_t *)c; c += 4 * (size_t)m;
     st->bmax = (int32_t *)c; c += 4 * (size_t)m;
@@ -819,7 +1024,7 @@ int row_stats(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcsr *M,
     st->acc = (int8_t *)c;
     if (m == 0) return GK_OK;
     int blocks = grid_for(ctx, (m + 3) / 4, 32);
-    k_row_stats<<<blocks, 256, 0, ctx->stream>>>(m, A->rst, A->col, B->rst, M->rst, *st);
+    k_row_stats<<<blocks, 256, 0, ctx->stream>>>(m, A->rst, A->col, B->rst, M->rst, *st, hub_n);
     return launch_check("k_row_stats");
 }
 
@@ -863,7 +1068,7 @@ int plan_tasks(gk_ctx *ctx, const gk_dcsr *A, RowStats &st, const SelectParams &
     plan->tasks = (Task *)tp;
     plan->bin_off = bin_off;
     if (m > 0) {
-        k_scatter_tasks<<<blocks, 256, 0, ctx->stream>>>(m, st, A->rst, bin_cur, plan->tasks);
+        k_scatter_tasks<<<blocks, 256, 0, ctx->stream>>>(m, st, sp, A->rst, bin_cur, plan->tasks);
         GK_TRY(launch_check("k_scatter_tasks"));
     }
     return GK_OK;
@@ -1024,6 +1229,7 @@ SelectParams select_params(const gk_opts &o) {
     sp.split_flop = o.split_flop > 0 ? o.split_flop : (1 << 18);
     sp.small_max_mask = std::max(1, std::min<int32_t>(o.small_max_mask > 0 ? o.small_max_mask : kLdsSmallMask,
                                                       kLdsSmallMask));
+    sp.hub = 0;
     return sp;
 }
 
@@ -1039,15 +1245,18 @@ int masked_product(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcs
     total.start();
     sel.start();
     RowStats st;
-    GK_TRY(row_stats(ctx, A, B, M, &st));
+    const int32_t hub_n = hub_rows(B, mode, o);
+    GK_TRY(row_stats(ctx, A, B, M, &st, hub_n));
     SelectParams sp = select_params(o);
+    sp.hub = hub_n > 0;
     if (sp.selector == GK_SEL_LEARNED)
         GK_TRY(model_select(ctx, st, A->m, B->n, A->rst == M->rst && A->col == M->col));
     Plan plan;
     GK_TRY(plan_tasks(ctx, A, st, sp, &plan));
+    Prob P{};
+    GK_TRY(build_hub(ctx, B, hub_n, &P));
     sel.stop();
     if (mode == MODE_VALUES || mode == MODE_SUPPORT) GK_TRY(zero_u32(ctx, d_out, M->nz));
-    Prob P;
     P.arst = A->rst; P.acol = A->col; P.aval = A->val;
     P.brst = B->rst; P.bcol = B->col; P.bval = B->val;
     P.mrst = M->rst; P.mcol = M->col;

@@ -33,9 +33,11 @@ def _f32(a):
 
 
 def samples(ctx: Context, A: DeviceCsr, B: DeviceCsr | None = None, M: DeviceCsr | None = None,
-            mode: int = 0) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+            mode: int = 0, opts=None) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
     """Features X (m x 16), labels T (m x 4, log2 ns) and validity
-    (m x 4) of every row of C<M> = A*B; mode 0 = triangle counting."""
+    (m x 4) of every row of C<M> = A*B; mode 0 = triangle counting.  The
+    rows are timed under ``opts`` (a GkOpts or None for defaults) with each
+    accumulator forced in turn."""
     B = B or A
     M = M or A
     m = A.m
@@ -44,7 +46,8 @@ def samples(ctx: Context, A: DeviceCsr, B: DeviceCsr | None = None, M: DeviceCsr
     V = np.zeros((m, N_OUT), np.uint8)
     check(lib().gk_selector_samples(ctx.handle, A.handle, B.handle, M.handle, mode,
                                     X.ctypes.data_as(C.POINTER(C.c_float)), T.ctypes.data_as(C.POINTER(C.c_float)),
-                                    V.ctypes.data_as(C.POINTER(C.c_uint8))), "gk_selector_samples")
+                                    V.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                    C.byref(opts) if opts is not None else None), "gk_selector_samples")
     keep = V.any(axis=1)
     return X[keep], T[keep], V[keep]
 
@@ -97,13 +100,14 @@ def load(path) -> np.ndarray:
     return np.load(Path(path), allow_pickle=False)
 
 
-def train_from_graphs(ctx: Context, graphs, epochs: int = 40, seed: int = 1, mode: int = 0) -> dict:
-    """Sample every graph's L (triangle counting), train, return a report."""
+def train_from_graphs(ctx: Context, graphs, epochs: int = 40, seed: int = 1, mode: int = 0, opts=None) -> dict:
+    """Sample every graph's L (triangle counting) under ``opts``, train,
+    return a report."""
     Xs, Ts, Vs = [], [], []
     t0 = time.perf_counter()
     for g in graphs:
         d = ctx.upload(g.L if mode == 0 else g.A)
-        X, T, V = samples(ctx, d, mode=mode)
+        X, T, V = samples(ctx, d, mode=mode, opts=opts)
         Xs.append(X)
         Ts.append(T)
         Vs.append(V)
@@ -124,6 +128,7 @@ SEARCH_SPACE = {
     "blocks_long": [4, 6, 8],
     "blocks_mca": [4, 8, 16],
     "split_flop": [1 << 16, 1 << 18, 1 << 20],
+    "hub_window": [0, 4096, 8192, 16384, 32768],
 }
 
 

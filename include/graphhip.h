@@ -105,6 +105,10 @@ typedef struct gk_opts {
     int32_t blocks_global;  /* ... hash-global and dense-global kernels              */
     int32_t batch_blocks;   /* batching: peeling workgroups per CU                   */
     int32_t batch_order;    /* batching: 0 = graphs largest first, 1 = given order   */
+    int32_t hub_window;     /* triangle counting: B rows below this id (<= 32768) are
+                             * kept as bitmaps and intersected with the mask row's
+                             * bitmap word by word where that reads fewer words than
+                             * the row has entries; 0 = off                          */
 } gk_opts;
 void gk_opts_default(gk_opts *o);
 
@@ -270,10 +274,12 @@ int gk_model_train(gk_ctx *ctx, const float *X, const float *T, const uint8_t *v
                    float *final_loss);
 /* Training samples of C<M> = A*B (mode 0: triangle counting, 1: values):
  * X[m x 16] features, T[m x 4] measured log2 ns per accumulator (the row's
- * share of its kernel's time), valid[m x 4]. */
+ * share of its kernel's time), valid[m x 4].  The rows are timed under opts
+ * (NULL: defaults) with the accumulator forced in turn, so the engine knobs
+ * the selector will run with (hub_window, thresholds) shape the labels. */
 int gk_selector_samples(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B,
                         const gk_dcsr *M, int mode, float *X, float *T,
-                        uint8_t *valid);
+                        uint8_t *valid, const gk_opts *opts);
 
 /* ---- device buffers for the *_dev entry points ------------------------- */
 int  gk_dmalloc(gk_ctx *ctx, size_t bytes, void **dptr);

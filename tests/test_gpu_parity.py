@@ -45,6 +45,8 @@ def test_golden_all(gpu, name):
     for acc in ACCS:
         assert gpu.tc_count(G.L, forced(acc)) == int(z["triangles"]), acc
     assert gpu.tc_count(G.L) == int(z["triangles"])
+    for hw in (32, 1024):
+        assert gpu.tc_count(G.L, {**forced(ACC_MSA_LDS), "hub_window": hw}) == int(z["triangles"]), hw
     assert np.array_equal(gpu.ktruss_support(A).astype(np.int64), z["support"].astype(np.int64))
     tr, kmax = gpu.truss_decompose(A)
     assert np.array_equal(tr, z["truss"])
@@ -95,6 +97,35 @@ def test_tc_per_row_and_drop_in(gpu):
     assert np.array_equal(rows.astype(np.int64), per_row)
     assert triangle_counting_masked_sandia(G.A, mxmm_msa, True) == O.tc_sandia(G.L)
     assert triangle_counting_masked_sandia(G.A, mxmm_mca, True) == O.tc_sandia(G.L)
+
+
+# Hub bitmaps: B rows below hub_window read as bitmap words where that is
+# cheaper.  Windows that are not multiples of 32, wider than the graph, and
+# the largest one; the window kernels forced, the heuristic, per-row counts.
+HUB_WINDOWS = [37, 64, 4096, 32768]
+
+
+@pytest.mark.parametrize("kind", ["rmat12", "rmat14", "chunglu", "grid", "er"])
+def test_tc_hub_bitmaps(gpu, kind):
+    say("tc hub", kind)
+    G = {"rmat12": lambda: HostGraph.rmat(12, 16, seed=3), "rmat14": lambda: HostGraph.rmat(14, 16, seed=1),
+         "chunglu": lambda: HostGraph.chung_lu(20000, 2.0, 16.0, seed=5), "grid": lambda: HostGraph.grid(64),
+         "er": lambda: HostGraph.er(5000, 24.0, seed=3)}[kind]()
+    want = O.tc_sandia(G.L)
+    d = gpu.upload(G.L)
+    rows_want = gpu.tc_rows(d)
+    assert int(rows_want.sum()) == want
+    for hw in HUB_WINDOWS:
+        assert gpu.tc_count(d, {"hub_window": hw}) == want, hw
+        for acc in ACCS:
+            assert gpu.tc_count(d, {**forced(acc), "hub_window": hw}) == want, (hw, acc)
+        # long rows through the block-wide kernel, short rows through the wave kernel
+        for sm in (1, 128):
+            assert gpu.tc_count(d, {**forced(ACC_MSA_LDS), "hub_window": hw, "small_max_mask": sm}) == want, (hw, sm)
+        assert np.array_equal(gpu.tc_rows(d, {"hub_window": hw}), rows_want), hw
+    # split rows: hub words and streamed entries of one row in several tasks
+    assert gpu.tc_count(d, {**forced(ACC_MSA_LDS), "hub_window": 4096, "split_flop": 64}) == want
+    d.free()
 
 
 def test_tc_edge_cases(gpu):
