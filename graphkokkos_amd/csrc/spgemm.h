@@ -46,7 +46,9 @@ constexpr int kGlobalBlocksPerCu = 4;
 struct RowStats {
     int64_t *flop;   // words the window kernels read per A row: sum of |B_k|, with
                      // hub rows counted by their bitmap words where that is fewer
-    int64_t *sflop;  // sum over A row of |B_k| (what the other kernels stream)
+    int64_t *sflop;  // sum over A row of |B_k| (what the global kernels stream)
+    int64_t *mflop;  // MCA-wave work: hub rows cost the mask entries below k
+                     // (one bit test each) where that is fewer than |B_k|
     int32_t *mlen;   // mask row length
     int32_t *alen;   // A row length
     int32_t *bmax;   // max |B_k|
@@ -98,7 +100,9 @@ struct Prob {
     int32_t hub_n, hub_w;
 };
 
-// Whether A entry k of a window task reads B row k as bitmap words.
+// Whether A entry k of a window task reads B row k as bitmap words (the
+// bitmaps of all nonempty rows below hub_n exist; MCA-wave tests single
+// bits of them).
 __host__ __device__ __forceinline__ bool hub_row(int32_t hub_n, int32_t k, int32_t len) {
     return k < hub_n && ((k + 31) >> 5) < len;
 }

@@ -5,7 +5,7 @@
 // of B.n entries (MSA, include/matrix_la.h:576-635) or a mask-indexed
 // accumulator with a merge walk (MCA, include/matrix_la.h:493-528).  Here:
 //
-//  1. k_row_stats: one wave per row records mask length, A-row length,
+//  1. k_row_stats: a tile of 8 lanes per row records mask length, A-row length,
 //     flop = sum_k |B_k| and max |B_k|.
 //  2. Every row gets an accumulator (fixed thresholds, the learned cost
 //     model, or a forced choice) and is cut into tasks of at most
@@ -41,32 +41,47 @@ namespace gk {
 // ---------------------------------------------------------------------------
 // features and task construction
 
+// Tiles of G lanes per row (rows average a few dozen entries): each lane
+// walks every G-th A entry, then the tile reduces.
+template <int G, class T>
+__device__ __forceinline__ T tile_sum(T v) {
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
+    return v;
+}
+
+template <int G>
 __global__ __launch_bounds__(256) void k_row_stats(int64_t m, const int32_t *__restrict__ arst,
                                                    const int32_t *__restrict__ acol,
                                                    const int32_t *__restrict__ brst,
                                                    const int32_t *__restrict__ mrst, RowStats st,
-                                                   int32_t hub_n) {
-    const int lane = threadIdx.x & 63;
-    const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
-    for (int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < m; i += waves) {
+                                                   int32_t hub_n, int same_am) {
+    const int gl = threadIdx.x % G;
+    const int64_t per_block = blockDim.x / G;
+    for (int64_t i = (int64_t)blockIdx.x * per_block + threadIdx.x / G; i < m; i += (int64_t)gridDim.x * per_block) {
         const int32_t a0 = arst[i], a1 = arst[i + 1];
-        int64_t flop = 0, sflop = 0;
+        int64_t flop = 0, sflop = 0, mflop = 0;
         int32_t bmax = 0;
-        for (int32_t p = a0 + lane; p < a1; p += 64) {
-            int32_t k = acol[p];
-            int32_t len = brst[k + 1] - brst[k];
+        for (int32_t p = a0 + gl; p < a1; p += G) {
+            const int32_t k = acol[p];
+            const int32_t len = brst[k + 1] - brst[k];
             sflop += len;
             flop += hub_row(hub_n, k, len) ? (k + 31) >> 5 : len;
+            // with A the mask, the mask entries below k are the p - a0 before it
+            mflop += same_am && k < hub_n ? min(len, p - a0) : len;
             bmax = max(bmax, len);
         }
-        flop = dev::wave_sum(flop);
-        sflop = dev::wave_sum(sflop);
-        bmax = dev::wave_max(bmax);
-        if (lane == 0) {
+        flop = tile_sum<G>(flop);
+        sflop = tile_sum<G>(sflop);
+        mflop = tile_sum<G>(mflop);
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1) bmax = max(bmax, __shfl_xor(bmax, o, G));
+        if (gl == 0) {
             st.mlen[i] = mrst[i + 1] - mrst[i];
             st.alen[i] = a1 - a0;
             st.flop[i] = flop;
             st.sflop[i] = sflop;
+            st.mflop[i] = mflop;
             st.bmax[i] = bmax;
         }
     }
@@ -88,10 +103,14 @@ __device__ __forceinline__ int flop_bin(int64_t flop) {
     return lg > kFlopBins - 1 ? kFlopBins - 1 : lg;
 }
 
-// Work estimate of a row in kernel class cls: the window kernels read hub
-// rows as bitmaps, the others stream every B entry.
-__device__ __forceinline__ int64_t class_flop(const SelectParams &sp, int cls, int64_t flop, int64_t sflop) {
-    return sp.hub && (cls == GK_ACC_MSA_LDS || cls == kClassMsaLarge) ? flop : sflop;
+// Work estimate of row i in kernel class cls: the window kernels read hub
+// rows as bitmap words, MCA-wave tests single bits of them, the global
+// kernels stream every B entry.
+__device__ __forceinline__ int64_t class_flop(const SelectParams &sp, int cls, const RowStats &st, int64_t i) {
+    if (!sp.hub) return st.sflop[i];
+    if (cls == GK_ACC_MSA_LDS || cls == kClassMsaLarge) return st.flop[i];
+    if (cls == GK_ACC_MCA_WAVE) return st.mflop[i];
+    return st.sflop[i];
 }
 
 __device__ __forceinline__ int32_t task_count(const SelectParams &sp, int32_t alen, int64_t flop) {
@@ -138,7 +157,7 @@ __global__ __launch_bounds__(256) void k_select_bin(int64_t m, RowStats st, Sele
                 if (acc == GK_ACC_MSA_LDS && !lds_fits(mlen)) acc = GK_ACC_HASH_GLOBAL;
                 if (acc == GK_ACC_HASH_GLOBAL && mlen > kHashGlobalMaxMask) acc = GK_ACC_DENSE_GLOBAL;
                 if (acc == GK_ACC_MSA_LDS && mlen > sp.small_max_mask) acc = kClassMsaLarge;
-                flop = class_flop(sp, acc, st.flop[i], flop);
+                flop = class_flop(sp, acc, st, i);
                 nt = task_count(sp, alen, flop);
                 bin = acc * kFlopBins + (kFlopBins - 1 - flop_bin(flop));
             }
@@ -211,7 +230,7 @@ __global__ __launch_bounds__(256) void k_scatter_tasks(int64_t m, RowStats st, S
             cls = st.acc[i];
             if (cls >= 0) {
                 nt = st.ntask[i];
-                bin = cls * kFlopBins + (kFlopBins - 1 - flop_bin(class_flop(sp, cls, st.flop[i], st.sflop[i])));
+                bin = cls * kFlopBins + (kFlopBins - 1 - flop_bin(class_flop(sp, cls, st, i)));
                 off = atomicAdd(&lcnt[bin], nt);
             }
         }
@@ -729,6 +748,62 @@ __global__ __launch_bounds__(64 * NW) void k_msa_window(Prob P, int cls) {
 }
 
 // ---- MCA-wave -------------------------------------------------------------
+// Triangle counting with hub bitmaps: an A entry k below hub_n whose B row
+// has more entries than the mask row has columns below k is counted by
+// testing those columns' bits in row k's bitmap instead of streaming the
+// row.  pre/bst as build_prefix for the streamed entries; hpre[x] = bit
+// tests before A entry a0+x, hbst[x] = word offset of its bitmap row.
+// m[0, mlen) is the tile's sorted mask row.  Returns (streamed, tests).
+template <int G>
+__device__ __forceinline__ int2 build_prefix_probe(const Prob &P, int32_t a0, int32_t a1, const int32_t *m,
+                                                   int32_t mlen, int32_t *pre, int32_t *bst, int32_t *hpre,
+                                                   int32_t *hbst, int gl) {
+    int32_t total = 0, htotal = 0;
+    for (int32_t base = a0; base < a1; base += G) {
+        const int32_t p = base + gl;
+        int32_t len = 0, bs = 0, hlen = 0, hbs = 0;
+        if (p < a1) {
+            const int32_t k = P.acol[p];
+            bs = P.brst[k];
+            len = P.brst[k + 1] - bs;
+            if (k < P.hub_n && len > 0) {
+                int32_t l = 0, h = mlen;  // mask columns below k
+                while (l < h) {
+                    const int32_t mid = (l + h) >> 1;
+                    if (m[mid] < k) l = mid + 1; else h = mid;
+                }
+                if (l < len) {
+                    hlen = l;
+                    hbs = k * P.hub_w;
+                    len = 0;
+                }
+            }
+        }
+        int32_t incl = len, hincl = hlen;
+#pragma unroll
+        for (int o = 1; o < G; o <<= 1) {
+            const int32_t w = __shfl_up(incl, o, G), hw = __shfl_up(hincl, o, G);
+            if (gl >= o) {
+                incl += w;
+                hincl += hw;
+            }
+        }
+        if (p < a1) {
+            pre[p - a0] = total + incl - len;
+            bst[p - a0] = bs - (total + incl - len);
+            hpre[p - a0] = htotal + hincl - hlen;
+            hbst[p - a0] = hbs;
+        }
+        total += __shfl(incl, G - 1, G);
+        htotal += __shfl(hincl, G - 1, G);
+    }
+    if (gl == 0) {
+        pre[a1 - a0] = total;
+        hpre[a1 - a0] = htotal;
+    }
+    return make_int2(total, htotal);
+}
+
 // Tiles of G lanes, 64/G tasks per wave.  The sorted mask row sits in LDS;
 // every lane binary-searches its streamed B entry in it (after a range
 // check against the row's first and last column).  In TC mode the tile's
@@ -742,6 +817,10 @@ __global__ __launch_bounds__(64) void k_mca_wave(Prob P, int cls) {
     __shared__ int32_t acnt[T][MODE == MODE_SUPPORT ? kMaxTaskA : 1];
     __shared__ int32_t pre[T][kMaxTaskA + 1];
     __shared__ int32_t bst[T][kMaxTaskA];
+    constexpr bool TCH = MODE == MODE_TC;  // hub bit tests possible
+    __shared__ int32_t hpre[TCH ? T : 1][kMaxTaskA + 1];
+    __shared__ int32_t hbst[TCH ? T : 1][kMaxTaskA];
+    const bool hub = TCH && P.hub != nullptr;
     const int lane = threadIdx.x, tile = lane / G, gl = lane % G;
     const int32_t lo = P.bin_off[cls * kFlopBins], hi = P.bin_off[(cls + 1) * kFlopBins];
     uint32_t cnt = 0;
@@ -766,8 +845,18 @@ __global__ __launch_bounds__(64) void k_mca_wave(Prob P, int cls) {
         }
         if (MODE == MODE_SUPPORT)
             for (int32_t x = gl; x < tk.a1 - tk.a0; x += G) acnt[tile][x] = 0;
-        int32_t total = 0;
-        if (live) total = build_prefix<G>(P, tk.a0, tk.a1, pre[tile], bst[tile], gl);
+        int32_t total = 0, htotal = 0;
+        if (hub) {
+            __syncthreads();  // the mask rows, for the bit-test counts
+            if (live) {
+                const int2 tt = build_prefix_probe<G>(P, tk.a0, tk.a1, mk[tile], mlen, pre[tile], bst[tile],
+                                                      hpre[TCH ? tile : 0], hbst[TCH ? tile : 0], gl);
+                total = tt.x;
+                htotal = tt.y;
+            }
+        } else if (live) {
+            total = build_prefix<G>(P, tk.a0, tk.a1, pre[tile], bst[tile], gl);
+        }
         __syncthreads();
         uint32_t cnt_task = 0;
         if (live && mlen > 0 && total > 0) {
@@ -813,6 +902,37 @@ __global__ __launch_bounds__(64) void k_mca_wave(Prob P, int cls) {
                         Hit<MODE>::on(P, acc[tile], acnt[tile], q, mb, tk.a0, kx[u], t[u], cnt_task);
                     }
                 }
+            }
+        }
+        if (TCH && live && htotal > 0) {
+            // bit tests: test g of A entry kk is mask column m[g - first test of kk]
+            const int32_t *m = mk[tile];
+            const int32_t *hp = hpre[TCH ? tile : 0], *hb = hbst[TCH ? tile : 0];
+            int32_t kk = 0, cur = 0, nxt = hp[1], base = hb[0];
+            for (int32_t g0 = gl; g0 < htotal; g0 += G * kMcaU) {
+                int32_t a[kMcaU], bit[kMcaU];
+#pragma unroll
+                for (int u = 0; u < kMcaU; ++u) {
+                    const int32_t g = g0 + G * u;
+                    a[u] = -1;
+                    bit[u] = 0;
+                    if (g < htotal) {
+                        while (g >= nxt) {
+                            ++kk;
+                            cur = nxt;
+                            nxt = hp[kk + 1];
+                            base = hb[kk];
+                        }
+                        const int32_t j = m[g - cur];
+                        a[u] = base + (j >> 5);
+                        bit[u] = j & 31;
+                    }
+                }
+                uint32_t w[kMcaU];
+#pragma unroll
+                for (int u = 0; u < kMcaU; ++u) w[u] = a[u] >= 0 ? P.hub[a[u]] : 0u;
+#pragma unroll
+                for (int u = 0; u < kMcaU; ++u) cnt_task += (w[u] >> bit[u]) & 1u;
             }
         }
         __syncthreads();
@@ -944,15 +1064,15 @@ __global__ __launch_bounds__(64) void k_dense_global(Prob P, int cls, int32_t *_
     finish_kernel<MODE>(P, cnt, cls, lane);
 }
 
-// Bitmaps of the hub rows of B: rows k < hub_n read as words (hub_row) get
-// words [0, ceil(k / 32)) of their bitmap, built in LDS by one workgroup.
-// B is strictly lower, so every column of row k is below k.
+// Bitmaps of the hub rows of B: every nonempty row k < hub_n gets words
+// [0, ceil(k / 32)) of its bitmap, built in LDS by one workgroup.  B is
+// strictly lower, so every column of row k is below k.
 __global__ __launch_bounds__(256) void k_hub_build(int32_t hub_n, int32_t hub_w, const int32_t *__restrict__ brst,
                                                    const int32_t *__restrict__ bcol, uint32_t *__restrict__ hub) {
     __shared__ uint32_t words[kWindow / 32];
     for (int32_t k = blockIdx.x; k < hub_n; k += gridDim.x) {
         const int32_t b0 = brst[k], b1 = brst[k + 1];
-        if (!hub_row(hub_n, k, b1 - b0)) continue;
+        if (b1 == b0) continue;
         const int32_t wk = (k + 31) >> 5;
         for (int32_t w = threadIdx.x; w < wk; w += blockDim.x) words[w] = 0;
         __syncthreads();
@@ -1012,19 +1132,22 @@ int build_hub(gk_ctx *ctx, const gk_dcsr *B, int32_t hub_n, Prob *P) {
 int row_stats(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcsr *M, RowStats *st, int32_t hub_n) {
     const int64_t m = A->m;
     void *p;
-    size_t need = (size_t)m * (8 + 8 + 4 + 4 + 4 + 4 + 1) + 64;
+    size_t need = (size_t)m * (8 + 8 + 8 + 4 + 4 + 4 + 4 + 1) + 64;
     GK_TRY(scratch(ctx, SC_FEAT, need, &p));
     char *c = (char *)p;
     st->flop = (int64_t *)c; c += 8 * (size_t)m;
     st->sflop = (int64_t *)c; c += 8 * (size_t)m;
+    st->mflop = (int64_t *)c; c += 8 * (size_t)m;
     st->mlen = (int32_t *)c; c += 4 * (size_t)m;
     st->alen = (int32_t *)c; c += 4 * (size_t)m;
     st->bmax = (int32_t *)c; c += 4 * (size_t)m;
     st->ntask = (int32_t *)c; c += 4 * (size_t)m;
     st->acc = (int8_t *)c;
     if (m == 0) return GK_OK;
-    int blocks = grid_for(ctx, (m + 3) / 4, 32);
-    k_row_stats<<<blocks, 256, 0, ctx->stream>>>(m, A->rst, A->col, B->rst, M->rst, *st, hub_n);
+    constexpr int G = 8;
+    int blocks = grid_for(ctx, (m + 256 / G - 1) / (256 / G), 16);
+    k_row_stats<G><<<blocks, 256, 0, ctx->stream>>>(m, A->rst, A->col, B->rst, M->rst, *st, hub_n,
+                                                    A->rst == M->rst && A->col == M->col);
     return launch_check("k_row_stats");
 }
 
