@@ -148,6 +148,18 @@ def bc_bytes(n: int, nnz: int, levels: int, nsrc: int) -> float:
     return 2 * levels * (4 * (n + 1) + 4 * nnz) + 24.0 * nsrc * nnz + 24.0 * nsrc * n
 
 
+# The plain fixed-heuristic kernel: fixed mask-length thresholds pick each
+# row's accumulator and every B row is streamed (hub bitmaps off).  The
+# library default adds hub bitmaps below a fixed window ("heuristic_hub").
+PLAIN = {"hub_window": 0}
+
+
+def tc_variants(tuned_h: dict, tuned_l: dict) -> list:
+    from graphkokkos_amd import SEL_LEARNED
+    return [("plain", PLAIN), ("heuristic_hub", {}), ("tuned_heuristic", tuned_h),
+            ("learned", {"selector": SEL_LEARNED}), ("learned_tuned", tuned_l)]
+
+
 def run_learned_graphs(ctx, tuned_h: dict, tuned_l: dict) -> dict:
     """The learned selector (already trained) against the fixed-threshold
     heuristic on the other two graphs of the learned-selector evaluation:
@@ -166,16 +178,15 @@ def run_learned_graphs(ctx, tuned_h: dict, tuned_l: dict) -> dict:
         log(f"{name}: n={G.n} |E|={G.edges} built in {time.perf_counter() - t0:.1f}s")
         res = {"edges": G.edges}
         base = None
-        for var, opts in (("plain", None), ("tuned_heuristic", tuned_h), ("learned", {"selector": SEL_LEARNED}),
-                          ("learned_tuned", tuned_l)):
+        for var, opts in tc_variants(tuned_h, tuned_l):
             r = run_tc(ctx, G.L, 5, 1, opts)
             if base is None:
                 base = r
             assert r["count"] == base["count"], f"{name}: {var} changed the triangle count"
             res[f"tc_{var}_ms"] = r["wall_s"] * 1e3
         res["triangles"] = base["count"]
-        res["tc_learned_speedup"] = res["tc_plain_ms"] / res["tc_learned_ms"]
-        res["tc_learned_tuned_speedup"] = res["tc_plain_ms"] / res["tc_learned_tuned_ms"]
+        for var in ("heuristic_hub", "tuned_heuristic", "learned", "learned_tuned"):
+            res[f"tc_{var}_speedup"] = res["tc_plain_ms"] / res[f"tc_{var}_ms"]
         dA = ctx.upload(G.A)
         buf = ctx.alloc(4 * max(G.A.nz, 1))
         kmax = C.c_int32(0)
@@ -196,7 +207,8 @@ def run_learned_graphs(ctx, tuned_h: dict, tuned_l: dict) -> dict:
         res["ktruss_learned_speedup"] = res["ktruss_heuristic_ms"] / res["ktruss_learned_ms"]
         buf.free()
         dA.free()
-        log(f"{name}: tc plain {res['tc_plain_ms']:.2f} ms, learned {res['tc_learned_ms']:.2f}, "
+        log(f"{name}: tc plain {res['tc_plain_ms']:.2f} ms, heuristic+hub {res['tc_heuristic_hub_ms']:.2f}, "
+            f"tuned {res['tc_tuned_heuristic_ms']:.2f}, learned {res['tc_learned_ms']:.2f}, "
             f"learned+tuned {res['tc_learned_tuned_ms']:.2f}; k-truss heuristic {res['ktruss_heuristic_ms']:.1f} ms, "
             f"learned {res['ktruss_learned_ms']:.1f} (kmax {res['kmax']})")
         out[name] = res
@@ -338,11 +350,11 @@ def main() -> int:
         t0 = time.perf_counter()
         G22 = HostGraph.rmat(a.tc_scale, 16, seed=a.seed, **RMAT)
         log(f"R-MAT s{a.tc_scale}: n={G22.n} |E|={G22.edges} built in {time.perf_counter() - t0:.1f}s")
-        plain = run_tc(ctx, G22.L, a.steps, a.warmup)
+        plain = run_tc(ctx, G22.L, a.steps, a.warmup, PLAIN)
         pd = dominant(plain["timing"])
         tc = {"workload": f"triangle counting C<L>=L*L, R-MAT scale {a.tc_scale} ef16", "edges": G22.edges,
               "triangles": plain["count"],
-              "plain": {"value": G22.edges / plain["wall_s"], "ms": plain["wall_s"] * 1e3,
+              "plain": {"value": G22.edges / plain["wall_s"], "ms": plain["wall_s"] * 1e3, "opts": PLAIN,
                         "kernel_ms": plain["timing"]["kernel_ms"], "select_ms": plain["timing"]["select_ms"],
                         "roofline": roofline_obj(pd),
                         "bytes_alg_all": plain["timing"]["bytes_alg"],
@@ -360,7 +372,7 @@ def main() -> int:
         t0 = time.perf_counter()
         train_graphs = [HostGraph.rmat(18, 16, seed=a.seed + 1, **RMAT), HostGraph.grid(512),
                         HostGraph.chung_lu(1 << 18, 2.0, 16.0, seed=a.seed + 1)]
-        rep = SEL.train_from_graphs(ctx, train_graphs, epochs=30, seed=a.seed)
+        rep = SEL.train_from_graphs(ctx, train_graphs, epochs=30, seed=a.seed)  # under the library defaults
         del train_graphs
         log(f"selector trained on {rep['samples']} rows, loss {rep['final_loss']:.3f}, "
             f"{time.perf_counter() - t0:.1f}s")
@@ -372,8 +384,7 @@ def main() -> int:
         del Gt
         log(f"autotuned: heuristic {tuned_h}, learned {tuned_l} ({len(tlog_h) + len(tlog_l)} trials)")
         res = {}
-        for name, opts in (("tuned_heuristic", tuned_h), ("learned", {"selector": SEL_LEARNED}),
-                           ("learned_tuned", tuned_l)):
+        for name, opts in tc_variants(tuned_h, tuned_l)[1:]:
             r = run_tc(ctx, G22.L, a.steps, a.warmup, opts)
             assert r["count"] == plain["count"], f"{name} changed the triangle count"
             d = dominant(r["timing"])

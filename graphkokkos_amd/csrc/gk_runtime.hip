@@ -235,7 +235,7 @@ extern "C" void gk_opts_default(gk_opts *o) {
     o->blocks_global = 4;
     o->batch_blocks = 8;
     o->batch_order = 0;
-    o->hub_window = 0;
+    o->hub_window = 16384;
 }
 
 extern "C" int gk_init(int device, gk_ctx **out) {

@@ -13,6 +13,9 @@ enum { MODE_TC = 0, MODE_VALUES = 1, MODE_SUPPORT = 2 };
 // B-row prefix fits LDS; MSA-LDS holds mask rows up to kLdsMaxMask in a
 // hash of kLdsHashCap slots; MCA-wave holds mask rows up to kMcaMaxMask.
 constexpr int kMaxTaskA = 256;
+// MCA-wave tasks hold at most kMcaMaxTaskA A entries (four tasks share a
+// wave's LDS).
+constexpr int kMcaMaxTaskA = 64;
 constexpr int kLdsHashCap = 4096;
 constexpr int kLdsMaxMask = 1536;
 constexpr int kMcaMaxMask = 64;

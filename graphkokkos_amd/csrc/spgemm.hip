@@ -113,8 +113,9 @@ __device__ __forceinline__ int64_t class_flop(const SelectParams &sp, int cls, c
     return st.sflop[i];
 }
 
-__device__ __forceinline__ int32_t task_count(const SelectParams &sp, int32_t alen, int64_t flop) {
-    int64_t by_a = (alen + kMaxTaskA - 1) / kMaxTaskA;
+__device__ __forceinline__ int32_t task_count(const SelectParams &sp, int cls, int32_t alen, int64_t flop) {
+    const int32_t maxa = cls == GK_ACC_MCA_WAVE ? kMcaMaxTaskA : kMaxTaskA;
+    int64_t by_a = (alen + maxa - 1) / maxa;
     int64_t by_f = (flop + sp.split_flop - 1) / sp.split_flop;
     int64_t t = by_a > by_f ? by_a : by_f;
     if (t > alen) t = alen;
@@ -158,7 +159,7 @@ __global__ __launch_bounds__(256) void k_select_bin(int64_t m, RowStats st, Sele
                 if (acc == GK_ACC_HASH_GLOBAL && mlen > kHashGlobalMaxMask) acc = GK_ACC_DENSE_GLOBAL;
                 if (acc == GK_ACC_MSA_LDS && mlen > sp.small_max_mask) acc = kClassMsaLarge;
                 flop = class_flop(sp, acc, st, i);
-                nt = task_count(sp, alen, flop);
+                nt = task_count(sp, acc, alen, flop);
                 bin = acc * kFlopBins + (kFlopBins - 1 - flop_bin(flop));
             }
             st.acc[i] = (int8_t)acc;  // kernel class from here on
@@ -642,7 +643,7 @@ __device__ __forceinline__ int2 build_prefix_block_hub(const Prob &P, int32_t a0
 // the sorted mask row is copied to LDS and a hit below kWindow finds its
 // position by binary search there (hits are a few percent of the probes);
 // the hash for the columns above the window stores positions directly.
-template <int MODE, int NW, int HC, int MAXM>
+template <int MODE, int NW, int HC, int MAXM, bool HUB>
 __global__ __launch_bounds__(64 * NW) void k_msa_window(Prob P, int cls) {
     static_assert(MODE == MODE_TC || MODE == MODE_SUPPORT, "window kernel modes");
     constexpr int NT = 64 * NW;
@@ -655,12 +656,12 @@ __global__ __launch_bounds__(64 * NW) void k_msa_window(Prob P, int cls) {
     __shared__ int32_t acnt[SUP ? kMaxTaskA : 1];
     __shared__ int32_t pre[kMaxTaskA + 1];
     __shared__ int32_t bst[kMaxTaskA];
-    // TC: prefix of the hub rows' bitmap words
-    __shared__ int32_t hpre[SUP ? 1 : kMaxTaskA + 1];
-    __shared__ int32_t hbst[SUP ? 1 : kMaxTaskA];
+    // TC with hub bitmaps: prefix of the hub rows' bitmap words
+    static_assert(!HUB || !SUP, "hub bitmaps count triangles only");
+    __shared__ int32_t hpre[HUB ? kMaxTaskA + 1 : 1];
+    __shared__ int32_t hbst[HUB ? kMaxTaskA : 1];
     __shared__ int32_t wsum[2 * NW];
     const int tid = threadIdx.x;
-    const bool hub = !SUP && P.hub != nullptr;
     for (int w = tid; w < kWindow / 32; w += NT) bm[w] = 0;
     const int32_t lo = P.bin_off[cls * kFlopBins], hi = P.bin_off[(cls + 1) * kFlopBins];
     uint32_t cnt = 0;
@@ -697,7 +698,7 @@ __global__ __launch_bounds__(64 * NW) void k_msa_window(Prob P, int cls) {
             if (SUP) hpos[s] = q;
         }
         int32_t total, htotal = 0;
-        if (hub) {
+        if constexpr (HUB) {
             int2 tt;
             if constexpr (NW == 1) tt = build_prefix_hub<64>(P, tk.a0, tk.a1, pre, bst, hpre, hbst, tid);
             else tt = build_prefix_block_hub<NT>(P, tk.a0, tk.a1, pre, bst, hpre, hbst, wsum);
@@ -710,7 +711,7 @@ __global__ __launch_bounds__(64 * NW) void k_msa_window(Prob P, int cls) {
         }
         __syncthreads();
         uint32_t c = 0;
-        if constexpr (!SUP)
+        if constexpr (HUB)
             if (htotal) c = stream_hub<NT, kWindowU>(P, hpre, hbst, htotal, tid, bm);
         c += stream_entries<MODE, NT, kWindowU>(P, tk, mb, pre, bst, total, tid, acc, acnt,
                                                        [&](int32_t b) -> int32_t {
@@ -809,18 +810,18 @@ __device__ __forceinline__ int2 build_prefix_probe(const Prob &P, int32_t a0, in
 // check against the row's first and last column).  In TC mode the tile's
 // hits are counted from the wave's ballot of found entries with a popcount
 // of the tile's bits.
-template <int MODE, int G>
+template <int MODE, int G, bool HUB>
 __global__ __launch_bounds__(64) void k_mca_wave(Prob P, int cls) {
     constexpr int T = 64 / G;
     __shared__ int32_t mk[T][kMcaMaxMask];
     __shared__ int32_t acc[T][HasAcc<MODE>::v ? kMcaMaxMask : 1];
-    __shared__ int32_t acnt[T][MODE == MODE_SUPPORT ? kMaxTaskA : 1];
-    __shared__ int32_t pre[T][kMaxTaskA + 1];
-    __shared__ int32_t bst[T][kMaxTaskA];
-    constexpr bool TCH = MODE == MODE_TC;  // hub bit tests possible
-    __shared__ int32_t hpre[TCH ? T : 1][kMaxTaskA + 1];
-    __shared__ int32_t hbst[TCH ? T : 1][kMaxTaskA];
-    const bool hub = TCH && P.hub != nullptr;
+    __shared__ int32_t acnt[T][MODE == MODE_SUPPORT ? kMcaMaxTaskA : 1];
+    __shared__ int32_t pre[T][kMcaMaxTaskA + 1];
+    __shared__ int32_t bst[T][kMcaMaxTaskA];
+    static_assert(!HUB || MODE == MODE_TC, "hub bitmaps count triangles only");
+    constexpr bool TCH = HUB;
+    __shared__ int32_t hpre[TCH ? T : 1][kMcaMaxTaskA + 1];
+    __shared__ int32_t hbst[TCH ? T : 1][kMcaMaxTaskA];
     const int lane = threadIdx.x, tile = lane / G, gl = lane % G;
     const int32_t lo = P.bin_off[cls * kFlopBins], hi = P.bin_off[(cls + 1) * kFlopBins];
     uint32_t cnt = 0;
@@ -846,7 +847,7 @@ __global__ __launch_bounds__(64) void k_mca_wave(Prob P, int cls) {
         if (MODE == MODE_SUPPORT)
             for (int32_t x = gl; x < tk.a1 - tk.a0; x += G) acnt[tile][x] = 0;
         int32_t total = 0, htotal = 0;
-        if (hub) {
+        if constexpr (HUB) {
             __syncthreads();  // the mask rows, for the bit-test counts
             if (live) {
                 const int2 tt = build_prefix_probe<G>(P, tk.a0, tk.a1, mk[tile], mlen, pre[tile], bst[tile],
@@ -1282,8 +1283,12 @@ int launch_classes(gk_ctx *ctx, const Prob &P, const ClassLaunch &L, Timer *tm) 
     for (int i = 0; i < gk_ctx::kAux; ++i) GK_HIP(hipStreamWaitEvent(ctx->aux[i], ctx->fork, 0));
     if (L.grid[kClassMsaLarge]) {
         if (tm) tm[kClassMsaLarge].start();
-        if (MODE == MODE_TC || MODE == MODE_SUPPORT)
-            k_msa_window<MODE == MODE_VALUES ? MODE_TC : MODE, kWindowWavesLong, kWindowHashLong, kLdsMaxMask>
+        constexpr int WM = MODE == MODE_VALUES ? MODE_TC : MODE;
+        if (MODE == MODE_TC && P.hub)
+            k_msa_window<MODE_TC, kWindowWavesLong, kWindowHashLong, kLdsMaxMask, true>
+                <<<L.grid[kClassMsaLarge], 64 * kWindowWavesLong, 0, sc[kClassMsaLarge]>>>(P, kClassMsaLarge);
+        else if (MODE == MODE_TC || MODE == MODE_SUPPORT)
+            k_msa_window<WM, kWindowWavesLong, kWindowHashLong, kLdsMaxMask, false>
                 <<<L.grid[kClassMsaLarge], 64 * kWindowWavesLong, 0, sc[kClassMsaLarge]>>>(P, kClassMsaLarge);
         else
             k_msa_lds<MODE, kLdsHashCap, kLdsMaxMask><<<L.grid_lds_large, 64, 0, sc[kClassMsaLarge]>>>(P,
@@ -1293,8 +1298,12 @@ int launch_classes(gk_ctx *ctx, const Prob &P, const ClassLaunch &L, Timer *tm) 
     }
     if (L.grid[GK_ACC_MSA_LDS]) {
         if (tm) tm[GK_ACC_MSA_LDS].start();
-        if (MODE == MODE_TC || MODE == MODE_SUPPORT)
-            k_msa_window<MODE == MODE_VALUES ? MODE_TC : MODE, 1, kLdsSmallCap, kLdsSmallMask>
+        constexpr int WM = MODE == MODE_VALUES ? MODE_TC : MODE;
+        if (MODE == MODE_TC && P.hub)
+            k_msa_window<MODE_TC, 1, kLdsSmallCap, kLdsSmallMask, true>
+                <<<L.grid[GK_ACC_MSA_LDS], 64, 0, sc[GK_ACC_MSA_LDS]>>>(P, GK_ACC_MSA_LDS);
+        else if (MODE == MODE_TC || MODE == MODE_SUPPORT)
+            k_msa_window<WM, 1, kLdsSmallCap, kLdsSmallMask, false>
                 <<<L.grid[GK_ACC_MSA_LDS], 64, 0, sc[GK_ACC_MSA_LDS]>>>(P, GK_ACC_MSA_LDS);
         else
             k_msa_lds<MODE, kLdsSmallCap, kLdsSmallMask><<<L.grid[GK_ACC_MSA_LDS], 64, 0, sc[GK_ACC_MSA_LDS]>>>(
@@ -1304,7 +1313,10 @@ int launch_classes(gk_ctx *ctx, const Prob &P, const ClassLaunch &L, Timer *tm) 
     }
     if (L.grid[GK_ACC_MCA_WAVE]) {
         if (tm) tm[GK_ACC_MCA_WAVE].start();
-        k_mca_wave<MODE, 16><<<L.grid[GK_ACC_MCA_WAVE], 64, 0, sc[GK_ACC_MCA_WAVE]>>>(P, GK_ACC_MCA_WAVE);
+        if (MODE == MODE_TC && P.hub)
+            k_mca_wave<MODE_TC, 16, true><<<L.grid[GK_ACC_MCA_WAVE], 64, 0, sc[GK_ACC_MCA_WAVE]>>>(P, GK_ACC_MCA_WAVE);
+        else
+            k_mca_wave<MODE, 16, false><<<L.grid[GK_ACC_MCA_WAVE], 64, 0, sc[GK_ACC_MCA_WAVE]>>>(P, GK_ACC_MCA_WAVE);
         if (tm) tm[GK_ACC_MCA_WAVE].stop();
         GK_TRY(launch_check("k_mca_wave"));
     }

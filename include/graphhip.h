@@ -106,9 +106,9 @@ typedef struct gk_opts {
     int32_t batch_blocks;   /* batching: peeling workgroups per CU                   */
     int32_t batch_order;    /* batching: 0 = graphs largest first, 1 = given order   */
     int32_t hub_window;     /* triangle counting: B rows below this id (<= 32768) are
-                             * kept as bitmaps and intersected with the mask row's
-                             * bitmap word by word where that reads fewer words than
-                             * the row has entries; 0 = off                          */
+                             * also kept as bitmaps; a row is read as bitmap words
+                             * (window kernels) or single bits (MCA-wave) where that
+                             * reads less than the row; 0 = off, default 16384       */
 } gk_opts;
 void gk_opts_default(gk_opts *o);
 
