@@ -68,6 +68,47 @@ def test_samples_learned_selection_and_gain(gpu):
     assert t_l < 3.0 * t_h
 
 
+def _heuristic_choice(X, V):
+    """The fixed-threshold rule on the features' mask length (defaults:
+    MCA-wave up to 8, MSA-LDS up to 1536, dense-global from 16384)."""
+    mlen = np.round(2 ** X[:, 0] - 1).astype(np.int64)
+    c = np.where(mlen <= 8, 1, np.where(mlen <= 1536, 0, np.where(mlen >= 16384, 3, 2)))
+    c = np.where(V[np.arange(len(c)), c].astype(bool), c, 0)  # capacity fallback
+    return c
+
+
+def test_network_gain_on_held_out_rows(gpu):
+    """What the trained network gains: train on some graphs, then on the rows
+    of a held-out graph compare the measured cost (the labels) of the
+    accumulators picked by the network, by the fixed-threshold heuristic and
+    by a trivial predictor (each accumulator's mean label, one choice for
+    every row).  The network's predictions must explain most of the label
+    variance and its picks must cost less than the trivial predictor's."""
+    train_graphs = [HostGraph.rmat(14, 16, seed=5), HostGraph.grid(128), HostGraph.chung_lu(20000, 2.1, 16, seed=5)]
+    selector.train_from_graphs(gpu, train_graphs, epochs=30, seed=1)
+    G = HostGraph.rmat(15, 16, seed=11)  # held out
+    d = gpu.upload(G.L)
+    X, T, V = selector.samples(gpu, d)
+    Vb = V.astype(bool)
+    Y = selector.forward(gpu, X)
+    resid = (Y - T)[Vb]
+    r2 = 1.0 - np.mean(resid ** 2) / np.var(T[Vb])
+    rows = np.arange(len(X))
+    net = np.where(Vb, Y, np.inf).argmin(axis=1)
+    heur = _heuristic_choice(X, V)
+    mean_label = np.array([T[Vb[:, a], a].mean() if Vb[:, a].any() else np.inf for a in range(4)])
+    triv = np.where(Vb, mean_label[None, :], np.inf).argmin(axis=1)
+    cost = lambda c: float(np.exp2(T[rows, c]).sum())
+    c_net, c_heur, c_triv = cost(net), cost(heur), cost(triv)
+    best = float(np.exp2(np.where(Vb, T, np.inf)).min(axis=1).sum())
+    print(f"held-out R-MAT s15 rows {len(X)}: R^2 {r2:.3f}; summed label ns of the picks: network {c_net:.4g}, "
+          f"heuristic {c_heur:.4g}, trivial {c_triv:.4g}, per-row best {best:.4g}; "
+          f"network/heuristic {c_heur / c_net:.3f}x", flush=True)
+    assert r2 > 0.5
+    assert c_net <= c_triv
+    assert c_net >= best * (1 - 1e-6)
+
+
 def test_sample_labels_follow_capacities(gpu):
     G = HostGraph.rmat(12, 16, seed=6)
     d = gpu.upload(G.L)
