@@ -11,13 +11,14 @@ with the graph resident in HBM.  A step is one whole decomposition.
 
 Also reported, each in its own object:
   tc  triangle counting C<L> = L*L on R-MAT scale 22 (same generator),
-      edges/s, fixed-threshold selector and learned selector when a trained
-      model is present;
+      edges/s: plain fixed thresholds (every B row streamed), the library
+      default (plus hub bitmaps), autotuned, learned, learned + autotuned;
+      the same variants and the k-truss on a 2048^2 grid and a 4M-vertex
+      Chung-Lu graph;
   bc  batched Brandes BC, 4 batches of 256 seeded sources on the scale-20
       graph, FP64, source-edges/s;
   batch  triangle count + 4-truss of each of 4096 seeded uneven graphs in one
-      hipGraph-captured step, edges/s, knobs autotuned (switched off: see
-      run_batch).
+      hipGraph-captured step, edges/s, knobs autotuned.
 `roofline` is for the dominant kernel of the headline run (the accumulator
 kernel with the most device time): algorithmic bytes per launch over its
 average launch duration (HIP events on the library's stream).
@@ -220,10 +221,6 @@ def run_batch(ctx, a) -> dict:
     graphs (vertex counts log-uniform in [1024, 65536], half Erdos-Renyi with
     mean degree U[8,32], half Chung-Lu with exponent U[2.1,2.8] and mean
     degree 16), as one hipGraph-captured step; knobs autotuned."""
-    # The full-size run (4096 graphs, a 4.4 GB union CSR) faulted the GPU once
-    # and the cause is not established; the leg stays switched off until it is.
-    raise RuntimeError("the batching leg is switched off: its full-size run faulted the GPU and the cause "
-                       "is not established (the batching tests at small sizes run)")
     from graphkokkos_amd import HostGraph
     from graphkokkos_amd import selector as SEL
     from graphkokkos_amd.graph import batch_config
@@ -234,13 +231,16 @@ def run_batch(ctx, a) -> dict:
     log(f"batch: {a.batch_graphs} graphs, {U.n} vertices, {E} edges built in {time.perf_counter() - t0:.1f}s")
     dA = ctx.upload(U.A)
     voff = U.voff
+    log(f"batch: union uploaded ({U.A.nz} entries)")
     eager = ctx.batch_tc_ktruss(dA, voff, k=4, use_graph=False, reps=2)
+    log(f"batch: eager step {eager['ms']:.2f} ms")
     plain = ctx.batch_tc_ktruss(dA, voff, k=4, use_graph=True, reps=max(a.steps, 3))
     assert np.array_equal(eager["triangles"], plain["triangles"]) and \
         np.array_equal(eager["truss_edges"], plain["truss_edges"]), "graph replay differs from the eager run"
     log(f"batch: eager {eager['ms']:.2f} ms, graph {plain['ms']:.2f} ms per step")
     t0 = time.perf_counter()
-    tuned, _, tlog = SEL.autotune_batch(ctx, dA, voff, k=4, rounds=1, reps=2)
+    tuned, _, tlog = SEL.autotune_batch(ctx, dA, voff, k=4, rounds=1, reps=2,
+                                        progress=lambda o, t: log(f"batch: trial {o}: {t:.2f} ms"))
     best = ctx.batch_tc_ktruss(dA, voff, k=4, use_graph=True, reps=max(a.steps, 3), opts=tuned)
     assert np.array_equal(best["triangles"], plain["triangles"]) and \
         np.array_equal(best["truss_edges"], plain["truss_edges"]), "tuned options changed the results"
@@ -307,7 +307,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--legs", default="ktruss,tc,bc")
+    ap.add_argument("--legs", default="ktruss,tc,bc,batch")
     ap.add_argument("--ktruss-scale", type=int, default=20)
     ap.add_argument("--tc-scale", type=int, default=22)
     ap.add_argument("--bc-batches", type=int, default=4)

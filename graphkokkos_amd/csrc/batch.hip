@@ -140,7 +140,7 @@ __global__ __launch_bounds__(kPeelThreads) void k_peel_graphs(
                 if (it < total) {
                     int32_t lo = 0, hi = nF - 1;
                     while (lo < hi) {  // largest f with gp[f] <= it
-                        const int32_t mid = (lo + hi + 1) >> 1;
+                        const int32_t mid = lo + ((hi - lo + 1) >> 1);
                         if (gp[mid] <= it) lo = mid; else hi = mid - 1;
                     }
                     const int32_t e = F[lo], y = gy[lo], x = gx[lo];
@@ -150,7 +150,7 @@ __global__ __launch_bounds__(kPeelThreads) void k_peel_graphs(
                     if (!(s2 == 2 || (s2 == 1 && b < e))) {
                         int32_t l = arst[y], h = arst[y + 1];
                         while (l < h) {
-                            const int32_t mid = (l + h) >> 1;
+                            const int32_t mid = l + ((h - l) >> 1);
                             if (acol[mid] < w) l = mid + 1; else h = mid;
                         }
                         if (l < arst[y + 1] && acol[l] == w) {
@@ -202,7 +202,7 @@ __global__ void k_check_blocks(int64_t m, int64_t ngraphs, const int64_t *__rest
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
         int64_t lo = 0, hi = ngraphs - 1;
         while (lo < hi) {  // largest g with voff[g] <= i
-            const int64_t mid = (lo + hi + 1) >> 1;
+            const int64_t mid = lo + ((hi - lo + 1) >> 1);
             if (voff[mid] <= i) lo = mid; else hi = mid - 1;
         }
         const int64_t b = voff[lo], e = voff[lo + 1];

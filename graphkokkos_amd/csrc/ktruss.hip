@@ -59,7 +59,7 @@ __global__ void k_lower_len(int64_t m, const int32_t *__restrict__ rst, const in
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
         int32_t l = rst[i], h = rst[i + 1];
         while (l < h) {
-            int32_t mid = (l + h) >> 1;
+            int32_t mid = l + ((h - l) >> 1);
             if (col[mid] < (int32_t)i) l = mid + 1; else h = mid;
         }
         len[i] = l - rst[i];
@@ -96,7 +96,7 @@ __global__ void k_truss_to_sym(int64_t m, const int32_t *__restrict__ rst, const
                 // position of i in row j; L_j is the prefix of row j.
                 int32_t l = rst[j], h = rst[j + 1];
                 while (l < h) {
-                    int32_t mid = (l + h) >> 1;
+                    int32_t mid = l + ((h - l) >> 1);
                     if (col[mid] < (int32_t)i) l = mid + 1; else h = mid;
                 }
                 e = lrst[j] + (l - rst[j]);
@@ -222,7 +222,7 @@ __global__ void k_sym_eid(int64_t m, const int32_t *__restrict__ rst, const int3
             } else {
                 int32_t l = rst[j], h = rst[j + 1];
                 while (l < h) {
-                    int32_t mid = (l + h) >> 1;
+                    int32_t mid = l + ((h - l) >> 1);
                     if (col[mid] < (int32_t)i) l = mid + 1; else h = mid;
                 }
                 eid[p] = lrst[j] + (l - rst[j]);
@@ -443,13 +443,13 @@ __device__ __forceinline__ void k_peel_body(int32_t nF, const int32_t *__restric
         const int32_t glast = (int32_t)min<int64_t>(g0 + CH, total) - 1;
         int32_t lo = 0, hi = nF - 1;
         while (lo < hi) {  // largest f with pre[f] <= g0
-            int32_t mid = (lo + hi + 1) >> 1;
+            int32_t mid = lo + ((hi - lo + 1) >> 1);
             if (pre[mid] <= g0) lo = mid; else hi = mid - 1;
         }
         const int32_t f_lo = lo;
         hi = nF - 1;
         while (lo < hi) {  // largest f with pre[f] <= glast
-            int32_t mid = (lo + hi + 1) >> 1;
+            int32_t mid = lo + ((hi - lo + 1) >> 1);
             if (pre[mid] <= glast) lo = mid; else hi = mid - 1;
         }
         const int32_t f_hi = lo;
@@ -461,7 +461,7 @@ __device__ __forceinline__ void k_peel_body(int32_t nF, const int32_t *__restric
             if (g < total) {
                 int32_t l = f_lo, h = f_hi;
                 while (l < h) {
-                    int32_t mid = (l + h + 1) >> 1;
+                    int32_t mid = l + ((h - l + 1) >> 1);
                     if (pre[mid] <= g) l = mid; else h = mid - 1;
                 }
                 fe[u] = F[l];
@@ -1137,7 +1137,7 @@ __global__ void k_alive_to_sym(int64_t m, const int32_t *__restrict__ rst, const
             } else {
                 int32_t l = rst[j], h = rst[j + 1];
                 while (l < h) {
-                    int32_t mid = (l + h) >> 1;
+                    int32_t mid = l + ((h - l) >> 1);
                     if (col[mid] < (int32_t)i) l = mid + 1; else h = mid;
                 }
                 e = lrst[j] + (l - rst[j]);

@@ -672,7 +672,7 @@ __global__ __launch_bounds__(64 * NW) void k_msa_window(Prob P, int cls) {
         // mask columns are sorted: the ones >= kWindow form the row's tail
         int32_t l = 0, h = mlen;
         while (l < h) {
-            int32_t mid = (l + h) >> 1;
+            int32_t mid = l + ((h - l) >> 1);
             if (P.mcol[mb + mid] < kWindow) l = mid + 1; else h = mid;
         }
         const int32_t nlow = l, nhigh = mlen - l;
@@ -720,7 +720,7 @@ __global__ __launch_bounds__(64 * NW) void k_msa_window(Prob P, int cls) {
                 if (!SUP) return 0;
                 int32_t x = 0, y = nlow;  // position of b in mk[0, nlow)
                 while (x < y) {
-                    int32_t mid = (x + y) >> 1;
+                    int32_t mid = x + ((y - x) >> 1);
                     if (mk[mid] < b) x = mid + 1; else y = mid;
                 }
                 return x;
@@ -770,7 +770,7 @@ __device__ __forceinline__ int2 build_prefix_probe(const Prob &P, int32_t a0, in
             if (k < P.hub_n && len > 0) {
                 int32_t l = 0, h = mlen;  // mask columns below k
                 while (l < h) {
-                    const int32_t mid = (l + h) >> 1;
+                    const int32_t mid = l + ((h - l) >> 1);
                     if (m[mid] < k) l = mid + 1; else h = mid;
                 }
                 if (l < len) {
@@ -890,7 +890,7 @@ __global__ __launch_bounds__(64) void k_mca_wave(Prob P, int cls) {
                     if (t[u] >= 0 && b[u] >= mlo && b[u] <= mhi) {
                         int32_t l = 0, h = mlen;  // lower_bound in m[0, mlen)
                         while (l < h) {
-                            int32_t mid = (l + h) >> 1;
+                            int32_t mid = l + ((h - l) >> 1);
                             if (m[mid] < b[u]) l = mid + 1; else h = mid;
                         }
                         if (m[l] == b[u]) q = l;

@@ -142,12 +142,16 @@ def time_tc(ctx: Context, L: DeviceCsr, opts, reps: int = 3) -> float:
     return (time.perf_counter() - t0) / reps
 
 
-def coordinate_descent(measure, space: dict, base: dict | None = None, rounds: int = 2) -> tuple[dict, float, list]:
+def coordinate_descent(measure, space: dict, base: dict | None = None, rounds: int = 2,
+                       progress=None) -> tuple[dict, float, list]:
     """Minimise measure(options dict) one knob at a time over `space`;
-    returns (best options, best value, trial log)."""
+    returns (best options, best value, trial log).  progress(options, value)
+    is called after every trial when given."""
     best = dict(base or {})
     best_t = measure(best)
     log = [(dict(best), best_t)]
+    if progress:
+        progress(best, best_t)
     for _ in range(rounds):
         improved = False
         for key, values in space.items():
@@ -158,6 +162,8 @@ def coordinate_descent(measure, space: dict, base: dict | None = None, rounds: i
                 cand[key] = v
                 t = measure(cand)
                 log.append((dict(cand), t))
+                if progress:
+                    progress(cand, t)
                 if t < best_t * 0.98:
                     best, best_t, improved = cand, t, True
         if not improved:
@@ -184,11 +190,11 @@ BATCH_SPACE = {
 
 
 def autotune_batch(ctx: Context, A: DeviceCsr, voff, k: int = 4, base: dict | None = None, rounds: int = 2,
-                   reps: int = 2, space: dict | None = None) -> tuple[dict, float, list]:
+                   reps: int = 2, space: dict | None = None, progress=None) -> tuple[dict, float, list]:
     """Coordinate descent over the batching knobs (peeling workgroups per CU,
     graph order) and the support product's bin thresholds and tile/grid
     sizes, timing the captured batch step; returns (best options, best ms,
     trial log)."""
     def measure(o):
         return ctx.batch_tc_ktruss(A, voff, k=k, use_graph=True, reps=reps, opts=default_opts(**o))["ms"]
-    return coordinate_descent(measure, space or BATCH_SPACE, base, rounds)
+    return coordinate_descent(measure, space or BATCH_SPACE, base, rounds, progress)
