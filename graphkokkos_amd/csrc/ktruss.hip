@@ -345,8 +345,9 @@ __device__ __forceinline__ int32_t adj_probe(const int64_t *__restrict__ hoff, c
 // Round prologue.  The previous round's frontier Fp becomes removed
 // (state 2); the new frontier F becomes state 1 with truss label `label`.
 // Per frontier edge f = (y, x): y = endpoint with the longer current row
-// (probed through its table), x = the other (its alive row is streamed);
-// len[f] = current row length of x, or 0 when f's support is 0: the
+// (probed through its table), x = the other (its alive row is streamed;
+// frs[f] = where that row starts); len[f] = current row length of x, or 0
+// when f's support is 0: the
 // support of a frontier edge counts its triangles that are still intact
 // (every decrement of earlier rounds has landed), so such an edge closes
 // no triangle this round and its row need not be streamed.
@@ -354,15 +355,28 @@ __device__ __forceinline__ int32_t prep_edge(int32_t f, const int32_t *__restric
                                              uint32_t *__restrict__ truss, uint32_t label,
                                              const int32_t *__restrict__ eu, const int32_t *__restrict__ ev,
                                              const int32_t *__restrict__ sup, const int32_t *__restrict__ rst,
-                                             int32_t *__restrict__ fy, int32_t *__restrict__ fx) {
+                                             int32_t *__restrict__ fy, int32_t *__restrict__ frs) {
     const int32_t e = F[f], u = eu[e], v = ev[e];
     st[e] = 1;
     truss[e] = label;
-    const int32_t du = rst[u + 1] - rst[u], dv = rst[v + 1] - rst[v];
+    const int32_t ru = rst[u], rv = rst[v];
+    const int32_t du = rst[u + 1] - ru, dv = rst[v + 1] - rv;
     const bool uy = du > dv || (du == dv && u > v);
     fy[f] = uy ? u : v;
-    fx[f] = uy ? v : u;
+    frs[f] = uy ? rv : ru;
     return sup[e] > 0 ? (uy ? dv : du) : 0;
+}
+
+// The streamed items of a round are cut into chunks of kPeelChunk (one
+// wave's share per step); cstart[c] is the frontier edge owning item
+// c * kPeelChunk, written by the owner of each chunk start (f with items
+// [pre_f, pre_f + len) and len > 0), so the stream kernel needs no search
+// over the whole frontier.  Chunks at or beyond ccap are not recorded (a
+// round that long searches instead).
+__device__ __forceinline__ void mark_chunks(int32_t f, int32_t pre_f, int32_t len, int32_t chunk,
+                                            int32_t *__restrict__ cstart, int32_t ccap) {
+    if (len <= 0) return;
+    for (int32_t c = (pre_f + chunk - 1) / chunk; c < ccap && c * chunk < pre_f + len; ++c) cstart[c] = f;
 }
 
 
@@ -391,7 +405,30 @@ struct RoundCtl {
     int32_t stop;     // set when a round cannot run on the device path
     int32_t rounds;   // rounds run by the batch
     int32_t removed;  // frontier edges of those rounds
-    int32_t pad;
+    int32_t adv;      // a streamed round waits for its advance (one-block path)
+};
+
+// Moves the control block past a streamed round: its frontier becomes the
+// previous one (to be marked removed), its crossings the next frontier.
+__host__ __device__ __forceinline__ void advance_round(RoundCtl &c) {
+    c.rounds += 1;
+    c.removed += c.nF;
+    c.nP = c.nF;
+    c.nF = c.nFn;
+    c.cur ^= 1;
+    c.adv = 0;
+}
+
+// Control block of a decomposition in one allocation, so the host reads
+// it back with one copy: the round control and the counters of the list
+// scans (frontier size, complemented minimum support, near-list length,
+// alive-list compaction).
+struct PeelCtl {
+    RoundCtl rc;
+    int32_t n;
+    uint32_t mn;
+    int32_t nl;
+    int32_t ac;
 };
 
 // Near list: the alive edges whose support is below `hi` (plus removed
@@ -424,85 +461,129 @@ __device__ __forceinline__ void wave_append(const int32_t (&v)[N], int32_t *__re
 
 template <int U>
 __device__ __forceinline__ void k_peel_body(int32_t nF, const int32_t *__restrict__ pre,
-                                                     const int32_t *__restrict__ F, const int32_t *__restrict__ fy,
-                                                     const int32_t *__restrict__ fx, const int32_t *__restrict__ rst,
-                                                     const int32_t *__restrict__ col, const int32_t *__restrict__ eid,
-                                                     const int64_t *__restrict__ hoff, const int2 *__restrict__ tab,
-                                                     const uint8_t *__restrict__ st, int32_t *__restrict__ sup,
-                                                     int32_t thresh, int32_t *__restrict__ Fn,
-                                                     int32_t *__restrict__ nFn, const NearList nl,
-                                                     unsigned long long *__restrict__ ctr) {
+                                            const int32_t *__restrict__ cstart, int32_t ccap,
+                                            const int32_t *__restrict__ F,
+                                            const int32_t *__restrict__ fy, const int32_t *__restrict__ frs,
+                                            const int32_t *__restrict__ col, const int32_t *__restrict__ eid,
+                                            const int64_t *__restrict__ hoff, const int2 *__restrict__ tab,
+                                            const uint8_t *__restrict__ st, int32_t *__restrict__ sup,
+                                            int32_t thresh, int32_t *__restrict__ Fn, int32_t *__restrict__ nFn,
+                                            const NearList nl, unsigned long long *__restrict__ ctr) {
     constexpr int CH = 64 * U;
     const int lane = threadIdx.x & 63;
     const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
     const int32_t total = pre[nF];
     uint32_t cnt = 0;
+    // Each step resolves U items per lane in stages, every stage's loads
+    // issued for all U items before any is used: owner edge -> row entry
+    // (column w, edge id b) and y's table bounds -> state of b and the first
+    // table slot -> (rare) further probes -> state of a -> support updates.
     for (int64_t g0 = wave * CH; g0 < total; g0 += nwaves * CH) {
-        // frontier edges covering items [g0, g0 + CH): f_lo .. f_hi
-        const int32_t glast = (int32_t)min<int64_t>(g0 + CH, total) - 1;
-        int32_t lo = 0, hi = nF - 1;
-        while (lo < hi) {  // largest f with pre[f] <= g0
-            int32_t mid = lo + ((hi - lo + 1) >> 1);
-            if (pre[mid] <= g0) lo = mid; else hi = mid - 1;
+        const int64_t c = g0 / CH;
+        int32_t f_lo, f_hi;
+        if (c + 1 < ccap) {
+            f_lo = cstart[c];
+            f_hi = g0 + CH < total ? cstart[c + 1] : nF - 1;
+        } else {  // beyond the recorded chunks: search the frontier
+            const int32_t glast = (int32_t)min<int64_t>(g0 + CH, total) - 1;
+            int32_t lo = 0, hi = nF - 1;
+            while (lo < hi) {  // largest f with pre[f] <= g0
+                const int32_t mid = lo + ((hi - lo + 1) >> 1);
+                if (pre[mid] <= g0) lo = mid; else hi = mid - 1;
+            }
+            f_lo = lo;
+            hi = nF - 1;
+            while (lo < hi) {  // largest f with pre[f] <= glast
+                const int32_t mid = lo + ((hi - lo + 1) >> 1);
+                if (pre[mid] <= glast) lo = mid; else hi = mid - 1;
+            }
+            f_hi = lo;
         }
-        const int32_t f_lo = lo;
-        hi = nF - 1;
-        while (lo < hi) {  // largest f with pre[f] <= glast
-            int32_t mid = lo + ((hi - lo + 1) >> 1);
-            if (pre[mid] <= glast) lo = mid; else hi = mid - 1;
-        }
-        const int32_t f_hi = lo;
-        int32_t t[U], fe[U], yy[U], w[U], e2[U];
+        int32_t t[U], fe[U], yy[U];
+        int32_t l = f_lo;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int32_t g = (int32_t)g0 + 64 * u + lane;
             t[u] = -1;
+            fe[u] = yy[u] = 0;
             if (g < total) {
-                int32_t l = f_lo, h = f_hi;
+                int32_t h = f_hi;  // largest f in [l, f_hi] with pre[f] <= g (items grow with u)
                 while (l < h) {
-                    int32_t mid = l + ((h - l + 1) >> 1);
+                    const int32_t mid = l + ((h - l + 1) >> 1);
                     if (pre[mid] <= g) l = mid; else h = mid - 1;
                 }
                 fe[u] = F[l];
                 yy[u] = fy[l];
-                t[u] = rst[fx[l]] + (g - pre[l]);
+                t[u] = frs[l] + (g - pre[l]);
             }
         }
+        int32_t w[U], b[U];
+        int64_t h0[U];
+        uint32_t cap[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u)
+        for (int u = 0; u < U; ++u) {
+            w[u] = b[u] = 0;
+            h0[u] = 0;
+            cap[u] = 1;
             if (t[u] >= 0) {
                 w[u] = col[t[u]];
-                e2[u] = eid[t[u]];
+                b[u] = eid[t[u]];
+                h0[u] = hoff[yy[u]];
+                cap[u] = (uint32_t)(hoff[yy[u] + 1] - h0[u]);
             }
+        }
+        uint8_t s2[U];
+        uint32_t slot[U];
+        int2 kv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            s2[u] = 2;
+            slot[u] = 0;
+            kv[u] = make_int2(-1, -1);
+            if (t[u] >= 0) {
+                s2[u] = st[b[u]];
+                const int lg = 31 - __clz((int)cap[u]);
+                slot[u] = dev::hash32((uint32_t)w[u]) >> (32 - lg);
+                kv[u] = tab[h0[u] + slot[u]];
+            }
+        }
+        int32_t a[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            a[u] = -1;
+            if (t[u] < 0 || s2[u] == 2 || (s2[u] == 1 && b[u] < fe[u])) continue;
+            for (;;) {  // y's table: (neighbour, edge id), -1 marks an empty slot
+                if (kv[u].x == w[u]) { a[u] = kv[u].y; break; }
+                if (kv[u].x < 0) break;
+                slot[u] = (slot[u] + 1) & (cap[u] - 1);
+                kv[u] = tab[h0[u] + slot[u]];
+            }
+        }
+        uint8_t s1[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) s1[u] = a[u] >= 0 ? st[a[u]] : 2;
         // edges leaving the threshold / entering the near list (-1: none)
         int32_t xf[2 * U], xn[2 * U];
 #pragma unroll
         for (int u = 0; u < 2 * U; ++u) xf[u] = xn[u] = -1;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            if (t[u] < 0) continue;
-            const int32_t e = fe[u], b = e2[u];
-            const uint8_t s2 = st[b];
-            if (s2 == 2 || (s2 == 1 && b < e)) continue;
-            const int32_t a = adj_probe(hoff, tab, yy[u], w[u]);
-            if (a < 0) continue;
-            const uint8_t s1 = st[a];
-            if (s1 == 2 || (s1 == 1 && a < e)) continue;
+            if (a[u] < 0 || s1[u] == 2 || (s1[u] == 1 && a[u] < fe[u])) continue;
             ++cnt;
             // an alive edge whose support falls below the threshold right
             // now (old value == thresh; supports drop by one at a time)
             // joins the next round's frontier; one falling below the near
             // list's bound joins the near list
-            if (s1 == 0) {
-                const int32_t old = atomicSub(&sup[a], 1);
-                if (old == thresh) xf[2 * u] = a;
-                else if (old == nl.hi) xn[2 * u] = a;
+            if (s1[u] == 0) {
+                const int32_t old = atomicSub(&sup[a[u]], 1);
+                if (old == thresh) xf[2 * u] = a[u];
+                else if (old == nl.hi) xn[2 * u] = a[u];
             }
-            if (s2 == 0) {
-                const int32_t old = atomicSub(&sup[b], 1);
-                if (old == thresh) xf[2 * u + 1] = b;
-                else if (old == nl.hi) xn[2 * u + 1] = b;
+            if (s2[u] == 0) {
+                const int32_t old = atomicSub(&sup[b[u]], 1);
+                if (old == thresh) xf[2 * u + 1] = b[u];
+                else if (old == nl.hi) xn[2 * u + 1] = b[u];
             }
         }
         wave_append(xf, Fn, nFn);
@@ -513,24 +594,37 @@ __device__ __forceinline__ void k_peel_body(int32_t nF, const int32_t *__restric
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctr[0], (unsigned long long)total);
 }
 
+// streamed items per wave step of the peeling kernels
+constexpr int kPeelChunk = 64 * kPeelU;
 
-// Device-driven round: prologue + scan in one block,
-// sizes from the control block.  A frontier that is empty or too big for
-// one block stops the batch; the remaining kernels of the batch return.
+// Device-driven round: the previous round's advance, then prologue + scan in
+// one block, sizes from the control block.  A frontier that is empty or too
+// big for one block stops the batch; the remaining kernels of the batch
+// return.  The round's own advance is left pending (adv) for the next
+// prologue, or for the host after the batch.
 __global__ __launch_bounds__(kSmallRoundThreads) void k_dround_prologue(
     RoundCtl *__restrict__ ctl, int32_t *__restrict__ buf0, int32_t *__restrict__ buf1, uint8_t *__restrict__ st,
     uint32_t *__restrict__ truss, uint32_t label, const int32_t *__restrict__ eu, const int32_t *__restrict__ ev,
     const int32_t *__restrict__ sup, const int32_t *__restrict__ rst, int32_t *__restrict__ fy,
-    int32_t *__restrict__ fx, int32_t *__restrict__ pre) {
+    int32_t *__restrict__ fx, int32_t *__restrict__ pre, int32_t *__restrict__ cstart, int32_t ccap) {
     __shared__ int32_t wsum[kSmallRoundThreads / 64];
-    if (ctl->stop) return;
-    const int32_t nF = ctl->nF, nP = ctl->nP;
+    __shared__ RoundCtl c;
+    if (threadIdx.x == 0) {
+        c = *ctl;
+        if (!c.stop && c.adv) {
+            advance_round(c);
+            *ctl = c;
+        }
+    }
+    __syncthreads();
+    if (c.stop) return;
+    const int32_t nF = c.nF, nP = c.nP;
     if (nF == 0 || nF >= kSmallRound) {
         if (threadIdx.x == 0) ctl->stop = 1;
         return;
     }
-    const int32_t *F = ctl->cur ? buf1 : buf0;
-    const int32_t *Fp = ctl->cur ? buf0 : buf1;
+    const int32_t *F = c.cur ? buf1 : buf0;
+    const int32_t *Fp = c.cur ? buf0 : buf1;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     for (int32_t f = tid; f < nP; f += kSmallRoundThreads) st[Fp[f]] = 2;
     int32_t l[kSmallRoundPer], run = 0;
@@ -550,16 +644,20 @@ __global__ __launch_bounds__(kSmallRoundThreads) void k_dround_prologue(
     for (int j = 0; j < kSmallRoundPer; ++j) {
         const int32_t f = tid * kSmallRoundPer + j;
         if (f <= nF) pre[f] = ex;
+        if (f < nF) mark_chunks(f, ex, l[j], kPeelChunk, cstart, ccap);
         ex += l[j];
     }
-    if (tid == 0) ctl->nFn = 0;
+    if (tid == 0) {
+        ctl->nFn = 0;
+        ctl->adv = 1;  // the stream kernel that follows runs this round
+    }
 }
 
 template <int U>
 __global__ __launch_bounds__(256) void k_dround_stream(const RoundCtl *__restrict__ ctl, int32_t *__restrict__ buf0,
                                                        int32_t *__restrict__ buf1, const int32_t *__restrict__ pre,
                                                        const int32_t *__restrict__ fy, const int32_t *__restrict__ fx,
-                                                       const int32_t *__restrict__ rst,
+                                                       const int32_t *__restrict__ cstart, int32_t ccap,
                                                        const int32_t *__restrict__ col,
                                                        const int32_t *__restrict__ eid,
                                                        const int64_t *__restrict__ hoff, const int2 *__restrict__ tab,
@@ -569,7 +667,7 @@ __global__ __launch_bounds__(256) void k_dround_stream(const RoundCtl *__restric
     if (ctl->stop) return;
     const int32_t cur = ctl->cur;
     // the crossings go to buf[cur ^ 1], counted in ctl->nFn
-    k_peel_body<U>(ctl->nF, pre, cur ? buf1 : buf0, fy, fx, rst, col, eid, hoff, tab, st, sup, thresh,
+    k_peel_body<U>(ctl->nF, pre, cstart, ccap, cur ? buf1 : buf0, fy, fx, col, eid, hoff, tab, st, sup, thresh,
                    cur ? buf0 : buf1, nFn, nl, ctr);
 }
 
@@ -579,11 +677,9 @@ __global__ __launch_bounds__(256) void k_dround_stream(const RoundCtl *__restric
 // L2.)
 __global__ void k_dround_advance(RoundCtl *__restrict__ ctl) {
     if (threadIdx.x != 0 || ctl->stop) return;
-    ctl->rounds += 1;
-    ctl->removed += ctl->nF;
-    ctl->nP = ctl->nF;
-    ctl->nF = ctl->nFn;
-    ctl->cur ^= 1;
+    RoundCtl c = *ctl;
+    advance_round(c);
+    *ctl = c;
 }
 
 // Device-driven rounds for frontiers of at least kSmallRound edges, in
@@ -642,7 +738,8 @@ __global__ __launch_bounds__(kBigBlocksMax) void k_big_scan_top(RoundCtl *__rest
 
 __global__ __launch_bounds__(256) void k_big_scan_apply(const RoundCtl *__restrict__ ctl,
                                                         const int32_t *__restrict__ len,
-                                                        const int32_t *__restrict__ boff, int32_t *__restrict__ pre) {
+                                                        const int32_t *__restrict__ boff, int32_t *__restrict__ pre,
+                                                        int32_t *__restrict__ cstart, int32_t ccap) {
     __shared__ int32_t ws[256 / 64];
     if (ctl->stop) return;
     const int32_t nF = ctl->nF;
@@ -661,7 +758,10 @@ __global__ __launch_bounds__(256) void k_big_scan_apply(const RoundCtl *__restri
             before += x < w ? ws[x] : 0;
             tot += ws[x];
         }
-        if (f < f1) pre[f] = carry + before + incl - l;
+        if (f < f1) {
+            pre[f] = carry + before + incl - l;
+            mark_chunks(f, carry + before + incl - l, l, kPeelChunk, cstart, ccap);
+        }
         carry += tot;
         __syncthreads();
     }
@@ -759,10 +859,8 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
     int64_t *caps, *hoff;
     int2 *tab;
     uint8_t *st;
-    uint32_t *dmin;
-    int32_t *dn;
     unsigned long long *dctr;  // [0] items streamed, [1] triangles destroyed
-    RoundCtl *dctl;
+    PeelCtl *dpc;
     GK_TRY(B.alloc(&sup, nL));
     GK_TRY(B.alloc(&eu, nL));
     GK_TRY(B.alloc(&ev, nL));
@@ -774,18 +872,23 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
     GK_TRY(B.alloc(&fx, nL));
     GK_TRY(B.alloc(&flen, nL + 1));
     GK_TRY(B.alloc(&fpre, nL + 1));
+    // owner of every chunk start, for rounds of up to about A.nz items
+    // (longer rounds search the frontier instead)
+    const int32_t ccap = (int32_t)std::min<int64_t>(A->nz / kPeelChunk + 2, INT32_MAX);
+    int32_t *cstart;
+    GK_TRY(B.alloc(&cstart, ccap));
     GK_TRY(B.alloc(&st, nL));
     GK_TRY(B.alloc(&cnt, m + 1));
     GK_TRY(B.alloc(&caps, m + 1));
     GK_TRY(B.alloc(&hoff, m + 1));
-    GK_TRY(B.alloc(&dmin, 1));
-    GK_TRY(B.alloc(&dn, 2));
     GK_TRY(B.alloc(&dctr, 2));
-    GK_TRY(B.alloc(&dctl, 1));
-    int32_t *NL, *NL2, *dnl;
+    GK_TRY(B.alloc(&dpc, 1));
+    RoundCtl *dctl = &dpc->rc;
+    int32_t *dn = &dpc->n, *dnl = &dpc->nl, *dac = &dpc->ac;
+    uint32_t *dmin = &dpc->mn;
+    int32_t *NL, *NL2;
     GK_TRY(B.alloc(&NL, nL));
     GK_TRY(B.alloc(&NL2, nL));
-    GK_TRY(B.alloc(&dnl, 1));
     for (int b = 0; b < 2; ++b) {
         GK_TRY(B.alloc(&arst[b], m + 1));
         GK_TRY(B.alloc(&acol[b], A->nz));
@@ -832,7 +935,9 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
     int k = fixed_k > 0 ? fixed_k : 3;
     int32_t kbest = nL ? 2 : 0;
     int32_t *h = (int32_t *)(h64 + 2);
-    RoundCtl *hctl = (RoundCtl *)(h64 + 4);
+    static_assert(sizeof(PeelCtl) <= 128 - 32, "control block fits the pinned staging");
+    PeelCtl *hpc = (PeelCtl *)(h64 + 4);
+    RoundCtl *hctl = &hpc->rc;
     std::vector<hipEvent_t> &ev_pool = ctx->events;
     size_t ev_used = 0;
     auto ev_pair = [&](size_t *at) -> int {
@@ -852,8 +957,14 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
     int32_t nF = 0;              // current frontier
     bool scan_needed = true;     // the frontier comes from a scan
     bool started = false;
-    constexpr int kBatch = 16;   // device-driven small rounds per host check
-    constexpr int kBigBatch = 4; // device-driven large rounds per host check
+    // Device-driven rounds per host check, adapted to the cascades: a batch
+    // whose rounds all ran doubles the next one; a batch cut short (the
+    // level ended, or the frontier changed size class) sizes the next one
+    // from the rounds it ran, so few launches are wasted on stopped rounds.
+    int batch_small = 8, batch_big = 4;
+    auto next_batch = [](int size, int32_t ran, int cap) {
+        return ran >= size ? std::min(2 * size, cap) : std::max(2, std::min<int>(ran + ran / 2 + 1, cap));
+    };
     const int nbig = (int)std::min<int64_t>(kBigBlocksMax, (int64_t)ctx->num_cus * 2);
     int32_t *bsum;
     GK_TRY(B.alloc(&bsum, kBigBlocksMax));
@@ -869,9 +980,7 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
     // one scan at threshold thr: nF, the minimum support over the scanned
     // edges (h[1], complemented) and the near list's new length
     auto scan = [&](bool full, int32_t thr) -> int {
-        GK_HIP(hipMemsetAsync(dn, 0, 4, ctx->stream));
-        GK_HIP(hipMemsetAsync(dmin, 0, 4, ctx->stream));
-        GK_HIP(hipMemsetAsync(dnl, 0, 4, ctx->stream));
+        GK_HIP(hipMemsetAsync(dn, 0, 12, ctx->stream));  // n, mn, nl
         if (full) {
             const int32_t hi = fixed_k > 0 ? thr : (int32_t)std::min<int64_t>((int64_t)thr + nl_window, INT32_MAX - 1);
             k_frontier<<<grid_scan(ctx, na), 256, 0, ctx->stream>>>(na, alist, st, sup, thr, fb[fc], dn, dmin,
@@ -883,9 +992,7 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
             std::swap(NL, NL2);
         }
         GK_TRY(launch_check("k_frontier"));
-        GK_HIP(hipMemcpyAsync(h, dn, 4, hipMemcpyDeviceToHost, ctx->stream));
-        GK_HIP(hipMemcpyAsync(h + 1, dmin, 4, hipMemcpyDeviceToHost, ctx->stream));
-        GK_HIP(hipMemcpyAsync(h + 2, dnl, 4, hipMemcpyDeviceToHost, ctx->stream));
+        GK_HIP(hipMemcpyAsync(h, dn, 12, hipMemcpyDeviceToHost, ctx->stream));
         GK_HIP(hipStreamSynchronize(ctx->stream));
         nF = h[0];
         nNL = h[2];
@@ -932,25 +1039,25 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
             // a batch of large rounds driven from the device
             *hctl = RoundCtl{nP, nF, 0, fc, 0, 0, 0, 0};
             GK_HIP(hipMemcpyAsync(dctl, hctl, sizeof(RoundCtl), hipMemcpyHostToDevice, ctx->stream));
-            for (int r = 0; r < kBigBatch; ++r) {
+            for (int r = 0; r < batch_big; ++r) {
                 k_big_prologue<<<nbig, 256, 0, ctx->stream>>>(dctl, fb[0], fb[1], st, tl, label, eu, ev, sup,
                                                               arst[cur], fy, fx, flen, bsum);
                 k_big_scan_top<<<1, kBigBlocksMax, 0, ctx->stream>>>(dctl, bsum, nbig);
-                k_big_scan_apply<<<nbig, 256, 0, ctx->stream>>>(dctl, flen, bsum, fpre);
+                k_big_scan_apply<<<nbig, 256, 0, ctx->stream>>>(dctl, flen, bsum, fpre, cstart, ccap);
                 size_t e0;
                 GK_TRY(ev_pair(&e0));
                 GK_HIP(hipEventRecord(ev_pool[e0], ctx->stream));
-                k_dround_stream<kPeelU><<<gs, 256, 0, ctx->stream>>>(dctl, fb[0], fb[1], fpre, fy, fx, arst[cur],
+                k_dround_stream<kPeelU><<<gs, 256, 0, ctx->stream>>>(dctl, fb[0], fb[1], fpre, fy, fx, cstart, ccap,
                                                                 acol[cur], aeid[cur], hoff, tab, st, sup, k - 2,
                                                                 &dctl->nFn, near_list(), dctr);
                 GK_HIP(hipEventRecord(ev_pool[e0 + 1], ctx->stream));
                 k_dround_advance<<<1, 64, 0, ctx->stream>>>(dctl);
             }
             GK_TRY(launch_check("device large rounds"));
-            GK_HIP(hipMemcpyAsync(hctl, dctl, sizeof(RoundCtl), hipMemcpyDeviceToHost, ctx->stream));
-            GK_HIP(hipMemcpyAsync(h + 2, dnl, 4, hipMemcpyDeviceToHost, ctx->stream));
+            GK_HIP(hipMemcpyAsync(hpc, dpc, sizeof(PeelCtl), hipMemcpyDeviceToHost, ctx->stream));
             GK_HIP(hipStreamSynchronize(ctx->stream));
-            nNL = h[2];
+            batch_big = next_batch(batch_big, hctl->rounds, 32);
+            nNL = hpc->nl;
             rounds += hctl->rounds;
             alive -= hctl->removed;
             marked = (int64_t)nP + hctl->removed - hctl->nP;
@@ -962,23 +1069,24 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
             *hctl = RoundCtl{nP, nF, 0, fc, 0, 0, 0, 0};
             GK_HIP(hipMemcpyAsync(dctl, hctl, sizeof(RoundCtl), hipMemcpyHostToDevice, ctx->stream));
             const int gsd = std::min(gs, 1024);
-            for (int r = 0; r < kBatch; ++r) {
+            for (int r = 0; r < batch_small; ++r) {
                 k_dround_prologue<<<1, kSmallRoundThreads, 0, ctx->stream>>>(dctl, fb[0], fb[1], st, tl, label, eu,
-                                                                            ev, sup, arst[cur], fy, fx, fpre);
+                                                                            ev, sup, arst[cur], fy, fx, fpre, cstart,
+                                                                            ccap);
                 size_t e0;
                 GK_TRY(ev_pair(&e0));
                 GK_HIP(hipEventRecord(ev_pool[e0], ctx->stream));
-                k_dround_stream<kPeelU><<<gsd, 256, 0, ctx->stream>>>(dctl, fb[0], fb[1], fpre, fy, fx, arst[cur],
+                k_dround_stream<kPeelU><<<gsd, 256, 0, ctx->stream>>>(dctl, fb[0], fb[1], fpre, fy, fx, cstart, ccap,
                                                                  acol[cur], aeid[cur], hoff, tab, st, sup, k - 2,
                                                                  &dctl->nFn, near_list(), dctr);
                 GK_HIP(hipEventRecord(ev_pool[e0 + 1], ctx->stream));
-                k_dround_advance<<<1, 64, 0, ctx->stream>>>(dctl);
             }
             GK_TRY(launch_check("device rounds"));
-            GK_HIP(hipMemcpyAsync(hctl, dctl, sizeof(RoundCtl), hipMemcpyDeviceToHost, ctx->stream));
-            GK_HIP(hipMemcpyAsync(h + 2, dnl, 4, hipMemcpyDeviceToHost, ctx->stream));
+            GK_HIP(hipMemcpyAsync(hpc, dpc, sizeof(PeelCtl), hipMemcpyDeviceToHost, ctx->stream));
             GK_HIP(hipStreamSynchronize(ctx->stream));
-            nNL = h[2];
+            if (hctl->adv) advance_round(*hctl);  // the batch's last round
+            batch_small = next_batch(batch_small, hctl->rounds, 64);
+            nNL = hpc->nl;
             rounds += hctl->rounds;
             alive -= hctl->removed;
             marked = (int64_t)nP + hctl->removed - hctl->nP;
@@ -989,10 +1097,10 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
         dead_pending += marked;
         // the alive list is compacted once half of it is gone
         if (alive > 0 && 2 * (alive - nF) < na && !(2 * (dead_pending + nP) * 4 > aal_nz)) {
-            GK_HIP(hipMemsetAsync(dn + 1, 0, 4, ctx->stream));
-            k_alist_compact<<<grid_scan(ctx, na), 256, 0, ctx->stream>>>(na, alist, st, alist2, dn + 1);
+            GK_HIP(hipMemsetAsync(dac, 0, 4, ctx->stream));
+            k_alist_compact<<<grid_scan(ctx, na), 256, 0, ctx->stream>>>(na, alist, st, alist2, dac);
             GK_TRY(launch_check("k_alist_compact"));
-            GK_HIP(hipMemcpyAsync(h + 1, dn + 1, 4, hipMemcpyDeviceToHost, ctx->stream));
+            GK_HIP(hipMemcpyAsync(h + 1, dac, 4, hipMemcpyDeviceToHost, ctx->stream));
             GK_HIP(hipStreamSynchronize(ctx->stream));
             na = h[1];
             std::swap(alist, alist2);
@@ -1006,11 +1114,11 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
             GK_TRY(exclusive_scan_i32(ctx, cnt, arst[nx], m + 1));
             k_alive_scatter<<<gw, 256, 0, ctx->stream>>>(m, arst[cur], acol[cur], aeid[cur], st, arst[nx], acol[nx],
                                                          aeid[nx]);
-            GK_HIP(hipMemsetAsync(dn + 1, 0, 4, ctx->stream));
-            k_alist_compact<<<grid_scan(ctx, na), 256, 0, ctx->stream>>>(na, alist, st, alist2, dn + 1);
+            GK_HIP(hipMemsetAsync(dac, 0, 4, ctx->stream));
+            k_alist_compact<<<grid_scan(ctx, na), 256, 0, ctx->stream>>>(na, alist, st, alist2, dac);
             GK_TRY(launch_check("compaction"));
             GK_HIP(hipMemcpyAsync(h, arst[nx] + m, 4, hipMemcpyDeviceToHost, ctx->stream));
-            GK_HIP(hipMemcpyAsync(h + 1, dn + 1, 4, hipMemcpyDeviceToHost, ctx->stream));
+            GK_HIP(hipMemcpyAsync(h + 1, dac, 4, hipMemcpyDeviceToHost, ctx->stream));
             GK_HIP(hipStreamSynchronize(ctx->stream));
             aal_nz = h[0];
             na = h[1];
