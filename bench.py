@@ -313,8 +313,9 @@ def main() -> int:
     ap.add_argument("--bc-batches", type=int, default=4)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--tc-variants", default="all", choices=["all", "plain"],
-                    help="plain: only the fixed-heuristic triangle count (no selector training or autotuning)")
+    ap.add_argument("--tc-variants", default="all", choices=["all", "plain", "default"],
+                    help="plain: only the fixed-heuristic triangle count (no selector training or autotuning); "
+                         "default: plain and the library default (hub bitmaps)")
     ap.add_argument("--batch-graphs", type=int, default=4096)
     ap.add_argument("--no-learned-graphs", action="store_true",
                     help="skip the grid and Chung-Lu graphs of the learned-selector evaluation")
@@ -364,6 +365,17 @@ def main() -> int:
         log(f"tc plain: {plain['wall_s'] * 1e3:.2f} ms, T={plain['count']}, dominant {pd['kernel']} "
             f"{pd['achieved_gbs']:.0f} GB/s")
         tc["plain"]["rows_per_acc"] = plain["timing"]["rows_per_acc"]
+    if tc is not None and a.tc_variants == "default":
+        r = run_tc(ctx, G22.L, a.steps, a.warmup, {})
+        assert r["count"] == plain["count"], "the default options changed the triangle count"
+        d = dominant(r["timing"])
+        tc["heuristic_hub"] = {"value": G22.edges / r["wall_s"], "ms": r["wall_s"] * 1e3, "opts": {},
+                               "speedup_vs_plain": plain["wall_s"] / r["wall_s"],
+                               "roofline": roofline_obj(d, f"tc-rmat{a.tc_scale}-hub"),
+                               "traffic_all_kernels": measured_traffic("tc-product-hub", f"tc-rmat{a.tc_scale}"),
+                               "frac_all_kernels": (r["timing"]["bytes_alg"] / 1e9) / r["wall_s"] / HBM_PEAK_GBS,
+                               "rows_per_acc": r["timing"]["rows_per_acc"]}
+        log(f"tc heuristic_hub: {r['wall_s'] * 1e3:.2f} ms ({plain['wall_s'] / r['wall_s']:.3f}x plain)")
     if tc is not None and a.tc_variants == "all":
         # learned selector: trained on timing samples the product generates
         # on other seeded graphs, then the global knobs autotuned on a
