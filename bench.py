@@ -391,7 +391,8 @@ def main() -> int:
         Gt = HostGraph.rmat(20, 16, seed=a.seed + 2, **RMAT)
         dtune = ctx.upload(Gt.L)
         tuned_h, _, tlog_h = SEL.autotune_tc(ctx, dtune, rounds=2, reps=3)
-        tuned_l, _, tlog_l = SEL.autotune_tc(ctx, dtune, base={"selector": SEL_LEARNED}, rounds=2, reps=3)
+        tuned_l, _, tlog_l = SEL.autotune_tc(ctx, dtune, base={"selector": SEL_LEARNED}, rounds=2, reps=3,
+                                             space=SEL.LEARNED_SPACE)
         dtune.free()
         del Gt
         log(f"autotuned: heuristic {tuned_h}, learned {tuned_l} ({len(tlog_h) + len(tlog_l)} trials)")

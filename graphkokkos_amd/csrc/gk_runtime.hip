@@ -236,6 +236,7 @@ extern "C" void gk_opts_default(gk_opts *o) {
     o->batch_blocks = 8;
     o->batch_order = 0;
     o->hub_window = 16384;
+    o->learned_margin = 250;
 }
 
 extern "C" int gk_init(int device, gk_ctx **out) {

@@ -199,8 +199,16 @@ __device__ __forceinline__ void mlp_tile(const float *__restrict__ w, TileLds &t
     wave_sync();
 }
 
+// Rows this small keep the fixed-threshold choice without inference: their
+// cost is the per-row overhead whichever accumulator runs them, and on
+// graphs of such rows (meshes, the long low-degree tail of power-law
+// graphs in degree order) the inference would cost more than any choice
+// saves.  A tile of 32 rows all this small skips the network.
+constexpr int32_t kTrivialMask = 8;
+constexpr int64_t kTrivialFlop = 256;
+
 __global__ __launch_bounds__(64 * kMlpWaves) void k_mlp_select(int64_t m, RowStats st, int same_am,
-                                                               const float *__restrict__ params) {
+                                                               SelectParams sp, const float *__restrict__ params) {
     __shared__ float w[kParams];
     __shared__ TileLds tl[kMlpWaves];
     for (int x = threadIdx.x; x < kParams; x += blockDim.x) w[x] = params[x];
@@ -210,6 +218,11 @@ __global__ __launch_bounds__(64 * kMlpWaves) void k_mlp_select(int64_t m, RowSta
     const int64_t ntiles = (m + kTile - 1) / kTile;
     for (int64_t tile = (int64_t)blockIdx.x * kMlpWaves + wv; tile < ntiles; tile += (int64_t)gridDim.x * kMlpWaves) {
         const int64_t i = tile * kTile + (lane & 31);
+        const bool trivial = lane >= kTile || i >= m || (st.mlen[i] <= kTrivialMask && st.flop[i] <= kTrivialFlop);
+        if (__ballot(!trivial) == 0) {
+            if (lane < kTile && i < m) st.acc[i] = (int8_t)heuristic_accum(sp, st.mlen[i]);
+            continue;
+        }
         if (lane < kTile) {
             float f[kIn];
             if (i < m) row_features(st, i, same_am, f);
@@ -222,13 +235,17 @@ __global__ __launch_bounds__(64 * kMlpWaves) void k_mlp_select(int64_t m, RowSta
         float y[kOut];
         mlp_tile(w, t, y);
         if (lane < kTile && i < m) {
-            // cheapest predicted accumulator among those that can hold the row
+            // cheapest predicted accumulator among those that can hold the
+            // row; the fixed-threshold choice stays unless it is predicted
+            // to cost more than 2^margin times that
             const int32_t mlen = st.mlen[i];
             const bool can[kOut] = {mlen <= kLdsMaxMask, mlen <= kMcaMaxMask, mlen <= kHashGlobalMaxMask, true};
             int best = GK_ACC_DENSE_GLOBAL;
 #pragma unroll
             for (int o = 0; o < kOut; ++o)
                 if (can[o] && y[o] < y[best]) best = o;
+            const int h = heuristic_accum(sp, mlen);
+            if (can[h] && y[h] - y[best] <= sp.margin) best = h;
             st.acc[i] = (int8_t)best;
         }
     }
@@ -345,15 +362,14 @@ __global__ void k_row_cycles(int32_t lo, int32_t hi, const Task *__restrict__ ta
 
 // ---- engine hook -----------------------------------------------------------
 
-int model_select(gk_ctx *ctx, RowStats &st, int64_t m, int64_t ncols, int same_am) {
-    (void)ncols;
+int model_select(gk_ctx *ctx, RowStats &st, int64_t m, int same_am, const SelectParams &sp) {
     if (!ctx->model || !ctx->model->ready) {
         set_error("learned selector requested but no model is loaded (gk_model_set / gk_model_train)");
         return GK_ESTATE;
     }
     if (m == 0) return GK_OK;
     const int g = (int)std::max<int64_t>(1, std::min<int64_t>((m + 255) / 256, (int64_t)ctx->num_cus * 4));
-    k_mlp_select<<<g, 256, 0, ctx->stream>>>(m, st, same_am, ctx->model->params);
+    k_mlp_select<<<g, 256, 0, ctx->stream>>>(m, st, same_am, sp, ctx->model->params);
     return launch_check("k_mlp_select");
 }
 

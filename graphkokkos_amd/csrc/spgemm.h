@@ -65,11 +65,20 @@ struct SelectParams {
     int32_t split_flop;
     int32_t small_max_mask;
     int32_t hub;     // hub bitmaps in use (window classes read rows by the cheaper form)
+    float margin;    // learned selector: log2 cost gain needed to leave the heuristic
 };
 
 struct Task {
     int32_t row, a0, a1, split;
 };
+
+// The fixed-threshold rule: accumulator of a mask row of mlen columns.
+__device__ __forceinline__ int heuristic_accum(const SelectParams &sp, int32_t mlen) {
+    if (mlen <= sp.mca_max_mask) return GK_ACC_MCA_WAVE;
+    if (mlen <= sp.lds_max_mask) return GK_ACC_MSA_LDS;
+    if (mlen >= sp.dense_min_mask) return GK_ACC_DENSE_GLOBAL;
+    return GK_ACC_HASH_GLOBAL;
+}
 
 struct Plan {
     int32_t ntasks = 0;
@@ -166,6 +175,6 @@ template <int MODE>
 int launch_classes(gk_ctx *ctx, const Prob &P, const ClassLaunch &L, Timer *tm);
 
 // Learned selector (selector.hip): writes st.acc for every row.
-int model_select(gk_ctx *ctx, RowStats &st, int64_t m, int64_t ncols, int same_am);
+int model_select(gk_ctx *ctx, RowStats &st, int64_t m, int same_am, const SelectParams &sp);
 
 }  // namespace gk

@@ -87,13 +87,6 @@ __global__ __launch_bounds__(256) void k_row_stats(int64_t m, const int32_t *__r
     }
 }
 
-__device__ __forceinline__ int heuristic_accum(const SelectParams &sp, int32_t mlen) {
-    if (mlen <= sp.mca_max_mask) return GK_ACC_MCA_WAVE;
-    if (mlen <= sp.lds_max_mask) return GK_ACC_MSA_LDS;
-    if (mlen >= sp.dense_min_mask) return GK_ACC_DENSE_GLOBAL;
-    return GK_ACC_HASH_GLOBAL;
-}
-
 // Capacity rules shared by the host and the kernels.
 __host__ __device__ __forceinline__ int lds_fits(int32_t mlen) { return mlen <= kLdsMaxMask; }
 __host__ __device__ __forceinline__ int mca_fits(int32_t mlen) { return mlen <= kMcaMaxMask; }
@@ -1365,6 +1358,7 @@ SelectParams select_params(const gk_opts &o) {
     sp.small_max_mask = std::max(1, std::min<int32_t>(o.small_max_mask > 0 ? o.small_max_mask : kLdsSmallMask,
                                                       kLdsSmallMask));
     sp.hub = 0;
+    sp.margin = o.learned_margin / 1000.0f;
     return sp;
 }
 
@@ -1385,7 +1379,7 @@ int masked_product(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcs
     SelectParams sp = select_params(o);
     sp.hub = hub_n > 0;
     if (sp.selector == GK_SEL_LEARNED)
-        GK_TRY(model_select(ctx, st, A->m, B->n, A->rst == M->rst && A->col == M->col));
+        GK_TRY(model_select(ctx, st, A->m, A->rst == M->rst && A->col == M->col, sp));
     Plan plan;
     GK_TRY(plan_tasks(ctx, A, st, sp, &plan));
     Prob P{};
@@ -1446,7 +1440,7 @@ int prepare_product(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dc
     GK_TRY(row_stats(ctx, A, B, M, &st));
     SelectParams sp = select_params(o);
     if (sp.selector == GK_SEL_LEARNED)
-        GK_TRY(model_select(ctx, st, A->m, B->n, A->rst == M->rst && A->col == M->col));
+        GK_TRY(model_select(ctx, st, A->m, A->rst == M->rst && A->col == M->col, sp));
     Plan plan;
     GK_TRY(plan_tasks(ctx, A, st, sp, &plan));
     *P = Prob{};

@@ -132,6 +132,11 @@ SEARCH_SPACE = {
 }
 
 
+# the learned selector adds the margin by which it must predict a gain
+# before it leaves the fixed-threshold choice (milli-log2 units)
+LEARNED_SPACE = {**SEARCH_SPACE, "learned_margin": [0, 125, 250, 500, 1000]}
+
+
 def time_tc(ctx: Context, L: DeviceCsr, opts, reps: int = 3) -> float:
     ctx.tc_count(L, opts)  # warm-up
     ctx.sync()

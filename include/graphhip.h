@@ -109,6 +109,10 @@ typedef struct gk_opts {
                              * also kept as bitmaps; a row is read as bitmap words
                              * (window kernels) or single bits (MCA-wave) where that
                              * reads less than the row; 0 = off, default 16384       */
+    int32_t learned_margin; /* learned selector: it leaves the fixed-threshold choice of
+                             * a row only where the network predicts that choice to
+                             * cost more than 2^(margin/1000) times its best one;
+                             * 0 = plain argmin, default 250                         */
 } gk_opts;
 void gk_opts_default(gk_opts *o);
 

@@ -66,6 +66,11 @@ def test_samples_learned_selection_and_gain(gpu):
     print(f"R-MAT s16 triangle counting: heuristic {t_h * 1e3:.3f} ms, learned {t_l * 1e3:.3f} ms, "
           f"learned/heuristic speedup {t_h / t_l:.3f}", flush=True)
     assert t_l < 3.0 * t_h
+    # a margin no prediction can exceed keeps every fixed-threshold choice
+    assert gpu.tc_count(d, {"selector": SEL_LEARNED, "learned_margin": 10 ** 8}) == want
+    rows_l = gpu.last_timing()["rows_per_acc"]
+    gpu.tc_count(d)
+    assert rows_l == gpu.last_timing()["rows_per_acc"]
 
 
 def _heuristic_choice(X, V):
