@@ -32,6 +32,7 @@ namespace gk {
 namespace {
 
 constexpr int kPeelThreads = 256;
+constexpr int kPeelItems = 8;  // consecutive streamed items per thread and step
 
 template <int NT>
 __device__ __forceinline__ int32_t block_incl_scan(int32_t v, int32_t *lds, int32_t *total) {
@@ -134,37 +135,74 @@ __global__ __launch_bounds__(kPeelThreads) void k_peel_graphs(
             }
             __syncthreads();
             const int32_t total = carry;
-            for (int32_t base = 0; base < total; base += kPeelThreads) {
-                const int32_t it = base + tid;
-                int32_t xa = -1, xb = -1;
-                if (it < total) {
+            // Each thread takes kPeelItems consecutive items per step: one
+            // search over the frontier prefix finds the owner of its first
+            // item, a cursor walks to the next owners, and since the items
+            // of one frontier edge visit x's row in increasing column order,
+            // the search for w in y's row gallops on from the last position.
+            for (int32_t base = 0; base < total; base += kPeelThreads * kPeelItems) {
+                const int32_t it0 = base + tid * kPeelItems;
+                bool xk[2 * kPeelItems];
+                int32_t xv[2 * kPeelItems];
+#pragma unroll
+                for (int j = 0; j < 2 * kPeelItems; ++j) {
+                    xk[j] = false;
+                    xv[j] = 0;
+                }
+                if (it0 < total) {
                     int32_t lo = 0, hi = nF - 1;
-                    while (lo < hi) {  // largest f with gp[f] <= it
+                    while (lo < hi) {  // largest f with gp[f] <= it0
                         const int32_t mid = lo + ((hi - lo + 1) >> 1);
-                        if (gp[mid] <= it) lo = mid; else hi = mid - 1;
+                        if (gp[mid] <= it0) lo = mid; else hi = mid - 1;
                     }
-                    const int32_t e = F[lo], y = gy[lo], x = gx[lo];
-                    const int32_t p = arst[x] + (it - gp[lo]);
-                    const int32_t w = acol[p], b = aeid[p];
-                    const uint8_t s2 = st[b];
-                    if (!(s2 == 2 || (s2 == 1 && b < e))) {
-                        int32_t l = arst[y], h = arst[y + 1];
+                    int32_t f = lo, fend = gp[f + 1];
+                    int32_t e = F[f], y = gy[f], p0 = arst[gx[f]] - gp[f];
+                    int32_t yl = arst[y], yh = arst[y + 1];
+#pragma unroll
+                    for (int j = 0; j < kPeelItems; ++j) {
+                        const int32_t it = it0 + j;
+                        if (it >= total) break;
+                        if (it >= fend) {
+                            while (it >= fend) fend = gp[++f + 1];
+                            e = F[f];
+                            y = gy[f];
+                            p0 = arst[gx[f]] - gp[f];
+                            yl = arst[y];
+                            yh = arst[y + 1];
+                        }
+                        const int32_t p = p0 + it;
+                        const int32_t w = acol[p], b = aeid[p];
+                        const uint8_t s2 = st[b];
+                        if (s2 == 2 || (s2 == 1 && b < e)) continue;
+                        // lower bound of w in acol[yl, yh): gallop, then bisect
+                        int32_t step = 1, l = yl;
+                        while (l + step < yh && acol[l + step] < w) {
+                            l += step;
+                            step <<= 1;
+                        }
+                        int32_t h = min(l + step, yh);
                         while (l < h) {
                             const int32_t mid = l + ((h - l) >> 1);
                             if (acol[mid] < w) l = mid + 1; else h = mid;
                         }
-                        if (l < arst[y + 1] && acol[l] == w) {
+                        yl = l;
+                        if (l < yh && acol[l] == w) {
                             const int32_t a = aeid[l];
                             const uint8_t s1 = st[a];
                             if (!(s1 == 2 || (s1 == 1 && a < e))) {
-                                if (s1 == 0 && atomicSub(&sup[a], 1) == thresh) xa = a;
-                                if (s2 == 0 && atomicSub(&sup[b], 1) == thresh) xb = b;
+                                if (s1 == 0 && atomicSub(&sup[a], 1) == thresh) {
+                                    xk[2 * j] = true;
+                                    xv[2 * j] = a;
+                                }
+                                if (s2 == 0 && atomicSub(&sup[b], 1) == thresh) {
+                                    xk[2 * j + 1] = true;
+                                    xv[2 * j + 1] = b;
+                                }
                             }
                         }
                     }
                 }
-                dev::block_append<kPeelThreads>(xa >= 0, xa, Fn, &s_n, lds);
-                dev::block_append<kPeelThreads>(xb >= 0, xb, Fn, &s_n, lds);
+                dev::block_append_n<kPeelThreads, 2 * kPeelItems>(xk, xv, Fn, &s_n, lds);
             }
             for (int32_t f = tid; f < nF; f += kPeelThreads) st[F[f]] = 2;
             __syncthreads();
