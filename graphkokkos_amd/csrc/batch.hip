@@ -31,7 +31,7 @@ using namespace gk;
 namespace gk {
 namespace {
 
-constexpr int kPeelThreads = 256;
+constexpr int kPeelThreads = 1024;
 constexpr int kPeelItems = 8;  // consecutive streamed items per thread and step
 
 template <int NT>
