@@ -127,6 +127,21 @@ def run_tc(ctx, L, steps: int, warmup: int, opts=None) -> dict:
     return {"wall_s": wall, "count": count, "timing": ctx.last_timing()}
 
 
+def run_graph_build(ctx, scale: int, seed: int, reps: int = 3) -> float:
+    """Device time (ms) of building the R-MAT graph in HBM (generate,
+    symmetrise, deduplicate, degree-relabel, CSR and its strict lower part)
+    with gk_dgraph_rmat; the tests check it against the host builder."""
+    ctx.dgraph_rmat(scale, 16, seed=seed, **RMAT)  # warm-up
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        A, L = ctx.dgraph_rmat(scale, 16, seed=seed, **RMAT)
+        A.free()
+        L.free()
+    ctx.sync()
+    return (time.perf_counter() - t0) / reps * 1e3
+
+
 def run_bc(ctx, A, sources, steps: int) -> dict:
     dA = ctx.upload(A)
     out = ctx.alloc(8 * A.m)
@@ -350,7 +365,10 @@ def main() -> int:
     if "tc" in legs:
         t0 = time.perf_counter()
         G22 = HostGraph.rmat(a.tc_scale, 16, seed=a.seed, **RMAT)
-        log(f"R-MAT s{a.tc_scale}: n={G22.n} |E|={G22.edges} built in {time.perf_counter() - t0:.1f}s")
+        host_build_s = time.perf_counter() - t0
+        log(f"R-MAT s{a.tc_scale}: n={G22.n} |E|={G22.edges} built in {host_build_s:.1f}s")
+        dev_build_ms = run_graph_build(ctx, a.tc_scale, a.seed)
+        log(f"R-MAT s{a.tc_scale} built on the device in {dev_build_ms:.1f} ms")
         plain = run_tc(ctx, G22.L, a.steps, a.warmup, PLAIN)
         pd = dominant(plain["timing"])
         tc = {"workload": f"triangle counting C<L>=L*L, R-MAT scale {a.tc_scale} ef16", "edges": G22.edges,
@@ -365,6 +383,10 @@ def main() -> int:
         log(f"tc plain: {plain['wall_s'] * 1e3:.2f} ms, T={plain['count']}, dominant {pd['kernel']} "
             f"{pd['achieved_gbs']:.0f} GB/s")
         tc["plain"]["rows_per_acc"] = plain["timing"]["rows_per_acc"]
+        tc["graph_build"] = {"what": f"R-MAT scale {a.tc_scale} ef16 generated, symmetrised, deduplicated, "
+                                     "degree-relabelled, CSR + strict lower part",
+                             "device_ms": dev_build_ms, "host_s": host_build_s,
+                             "host_cpus": os.cpu_count()}
     if tc is not None and a.tc_variants == "default":
         r = run_tc(ctx, G22.L, a.steps, a.warmup, {})
         assert r["count"] == plain["count"], "the default options changed the triangle count"

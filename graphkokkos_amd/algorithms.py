@@ -41,6 +41,22 @@ class DeviceCsr:
         check(lib().gk_dcsr_upload(ctx.handle, C.byref(csr), C.byref(h)), "gk_dcsr_upload")
         self.handle = h
 
+    @classmethod
+    def adopt(cls, ctx: "Context", handle: C.c_void_p) -> "DeviceCsr":
+        """Wrap a gk_dcsr the library built (freed with this object)."""
+        obj = cls.__new__(cls)
+        obj.ctx, obj.handle = ctx, handle
+        m, n, nz = C.c_int64(), C.c_int64(), C.c_int64()
+        check(lib().gk_dcsr_dims(handle, C.byref(m), C.byref(n), C.byref(nz)), "gk_dcsr_dims")
+        obj.m, obj.n, obj.nz = int(m.value), int(n.value), int(nz.value)
+        return obj
+
+    def to_host(self) -> SpMtx:
+        rst = np.zeros(self.m + 1, np.int32)
+        col = np.zeros(max(self.nz, 1), np.int32)
+        check(lib().gk_dcsr_download(self.ctx.handle, self.handle, _ptr(rst), _ptr(col)), "gk_dcsr_download")
+        return SpMtx(self.m, self.n, rst, col[:self.nz].copy())
+
     def free(self):
         if self.handle:
             lib().gk_dcsr_free(self.handle)
@@ -106,6 +122,44 @@ class Context:
 
     def alloc(self, nbytes: int) -> DeviceBuffer:
         return DeviceBuffer(self, nbytes)
+
+    # ---- on-device graph construction (the HostGraph generators on the GPU) ----
+    def _dgraph(self, fn: str, *args, relabel: bool = True, perm: bool = False):
+        A, L = C.c_void_p(), C.c_void_p()
+        buf = None
+        if perm:
+            buf = self.alloc(4 * max(int(args[-1]), 1))
+            args = args[:-1]
+        check(getattr(lib(), fn)(self.handle, *args, int(relabel), C.byref(A), C.byref(L),
+                                 buf.ptr if buf is not None else None), fn)
+        out = (DeviceCsr.adopt(self, A), DeviceCsr.adopt(self, L))
+        if perm:
+            p = buf.to_host(np.int32, out[0].m)
+            buf.free()
+            return out + (p,)
+        return out
+
+    def dgraph_rmat(self, scale: int, edge_factor: int = 16, a: float = 0.57, b: float = 0.19, c: float = 0.19,
+                    seed: int = 1, relabel: bool = True, perm: bool = False):
+        """R-MAT built in HBM: (A, L) as DeviceCsr (and the relabelling with perm=True)."""
+        extra = (1 << scale,) if perm else ()
+        return self._dgraph("gk_dgraph_rmat", scale, edge_factor, a, b, c, seed, *extra, relabel=relabel, perm=perm)
+
+    def dgraph_er(self, n: int, mean_degree: float, seed: int = 1, relabel: bool = True, perm: bool = False):
+        extra = (n,) if perm else ()
+        return self._dgraph("gk_dgraph_er", n, mean_degree, seed, *extra, relabel=relabel, perm=perm)
+
+    def dgraph_chung_lu(self, n: int, exponent: float, mean_degree: float = 16.0, seed: int = 1,
+                        relabel: bool = True, perm: bool = False):
+        extra = (n,) if perm else ()
+        return self._dgraph("gk_dgraph_chung_lu", n, exponent, mean_degree, seed, *extra, relabel=relabel,
+                            perm=perm)
+
+    def dgraph_grid(self, rows: int, cols: int | None = None, diagonal: bool = True, relabel: bool = True,
+                    perm: bool = False):
+        cols = rows if cols is None else cols
+        extra = (rows * cols,) if perm else ()
+        return self._dgraph("gk_dgraph_grid", rows, cols, int(diagonal), *extra, relabel=relabel, perm=perm)
 
     def last_timing(self) -> dict:
         t = GkTiming()

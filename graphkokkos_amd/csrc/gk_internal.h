@@ -22,6 +22,11 @@ void clear_error();
 // Number of host threads used by construction loops.
 int host_threads();
 
+// Chung-Lu inverse-CDF table: vertex i has weight (i+1)^(-1/(exponent-1));
+// cdf[i] = normalised prefix sum, cdf[n-1] = 1.  An endpoint drawn from u in
+// [0,1) is the first i with cdf[i] > u.
+void chung_lu_cdf(int64_t n, double exponent, std::vector<double> &cdf);
+
 // Static block partition of [0, n) over host threads; f(begin, end, tid).
 template <class F>
 void par_blocks(int64_t n, F &&f) {

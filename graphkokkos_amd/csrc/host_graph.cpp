@@ -37,6 +37,15 @@ void clear_error() { g_err.clear(); }
 // Set on the workers of a batch build: each builds whole graphs on its own.
 static thread_local bool t_serial = false;
 
+void chung_lu_cdf(int64_t n, double exponent, std::vector<double> &cdf) {
+    cdf.resize(n);
+    const double p = -1.0 / (exponent - 1.0);
+    double acc = 0;
+    for (int64_t i = 0; i < n; ++i) { acc += std::pow(double(i + 1), p); cdf[i] = acc; }
+    for (auto &x : cdf) x /= acc;
+    cdf[n - 1] = 1.0;
+}
+
 int host_threads() {
     if (t_serial) return 1;
     if (g_host_threads > 0) return g_host_threads;
@@ -217,12 +226,8 @@ extern "C" int gk_hgraph_chung_lu(int64_t n, double exponent, double mean_degree
     GK_CHECK_ARG(n >= 2 && exponent > 1.0 && mean_degree > 0, "Chung-Lu needs n >= 2, exponent > 1, degree > 0");
     // Weight of vertex i is (i+1)^(-1/(exponent-1)); both endpoints of each of
     // the m = n*d/2 edges are drawn proportionally to weight (inverse CDF).
-    std::vector<double> cdf(n);
-    const double p = -1.0 / (exponent - 1.0);
-    double acc = 0;
-    for (int64_t i = 0; i < n; ++i) { acc += std::pow(double(i + 1), p); cdf[i] = acc; }
-    for (auto &x : cdf) x /= acc;
-    cdf[n - 1] = 1.0;
+    std::vector<double> cdf;
+    chung_lu_cdf(n, exponent, cdf);
     const int64_t ne = (int64_t)std::llround(n * mean_degree / 2.0);
     auto draw = [&](uint64_t r) {
         double x = u01(r);

@@ -186,6 +186,28 @@ typedef struct gk_dcsr gk_dcsr;
 int  gk_dcsr_upload(gk_ctx *ctx, const gk_csr *h, gk_dcsr **out);
 void gk_dcsr_free(gk_dcsr *d);
 int64_t gk_dcsr_nnz(const gk_dcsr *d);
+int  gk_dcsr_dims(const gk_dcsr *d, int64_t *m, int64_t *n, int64_t *nz);
+/* Copies a device CSR's row pointers (m+1) and columns (nz) to host arrays. */
+int  gk_dcsr_download(gk_ctx *ctx, const gk_dcsr *d, int32_t *rst, int32_t *col);
+
+/* ---- on-device graph construction ----------------------------------------
+ * The gk_hgraph_* generators and cleaning pipeline on the GPU: the same
+ * graphs bit for bit (counter-based generators, sort-based cleaning), left
+ * in HBM as *A (symmetric, sorted rows, no self loops or duplicates) and
+ * *L (strict lower part), both freed with gk_dcsr_free.  d_perm (device, n
+ * entries, or NULL) receives the new id of every generated vertex.  This is
+ * the step before the path: the reference builds its CSR on the host
+ * (build_adjacency_matrix / extract_lower_triangle, include/matrix.h:643-726). */
+int gk_dgraph_rmat(gk_ctx *ctx, int scale, int edge_factor, double a, double b,
+                   double c, uint64_t seed, int relabel, gk_dcsr **A,
+                   gk_dcsr **L, int32_t *d_perm);
+int gk_dgraph_er(gk_ctx *ctx, int64_t n, double mean_degree, uint64_t seed,
+                 int relabel, gk_dcsr **A, gk_dcsr **L, int32_t *d_perm);
+int gk_dgraph_chung_lu(gk_ctx *ctx, int64_t n, double exponent,
+                       double mean_degree, uint64_t seed, int relabel,
+                       gk_dcsr **A, gk_dcsr **L, int32_t *d_perm);
+int gk_dgraph_grid(gk_ctx *ctx, int64_t rows, int64_t cols, int diagonal,
+                   int relabel, gk_dcsr **A, gk_dcsr **L, int32_t *d_perm);
 
 /* ---- masked sparse-sparse product C<M> = A*B ----------------------------
  * Output pattern := pattern of M (include/matrix_la.h:564-568); c_val[p]
