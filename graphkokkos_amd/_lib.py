@@ -83,6 +83,8 @@ _SIGS = {
     "gk_dcsr_upload": (C.c_int, [C.c_void_p, C.POINTER(GkCsr), C.POINTER(C.c_void_p)]),
     "gk_dcsr_free": (None, [C.c_void_p]),
     "gk_dcsr_nnz": (C.c_int64, [C.c_void_p]),
+    "gk_matrix_read": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p)]),
+    "gk_matrix_write": (C.c_int, [C.c_char_p, C.c_char_p, C.c_void_p, C.POINTER(C.c_int32), C.c_char_p]),
     "gk_dcsr_dims": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "gk_dcsr_download": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "gk_dgraph_rmat": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_uint64,

@@ -472,3 +472,212 @@ extern "C" int gk_mtx_view(const gk_mtx *mt, gk_csr *pattern, const double **val
 }
 
 extern "C" void gk_mtx_free(gk_mtx *mt) { delete mt; }
+
+// ---- the other on-disk formats --------------------------------------------
+// Readers with the layouts of include/matrix.h:396-533 and writers after
+// :209-257, into / from the same gk_mtx as the Matrix Market reader.
+//   crs    text: "m nz matcode", then nz columns, nz values, m+1 row pointers
+//   bin    4 matcode bytes, m n nz as 64-bit sizes, int32 row pointers [m+1],
+//          int32 columns [nz], int32 values [nz] (the reference's writer puts
+//          the row pointers in all three arrays, :220-222; here each array
+//          gets its own data)
+//   graph  text: "m nz", then one line of 0-based columns per row
+//   rmat   text: "m n nz", then nz pairs "x y" (0-based); both directions
+//          are stored, rows sorted and deduplicated (the reference keeps
+//          insertion order and repeats, :488-533)
+namespace {
+
+bool csr_consistent(const gk_mtx &mt, const char *path) {
+    if (mt.rst.empty() || mt.rst[0] != 0 || mt.rst[mt.m] != mt.nz) {
+        set_error("%s: row pointers must run from 0 to nz", path);
+        return false;
+    }
+    for (int64_t i = 0; i < mt.m; ++i)
+        if (mt.rst[i + 1] < mt.rst[i]) { set_error("%s: row pointers decrease at row %lld", path, (long long)i); return false; }
+    for (int64_t p = 0; p < mt.nz; ++p)
+        if (mt.col[p] < 0 || mt.col[p] >= mt.n) { set_error("%s: column %lld out of range", path, (long long)p); return false; }
+    return true;
+}
+
+int read_crs(const char *path, gk_mtx &mt) {
+    std::ifstream in(path);
+    if (!in.is_open()) { set_error("cannot open %s", path); return GK_EIO; }
+    long long m = 0, nz = 0;
+    std::string code;
+    if (!(in >> m >> nz >> code)) { set_error("%s: bad crs header", path); return GK_EIO; }
+    if (m < 0 || m > INT32_MAX - 1 || nz < 0 || nz > INT32_MAX) { set_error("%s: sizes out of range", path); return GK_ERANGE; }
+    // The reference leaves n unset for this format; a square matrix is assumed.
+    mt.m = mt.n = m;
+    mt.nz = nz;
+    std::memcpy(mt.matcode, "MCIG", 4);
+    for (size_t x = 0; x < 4 && x < code.size(); ++x) mt.matcode[x] = code[x];
+    mt.col.resize(nz);
+    mt.val.resize(nz);
+    mt.rst.resize(m + 1);
+    for (long long p = 0; p < nz; ++p)
+        if (!(in >> mt.col[p])) { set_error("%s: column %lld unreadable", path, p); return GK_EIO; }
+    for (long long p = 0; p < nz; ++p)
+        if (!(in >> mt.val[p])) { set_error("%s: value %lld unreadable", path, p); return GK_EIO; }
+    for (long long i = 0; i <= m; ++i)
+        if (!(in >> mt.rst[i])) { set_error("%s: row pointer %lld unreadable", path, i); return GK_EIO; }
+    return csr_consistent(mt, path) ? GK_OK : GK_EIO;
+}
+
+int read_bin(const char *path, gk_mtx &mt) {
+    std::unique_ptr<FILE, int (*)(FILE *)> fp(std::fopen(path, "rb"), std::fclose);
+    if (!fp) { set_error("cannot open %s", path); return GK_EIO; }
+    uint64_t mnz[3];
+    if (std::fread(mt.matcode, 1, 4, fp.get()) != 4 || std::fread(mnz, sizeof(uint64_t), 3, fp.get()) != 3) {
+        set_error("%s: short bin header", path);
+        return GK_EIO;
+    }
+    if (mnz[0] > INT32_MAX - 1 || mnz[1] > INT32_MAX - 1 || mnz[2] > INT32_MAX) {
+        set_error("%s: sizes out of range", path);
+        return GK_ERANGE;
+    }
+    mt.m = (int64_t)mnz[0];
+    mt.n = (int64_t)mnz[1];
+    mt.nz = (int64_t)mnz[2];
+    mt.rst.resize(mt.m + 1);
+    mt.col.resize(mt.nz);
+    std::vector<int32_t> v(mt.nz);
+    if (std::fread(mt.rst.data(), 4, mt.m + 1, fp.get()) != (size_t)(mt.m + 1) ||
+        std::fread(mt.col.data(), 4, mt.nz, fp.get()) != (size_t)mt.nz ||
+        std::fread(v.data(), 4, mt.nz, fp.get()) != (size_t)mt.nz) {
+        set_error("%s: short bin body", path);
+        return GK_EIO;
+    }
+    mt.val.assign(v.begin(), v.end());
+    return csr_consistent(mt, path) ? GK_OK : GK_EIO;
+}
+
+int read_graph(const char *path, gk_mtx &mt) {
+    std::ifstream in(path);
+    if (!in.is_open()) { set_error("cannot open %s", path); return GK_EIO; }
+    long long m = 0, nz = 0;
+    std::string line;
+    if (!std::getline(in, line)) { set_error("%s: empty file", path); return GK_EIO; }
+    {
+        std::istringstream hs(line);
+        if (!(hs >> m >> nz)) { set_error("%s: bad graph header", path); return GK_EIO; }
+    }
+    if (m < 0 || m > INT32_MAX - 1 || nz < 0 || nz > INT32_MAX) { set_error("%s: sizes out of range", path); return GK_ERANGE; }
+    mt.m = mt.n = m;
+    std::memcpy(mt.matcode, "MCIG", 4);
+    mt.rst.assign(m + 1, 0);
+    mt.col.clear();
+    mt.col.reserve(nz);
+    for (long long i = 0; i < m; ++i) {
+        if (!std::getline(in, line)) line.clear();  // missing trailing rows are empty
+        std::istringstream ls(line);
+        long long c;
+        while (ls >> c) {
+            if (c < 0 || c >= m) { set_error("%s: row %lld: column %lld out of range", path, i, c); return GK_EIO; }
+            mt.col.push_back((int32_t)c);
+        }
+        if ((long long)mt.col.size() > nz) { set_error("%s: more than nz = %lld columns", path, nz); return GK_EIO; }
+        mt.rst[i + 1] = (int32_t)mt.col.size();
+    }
+    if ((long long)mt.col.size() != nz) { set_error("%s: %zu columns for nz = %lld", path, mt.col.size(), nz); return GK_EIO; }
+    mt.nz = nz;
+    mt.val.assign(nz, 1.0);
+    return GK_OK;
+}
+
+int read_rmat(const char *path, gk_mtx &mt) {
+    std::ifstream in(path);
+    if (!in.is_open()) { set_error("cannot open %s", path); return GK_EIO; }
+    long long m = 0, n = 0, ne = 0;
+    if (!(in >> m >> n >> ne)) { set_error("%s: bad rmat header", path); return GK_EIO; }
+    if (m < 0 || m > INT32_MAX - 1 || n != m || ne < 0 || 2 * ne > INT32_MAX) {
+        set_error("%s: sizes out of range (a square matrix is required)", path);
+        return GK_ERANGE;
+    }
+    std::vector<std::vector<int32_t>> rows(m);
+    for (long long e = 0; e < ne; ++e) {
+        long long x, y;
+        if (!(in >> x >> y)) { set_error("%s: pair %lld unreadable", path, e); return GK_EIO; }
+        if (x < 0 || x >= m || y < 0 || y >= m) { set_error("%s: pair %lld out of range", path, e); return GK_EIO; }
+        rows[x].push_back((int32_t)y);
+        rows[y].push_back((int32_t)x);
+    }
+    mt.m = mt.n = m;
+    std::memcpy(mt.matcode, "MCIG", 4);
+    mt.rst.assign(m + 1, 0);
+    mt.col.clear();
+    for (long long i = 0; i < m; ++i) {
+        auto &r = rows[i];
+        std::sort(r.begin(), r.end());
+        r.erase(std::unique(r.begin(), r.end()), r.end());
+        mt.col.insert(mt.col.end(), r.begin(), r.end());
+        mt.rst[i + 1] = (int32_t)mt.col.size();
+    }
+    mt.nz = (int64_t)mt.col.size();
+    mt.val.assign(mt.nz, 1.0);
+    return GK_OK;
+}
+
+}  // namespace
+
+extern "C" int gk_matrix_read(const char *path, const char *format, gk_mtx **out) {
+    GK_CHECK_ARG(path && format && out, "null argument");
+    const std::string f = format;
+    if (f == "mtx") return gk_mtx_read(path, out);
+    auto mt = std::make_unique<gk_mtx>();
+    int rc;
+    if (f == "crs") rc = read_crs(path, *mt);
+    else if (f == "bin") rc = read_bin(path, *mt);
+    else if (f == "graph") rc = read_graph(path, *mt);
+    else if (f == "rmat") rc = read_rmat(path, *mt);
+    else { set_error("%s: unknown format %s", path, format); return GK_EINVAL; }
+    if (rc != GK_OK) return rc;
+    *out = mt.release();
+    return GK_OK;
+}
+
+extern "C" int gk_matrix_write(const char *path, const char *format, const gk_csr *A, const int32_t *val,
+                               const char matcode[4]) {
+    GK_CHECK_ARG(path && format && A && A->rst && (A->col || A->nz == 0), "null argument");
+    const std::string f = format;
+    char code[4];
+    if (matcode) std::memcpy(code, matcode, 4);
+    else std::memcpy(code, val ? "MCIG" : "MCPG", 4);
+    if (f == "bin") {
+        std::unique_ptr<FILE, int (*)(FILE *)> fp(std::fopen(path, "wb"), std::fclose);
+        if (!fp) { set_error("cannot create %s", path); return GK_EIO; }
+        const uint64_t mnz[3] = {(uint64_t)A->m, (uint64_t)A->n, (uint64_t)A->nz};
+        std::vector<int32_t> ones;
+        if (!val) ones.assign(A->nz, 1);
+        const int32_t *v = val ? val : ones.data();
+        bool ok = std::fwrite(code, 1, 4, fp.get()) == 4 && std::fwrite(mnz, sizeof(uint64_t), 3, fp.get()) == 3 &&
+                  std::fwrite(A->rst, 4, A->m + 1, fp.get()) == (size_t)(A->m + 1) &&
+                  std::fwrite(A->col, 4, A->nz, fp.get()) == (size_t)A->nz &&
+                  std::fwrite(v, 4, A->nz, fp.get()) == (size_t)A->nz;
+        if (!ok) { set_error("%s: write failed", path); return GK_EIO; }
+        return GK_OK;
+    }
+    std::ofstream o(path);
+    if (!o.is_open()) { set_error("cannot create %s", path); return GK_EIO; }
+    if (f == "mtx") {
+        // general coordinate; integer values, or pattern without values
+        // (write_crs_to_mtx, include/matrix.h:228-257)
+        o << "%%MatrixMarket matrix coordinate " << (val ? "integer" : "pattern") << " general\n";
+        o << A->m << ' ' << A->n << ' ' << A->nz << '\n';
+        for (int64_t i = 0; i < A->m; ++i)
+            for (int32_t p = A->rst[i]; p < A->rst[i + 1]; ++p) {
+                o << i + 1 << ' ' << A->col[p] + 1;
+                if (val) o << ' ' << val[p];
+                o << '\n';
+            }
+    } else if (f == "crs") {
+        o << A->m << ' ' << A->nz << ' ' << std::string(code, 4) << '\n';
+        for (int64_t p = 0; p < A->nz; ++p) o << A->col[p] << (p + 1 < A->nz ? ' ' : '\n');
+        for (int64_t p = 0; p < A->nz; ++p) o << (val ? val[p] : 1) << (p + 1 < A->nz ? ' ' : '\n');
+        for (int64_t i = 0; i <= A->m; ++i) o << A->rst[i] << (i < A->m ? ' ' : '\n');
+    } else {
+        set_error("%s: the writers are bin, mtx and crs (not %s)", path, format);
+        return GK_EINVAL;
+    }
+    if (!o) { set_error("%s: write failed", path); return GK_EIO; }
+    return GK_OK;
+}

@@ -84,12 +84,11 @@ class SpMtx:
 
     @classmethod
     def from_file(cls, path: str, fmt: str = "mtx") -> "SpMtx":
-        """spMtx(filename, format) for the Matrix Market format
-        (include/matrix.h:44-61, read_mtx_to_crs :259-395)."""
-        if fmt != "mtx":
-            raise ValueError(f"format {fmt!r} is not supported (only 'mtx')")
+        """spMtx(filename, format) (include/matrix.h:44-61): "mtx"
+        (read_mtx_to_crs :259-395), "crs", "bin", "graph", "rmat"
+        (:396-533; deviations in include/graphhip.h)."""
         h = C.c_void_p()
-        check(lib().gk_mtx_read(str(path).encode(), C.byref(h)), "gk_mtx_read")
+        check(lib().gk_matrix_read(str(path).encode(), fmt.encode(), C.byref(h)), "gk_matrix_read")
         try:
             csr = GkCsr()
             val = _lib.f64p()
@@ -105,6 +104,22 @@ class SpMtx:
             return cls(int(m), int(csr.n), rst, col, v, matcode)
         finally:
             lib().gk_mtx_free(h)
+
+
+def write_matrix(a: SpMtx, path: str, fmt: str = "bin", values=None) -> None:
+    """write_crs_to_bin / write_crs_to_mtx (include/matrix.h:209-257) and
+    the crs text form.  ``values`` (int32 per entry, e.g. supports or truss
+    numbers) default to the matrix's own int32 values; a pattern matrix is
+    written as a pattern (bin / crs: values 1)."""
+    csr = a.gk(with_values=False)
+    v = values if values is not None else a.values_i32()
+    vp = None
+    if v is not None:
+        v = np.ascontiguousarray(v, np.int32)
+        if len(v) != a.nz:
+            raise ValueError(f"{len(v)} values for {a.nz} entries")
+        vp = v.ctypes.data_as(C.POINTER(C.c_int32))
+    check(lib().gk_matrix_write(str(path).encode(), fmt.encode(), C.byref(csr), vp, None), "gk_matrix_write")
 
 
 def build_adjacency_matrix(g: SpMtx) -> SpMtx:

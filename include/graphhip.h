@@ -181,6 +181,26 @@ int  gk_mtx_view(const gk_mtx *mt, gk_csr *pattern, const double **val,
                  char matcode[4]);
 void gk_mtx_free(gk_mtx *mt);
 
+/* The reference's other formats (spMtx(filename, format),
+ * include/matrix.h:44-61; readers :396-533, writers :209-257), into the same
+ * gk_mtx: "mtx" (as gk_mtx_read), "crs" (text: m nz matcode, columns,
+ * values, row pointers), "bin" (4 matcode bytes, m n nz as 64-bit sizes,
+ * int32 row pointers, columns, values), "graph" (text: m nz, then one line
+ * of 0-based columns per row), "rmat" (text: m n nz, then 0-based pairs;
+ * both directions stored, rows sorted and deduplicated).  Deviations from
+ * the reference: crs sets n = m (the reference leaves it 0), graph reads
+ * row i from line i after the header (the reference's first getline takes
+ * the rest of the header line as row 0), rmat rows come out sorted and
+ * deduplicated.                                                             */
+int gk_matrix_read(const char *path, const char *format, gk_mtx **out);
+/* Writers "bin" (each array with its own data: the reference's writer puts
+ * the row pointers in all three, include/matrix.h:220-222), "mtx" (general
+ * coordinate, integer values or pattern when val is NULL) and "crs".  val
+ * (NULL: pattern, written as ones in bin/crs) holds int32 values per entry,
+ * e.g. edge supports or truss numbers; matcode NULL picks MCIG / MCPG.     */
+int gk_matrix_write(const char *path, const char *format, const gk_csr *A,
+                    const int32_t *val, const char matcode[4]);
+
 /* ---- device-resident CSR ----------------------------------------------- */
 typedef struct gk_dcsr gk_dcsr;
 int  gk_dcsr_upload(gk_ctx *ctx, const gk_csr *h, gk_dcsr **out);

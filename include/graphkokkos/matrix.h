@@ -77,19 +77,24 @@ public:
     gk::View<int> Col;
     gk::View<ValT> Val;
 
-    // include/matrix.h:44-61.  "mtx" is read by libgraphhip's Matrix Market
-    // reader (1-based coordinates, self loops dropped, symmetric entries
-    // mirrored, rows in file order, duplicates kept).
+    // include/matrix.h:44-61, read by libgraphhip (gk_matrix_read): "mtx"
+    // (1-based coordinates, self loops dropped, symmetric entries mirrored,
+    // rows in file order, duplicates kept), "crs", "bin", "graph", "rmat"
+    // (layouts and deviations in graphhip.h).  The reference's messages and
+    // string throws are kept.
     spMtx(const char *filename, const std::string &format) {
-        if (format == "mtx") {
-            if (read_mtx_to_crs(filename)) {
-                std::cout << "Can't read MTX from file\n";
-                throw "Can't read MTX from file";
+        static const char *const names[][2] = {{"mtx", "MTX"}, {"crs", "CRS"}, {"bin", "BIN"},
+                                               {"graph", "GRAPH"}, {"rmat", "RMAT"}};
+        for (const auto &f : names)
+            if (format == f[0]) {
+                if (read_file(filename, f[0])) {
+                    static thread_local std::string msg;
+                    msg = std::string("Can't read ") + f[1] + " from file";
+                    std::cout << msg << "\n";
+                    throw msg.c_str();
+                }
+                return;
             }
-        } else {
-            std::cout << "Can't read " << format << " from file\n";
-            throw "Unsupported matrix file format (graphkokkos_amd reads mtx)";
-        }
     }
 
     spMtx() {}
@@ -173,20 +178,10 @@ public:
     }
 
     // general coordinate Matrix Market file (include/matrix.h:225-252)
-    int write_crs_to_mtx(const char *filename) const {
-        std::ofstream f(filename);
-        if (!f) return -1;
-        const bool pattern = matcode[2] == 'P';
-        f << "%%MatrixMarket matrix coordinate " << (pattern ? "pattern" : "integer") << " general\n";
-        f << m << ' ' << n << ' ' << nz << '\n';
-        for (size_t i = 0; i < m; ++i)
-            for (int j = Rst(i); j < Rst(i + 1); ++j) {
-                f << i + 1 << ' ' << Col(j) + 1;
-                if (!pattern) f << ' ' << Val(j);
-                f << '\n';
-            }
-        return f ? 0 : -1;
-    }
+    int write_crs_to_mtx(const char *filename) const { return write_file(filename, "mtx", matcode[2] != 'P'); }
+
+    // binary CRS (include/matrix.h:209-223), each array with its own data
+    int write_crs_to_bin(const char *filename) const { return write_file(filename, "bin", true); }
 
     // Borrowed C-ABI view of the arrays; values only when ValT is int32 and
     // with_values is set (otherwise a pattern: every value 1).
@@ -202,9 +197,19 @@ public:
     }
 
 private:
-    int read_mtx_to_crs(const char *filename) {
+    int write_file(const char *filename, const char *format, bool with_values) const {
+        std::vector<int32_t> v;
+        if (with_values) {
+            v.resize(nz);
+            for (size_t j = 0; j < nz; ++j) v[j] = static_cast<int32_t>(Val(j));
+        }
+        const gk_csr c = csr(false);
+        return gk_matrix_write(filename, format, &c, with_values ? v.data() : nullptr, matcode) == GK_OK ? 0 : -1;
+    }
+
+    int read_file(const char *filename, const char *format) {
         gk_mtx *mt = nullptr;
-        if (gk_mtx_read(filename, &mt) != GK_OK) return -1;
+        if (gk_matrix_read(filename, format, &mt) != GK_OK) return -1;
         gk_csr p;
         const double *v = nullptr;
         char code[4];
