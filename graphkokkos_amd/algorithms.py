@@ -51,11 +51,18 @@ class DeviceCsr:
         obj.m, obj.n, obj.nz = int(m.value), int(n.value), int(nz.value)
         return obj
 
-    def to_host(self) -> SpMtx:
+    def to_host(self, values: bool = False) -> SpMtx:
+        """Rows and columns; the values too when `values` (a library-built
+        CSR that holds them, such as a complement-mask product)."""
         rst = np.zeros(self.m + 1, np.int32)
         col = np.zeros(max(self.nz, 1), np.int32)
         check(lib().gk_dcsr_download(self.ctx.handle, self.handle, _ptr(rst), _ptr(col)), "gk_dcsr_download")
-        return SpMtx(self.m, self.n, rst, col[:self.nz].copy())
+        val = None
+        if values:
+            val = np.zeros(max(self.nz, 1), np.int32)
+            check(lib().gk_dcsr_download_val(self.ctx.handle, self.handle, _ptr(val)), "gk_dcsr_download_val")
+            val = val[:self.nz].copy()
+        return SpMtx(self.m, self.n, rst, col[:self.nz].copy(), val)
 
     def free(self):
         if self.handle:
@@ -191,6 +198,29 @@ class Context:
         o = self._opts(opts)
         check(lib().gk_masked_spgemm_dev(self.handle, A.handle, B.handle, M.handle, out.ptr, C.byref(o)),
               "gk_masked_spgemm_dev")
+
+    def masked_spgemm_cmask_dev(self, A: DeviceCsr, B: DeviceCsr, M: DeviceCsr) -> DeviceCsr:
+        """C<not M> = A*B on device handles (M with sorted rows); the result
+        is a library-built CSR with values."""
+        h = C.c_void_p()
+        check(lib().gk_masked_spgemm_cmask_dev(self.handle, A.handle, B.handle, M.handle, C.byref(h)),
+              "gk_masked_spgemm_cmask_dev")
+        return DeviceCsr.adopt(self, h)
+
+    def masked_spgemm_cmask(self, A: SpMtx, B: SpMtx, M: SpMtx) -> SpMtx:
+        """C<not M> = A*B: the entries some product term reaches outside M's
+        pattern, with their int32 sums (explicit zeros kept)."""
+        Ms, _ = _canonical_mask(M)
+        dA, dB, dM = self.upload(A), self.upload(B), self.upload(Ms)
+        try:
+            dC = self.masked_spgemm_cmask_dev(dA, dB, dM)
+            try:
+                return dC.to_host(values=True)
+            finally:
+                dC.free()
+        finally:
+            for d in (dA, dB, dM):
+                d.free()
 
     def tc_count(self, L: SpMtx | DeviceCsr, opts=None) -> int:
         o = self._opts(opts)
@@ -337,6 +367,18 @@ def mxmm_msa(isParallel: bool, A: SpMtx, B: SpMtx, M: SpMtx, C_: SpMtx) -> None:
 def mxmm_mca(isParallel: bool, A: SpMtx, B: SpMtx, M: SpMtx, C_: SpMtx) -> None:
     """C<M> = A*B (include/matrix_la.h:478-491)."""
     _product_into(A, B, M, C_)
+
+
+def mxmm_msa_cmask(isParallel: bool, A: SpMtx, B: SpMtx, M: SpMtx, C_: SpMtx) -> None:
+    """C<not M> = A*B (include/matrix_la.h:733-934): C gets M's shape and
+    the entries some term A(i,k)*B(k,j) reaches outside M's pattern, rows
+    in column order, explicit zeros kept.  Repeated mask columns count once
+    (the reference's symbolic pass would subtract them twice)."""
+    if A.n != B.m:
+        raise ValueError("A.n must equal B.m")
+    R = default_context().masked_spgemm_cmask(A, B, M)
+    C_.m, C_.n = R.m, R.n
+    C_.Rst, C_.Col, C_.Val = R.Rst, R.Col, R.Val
 
 
 MxmOp = Callable[[bool, SpMtx, SpMtx, SpMtx, SpMtx], None]

@@ -240,6 +240,17 @@ int gk_masked_spgemm_i32(gk_ctx *ctx, const gk_csr *A, const gk_csr *B,
 int gk_masked_spgemm_dev(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B,
                          const gk_dcsr *M, int32_t *d_cval, const gk_opts *o);
 
+/* ---- complement-mask product C<not M> = A*B ------------------------------
+ * mxmm_msa_cmask (include/matrix_la.h:733-934): C holds every (i,j) that a
+ * product term A(i,k)*B(k,j) reaches and M does not hold (C.m = M.m, C.n =
+ * M.n), with the int32 (wrap-around) sum of the terms, explicit zeros kept,
+ * rows in column order.  The pattern is not known beforehand, so *C is a
+ * library-built device CSR with values (gk_dcsr_dims, gk_dcsr_download,
+ * gk_dcsr_download_val; freed with gk_dcsr_free).  M needs sorted rows.   */
+int gk_masked_spgemm_cmask_dev(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B,
+                               const gk_dcsr *M, gk_dcsr **C);
+int gk_dcsr_download_val(gk_ctx *ctx, const gk_dcsr *d, int32_t *val);
+
 /* ---- triangle counting C<L> = L*L, sum of C -----------------------------
  * src/main.cpp:69-90; L strictly lower, sorted columns.                    */
 int gk_tc_count(gk_ctx *ctx, const gk_csr *L, uint64_t *count, const gk_opts *o);

@@ -3,7 +3,8 @@
 // signatures (include/matrix_la.h:80-123) and run C<M> = A*B on the GPU
 // through gk_masked_spgemm_i32.  mxmm_gpu is the same operator under a name
 // of its own; every one of them converts to the reference's product
-// pointer type mxmOp<int> (src/main.cpp:13-14).
+// pointer type mxmOp<int> (src/main.cpp:13-14).  mxmm_msa_cmask, the
+// complement-mask product, runs through gk_masked_spgemm_cmask_dev.
 //
 // Semantics (include/matrix_la.h:478-491, 562-574): C takes M's shape and
 // pattern (Rst and Col copied from M), C.Val[p] = sum_k A(i,k)*B(k,j) for
@@ -81,6 +82,36 @@ void masked_product(const spMtx<T> &A, const spMtx<T> &B, const spMtx<U> &M, spM
     C = std::move(R);
 }
 
+// C<not M> = A*B: the output pattern comes from the device, so A, B and the
+// canonical mask go up as device CSRs and the library-built result comes
+// back whole (rows, columns, values).
+struct DeviceCsr {
+    gk_dcsr *d = nullptr;
+    ~DeviceCsr() {
+        if (d) gk_dcsr_free(d);
+    }
+};
+
+template <typename T, typename U>
+void complement_product(const spMtx<T> &A, const spMtx<T> &B, const spMtx<U> &M, spMtx<T> &C) {
+    static_assert(std::is_same<T, int>::value, "the GPU masked product computes int32 values (spMtx<int>)");
+    if (A.n != B.m) throw error(GK_EINVAL, "complement-mask product: A.n != B.m");
+    const gk_csr a = A.csr(), b = B.csr();
+    const CanonicalMask cm = canonical_mask(M);
+    DeviceCsr da, db, dm, dc;
+    check(gk_dcsr_upload(context(), &a, &da.d), "gk_dcsr_upload");
+    check(gk_dcsr_upload(context(), &b, &db.d), "gk_dcsr_upload");
+    check(gk_dcsr_upload(context(), &cm.csr, &dm.d), "gk_dcsr_upload");
+    check(gk_masked_spgemm_cmask_dev(context(), da.d, db.d, dm.d, &dc.d), "gk_masked_spgemm_cmask_dev");
+    int64_t m = 0, n = 0, nz = 0;
+    check(gk_dcsr_dims(dc.d, &m, &n, &nz), "gk_dcsr_dims");
+    spMtx<T> R((size_t)m, (size_t)n, (size_t)nz);
+    std::memcpy(R.matcode, C.matcode, sizeof(MM_typecode));
+    check(gk_dcsr_download(context(), dc.d, R.Rst.data(), R.Col.data()), "gk_dcsr_download");
+    check(gk_dcsr_download_val(context(), dc.d, R.Val.data()), "gk_dcsr_download_val");
+    C = std::move(R);
+}
+
 }  // namespace detail
 }  // namespace gk
 
@@ -114,4 +145,15 @@ spMtx<T> mxmm_mca(bool isParallel, const spMtx<T> &A, const spMtx<U> &B, const s
     spMtx<T> C;
     mxmm_mca(isParallel, A, B, M, C);
     return C;
+}
+
+// include/matrix_la.h:126-131, 733-934: C<not M> = A*B.  C gets M's shape
+// and every (i,j) that some term A(i,k)*B(k,j) reaches and M does not hold,
+// rows in column order, explicit zeros kept, int32 wrap-around.  Repeated
+// mask columns count once (the reference's symbolic pass would subtract
+// them twice and mis-size the row).
+template <typename T, typename U = T>
+void mxmm_msa_cmask(bool isParallel, const spMtx<T> &A, const spMtx<T> &B, const spMtx<U> &M, spMtx<T> &C) {
+    (void)isParallel;
+    gk::detail::complement_product(A, B, M, C);
 }

@@ -56,6 +56,9 @@ def lib():
         L.or_mxmm.restype = C.c_int
         L.or_mxmm.argtypes = [C.c_int, C.c_int] + [C.c_int64, C.c_int64, C.c_int64, _i32, _i32, _i32] * 2 + \
             [C.c_int64, C.c_int64, C.c_int64, _i32, _i32, _i32]
+        L.or_mxmm_cmask.restype = C.c_int64
+        L.or_mxmm_cmask.argtypes = [C.c_int64, C.c_int64, C.c_int64, _i32, _i32, _i32] * 2 + \
+            [C.c_int64, C.c_int64, C.c_int64, _i32, _i32, _i32, _i32, _i32]
         L.or_tc_sandia.restype = C.c_int64
         L.or_tc_sandia.argtypes = [C.c_int, C.c_int, C.c_int64, C.c_int64, _i32, _i32]
         L.or_support.restype = C.c_int
@@ -107,6 +110,24 @@ def mxmm(A, B, M, accum: int = MSA, parallel: bool = True) -> np.ndarray:
     if rc != 0:
         raise ValueError("shape mismatch")
     return out
+
+
+def mxmm_cmask(A, B, M) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """C<not M> = A*B (mxmm_msa_cmask): (rst, col, val) of C, rows in column
+    order, explicit zeros kept, int32 wrap-around."""
+    am, an, anz, ar, ac, av = _csr(A)
+    bm, bn, bnz, br, bc, bv = _csr(B)
+    mm, mn, mnz, mr, mc, _ = _csr(M)
+    rst = np.zeros(am + 1, np.int32)
+    nz = lib().or_mxmm_cmask(am, an, anz, _p(ar), _p(ac), _p(av), bm, bn, bnz, _p(br), _p(bc), _p(bv),
+                             mm, mn, mnz, _p(mr), _p(mc), _p(rst), None, None)
+    if nz < 0:
+        raise ValueError("shape mismatch")
+    col = np.zeros(max(nz, 1), np.int32)
+    val = np.zeros(max(nz, 1), np.int32)
+    lib().or_mxmm_cmask(am, an, anz, _p(ar), _p(ac), _p(av), bm, bn, bnz, _p(br), _p(bc), _p(bv),
+                        mm, mn, mnz, _p(mr), _p(mc), _p(rst), _p(col), _p(val))
+    return rst, col[:nz], val[:nz]
 
 
 def tc_sandia(L, accum: int = MSA, parallel: bool = True) -> int:
@@ -167,6 +188,26 @@ def dense_masked_product(A, B, M) -> np.ndarray:
     P = dense(A) @ dense(B)
     rows = np.repeat(np.arange(M.m), np.diff(M.Rst))
     return (P[rows, np.asarray(M.Col)] & 0xFFFFFFFF).astype(np.uint32).view(np.int32)
+
+
+def dense_cmask_product(A, B, M) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """C<not M> = A*B by brute force: the entries some product term reaches
+    (a boolean product of the patterns), minus M's pattern, with the int64
+    products reduced mod 2^32; rows in column order."""
+    def dense(X, val=True):
+        D = np.zeros((X.m, X.n), np.int64)
+        v = np.ones(int(X.Rst[-1]), np.int64) if (getattr(X, "Val", None) is None or not val) else \
+            np.asarray(X.Val, np.int64)
+        rows = np.repeat(np.arange(X.m), np.diff(X.Rst))
+        np.add.at(D, (rows, np.asarray(X.Col)), v)
+        return D
+    reach = (dense(A, False) != 0).astype(np.int64) @ (dense(B, False) != 0).astype(np.int64) > 0
+    Mp = dense(M, False) != 0
+    keep = reach & ~Mp
+    P = dense(A) @ dense(B)
+    rows, cols = np.nonzero(keep)
+    rst = np.concatenate([[0], np.cumsum(keep.sum(axis=1))]).astype(np.int32)
+    return rst, cols.astype(np.int32), (P[rows, cols] & 0xFFFFFFFF).astype(np.uint32).view(np.int32)
 
 
 def _adj(A) -> list[list[int]]:

@@ -66,6 +66,12 @@ int main(int argc, char **argv) {
     spMtx<int> C1;
     msa(true, A, A, A, C1);
     put_array(f, "mxm_loaded", view_vec(C1.Val, C1.nz));
+    // C<not A> = A*A: the pairs two hops apart that are not neighbours
+    spMtx<int> N1;
+    mxmm_msa_cmask(true, A, A, A, N1);
+    put_array(f, "cmask_rst", view_vec(N1.Rst, N1.m + 1));
+    put_array(f, "cmask_col", view_vec(N1.Col, N1.nz));
+    put_array(f, "cmask_val", view_vec(N1.Val, N1.nz));
     if (!loaded_only) {
         spMtx<int> C2;
         mca(false, S, S, S, C2);

@@ -3,7 +3,8 @@
 tests/cpp/dropin_check is a client of libgraphhip.so written the way
 GraphKokkos's src/main.cpp uses its own headers: it loads a Matrix Market
 file into spMtx<int>, builds the adjacency and symmetric patterns, and runs
-mxmm_msa / mxmm_mca through mxmOp<int> pointers,
+mxmm_msa / mxmm_mca through mxmOp<int> pointers, the complement-mask
+product mxmm_msa_cmask,
 triangle_counting_masked_sandia and k_truss (both with the library's
 product, which takes the fused GPU entry points, and with a wrapped product,
 which takes the reference's loop), the truss decomposition, the supports and
@@ -75,6 +76,9 @@ def check_against_oracle(res, mtx, k, batch, loaded_only=False):
     assert np.array_equal(res["loaded_rst"], A.Rst) and np.array_equal(res["loaded_col"], A.Col)
     Ap = SpMtx(A.m, A.n, A.Rst, A.Col, np.ones(A.nz, np.int32))
     assert np.array_equal(res["mxm_loaded"], O.dense_masked_product(Ap, Ap, Ap))
+    rst, col, val = O.dense_cmask_product(Ap, Ap, Ap)
+    assert np.array_equal(res["cmask_rst"], rst) and np.array_equal(res["cmask_col"], col)
+    assert np.array_equal(res["cmask_val"], val)
     want_bc = O.bc_batch(Ap, np.arange(min(A.m, batch)))
     got_bc = np.asarray(res["bc"], np.float64)
     assert np.all(np.abs(got_bc - want_bc) <= 2e-7 * np.maximum(np.abs(want_bc), 1.0))

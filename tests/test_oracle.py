@@ -8,6 +8,7 @@ from math import comb
 
 import numpy as np
 import pytest
+import scipy.sparse
 
 from conftest import GOLDEN_NAMES, load_golden
 from graphkokkos_amd import HostGraph, SpMtx
@@ -127,6 +128,46 @@ def test_masked_product_brute_force(seed):
     for acc in (O.MSA, O.MCA):
         for par in (True, False):
             assert np.array_equal(O.mxmm(A, B, M, acc, par), want)
+
+
+@pytest.mark.parametrize("seed,shape,dens", [(0, (23, 17, 19), 0.3), (1, (40, 40, 40), 0.1), (2, (1, 6, 3), 0.9),
+                                             (3, (30, 8, 50), 0.25)])
+def test_complement_mask_product_brute_force(seed, shape, dens):
+    rng = np.random.default_rng(seed)
+    m, k, n = shape
+    A = random_spmtx(rng, m, k, dens, lo=-3, hi=4)   # explicit zeros among the values
+    B = random_spmtx(rng, k, n, dens, lo=-3, hi=4)
+    M = random_spmtx(rng, m, n, 0.3)
+    rst, col, val = O.mxmm_cmask(A, B, M)
+    wr, wc, wv = O.dense_cmask_product(A, B, M)
+    assert np.array_equal(rst, wr) and np.array_equal(col, wc) and np.array_equal(val, wv)
+    # an empty mask gives the whole structural product; a full one nothing
+    E = SpMtx(m, n, np.zeros(m + 1, np.int32), np.zeros(0, np.int32))
+    rst, col, val = O.mxmm_cmask(A, B, E)
+    wr, wc, wv = O.dense_cmask_product(A, B, E)
+    assert np.array_equal(rst, wr) and np.array_equal(col, wc) and np.array_equal(val, wv)
+    F = SpMtx(m, n, (np.arange(m + 1) * n).astype(np.int32), np.tile(np.arange(n), m).astype(np.int32))
+    assert O.mxmm_cmask(A, B, F)[0][-1] == 0
+
+
+def test_complement_mask_two_hop_identity():
+    """C<not A> = A*A on a symmetric pattern: its values count common
+    neighbours of non-adjacent pairs (the diagonal included), so they sum to
+    sum(deg^2) minus 6 per triangle; its pattern is the 2-hop pairs minus the
+    edges."""
+    g = HostGraph.rmat(11, 16, seed=3)
+    A = g.A
+    Av = SpMtx(A.m, A.n, A.Rst, A.Col, np.ones(A.nz, np.int32))
+    rst, col, val = O.mxmm_cmask(Av, Av, A)
+    deg = np.diff(A.Rst).astype(np.int64)
+    assert int(val.astype(np.int64).sum()) == int((deg * deg).sum()) - 6 * O.tc_sandia(g.L)
+    S = scipy.sparse.csr_matrix((np.ones(A.nz), A.Col, A.Rst), shape=(A.m, A.n))
+    reach = ((S @ S) > 0).astype(np.int8)
+    two = reach - reach.multiply(S > 0)
+    two.eliminate_zeros()
+    two = two.tocsr()
+    two.sort_indices()
+    assert np.array_equal(rst, two.indptr) and np.array_equal(col, two.indices)
 
 
 def test_int32_wraparound_matches_dense():
