@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark of the masked-product path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--legs ktruss,tc,bc,batch] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--legs ktruss,tc,bc,batch,cmask] [--no-cpu]
 
 Headline (`value`): full truss decomposition (k = 3..max) of the seeded
 R-MAT scale-20 graph (edge factor 16, a/b/c/d = .57/.19/.19/.05,
@@ -18,7 +18,9 @@ Also reported, each in its own object:
   bc  batched Brandes BC, 4 batches of 256 seeded sources on the scale-20
       graph, FP64, source-edges/s;
   batch  triangle count + 4-truss of each of 4096 seeded uneven graphs in one
-      hipGraph-captured step, edges/s, knobs autotuned.
+      hipGraph-captured step, edges/s, knobs autotuned;
+  cmask  complement-mask product C<not A> = A*A on R-MAT scale 16, product
+      terms/s.
 `roofline` is for the dominant kernel of the headline run (the accumulator
 kernel with the most device time): algorithmic bytes per launch over its
 average launch duration (HIP events on the library's stream).
@@ -155,6 +157,52 @@ def run_bc(ctx, A, sources, steps: int) -> dict:
     t = ctx.last_timing()
     bc = out.to_host(np.float64, A.m)
     return {"wall_s": wall, "timing": t, "levels": int(t["iterations"]), "bc_sum": float(bc.sum())}
+
+
+def run_cmask(ctx, scale: int, seed: int, steps: int, no_cpu: bool) -> dict:
+    """C<not A> = A*A on the symmetric R-MAT graph: every pair two hops
+    apart that is not an edge, with its number of common neighbours
+    (the diagonal included), through gk_masked_spgemm_cmask_dev."""
+    from graphkokkos_amd import HostGraph
+    G = HostGraph.rmat(scale, 16, seed=seed, **RMAT)
+    A = G.A
+    dA = ctx.upload(A)
+    C_ = ctx.masked_spgemm_cmask_dev(dA, dA, dA)  # warm-up
+    nz = C_.nz
+    C_.free()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        C_ = ctx.masked_spgemm_cmask_dev(dA, dA, dA)
+        C_.free()
+    ctx.sync()
+    wall = (time.perf_counter() - t0) / steps
+    t = ctx.last_timing()
+    terms = t["flop"]
+    exp_gbs = t["bytes_alg"] / 1e9 / (t["kernel_ms"] / 1e3) if t["kernel_ms"] > 0 else 0.0
+    out = {"workload": f"complement-mask product C<not A> = A*A (two-hop non-adjacent pairs with their common "
+                       f"neighbour counts), R-MAT scale {scale} ef16, pattern values",
+           "vertices": G.n, "edges": G.edges, "terms": terms, "entries": nz,
+           "value": terms / wall, "unit": "product terms/s", "ms": wall * 1e3, "device_ms": t["total_ms"],
+           "expand_ms": t["kernel_ms"], "sort_reduce_ms": t["select_ms"], "chunks": t["launches"],
+           "expand_roofline": {"bound": "hbm", "achieved": round(exp_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                               "frac": round(exp_gbs / HBM_PEAK_GBS, 5), "traffic": None,
+                               "kernel": "k_cm_expand", "bytes": t["bytes_alg"]}}
+    log(f"cmask s{scale}: {wall * 1e3:.1f} ms, {terms:.3g} terms, {nz} entries "
+        f"(expand {t['kernel_ms']:.1f} ms at {exp_gbs:.0f} GB/s, sort+reduce {t['select_ms']:.1f} ms)")
+    if not no_cpu:
+        from oracle import oracle as O
+        t0 = time.perf_counter()
+        rst, _, _ = O.mxmm_cmask(A, A, A)
+        dt = time.perf_counter() - t0
+        assert int(rst[-1]) == nz, "oracle and GPU entry counts differ"
+        out["cpu_baseline"] = {"value": terms / dt, "unit": "product terms/s", "cores": O.threads(), "kind": "port",
+                               "sample": f"the same product (R-MAT scale {scale}, {terms:.3g} terms) by the "
+                                         f"parallel restatement of the reference's two-pass complement-mask MSA "
+                                         f"in {dt:.2f}s"}
+        log(f"cpu baseline cmask: {out['cpu_baseline']['value']:.3g} terms/s")
+    dA.free()
+    return out
 
 
 def bc_bytes(n: int, nnz: int, levels: int, nsrc: int) -> float:
@@ -322,7 +370,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--legs", default="ktruss,tc,bc,batch")
+    ap.add_argument("--legs", default="ktruss,tc,bc,batch,cmask")
+    ap.add_argument("--cmask-scale", type=int, default=16)
     ap.add_argument("--ktruss-scale", type=int, default=20)
     ap.add_argument("--tc-scale", type=int, default=22)
     ap.add_argument("--bc-batches", type=int, default=4)
@@ -453,6 +502,10 @@ def main() -> int:
     if "batch" in legs:
         batch = run_batch(ctx, a)
 
+    cmask = None
+    if "cmask" in legs:
+        cmask = run_cmask(ctx, a.cmask_scale, a.seed, a.steps, a.no_cpu)
+
     cpu = None
     if not a.no_cpu:
         from oracle import oracle as O
@@ -513,6 +566,8 @@ def main() -> int:
         result["bc"] = bc
     if batch is not None:
         result["batch"] = batch
+    if cmask is not None:
+        result["cmask"] = cmask
     ctx.close()
     print(json.dumps(result), flush=True)
     return 0

@@ -4,14 +4,18 @@
 // terms (int32, wrap-around), explicit zeros kept, rows in column order.
 //
 // Unlike C<M> = A*B the output pattern is not known beforehand, so the
-// product runs as expand / sort / reduce over 64-bit keys (row << 32 | col):
-//   1. k_cm_flop: product terms per row; scanned, they cut the rows into
-//      chunks of at most kChunkTerms terms;
-//   2. per chunk, k_cm_expand writes one (key, value) per term -- a term at
-//      a mask column gets the all-ones key, which sorts last -- then a radix
-//      sort of the pairs and a reduce-by-key sum them per (i, j);
-//   3. the chunks' results are concatenated and the row pointers come from
-//      a count of each row's keys.
+// product runs as expand / sort / reduce:
+//   1. k_cm_entries: product terms per A entry (its B row's length);
+//      scanned, they give every entry's and every row's first term and cut
+//      the rows into chunks of at most kChunkTerms terms;
+//   2. per chunk, k_cm_expand writes one (key, value) per term with
+//      key = (i - r0) << cb | j (cb = bits of a column, r0 = the chunk's
+//      first row); a term at a mask column gets the key (r1 - r0) << cb,
+//      above every real key of the chunk.  A radix sort of the pairs over
+//      the key's bits only (32-bit keys when they fit) and a reduce-by-key
+//      sum the terms per (i, j);
+//   3. the chunks' results are concatenated; each row pointer is the
+//      position of the row's first key among its chunk's sorted keys.
 // The reference's two passes (symbolic row counts, then values through a
 // dense B.n accumulator per thread) become sorts here: HBM-bound key
 // traffic instead of a B.n-sized scan per row.
@@ -28,61 +32,138 @@ using namespace gk;
 namespace gk {
 namespace {
 
-constexpr uint64_t kMasked = ~0ull;
 constexpr int64_t kChunkTerms = int64_t(1) << 27;
+constexpr int kMaskLds = 1024;  // mask rows up to this long are searched in LDS
+constexpr int kWaves = 4;       // waves per block of the expand kernel
+constexpr int64_t kTaskTerms = 4096;  // product terms per expansion task
 
-__global__ void k_cm_flop(int64_t m, const int32_t *__restrict__ arst, const int32_t *__restrict__ acol,
-                          const int32_t *__restrict__ brst, int64_t *__restrict__ flop) {
-    const int lane = threadIdx.x & 63;
-    const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
-    for (int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < m; i += waves) {
-        int64_t f = 0;
-        for (int32_t p = arst[i] + lane; p < arst[i + 1]; p += 64) f += brst[acol[p] + 1] - brst[acol[p]];
-        f = dev::wave_sum(f);
-        if (lane == 0) flop[i] = f;
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) flop[m] = 0;
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
 }
 
-// One wave per row of [r0, r1): the row's terms go to [foff[i] - base, ...),
-// A entry by A entry, the lanes striding over B's row.
-__global__ void k_cm_expand(int64_t r0, int64_t r1, int64_t base, const int32_t *__restrict__ arst,
-                            const int32_t *__restrict__ acol, const int32_t *__restrict__ aval,
-                            const int32_t *__restrict__ brst, const int32_t *__restrict__ bcol,
-                            const int32_t *__restrict__ bval, const int32_t *__restrict__ mrst,
-                            const int32_t *__restrict__ mcol, const int64_t *__restrict__ foff,
-                            uint64_t *__restrict__ key, uint32_t *__restrict__ val) {
+// Per A entry t: blen[t] = |B row A.col[t]| and arow[t] = its row; one
+// wave per row.  blen[A.nz] = 0 closes the scan.
+__global__ void k_cm_entries(int64_t m, int64_t nz, const int32_t *__restrict__ arst,
+                             const int32_t *__restrict__ acol, const int32_t *__restrict__ brst,
+                             int64_t *__restrict__ blen, int32_t *__restrict__ arow) {
     const int lane = threadIdx.x & 63;
     const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
-    for (int64_t i = r0 + (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < r1; i += waves) {
-        int64_t out = foff[i] - base;
-        const int32_t m0 = mrst[i], m1 = mrst[i + 1];
-        for (int32_t t = arst[i]; t < arst[i + 1]; ++t) {
-            const int32_t k = acol[t];
-            const uint32_t a = aval ? (uint32_t)aval[t] : 1u;
-            const int32_t b0 = brst[k], len = brst[k + 1] - b0;
-            for (int32_t x = lane; x < len; x += 64) {
-                const int32_t j = bcol[b0 + x];
-                int32_t l = m0, h = m1;  // j among M's row (sorted)
-                while (l < h) {
-                    const int32_t mid = l + ((h - l) >> 1);
-                    if (mcol[mid] < j) l = mid + 1; else h = mid;
-                }
-                const bool masked = l < m1 && mcol[l] == j;
-                key[out + x] = masked ? kMasked : ((uint64_t)i << 32 | (uint32_t)j);
-                val[out + x] = a * (bval ? (uint32_t)bval[b0 + x] : 1u);
+    for (int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < m; i += waves)
+        for (int32_t p = arst[i] + lane; p < arst[i + 1]; p += 64) {
+            const int32_t k = acol[p];
+            blen[p] = brst[k + 1] - brst[k];
+            arow[p] = (int32_t)i;
+        }
+    if (blockIdx.x == 0 && threadIdx.x == 0) blen[nz] = 0;
+}
+
+// foff[i] = aoff[A.rst[i]]: the first product term of row i.
+__global__ void k_cm_row_terms(int64_t m, const int32_t *__restrict__ arst, const int64_t *__restrict__ aoff,
+                               int64_t *__restrict__ foff) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i <= m; i += (int64_t)gridDim.x * blockDim.x)
+        foff[i] = aoff[arst[i]];
+}
+
+// Whether j is among the sorted p[0, len).
+__device__ __forceinline__ bool in_sorted(const int32_t *p, int32_t len, int32_t j) {
+    int32_t l = 0, h = len;
+    while (l < h) {
+        const int32_t mid = l + ((h - l) >> 1);
+        if (p[mid] < j) l = mid + 1; else h = mid;
+    }
+    return l < len && p[l] == j;
+}
+
+// The chunk's terms [base, base + terms), in A-entry order, cut into tasks
+// of kTaskTerms terms, one wave per task: the task's first A entry comes
+// from a search over aoff (the scan of B row lengths over A's entries);
+// then A entries are taken 64 at a time into LDS (term offset, B row
+// start, A value, row, mask row) and every lane takes every 64th term of
+// the task in the group, naming its A entry by a search over the offsets.
+// Work per wave is bounded whatever the row and B-row lengths.
+template <class K>
+__global__ __launch_bounds__(64 * kWaves) void k_cm_expand(
+    int64_t r0, int64_t base, int64_t terms, int cb, K masked, int64_t anz, const int64_t *__restrict__ aoff,
+    const int32_t *__restrict__ arow, const int32_t *__restrict__ acol, const int32_t *__restrict__ aval,
+    const int32_t *__restrict__ brst, const int32_t *__restrict__ bcol, const int32_t *__restrict__ bval,
+    const int32_t *__restrict__ mrst, const int32_t *__restrict__ mcol, K *__restrict__ key,
+    uint32_t *__restrict__ val) {
+    __shared__ int64_t s_beg[kWaves][65];  // term offsets of the group's entries (+ the group's end)
+    __shared__ int32_t s_b0[kWaves][64];   // B row starts
+    __shared__ uint32_t s_a[kWaves][64];   // A values
+    __shared__ int32_t s_row[kWaves][64];  // rows
+    __shared__ int32_t s_m0[kWaves][64], s_ml[kWaves][64];  // mask rows
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t ntasks = (terms + kTaskTerms - 1) / kTaskTerms;
+    for (int64_t task = (int64_t)blockIdx.x * kWaves + w; task < ntasks; task += (int64_t)gridDim.x * kWaves) {
+        const int64_t q0 = base + task * kTaskTerms, q1 = min(q0 + kTaskTerms, base + terms);
+        // the A entry holding term q0: the last t with aoff[t] <= q0
+        int64_t l = 0, h = anz;
+        while (l < h) {
+            const int64_t mid = l + ((h - l + 1) >> 1);
+            if (aoff[mid] <= q0) l = mid; else h = mid - 1;
+        }
+        for (int64_t t = l;; t += 64) {
+            const int64_t te = t + lane;
+            wave_sync();
+            if (te < anz) {
+                const int32_t k = acol[te], i = arow[te];
+                s_beg[w][lane] = aoff[te];
+                s_b0[w][lane] = brst[k];
+                s_a[w][lane] = aval ? (uint32_t)aval[te] : 1u;
+                s_row[w][lane] = i;
+                s_m0[w][lane] = mrst[i];
+                s_ml[w][lane] = mrst[i + 1] - mrst[i];
+            } else {
+                s_beg[w][lane] = INT64_MAX;
             }
-            out += len;
+            if (lane == 63) s_beg[w][64] = aoff[min(t + 64, anz)];
+            wave_sync();
+            const int64_t gend = s_beg[w][64];
+            const int64_t lo = max(q0, s_beg[w][0]), hi = min(q1, gend);
+            for (int64_t q = lo + lane; q < hi; q += 64) {
+                int32_t el = 0, eh = 63;  // the last entry with offset <= q
+                while (el < eh) {
+                    const int32_t mid = (el + eh + 1) >> 1;
+                    if (s_beg[w][mid] <= q) el = mid; else eh = mid - 1;
+                }
+                const int32_t pb = s_b0[w][el] + (int32_t)(q - s_beg[w][el]);
+                const int32_t j = bcol[pb];
+                const bool hit = in_sorted(mcol + s_m0[w][el], s_ml[w][el], j);
+                key[q - base] = hit ? masked : ((K)(s_row[w][el] - r0) << cb | (K)(uint32_t)j);
+                val[q - base] = s_a[w][el] * (bval ? (uint32_t)bval[pb] : 1u);
+            }
+            if (gend >= q1 || t + 64 >= anz) break;
         }
     }
 }
 
-__global__ void k_cm_split(int64_t n, const uint64_t *__restrict__ key, int32_t *__restrict__ col,
-                           int32_t *__restrict__ rowcnt) {
-    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
-        col[p] = (int32_t)(key[p] & 0xffffffffull);
-        atomicAdd(&rowcnt[key[p] >> 32], 1);
+// Columns of a chunk's unique keys, written at the chunk's place in C.
+template <class K>
+__global__ void k_cm_cols(int64_t n, int cb, const K *__restrict__ key, int32_t *__restrict__ col) {
+    const K cmask = ((K)1 << cb) - 1;
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x)
+        col[p] = (int32_t)(key[p] & cmask);
+}
+
+// Row pointers of the chunk's rows [r0, r1): the first key of each row in
+// the chunk's sorted keys, offset by the chunk's place `at` in C; the last
+// chunk also closes rst[r1] = at + n.
+template <class K>
+__global__ void k_cm_rowptr(int64_t r0, int64_t r1, int64_t m, int cb, const K *__restrict__ key, int64_t n,
+                            int64_t at, int32_t *__restrict__ rst) {
+    for (int64_t r = r0 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < r1;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        const K first = (K)(r - r0) << cb;
+        int64_t l = 0, h = n;
+        while (l < h) {
+            const int64_t mid = l + ((h - l) >> 1);
+            if (key[mid] < first) l = mid + 1; else h = mid;
+        }
+        rst[r] = (int32_t)(at + l);
     }
+    if (r1 == m && blockIdx.x == 0 && threadIdx.x == 0) rst[m] = (int32_t)(at + n);
 }
 
 struct Mem {
@@ -105,6 +186,81 @@ int grid_of(gk_ctx *ctx, int64_t n, int64_t per_block) {
     return (int)std::max<int64_t>(1, std::min<int64_t>((n + per_block - 1) / per_block, (int64_t)ctx->num_cus * 16));
 }
 
+// Bits needed to hold v (v >= 1).
+int bits_of(int64_t v) { return 64 - __builtin_clzll((unsigned long long)v); }
+
+// Buffers of the chunk loop, sized for the largest chunk; keys are 32- or
+// 64-bit per chunk, so the key buffers hold 8 bytes a term.
+struct Work {
+    char *key, *skey, *ukey;
+    uint32_t *val, *sval, *uval;
+    int64_t *nu;
+    char *stmp, *rtmp;
+    size_t sort_bytes, red_bytes;
+};
+
+// One chunk's result: n unique (key, sum) pairs with key = (i - r0) << cb | j.
+struct Chunk {
+    int64_t r0, r1, n;
+    int cb;
+    bool wide;  // 64-bit keys
+    void *key;
+    uint32_t *val;
+    double expand_ms, sort_ms;  // device time of the expansion, of sort + reduce
+};
+
+template <class K>
+int chunk_tmp(gk_ctx *ctx, const Work &W, int64_t terms, size_t *sort_bytes, size_t *red_bytes) {
+    GK_HIP(rocprim::radix_sort_pairs(nullptr, *sort_bytes, (K *)W.key, (K *)W.skey, W.val, W.sval, (size_t)terms, 0,
+                                     (int)(8 * sizeof(K)), ctx->stream));
+    GK_HIP(rocprim::reduce_by_key(nullptr, *red_bytes, (K *)W.skey, W.sval, (size_t)terms, (K *)W.ukey, W.uval, W.nu,
+                                  rocprim::plus<uint32_t>(), rocprim::equal_to<K>(), ctx->stream));
+    return GK_OK;
+}
+
+// Expand, sort over the key's bits, reduce; keeps the unique pairs below
+// the mask key in buffers of their own (owned by T).
+template <class K>
+int run_chunk(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcsr *M, const int64_t *aoff,
+              const int32_t *arow, int64_t base, int64_t terms, Work &W, Mem &T, Chunk &ch) {
+    K *key = (K *)W.key, *skey = (K *)W.skey, *ukey = (K *)W.ukey;
+    const int end_bit = bits_of(ch.r1 - ch.r0) + ch.cb;
+    const int64_t tasks = (terms + kTaskTerms - 1) / kTaskTerms;
+    const int blocks = (int)std::min<int64_t>((tasks + kWaves - 1) / kWaves, (int64_t)ctx->num_cus * 16);
+    const K masked = (K)(ch.r1 - ch.r0) << ch.cb;
+    Timer te(ctx), ts(ctx);
+    te.start();
+    k_cm_expand<K><<<blocks, 64 * kWaves, 0, ctx->stream>>>(ch.r0, base, terms, ch.cb, masked, A->nz, aoff, arow,
+                                                            A->col, A->val, B->rst, B->col, B->val, M->rst, M->col,
+                                                            key, W.val);
+    GK_TRY(launch_check("k_cm_expand"));
+    te.stop();
+    ts.start();
+    size_t sb = W.sort_bytes, rb = W.red_bytes;
+    GK_HIP(rocprim::radix_sort_pairs(W.stmp, sb, key, skey, W.val, W.sval, (size_t)terms, 0, end_bit, ctx->stream));
+    GK_HIP(rocprim::reduce_by_key(W.rtmp, rb, skey, W.sval, (size_t)terms, ukey, W.uval, W.nu,
+                                  rocprim::plus<uint32_t>(), rocprim::equal_to<K>(), ctx->stream));
+    ts.stop();
+    int64_t hn = 0;
+    GK_HIP(d2h(ctx, &hn, W.nu, sizeof hn));
+    K last = 0;
+    GK_HIP(d2h(ctx, &last, ukey + hn - 1, sizeof last));
+    if (last == masked) --hn;  // the masked terms' key
+    ch.n = hn;
+    ch.expand_ms = te.ms();
+    ch.sort_ms = ts.ms();
+    if (hn == 0) return GK_OK;
+    K *k2;
+    uint32_t *v2;
+    GK_TRY(T.alloc(&k2, hn));
+    GK_TRY(T.alloc(&v2, hn));
+    GK_HIP(hipMemcpyAsync(k2, ukey, sizeof(K) * (size_t)hn, hipMemcpyDeviceToDevice, ctx->stream));
+    GK_HIP(hipMemcpyAsync(v2, W.uval, 4 * (size_t)hn, hipMemcpyDeviceToDevice, ctx->stream));
+    ch.key = k2;
+    ch.val = v2;
+    return GK_OK;
+}
+
 }  // namespace
 }  // namespace gk
 
@@ -118,21 +274,29 @@ extern "C" int gk_masked_spgemm_cmask_dev(gk_ctx *ctx, const gk_dcsr *A, const g
     }
     GK_TRY(require_sorted(M, "M"));
     const int64_t m = A->m;
+    Timer total(ctx);
+    total.start();
     Mem T;
-    int64_t *flop, *foff;
-    GK_TRY(T.alloc(&flop, m + 1));
+    const int64_t anz = A->nz;
+    int64_t *blen, *aoff, *foff;
+    int32_t *arow;
+    GK_TRY(T.alloc(&blen, anz + 1));
+    GK_TRY(T.alloc(&aoff, anz + 1));
+    GK_TRY(T.alloc(&arow, anz));
     GK_TRY(T.alloc(&foff, m + 1));
-    k_cm_flop<<<grid_of(ctx, m, 4), 256, 0, ctx->stream>>>(m, A->rst, A->col, B->rst, flop);
-    GK_TRY(launch_check("k_cm_flop"));
+    k_cm_entries<<<grid_of(ctx, m, 4), 256, 0, ctx->stream>>>(m, anz, A->rst, A->col, B->rst, blen, arow);
+    GK_TRY(launch_check("k_cm_entries"));
     {
         size_t tmp = 0;
-        GK_HIP(rocprim::exclusive_scan(nullptr, tmp, flop, foff, (int64_t)0, (size_t)(m + 1),
+        GK_HIP(rocprim::exclusive_scan(nullptr, tmp, blen, aoff, (int64_t)0, (size_t)(anz + 1),
                                        rocprim::plus<int64_t>(), ctx->stream));
         void *t;
         GK_TRY(T.alloc((char **)&t, (int64_t)tmp));
-        GK_HIP(rocprim::exclusive_scan(t, tmp, flop, foff, (int64_t)0, (size_t)(m + 1), rocprim::plus<int64_t>(),
+        GK_HIP(rocprim::exclusive_scan(t, tmp, blen, aoff, (int64_t)0, (size_t)(anz + 1), rocprim::plus<int64_t>(),
                                        ctx->stream));
     }
+    k_cm_row_terms<<<grid_of(ctx, m + 1, 256), 256, 0, ctx->stream>>>(m, A->rst, aoff, foff);
+    GK_TRY(launch_check("k_cm_row_terms"));
     std::vector<int64_t> hoff(m + 1);
     GK_HIP(d2h(ctx, hoff.data(), foff, sizeof(int64_t) * (size_t)(m + 1)));
     // row chunks of at most kChunkTerms terms (a longer row is a chunk alone)
@@ -150,54 +314,51 @@ extern "C" int gk_masked_spgemm_cmask_dev(gk_ctx *ctx, const gk_dcsr *A, const g
         return GK_ERANGE;
     }
     // values as uint32: the sums wrap around like the reference's int arithmetic
-    uint64_t *key, *skey, *ukey;
-    uint32_t *val, *sval, *uval;
-    int64_t *nu;
-    GK_TRY(T.alloc(&key, maxterms));
-    GK_TRY(T.alloc(&skey, maxterms));
-    GK_TRY(T.alloc(&ukey, maxterms));
-    GK_TRY(T.alloc(&val, maxterms));
-    GK_TRY(T.alloc(&sval, maxterms));
-    GK_TRY(T.alloc(&uval, maxterms));
-    GK_TRY(T.alloc(&nu, 1));
-    size_t sort_tmp = 0, red_tmp = 0;
-    GK_HIP(rocprim::radix_sort_pairs(nullptr, sort_tmp, key, skey, val, sval, (size_t)maxterms, 0, 64, ctx->stream));
-    GK_HIP(rocprim::reduce_by_key(nullptr, red_tmp, skey, sval, (size_t)maxterms, ukey, uval, nu,
-                                  rocprim::plus<uint32_t>(), rocprim::equal_to<uint64_t>(), ctx->stream));
-    void *stmp, *rtmp;
-    GK_TRY(T.alloc((char **)&stmp, (int64_t)sort_tmp));
-    GK_TRY(T.alloc((char **)&rtmp, (int64_t)red_tmp));
+    const int cb = bits_of(std::max<int64_t>(M->n - 1, 1));
+    Work W{};
+    GK_TRY(T.alloc(&W.key, 8 * maxterms));
+    GK_TRY(T.alloc(&W.skey, 8 * maxterms));
+    GK_TRY(T.alloc(&W.ukey, 8 * maxterms));
+    GK_TRY(T.alloc(&W.val, maxterms));
+    GK_TRY(T.alloc(&W.sval, maxterms));
+    GK_TRY(T.alloc(&W.uval, maxterms));
+    GK_TRY(T.alloc(&W.nu, 1));
+    {
+        size_t s32 = 0, s64 = 0, r32 = 0, r64 = 0;
+        GK_TRY(chunk_tmp<uint32_t>(ctx, W, maxterms, &s32, &r32));
+        GK_TRY(chunk_tmp<uint64_t>(ctx, W, maxterms, &s64, &r64));
+        W.sort_bytes = std::max(s32, s64);
+        W.red_bytes = std::max(r32, r64);
+        GK_TRY(T.alloc(&W.stmp, (int64_t)W.sort_bytes));
+        GK_TRY(T.alloc(&W.rtmp, (int64_t)W.red_bytes));
+    }
     // per chunk: unique (key, sum) pairs, kept until the total is known
-    std::vector<uint64_t *> ck;
-    std::vector<uint32_t *> cv;
-    std::vector<int64_t> cn;
+    std::vector<Chunk> chunks;
+    double expand_ms = 0, sort_ms = 0, bytes = 0;
+    int64_t runs = 0;
     for (size_t c = 1; c < cuts.size(); ++c) {
-        const int64_t r0 = cuts[c - 1], r1 = cuts[c], terms = hoff[r1] - hoff[r0];
-        if (terms == 0) continue;
-        k_cm_expand<<<grid_of(ctx, r1 - r0, 4), 256, 0, ctx->stream>>>(
-            r0, r1, hoff[r0], A->rst, A->col, A->val, B->rst, B->col, B->val, M->rst, M->col, foff, key, val);
-        GK_TRY(launch_check("k_cm_expand"));
-        GK_HIP(rocprim::radix_sort_pairs(stmp, sort_tmp, key, skey, val, sval, (size_t)terms, 0, 64, ctx->stream));
-        GK_HIP(rocprim::reduce_by_key(rtmp, red_tmp, skey, sval, (size_t)terms, ukey, uval, nu,
-                                      rocprim::plus<uint32_t>(), rocprim::equal_to<uint64_t>(), ctx->stream));
-        int64_t hn = 0;
-        GK_HIP(d2h(ctx, &hn, nu, sizeof hn));
-        uint64_t last = 0;
-        GK_HIP(d2h(ctx, &last, ukey + hn - 1, sizeof last));
-        if (last == kMasked) --hn;  // the masked terms' key
-        if (hn == 0) continue;
-        uint64_t *k2;
-        uint32_t *v2;
-        GK_TRY(T.alloc(&k2, hn));
-        GK_TRY(T.alloc(&v2, hn));
-        GK_HIP(hipMemcpyAsync(k2, ukey, 8 * (size_t)hn, hipMemcpyDeviceToDevice, ctx->stream));
-        GK_HIP(hipMemcpyAsync(v2, uval, 4 * (size_t)hn, hipMemcpyDeviceToDevice, ctx->stream));
-        ck.push_back(k2);
-        cv.push_back(v2);
-        cn.push_back(hn);
+        Chunk ch{};
+        ch.r0 = cuts[c - 1];
+        ch.r1 = cuts[c];
+        ch.cb = cb;
+        const int64_t terms = hoff[ch.r1] - hoff[ch.r0];
+        if (terms == 0) {  // rows with no product terms: empty in C
+            chunks.push_back(ch);
+            continue;
+        }
+        ch.wide = bits_of(ch.r1 - ch.r0) + cb > 32;
+        GK_TRY(ch.wide ? run_chunk<uint64_t>(ctx, A, B, M, aoff, arow, hoff[ch.r0], terms, W, T, ch)
+                       : run_chunk<uint32_t>(ctx, A, B, M, aoff, arow, hoff[ch.r0], terms, W, T, ch));
+        ++runs;
+        expand_ms += ch.expand_ms;
+        sort_ms += ch.sort_ms;
+        // expansion bytes: per term B's column (and value) read, key and
+        // value written
+        bytes += (double)terms * (4 + (B->val ? 4 : 0) + (ch.wide ? 8 : 4) + 4);
+        chunks.push_back(ch);
     }
     int64_t nz = 0;
-    for (int64_t x : cn) nz += x;
+    for (const Chunk &ch : chunks) nz += ch.n;
     if (nz > INT32_MAX) {
         set_error("C<not M> has %lld entries (int32 CSR limit)", (long long)nz);
         return GK_ERANGE;
@@ -220,33 +381,42 @@ extern "C" int gk_masked_spgemm_cmask_dev(gk_ctx *ctx, const gk_dcsr *A, const g
         set_error("hipMalloc of C<not M> failed");
         return fail(GK_ENOMEM);
     }
-    int32_t *rowcnt;
-    int rc = T.alloc(&rowcnt, m + 1);
-    if (rc != GK_OK) return fail(rc);
-    if (hipMemsetAsync(rowcnt, 0, 4 * (size_t)(m + 1), ctx->stream) != hipSuccess) return fail(GK_EHIP);
+    int rc;
     int64_t at = 0;
-    for (size_t c = 0; c < ck.size(); ++c) {
-        k_cm_split<<<grid_of(ctx, cn[c], 256), 256, 0, ctx->stream>>>(cn[c], ck[c], d->col + at, rowcnt);
-        if (hipMemcpyAsync(d->val + at, cv[c], 4 * (size_t)cn[c], hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess)
+    if (m == 0) k_cm_rowptr<uint32_t><<<1, 64, 0, ctx->stream>>>(0, 0, 0, cb, nullptr, 0, 0, d->rst);
+    for (const Chunk &ch : chunks) {
+        const int gr = grid_of(ctx, ch.r1 - ch.r0, 256), gc = grid_of(ctx, ch.n, 256);
+        if (ch.wide) {
+            const uint64_t *k = (const uint64_t *)ch.key;
+            k_cm_rowptr<uint64_t><<<gr, 256, 0, ctx->stream>>>(ch.r0, ch.r1, m, ch.cb, k, ch.n, at, d->rst);
+            if (ch.n) k_cm_cols<uint64_t><<<gc, 256, 0, ctx->stream>>>(ch.n, ch.cb, k, d->col + at);
+        } else {
+            const uint32_t *k = (const uint32_t *)ch.key;
+            k_cm_rowptr<uint32_t><<<gr, 256, 0, ctx->stream>>>(ch.r0, ch.r1, m, ch.cb, k, ch.n, at, d->rst);
+            if (ch.n) k_cm_cols<uint32_t><<<gc, 256, 0, ctx->stream>>>(ch.n, ch.cb, k, d->col + at);
+        }
+        if (ch.n &&
+            hipMemcpyAsync(d->val + at, ch.val, 4 * (size_t)ch.n, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess)
             return fail(GK_EHIP);
-        at += cn[c];
+        at += ch.n;
     }
-    if ((rc = launch_check("k_cm_split")) != GK_OK) return fail(rc);
-    {
-        size_t tmp = 0;
-        if (rocprim::exclusive_scan(nullptr, tmp, rowcnt, d->rst, 0, (size_t)(m + 1), rocprim::plus<int32_t>(),
-                                    ctx->stream) != hipSuccess)
-            return fail(GK_EHIP);
-        void *t;
-        if ((rc = T.alloc((char **)&t, (int64_t)tmp)) != GK_OK) return fail(rc);
-        if (rocprim::exclusive_scan(t, tmp, rowcnt, d->rst, 0, (size_t)(m + 1), rocprim::plus<int32_t>(),
-                                    ctx->stream) != hipSuccess)
-            return fail(GK_EHIP);
-    }
+    if ((rc = launch_check("k_cm_rowptr")) != GK_OK) return fail(rc);
+    total.stop();
     if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
         set_error("C<not M> failed on the device");
         return fail(GK_EHIP);
     }
+    gk_timing &tm = ctx->last;
+    tm = gk_timing{};
+    tm.total_ms = total.ms();
+    tm.launches = runs;
+    tm.flop = (double)hoff[m];
+    tm.hits = (double)nz;
+    tm.kernel_ms = expand_ms;
+    tm.select_ms = sort_ms;
+    // plus, per A entry, its column (and value) and B's two row pointers;
+    // the mask rows and the row offsets once
+    tm.bytes_alg = bytes + (double)A->nz * (4 + (A->val ? 4 : 0) + 8) + (double)M->nz * 4 + 16.0 * (double)m;
     *C = d;
     return GK_OK;
 }

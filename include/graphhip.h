@@ -246,7 +246,10 @@ int gk_masked_spgemm_dev(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B,
  * M.n), with the int32 (wrap-around) sum of the terms, explicit zeros kept,
  * rows in column order.  The pattern is not known beforehand, so *C is a
  * library-built device CSR with values (gk_dcsr_dims, gk_dcsr_download,
- * gk_dcsr_download_val; freed with gk_dcsr_free).  M needs sorted rows.   */
+ * gk_dcsr_download_val; freed with gk_dcsr_free).  M needs sorted rows.
+ * gk_last_timing afterwards: kernel_ms = the term expansion, select_ms =
+ * sort and reduce, launches = row chunks, flop = product terms, hits =
+ * entries of C, bytes_alg = the expansion's algorithmic bytes.            */
 int gk_masked_spgemm_cmask_dev(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B,
                                const gk_dcsr *M, gk_dcsr **C);
 int gk_dcsr_download_val(gk_ctx *ctx, const gk_dcsr *d, int32_t *val);

@@ -16,8 +16,9 @@
 //   or_truss_decompose  k_truss looped over k (every edge's largest k)
 //   or_truss_peel       independent queue-based peeling (cross-check of
 //                       or_truss_decompose, fast enough for large graphs)
-//   or_mxmm_cmask _mxmm_msa_cmask_sequential  include/matrix_la.h:871-930
-//                 (C<not M> = A*B: symbolic row counts, then values)
+//   or_mxmm_cmask _mxmm_msa_cmask_sequential  include/matrix_la.h:853-934
+//                 and its parallel twin :745-851 (C<not M> = A*B:
+//                 symbolic row counts, then values)
 //   or_bc_batch   betweenness_centrality_batch  include/betweenness_centrality.h:406-506
 //                 with bc_forward_step (:313-401, OpenMP twin :248-309) and
 //                 bc_backward_step (:182-242, OpenMP twin :128-178)
@@ -207,12 +208,15 @@ int or_mxmm(int accum, int parallel,
     return 0;
 }
 
-// C<not M> = A*B by the reference's sequential complement-mask MSA: pass 1
-// counts, per row, the distinct columns some A(i,k)*B(k,j) reaches minus
-// those in M's row; pass 2 accumulates them (mask columns UNALLOWED) and
-// writes them in column order, explicit zeros kept.  Called with col ==
-// NULL it fills rst only and returns nz; then again with col and val.
-int64_t or_mxmm_cmask(int64_t am, int64_t an, int64_t anz, const int32_t *arst, const int32_t *acol,
+// C<not M> = A*B by the reference's complement-mask MSA: pass 1 counts, per
+// row, the distinct columns some A(i,k)*B(k,j) reaches minus those in M's
+// row; pass 2 accumulates them (mask columns UNALLOWED) and writes them in
+// column order, explicit zeros kept.  Called with col == NULL it fills rst
+// only and returns nz; then again with col and val.  `parallel` splits the
+// rows over OpenMP threads, each with its own accumulator, as the
+// reference's parallel variant does (include/matrix_la.h:745-851); the
+// rows' results do not depend on it.
+int64_t or_mxmm_cmask(int parallel, int64_t am, int64_t an, int64_t anz, const int32_t *arst, const int32_t *acol,
                       const int32_t *aval, int64_t bm, int64_t bn, int64_t bnz, const int32_t *brst,
                       const int32_t *bcol, const int32_t *bval, int64_t mm, int64_t mn, int64_t mnz,
                       const int32_t *mrst, const int32_t *mcol, int32_t *rst, int32_t *col, int32_t *val) {
@@ -220,57 +224,68 @@ int64_t or_mxmm_cmask(int64_t am, int64_t an, int64_t anz, const int32_t *arst, 
     Csr A{am, an, anz, arst, acol, aval}, B{bm, bn, bnz, brst, bcol, bval};
     (void)mnz;
     enum { UNALLOWED = 0, ALLOWED = 1, SET = 2 };
-    std::vector<uint8_t> state(bn, UNALLOWED);
-    std::vector<int32_t> changed;
     if (!col) {
-        rst[0] = 0;
-        for (int64_t i = 0; i < A.m; ++i) {
-            int64_t row_nz = 0;
-            for (int64_t t = A.rst[i]; t < A.rst[i + 1]; ++t) {
-                const int32_t k = A.col[t];
-                for (int64_t j = B.rst[k]; j < B.rst[k + 1]; ++j)
-                    if (state[B.col[j]] == UNALLOWED) {
-                        state[B.col[j]] = ALLOWED;
-                        changed.push_back(B.col[j]);
-                        ++row_nz;
-                    }
+        std::vector<int64_t> cnt(A.m + 1, 0);
+#pragma omp parallel if (parallel)
+        {
+            std::vector<uint8_t> state(bn, UNALLOWED);
+            std::vector<int32_t> changed;
+#pragma omp for schedule(dynamic, 64)
+            for (int64_t i = 0; i < A.m; ++i) {
+                int64_t row_nz = 0;
+                for (int64_t t = A.rst[i]; t < A.rst[i + 1]; ++t) {
+                    const int32_t k = A.col[t];
+                    for (int64_t j = B.rst[k]; j < B.rst[k + 1]; ++j)
+                        if (state[B.col[j]] == UNALLOWED) {
+                            state[B.col[j]] = ALLOWED;
+                            changed.push_back(B.col[j]);
+                            ++row_nz;
+                        }
+                }
+                for (int64_t j = mrst[i]; j < mrst[i + 1]; ++j)
+                    if (state[mcol[j]] == ALLOWED) --row_nz;
+                cnt[i] = row_nz;
+                for (int32_t c : changed) state[c] = UNALLOWED;
+                changed.clear();
             }
-            for (int64_t j = mrst[i]; j < mrst[i + 1]; ++j)
-                if (state[mcol[j]] == ALLOWED) --row_nz;
-            rst[i + 1] = (int32_t)(rst[i] + row_nz);
-            for (int32_t c : changed) state[c] = UNALLOWED;
-            changed.clear();
         }
+        rst[0] = 0;
+        for (int64_t i = 0; i < A.m; ++i) rst[i + 1] = (int32_t)(rst[i] + cnt[i]);
         return rst[A.m];
     }
-    std::fill(state.begin(), state.end(), (uint8_t)ALLOWED);
-    std::vector<uint32_t> value(bn, 0);
-    for (int64_t i = 0; i < A.m; ++i) {
-        for (int64_t j = mrst[i]; j < mrst[i + 1]; ++j) state[mcol[j]] = UNALLOWED;
-        for (int64_t t = A.rst[i]; t < A.rst[i + 1]; ++t) {
-            const int32_t k = A.col[t];
-            const uint32_t a = (uint32_t)A.v(t);
-            for (int64_t j = B.rst[k]; j < B.rst[k + 1]; ++j) {
-                const int32_t c = B.col[j];
-                if (state[c] == ALLOWED) {
-                    state[c] = SET;
-                    changed.push_back(c);
-                    value[c] = a * (uint32_t)B.v(j);
-                } else if (state[c] == SET) {
-                    value[c] += a * (uint32_t)B.v(j);
+#pragma omp parallel if (parallel)
+    {
+        std::vector<uint8_t> state(bn, ALLOWED);
+        std::vector<uint32_t> value(bn, 0);
+        std::vector<int32_t> changed;
+#pragma omp for schedule(dynamic, 64)
+        for (int64_t i = 0; i < A.m; ++i) {
+            for (int64_t j = mrst[i]; j < mrst[i + 1]; ++j) state[mcol[j]] = UNALLOWED;
+            for (int64_t t = A.rst[i]; t < A.rst[i + 1]; ++t) {
+                const int32_t k = A.col[t];
+                const uint32_t a = (uint32_t)A.v(t);
+                for (int64_t j = B.rst[k]; j < B.rst[k + 1]; ++j) {
+                    const int32_t c = B.col[j];
+                    if (state[c] == ALLOWED) {
+                        state[c] = SET;
+                        changed.push_back(c);
+                        value[c] = a * (uint32_t)B.v(j);
+                    } else if (state[c] == SET) {
+                        value[c] += a * (uint32_t)B.v(j);
+                    }
                 }
             }
+            std::sort(changed.begin(), changed.end());
+            int64_t pos = rst[i];
+            for (int32_t c : changed) {
+                col[pos] = c;
+                val[pos++] = (int32_t)value[c];
+                value[c] = 0;
+                state[c] = ALLOWED;
+            }
+            changed.clear();
+            for (int64_t j = mrst[i]; j < mrst[i + 1]; ++j) state[mcol[j]] = ALLOWED;
         }
-        std::sort(changed.begin(), changed.end());
-        int64_t pos = rst[i];
-        for (int32_t c : changed) {
-            col[pos] = c;
-            val[pos++] = (int32_t)value[c];
-            value[c] = 0;
-            state[c] = ALLOWED;
-        }
-        changed.clear();
-        for (int64_t j = mrst[i]; j < mrst[i + 1]; ++j) state[mcol[j]] = ALLOWED;
     }
     return rst[A.m];
 }

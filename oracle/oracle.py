@@ -57,7 +57,7 @@ def lib():
         L.or_mxmm.argtypes = [C.c_int, C.c_int] + [C.c_int64, C.c_int64, C.c_int64, _i32, _i32, _i32] * 2 + \
             [C.c_int64, C.c_int64, C.c_int64, _i32, _i32, _i32]
         L.or_mxmm_cmask.restype = C.c_int64
-        L.or_mxmm_cmask.argtypes = [C.c_int64, C.c_int64, C.c_int64, _i32, _i32, _i32] * 2 + \
+        L.or_mxmm_cmask.argtypes = [C.c_int] + [C.c_int64, C.c_int64, C.c_int64, _i32, _i32, _i32] * 2 + \
             [C.c_int64, C.c_int64, C.c_int64, _i32, _i32, _i32, _i32, _i32]
         L.or_tc_sandia.restype = C.c_int64
         L.or_tc_sandia.argtypes = [C.c_int, C.c_int, C.c_int64, C.c_int64, _i32, _i32]
@@ -112,20 +112,20 @@ def mxmm(A, B, M, accum: int = MSA, parallel: bool = True) -> np.ndarray:
     return out
 
 
-def mxmm_cmask(A, B, M) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+def mxmm_cmask(A, B, M, parallel: bool = True) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
     """C<not M> = A*B (mxmm_msa_cmask): (rst, col, val) of C, rows in column
     order, explicit zeros kept, int32 wrap-around."""
     am, an, anz, ar, ac, av = _csr(A)
     bm, bn, bnz, br, bc, bv = _csr(B)
     mm, mn, mnz, mr, mc, _ = _csr(M)
     rst = np.zeros(am + 1, np.int32)
-    nz = lib().or_mxmm_cmask(am, an, anz, _p(ar), _p(ac), _p(av), bm, bn, bnz, _p(br), _p(bc), _p(bv),
+    nz = lib().or_mxmm_cmask(int(parallel), am, an, anz, _p(ar), _p(ac), _p(av), bm, bn, bnz, _p(br), _p(bc), _p(bv),
                              mm, mn, mnz, _p(mr), _p(mc), _p(rst), None, None)
     if nz < 0:
         raise ValueError("shape mismatch")
     col = np.zeros(max(nz, 1), np.int32)
     val = np.zeros(max(nz, 1), np.int32)
-    lib().or_mxmm_cmask(am, an, anz, _p(ar), _p(ac), _p(av), bm, bn, bnz, _p(br), _p(bc), _p(bv),
+    lib().or_mxmm_cmask(int(parallel), am, an, anz, _p(ar), _p(ac), _p(av), bm, bn, bnz, _p(br), _p(bc), _p(bv),
                         mm, mn, mnz, _p(mr), _p(mc), _p(rst), _p(col), _p(val))
     return rst, col[:nz], val[:nz]
 

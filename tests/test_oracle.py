@@ -138,9 +138,10 @@ def test_complement_mask_product_brute_force(seed, shape, dens):
     A = random_spmtx(rng, m, k, dens, lo=-3, hi=4)   # explicit zeros among the values
     B = random_spmtx(rng, k, n, dens, lo=-3, hi=4)
     M = random_spmtx(rng, m, n, 0.3)
-    rst, col, val = O.mxmm_cmask(A, B, M)
     wr, wc, wv = O.dense_cmask_product(A, B, M)
-    assert np.array_equal(rst, wr) and np.array_equal(col, wc) and np.array_equal(val, wv)
+    for par in (False, True):
+        rst, col, val = O.mxmm_cmask(A, B, M, parallel=par)
+        assert np.array_equal(rst, wr) and np.array_equal(col, wc) and np.array_equal(val, wv)
     # an empty mask gives the whole structural product; a full one nothing
     E = SpMtx(m, n, np.zeros(m + 1, np.int32), np.zeros(0, np.int32))
     rst, col, val = O.mxmm_cmask(A, B, E)
@@ -159,6 +160,8 @@ def test_complement_mask_two_hop_identity():
     A = g.A
     Av = SpMtx(A.m, A.n, A.Rst, A.Col, np.ones(A.nz, np.int32))
     rst, col, val = O.mxmm_cmask(Av, Av, A)
+    for x, y in zip((rst, col, val), O.mxmm_cmask(Av, Av, A, parallel=False)):
+        assert np.array_equal(x, y)
     deg = np.diff(A.Rst).astype(np.int64)
     assert int(val.astype(np.int64).sum()) == int((deg * deg).sum()) - 6 * O.tc_sandia(g.L)
     S = scipy.sparse.csr_matrix((np.ones(A.nz), A.Col, A.Rst), shape=(A.m, A.n))
