@@ -167,17 +167,23 @@ def run_cmask(ctx, scale: int, seed: int, steps: int, no_cpu: bool) -> dict:
     G = HostGraph.rmat(scale, 16, seed=seed, **RMAT)
     A = G.A
     dA = ctx.upload(A)
-    C_ = ctx.masked_spgemm_cmask_dev(dA, dA, dA)  # warm-up
-    nz = C_.nz
-    C_.free()
-    ctx.sync()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        C_ = ctx.masked_spgemm_cmask_dev(dA, dA, dA)
+
+    def timed(dense_rows):
+        C_ = ctx.masked_spgemm_cmask_dev(dA, dA, dA, dense_rows)  # warm-up
+        nz = C_.nz
         C_.free()
-    ctx.sync()
-    wall = (time.perf_counter() - t0) / steps
-    t = ctx.last_timing()
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            C_ = ctx.masked_spgemm_cmask_dev(dA, dA, dA, dense_rows)
+            C_.free()
+        ctx.sync()
+        return (time.perf_counter() - t0) / steps, nz, ctx.last_timing()
+
+    wall_sorted, nz_sorted, t_sorted = timed(0)  # every row expanded and sorted
+    log(f"cmask s{scale} sorted rows only: {wall_sorted * 1e3:.1f} ms")
+    wall, nz, t = timed(-1)
+    assert nz == nz_sorted, "the two ways of the rows disagree on C's size"
     terms = t["flop"]
     exp_gbs = t["bytes_alg"] / 1e9 / (t["kernel_ms"] / 1e3) if t["kernel_ms"] > 0 else 0.0
     out = {"workload": f"complement-mask product C<not A> = A*A (two-hop non-adjacent pairs with their common "
@@ -185,11 +191,15 @@ def run_cmask(ctx, scale: int, seed: int, steps: int, no_cpu: bool) -> dict:
            "vertices": G.n, "edges": G.edges, "terms": terms, "entries": nz,
            "value": terms / wall, "unit": "product terms/s", "ms": wall * 1e3, "device_ms": t["total_ms"],
            "expand_ms": t["kernel_ms"], "sort_reduce_ms": t["select_ms"], "chunks": t["launches"],
+           "dense_rows": t["iterations"], "dense_ms": t["acc_ms"][0],
+           "sorted_rows_only": {"ms": wall_sorted * 1e3, "value": terms / wall_sorted,
+                                "expand_ms": t_sorted["kernel_ms"], "sort_reduce_ms": t_sorted["select_ms"]},
            "expand_roofline": {"bound": "hbm", "achieved": round(exp_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                "frac": round(exp_gbs / HBM_PEAK_GBS, 5), "traffic": None,
                                "kernel": "k_cm_expand", "bytes": t["bytes_alg"]}}
-    log(f"cmask s{scale}: {wall * 1e3:.1f} ms, {terms:.3g} terms, {nz} entries "
-        f"(expand {t['kernel_ms']:.1f} ms at {exp_gbs:.0f} GB/s, sort+reduce {t['select_ms']:.1f} ms)")
+    log(f"cmask s{scale}: {wall * 1e3:.1f} ms, {terms:.3g} terms, {nz} entries, {t['iterations']} dense rows "
+        f"({t['acc_ms'][0]:.1f} ms), sorted rows: expand {t['kernel_ms']:.1f} ms at {exp_gbs:.0f} GB/s, "
+        f"sort+reduce {t['select_ms']:.1f} ms")
     if not no_cpu:
         from oracle import oracle as O
         t0 = time.perf_counter()

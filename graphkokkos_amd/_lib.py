@@ -88,7 +88,8 @@ _SIGS = {
     "gk_dcsr_dims": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "gk_dcsr_download": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "gk_dcsr_download_val": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int32)]),
-    "gk_masked_spgemm_cmask_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
+    "gk_masked_spgemm_cmask_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
+                                             C.POINTER(C.c_void_p)]),
     "gk_dgraph_rmat": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_uint64,
                                  C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_void_p]),
     "gk_dgraph_er": (C.c_int, [C.c_void_p, C.c_int64, C.c_double, C.c_uint64, C.c_int, C.POINTER(C.c_void_p),

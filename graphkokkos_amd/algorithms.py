@@ -199,21 +199,22 @@ class Context:
         check(lib().gk_masked_spgemm_dev(self.handle, A.handle, B.handle, M.handle, out.ptr, C.byref(o)),
               "gk_masked_spgemm_dev")
 
-    def masked_spgemm_cmask_dev(self, A: DeviceCsr, B: DeviceCsr, M: DeviceCsr) -> DeviceCsr:
+    def masked_spgemm_cmask_dev(self, A: DeviceCsr, B: DeviceCsr, M: DeviceCsr, dense_rows: int = -1) -> DeviceCsr:
         """C<not M> = A*B on device handles (M with sorted rows); the result
-        is a library-built CSR with values."""
+        is a library-built CSR with values.  dense_rows: -1 picks each row's
+        way (sorted or dense accumulation), 0 / 1 force one."""
         h = C.c_void_p()
-        check(lib().gk_masked_spgemm_cmask_dev(self.handle, A.handle, B.handle, M.handle, C.byref(h)),
+        check(lib().gk_masked_spgemm_cmask_dev(self.handle, A.handle, B.handle, M.handle, dense_rows, C.byref(h)),
               "gk_masked_spgemm_cmask_dev")
         return DeviceCsr.adopt(self, h)
 
-    def masked_spgemm_cmask(self, A: SpMtx, B: SpMtx, M: SpMtx) -> SpMtx:
+    def masked_spgemm_cmask(self, A: SpMtx, B: SpMtx, M: SpMtx, dense_rows: int = -1) -> SpMtx:
         """C<not M> = A*B: the entries some product term reaches outside M's
         pattern, with their int32 sums (explicit zeros kept)."""
         Ms, _ = _canonical_mask(M)
         dA, dB, dM = self.upload(A), self.upload(B), self.upload(Ms)
         try:
-            dC = self.masked_spgemm_cmask_dev(dA, dB, dM)
+            dC = self.masked_spgemm_cmask_dev(dA, dB, dM, dense_rows)
             try:
                 return dC.to_host(values=True)
             finally:

@@ -20,8 +20,11 @@
 // dense B.n accumulator per thread) become sorts here: HBM-bound key
 // traffic instead of a B.n-sized scan per row.
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_reduce.hpp>
 #include <rocprim/device/device_reduce_by_key.hpp>
 #include <rocprim/device/device_scan.hpp>
+#include <rocprim/device/device_select.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
 
 #include <vector>
 
@@ -42,17 +45,18 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
-// Per A entry t: blen[t] = |B row A.col[t]| and arow[t] = its row; one
-// wave per row.  blen[A.nz] = 0 closes the scan.
+// Per A entry t: blen[t] = |B row A.col[t]| (0 in a dense row) and arow[t]
+// = its row; one wave per row.  blen[A.nz] = 0 closes the scan.
 __global__ void k_cm_entries(int64_t m, int64_t nz, const int32_t *__restrict__ arst,
                              const int32_t *__restrict__ acol, const int32_t *__restrict__ brst,
-                             int64_t *__restrict__ blen, int32_t *__restrict__ arow) {
+                             const uint8_t *__restrict__ dense, int64_t *__restrict__ blen,
+                             int32_t *__restrict__ arow) {
     const int lane = threadIdx.x & 63;
     const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
     for (int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < m; i += waves)
         for (int32_t p = arst[i] + lane; p < arst[i + 1]; p += 64) {
             const int32_t k = acol[p];
-            blen[p] = brst[k + 1] - brst[k];
+            blen[p] = dense[i] ? 0 : brst[k + 1] - brst[k];
             arow[p] = (int32_t)i;
         }
     if (blockIdx.x == 0 && threadIdx.x == 0) blen[nz] = 0;
@@ -139,31 +143,238 @@ __global__ __launch_bounds__(64 * kWaves) void k_cm_expand(
     }
 }
 
-// Columns of a chunk's unique keys, written at the chunk's place in C.
-template <class K>
-__global__ void k_cm_cols(int64_t n, int cb, const K *__restrict__ key, int32_t *__restrict__ col) {
-    const K cmask = ((K)1 << cb) - 1;
-    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x)
-        col[p] = (int32_t)(key[p] & cmask);
+// ---- dense rows ------------------------------------------------------------
+// A row with many product terms (relative to its A entries and to C's width)
+// is accumulated densely instead: its columns are cut into windows of W
+// (a power of two, at most kDenseWindow) columns and one workgroup takes a
+// (row, window) pair at a time, with a bitmap of the window's reached
+// columns and, in the numeric pass, their int32 sums in LDS.  A first pass
+// counts each pair's entries (bitmap only); the second, with the row
+// pointers known, writes them in column order straight into C.  Each A
+// entry's B row is cut to the window by two binary searches (B rows sorted;
+// a single window needs none).
+constexpr int kDenseWindowLog = 15;  // 32768 columns: 128 KiB of sums + 4 KiB of bits
+constexpr int kDenseThreads = 1024;
+constexpr int kDenseWaves = kDenseThreads / 64;
+
+__global__ void k_cm_flop(int64_t m, const int32_t *__restrict__ arst, const int32_t *__restrict__ acol,
+                          const int32_t *__restrict__ brst, int64_t *__restrict__ flop) {
+    const int lane = threadIdx.x & 63;
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < m; i += waves) {
+        int64_t f = 0;
+        for (int32_t p = arst[i] + lane; p < arst[i + 1]; p += 64) f += brst[acol[p] + 1] - brst[acol[p]];
+        f = dev::wave_sum(f);
+        if (lane == 0) flop[i] = f;
+    }
 }
 
-// Row pointers of the chunk's rows [r0, r1): the first key of each row in
-// the chunk's sorted keys, offset by the chunk's place `at` in C; the last
-// chunk also closes rst[r1] = at + n.
+// dense[i]: row i takes the dense path.  mode 0: none; 1: every row with
+// terms; -1: rows whose terms outweigh the per-window work (a fixed cost
+// per window plus two searches per A entry and window).
+__global__ void k_cm_classify(int64_t m, int mode, int64_t windows, const int64_t *__restrict__ flop,
+                              const int32_t *__restrict__ arst, uint8_t *__restrict__ dense) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t f = flop[i], deg = arst[i + 1] - arst[i];
+        bool d = false;
+        if (mode == 1) d = f > 0;
+        else if (mode < 0) d = f >= windows * (2048 + 16 * deg);
+        dense[i] = d ? 1 : 0;
+    }
+}
+
+__device__ __forceinline__ int32_t lower_bound_col(const int32_t *__restrict__ c, int32_t l, int32_t h, int32_t v) {
+    while (l < h) {
+        const int32_t mid = l + ((h - l) >> 1);
+        if (c[mid] < v) l = mid + 1; else h = mid;
+    }
+    return l;
+}
+
+// One pass over the (dense row, window) pairs; NUMERIC = false counts each
+// pair's entries into pcnt, NUMERIC = true writes them at
+// rst[row] + poff[pair] (poff: the pair's place within its row).
+template <bool NUMERIC>
+__global__ __launch_bounds__(kDenseThreads) void k_cm_dense(
+    int64_t npairs, int64_t windows, int wlog, int64_t n, const int32_t *__restrict__ drows,
+    const int32_t *__restrict__ arst, const int32_t *__restrict__ acol, const int32_t *__restrict__ aval,
+    const int32_t *__restrict__ brst, const int32_t *__restrict__ bcol, const int32_t *__restrict__ bval,
+    const int32_t *__restrict__ mrst, const int32_t *__restrict__ mcol, int32_t *__restrict__ pcnt,
+    const int32_t *__restrict__ poff, const int32_t *__restrict__ rst, int32_t *__restrict__ ccol,
+    int32_t *__restrict__ cval) {
+    __shared__ uint32_t bits[1 << (kDenseWindowLog - 5)];
+    __shared__ uint32_t vals[NUMERIC ? 1 << kDenseWindowLog : 1];
+    __shared__ int64_t s_end[kDenseThreads];  // block scan of the group's B row cuts
+    __shared__ int32_t s_b0[kDenseThreads];
+    __shared__ uint32_t s_a[kDenseThreads];
+    __shared__ int64_t s_red[kDenseWaves];
+    __shared__ int32_t s_mlo;
+    const int W = 1 << wlog, words = W >> 5;  // wlog <= kDenseWindowLog
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int64_t pair = blockIdx.x; pair < npairs; pair += gridDim.x) {
+        const int32_t row = drows[pair / windows];
+        const int64_t win = pair % windows;
+        const int32_t w0 = (int32_t)(win << wlog);
+        const int32_t w1 = (int32_t)min((int64_t)w0 + W, n);
+        for (int x = threadIdx.x; x < words; x += kDenseThreads) bits[x] = 0;
+        if (NUMERIC)
+            for (int x = threadIdx.x; x < W; x += kDenseThreads) vals[x] = 0;
+        __syncthreads();
+        const int32_t a0 = arst[row], a1 = arst[row + 1];
+        if (threadIdx.x == 0) s_mlo = lower_bound_col(mcol, mrst[row], mrst[row + 1], w0);
+        // A entries 1024 at a time; their B rows' terms spread over the
+        // whole block (a row's hub entries come first and hold most terms)
+        for (int32_t g = a0; g < a1; g += kDenseThreads) {
+            const int32_t t = g + threadIdx.x;
+            int32_t len = 0, s = 0;
+            uint32_t a = 0;
+            if (t < a1) {
+                const int32_t k = acol[t];
+                int32_t b0 = brst[k], b1 = brst[k + 1];
+                if (w0 > 0) b0 = lower_bound_col(bcol, b0, b1, w0);  // columns are < n
+                if (w1 < n) b1 = lower_bound_col(bcol, b0, b1, w1);
+                s = b0;
+                len = b1 - b0;
+                a = aval ? (uint32_t)aval[t] : 1u;
+            }
+            const int64_t wincl = dev::wave_incl_scan((int64_t)len);
+            if (lane == 63) s_red[w] = wincl;
+            __syncthreads();
+            int64_t before = 0, total = 0;
+            for (int x = 0; x < kDenseWaves; ++x) {
+                const int64_t r = s_red[x];
+                before += x < w ? r : 0;
+                total += r;
+            }
+            s_end[threadIdx.x] = before + wincl;
+            s_b0[threadIdx.x] = s;
+            s_a[threadIdx.x] = a;
+            __syncthreads();
+            int32_t el = 0;  // the first entry whose scan exceeds q; q only grows
+            for (int64_t q = threadIdx.x; q < total; q += kDenseThreads) {
+                if (s_end[el] <= q) {
+                    int32_t eh = kDenseThreads - 1;
+                    ++el;
+                    while (el < eh) {
+                        const int32_t mid = (el + eh) >> 1;
+                        if (s_end[mid] <= q) el = mid + 1; else eh = mid;
+                    }
+                }
+                const int32_t pb = s_b0[el] + (int32_t)(q - (el ? s_end[el - 1] : 0));
+                const int32_t j = bcol[pb] - w0;
+                // a B row's consecutive columns share bitmap words and the
+                // hub columns recur: test first, so that a set bit costs a
+                // (broadcast) read rather than a contended atomic
+                const uint32_t bit = 1u << (j & 31);
+                if (!(bits[j >> 5] & bit)) atomicOr(&bits[j >> 5], bit);
+                if (NUMERIC) atomicAdd(&vals[j], s_a[el] * (bval ? (uint32_t)bval[pb] : 1u));
+            }
+            __syncthreads();
+        }
+        __syncthreads();
+        {  // the mask row's columns in the window leave the bitmap
+            const int32_t m1 = mrst[row + 1];
+            for (int32_t p = s_mlo + threadIdx.x; p < m1; p += kDenseThreads) {
+                const int32_t j = mcol[p];
+                if (j >= w1) break;
+                atomicAnd(&bits[(j - w0) >> 5], ~(1u << ((j - w0) & 31)));
+            }
+        }
+        __syncthreads();
+        // each thread takes a run of `per` words
+        const int per = words > kDenseThreads ? words / kDenseThreads : 1;
+        const int wb = threadIdx.x * per;
+        int32_t c = 0;
+        for (int x = 0; x < per; ++x)
+            if (wb + x < words) c += __popc(bits[wb + x]);
+        const int32_t incl = dev::wave_incl_scan(c);
+        if (lane == 63) s_red[w] = incl;
+        __syncthreads();
+        if (!NUMERIC) {
+            if (threadIdx.x == 0) {
+                int64_t tot = 0;
+                for (int x = 0; x < kDenseWaves; ++x) tot += s_red[x];
+                pcnt[pair] = (int32_t)tot;
+            }
+        } else {
+            int64_t before = 0;
+            for (int x = 0; x < w; ++x) before += s_red[x];
+            int64_t out = (int64_t)rst[row] + poff[pair] + before + incl - c;
+            for (int x = 0; x < per; ++x) {
+                if (wb + x >= words) break;
+                uint32_t b = bits[wb + x];
+                while (b) {
+                    const int bit = __ffs(b) - 1;
+                    b &= b - 1;
+                    const int j = ((wb + x) << 5) + bit;
+                    ccol[out] = w0 + j;
+                    cval[out] = (int32_t)vals[j];
+                    ++out;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Per dense row: the pairs' places within the row and the row's count.
+__global__ void k_cm_dense_rows(int64_t ndense, int64_t windows, const int32_t *__restrict__ drows,
+                                const int32_t *__restrict__ pcnt, int32_t *__restrict__ poff,
+                                int64_t *__restrict__ cnt) {
+    for (int64_t h = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; h < ndense;
+         h += (int64_t)gridDim.x * blockDim.x) {
+        int64_t run = 0;
+        for (int64_t x = 0; x < windows; ++x) {
+            poff[h * windows + x] = (int32_t)run;
+            run += pcnt[h * windows + x];
+        }
+        cnt[drows[h]] = run;
+    }
+}
+
+// ---- sorted rows: counts and placement ----------------------------------
+// For the chunk's sorted rows in [r0, r1): cnt[r] = the row's entries among
+// the chunk's sorted unique keys and first[r] = the position of its first.
 template <class K>
-__global__ void k_cm_rowptr(int64_t r0, int64_t r1, int64_t m, int cb, const K *__restrict__ key, int64_t n,
-                            int64_t at, int32_t *__restrict__ rst) {
+__global__ void k_cm_rowcount(int64_t r0, int64_t r1, int cb, const K *__restrict__ key, int64_t n,
+                              const uint8_t *__restrict__ dense, int64_t *__restrict__ cnt,
+                              int64_t *__restrict__ first) {
     for (int64_t r = r0 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < r1;
          r += (int64_t)gridDim.x * blockDim.x) {
-        const K first = (K)(r - r0) << cb;
-        int64_t l = 0, h = n;
-        while (l < h) {
-            const int64_t mid = l + ((h - l) >> 1);
-            if (key[mid] < first) l = mid + 1; else h = mid;
+        if (dense[r]) continue;  // counted by the dense pass
+        int64_t b[2];
+        for (int e = 0; e < 2; ++e) {
+            const K v = (K)(r - r0 + e) << cb;
+            int64_t l = 0, h = n;
+            while (l < h) {
+                const int64_t mid = l + ((h - l) >> 1);
+                if (key[mid] < v) l = mid + 1; else h = mid;
+            }
+            b[e] = l;
         }
-        rst[r] = (int32_t)(at + l);
+        first[r] = b[0];
+        cnt[r] = b[1] - b[0];
     }
-    if (r1 == m && blockIdx.x == 0 && threadIdx.x == 0) rst[m] = (int32_t)(at + n);
+}
+
+// The chunk's entries to their places in C.
+template <class K>
+__global__ void k_cm_place(int64_t n, int64_t r0, int cb, const K *__restrict__ key, const uint32_t *__restrict__ v,
+                           const int64_t *__restrict__ first, const int64_t *__restrict__ rst64,
+                           int32_t *__restrict__ ccol, int32_t *__restrict__ cval) {
+    const K cmask = ((K)1 << cb) - 1;
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+        const K k = key[p];
+        const int64_t r = r0 + (int64_t)(k >> cb);
+        const int64_t at = rst64[r] + (p - first[r]);
+        ccol[at] = (int32_t)(k & cmask);
+        cval[at] = (int32_t)v[p];
+    }
+}
+
+__global__ void k_cm_rst32(int64_t m, const int64_t *__restrict__ rst64, int32_t *__restrict__ rst) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i <= m; i += (int64_t)gridDim.x * blockDim.x)
+        rst[i] = (int32_t)rst64[i];
 }
 
 struct Mem {
@@ -265,33 +476,76 @@ int run_chunk(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcsr *M,
 }  // namespace gk
 
 extern "C" int gk_masked_spgemm_cmask_dev(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcsr *M,
-                                          gk_dcsr **C) {
+                                          int32_t dense_rows, gk_dcsr **C) {
     GK_CHECK_ARG(ctx && A && B && M && C, "null argument");
+    GK_CHECK_ARG(dense_rows >= -1 && dense_rows <= 1, "dense_rows must be -1, 0 or 1");
     if (A->n != B->m || A->m != M->m || B->n != M->n) {
         set_error("shape mismatch: A %lldx%lld, B %lldx%lld, M %lldx%lld", (long long)A->m, (long long)A->n,
                   (long long)B->m, (long long)B->n, (long long)M->m, (long long)M->n);
         return GK_EINVAL;
     }
     GK_TRY(require_sorted(M, "M"));
-    const int64_t m = A->m;
-    Timer total(ctx);
+    const int64_t m = A->m, n = M->n, anz = A->nz;
+    Timer total(ctx), tdense(ctx);
     total.start();
     Mem T;
-    const int64_t anz = A->nz;
+    // dense windows: W = the power of two >= n, at most 2^kDenseWindowLog
+    int wlog = 5;
+    while (wlog < kDenseWindowLog && ((int64_t)1 << wlog) < n) ++wlog;
+    const int64_t windows = std::max<int64_t>(1, (n + ((int64_t)1 << wlog) - 1) >> wlog);
+    // a window cut of B rows needs them sorted
+    const int mode = (windows > 1 && !B->sorted) ? 0 : dense_rows;
+
+    // 1. terms per row, the dense rows
+    int64_t *flop;
+    uint8_t *dflag;
+    int32_t *drows, *nd32;
+    GK_TRY(T.alloc(&flop, m + 1));
+    GK_TRY(T.alloc(&dflag, m + 1));
+    GK_TRY(T.alloc(&drows, m + 1));
+    GK_TRY(T.alloc(&nd32, 1));
+    k_cm_flop<<<grid_of(ctx, m, 4), 256, 0, ctx->stream>>>(m, A->rst, A->col, B->rst, flop);
+    k_cm_classify<<<grid_of(ctx, m, 256), 256, 0, ctx->stream>>>(m, mode, windows, flop, A->rst, dflag);
+    GK_TRY(launch_check("k_cm_classify"));
+    int64_t ndense = 0, terms_all = 0;
+    if (m > 0) {  // all product terms, for the timing record
+        int64_t *tsum;
+        size_t tmp = 0;
+        GK_TRY(T.alloc(&tsum, 1));
+        GK_HIP(rocprim::reduce(nullptr, tmp, flop, tsum, (int64_t)0, (size_t)m, rocprim::plus<int64_t>(),
+                               ctx->stream));
+        char *t;
+        GK_TRY(T.alloc(&t, (int64_t)tmp));
+        GK_HIP(rocprim::reduce(t, tmp, flop, tsum, (int64_t)0, (size_t)m, rocprim::plus<int64_t>(), ctx->stream));
+        GK_HIP(d2h(ctx, &terms_all, tsum, sizeof terms_all));
+    }
+    if (mode != 0 && m > 0) {
+        size_t tmp = 0;
+        rocprim::counting_iterator<int32_t> ids(0);
+        GK_HIP(rocprim::select(nullptr, tmp, ids, dflag, drows, nd32, (size_t)m, ctx->stream));
+        char *t;
+        GK_TRY(T.alloc(&t, (int64_t)tmp));
+        GK_HIP(rocprim::select(t, tmp, ids, dflag, drows, nd32, (size_t)m, ctx->stream));
+        int32_t h = 0;
+        GK_HIP(d2h(ctx, &h, nd32, sizeof h));
+        ndense = h;
+    }
+
+    // 2. the sorted rows' terms: per A entry (0 in dense rows), scanned
     int64_t *blen, *aoff, *foff;
     int32_t *arow;
     GK_TRY(T.alloc(&blen, anz + 1));
     GK_TRY(T.alloc(&aoff, anz + 1));
     GK_TRY(T.alloc(&arow, anz));
     GK_TRY(T.alloc(&foff, m + 1));
-    k_cm_entries<<<grid_of(ctx, m, 4), 256, 0, ctx->stream>>>(m, anz, A->rst, A->col, B->rst, blen, arow);
+    k_cm_entries<<<grid_of(ctx, m, 4), 256, 0, ctx->stream>>>(m, anz, A->rst, A->col, B->rst, dflag, blen, arow);
     GK_TRY(launch_check("k_cm_entries"));
     {
         size_t tmp = 0;
         GK_HIP(rocprim::exclusive_scan(nullptr, tmp, blen, aoff, (int64_t)0, (size_t)(anz + 1),
                                        rocprim::plus<int64_t>(), ctx->stream));
-        void *t;
-        GK_TRY(T.alloc((char **)&t, (int64_t)tmp));
+        char *t;
+        GK_TRY(T.alloc(&t, (int64_t)tmp));
         GK_HIP(rocprim::exclusive_scan(t, tmp, blen, aoff, (int64_t)0, (size_t)(anz + 1), rocprim::plus<int64_t>(),
                                        ctx->stream));
     }
@@ -299,7 +553,30 @@ extern "C" int gk_masked_spgemm_cmask_dev(gk_ctx *ctx, const gk_dcsr *A, const g
     GK_TRY(launch_check("k_cm_row_terms"));
     std::vector<int64_t> hoff(m + 1);
     GK_HIP(d2h(ctx, hoff.data(), foff, sizeof(int64_t) * (size_t)(m + 1)));
-    // row chunks of at most kChunkTerms terms (a longer row is a chunk alone)
+
+    // 3. the dense rows' counts (first pass), while the host cuts chunks
+    int32_t *pcnt, *poff;
+    int64_t *cnt, *first;
+    const int64_t npairs = ndense * windows;
+    GK_TRY(T.alloc(&pcnt, npairs));
+    GK_TRY(T.alloc(&poff, npairs));
+    GK_TRY(T.alloc(&cnt, m + 1));
+    GK_TRY(T.alloc(&first, m + 1));
+    GK_HIP(hipMemsetAsync(cnt, 0, sizeof(int64_t) * (size_t)(m + 1), ctx->stream));
+    tdense.start();
+    if (npairs) {
+        const int g = (int)std::min<int64_t>(npairs, (int64_t)ctx->num_cus * 2);
+        k_cm_dense<false><<<g, kDenseThreads, 0, ctx->stream>>>(
+            npairs, windows, wlog, n, drows, A->rst, A->col, A->val, B->rst, B->col, B->val, M->rst, M->col, pcnt,
+            nullptr, nullptr, nullptr, nullptr);
+        GK_TRY(launch_check("k_cm_dense<count>"));
+        k_cm_dense_rows<<<grid_of(ctx, ndense, 256), 256, 0, ctx->stream>>>(ndense, windows, drows, pcnt, poff, cnt);
+        GK_TRY(launch_check("k_cm_dense_rows"));
+    }
+    tdense.stop();
+    double dense_ms = 0;
+
+    // row chunks of at most kChunkTerms sorted-row terms (a longer row is a chunk alone)
     std::vector<int64_t> cuts{0};
     while (cuts.back() < m) {
         const int64_t r0 = cuts.back();
@@ -314,7 +591,7 @@ extern "C" int gk_masked_spgemm_cmask_dev(gk_ctx *ctx, const gk_dcsr *A, const g
         return GK_ERANGE;
     }
     // values as uint32: the sums wrap around like the reference's int arithmetic
-    const int cb = bits_of(std::max<int64_t>(M->n - 1, 1));
+    const int cb = bits_of(std::max<int64_t>(n - 1, 1));
     Work W{};
     GK_TRY(T.alloc(&W.key, 8 * maxterms));
     GK_TRY(T.alloc(&W.skey, 8 * maxterms));
@@ -332,7 +609,7 @@ extern "C" int gk_masked_spgemm_cmask_dev(gk_ctx *ctx, const gk_dcsr *A, const g
         GK_TRY(T.alloc(&W.stmp, (int64_t)W.sort_bytes));
         GK_TRY(T.alloc(&W.rtmp, (int64_t)W.red_bytes));
     }
-    // per chunk: unique (key, sum) pairs, kept until the total is known
+    // 4. sorted rows: per chunk expand / sort / reduce, then the rows' counts
     std::vector<Chunk> chunks;
     double expand_ms = 0, sort_ms = 0, bytes = 0;
     int64_t runs = 0;
@@ -342,10 +619,7 @@ extern "C" int gk_masked_spgemm_cmask_dev(gk_ctx *ctx, const gk_dcsr *A, const g
         ch.r1 = cuts[c];
         ch.cb = cb;
         const int64_t terms = hoff[ch.r1] - hoff[ch.r0];
-        if (terms == 0) {  // rows with no product terms: empty in C
-            chunks.push_back(ch);
-            continue;
-        }
+        if (terms == 0) continue;  // no sorted-row entries: counts stay 0
         ch.wide = bits_of(ch.r1 - ch.r0) + cb > 32;
         GK_TRY(ch.wide ? run_chunk<uint64_t>(ctx, A, B, M, aoff, arow, hoff[ch.r0], terms, W, T, ch)
                        : run_chunk<uint32_t>(ctx, A, B, M, aoff, arow, hoff[ch.r0], terms, W, T, ch));
@@ -355,10 +629,33 @@ extern "C" int gk_masked_spgemm_cmask_dev(gk_ctx *ctx, const gk_dcsr *A, const g
         // expansion bytes: per term B's column (and value) read, key and
         // value written
         bytes += (double)terms * (4 + (B->val ? 4 : 0) + (ch.wide ? 8 : 4) + 4);
+        if (ch.n == 0) continue;
+        const int gr = grid_of(ctx, ch.r1 - ch.r0, 256);
+        if (ch.wide)
+            k_cm_rowcount<uint64_t><<<gr, 256, 0, ctx->stream>>>(ch.r0, ch.r1, cb, (const uint64_t *)ch.key, ch.n,
+                                                                 dflag, cnt, first);
+        else
+            k_cm_rowcount<uint32_t><<<gr, 256, 0, ctx->stream>>>(ch.r0, ch.r1, cb, (const uint32_t *)ch.key, ch.n,
+                                                                 dflag, cnt, first);
+        GK_TRY(launch_check("k_cm_rowcount"));
         chunks.push_back(ch);
     }
+    dense_ms += tdense.ms();
+
+    // 5. row pointers
+    int64_t *rst64;
+    GK_TRY(T.alloc(&rst64, m + 1));
+    {
+        size_t tmp = 0;
+        GK_HIP(rocprim::exclusive_scan(nullptr, tmp, cnt, rst64, (int64_t)0, (size_t)(m + 1),
+                                       rocprim::plus<int64_t>(), ctx->stream));
+        char *t;
+        GK_TRY(T.alloc(&t, (int64_t)tmp));
+        GK_HIP(rocprim::exclusive_scan(t, tmp, cnt, rst64, (int64_t)0, (size_t)(m + 1), rocprim::plus<int64_t>(),
+                                       ctx->stream));
+    }
     int64_t nz = 0;
-    for (const Chunk &ch : chunks) nz += ch.n;
+    GK_HIP(d2h(ctx, &nz, rst64 + m, sizeof nz));
     if (nz > INT32_MAX) {
         set_error("C<not M> has %lld entries (int32 CSR limit)", (long long)nz);
         return GK_ERANGE;
@@ -366,8 +663,8 @@ extern "C" int gk_masked_spgemm_cmask_dev(gk_ctx *ctx, const gk_dcsr *A, const g
     // the result (owned by the returned handle)
     auto *d = new gk_dcsr();
     d->ctx = ctx;
-    d->m = M->m;
-    d->n = M->n;
+    d->m = m;
+    d->n = n;
     d->nz = nz;
     d->owns = true;
     d->sorted = true;
@@ -376,44 +673,50 @@ extern "C" int gk_masked_spgemm_cmask_dev(gk_ctx *ctx, const gk_dcsr *A, const g
         delete d;
         return rc;
     };
-    if (hipMalloc(&d->rst, 4 * (size_t)(m + 1)) != hipSuccess || hipMalloc(&d->col, 4 * (size_t)std::max<int64_t>(nz, 1)) != hipSuccess ||
+    if (hipMalloc(&d->rst, 4 * (size_t)(m + 1)) != hipSuccess ||
+        hipMalloc(&d->col, 4 * (size_t)std::max<int64_t>(nz, 1)) != hipSuccess ||
         hipMalloc(&d->val, 4 * (size_t)std::max<int64_t>(nz, 1)) != hipSuccess) {
         set_error("hipMalloc of C<not M> failed");
         return fail(GK_ENOMEM);
     }
-    int rc;
-    int64_t at = 0;
-    if (m == 0) k_cm_rowptr<uint32_t><<<1, 64, 0, ctx->stream>>>(0, 0, 0, cb, nullptr, 0, 0, d->rst);
+    k_cm_rst32<<<grid_of(ctx, m + 1, 256), 256, 0, ctx->stream>>>(m, rst64, d->rst);
+    // 6. the entries to their places: sorted rows, then dense rows (second pass)
     for (const Chunk &ch : chunks) {
-        const int gr = grid_of(ctx, ch.r1 - ch.r0, 256), gc = grid_of(ctx, ch.n, 256);
-        if (ch.wide) {
-            const uint64_t *k = (const uint64_t *)ch.key;
-            k_cm_rowptr<uint64_t><<<gr, 256, 0, ctx->stream>>>(ch.r0, ch.r1, m, ch.cb, k, ch.n, at, d->rst);
-            if (ch.n) k_cm_cols<uint64_t><<<gc, 256, 0, ctx->stream>>>(ch.n, ch.cb, k, d->col + at);
-        } else {
-            const uint32_t *k = (const uint32_t *)ch.key;
-            k_cm_rowptr<uint32_t><<<gr, 256, 0, ctx->stream>>>(ch.r0, ch.r1, m, ch.cb, k, ch.n, at, d->rst);
-            if (ch.n) k_cm_cols<uint32_t><<<gc, 256, 0, ctx->stream>>>(ch.n, ch.cb, k, d->col + at);
-        }
-        if (ch.n &&
-            hipMemcpyAsync(d->val + at, ch.val, 4 * (size_t)ch.n, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess)
-            return fail(GK_EHIP);
-        at += ch.n;
+        const int g = grid_of(ctx, ch.n, 256);
+        if (ch.wide)
+            k_cm_place<uint64_t><<<g, 256, 0, ctx->stream>>>(ch.n, ch.r0, cb, (const uint64_t *)ch.key, ch.val, first,
+                                                              rst64, d->col, d->val);
+        else
+            k_cm_place<uint32_t><<<g, 256, 0, ctx->stream>>>(ch.n, ch.r0, cb, (const uint32_t *)ch.key, ch.val, first,
+                                                              rst64, d->col, d->val);
     }
-    if ((rc = launch_check("k_cm_rowptr")) != GK_OK) return fail(rc);
+    int rc;
+    if ((rc = launch_check("k_cm_place")) != GK_OK) return fail(rc);
+    if (npairs) {
+        tdense.start();
+        const int g = (int)std::min<int64_t>(npairs, (int64_t)ctx->num_cus);
+        k_cm_dense<true><<<g, kDenseThreads, 0, ctx->stream>>>(
+            npairs, windows, wlog, n, drows, A->rst, A->col, A->val, B->rst, B->col, B->val, M->rst, M->col, nullptr,
+            poff, d->rst, d->col, d->val);
+        if ((rc = launch_check("k_cm_dense<write>")) != GK_OK) return fail(rc);
+        tdense.stop();
+    }
     total.stop();
     if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
         set_error("C<not M> failed on the device");
         return fail(GK_EHIP);
     }
+    if (npairs) dense_ms += tdense.ms();
     gk_timing &tm = ctx->last;
     tm = gk_timing{};
     tm.total_ms = total.ms();
     tm.launches = runs;
-    tm.flop = (double)hoff[m];
+    tm.iterations = ndense;
+    tm.flop = (double)terms_all;
     tm.hits = (double)nz;
     tm.kernel_ms = expand_ms;
     tm.select_ms = sort_ms;
+    tm.acc_ms[0] = dense_ms;
     // plus, per A entry, its column (and value) and B's two row pointers;
     // the mask rows and the row offsets once
     tm.bytes_alg = bytes + (double)A->nz * (4 + (A->val ? 4 : 0) + 8) + (double)M->nz * 4 + 16.0 * (double)m;

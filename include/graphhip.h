@@ -247,11 +247,18 @@ int gk_masked_spgemm_dev(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B,
  * rows in column order.  The pattern is not known beforehand, so *C is a
  * library-built device CSR with values (gk_dcsr_dims, gk_dcsr_download,
  * gk_dcsr_download_val; freed with gk_dcsr_free).  M needs sorted rows.
- * gk_last_timing afterwards: kernel_ms = the term expansion, select_ms =
- * sort and reduce, launches = row chunks, flop = product terms, hits =
- * entries of C, bytes_alg = the expansion's algorithmic bytes.            */
+ * Rows go one of two ways (the result does not depend on it): sorted
+ * (product terms expanded, radix-sorted and summed) or dense (a bitmap and
+ * sums per window of columns in LDS; needs B's rows sorted when C is wider
+ * than one window).  dense_rows: GK_CMASK_AUTO picks per row by its terms
+ * against its A entries and C's width, GK_CMASK_SORTED and GK_CMASK_DENSE
+ * force one way where allowed.  gk_last_timing afterwards: kernel_ms = the
+ * term expansion, select_ms = sort and reduce, acc_ms[0] = the dense
+ * passes, launches = row chunks, iterations = dense rows, hits = entries of
+ * C, bytes_alg = the expansion's algorithmic bytes.                       */
+enum gk_cmask_rows { GK_CMASK_AUTO = -1, GK_CMASK_SORTED = 0, GK_CMASK_DENSE = 1 };
 int gk_masked_spgemm_cmask_dev(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B,
-                               const gk_dcsr *M, gk_dcsr **C);
+                               const gk_dcsr *M, int32_t dense_rows, gk_dcsr **C);
 int gk_dcsr_download_val(gk_ctx *ctx, const gk_dcsr *d, int32_t *val);
 
 /* ---- triangle counting C<L> = L*L, sum of C -----------------------------

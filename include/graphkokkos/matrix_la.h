@@ -102,7 +102,8 @@ void complement_product(const spMtx<T> &A, const spMtx<T> &B, const spMtx<U> &M,
     check(gk_dcsr_upload(context(), &a, &da.d), "gk_dcsr_upload");
     check(gk_dcsr_upload(context(), &b, &db.d), "gk_dcsr_upload");
     check(gk_dcsr_upload(context(), &cm.csr, &dm.d), "gk_dcsr_upload");
-    check(gk_masked_spgemm_cmask_dev(context(), da.d, db.d, dm.d, &dc.d), "gk_masked_spgemm_cmask_dev");
+    check(gk_masked_spgemm_cmask_dev(context(), da.d, db.d, dm.d, GK_CMASK_AUTO, &dc.d),
+          "gk_masked_spgemm_cmask_dev");
     int64_t m = 0, n = 0, nz = 0;
     check(gk_dcsr_dims(dc.d, &m, &n, &nz), "gk_dcsr_dims");
     spMtx<T> R((size_t)m, (size_t)n, (size_t)nz);
