@@ -520,12 +520,12 @@ def main() -> int:
     if not a.no_cpu:
         from oracle import oracle as O
         th = O.threads()
-        Gs = HostGraph.rmat(14, 16, seed=a.seed, **RMAT)
+        Gs = HostGraph.rmat(15, 16, seed=a.seed, **RMAT)
         t0 = time.perf_counter()
         O.truss_decompose(Gs.A)  # the reference loop: S<C>=C*C (parallel MSA) + prune, per k
         dt = time.perf_counter() - t0
         cpu = {"value": Gs.edges / dt, "unit": "edges/s", "cores": th, "kind": "port",
-               "sample": f"full truss decomposition of R-MAT scale 14 ef16 (same generator, |E|={Gs.edges}) "
+               "sample": f"full truss decomposition of R-MAT scale 15 ef16 (same generator, |E|={Gs.edges}) "
                          f"by the reference loop (parallel MSA product + prune per k) in {dt:.2f}s; "
                          "the per-edge CPU cost grows with scale, so this overstates the CPU rate at scale 20"}
         log(f"cpu baseline ktruss: {cpu['value']:.3g} edges/s on {th} threads")
