@@ -103,6 +103,8 @@ _SIGS = {
     "gk_masked_spgemm_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                        C.POINTER(GkOpts)]),
     "gk_tc_count": (C.c_int, [C.c_void_p, C.POINTER(GkCsr), u64p, C.POINTER(GkOpts)]),
+    "gk_tc_vertex": (C.c_int, [C.c_void_p, C.POINTER(GkCsr), i32p, C.POINTER(GkOpts)]),
+    "gk_tc_vertex_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(GkOpts)]),
     "gk_tc_count_dev": (C.c_int, [C.c_void_p, C.c_void_p, u64p, C.POINTER(GkOpts)]),
     "gk_tc_rows_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(GkOpts)]),
     "gk_ktruss_support": (C.c_int, [C.c_void_p, C.POINTER(GkCsr), u32p, C.POINTER(GkOpts)]),

@@ -27,7 +27,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import GkCsr, GkOpts, GkTiming, check, default_opts, lib
-from .graph import SpMtx, _ptr, extract_lower_triangle
+from .graph import SpMtx, _ptr, extract_lower_triangle, transpose
 
 
 class DeviceCsr:
@@ -223,6 +223,15 @@ class Context:
             for d in (dA, dB, dM):
                 d.free()
 
+    def tc_vertex(self, A: SpMtx, opts=None) -> np.ndarray:
+        """(row sums of SQ<A> = A*A) >> 1 in int32: triangles through each
+        vertex of a symmetric pattern A (A's rows sorted)."""
+        out = np.zeros(max(A.m, 1), np.int32)
+        a = A.gk()
+        o = self._opts(opts)
+        check(lib().gk_tc_vertex(self.handle, C.byref(a), _ptr(out), C.byref(o)), "gk_tc_vertex")
+        return out[:A.m]
+
     def tc_count(self, L: SpMtx | DeviceCsr, opts=None) -> int:
         o = self._opts(opts)
         cnt = C.c_uint64(0)
@@ -393,6 +402,36 @@ def triangle_counting_masked_sandia(A: SpMtx, matrixMult: MxmOp = mxmm_msa, isPa
     C_ = SpMtx()
     matrixMult(isParallel, L, L, L, C_)
     return int(np.asarray(C_.Val, np.int64).sum())
+
+
+def _rows_sorted(A: SpMtx) -> bool:
+    rows = np.repeat(np.arange(A.m), np.diff(A.Rst))
+    same = rows[1:] == rows[:-1]
+    return bool(np.all(A.Col[1:][same] > A.Col[:-1][same]))
+
+
+def triangle_counting_vertex(A: SpMtx, matrixMult: MxmOp = mxmm_msa, isParallel: bool = True) -> np.ndarray:
+    """Triangles through every vertex: (row sums of SQ<A> = A*A) >> 1 in
+    int32 arithmetic (src/main.cpp:16-39).  With this library's product and
+    sorted rows the sums are taken on the device (gk_tc_vertex)."""
+    if matrixMult in (mxmm_msa, mxmm_mca) and A.m == A.n and _rows_sorted(A):
+        return default_context().tc_vertex(A)
+    SQ = SpMtx()
+    matrixMult(isParallel, A, A, A, SQ)
+    rows = np.repeat(np.arange(SQ.m), np.diff(SQ.Rst))
+    sums = np.zeros(SQ.m, np.int64)
+    np.add.at(sums, rows, np.asarray(SQ.Val, np.int64))
+    return ((sums & 0xFFFFFFFF).astype(np.uint32).view(np.int32) >> 1).astype(np.int32)
+
+
+def triangle_counting_masked_lu(A: SpMtx, matrixMult: MxmOp = mxmm_msa, isParallel: bool = True) -> int:
+    """sum(C<A> = L*U) / 2 with L the strictly lower part of A and U its
+    transpose (src/main.cpp:42-66)."""
+    L = extract_lower_triangle(A)
+    U = transpose(L)
+    C_ = SpMtx()
+    matrixMult(isParallel, L, U, A, C_)
+    return int(np.asarray(C_.Val, np.int64).sum()) >> 1
 
 
 def k_truss(A: SpMtx, k: int, matrixMult: MxmOp = mxmm_msa, isParallel: bool = True) -> SpMtx:

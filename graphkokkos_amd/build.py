@@ -25,7 +25,8 @@ BUILD_DIR = PKG / "_build"
 DROPIN_SRC = ROOT / "tests" / "cpp" / "dropin_check.cpp"
 DROPIN_BIN = ROOT / "tests" / "cpp" / "_build" / "dropin_check"
 
-HIP_SOURCES = ["gk_runtime.hip", "spgemm.hip", "ktruss.hip", "bc.hip", "selector.hip", "batch.hip", "graph_device.hip", "cmask.hip"]
+HIP_SOURCES = ["gk_runtime.hip", "spgemm.hip", "ktruss.hip", "bc.hip", "selector.hip", "batch.hip", "graph_device.hip", "cmask.hip",
+               "tc_vertex.hip"]
 HOST_SOURCES = ["host_graph.cpp"]
 ARCH = os.environ.get("GK_OFFLOAD_ARCH", "gfx950")
 

@@ -268,6 +268,15 @@ int gk_tc_count_dev(gk_ctx *ctx, const gk_dcsr *L, uint64_t *count, const gk_opt
 /* per-row triangle counts (sum over the mask row), length L.m */
 int gk_tc_rows_dev(gk_ctx *ctx, const gk_dcsr *L, uint64_t *d_rowcount, const gk_opts *o);
 
+/* ---- triangles through every vertex ---------------------------------------
+ * triangle_counting_vertex (src/main.cpp:16-39): counts[i] = (the int sum
+ * of row i of SQ<A> = A*A) >> 1, with A's values; A square with sorted rows
+ * (it is the mask).  For a symmetric pattern this is the number of
+ * triangles through vertex i.  _dev: A resident, counts in device memory
+ * (length A.m).                                                           */
+int gk_tc_vertex(gk_ctx *ctx, const gk_csr *A, int32_t *counts, const gk_opts *o);
+int gk_tc_vertex_dev(gk_ctx *ctx, const gk_dcsr *A, int32_t *d_counts, const gk_opts *o);
+
 /* ---- supports S<A> = A*A at A's pattern (symmetric pattern A) ----------- */
 int gk_ktruss_support(gk_ctx *ctx, const gk_csr *A, uint32_t *support, const gk_opts *o);
 

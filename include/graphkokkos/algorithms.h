@@ -1,7 +1,9 @@
 // graphkokkos/algorithms.h -- the graph drivers of GraphKokkos's
 // src/main.cpp in a header: triangle_counting_masked_sandia
-// (src/main.cpp:69-90) and k_truss (src/main.cpp:93-141) with their
-// signatures, plus the full truss decomposition and the support product.
+// (src/main.cpp:69-90), k_truss (src/main.cpp:93-141) and the two unwired
+// variants triangle_counting_vertex and triangle_counting_masked_lu
+// (src/main.cpp:16-66) with their signatures, plus the full truss
+// decomposition and the support product.
 //
 // Given one of this library's GPU products (mxmm_gpu, mxmm_msa, mxmm_mca)
 // the drivers run the fused GPU entry points (gk_tc_count, gk_ktruss),
@@ -13,6 +15,7 @@
 #include <chrono>
 #include <cstdint>
 #include <iostream>
+#include <memory>
 #include <vector>
 
 #include "matrix_la.h"
@@ -73,6 +76,48 @@ inline int64_t triangle_counting_masked_sandia(const spMtx<int> &A, mxmOp<int> m
         matrixMult(isParallel, L, L, L, C);
         for (size_t p = 0; p < C.nz; ++p) num_of_tr += C.Val(p);
     }
+    std::cout << "Time:       " << gk::detail::ms_since(t0) << " ms\n";
+    std::cout << "Triangles:  " << num_of_tr << '\n';
+    return num_of_tr;
+}
+
+// Triangles through every vertex: the int sum of each row of SQ<A> = A*A,
+// shifted right once (src/main.cpp:16-39).  As in the reference the result
+// is new int[A.m], which the caller deletes, and only "Time:" is printed.
+// Given one of this library's products and sorted rows, gk_tc_vertex takes
+// the sums on the device and SQ never reaches the host.
+inline int *triangle_counting_vertex(const spMtx<int> &A, mxmOp<int> matrixMult = mxmm_gpu<int, int>,
+                                     bool isParallel = true) {
+    std::unique_ptr<int[]> nums_of_tr(new int[A.m]);
+    const auto t0 = std::chrono::steady_clock::now();
+    if (gk::detail::is_gpu_product(matrixMult) && A.m == A.n && gk::detail::rows_sorted(A)) {
+        const gk_csr a = A.csr();
+        gk::check(gk_tc_vertex(gk::context(), &a, nums_of_tr.get(), &gk::options()), "gk_tc_vertex");
+    } else {
+        spMtx<int> SQ;
+        matrixMult(isParallel, A, A, A, SQ);
+        for (size_t i = 0; i < A.m; ++i) {
+            uint32_t s = 0;  // int sum, wrapping
+            for (int j = SQ.Rst(i); j < SQ.Rst(i + 1); ++j) s += (uint32_t)SQ.Val(j);
+            nums_of_tr[i] = (int)s >> 1;
+        }
+    }
+    std::cout << "Time:       " << gk::detail::ms_since(t0) << '\n';
+    return nums_of_tr.release();
+}
+
+// Number of triangles as sum(C<A> = L*U) / 2, L the strictly lower part of
+// A and U its transpose (src/main.cpp:42-66).  The product is matrixMult.
+inline int64_t triangle_counting_masked_lu(const spMtx<int> &A, mxmOp<int> matrixMult = mxmm_gpu<int, int>,
+                                           bool isParallel = true) {
+    const spMtx<int> L = extract_lower_triangle(A);
+    const spMtx<int> U = transpose(L);
+    spMtx<int> C;
+    const auto t0 = std::chrono::steady_clock::now();
+    matrixMult(isParallel, L, U, A, C);
+    int64_t num_of_tr = 0;
+    for (size_t p = 0; p < C.nz; ++p) num_of_tr += C.Val(p);
+    num_of_tr >>= 1;
     std::cout << "Time:       " << gk::detail::ms_since(t0) << " ms\n";
     std::cout << "Triangles:  " << num_of_tr << '\n';
     return num_of_tr;

@@ -8,6 +8,7 @@
 // With "loaded" only the products on the pattern as loaded (rows in file
 // order, duplicates kept) and BC run: files whose rows are not sorted have
 // no well-defined symmetric pattern for the truss drivers.
+#include <algorithm>
 #include <cstdio>
 #include <fstream>
 #include <string>
@@ -84,6 +85,16 @@ int main(int argc, char **argv) {
         const int64_t tc_fused = triangle_counting_masked_sandia(S, msa, true);
         const int64_t tc_loop = triangle_counting_masked_sandia(S, wrapped_product, true);
         f << "\"tc_fused\": " << tc_fused << ",\n\"tc_loop\": " << tc_loop << ",\n";
+
+        // the unwired drivers: triangles per vertex and sum(C<A> = L*U) / 2
+        int *tv = triangle_counting_vertex(S, msa, true);
+        int *tv_loop = triangle_counting_vertex(S, wrapped_product, true);
+        put_array(f, "tc_vertex", std::vector<int>(tv, tv + S.m));
+        f << "\"tc_vertex_loop_equal\": " << (std::equal(tv, tv + S.m, tv_loop) ? 1 : 0) << ",\n";
+        delete[] tv;
+        delete[] tv_loop;
+        f << "\"tc_lu\": " << triangle_counting_masked_lu(S, msa, true) << ",\n";
+        f << "\"tc_lu_loop\": " << triangle_counting_masked_lu(S, wrapped_product, true) << ",\n";
 
         const spMtx<int> K1 = k_truss(S, k, mca, true);
         const spMtx<int> K2 = k_truss(S, k, wrapped_product, true);

@@ -4,7 +4,8 @@ tests/cpp/dropin_check is a client of libgraphhip.so written the way
 GraphKokkos's src/main.cpp uses its own headers: it loads a Matrix Market
 file into spMtx<int>, builds the adjacency and symmetric patterns, and runs
 mxmm_msa / mxmm_mca through mxmOp<int> pointers, the complement-mask
-product mxmm_msa_cmask,
+product mxmm_msa_cmask, the unwired drivers triangle_counting_vertex and
+triangle_counting_masked_lu,
 triangle_counting_masked_sandia and k_truss (both with the library's
 product, which takes the fused GPU entry points, and with a wrapped product,
 which takes the reference's loop), the truss decomposition, the supports and
@@ -93,6 +94,11 @@ def check_against_oracle(res, mtx, k, batch, loaded_only=False):
     L = extract_lower_triangle(S)
     t = O.tc_sandia(L)
     assert res["tc_fused"] == t and res["tc_loop"] == t
+    assert res["tc_lu"] == t and res["tc_lu_loop"] == t
+    sup = O.support(S)
+    vrows = np.repeat(np.arange(S.m), np.diff(S.Rst))
+    assert np.array_equal(res["tc_vertex"], np.bincount(vrows, weights=sup, minlength=S.m).astype(np.int64) // 2)
+    assert res["tc_vertex_loop_equal"] == 1
     keep, _ = O.ktruss(S, k)
     rows = np.repeat(np.arange(S.m), np.diff(S.Rst))
     want_rst = np.concatenate([[0], np.cumsum(np.bincount(rows[keep], minlength=S.m))])

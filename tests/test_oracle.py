@@ -53,6 +53,19 @@ def test_oracle_matches_golden(name):
     np.testing.assert_allclose(bc, z["bc"], rtol=1e-12, atol=1e-12)
 
 
+@pytest.mark.parametrize("name", GOLDEN_NAMES)
+def test_vertex_triangles_from_supports(name):
+    """Triangles through a vertex = half the supports of its edges = diag(A^3)/2:
+    the expectation the per-vertex driver is checked against."""
+    A, z = load_golden(name)
+    rows = np.repeat(np.arange(A.m), np.diff(A.Rst))
+    half = np.bincount(rows, weights=O.support(A), minlength=A.m).astype(np.int64) // 2
+    D = np.zeros((A.m, A.n), np.int64)
+    D[rows, A.Col] = 1
+    assert np.array_equal(half, np.diag(D @ D @ D) // 2)
+    assert int(half.sum()) == 3 * int(z["triangles"])
+
+
 # ---- closed forms -----------------------------------------------------------
 
 @pytest.mark.parametrize("n", [3, 5, 8, 13])
