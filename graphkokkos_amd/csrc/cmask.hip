@@ -377,18 +377,51 @@ __global__ void k_cm_rst32(int64_t m, const int64_t *__restrict__ rst64, int32_t
         rst[i] = (int32_t)rst64[i];
 }
 
+// Work buffers of one product.  They come from the context's grow-only
+// cache (the n-th allocation of a call reuses the n-th buffer, freed by
+// gk_finalize), so repeated products allocate nothing once warm; a call
+// made while another holds the cache allocates and frees its own.
 struct Mem {
-    std::vector<void *> p;
+    gk_ctx *ctx;
+    bool cached;
+    size_t next = 0;
+    std::vector<void *> own;
+    explicit Mem(gk_ctx *c) : ctx(c), cached(!c->cmask_cache_busy) {
+        if (cached) ctx->cmask_cache_busy = true;
+    }
     ~Mem() {
-        for (void *x : p) (void)hipFree(x);
+        if (cached) ctx->cmask_cache_busy = false;
+        for (void *x : own) (void)hipFree(x);
     }
     template <class T>
     int alloc(T **out, int64_t n) {
-        if (hipMalloc((void **)out, sizeof(T) * (size_t)(n > 0 ? n : 1)) != hipSuccess) {
-            set_error("hipMalloc of %lld bytes for C<not M> failed", (long long)(sizeof(T) * n));
+        const size_t bytes = sizeof(T) * (size_t)(n > 0 ? n : 1);
+        if (cached) {
+            auto &cache = ctx->cmask_cache;
+            if (next == cache.size()) cache.emplace_back(nullptr, 0);
+            auto &slot = cache[next++];
+            if (slot.second < bytes) {
+                if (slot.first) {
+                    (void)hipStreamSynchronize(ctx->stream);
+                    (void)hipFree(slot.first);
+                    slot = {nullptr, 0};
+                }
+                const size_t want = bytes + bytes / 8;  // headroom against regrowth
+                if (hipMalloc(&slot.first, want) != hipSuccess) {
+                    slot = {nullptr, 0};
+                    set_error("hipMalloc of %zu bytes for C<not M> failed", want);
+                    return GK_ENOMEM;
+                }
+                slot.second = want;
+            }
+            *out = (T *)slot.first;
+            return GK_OK;
+        }
+        if (hipMalloc((void **)out, bytes) != hipSuccess) {
+            set_error("hipMalloc of %zu bytes for C<not M> failed", bytes);
             return GK_ENOMEM;
         }
-        p.push_back(*out);
+        own.push_back(*out);
         return GK_OK;
     }
 };
@@ -488,7 +521,7 @@ extern "C" int gk_masked_spgemm_cmask_dev(gk_ctx *ctx, const gk_dcsr *A, const g
     const int64_t m = A->m, n = M->n, anz = A->nz;
     Timer total(ctx), tdense(ctx);
     total.start();
-    Mem T;
+    Mem T(ctx);
     // dense windows: W = the power of two >= n, at most 2^kDenseWindowLog
     int wlog = 5;
     while (wlog < kDenseWindowLog && ((int64_t)1 << wlog) < n) ++wlog;

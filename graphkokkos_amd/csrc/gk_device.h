@@ -67,6 +67,9 @@ struct gk_ctx {
     // by gk_finalize): the n-th allocation of a run reuses the n-th buffer.
     std::vector<std::pair<void *, size_t>> truss_cache;
     bool truss_cache_busy = false;
+    // The same for the work buffers of the complement-mask product.
+    std::vector<std::pair<void *, size_t>> cmask_cache;
+    bool cmask_cache_busy = false;
 };
 
 struct gk_dcsr {

@@ -276,6 +276,8 @@ extern "C" void gk_finalize(gk_ctx *ctx) {
     if (ctx->dense_map) (void)hipFree(ctx->dense_map);
     for (auto &b : ctx->truss_cache)
         if (b.first) (void)hipFree(b.first);
+    for (auto &b : ctx->cmask_cache)
+        if (b.first) (void)hipFree(b.first);
     if (ctx->model) model_free(ctx->model);
     for (int i = 0; i < gk_ctx::kAux; ++i) {
         if (ctx->aux[i]) (void)hipStreamSynchronize(ctx->aux[i]);
