@@ -186,6 +186,7 @@ def run_cmask(ctx, scale: int, seed: int, steps: int, no_cpu: bool) -> dict:
     assert nz == nz_sorted, "the two ways of the rows disagree on C's size"
     terms = t["flop"]
     exp_gbs = t["bytes_alg"] / 1e9 / (t["kernel_ms"] / 1e3) if t["kernel_ms"] > 0 else 0.0
+    dense_gbs = t["acc_bytes"][0] / 1e9 / (t["acc_ms"][0] / 1e3) if t["acc_ms"][0] > 0 else 0.0
     out = {"workload": f"complement-mask product C<not A> = A*A (two-hop non-adjacent pairs with their common "
                        f"neighbour counts), R-MAT scale {scale} ef16, pattern values",
            "vertices": G.n, "edges": G.edges, "terms": terms, "entries": nz,
@@ -194,6 +195,11 @@ def run_cmask(ctx, scale: int, seed: int, steps: int, no_cpu: bool) -> dict:
            "dense_rows": t["iterations"], "dense_ms": t["acc_ms"][0],
            "sorted_rows_only": {"ms": wall_sorted * 1e3, "value": terms / wall_sorted,
                                 "expand_ms": t_sorted["kernel_ms"], "sort_reduce_ms": t_sorted["select_ms"]},
+           "roofline": {"bound": "hbm", "achieved": round(dense_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(dense_gbs / HBM_PEAK_GBS, 5),
+                        "traffic": measured_traffic("k_cm_dense", f"cmask-rmat{scale}"),
+                        "kernel": "k_cm_dense (count + write passes)", "bytes_per_launch": t["acc_bytes"][0] / 2,
+                        "avg_launch_ms": t["acc_ms"][0] / 2, "launches": 2},
            "expand_roofline": {"bound": "hbm", "achieved": round(exp_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                "frac": round(exp_gbs / HBM_PEAK_GBS, 5), "traffic": None,
                                "kernel": "k_cm_expand", "bytes": t["bytes_alg"]}}

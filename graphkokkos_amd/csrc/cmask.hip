@@ -750,6 +750,16 @@ extern "C" int gk_masked_spgemm_cmask_dev(gk_ctx *ctx, const gk_dcsr *A, const g
     tm.kernel_ms = expand_ms;
     tm.select_ms = sort_ms;
     tm.acc_ms[0] = dense_ms;
+    tm.acc_launches[0] = npairs ? 2 : 0;
+    {
+        // dense passes: per term B's column read in both passes (and its
+        // value in the second), per entry its column and value written once
+        // (the A entries and mask rows, a few per cent of this, left out)
+        int64_t sorted_nz = 0;
+        for (const Chunk &ch : chunks) sorted_nz += ch.n;
+        const double dterms = (double)(terms_all - hoff[m]);
+        tm.acc_bytes[0] = dterms * (4 + 4 + (B->val ? 4 : 0)) + (double)(nz - sorted_nz) * 8;
+    }
     // plus, per A entry, its column (and value) and B's two row pointers;
     // the mask rows and the row offsets once
     tm.bytes_alg = bytes + (double)A->nz * (4 + (A->val ? 4 : 0) + 8) + (double)M->nz * 4 + 16.0 * (double)m;

@@ -253,9 +253,10 @@ int gk_masked_spgemm_dev(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B,
  * than one window).  dense_rows: GK_CMASK_AUTO picks per row by its terms
  * against its A entries and C's width, GK_CMASK_SORTED and GK_CMASK_DENSE
  * force one way where allowed.  gk_last_timing afterwards: kernel_ms = the
- * term expansion, select_ms = sort and reduce, acc_ms[0] = the dense
- * passes, launches = row chunks, iterations = dense rows, hits = entries of
- * C, bytes_alg = the expansion's algorithmic bytes.                       */
+ * term expansion, select_ms = sort and reduce, acc_ms[0] / acc_bytes[0] =
+ * the dense passes' time and algorithmic bytes, launches = row chunks,
+ * iterations = dense rows, flop = product terms, hits = entries of C,
+ * bytes_alg = the expansion's algorithmic bytes.                          */
 enum gk_cmask_rows { GK_CMASK_AUTO = -1, GK_CMASK_SORTED = 0, GK_CMASK_DENSE = 1 };
 int gk_masked_spgemm_cmask_dev(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B,
                                const gk_dcsr *M, int32_t dense_rows, gk_dcsr **C);
