@@ -39,7 +39,8 @@ class GkOpts(C.Structure):
                 ("split_flop", C.c_int32), ("block_threads", C.c_int32), ("verbose", C.c_int32),
                 ("small_max_mask", C.c_int32), ("blocks_small", C.c_int32), ("blocks_long", C.c_int32),
                 ("blocks_mca", C.c_int32), ("blocks_global", C.c_int32), ("batch_blocks", C.c_int32),
-                ("batch_order", C.c_int32), ("hub_window", C.c_int32), ("learned_margin", C.c_int32)]
+                ("batch_order", C.c_int32), ("hub_window", C.c_int32), ("learned_margin", C.c_int32),
+                ("hub_probe", C.c_int32)]
 
 
 class GkTiming(C.Structure):

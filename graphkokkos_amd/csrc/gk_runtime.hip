@@ -235,8 +235,9 @@ extern "C" void gk_opts_default(gk_opts *o) {
     o->blocks_global = 4;
     o->batch_blocks = 8;
     o->batch_order = 0;
-    o->hub_window = 16384;
+    o->hub_window = 32768;
     o->learned_margin = 250;
+    o->hub_probe = 4;
 }
 
 extern "C" int gk_init(int device, gk_ctx **out) {

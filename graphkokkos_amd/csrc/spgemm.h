@@ -110,6 +110,11 @@ struct Prob {
     // words at hub + k * hub_w (bit c set iff column c is in the row).
     const uint32_t *__restrict__ hub;
     int32_t hub_n, hub_w;
+    // A's rows are M's rows (C<L> = L*L): the mask columns below an A entry
+    // are the row's entries before it, so a window task may test them as
+    // single bits of the entry's hub row (hub_choice) where `probe` times
+    // their number is below the row's words and entries; 0 = never
+    int32_t probe;
 };
 
 // Whether A entry k of a window task reads B row k as bitmap words (the
@@ -117,6 +122,28 @@ struct Prob {
 // bits of them).
 __host__ __device__ __forceinline__ bool hub_row(int32_t hub_n, int32_t k, int32_t len) {
     return k < hub_n && ((k + 31) >> 5) < len;
+}
+
+// How a window task reads B row k of len entries, `below` mask columns
+// lying below k: streamed entries (unchanged), or the row's bitmap words
+// (hlen words at word offset hbs; hub_row), or, where the mask has fewer
+// columns below k than both (weighted by P.probe), single-bit tests of
+// those columns in the row's bitmap (plen tests of the row at word offset
+// pbs).  A moved row gets len = 0.
+__device__ __forceinline__ void hub_choice(const Prob &P, int32_t k, int32_t below, int32_t &len, int32_t &hlen,
+                                           int32_t &hbs, int32_t &plen, int32_t &pbs) {
+    if (k >= P.hub_n || len == 0) return;
+    const int32_t words = (k + 31) >> 5;
+    const int32_t cost = P.probe * below;  // a test is a gathered word, a word or entry a streamed one
+    if (P.probe && cost < len && cost < words) {
+        plen = below;
+        pbs = k * P.hub_w;
+        len = 0;
+    } else if (words < len) {
+        hlen = words;
+        hbs = k * P.hub_w;
+        len = 0;
+    }
 }
 
 // Launch shape of one product's class kernels and the scratch of the

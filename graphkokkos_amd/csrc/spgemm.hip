@@ -283,34 +283,34 @@ __device__ __forceinline__ int32_t build_prefix(const Prob &P, int32_t a0, int32
     return total;
 }
 
-// The same with hub bitmaps: A entries whose B row is read as bitmap words
-// (hub_row) get length 0 in pre/bst and their word count in hpre/hbst
-// (hbst[x] = word offset of the row in P.hub minus hpre[x]); the others
-// the reverse.  Returns (streamed entries, bitmap words).
+// The same with hub bitmaps: each A entry's B row is streamed, read as
+// bitmap words or bit-tested (hub_choice; mb = the mask row's first entry).
+// pre/bst as build_prefix for the streamed rows (0 for the others);
+// hpre[x] = bitmap words before A entry a0+x, hbst[x] = its row's word
+// offset in P.hub minus hpre[x]; ppre[x] = bit tests before it, pofs[x] =
+// its row's word offset.  Returns (streamed entries, words, tests).
 template <int G>
-__device__ __forceinline__ int2 build_prefix_hub(const Prob &P, int32_t a0, int32_t a1, int32_t *pre, int32_t *bst,
-                                                 int32_t *hpre, int32_t *hbst, int gl) {
-    int32_t total = 0, htotal = 0;
+__device__ __forceinline__ int3 build_prefix_hub(const Prob &P, int32_t a0, int32_t a1, int32_t mb, int32_t *pre,
+                                                 int32_t *bst, int32_t *hpre, int32_t *hbst, int32_t *ppre,
+                                                 int32_t *pofs, int gl) {
+    int32_t total = 0, htotal = 0, ptotal = 0;
     for (int32_t base = a0; base < a1; base += G) {
         const int32_t p = base + gl;
-        int32_t len = 0, bs = 0, hlen = 0, hbs = 0;
+        int32_t len = 0, bs = 0, hlen = 0, hbs = 0, plen = 0, pbs = 0;
         if (p < a1) {
             const int32_t k = P.acol[p];
             bs = P.brst[k];
             len = P.brst[k + 1] - bs;
-            if (hub_row(P.hub_n, k, len)) {
-                hlen = (k + 31) >> 5;
-                hbs = k * P.hub_w;
-                len = 0;
-            }
+            hub_choice(P, k, p - mb, len, hlen, hbs, plen, pbs);
         }
-        int32_t incl = len, hincl = hlen;
+        int32_t incl = len, hincl = hlen, pincl = plen;
 #pragma unroll
         for (int o = 1; o < G; o <<= 1) {
-            const int32_t w = __shfl_up(incl, o, G), hw = __shfl_up(hincl, o, G);
+            const int32_t w = __shfl_up(incl, o, G), hw = __shfl_up(hincl, o, G), pw = __shfl_up(pincl, o, G);
             if (gl >= o) {
                 incl += w;
                 hincl += hw;
+                pincl += pw;
             }
         }
         if (p < a1) {
@@ -318,15 +318,19 @@ __device__ __forceinline__ int2 build_prefix_hub(const Prob &P, int32_t a0, int3
             bst[p - a0] = bs - (total + incl - len);
             hpre[p - a0] = htotal + hincl - hlen;
             hbst[p - a0] = hbs - (htotal + hincl - hlen);
+            ppre[p - a0] = ptotal + pincl - plen;
+            pofs[p - a0] = pbs;
         }
         total += __shfl(incl, G - 1, G);
         htotal += __shfl(hincl, G - 1, G);
+        ptotal += __shfl(pincl, G - 1, G);
     }
     if (gl == 0) {
         pre[a1 - a0] = total;
         hpre[a1 - a0] = htotal;
+        ppre[a1 - a0] = ptotal;
     }
-    return make_int2(total, htotal);
+    return make_int3(total, htotal, ptotal);
 }
 
 // Hit handling common to every accumulator.  q = mask position (0-based in
@@ -472,6 +476,43 @@ __device__ __forceinline__ uint32_t stream_hub(const Prob &P, const int32_t *hpr
     return cnt;
 }
 
+// Triangle counting by bit tests (hub_choice): test j of A entry x is the
+// mask row's column c = mcol[mb + j] (below the entry's column k) against
+// bit c of hub row k at word offset pofs[x]; U tests per lane in flight.
+// Returns the number of hits.
+template <int G, int U>
+__device__ __forceinline__ uint32_t stream_probe(const Prob &P, const int32_t *ppre, const int32_t *pofs,
+                                                 int32_t total, int gl, int32_t mb) {
+    uint32_t cnt = 0;
+    if (total <= 0) return 0;
+    int32_t kk = 0, cur = 0, nxt = ppre[1], base = pofs[0];
+    for (int32_t g0 = gl; g0 < total; g0 += G * U) {
+        int32_t off[U], c[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int32_t g = g0 + G * u;
+            off[u] = -1;
+            c[u] = 0;
+            if (g < total) {
+                while (g >= nxt) {
+                    ++kk;
+                    cur = nxt;
+                    nxt = ppre[kk + 1];
+                    base = pofs[kk];
+                }
+                off[u] = base;
+                c[u] = P.mcol[mb + (g - cur)];
+            }
+        }
+        uint32_t v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = off[u] >= 0 ? P.hub[off[u] + (c[u] >> 5)] : 0u;
+#pragma unroll
+        for (int u = 0; u < U; ++u) cnt += (v[u] >> (c[u] & 31)) & 1u;
+    }
+    return cnt;
+}
+
 template <int MODE>
 __device__ __forceinline__ void finish_kernel(const Prob &P, uint32_t cnt, int cls, int lane) {
     unsigned long long s = dev::wave_sum((unsigned long long)cnt);
@@ -584,39 +625,40 @@ __device__ __forceinline__ int32_t build_prefix_block(const Prob &P, int32_t a0,
 }
 
 // build_prefix_block with hub bitmaps (see build_prefix_hub); wsum holds
-// 2 * NT/64 wave totals.
+// 3 * NT/64 wave totals.
 template <int NT>
-__device__ __forceinline__ int2 build_prefix_block_hub(const Prob &P, int32_t a0, int32_t a1, int32_t *pre,
-                                                       int32_t *bst, int32_t *hpre, int32_t *hbst, int32_t *wsum) {
+__device__ __forceinline__ int3 build_prefix_block_hub(const Prob &P, int32_t a0, int32_t a1, int32_t mb,
+                                                       int32_t *pre, int32_t *bst, int32_t *hpre, int32_t *hbst,
+                                                       int32_t *ppre, int32_t *pofs, int32_t *wsum) {
     static_assert(NT >= kMaxTaskA, "one A entry per thread");
     constexpr int NW = NT / 64;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int32_t p = a0 + tid;
-    int32_t len = 0, bs = 0, hlen = 0, hbs = 0;
+    int32_t len = 0, bs = 0, hlen = 0, hbs = 0, plen = 0, pbs = 0;
     if (p < a1) {
         const int32_t k = P.acol[p];
         bs = P.brst[k];
         len = P.brst[k + 1] - bs;
-        if (hub_row(P.hub_n, k, len)) {
-            hlen = (k + 31) >> 5;
-            hbs = k * P.hub_w;
-            len = 0;
-        }
+        hub_choice(P, k, p - mb, len, hlen, hbs, plen, pbs);
     }
     const int32_t incl = dev::wave_incl_scan(len), hincl = dev::wave_incl_scan(hlen);
+    const int32_t pincl = dev::wave_incl_scan(plen);
     if (lane == 63) {
         wsum[w] = incl;
         wsum[NW + w] = hincl;
+        wsum[2 * NW + w] = pincl;
     }
     __syncthreads();
-    int32_t before = 0, total = 0, hbefore = 0, htotal = 0;
+    int32_t before = 0, total = 0, hbefore = 0, htotal = 0, pbefore = 0, ptotal = 0;
 #pragma unroll
     for (int x = 0; x < NW; ++x) {
-        const int32_t v = wsum[x], hv = wsum[NW + x];
+        const int32_t v = wsum[x], hv = wsum[NW + x], pv = wsum[2 * NW + x];
         before += x < w ? v : 0;
         total += v;
         hbefore += x < w ? hv : 0;
         htotal += hv;
+        pbefore += x < w ? pv : 0;
+        ptotal += pv;
     }
     if (p < a1) {
         const int32_t ex = before + incl - len, hex = hbefore + hincl - hlen;
@@ -624,12 +666,15 @@ __device__ __forceinline__ int2 build_prefix_block_hub(const Prob &P, int32_t a0
         bst[p - a0] = bs - ex;
         hpre[p - a0] = hex;
         hbst[p - a0] = hbs - hex;
+        ppre[p - a0] = pbefore + pincl - plen;
+        pofs[p - a0] = pbs;
     }
     if (tid == 0) {
         pre[a1 - a0] = total;
         hpre[a1 - a0] = htotal;
+        ppre[a1 - a0] = ptotal;
     }
-    return make_int2(total, htotal);
+    return make_int3(total, htotal, ptotal);
 }
 
 // TC and SUPPORT modes.  SUPPORT also needs the mask position of a hit:
@@ -653,7 +698,9 @@ __global__ __launch_bounds__(64 * NW) void k_msa_window(Prob P, int cls) {
     static_assert(!HUB || !SUP, "hub bitmaps count triangles only");
     __shared__ int32_t hpre[HUB ? kMaxTaskA + 1 : 1];
     __shared__ int32_t hbst[HUB ? kMaxTaskA : 1];
-    __shared__ int32_t wsum[2 * NW];
+    __shared__ int32_t ppre[HUB ? kMaxTaskA + 1 : 1];
+    __shared__ int32_t pofs[HUB ? kMaxTaskA : 1];
+    __shared__ int32_t wsum[3 * NW];
     const int tid = threadIdx.x;
     for (int w = tid; w < kWindow / 32; w += NT) bm[w] = 0;
     const int32_t lo = P.bin_off[cls * kFlopBins], hi = P.bin_off[(cls + 1) * kFlopBins];
@@ -690,13 +737,16 @@ __global__ __launch_bounds__(64 * NW) void k_msa_window(Prob P, int cls) {
             while (atomicCAS(&hkey[s], -1, c) != -1) s = (s + 1) & cmask;
             if (SUP) hpos[s] = q;
         }
-        int32_t total, htotal = 0;
+        int32_t total, htotal = 0, ptotal = 0;
         if constexpr (HUB) {
-            int2 tt;
-            if constexpr (NW == 1) tt = build_prefix_hub<64>(P, tk.a0, tk.a1, pre, bst, hpre, hbst, tid);
-            else tt = build_prefix_block_hub<NT>(P, tk.a0, tk.a1, pre, bst, hpre, hbst, wsum);
+            int3 tt;
+            if constexpr (NW == 1)
+                tt = build_prefix_hub<64>(P, tk.a0, tk.a1, mb, pre, bst, hpre, hbst, ppre, pofs, tid);
+            else
+                tt = build_prefix_block_hub<NT>(P, tk.a0, tk.a1, mb, pre, bst, hpre, hbst, ppre, pofs, wsum);
             total = tt.x;
             htotal = tt.y;
+            ptotal = tt.z;
         } else if constexpr (NW == 1) {
             total = build_prefix<64>(P, tk.a0, tk.a1, pre, bst, tid);
         } else {
@@ -704,8 +754,10 @@ __global__ __launch_bounds__(64 * NW) void k_msa_window(Prob P, int cls) {
         }
         __syncthreads();
         uint32_t c = 0;
-        if constexpr (HUB)
+        if constexpr (HUB) {
             if (htotal) c = stream_hub<NT, kWindowU>(P, hpre, hbst, htotal, tid, bm);
+            if (ptotal) c += stream_probe<NT, kWindowU>(P, ppre, pofs, ptotal, tid, mb);
+        }
         c += stream_entries<MODE, NT, kWindowU>(P, tk, mb, pre, bst, total, tid, acc, acnt,
                                                        [&](int32_t b) -> int32_t {
             if (b < kWindow) {
@@ -1384,6 +1436,7 @@ int masked_product(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcs
     GK_TRY(plan_tasks(ctx, A, st, sp, &plan));
     Prob P{};
     GK_TRY(build_hub(ctx, B, hub_n, &P));
+    P.probe = (P.hub && A->rst == M->rst && A->col == M->col) ? std::max(o.hub_probe, 0) : 0;
     sel.stop();
     if (mode == MODE_VALUES || mode == MODE_SUPPORT) GK_TRY(zero_u32(ctx, d_out, M->nz));
     P.arst = A->rst; P.acol = A->col; P.aval = A->val;

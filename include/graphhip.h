@@ -108,11 +108,16 @@ typedef struct gk_opts {
     int32_t hub_window;     /* triangle counting: B rows below this id (<= 32768) are
                              * also kept as bitmaps; a row is read as bitmap words
                              * (window kernels) or single bits (MCA-wave) where that
-                             * reads less than the row; 0 = off, default 16384       */
+                             * reads less than the row; 0 = off, default 32768       */
     int32_t learned_margin; /* learned selector: it leaves the fixed-threshold choice of
                              * a row only where the network predicts that choice to
                              * cost more than 2^(margin/1000) times its best one;
                              * 0 = plain argmin, default 250                         */
+    int32_t hub_probe;      /* triangle counting with hub bitmaps: a window task tests
+                             * single bits of a hub row where hub_probe times the mask
+                             * row's columns below it is less than the row's words and
+                             * entries (a test is a gathered word, the others are
+                             * streamed); 0 = off, default 4                         */
 } gk_opts;
 void gk_opts_default(gk_opts *o);
 

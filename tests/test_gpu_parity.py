@@ -119,12 +119,18 @@ def test_tc_hub_bitmaps(gpu, kind):
         assert gpu.tc_count(d, {"hub_window": hw}) == want, hw
         for acc in ACCS:
             assert gpu.tc_count(d, {**forced(acc), "hub_window": hw}) == want, (hw, acc)
-        # long rows through the block-wide kernel, short rows through the wave kernel
+        # long rows through the block-wide kernel, short rows through the wave
+        # kernel; bit tests never, at the cost model's weights, and always
+        # (weight 1 picks them wherever they are fewer)
         for sm in (1, 128):
-            assert gpu.tc_count(d, {**forced(ACC_MSA_LDS), "hub_window": hw, "small_max_mask": sm}) == want, (hw, sm)
+            for pr in (0, 1, 4):
+                o = {**forced(ACC_MSA_LDS), "hub_window": hw, "small_max_mask": sm, "hub_probe": pr}
+                assert gpu.tc_count(d, o) == want, (hw, sm, pr)
         assert np.array_equal(gpu.tc_rows(d, {"hub_window": hw}), rows_want), hw
-    # split rows: hub words and streamed entries of one row in several tasks
-    assert gpu.tc_count(d, {**forced(ACC_MSA_LDS), "hub_window": 4096, "split_flop": 64}) == want
+    # split rows: hub words, bit tests and streamed entries of one row in
+    # several tasks (a task's bit tests start at the row's first mask column)
+    for pr in (0, 1):
+        assert gpu.tc_count(d, {**forced(ACC_MSA_LDS), "hub_window": 4096, "split_flop": 64, "hub_probe": pr}) == want
     d.free()
 
 
