@@ -579,7 +579,8 @@ extern "C" int gk_selector_samples(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr 
     GK_TRY(b.alloc(&dacc, m));
     // features from the row statistics
     RowStats st;
-    GK_TRY(row_stats(ctx, A, B, M, &st, hub_rows(B, mode == 0 ? MODE_TC : MODE_VALUES, base)));
+    GK_TRY(row_stats(ctx, A, B, M, &st, hub_rows(B, mode == 0 ? MODE_TC : MODE_VALUES, base),
+                     std::max(base.hub_probe, 0)));
     if (m) {
         const int g = (int)std::max<int64_t>(1, std::min<int64_t>((m + 255) / 256, (int64_t)ctx->num_cus * 4));
         k_features<<<g, 256, 0, ctx->stream>>>(m, st, same_am, dX);

@@ -174,7 +174,9 @@ void record_timing(gk_ctx *ctx, const ProductInfo &pi, bool reset);
 
 // hub_n > 0: rows of B below hub_n are read as bitmaps by the window kernels
 // (RowStats.flop counts their words).
-int row_stats(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcsr *M, RowStats *st, int32_t hub_n = 0);
+// hub_n: hub bitmap rows (0: none); probe: the bit-test weight of hub_choice
+int row_stats(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcsr *M, RowStats *st, int32_t hub_n = 0,
+              int32_t probe = 0);
 // Bitmaps of rows [0, hub_n) of B (B strictly lower: every column below its
 // row) into the context's hub scratch; fills P's hub fields.
 int build_hub(gk_ctx *ctx, const gk_dcsr *B, int32_t hub_n, Prob *P);

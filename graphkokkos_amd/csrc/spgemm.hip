@@ -55,7 +55,7 @@ __global__ __launch_bounds__(256) void k_row_stats(int64_t m, const int32_t *__r
                                                    const int32_t *__restrict__ acol,
                                                    const int32_t *__restrict__ brst,
                                                    const int32_t *__restrict__ mrst, RowStats st,
-                                                   int32_t hub_n, int same_am) {
+                                                   int32_t hub_n, int same_am, int32_t probe) {
     const int gl = threadIdx.x % G;
     const int64_t per_block = blockDim.x / G;
     for (int64_t i = (int64_t)blockIdx.x * per_block + threadIdx.x / G; i < m; i += (int64_t)gridDim.x * per_block) {
@@ -66,9 +66,15 @@ __global__ __launch_bounds__(256) void k_row_stats(int64_t m, const int32_t *__r
             const int32_t k = acol[p];
             const int32_t len = brst[k + 1] - brst[k];
             sflop += len;
-            flop += hub_row(hub_n, k, len) ? (k + 31) >> 5 : len;
-            // with A the mask, the mask entries below k are the p - a0 before it
-            mflop += same_am && k < hub_n ? min(len, p - a0) : len;
+            // window kernels: the reads of hub_choice's way (bit tests,
+            // bitmap words or entries); with A the mask, the mask entries
+            // below k are the p - a0 before it
+            const int32_t words = (k + 31) >> 5, below = p - a0;
+            if (k < hub_n && len > 0 && same_am && probe > 0 && probe * below < len && probe * below < words)
+                flop += below;
+            else
+                flop += hub_row(hub_n, k, len) ? words : len;
+            mflop += same_am && k < hub_n ? min(len, below) : len;
             bmax = max(bmax, len);
         }
         flop = tile_sum<G>(flop);
@@ -1175,7 +1181,8 @@ int build_hub(gk_ctx *ctx, const gk_dcsr *B, int32_t hub_n, Prob *P) {
 }
 
 // Features only (for the selector's training and inference).
-int row_stats(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcsr *M, RowStats *st, int32_t hub_n) {
+int row_stats(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcsr *M, RowStats *st, int32_t hub_n,
+              int32_t probe) {
     const int64_t m = A->m;
     void *p;
     size_t need = (size_t)m * (8 + 8 + 8 + 4 + 4 + 4 + 4 + 1) + 64;
@@ -1193,7 +1200,7 @@ int row_stats(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcsr *M,
     constexpr int G = 8;
     int blocks = grid_for(ctx, (m + 256 / G - 1) / (256 / G), 16);
     k_row_stats<G><<<blocks, 256, 0, ctx->stream>>>(m, A->rst, A->col, B->rst, M->rst, *st, hub_n,
-                                                    A->rst == M->rst && A->col == M->col);
+                                                    A->rst == M->rst && A->col == M->col, probe);
     return launch_check("k_row_stats");
 }
 
@@ -1427,7 +1434,7 @@ int masked_product(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcs
     sel.start();
     RowStats st;
     const int32_t hub_n = hub_rows(B, mode, o);
-    GK_TRY(row_stats(ctx, A, B, M, &st, hub_n));
+    GK_TRY(row_stats(ctx, A, B, M, &st, hub_n, std::max(o.hub_probe, 0)));
     SelectParams sp = select_params(o);
     sp.hub = hub_n > 0;
     if (sp.selector == GK_SEL_LEARNED)
