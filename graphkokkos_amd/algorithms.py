@@ -199,14 +199,22 @@ class Context:
         check(lib().gk_masked_spgemm_dev(self.handle, A.handle, B.handle, M.handle, out.ptr, C.byref(o)),
               "gk_masked_spgemm_dev")
 
-    def masked_spgemm_cmask_dev(self, A: DeviceCsr, B: DeviceCsr, M: DeviceCsr, dense_rows: int = -1) -> DeviceCsr:
+    def masked_spgemm_cmask_dev(self, A: DeviceCsr, B: DeviceCsr, M: DeviceCsr, dense_rows: int = -1,
+                                out: DeviceCsr | None = None) -> DeviceCsr:
         """C<not M> = A*B on device handles (M with sorted rows); the result
-        is a library-built CSR with values.  dense_rows: -1 picks each row's
-        way (sorted or dense accumulation), 0 / 1 force one."""
-        h = C.c_void_p()
+        is a library-built CSR with values: a new one, or `out` (a result of
+        an earlier call) refilled, its storage grown only when too small.
+        dense_rows: -1 picks each row's way (sorted or dense accumulation),
+        0 / 1 force one."""
+        h = C.c_void_p(out.handle.value if out is not None else None)
         check(lib().gk_masked_spgemm_cmask_dev(self.handle, A.handle, B.handle, M.handle, dense_rows, C.byref(h)),
               "gk_masked_spgemm_cmask_dev")
-        return DeviceCsr.adopt(self, h)
+        if out is None:
+            return DeviceCsr.adopt(self, h)
+        m, n, nz = C.c_int64(), C.c_int64(), C.c_int64()
+        check(lib().gk_dcsr_dims(h, C.byref(m), C.byref(n), C.byref(nz)), "gk_dcsr_dims")
+        out.m, out.n, out.nz = int(m.value), int(n.value), int(nz.value)
+        return out
 
     def masked_spgemm_cmask(self, A: SpMtx, B: SpMtx, M: SpMtx, dense_rows: int = -1) -> SpMtx:
         """C<not M> = A*B: the entries some product term reaches outside M's

@@ -511,6 +511,8 @@ int run_chunk(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcsr *M,
 extern "C" int gk_masked_spgemm_cmask_dev(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcsr *M,
                                           int32_t dense_rows, gk_dcsr **C) {
     GK_CHECK_ARG(ctx && A && B && M && C, "null argument");
+    GK_CHECK_ARG(*C == nullptr || (*C != A && *C != B && *C != M && (*C)->cap_rst > 0),
+                 "*C must be NULL or a result of an earlier gk_masked_spgemm_cmask_dev (not an operand)");
     GK_CHECK_ARG(dense_rows >= -1 && dense_rows <= 1, "dense_rows must be -1, 0 or 1");
     if (A->n != B->m || A->m != M->m || B->n != M->n) {
         set_error("shape mismatch: A %lldx%lld, B %lldx%lld, M %lldx%lld", (long long)A->m, (long long)A->n,
@@ -694,24 +696,51 @@ extern "C" int gk_masked_spgemm_cmask_dev(gk_ctx *ctx, const gk_dcsr *A, const g
         return GK_ERANGE;
     }
     // the result (owned by the returned handle)
-    auto *d = new gk_dcsr();
+    // the result: a new CSR, or the caller's from an earlier call refilled,
+    // its storage grown only when too small (as the reference's resizeVals)
+    gk_dcsr *d = *C;
+    const bool fresh = d == nullptr;
+    if (fresh) d = new gk_dcsr();
+    auto fail = [&](int rc) {
+        if (fresh) {
+            free_csr(d);
+            delete d;
+        } else {
+            d->m = d->n = d->nz = 0;  // left empty but valid for gk_dcsr_free
+        }
+        return rc;
+    };
     d->ctx = ctx;
+    d->owns = true;
+    d->sorted = true;
+    d->lower = d->symmetric = false;
+    if (d->cap_rst < m + 1) {
+        if (d->rst) (void)hipFree(d->rst);
+        d->rst = nullptr;
+        d->cap_rst = 0;
+        if (hipMalloc(&d->rst, 4 * (size_t)(m + 1)) != hipSuccess) {
+            d->rst = nullptr;
+            set_error("hipMalloc of C<not M> failed");
+            return fail(GK_ENOMEM);
+        }
+        d->cap_rst = m + 1;
+    }
+    const int64_t want_nz = std::max<int64_t>(nz, 1);
+    if (d->cap_nz < want_nz || !d->col || !d->val) {
+        if (d->col) (void)hipFree(d->col);
+        if (d->val) (void)hipFree(d->val);
+        d->col = d->val = nullptr;
+        d->cap_nz = 0;
+        if (hipMalloc(&d->col, 4 * (size_t)want_nz) != hipSuccess ||
+            hipMalloc(&d->val, 4 * (size_t)want_nz) != hipSuccess) {
+            set_error("hipMalloc of C<not M> failed");
+            return fail(GK_ENOMEM);
+        }
+        d->cap_nz = want_nz;
+    }
     d->m = m;
     d->n = n;
     d->nz = nz;
-    d->owns = true;
-    d->sorted = true;
-    auto fail = [&](int rc) {
-        free_csr(d);
-        delete d;
-        return rc;
-    };
-    if (hipMalloc(&d->rst, 4 * (size_t)(m + 1)) != hipSuccess ||
-        hipMalloc(&d->col, 4 * (size_t)std::max<int64_t>(nz, 1)) != hipSuccess ||
-        hipMalloc(&d->val, 4 * (size_t)std::max<int64_t>(nz, 1)) != hipSuccess) {
-        set_error("hipMalloc of C<not M> failed");
-        return fail(GK_ENOMEM);
-    }
     k_cm_rst32<<<grid_of(ctx, m + 1, 256), 256, 0, ctx->stream>>>(m, rst64, d->rst);
     // 6. the entries to their places: sorted rows, then dense rows (second pass)
     for (const Chunk &ch : chunks) {

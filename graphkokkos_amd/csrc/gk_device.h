@@ -84,6 +84,9 @@ struct gk_dcsr {
     bool sorted = true;       // every row strictly increasing
     bool lower = false;       // every column below its row
     bool symmetric = false;   // square, pattern symmetric, no self loops
+    // Capacity of rst (entries) and of col / val of a CSR the library built
+    // for a result that later calls may refill (0: not refillable).
+    int64_t cap_rst = 0, cap_nz = 0;
 };
 
 namespace gk {

@@ -223,7 +223,7 @@ extern "C" void gk_opts_default(gk_opts *o) {
     o->struct_size = sizeof(gk_opts);
     o->selector = GK_SEL_HEURISTIC;
     o->forced_accum = GK_ACC_MSA_LDS;
-    o->mca_max_mask = 8;
+    o->mca_max_mask = 16;
     o->lds_max_mask = 1536;
     o->dense_min_mask = 16384;
     o->split_flop = 1 << 18;

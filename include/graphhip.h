@@ -92,7 +92,7 @@ typedef struct gk_opts {
     int32_t struct_size;    /* sizeof(gk_opts); 0 or a NULL opts means defaults */
     int32_t selector;       /* enum gk_selector                           */
     int32_t forced_accum;   /* enum gk_accum, used with GK_SEL_FORCED     */
-    int32_t mca_max_mask;   /* heuristic: mask rows up to this use MCA    */
+    int32_t mca_max_mask;   /* heuristic: mask rows up to this use MCA (default 16) */
     int32_t lds_max_mask;   /* heuristic: mask rows up to this use MSA-LDS*/
     int32_t dense_min_mask; /* heuristic: mask rows from this use dense-global */
     int32_t split_flop;     /* rows above this many B entries are split   */
@@ -251,8 +251,10 @@ int gk_masked_spgemm_dev(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B,
  * M.n), with the int32 (wrap-around) sum of the terms, explicit zeros kept,
  * rows in column order.  The pattern is not known beforehand, so *C is a
  * library-built device CSR with values (gk_dcsr_dims, gk_dcsr_download,
- * gk_dcsr_download_val; freed with gk_dcsr_free).  M needs sorted rows.
- * Rows go one of two ways (the result does not depend on it): sorted
+ * gk_dcsr_download_val; freed with gk_dcsr_free): with *C NULL on entry a
+ * new one, with *C a result of an earlier call that one refilled, its
+ * storage grown only when too small (as the reference's resizeVals grows
+ * C).  M needs sorted rows.  Rows go one of two ways (the result does not depend on it): sorted
  * (product terms expanded, radix-sorted and summed) or dense (a bitmap and
  * sums per window of columns in LDS; needs B's rows sorted when C is wider
  * than one window).  dense_rows: GK_CMASK_AUTO picks per row by its terms

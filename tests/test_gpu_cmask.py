@@ -158,3 +158,22 @@ def test_two_hop_pairs_rmat16_two_windows(gpu):
     t = gpu.last_timing()
     assert t["iterations"] > 0  # some rows took the dense way
     check(got, O.mxmm_cmask(A, A, A))
+
+
+def test_result_refilled_and_grown(gpu):
+    """A result handed back is refilled: smaller products reuse its storage,
+    larger ones grow it; operands and uploaded CSRs are refused as C."""
+    rng = np.random.default_rng(31)
+    cases = [(120, 90, 150, 0.08), (30, 20, 40, 0.1), (200, 150, 300, 0.06)]
+    C_ = None
+    for m, k, n, dens in cases:
+        A, B = rand_csr(rng, m, k, dens), rand_csr(rng, k, n, dens)
+        M = rand_csr(rng, m, n, 0.05, values=False)
+        dA, dB, dM = gpu.upload(A), gpu.upload(B), gpu.upload(M)
+        C_ = gpu.masked_spgemm_cmask_dev(dA, dB, dM, -1, out=C_)
+        check(C_.to_host(values=True), O.mxmm_cmask(A, B, M))
+        with pytest.raises(GraphHipError):
+            gpu.masked_spgemm_cmask_dev(dA, dB, dM, -1, out=dA)
+        for d in (dA, dB, dM):
+            d.free()
+    C_.free()

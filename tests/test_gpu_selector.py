@@ -6,7 +6,7 @@ fixed-threshold heuristic."""
 import numpy as np
 import pytest
 
-from graphkokkos_amd import SEL_LEARNED, HostGraph, selector
+from graphkokkos_amd import SEL_LEARNED, HostGraph, default_opts, selector
 from oracle import mlp
 from oracle import oracle as O
 
@@ -74,10 +74,12 @@ def test_samples_learned_selection_and_gain(gpu):
 
 
 def _heuristic_choice(X, V):
-    """The fixed-threshold rule on the features' mask length (defaults:
-    MCA-wave up to 8, MSA-LDS up to 1536, dense-global from 16384)."""
+    """The fixed-threshold rule on the features' mask length, at the
+    library's default thresholds (MCA-wave, MSA-LDS, dense-global)."""
+    o = default_opts()
     mlen = np.round(2 ** X[:, 0] - 1).astype(np.int64)
-    c = np.where(mlen <= 8, 1, np.where(mlen <= 1536, 0, np.where(mlen >= 16384, 3, 2)))
+    c = np.where(mlen <= o.mca_max_mask, 1,
+                 np.where(mlen <= o.lds_max_mask, 0, np.where(mlen >= o.dense_min_mask, 3, 2)))
     c = np.where(V[np.arange(len(c)), c].astype(bool), c, 0)  # capacity fallback
     return c
 
