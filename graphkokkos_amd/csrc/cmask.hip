@@ -695,7 +695,6 @@ extern "C" int gk_masked_spgemm_cmask_dev(gk_ctx *ctx, const gk_dcsr *A, const g
         set_error("C<not M> has %lld entries (int32 CSR limit)", (long long)nz);
         return GK_ERANGE;
     }
-    // the result (owned by the returned handle)
     // the result: a new CSR, or the caller's from an earlier call refilled,
     // its storage grown only when too small (as the reference's resizeVals)
     gk_dcsr *d = *C;
