@@ -199,7 +199,7 @@ def run_cmask(ctx, scale: int, seed: int, steps: int, no_cpu: bool) -> dict:
            "roofline": {"bound": "hbm", "achieved": round(dense_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(dense_gbs / HBM_PEAK_GBS, 5),
                         "traffic": measured_traffic("k_cm_dense", f"cmask-rmat{scale}"),
-                        "kernel": "k_cm_dense (count + write passes)", "bytes_per_launch": t["acc_bytes"][0] / 2,
+                        "kernel": "k_cm_dense (pool pass) + k_cm_dense_place", "bytes_per_launch": t["acc_bytes"][0] / 2,
                         "avg_launch_ms": t["acc_ms"][0] / 2, "launches": 2},
            "expand_roofline": {"bound": "hbm", "achieved": round(exp_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                "frac": round(exp_gbs / HBM_PEAK_GBS, 5), "traffic": None,

@@ -205,7 +205,7 @@ class Context:
         is a library-built CSR with values: a new one, or `out` (a result of
         an earlier call) refilled, its storage grown only when too small.
         dense_rows: -1 picks each row's way (sorted or dense accumulation),
-        0 / 1 force one."""
+        0 / 1 force one, 2 forces dense rows counted first (no work pool)."""
         h = C.c_void_p(out.handle.value if out is not None else None)
         check(lib().gk_masked_spgemm_cmask_dev(self.handle, A.handle, B.handle, M.handle, dense_rows, C.byref(h)),
               "gk_masked_spgemm_cmask_dev")

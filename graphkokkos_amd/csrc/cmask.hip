@@ -155,6 +155,9 @@ __global__ __launch_bounds__(64 * kWaves) void k_cm_expand(
 // a single window needs none).
 constexpr int kDenseWindowLog = 15;  // 32768 columns: 128 KiB of sums + 4 KiB of bits
 constexpr int kDenseThreads = 1024;
+// Dense rows are written in one pass into a pool, then copied into C, when
+// the pool's upper bound fits this; otherwise a count pass comes first.
+constexpr int64_t kPoolMaxBytes = int64_t(8) << 30;
 constexpr int kDenseWaves = kDenseThreads / 64;
 
 __global__ void k_cm_flop(int64_t m, const int32_t *__restrict__ arst, const int32_t *__restrict__ acol,
@@ -191,19 +194,27 @@ __device__ __forceinline__ int32_t lower_bound_col(const int32_t *__restrict__ c
     return l;
 }
 
-// One pass over the (dense row, window) pairs; NUMERIC = false counts each
-// pair's entries into pcnt, NUMERIC = true writes them at
-// rst[row] + poff[pair] (poff: the pair's place within its row).
-template <bool NUMERIC>
+// What a pass over the (dense row, window) pairs does with a pair's entries.
+enum DensePass {
+    kDenseCount = 0,  // count them into pcnt
+    kDenseWriteC = 1, // write them at rst[row] + poff[pair] (poff: the pair's place in its row)
+    kDenseWritePool = 2  // write them at a run of the pool reserved by one atomic on *pool_n;
+                         // the run's start into pstart, its length into pcnt
+};
+
+template <int PASS>
 __global__ __launch_bounds__(kDenseThreads) void k_cm_dense(
     int64_t npairs, int64_t windows, int wlog, int64_t n, const int32_t *__restrict__ drows,
     const int32_t *__restrict__ arst, const int32_t *__restrict__ acol, const int32_t *__restrict__ aval,
     const int32_t *__restrict__ brst, const int32_t *__restrict__ bcol, const int32_t *__restrict__ bval,
     const int32_t *__restrict__ mrst, const int32_t *__restrict__ mcol, int32_t *__restrict__ pcnt,
     const int32_t *__restrict__ poff, const int32_t *__restrict__ rst, int32_t *__restrict__ ccol,
-    int32_t *__restrict__ cval) {
+    int32_t *__restrict__ cval, unsigned long long *__restrict__ pool_n, int64_t pool_cap,
+    int64_t *__restrict__ pstart, int32_t *__restrict__ over) {
+    constexpr bool NUMERIC = PASS != kDenseCount;
     __shared__ uint32_t bits[1 << (kDenseWindowLog - 5)];
     __shared__ uint32_t vals[NUMERIC ? 1 << kDenseWindowLog : 1];
+    __shared__ long long s_base;
     __shared__ int64_t s_end[kDenseThreads];  // block scan of the group's B row cuts
     __shared__ int32_t s_b0[kDenseThreads];
     __shared__ uint32_t s_a[kDenseThreads];
@@ -290,7 +301,7 @@ __global__ __launch_bounds__(kDenseThreads) void k_cm_dense(
         const int32_t incl = dev::wave_incl_scan(c);
         if (lane == 63) s_red[w] = incl;
         __syncthreads();
-        if (!NUMERIC) {
+        if (PASS == kDenseCount) {
             if (threadIdx.x == 0) {
                 int64_t tot = 0;
                 for (int x = 0; x < kDenseWaves; ++x) tot += s_red[x];
@@ -299,8 +310,27 @@ __global__ __launch_bounds__(kDenseThreads) void k_cm_dense(
         } else {
             int64_t before = 0;
             for (int x = 0; x < w; ++x) before += s_red[x];
-            int64_t out = (int64_t)rst[row] + poff[pair] + before + incl - c;
-            for (int x = 0; x < per; ++x) {
+            int64_t at;
+            if (PASS == kDenseWritePool) {
+                if (threadIdx.x == 0) {
+                    int64_t tot = 0;
+                    for (int x = 0; x < kDenseWaves; ++x) tot += s_red[x];
+                    long long b = (long long)atomicAdd(pool_n, (unsigned long long)tot);
+                    if (b + tot > pool_cap) {  // cannot happen (the pool holds the bound); never write past it
+                        *over = 1;
+                        b = -1;
+                    }
+                    pcnt[pair] = (int32_t)tot;
+                    pstart[pair] = b;
+                    s_base = b;
+                }
+                __syncthreads();
+                at = s_base;
+            } else {
+                at = (int64_t)rst[row] + poff[pair];
+            }
+            int64_t out = at + before + incl - c;
+            for (int x = 0; x < per && at >= 0; ++x) {
                 if (wb + x >= words) break;
                 uint32_t b = bits[wb + x];
                 while (b) {
@@ -315,6 +345,31 @@ __global__ __launch_bounds__(kDenseThreads) void k_cm_dense(
         }
         __syncthreads();
     }
+}
+
+// Each pair's run of the pool to its place in C, one workgroup per pair.
+__global__ void k_cm_dense_place(int64_t npairs, int64_t windows, const int32_t *__restrict__ drows,
+                                 const int32_t *__restrict__ pcnt, const int64_t *__restrict__ pstart,
+                                 const int32_t *__restrict__ poff, const int32_t *__restrict__ rst,
+                                 const int32_t *__restrict__ pcol, const int32_t *__restrict__ pval,
+                                 int32_t *__restrict__ ccol, int32_t *__restrict__ cval) {
+    for (int64_t pair = blockIdx.x; pair < npairs; pair += gridDim.x) {
+        const int64_t src = pstart[pair], dst = (int64_t)rst[drows[pair / windows]] + poff[pair];
+        const int32_t len = pcnt[pair];
+        if (src < 0) continue;
+        for (int32_t x = threadIdx.x; x < len; x += blockDim.x) {
+            ccol[dst + x] = pcol[src + x];
+            cval[dst + x] = pval[src + x];
+        }
+    }
+}
+
+// Upper bound of the pool: per dense row min(its terms, C's width), as a
+// (row, window) pair holds at most min(its terms, the window's width).
+__global__ void k_cm_pool_bound(int64_t m, int64_t n, const int64_t *__restrict__ flop,
+                                const uint8_t *__restrict__ dense, int64_t *__restrict__ bound) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
+        bound[i] = dense[i] ? min(flop[i], n) : 0;
 }
 
 // Per dense row: the pairs' places within the row and the row's count.
@@ -513,7 +568,7 @@ extern "C" int gk_masked_spgemm_cmask_dev(gk_ctx *ctx, const gk_dcsr *A, const g
     GK_CHECK_ARG(ctx && A && B && M && C, "null argument");
     GK_CHECK_ARG(*C == nullptr || (*C != A && *C != B && *C != M && (*C)->cap_rst > 0),
                  "*C must be NULL or a result of an earlier gk_masked_spgemm_cmask_dev (not an operand)");
-    GK_CHECK_ARG(dense_rows >= -1 && dense_rows <= 1, "dense_rows must be -1, 0 or 1");
+    GK_CHECK_ARG(dense_rows >= -1 && dense_rows <= 2, "dense_rows must be -1, 0, 1 or 2");
     if (A->n != B->m || A->m != M->m || B->n != M->n) {
         set_error("shape mismatch: A %lldx%lld, B %lldx%lld, M %lldx%lld", (long long)A->m, (long long)A->n,
                   (long long)B->m, (long long)B->n, (long long)M->m, (long long)M->n);
@@ -529,7 +584,7 @@ extern "C" int gk_masked_spgemm_cmask_dev(gk_ctx *ctx, const gk_dcsr *A, const g
     while (wlog < kDenseWindowLog && ((int64_t)1 << wlog) < n) ++wlog;
     const int64_t windows = std::max<int64_t>(1, (n + ((int64_t)1 << wlog) - 1) >> wlog);
     // a window cut of B rows needs them sorted
-    const int mode = (windows > 1 && !B->sorted) ? 0 : dense_rows;
+    const int mode = (windows > 1 && !B->sorted) ? 0 : dense_rows == 2 ? 1 : dense_rows;
 
     // 1. terms per row, the dense rows
     int64_t *flop;
@@ -565,6 +620,24 @@ extern "C" int gk_masked_spgemm_cmask_dev(gk_ctx *ctx, const gk_dcsr *A, const g
         GK_HIP(d2h(ctx, &h, nd32, sizeof h));
         ndense = h;
     }
+    // the dense rows' entries are bounded by min(terms, n) per row: with a
+    // pool of that size they take one pass (written there, then copied)
+    int64_t pool_bound = 0;
+    if (ndense > 0) {
+        int64_t *bnd, *bsum;
+        GK_TRY(T.alloc(&bnd, m));
+        GK_TRY(T.alloc(&bsum, 1));
+        k_cm_pool_bound<<<grid_of(ctx, m, 256), 256, 0, ctx->stream>>>(m, n, flop, dflag, bnd);
+        GK_TRY(launch_check("k_cm_pool_bound"));
+        size_t tmp = 0;
+        GK_HIP(rocprim::reduce(nullptr, tmp, bnd, bsum, (int64_t)0, (size_t)m, rocprim::plus<int64_t>(),
+                               ctx->stream));
+        char *t;
+        GK_TRY(T.alloc(&t, (int64_t)tmp));
+        GK_HIP(rocprim::reduce(t, tmp, bnd, bsum, (int64_t)0, (size_t)m, rocprim::plus<int64_t>(), ctx->stream));
+        GK_HIP(d2h(ctx, &pool_bound, bsum, sizeof pool_bound));
+    }
+    const bool one_pass = ndense > 0 && dense_rows != 2 && pool_bound * 8 <= kPoolMaxBytes;
 
     // 2. the sorted rows' terms: per A entry (0 in dense rows), scanned
     int64_t *blen, *aoff, *foff;
@@ -589,22 +662,41 @@ extern "C" int gk_masked_spgemm_cmask_dev(gk_ctx *ctx, const gk_dcsr *A, const g
     std::vector<int64_t> hoff(m + 1);
     GK_HIP(d2h(ctx, hoff.data(), foff, sizeof(int64_t) * (size_t)(m + 1)));
 
-    // 3. the dense rows' counts (first pass), while the host cuts chunks
-    int32_t *pcnt, *poff;
-    int64_t *cnt, *first;
+    // 3. the dense rows: their entries into the pool (or their counts
+    // only), while the host cuts chunks
+    int32_t *pcnt, *poff, *pcol = nullptr, *pval = nullptr, *over = nullptr;
+    int64_t *cnt, *first, *pstart = nullptr;
+    unsigned long long *pool_n = nullptr;
     const int64_t npairs = ndense * windows;
     GK_TRY(T.alloc(&pcnt, npairs));
     GK_TRY(T.alloc(&poff, npairs));
     GK_TRY(T.alloc(&cnt, m + 1));
     GK_TRY(T.alloc(&first, m + 1));
     GK_HIP(hipMemsetAsync(cnt, 0, sizeof(int64_t) * (size_t)(m + 1), ctx->stream));
+    if (one_pass) {
+        GK_TRY(T.alloc(&pcol, pool_bound));
+        GK_TRY(T.alloc(&pval, pool_bound));
+        GK_TRY(T.alloc(&pstart, npairs));
+        GK_TRY(T.alloc(&pool_n, 1));
+        GK_TRY(T.alloc(&over, 1));
+        GK_HIP(hipMemsetAsync(pool_n, 0, sizeof *pool_n, ctx->stream));
+        GK_HIP(hipMemsetAsync(over, 0, sizeof *over, ctx->stream));
+    }
     tdense.start();
-    if (npairs) {
-        const int g = (int)std::min<int64_t>(npairs, (int64_t)ctx->num_cus * 2);
-        k_cm_dense<false><<<g, kDenseThreads, 0, ctx->stream>>>(
+    if (npairs && one_pass) {
+        const int g = (int)std::min<int64_t>(npairs, (int64_t)ctx->num_cus);
+        k_cm_dense<kDenseWritePool><<<g, kDenseThreads, 0, ctx->stream>>>(
             npairs, windows, wlog, n, drows, A->rst, A->col, A->val, B->rst, B->col, B->val, M->rst, M->col, pcnt,
-            nullptr, nullptr, nullptr, nullptr);
+            nullptr, nullptr, pcol, pval, pool_n, pool_bound, pstart, over);
+        GK_TRY(launch_check("k_cm_dense<pool>"));
+    } else if (npairs) {
+        const int g = (int)std::min<int64_t>(npairs, (int64_t)ctx->num_cus * 2);
+        k_cm_dense<kDenseCount><<<g, kDenseThreads, 0, ctx->stream>>>(
+            npairs, windows, wlog, n, drows, A->rst, A->col, A->val, B->rst, B->col, B->val, M->rst, M->col, pcnt,
+            nullptr, nullptr, nullptr, nullptr, nullptr, 0, nullptr, nullptr);
         GK_TRY(launch_check("k_cm_dense<count>"));
+    }
+    if (npairs) {
         k_cm_dense_rows<<<grid_of(ctx, ndense, 256), 256, 0, ctx->stream>>>(ndense, windows, drows, pcnt, poff, cnt);
         GK_TRY(launch_check("k_cm_dense_rows"));
     }
@@ -755,11 +847,24 @@ extern "C" int gk_masked_spgemm_cmask_dev(gk_ctx *ctx, const gk_dcsr *A, const g
     if ((rc = launch_check("k_cm_place")) != GK_OK) return fail(rc);
     if (npairs) {
         tdense.start();
-        const int g = (int)std::min<int64_t>(npairs, (int64_t)ctx->num_cus);
-        k_cm_dense<true><<<g, kDenseThreads, 0, ctx->stream>>>(
-            npairs, windows, wlog, n, drows, A->rst, A->col, A->val, B->rst, B->col, B->val, M->rst, M->col, nullptr,
-            poff, d->rst, d->col, d->val);
-        if ((rc = launch_check("k_cm_dense<write>")) != GK_OK) return fail(rc);
+        if (one_pass) {
+            int32_t o = 0;
+            if (d2h(ctx, &o, over, sizeof o) != hipSuccess) return fail(GK_EHIP);
+            if (o) {
+                set_error("C<not M>: dense rows exceeded their pool bound (internal error)");
+                return fail(GK_ESTATE);
+            }
+            const int g = (int)std::min<int64_t>(npairs, (int64_t)ctx->num_cus * 8);
+            k_cm_dense_place<<<g, 256, 0, ctx->stream>>>(npairs, windows, drows, pcnt, pstart, poff, d->rst, pcol,
+                                                         pval, d->col, d->val);
+            if ((rc = launch_check("k_cm_dense_place")) != GK_OK) return fail(rc);
+        } else {
+            const int g = (int)std::min<int64_t>(npairs, (int64_t)ctx->num_cus);
+            k_cm_dense<kDenseWriteC><<<g, kDenseThreads, 0, ctx->stream>>>(
+                npairs, windows, wlog, n, drows, A->rst, A->col, A->val, B->rst, B->col, B->val, M->rst, M->col,
+                nullptr, poff, d->rst, d->col, d->val, nullptr, 0, nullptr, nullptr);
+            if ((rc = launch_check("k_cm_dense<write>")) != GK_OK) return fail(rc);
+        }
         tdense.stop();
     }
     total.stop();
@@ -780,13 +885,15 @@ extern "C" int gk_masked_spgemm_cmask_dev(gk_ctx *ctx, const gk_dcsr *A, const g
     tm.acc_ms[0] = dense_ms;
     tm.acc_launches[0] = npairs ? 2 : 0;
     {
-        // dense passes: per term B's column read in both passes (and its
-        // value in the second), per entry its column and value written once
-        // (the A entries and mask rows, a few per cent of this, left out)
+        // dense rows: per term B's column (and value) read, in the count
+        // pass too when there is one; per entry its column and value
+        // written once, or to the pool and then copied (read and written
+        // again); the A entries and mask rows, a few per cent, left out
         int64_t sorted_nz = 0;
         for (const Chunk &ch : chunks) sorted_nz += ch.n;
-        const double dterms = (double)(terms_all - hoff[m]);
-        tm.acc_bytes[0] = dterms * (4 + 4 + (B->val ? 4 : 0)) + (double)(nz - sorted_nz) * 8;
+        const double dterms = (double)(terms_all - hoff[m]), dnz = (double)(nz - sorted_nz);
+        tm.acc_bytes[0] = one_pass ? dterms * (4 + (B->val ? 4 : 0)) + dnz * 24
+                                   : dterms * (4 + 4 + (B->val ? 4 : 0)) + dnz * 8;
     }
     // plus, per A entry, its column (and value) and B's two row pointers;
     // the mask rows and the row offsets once

@@ -259,12 +259,17 @@ int gk_masked_spgemm_dev(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B,
  * sums per window of columns in LDS; needs B's rows sorted when C is wider
  * than one window).  dense_rows: GK_CMASK_AUTO picks per row by its terms
  * against its A entries and C's width, GK_CMASK_SORTED and GK_CMASK_DENSE
- * force one way where allowed.  gk_last_timing afterwards: kernel_ms = the
+ * force one way where allowed.  Dense rows are written in one pass to a
+ * work pool of up to min(terms, C.n) entries per row and copied into C
+ * when that pool stays under 8 GiB, else counted in a first pass and
+ * written in a second; GK_CMASK_DENSE_COUNTED forces every row dense and
+ * counted (no pool).  gk_last_timing afterwards: kernel_ms = the
  * term expansion, select_ms = sort and reduce, acc_ms[0] / acc_bytes[0] =
- * the dense passes' time and algorithmic bytes, launches = row chunks,
+ * the dense rows' two kernels' time and algorithmic bytes (entries written
+ * to a pool then copied, or a count pass then the write), launches = row chunks,
  * iterations = dense rows, flop = product terms, hits = entries of C,
  * bytes_alg = the expansion's algorithmic bytes.                          */
-enum gk_cmask_rows { GK_CMASK_AUTO = -1, GK_CMASK_SORTED = 0, GK_CMASK_DENSE = 1 };
+enum gk_cmask_rows { GK_CMASK_AUTO = -1, GK_CMASK_SORTED = 0, GK_CMASK_DENSE = 1, GK_CMASK_DENSE_COUNTED = 2 };
 int gk_masked_spgemm_cmask_dev(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B,
                                const gk_dcsr *M, int32_t dense_rows, gk_dcsr **C);
 int gk_dcsr_download_val(gk_ctx *ctx, const gk_dcsr *d, int32_t *val);

@@ -3,7 +3,8 @@ include/matrix_la.h:733-934) on the GPU against the oracle's restatement of
 the reference's two-pass MSA and against a dense brute force: rows,
 columns and int32 values must agree entry for entry, whichever way the rows
 go (DENSE_ROWS: -1 the library's per-row choice, 0 every row sorted, 1
-every row dense)."""
+every row dense, written through a pool, 2 every row dense, counted
+first)."""
 import sys
 
 import numpy as np
@@ -15,7 +16,7 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-DENSE_ROWS = [-1, 0, 1]
+DENSE_ROWS = [-1, 0, 1, 2]
 
 
 def say(*a):
