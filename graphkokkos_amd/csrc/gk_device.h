@@ -55,6 +55,10 @@ struct gk_ctx {
     static constexpr int kAux = 4;
     hipStream_t aux[kAux] = {};
     hipEvent_t fork = nullptr, join[kAux] = {};
+    // A side stream (default priority) for work that runs beside a product
+    // on the streams above, forked from and joined into `stream`.
+    hipStream_t side = nullptr;
+    hipEvent_t side_fork = nullptr, side_join = nullptr;
     void *scratch[gk::SC_NUM] = {};
     size_t scratch_bytes[gk::SC_NUM] = {};
     void *pinned = nullptr;          // small pinned host staging buffer

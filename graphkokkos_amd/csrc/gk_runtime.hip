@@ -256,6 +256,9 @@ extern "C" int gk_init(int device, gk_ctx **out) {
     for (int i = 0; ok && i < gk_ctx::kAux; ++i)
         ok = hipStreamCreateWithFlags(&ctx->aux[i], hipStreamNonBlocking) == hipSuccess &&
              hipEventCreateWithFlags(&ctx->join[i], hipEventDisableTiming) == hipSuccess;
+    ok = ok && hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) == hipSuccess &&
+         hipEventCreateWithFlags(&ctx->side_fork, hipEventDisableTiming) == hipSuccess &&
+         hipEventCreateWithFlags(&ctx->side_join, hipEventDisableTiming) == hipSuccess;
     if (!ok) {
         delete ctx;
         set_error("stream or event creation failed");
@@ -286,6 +289,10 @@ extern "C" void gk_finalize(gk_ctx *ctx) {
         if (ctx->join[i]) (void)hipEventDestroy(ctx->join[i]);
     }
     if (ctx->fork) (void)hipEventDestroy(ctx->fork);
+    if (ctx->side) (void)hipStreamSynchronize(ctx->side);
+    if (ctx->side) (void)hipStreamDestroy(ctx->side);
+    if (ctx->side_fork) (void)hipEventDestroy(ctx->side_fork);
+    if (ctx->side_join) (void)hipEventDestroy(ctx->side_join);
     for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;

@@ -895,27 +895,7 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
         GK_TRY(B.alloc(&aeid[b], A->nz));
     }
     const int gw = waves_grid(ctx, m);
-    // supports of the whole graph: C<L0> = L0*L0 in SUPPORT mode
-    {
-        gk_dcsr Ld = view(ctx, m, m, nL, L0.rst, L0.col, nullptr);
-        ProductInfo pi;
-        GK_TRY(masked_product(ctx, &Ld, &Ld, &Ld, MODE_SUPPORT, sup, nullptr, nullptr, o, &pi));
-        record_timing(ctx, pi, false);
-    }
-    k_edge_ends<<<gw, 256, 0, ctx->stream>>>(m, L0.rst, L0.col, eu, ev);
-    GK_HIP(hipMemcpyAsync(arst[0], A->rst, 4 * (m + 1), hipMemcpyDeviceToDevice, ctx->stream));
-    GK_HIP(hipMemcpyAsync(acol[0], A->col, 4 * (size_t)A->nz, hipMemcpyDeviceToDevice, ctx->stream));
-    k_sym_eid<<<gw, 256, 0, ctx->stream>>>(m, A->rst, A->col, L0.rst, aeid[0]);
-    if (sym_eid_out) {
-        // the edge id of every entry of A, kept for the final per-entry output
-        // (aeid[0] itself is reused by the adjacency compaction)
-        GK_TRY(B.alloc(sym_eid_out, A->nz));
-        GK_HIP(hipMemcpyAsync(*sym_eid_out, aeid[0], 4 * (size_t)A->nz, hipMemcpyDeviceToDevice, ctx->stream));
-    }
-    k_iota<<<grid_items(ctx, nL), 256, 0, ctx->stream>>>(nL, alist);
-    GK_HIP(hipMemsetAsync(st, 0, nL, ctx->stream));
-    GK_HIP(hipMemsetAsync(dctr, 0, 16, ctx->stream));
-    // adjacency tables
+    // adjacency tables: their sizes first (the table is allocated from them)
     k_hash_caps<<<grid_items(ctx, m), 256, 0, ctx->stream>>>(m, A->rst, caps);
     GK_TRY(exclusive_scan_i64(ctx, caps, hoff, m + 1));
     int64_t *h64;
@@ -924,9 +904,40 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
     GK_HIP(hipStreamSynchronize(ctx->stream));
     const int64_t tab_n = h64[0];
     GK_TRY(B.alloc(&tab, tab_n));
-    GK_HIP(hipMemsetAsync(tab, 0xff, sizeof(int2) * (size_t)std::max<int64_t>(tab_n, 1), ctx->stream));
-    k_hash_fill<<<gw, 256, 0, ctx->stream>>>(m, A->rst, A->col, aeid[0], hoff, tab);
-    GK_TRY(launch_check("truss_peel setup"));
+    if (sym_eid_out) GK_TRY(B.alloc(sym_eid_out, A->nz));
+    // The set-up that does not need the supports (edge ends, the adjacency
+    // with edge ids, the tables, the lists) runs on the side stream, beside
+    // the support product on the others; it writes none of the product's
+    // buffers.
+    GK_HIP(hipEventRecord(ctx->side_fork, ctx->stream));
+    GK_HIP(hipStreamWaitEvent(ctx->side, ctx->side_fork, 0));
+    {
+        hipStream_t s = ctx->side;
+        k_edge_ends<<<gw, 256, 0, s>>>(m, L0.rst, L0.col, eu, ev);
+        GK_HIP(hipMemcpyAsync(arst[0], A->rst, 4 * (m + 1), hipMemcpyDeviceToDevice, s));
+        GK_HIP(hipMemcpyAsync(acol[0], A->col, 4 * (size_t)A->nz, hipMemcpyDeviceToDevice, s));
+        k_sym_eid<<<gw, 256, 0, s>>>(m, A->rst, A->col, L0.rst, aeid[0]);
+        // the edge id of every entry of A, kept for the final per-entry
+        // output (aeid[0] itself is reused by the adjacency compaction)
+        if (sym_eid_out)
+            GK_HIP(hipMemcpyAsync(*sym_eid_out, aeid[0], 4 * (size_t)A->nz, hipMemcpyDeviceToDevice, s));
+        k_iota<<<grid_items(ctx, nL), 256, 0, s>>>(nL, alist);
+        GK_HIP(hipMemsetAsync(st, 0, nL, s));
+        GK_HIP(hipMemsetAsync(dctr, 0, 16, s));
+        GK_HIP(hipMemsetAsync(tab, 0xff, sizeof(int2) * (size_t)std::max<int64_t>(tab_n, 1), s));
+        k_hash_fill<<<gw, 256, 0, s>>>(m, A->rst, A->col, aeid[0], hoff, tab);
+        GK_TRY(launch_check("truss_peel setup"));
+        GK_HIP(hipEventRecord(ctx->side_join, s));
+    }
+    // supports of the whole graph: C<L0> = L0*L0 in SUPPORT mode
+    {
+        gk_dcsr Ld = view(ctx, m, m, nL, L0.rst, L0.col, nullptr);
+        ProductInfo pi;
+        const int rc = masked_product(ctx, &Ld, &Ld, &Ld, MODE_SUPPORT, sup, nullptr, nullptr, o, &pi);
+        GK_HIP(hipStreamWaitEvent(ctx->stream, ctx->side_join, 0));
+        GK_TRY(rc);
+        record_timing(ctx, pi, false);
+    }
     int cur = 0;                 // adjacency buffer in use
     int64_t aal_nz = A->nz;      // entries in the current adjacency
     int64_t na = nL;             // entries in the alive list
