@@ -242,15 +242,17 @@ def tc_variants(tuned_h: dict, tuned_l: dict) -> list:
             ("learned", {"selector": SEL_LEARNED}), ("learned_tuned", tuned_l)]
 
 
-def run_learned_graphs(ctx, tuned_h: dict, tuned_l: dict) -> dict:
+def run_learned_graphs(ctx) -> dict:
     """The learned selector (already trained) against the fixed-threshold
     heuristic on the other two graphs of the learned-selector evaluation:
     a triangulated 2048 x 2048 grid (one diagonal per cell) and a Chung-Lu
     graph with exponent 2.0, 4M vertices, mean degree 16.  Triangle counting
-    in four variants and the full truss decomposition with each selector;
+    in the five variants, the knobs autotuned on each graph itself (outside
+    the timed runs), and the full truss decomposition with each selector;
     counts and truss numbers must agree."""
     from graphkokkos_amd import HostGraph, SEL_LEARNED
     from graphkokkos_amd import _lib
+    from graphkokkos_amd import selector as SEL
     import ctypes as C
     out = {}
     for name, make in (("grid-2048x2048-diag", lambda: HostGraph.grid(2048)),
@@ -259,6 +261,12 @@ def run_learned_graphs(ctx, tuned_h: dict, tuned_l: dict) -> dict:
         G = make()
         log(f"{name}: n={G.n} |E|={G.edges} built in {time.perf_counter() - t0:.1f}s")
         res = {"edges": G.edges}
+        dL = ctx.upload(G.L)
+        tuned_h, _, _ = SEL.autotune_tc(ctx, dL, rounds=1, reps=3)
+        tuned_l, _, _ = SEL.autotune_tc(ctx, dL, base={"selector": SEL_LEARNED}, rounds=1, reps=3,
+                                        space=SEL.LEARNED_SPACE)
+        dL.free()
+        res["tuned_heuristic_opts"], res["learned_tuned_opts"] = tuned_h, tuned_l
         base = None
         for var, opts in tc_variants(tuned_h, tuned_l):
             r = run_tc(ctx, G.L, 5, 1, opts)
@@ -497,7 +505,7 @@ def main() -> int:
         tc.update(res)
         tc["selector_training"] = rep
         if not a.no_learned_graphs:
-            tc["other_graphs"] = run_learned_graphs(ctx, tuned_h, tuned_l)
+            tc["other_graphs"] = run_learned_graphs(ctx)
     bc = None
     sources = None
     if "bc" in legs:
