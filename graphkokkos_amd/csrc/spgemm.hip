@@ -401,27 +401,60 @@ __device__ __forceinline__ void tc_finish(const Prob &P, const Task &tk, uint32_
     if (MODE == MODE_TC && P.rowcount && cnt_task) atomicAdd(&P.rowcount[tk.row], (unsigned long long)cnt_task);
 }
 
+// A lane's share of a stream of `total` entries cut into rows by the
+// prefix pre[0, na] (pre[0] = 0, pre[na] = total): a tile of G <= 64 lanes
+// strides over the whole stream; a block of G > 64 lanes gives each wave a
+// contiguous segment, so that a lane crosses the B rows of its wave's
+// segment only (64 / row length prefix steps per entry, not G / row
+// length).  first: the lane's first position; end: one past the segment;
+// kk: the prefix row that holds the segment's start.
+struct LaneRange {
+    int32_t first, end, kk;
+};
+template <int G>
+__device__ __forceinline__ LaneRange lane_range(const int32_t *pre, int32_t na, int32_t total, int gl) {
+    LaneRange r{gl, total, 0};
+    if constexpr (G > 64) {
+        constexpr int NWAVE = G / 64;
+        const int32_t per = (total + NWAVE - 1) / NWAVE;
+        const int32_t s0 = min(total, (gl >> 6) * per);
+        r.end = min(total, s0 + per);
+        r.first = s0 + (gl & 63);
+        int32_t lo = 0, hi = na - 1;  // the largest row with pre[row] <= s0
+        while (lo < hi) {
+            const int32_t mid = (lo + hi + 1) >> 1;
+            if (pre[mid] <= s0) lo = mid; else hi = mid - 1;
+        }
+        r.kk = lo;
+    }
+    return r;
+}
+
 // Stream the concatenated B rows of a task with a tile of G lanes, U
-// entries per lane per step: the cursor (A entry kk, its B-row base) lives
-// in registers and is advanced through the LDS prefix only when an entry
-// crosses into the next B row; the U loads of B columns are issued before
-// any of them is probed.  probe(b) returns the mask position of column b or
-// -1.  Returns the number of hits.
+// entries per lane per step (lane_range: blocks split it by waves): the
+// cursor (A entry kk, its B-row base) lives in registers and is advanced
+// through the LDS prefix only when an entry crosses into the next B row;
+// the U loads of B columns are issued before any of them is probed.
+// probe(b) returns the mask position of column b or -1.  Returns the number
+// of hits.
 template <int MODE, int G, int U, class Probe>
 __device__ __forceinline__ uint32_t stream_entries(const Prob &P, const Task &tk, int32_t mb, const int32_t *pre,
                                                    const int32_t *bst, int32_t total, int gl, int32_t *acc,
                                                    int32_t *acnt, Probe probe) {
     uint32_t cnt = 0;
     if (total <= 0) return 0;
-    int32_t kk = 0, nxt = pre[1], base = bst[0];
-    for (int32_t g0 = gl; g0 < total; g0 += G * U) {
+    constexpr int L = G > 64 ? 64 : G;
+    const LaneRange r = lane_range<G>(pre, tk.a1 - tk.a0, total, gl);
+    const int32_t end = r.end;
+    int32_t kk = r.kk, nxt = pre[kk + 1], base = bst[kk];
+    for (int32_t g0 = r.first; g0 < end; g0 += L * U) {
         int32_t t[U], k[U], b[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int32_t g = g0 + G * u;
+            const int32_t g = g0 + L * u;
             t[u] = -1;
             k[u] = 0;
-            if (g < total) {
+            if (g < end) {
                 while (g >= nxt) {
                     ++kk;
                     nxt = pre[kk + 1];
@@ -451,18 +484,21 @@ __device__ __forceinline__ uint32_t stream_entries(const Prob &P, const Task &tk
 // lanes read consecutive words of a row.  Returns the number of hits.
 template <int G, int U>
 __device__ __forceinline__ uint32_t stream_hub(const Prob &P, const int32_t *hpre, const int32_t *hbst,
-                                               int32_t total, int gl, const uint32_t *bm) {
+                                               int32_t na, int32_t total, int gl, const uint32_t *bm) {
     uint32_t cnt = 0;
     if (total <= 0) return 0;
-    int32_t kk = 0, cur = 0, nxt = hpre[1], base = hbst[0];
-    for (int32_t g0 = gl; g0 < total; g0 += G * U) {
+    constexpr int L = G > 64 ? 64 : G;
+    const LaneRange r = lane_range<G>(hpre, na, total, gl);
+    const int32_t end = r.end;
+    int32_t kk = r.kk, cur = hpre[kk], nxt = hpre[kk + 1], base = hbst[kk];
+    for (int32_t g0 = r.first; g0 < end; g0 += L * U) {
         int32_t t[U], w[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int32_t g = g0 + G * u;
+            const int32_t g = g0 + L * u;
             t[u] = -1;
             w[u] = 0;
-            if (g < total) {
+            if (g < end) {
                 while (g >= nxt) {
                     ++kk;
                     cur = nxt;
@@ -488,18 +524,21 @@ __device__ __forceinline__ uint32_t stream_hub(const Prob &P, const int32_t *hpr
 // Returns the number of hits.
 template <int G, int U>
 __device__ __forceinline__ uint32_t stream_probe(const Prob &P, const int32_t *ppre, const int32_t *pofs,
-                                                 int32_t total, int gl, int32_t mb) {
+                                                 int32_t na, int32_t total, int gl, int32_t mb) {
     uint32_t cnt = 0;
     if (total <= 0) return 0;
-    int32_t kk = 0, cur = 0, nxt = ppre[1], base = pofs[0];
-    for (int32_t g0 = gl; g0 < total; g0 += G * U) {
+    constexpr int L = G > 64 ? 64 : G;
+    const LaneRange r = lane_range<G>(ppre, na, total, gl);
+    const int32_t end = r.end;
+    int32_t kk = r.kk, cur = ppre[kk], nxt = ppre[kk + 1], base = pofs[kk];
+    for (int32_t g0 = r.first; g0 < end; g0 += L * U) {
         int32_t off[U], c[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int32_t g = g0 + G * u;
+            const int32_t g = g0 + L * u;
             off[u] = -1;
             c[u] = 0;
-            if (g < total) {
+            if (g < end) {
                 while (g >= nxt) {
                     ++kk;
                     cur = nxt;
@@ -761,8 +800,8 @@ __global__ __launch_bounds__(64 * NW) void k_msa_window(Prob P, int cls) {
         __syncthreads();
         uint32_t c = 0;
         if constexpr (HUB) {
-            if (htotal) c = stream_hub<NT, kWindowU>(P, hpre, hbst, htotal, tid, bm);
-            if (ptotal) c += stream_probe<NT, kWindowU>(P, ppre, pofs, ptotal, tid, mb);
+            if (htotal) c = stream_hub<NT, kWindowU>(P, hpre, hbst, tk.a1 - tk.a0, htotal, tid, bm);
+            if (ptotal) c += stream_probe<NT, kWindowU>(P, ppre, pofs, tk.a1 - tk.a0, ptotal, tid, mb);
         }
         c += stream_entries<MODE, NT, kWindowU>(P, tk, mb, pre, bst, total, tid, acc, acnt,
                                                        [&](int32_t b) -> int32_t {
