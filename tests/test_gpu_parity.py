@@ -210,6 +210,34 @@ def test_support_rmat(gpu, scale):
     assert np.array_equal(got.astype(np.int64), want.astype(np.int64))
 
 
+@pytest.mark.parametrize("kind", ["rmat12", "star_paths", "er"])
+def test_support_window_kernel_segments(gpu, kind):
+    """Supports with every row in the block-wide window kernel, whose waves
+    each stream a contiguous segment of a task's B rows and must report the
+    A entry of every hit: whole rows, rows split into tasks of a few
+    entries, and a graph whose hub rows meet many empty and one-entry B rows
+    (segments that start among them, and streams shorter than a wave)."""
+    say("support window segments", kind)
+    if kind == "star_paths":
+        n = 3000
+        rng = np.random.default_rng(7)
+        hubs = np.arange(8)
+        edges = [(h, v) for h in hubs for v in range(8, n) if rng.random() < 0.6]
+        edges += [(v, v + 1) for v in range(8, n - 1) if rng.random() < 0.5]
+        e = np.array(edges, np.int64)
+        G = HostGraph.from_edges(n, e[:, 0], e[:, 1])
+    else:
+        G = {"rmat12": lambda: HostGraph.rmat(12, 16, seed=4), "er": lambda: HostGraph.er(4000, 20.0, seed=9)}[kind]()
+    want = O.support(G.A).astype(np.int64)
+    for split in (1 << 18, 64, 5):
+        o = {**forced(ACC_MSA_LDS), "small_max_mask": 1, "split_flop": split}
+        got = gpu.ktruss_support(G.A, o).astype(np.int64)
+        assert np.array_equal(got, want), (kind, split)
+        d = gpu.upload(G.L)
+        assert gpu.tc_count(d, o) == O.tc_sandia(G.L), (kind, split)
+        d.free()
+
+
 @pytest.mark.parametrize("k", [3, 4, 6, 10])
 def test_ktruss_single_k(gpu, k):
     G = HostGraph.rmat(11, 16, seed=7)
