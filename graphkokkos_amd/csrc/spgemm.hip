@@ -436,8 +436,11 @@ __device__ __forceinline__ LaneRange lane_range(const int32_t *pre, int32_t na, 
 // through the LDS prefix only when an entry crosses into the next B row;
 // the U loads of B columns are issued before any of them is probed.
 // probe(b) returns the mask position of column b or -1.  Returns the number
-// of hits.
-template <int MODE, int G, int U, class Probe>
+// of hits.  CLAMP: positions past the lane range's end are clamped to its
+// last entry, so that the walk and the loads need no lane branch and only
+// the probe is skipped (faster where the streams are long; where many are
+// short, the branches skip whole steps of loads).
+template <int MODE, int G, int U, bool CLAMP = false, class Probe>
 __device__ __forceinline__ uint32_t stream_entries(const Prob &P, const Task &tk, int32_t mb, const int32_t *pre,
                                                    const int32_t *bst, int32_t total, int gl, int32_t *acc,
                                                    int32_t *acnt, Probe probe) {
@@ -447,6 +450,32 @@ __device__ __forceinline__ uint32_t stream_entries(const Prob &P, const Task &tk
     const LaneRange r = lane_range<G>(pre, tk.a1 - tk.a0, total, gl);
     const int32_t end = r.end;
     int32_t kk = r.kk, nxt = pre[kk + 1], base = bst[kk];
+    if constexpr (CLAMP) {
+        for (int32_t g0 = r.first; g0 < end; g0 += L * U) {
+            int32_t t[U], k[U], b[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int32_t g = min(g0 + L * u, end - 1);
+                while (g >= nxt) {
+                    ++kk;
+                    nxt = pre[kk + 1];
+                    base = bst[kk];
+                }
+                t[u] = base + g;
+                k[u] = kk;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) b[u] = P.bcol[t[u]];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (g0 + L * u < end) {
+                    const int32_t q = probe(b[u]);
+                    if (q >= 0) Hit<MODE>::on(P, acc, acnt, q, mb, tk.a0, k[u], t[u], cnt);
+                }
+            }
+        }
+        return cnt;
+    }
     for (int32_t g0 = r.first; g0 < end; g0 += L * U) {
         int32_t t[U], k[U], b[U];
 #pragma unroll
@@ -803,7 +832,8 @@ __global__ __launch_bounds__(64 * NW) void k_msa_window(Prob P, int cls) {
             if (htotal) c = stream_hub<NT, kWindowU>(P, hpre, hbst, tk.a1 - tk.a0, htotal, tid, bm);
             if (ptotal) c += stream_probe<NT, kWindowU>(P, ppre, pofs, tk.a1 - tk.a0, ptotal, tid, mb);
         }
-        c += stream_entries<MODE, NT, kWindowU>(P, tk, mb, pre, bst, total, tid, acc, acnt,
+        // the window kernels without hub bitmaps stream long B rows: clamped
+        c += stream_entries<MODE, NT, kWindowU, !HUB>(P, tk, mb, pre, bst, total, tid, acc, acnt,
                                                        [&](int32_t b) -> int32_t {
             if (b < kWindow) {
                 if (!((bm[b >> 5] >> (b & 31)) & 1u)) return -1;
