@@ -23,7 +23,9 @@
 // Backward (pull over A): for s with i in F_d,
 //   delta(i,s) = sigma(i,s) * sum over out-neighbours w in F_{d+1} for s of W(w,s).
 #include <cstring>
+#include <memory>
 
+#include "bc.h"
 #include "gk_device.h"
 
 using namespace gk;
@@ -34,7 +36,7 @@ int exclusive_scan_i32(gk_ctx *ctx, const int32_t *in, int32_t *out, int64_t n);
 
 namespace {
 
-constexpr int kSrc = 256;
+constexpr int kSrc = bc::kSrc;
 // vertices with this many neighbours get a workgroup of kHeavyWaves waves
 constexpr int kHeavy = 1024;
 constexpr int kHeavyWaves = 16;
@@ -301,20 +303,6 @@ int grid_waves(gk_ctx *ctx, int64_t rows) {
     return (int)std::max<int64_t>(1, std::min(g, cap));
 }
 
-struct Dev {
-    std::vector<void *> allocs;
-    ~Dev() { for (void *p : allocs) (void)hipFree(p); }
-    template <class T>
-    int alloc(T **p, size_t n) {
-        if (hipMalloc((void **)p, sizeof(T) * (n ? n : 1)) != hipSuccess) {
-            set_error("hipMalloc(%zu) in BC failed", sizeof(T) * n);
-            return GK_ENOMEM;
-        }
-        allocs.push_back(*p);
-        return GK_OK;
-    }
-};
-
 // Repeated neighbours in the rows of the sources, per batch of kSrc
 // sources: (column in the batch, neighbour, multiplicity) for every
 // multiplicity above 1.  Only patterns whose rows are not strictly
@@ -346,19 +334,186 @@ int repeated_neighbours(gk_ctx *ctx, const gk_dcsr *A, const int32_t *sources, i
     return GK_OK;
 }
 
-}  // namespace
-}  // namespace gk
+// Buffers of one gk_bc_batch_dev call: each one hipMalloc'd, all freed when
+// the call ends.
+struct DevPool final : bc::Pool {
+    int64_t n;
+    std::vector<void *> allocs;
+    std::vector<uint64_t *> F;
+    explicit DevPool(int64_t n_) : n(n_) {}
+    ~DevPool() override { for (void *p : allocs) (void)hipFree(p); }
+    int get(size_t bytes, void **p) override {
+        if (hipMalloc(p, bytes) != hipSuccess) {
+            set_error("hipMalloc(%zu) in BC failed", bytes);
+            return GK_ENOMEM;
+        }
+        allocs.push_back(*p);
+        return GK_OK;
+    }
+    int level(int d, uint64_t **p) override {
+        while ((int)F.size() <= d) {
+            uint64_t *f;
+            GK_TRY(alloc(&f, (size_t)n * 4));
+            F.push_back(f);
+        }
+        *p = F[d];
+        return GK_OK;
+    }
+};
 
-extern "C" int gk_bc_batch_dev(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *AT, const int32_t *sources, int64_t nsrc,
-                               double *d_bc, const gk_opts *opts) {
+}  // namespace
+
+namespace bc {
+
+// The heavy-vertex kernels run on an auxiliary stream beside the light ones:
+// the two write disjoint vertices (their sigma or W, frontier and visited
+// bits, bc) and read only vertices of the neighbouring level.
+int sweeps(gk_ctx *ctx, const gk_dcsr *A, const int32_t *trst, const int32_t *tcol, const int32_t *sources,
+           int64_t nsrc, double *d_bc, Pool &P, const std::vector<std::vector<int32_t>> *dups, bool timed) {
+    const int64_t n = A->m;
+    double *S;
+    uint64_t *vis;
+    int32_t *dsrc;
+    unsigned long long *dcnt;
+    int32_t *hin, *hout, *nh;
+    GK_TRY(P.alloc(&S, (size_t)n * kSrc));
+    GK_TRY(P.alloc(&vis, (size_t)n * 4));
+    GK_TRY(P.alloc(&dsrc, kSrc));
+    GK_TRY(P.alloc(&dcnt, 1));
+    GK_TRY(P.alloc(&hin, n));
+    GK_TRY(P.alloc(&hout, n));
+    GK_TRY(P.alloc(&nh, 2));
+    int32_t *ddup = nullptr;
+    size_t dup_max = 0;
+    if (dups)
+        for (auto &t : *dups) dup_max = std::max(dup_max, t.size());
+    if (dup_max) GK_TRY(P.alloc(&ddup, dup_max));
+    uint64_t *f0;
+    GK_TRY(P.level(0, &f0));
+    GK_HIP(hipMemsetAsync(nh, 0, 8, ctx->stream));
+    {
+        const int g = (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, ctx->num_cus * 16));
+        k_heavy_list<<<g, 256, 0, ctx->stream>>>(n, trst, hin, nh);
+        k_heavy_list<<<g, 256, 0, ctx->stream>>>(n, A->rst, hout, nh + 1);
+        GK_TRY(launch_check("k_heavy_list"));
+    }
+    unsigned long long *h;
+    GK_TRY(pinned(ctx, 16, (void **)&h));
+    GK_HIP(hipMemcpyAsync(h, nh, 8, hipMemcpyDeviceToHost, ctx->stream));
+    GK_HIP(hipStreamSynchronize(ctx->stream));
+    int32_t nheavy[2];
+    std::memcpy(nheavy, h, 8);
+    const int gw = grid_waves(ctx, n);
+    const int gi = (int)std::max<int64_t>(1, std::min<int64_t>((4 * n + 255) / 256, ctx->num_cus * 16));
+    hipStream_t hs = ctx->aux[0];
+    // fork the heavy-vertex stream from the context stream / join it back
+    auto fork = [&]() -> int {
+        GK_HIP(hipEventRecord(ctx->fork, ctx->stream));
+        GK_HIP(hipStreamWaitEvent(hs, ctx->fork, 0));
+        return GK_OK;
+    };
+    auto join = [&]() -> int {
+        GK_HIP(hipEventRecord(ctx->join[0], hs));
+        GK_HIP(hipStreamWaitEvent(ctx->stream, ctx->join[0], 0));
+        return GK_OK;
+    };
+    int64_t levels_total = 0;
+    double fwd_ms = 0, bwd_ms = 0;
+    for (int64_t b0 = 0; b0 < nsrc || (b0 == 0 && nsrc == 0); b0 += kSrc) {
+        const int ns = (int)std::min<int64_t>(kSrc, nsrc - b0);
+        k_bc_init<<<gi, 256, 0, ctx->stream>>>(n, vis, f0, d_bc, b0 == 0);
+        GK_TRY(launch_check("k_bc_init"));
+        if (ns <= 0) break;
+        GK_HIP(hipMemcpyAsync(dsrc, sources + b0, 4 * ns, hipMemcpyHostToDevice, ctx->stream));
+        k_bc_sources<<<1, 256, 0, ctx->stream>>>(dsrc, ns, vis, f0, S);
+        GK_TRY(launch_check("k_bc_sources"));
+        // forward
+        std::unique_ptr<Timer> tf(timed ? new Timer(ctx) : nullptr);
+        if (tf) tf->start();
+        int d = 0;
+        for (;;) {
+            uint64_t *fd, *fn;
+            GK_TRY(P.level(d, &fd));
+            GK_TRY(P.level(d + 1, &fn));
+            GK_HIP(hipMemsetAsync(dcnt, 0, 8, ctx->stream));
+            if (nheavy[0]) {
+                GK_TRY(fork());
+                k_bc_forward_heavy<<<std::min(nheavy[0], ctx->num_cus * 2), 64 * kHeavyWaves, 0, hs>>>(
+                    nheavy[0], hin, trst, tcol, fd, fn, vis, S, dcnt);
+            }
+            k_bc_forward<<<gw, 256, 0, ctx->stream>>>(n, trst, tcol, fd, fn, vis, S, dcnt);
+            if (nheavy[0]) GK_TRY(join());
+            GK_TRY(launch_check("k_bc_forward"));
+            GK_HIP(hipMemcpyAsync(h, dcnt, 8, hipMemcpyDeviceToHost, ctx->stream));
+            GK_HIP(hipStreamSynchronize(ctx->stream));
+            if (h[0] == 0) break;
+            ++d;
+        }
+        if (tf) {
+            tf->stop();
+            fwd_ms += tf->ms();
+        }
+        // d = deepest non-empty level
+        std::unique_ptr<Timer> tb(timed ? new Timer(ctx) : nullptr);
+        if (tb) tb->start();
+        if (d >= 1) {
+            uint64_t *fd, *fn;
+            GK_TRY(P.level(d, &fd));
+            k_bc_last<<<gw, 256, 0, ctx->stream>>>(n, fd, S);
+            GK_TRY(launch_check("k_bc_last"));
+            for (int l = d - 1; l >= 1; --l) {
+                GK_TRY(P.level(l, &fd));
+                GK_TRY(P.level(l + 1, &fn));
+                if (nheavy[1]) {
+                    GK_TRY(fork());
+                    k_bc_backward_heavy<<<std::min(nheavy[1], ctx->num_cus * 2), 64 * kHeavyWaves, 0, hs>>>(
+                        nheavy[1], hout, A->rst, A->col, fd, fn, S, d_bc);
+                }
+                k_bc_backward<<<gw, 256, 0, ctx->stream>>>(n, A->rst, A->col, fd, fn, S, d_bc);
+                if (nheavy[1]) GK_TRY(join());
+                GK_TRY(launch_check("k_bc_backward"));
+            }
+            if (dups) {
+                const auto &t = (*dups)[b0 / kSrc];
+                if (!t.empty()) {
+                    const int32_t np = (int32_t)(t.size() / 3);
+                    GK_HIP(hipMemcpyAsync(ddup, t.data(), 4 * t.size(), hipMemcpyHostToDevice, ctx->stream));
+                    k_bc_dup_fix<<<(np + 255) / 256, 256, 0, ctx->stream>>>(np, ddup, S, d_bc);
+                    GK_TRY(launch_check("k_bc_dup_fix"));
+                }
+            }
+        }
+        if (tb) {
+            tb->stop();
+            bwd_ms += tb->ms();
+        }
+        levels_total += d;
+    }
+    ctx->last.kernel_ms = fwd_ms + bwd_ms;
+    ctx->last.select_ms = fwd_ms;  // forward share (backward = kernel_ms - select_ms)
+    ctx->last.iterations = levels_total;
+    return GK_OK;
+}
+
+int check_args(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *AT, const int32_t *sources, int64_t nsrc,
+                  const double *d_bc) {
     GK_CHECK_ARG(ctx && A && d_bc && (sources || nsrc == 0), "null argument");
     GK_CHECK_ARG(A->m == A->n, "non-square matrix; BC is only for square matrices");
     GK_CHECK_ARG(!AT || (AT->m == A->n && AT->nz == A->nz), "AT does not match A");
     for (int64_t s = 0; s < nsrc; ++s)
         GK_CHECK_ARG(sources[s] >= 0 && sources[s] < A->m, "source %lld out of range", (long long)sources[s]);
+    return GK_OK;
+}
+
+}  // namespace bc
+}  // namespace gk
+
+extern "C" int gk_bc_batch_dev(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *AT, const int32_t *sources, int64_t nsrc,
+                               double *d_bc, const gk_opts *opts) {
+    GK_TRY(bc::check_args(ctx, A, AT, sources, nsrc, d_bc));
     (void)opts;
     const int64_t n = A->m;
-    Dev D;
+    DevPool D(n);
     const int32_t *trst = AT ? AT->rst : nullptr, *tcol = AT ? AT->col : nullptr;
     // a symmetric pattern is its own transpose (row order inside A^T does
     // not matter to the forward sums)
@@ -387,108 +542,12 @@ extern "C" int gk_bc_batch_dev(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *AT,
         trst = tr;
         tcol = tc;
     }
-    double *S;
-    uint64_t *vis;
-    int32_t *dsrc;
-    unsigned long long *dcnt;
-    GK_TRY(D.alloc(&S, (size_t)n * 256));
-    GK_TRY(D.alloc(&vis, (size_t)n * 4));
-    GK_TRY(D.alloc(&dsrc, 256));
-    GK_TRY(D.alloc(&dcnt, 1));
-    // heavy vertices: in-degree (forward) and out-degree (backward) lists
-    int32_t *hin, *hout, *nh;
-    GK_TRY(D.alloc(&hin, n));
-    GK_TRY(D.alloc(&hout, n));
-    GK_TRY(D.alloc(&nh, 2));
-    GK_HIP(hipMemsetAsync(nh, 0, 8, ctx->stream));
-    {
-        const int g = (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, ctx->num_cus * 16));
-        k_heavy_list<<<g, 256, 0, ctx->stream>>>(n, trst, hin, nh);
-        k_heavy_list<<<g, 256, 0, ctx->stream>>>(n, A->rst, hout, nh + 1);
-        GK_TRY(launch_check("k_heavy_list"));
-    }
-    int32_t nheavy[2];
-    GK_HIP(d2h(ctx, nheavy, nh, 8));
     std::vector<std::vector<int32_t>> dups;
     GK_TRY(repeated_neighbours(ctx, A, sources, nsrc, &dups));
-    int32_t *ddup = nullptr;
-    size_t dup_max = 0;
-    for (auto &t : dups) dup_max = std::max(dup_max, t.size());
-    if (dup_max) GK_TRY(D.alloc(&ddup, dup_max));
-    std::vector<uint64_t *> F;  // per-level frontier bitmaps, grown on demand
-    unsigned long long *h;
-    GK_TRY(pinned(ctx, 16, (void **)&h));
-    const int gw = grid_waves(ctx, n);
-    const int gi = (int)std::max<int64_t>(1, std::min<int64_t>((4 * n + 255) / 256, ctx->num_cus * 16));
-    int64_t levels_total = 0;
-    double fwd_ms = 0, bwd_ms = 0;
-    for (int64_t b0 = 0; b0 < nsrc || (b0 == 0 && nsrc == 0); b0 += kSrc) {
-        const int ns = (int)std::min<int64_t>(kSrc, nsrc - b0);
-        if (F.empty()) {
-            uint64_t *f0;
-            GK_TRY(D.alloc(&f0, (size_t)n * 4));
-            F.push_back(f0);
-        }
-        k_bc_init<<<gi, 256, 0, ctx->stream>>>(n, vis, F[0], d_bc, b0 == 0);
-        GK_TRY(launch_check("k_bc_init"));
-        if (ns <= 0) break;
-        GK_HIP(hipMemcpyAsync(dsrc, sources + b0, 4 * ns, hipMemcpyHostToDevice, ctx->stream));
-        k_bc_sources<<<1, 256, 0, ctx->stream>>>(dsrc, ns, vis, F[0], S);
-        GK_TRY(launch_check("k_bc_sources"));
-        // forward
-        Timer tf(ctx);
-        tf.start();
-        int d = 0;
-        for (;;) {
-            if ((int)F.size() < d + 2) {
-                uint64_t *f;
-                GK_TRY(D.alloc(&f, (size_t)n * 4));
-                F.push_back(f);
-            }
-            GK_HIP(hipMemsetAsync(dcnt, 0, 8, ctx->stream));
-            k_bc_forward<<<gw, 256, 0, ctx->stream>>>(n, trst, tcol, F[d], F[d + 1], vis, S, dcnt);
-            if (nheavy[0])
-                k_bc_forward_heavy<<<std::min(nheavy[0], ctx->num_cus * 2), 64 * kHeavyWaves, 0, ctx->stream>>>(
-                    nheavy[0], hin, trst, tcol, F[d], F[d + 1], vis, S, dcnt);
-            GK_TRY(launch_check("k_bc_forward"));
-            GK_HIP(hipMemcpyAsync(h, dcnt, 8, hipMemcpyDeviceToHost, ctx->stream));
-            GK_HIP(hipStreamSynchronize(ctx->stream));
-            if (h[0] == 0) break;
-            ++d;
-        }
-        tf.stop();
-        fwd_ms += tf.ms();
-        // d = deepest non-empty level
-        Timer tb(ctx);
-        tb.start();
-        if (d >= 1) {
-            k_bc_last<<<gw, 256, 0, ctx->stream>>>(n, F[d], S);
-            GK_TRY(launch_check("k_bc_last"));
-            for (int l = d - 1; l >= 1; --l) {
-                k_bc_backward<<<gw, 256, 0, ctx->stream>>>(n, A->rst, A->col, F[l], F[l + 1], S, d_bc);
-                if (nheavy[1])
-                    k_bc_backward_heavy<<<std::min(nheavy[1], ctx->num_cus * 2), 64 * kHeavyWaves, 0,
-                                          ctx->stream>>>(nheavy[1], hout, A->rst, A->col, F[l], F[l + 1], S, d_bc);
-                GK_TRY(launch_check("k_bc_backward"));
-            }
-            const auto &t = dups[b0 / kSrc];
-            if (!t.empty()) {
-                const int32_t np = (int32_t)(t.size() / 3);
-                GK_HIP(hipMemcpyAsync(ddup, t.data(), 4 * t.size(), hipMemcpyHostToDevice, ctx->stream));
-                k_bc_dup_fix<<<(np + 255) / 256, 256, 0, ctx->stream>>>(np, ddup, S, d_bc);
-                GK_TRY(launch_check("k_bc_dup_fix"));
-            }
-        }
-        tb.stop();
-        bwd_ms += tb.ms();
-        levels_total += d;
-    }
-    total.stop();
     ctx->last = gk_timing{};
+    GK_TRY(bc::sweeps(ctx, A, trst, tcol, sources, nsrc, d_bc, D, &dups, true));
+    total.stop();
     ctx->last.total_ms = total.ms();
-    ctx->last.kernel_ms = fwd_ms + bwd_ms;
-    ctx->last.select_ms = fwd_ms;  // forward share (backward = kernel_ms - select_ms)
-    ctx->last.iterations = levels_total;
     return GK_OK;
 }
 

@@ -14,6 +14,7 @@
 #include <cstdint>
 #include <cstring>
 #include <fstream>
+#include <iomanip>
 #include <iostream>
 #include <memory>
 #include <string>
@@ -227,6 +228,108 @@ private:
     }
 };
 
+// dst takes src's sizes, matcode and a copy of its arrays; resizing dst's
+// own arrays first, so other spMtx sharing them keep the old contents
+// (include/matrix.h:536-546).
+template <typename ValT>
+void deep_copy(spMtx<ValT> &dst, const spMtx<ValT> &src) {
+    dst.m = src.m;
+    dst.n = src.n;
+    dst.resizeRows(src.m);
+    dst.resizeVals(src.nz);
+    std::copy(src.Rst.data(), src.Rst.data() + src.m + 1, dst.Rst.data());
+    std::copy(src.Col.data(), src.Col.data() + src.nz, dst.Col.data());
+    std::copy(src.Val.data(), src.Val.data() + src.nz, dst.Val.data());
+    std::memcpy(dst.matcode, src.matcode, sizeof(MM_typecode));
+}
+
+// Dense row-major m x n matrix, Val(i * n + j), zero-initialised
+// (include/matrix.h:548-630).  Copies share Val like the reference's
+// Kokkos::View member; a sparse matrix assigned or converted to it is
+// scattered into zeros.  One deliberate difference: copy-assignment between
+// two distinct objects shares src's values here, while the reference's
+// operator= returns before copying anything when the objects differ
+// (include/matrix.h:575-577).
+template <typename ValT>
+class denseMtx {
+public:
+    size_t m = 0;
+    size_t n = 0;
+    gk::View<ValT> Val;
+
+    denseMtx() {}
+    denseMtx(size_t _m, size_t _n) : m(_m), n(_n), Val("", _m * _n) {}
+    denseMtx(const denseMtx &src) : m(src.m), n(src.n), Val(src.Val) {}
+    denseMtx(denseMtx &&mov) noexcept : m(mov.m), n(mov.n) { std::swap(Val, mov.Val); }
+    template <typename ValT2>
+    denseMtx(const spMtx<ValT2> &src) : m(src.m), n(src.n), Val("", src.m * src.n) {
+        scatter(src);
+    }
+
+    denseMtx &operator=(const denseMtx &src) {
+        if (this != &src) {
+            m = src.m;
+            n = src.n;
+            Val = src.Val;
+        }
+        return *this;
+    }
+    denseMtx &operator=(denseMtx &&mov) noexcept {
+        m = mov.m;
+        n = mov.n;
+        std::swap(Val, mov.Val);
+        return *this;
+    }
+    // every entry reset to ValT(), then src's entries written in
+    template <typename ValT2>
+    denseMtx &operator=(const spMtx<ValT2> &src) {
+        m = src.m;
+        n = src.n;
+        gk::resize(Val, m * n);
+        std::fill(Val.data(), Val.data() + m * n, ValT());
+        scatter(src);
+        return *this;
+    }
+
+    bool operator==(const denseMtx &B) const {
+        return m == B.m && n == B.n && Val.size() == B.Val.size() &&
+               std::equal(Val.data(), Val.data() + Val.size(), B.Val.data());
+    }
+
+    // Kokkos::resize of the flat array: the first min(old, new) values kept
+    void resize(size_t newM, size_t newN) {
+        m = newM;
+        n = newN;
+        gk::resize(Val, newM * newN);
+    }
+
+    void print() const {
+        for (size_t i = 0; i < m; ++i) {
+            for (size_t j = 0; j < n; ++j) std::cout << std::setw(5) << Val(i * n + j) << ' ';
+            std::cout << '\n';
+        }
+        std::cout << '\n';
+    }
+
+private:
+    template <typename ValT2>
+    void scatter(const spMtx<ValT2> &src) {
+        for (size_t i = 0; i < src.m; ++i)
+            for (int k = src.Rst(i); k < src.Rst(i + 1); ++k)
+                Val(i * n + (size_t)src.Col(k)) = static_cast<ValT>(src.Val(k));
+    }
+};
+
+// src's pattern with its values cast to ValT2; the matcode is not copied
+// (include/matrix.h:632-641).
+template <typename ValT2, typename ValT1>
+spMtx<ValT2> convertType(const spMtx<ValT1> &src) {
+    spMtx<ValT2> r;
+    r.deep_copy_pattern(src);
+    for (size_t j = 0; j < src.nz; ++j) r.Val(j) = static_cast<ValT2>(src.Val(j));
+    return r;
+}
+
 // Counting-sort transpose; rows of the result come out sorted
 // (include/matrix_la.h:155-179).
 template <typename T>
@@ -295,6 +398,31 @@ spMtx<T> build_symm_from_lower(const spMtx<T> &Low) {
             else          { R.Col(w) = U.Col(b);   R.Val(w++) = U.Val(b++); }
         }
         R.Rst(i + 1) = (int)w;
+    }
+    return R;
+}
+
+// What remains of each row after its leading run of columns on or below
+// the diagonal; on sorted rows that is the strictly upper part
+// (include/matrix.h:728-755; the reference's version assigns raw arrays to
+// its Views and does not compile when instantiated).
+template <typename T>
+spMtx<T> extract_upper_triangle(const spMtx<T> &G) {
+    spMtx<T> R(G.m, G.n);
+    std::memcpy(R.matcode, G.matcode, sizeof(MM_typecode));
+    for (size_t i = 0; i < G.m; ++i) {
+        int p = G.Rst(i);
+        while (p < G.Rst(i + 1) && (size_t)G.Col(p) <= i) ++p;
+        R.Rst(i + 1) = R.Rst(i) + (G.Rst(i + 1) - p);
+    }
+    R.nz = (size_t)R.Rst(G.m);
+    R.Col = gk::View<int>("", R.nz);
+    R.Val = gk::View<T>("", R.nz);
+    for (size_t i = 0; i < G.m; ++i) {
+        const int len = R.Rst(i + 1) - R.Rst(i);
+        const int from = G.Rst(i + 1) - len;
+        std::copy(G.Col.data() + from, G.Col.data() + from + len, R.Col.data() + R.Rst(i));
+        std::copy(G.Val.data() + from, G.Val.data() + from + len, R.Val.data() + R.Rst(i));
     }
     return R;
 }

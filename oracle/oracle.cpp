@@ -365,17 +365,44 @@ int or_truss_peel(int64_t m, int64_t nz, const int32_t *rst, const int32_t *col,
     for (int64_t i = 0; i < m; ++i)
         for (int64_t p = rst[i]; p < rst[i + 1]; ++p)
             if (col[p] > i) eid[p] = eid[find(col[p], (int32_t)i)];
+    // Common neighbours of u and v (both rows sorted).  Rows of similar
+    // length are merged; when one row is 16x longer or more, every entry of
+    // the shorter row is looked up in the longer one by a galloping search
+    // that resumes where the previous lookup ended, so a hub costs about
+    // min(deg) * log(max deg / min deg) instead of its whole row.  visit(a, b)
+    // receives the positions of each common column in u's and v's rows.
+    auto common = [&](int32_t u, int32_t v, auto &&visit) {
+        int64_t a = rst[u], ae = rst[u + 1], b = rst[v], be = rst[v + 1];
+        bool swapped = (ae - a) > (be - b);
+        if (swapped) { std::swap(a, b); std::swap(ae, be); }
+        if ((be - b) < 16 * (ae - a)) {
+            while (a < ae && b < be) {
+                if (col[a] < col[b]) ++a;
+                else if (col[a] > col[b]) ++b;
+                else {
+                    if (swapped) visit(b, a); else visit(a, b);
+                    ++a; ++b;
+                }
+            }
+            return;
+        }
+        for (; a < ae && b < be; ++a) {
+            const int32_t c = col[a];
+            int64_t step = 1, lo = b;
+            while (lo + step < be && col[lo + step] < c) { lo += step; step <<= 1; }
+            const int32_t *x = std::lower_bound(col + lo, col + std::min(be, lo + step + 1), c);
+            b = x - col;
+            if (b < be && col[b] == c) {
+                if (swapped) visit(b, a); else visit(a, b);
+                ++b;
+            }
+        }
+    };
     std::vector<int32_t> sup(ne, 0);
 #pragma omp parallel for schedule(dynamic, 256)
     for (int64_t e = 0; e < ne; ++e) {
-        int32_t u = eu[e], v = ev[e];
-        int64_t a = rst[u], ae = rst[u + 1], b = rst[v], be = rst[v + 1];
         int32_t s = 0;
-        while (a < ae && b < be) {
-            if (col[a] < col[b]) ++a;
-            else if (col[a] > col[b]) ++b;
-            else { ++s; ++a; ++b; }
-        }
+        common(eu[e], ev[e], [&](int64_t, int64_t) { ++s; });
         sup[e] = s;
     }
     std::vector<uint8_t> alive(ne, 1), queued(ne, 0);
@@ -388,22 +415,15 @@ int or_truss_peel(int64_t m, int64_t nz, const int32_t *rst, const int32_t *col,
             if (alive[e] && !queued[e] && sup[e] < k - 2) { q.push_back(e); queued[e] = 1; }
         for (size_t h = 0; h < q.size(); ++h) {
             int64_t e = q[h];
-            int32_t u = eu[e], v = ev[e];
-            int64_t a = rst[u], ae = rst[u + 1], b = rst[v], be = rst[v + 1];
-            while (a < ae && b < be) {
-                if (col[a] < col[b]) ++a;
-                else if (col[a] > col[b]) ++b;
-                else {
-                    int64_t e1 = eid[a], e2 = eid[b];
-                    if (alive[e1] && alive[e2]) {
-                        for (int64_t f : {e1, e2}) {
-                            --sup[f];
-                            if (!queued[f] && sup[f] < k - 2) { queued[f] = 1; q.push_back(f); }
-                        }
+            common(eu[e], ev[e], [&](int64_t a, int64_t b) {
+                int64_t e1 = eid[a], e2 = eid[b];
+                if (alive[e1] && alive[e2]) {
+                    for (int64_t f : {e1, e2}) {
+                        --sup[f];
+                        if (!queued[f] && sup[f] < k - 2) { queued[f] = 1; q.push_back(f); }
                     }
-                    ++a; ++b;
                 }
-            }
+            });
             alive[e] = 0;
             tr[e] = (uint32_t)(k - 1);
             --left;

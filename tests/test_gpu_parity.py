@@ -45,7 +45,7 @@ def test_golden_all(gpu, name):
     for acc in ACCS:
         assert gpu.tc_count(G.L, forced(acc)) == int(z["triangles"]), acc
     assert gpu.tc_count(G.L) == int(z["triangles"])
-    for hw in (32, 1024):
+    for hw in (0, 32, 1024):
         assert gpu.tc_count(G.L, {**forced(ACC_MSA_LDS), "hub_window": hw}) == int(z["triangles"]), hw
     assert np.array_equal(gpu.ktruss_support(A).astype(np.int64), z["support"].astype(np.int64))
     tr, kmax = gpu.truss_decompose(A)
@@ -101,8 +101,10 @@ def test_tc_per_row_and_drop_in(gpu):
 
 # Hub bitmaps: B rows below hub_window read as bitmap words where that is
 # cheaper.  Windows that are not multiples of 32, wider than the graph, and
-# the largest one; the window kernels forced, the heuristic, per-row counts.
-HUB_WINDOWS = [37, 64, 4096, 32768]
+# the largest one; 0 turns them off (every B row streamed, the clamped
+# stream of the window kernels); the window kernels forced, the heuristic,
+# per-row counts.
+HUB_WINDOWS = [0, 37, 64, 4096, 32768]
 
 
 @pytest.mark.parametrize("kind", ["rmat12", "rmat14", "chunglu", "grid", "er"])
@@ -229,13 +231,17 @@ def test_support_window_kernel_segments(gpu, kind):
     else:
         G = {"rmat12": lambda: HostGraph.rmat(12, 16, seed=4), "er": lambda: HostGraph.er(4000, 20.0, seed=9)}[kind]()
     want = O.support(G.A).astype(np.int64)
+    tc_want = O.tc_sandia(G.L)
+    d = gpu.upload(G.L)
     for split in (1 << 18, 64, 5):
         o = {**forced(ACC_MSA_LDS), "small_max_mask": 1, "split_flop": split}
         got = gpu.ktruss_support(G.A, o).astype(np.int64)
         assert np.array_equal(got, want), (kind, split)
-        d = gpu.upload(G.L)
-        assert gpu.tc_count(d, o) == O.tc_sandia(G.L), (kind, split)
-        d.free()
+        # triangle counting through the same kernel with hub bitmaps (the
+        # default) and without them (the clamped stream that the supports use)
+        for hw in (32768, 0):
+            assert gpu.tc_count(d, {**o, "hub_window": hw}) == tc_want, (kind, split, hw)
+    d.free()
 
 
 @pytest.mark.parametrize("k", [3, 4, 6, 10])
@@ -250,6 +256,19 @@ def test_ktruss_single_k(gpu, k):
     assert np.array_equal(res.Col, G.A.Col[keep])
     assert it_gpu == it
     assert k_truss(G.A, k, mxmm_msa, True) == res
+
+
+@pytest.mark.parametrize("r", [2, 3, 8, 13, 40])
+def test_ktruss_iteration_count_closed_form(gpu, r):
+    """gk_ktruss's iteration count (the reference's "Iterations:" line,
+    src/main.cpp:104-138) on the triangulated r x r grid: 2*floor(r/2)+1
+    for k = 4 (one perimeter layer peeled per product; derivation in
+    test_oracle.grid_ktruss4_iterations), 1 for k = 3, 2 for k = 5."""
+    say("ktruss iterations grid", r)
+    G = HostGraph.grid(r)
+    for k, want_it, want_nz in ((4, 2 * (r // 2) + 1, 0), (3, 1, G.A.nz), (5, 2, 0)):
+        res, it = gpu.ktruss(G.A, k)
+        assert it == want_it and res.nz == want_nz, (k, it, res.nz)
 
 
 @pytest.mark.parametrize("scale,seed", [(10, 1), (12, 2), (14, 1), (16, 1)])

@@ -92,6 +92,57 @@ def test_triangulated_grid_closed_forms(r):
     assert not keep.any()  # empty 4-truss
 
 
+def grid_ktruss4_iterations(r: int) -> int:
+    """The reference loop's iteration count (src/main.cpp:104-131, the value
+    it prints as "Iterations:") for k = 4 on the r x r triangulated grid.
+
+    Every interior edge has support 2 and every perimeter edge support 1, so
+    iteration 1 removes the perimeter.  A triangle is gone from the first
+    product after any of its edges is removed, and an interior edge (two
+    triangles) is removed by the product after the first of its triangles
+    is gone.  So a triangle d steps away from a perimeter triangle in the
+    dual graph (triangles adjacent when they share an edge) is gone after
+    iteration d + 1, and the last edges to go are those whose two triangles
+    are both deepest.  With the diagonal of cell (i, j) running from (i, j)
+    to (i+1, j+1), the upper triangle of the cell is min(2i, 2(r-2-j),
+    2(r-2-i)+1, 2j+1) steps deep and the lower one min(2(r-2-i), 2j, 2i+1,
+    2(r-2-j)+1); at the centre that is r - 2 for even r and r - 3 for odd r
+    (where no cell holds two triangles of depth r - 2).  The deepest edge
+    goes in iteration depth + 2, and the loop needs one more product that
+    removes nothing: 2*floor(r/2) + 1 iterations."""
+    return 2 * (r // 2) + 1
+
+
+@pytest.mark.parametrize("r", range(2, 14))
+def test_ktruss_iteration_count_closed_forms(r):
+    G = HostGraph.grid(r)
+    keep, it = O.ktruss(G.A, 4)
+    assert not keep.any() and it == grid_ktruss4_iterations(r)
+    # k = 3: every grid edge lies in a triangle, so the first product removes
+    # nothing; k = 5: every support is below 3, so all edges go at once and
+    # the second product (on the empty pattern) removes nothing
+    keep, it = O.ktruss(G.A, 3)
+    assert keep.all() and it == 1
+    keep, it = O.ktruss(G.A, 5)
+    assert not keep.any() and it == 2
+
+
+@pytest.mark.parametrize("n", [4, 7])
+def test_ktruss_iteration_count_pendant_and_complete(n):
+    # K_n: supports n-2, so k <= n keeps everything in one iteration and
+    # k = n + 1 removes everything in the first
+    G = complete(n)
+    for k, want_keep, want_it in ((3, True, 1), (n, True, 1), (n + 1, False, 2)):
+        keep, it = O.ktruss(G.A, k)
+        assert bool(keep.all() if want_keep else not keep.any()) and it == want_it, (k, it)
+    # a pendant edge lies in no triangle: its removal lowers no support, so
+    # k = 3 takes exactly two iterations whatever else the graph holds
+    u, v = zip(*[(i, j) for i in range(n) for j in range(i)] + [(0, n)])
+    G = HostGraph.from_edges(n + 1, u, v)
+    keep, it = O.ktruss(G.A, 3)
+    assert it == 2 and int(keep.sum()) == n * (n - 1)
+
+
 def test_bipartite_has_no_triangles():
     u, v = zip(*[(i, 10 + j) for i in range(10) for j in range(7)])
     G = HostGraph.from_edges(17, u, v)
