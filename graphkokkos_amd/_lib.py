@@ -115,6 +115,7 @@ _SIGS = {
     "gk_bc_batch_f64": (C.c_int, [C.c_void_p, C.POINTER(GkCsr), i32p, C.c_int64, f64p, C.POINTER(GkOpts)]),
     "gk_bc_batch_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, i32p, C.c_int64, C.c_void_p,
                                   C.POINTER(GkOpts)]),
+    "gk_bc_batch_dev_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, i32p, C.c_int64, C.c_void_p]),
     "gk_batch_tc_ktruss_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, i64p, C.c_int, C.c_int, C.c_int, u64p,
                                          i64p, C.c_void_p, f64p, u64p, C.POINTER(GkOpts)]),
     "gk_model_num_params": (C.c_int, []),

@@ -330,6 +330,13 @@ class Context:
         check(lib().gk_bc_batch_dev(self.handle, A.handle, AT.handle if AT else None, _ptr(src), len(src),
                                     out.ptr, C.byref(o)), "gk_bc_batch_dev")
 
+    def bc_batch_dev_noalloc(self, A: DeviceCsr, sources, out: DeviceBuffer, AT: DeviceCsr | None = None) -> None:
+        """gk_bc_batch_dev_noalloc: the same scores with every work buffer in
+        the library's fixed per-device scratch array (no device allocation)."""
+        src = np.ascontiguousarray(sources, np.int32)
+        check(lib().gk_bc_batch_dev_noalloc(self.handle, A.handle, AT.handle if AT else None, _ptr(src), len(src),
+                                            out.ptr), "gk_bc_batch_dev_noalloc")
+
 
 _default: Context | None = None
 

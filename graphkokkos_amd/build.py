@@ -26,7 +26,7 @@ DROPIN_SRC = ROOT / "tests" / "cpp" / "dropin_check.cpp"
 DROPIN_BIN = ROOT / "tests" / "cpp" / "_build" / "dropin_check"
 
 HIP_SOURCES = ["gk_runtime.hip", "spgemm.hip", "ktruss.hip", "bc.hip", "selector.hip", "batch.hip", "graph_device.hip", "cmask.hip",
-               "tc_vertex.hip"]
+               "tc_vertex.hip", "bc_noalloc.hip"]
 HOST_SOURCES = ["host_graph.cpp"]
 ARCH = os.environ.get("GK_OFFLOAD_ARCH", "gfx950")
 

@@ -74,6 +74,9 @@ struct gk_ctx {
     // The same for the work buffers of the complement-mask product.
     std::vector<std::pair<void *, size_t>> cmask_cache;
     bool cmask_cache_busy = false;
+    // Device address of this device's fixed scratch array of the
+    // allocation-free BC entry point (bc_noalloc.hip), set by gk_init.
+    void *bc_scratch = nullptr;
 };
 
 struct gk_dcsr {

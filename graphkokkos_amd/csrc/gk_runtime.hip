@@ -240,6 +240,8 @@ extern "C" void gk_opts_default(gk_opts *o) {
     o->hub_probe = 4;
 }
 
+namespace gk { int bc_noalloc_setup(gk_ctx *ctx); }
+
 extern "C" int gk_init(int device, gk_ctx **out) {
     GK_CHECK_ARG(out, "out is NULL");
     int count = 0;
@@ -263,6 +265,15 @@ extern "C" int gk_init(int device, gk_ctx **out) {
         delete ctx;
         set_error("stream or event creation failed");
         return GK_EHIP;
+    }
+    // Set-up of the allocation-free entry point: the pinned staging buffer
+    // and this device's scratch array, so that its calls allocate nothing.
+    void *staging = nullptr;
+    int rc = pinned(ctx, 16, &staging);
+    if (rc == GK_OK) rc = bc_noalloc_setup(ctx);
+    if (rc != GK_OK) {
+        gk_finalize(ctx);
+        return rc;
     }
     *out = ctx;
     return GK_OK;

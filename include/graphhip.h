@@ -320,6 +320,23 @@ int gk_bc_batch_dev(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *AT,
                     const int32_t *sources, int64_t nsrc, double *d_bc,
                     const gk_opts *o);
 
+/* ---- allocation-free BC ---------------------------------------------------
+ * No counterpart in GraphKokkos.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc): gk_bc_batch_dev's sweeps and results with every work
+ * buffer in a fixed array of GK_BC_NOALLOC_SCRATCH_BYTES per GPU, which the
+ * library holds from gk_init until the process ends; the call itself
+ * allocates nothing.  The caller passes A (and AT), the sources and d_bc.
+ * Limits: A's rows sorted without repeats (as gk_dcsr_upload records);
+ * AT given, or A symmetric (no transpose is built); n vertices need
+ * n x (2 KiB + 40 B) plus n x 32 B per BFS level, levels 0 to the first
+ * empty one (about 55,000 vertices at 8 levels), GK_ERANGE beyond that.  Calls on one GPU share its array, so
+ * calls that overlap on one GPU race.  gk_last_timing reports the levels
+ * only (no event timers).                                                  */
+#define GK_BC_NOALLOC_SCRATCH_BYTES ((size_t)128 << 20)
+int gk_bc_batch_dev_noalloc(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *AT,
+                            const int32_t *sources, int64_t nsrc, double *d_bc);
+
 /* ---- batching: many independent graphs in one launch sequence -----------
  * A is the block-diagonal union of ngraphs simple undirected graphs (graph g
  * on vertices [voff[g], voff[g+1]), voff[0] = 0, voff[ngraphs] = A.m; an
