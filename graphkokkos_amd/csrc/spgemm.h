@@ -135,15 +135,17 @@ __device__ __forceinline__ void hub_choice(const Prob &P, int32_t k, int32_t bel
     if (k >= P.hub_n || len == 0) return;
     const int32_t words = (k + 31) >> 5;
     const int32_t cost = P.probe * below;  // a test is a gathered word, a word or entry a streamed one
-    if (P.probe && cost < len && cost < words) {
-        plen = below;
-        pbs = k * P.hub_w;
-        len = 0;
-    } else if (words < len) {
-        hlen = words;
-        hbs = k * P.hub_w;
-        len = 0;
-    }
+    const int32_t off = k * P.hub_w;
+    // Value selects, not branches that store to different outputs: the
+    // branch form left the compiler choosing the output's address at run
+    // time, which put both outputs in private (scratch) memory.
+    const bool tests = P.probe && cost < len && cost < words;
+    const bool bitmap = !tests && words < len;
+    plen = tests ? below : plen;
+    pbs = tests ? off : pbs;
+    hlen = bitmap ? words : hlen;
+    hbs = bitmap ? off : hbs;
+    len = (tests || bitmap) ? 0 : len;
 }
 
 // Launch shape of one product's class kernels and the scratch of the
