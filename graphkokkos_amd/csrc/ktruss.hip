@@ -342,7 +342,7 @@ __device__ __forceinline__ int32_t adj_probe(const int64_t *__restrict__ hoff, c
     }
 }
 
-// Round prologue.  The previous round's frontier Fp becomes removed
+// Round prologue (prep_edges).  The previous round's frontier Fp becomes removed
 // (state 2); the new frontier F becomes state 1 with truss label `label`.
 // Per frontier edge f = (y, x): y = endpoint with the longer current row
 // (probed through its table), x = the other (its alive row is streamed;
@@ -351,20 +351,62 @@ __device__ __forceinline__ int32_t adj_probe(const int64_t *__restrict__ hoff, c
 // support of a frontier edge counts its triangles that are still intact
 // (every decrement of earlier rounds has landed), so such an edge closes
 // no triangle this round and its row need not be streamed.
-__device__ __forceinline__ int32_t prep_edge(int32_t f, const int32_t *__restrict__ F, uint8_t *__restrict__ st,
-                                             uint32_t *__restrict__ truss, uint32_t label,
-                                             const int32_t *__restrict__ eu, const int32_t *__restrict__ ev,
-                                             const int32_t *__restrict__ sup, const int32_t *__restrict__ rst,
-                                             int32_t *__restrict__ fy, int32_t *__restrict__ frs) {
-    const int32_t e = F[f], u = eu[e], v = ev[e];
-    st[e] = 1;
-    truss[e] = label;
-    const int32_t ru = rst[u], rv = rst[v];
-    const int32_t du = rst[u + 1] - ru, dv = rst[v + 1] - rv;
-    const bool uy = du > dv || (du == dv && u > v);
-    fy[f] = uy ? u : v;
-    frs[f] = uy ? rv : ru;
-    return sup[e] > 0 ? (uy ? dv : du) : 0;
+template <int P>
+__device__ __forceinline__ void prep_edges(const int32_t (&f)[P], int32_t nF, const int32_t *__restrict__ F,
+                                           uint8_t *__restrict__ st, uint32_t *__restrict__ truss, uint32_t label,
+                                           const int32_t *__restrict__ eu, const int32_t *__restrict__ ev,
+                                           const int32_t *__restrict__ sup, const int32_t *__restrict__ rst,
+                                           int32_t *__restrict__ fy, int32_t *__restrict__ frs, int32_t (&len)[P]) {
+    // P edges at a time, each level of loads issued for all of them before
+    // any is used: edge ids -> ends and supports -> row bounds
+    int32_t e[P], u[P], v[P], s[P], ru[P], ru1[P], rv[P], rv1[P];
+#pragma unroll
+    for (int j = 0; j < P; ++j) e[j] = f[j] < nF ? F[f[j]] : -1;
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        u[j] = v[j] = s[j] = 0;
+        if (e[j] >= 0) {
+            u[j] = eu[e[j]];
+            v[j] = ev[e[j]];
+            s[j] = sup[e[j]];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        ru[j] = ru1[j] = rv[j] = rv1[j] = 0;
+        if (e[j] >= 0) {
+            ru[j] = rst[u[j]];
+            ru1[j] = rst[u[j] + 1];
+            rv[j] = rst[v[j]];
+            rv1[j] = rst[v[j] + 1];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        len[j] = 0;
+        if (e[j] < 0) continue;
+        st[e[j]] = 1;
+        truss[e[j]] = label;
+        const int32_t du = ru1[j] - ru[j], dv = rv1[j] - rv[j];
+        const bool uy = du > dv || (du == dv && u[j] > v[j]);
+        fy[f[j]] = uy ? u[j] : v[j];
+        frs[f[j]] = uy ? rv[j] : ru[j];
+        len[j] = s[j] > 0 ? (uy ? dv : du) : 0;
+    }
+}
+
+// The previous frontier's edges become removed (state 2); P at a time.
+template <int P>
+__device__ __forceinline__ void mark_removed(int32_t nP, const int32_t *__restrict__ Fp, uint8_t *__restrict__ st,
+                                             int32_t first, int32_t stride) {
+    for (int32_t f0 = first; f0 < nP; f0 += P * stride) {
+        int32_t e[P];
+#pragma unroll
+        for (int j = 0; j < P; ++j) e[j] = f0 + j * stride < nP ? Fp[f0 + j * stride] : -1;
+#pragma unroll
+        for (int j = 0; j < P; ++j)
+            if (e[j] >= 0) st[e[j]] = 2;
+    }
 }
 
 // The streamed items of a round are cut into chunks of kPeelChunk (one
@@ -382,6 +424,9 @@ __device__ __forceinline__ void mark_chunks(int32_t f, int32_t pre_f, int32_t le
 
 // streamed entries per lane in flight in the peeling kernels
 constexpr int kPeelU = 4;
+// a stream chunk whose items belong to at most kOwnerWin + 1 frontier edges
+// finds each item's owner among their prefix values in registers
+constexpr int kOwnerWin = 16;
 
 // Frontiers of at most kSmallRound edges get their prologue and the scan of
 // len in one block (k_dround_prologue).
@@ -440,6 +485,64 @@ struct NearList {
     int32_t hi;
 };
 
+// A workgroup's appends to one list, staged in LDS: waves reserve room with
+// an LDS atomic, and the workgroup copies the staged entries out at its
+// end with one global atomic per 256 entries, instead of one global atomic
+// per wave and step on the list's counter, which every wave of the grid
+// shares (same-address atomics serialise in L2).  A wave whose entries do
+// not fit writes them out itself and fills its part of the stage with -1.
+constexpr int kStage = 1024;
+
+struct Stage {
+    int32_t n;
+    int32_t v[kStage];
+};
+
+template <int N>
+__device__ __forceinline__ void stage_append(const int32_t (&v)[N], Stage &S, int32_t *__restrict__ out,
+                                             int32_t *__restrict__ counter) {
+    int32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) c += v[i] >= 0;
+    if (!__ballot(c > 0)) return;
+    const int32_t incl = dev::wave_incl_scan(c);
+    const int lane = threadIdx.x & 63;
+    int32_t base = 0;
+    if (lane == 63) base = atomicAdd(&S.n, incl);
+    base = __shfl(base, 63, 64);
+    const int32_t tot = __shfl(incl, 63, 64);
+    int32_t o = base + incl - c;
+    if (base + tot <= kStage) {
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+            if (v[i] >= 0) S.v[o++] = v[i];
+        return;
+    }
+    for (int32_t x = base + lane; x < min(base + tot, kStage); x += 64) S.v[x] = -1;
+    int32_t g = 0;
+    if (lane == 63) g = atomicAdd(counter, tot);
+    o = __shfl(g, 63, 64) + incl - c;
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+        if (v[i] >= 0) out[o++] = v[i];
+}
+
+// Copies a stage's entries (those >= 0) to out; every thread of the block
+// calls it once, after all of the block's stage_append calls.
+template <int NT>
+__device__ __forceinline__ void stage_flush(Stage &S, int32_t *__restrict__ out, int32_t *__restrict__ counter,
+                                            int32_t *lds) {
+    __syncthreads();
+    const int32_t n = min(S.n, kStage);
+    for (int32_t b0 = 0; b0 < n; b0 += NT) {
+        const int32_t x = b0 + (int32_t)threadIdx.x;
+        const int32_t e = x < n ? S.v[x] : -1;
+        const bool pred[1] = {e >= 0};
+        const int32_t val[1] = {e};
+        dev::block_append_n<NT, 1>(pred, val, out, counter, lds);
+    }
+}
+
 // Appends the non-negative entries of every lane's v to out, one atomic on
 // *counter per wave (none when the wave has nothing).
 template <int N>
@@ -470,6 +573,16 @@ __device__ __forceinline__ void k_peel_body(int32_t nF, const int32_t *__restric
                                             int32_t thresh, int32_t *__restrict__ Fn, int32_t *__restrict__ nFn,
                                             const NearList nl, unsigned long long *__restrict__ ctr) {
     constexpr int CH = 64 * U;
+    // the block's staged crossings (next frontier) and near-list entries,
+    // and its count of destroyed triangles
+    __shared__ Stage xs, ns;
+    __shared__ unsigned long long hits;
+    __shared__ int32_t lds[256 / 64 + 1];
+    if (threadIdx.x == 0) {
+        xs.n = ns.n = 0;
+        hits = 0;
+    }
+    __syncthreads();
     const int lane = threadIdx.x & 63;
     const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
@@ -501,21 +614,49 @@ __device__ __forceinline__ void k_peel_body(int32_t nF, const int32_t *__restric
             f_hi = lo;
         }
         int32_t t[U], fe[U], yy[U];
-        int32_t l = f_lo;
+        if (f_hi - f_lo < kOwnerWin) {
+            // The chunk's owners are f_lo..f_hi: their prefix values, the
+            // same for the whole wave, are read at once and each item's
+            // owner (the last f with pre[f] <= g) is counted in registers.
+            int32_t win[kOwnerWin + 1];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int32_t g = (int32_t)g0 + 64 * u + lane;
-            t[u] = -1;
-            fe[u] = yy[u] = 0;
-            if (g < total) {
-                int32_t h = f_hi;  // largest f in [l, f_hi] with pre[f] <= g (items grow with u)
-                while (l < h) {
-                    const int32_t mid = l + ((h - l + 1) >> 1);
-                    if (pre[mid] <= g) l = mid; else h = mid - 1;
+            for (int k = 0; k <= kOwnerWin; ++k) win[k] = pre[min(f_lo + k, f_hi)];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int32_t g = (int32_t)g0 + 64 * u + lane;
+                t[u] = -1;
+                fe[u] = yy[u] = 0;
+                if (g < total) {
+                    int32_t k = 0, pk = win[0];
+#pragma unroll
+                    for (int q = 1; q <= kOwnerWin; ++q) {
+                        const bool in = win[q] <= g && q <= f_hi - f_lo;
+                        k = in ? q : k;
+                        pk = in ? win[q] : pk;
+                    }
+                    const int32_t l = f_lo + k;
+                    fe[u] = F[l];
+                    yy[u] = fy[l];
+                    t[u] = frs[l] + (g - pk);
                 }
-                fe[u] = F[l];
-                yy[u] = fy[l];
-                t[u] = frs[l] + (g - pre[l]);
+            }
+        } else {
+            int32_t l = f_lo;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int32_t g = (int32_t)g0 + 64 * u + lane;
+                t[u] = -1;
+                fe[u] = yy[u] = 0;
+                if (g < total) {
+                    int32_t h = f_hi;  // largest f in [l, f_hi] with pre[f] <= g (items grow with u)
+                    while (l < h) {
+                        const int32_t mid = l + ((h - l + 1) >> 1);
+                        if (pre[mid] <= g) l = mid; else h = mid - 1;
+                    }
+                    fe[u] = F[l];
+                    yy[u] = fy[l];
+                    t[u] = frs[l] + (g - pre[l]);
+                }
             }
         }
         int32_t w[U], b[U];
@@ -586,12 +727,17 @@ __device__ __forceinline__ void k_peel_body(int32_t nF, const int32_t *__restric
                 else if (old == nl.hi) xn[2 * u + 1] = b[u];
             }
         }
-        wave_append(xf, Fn, nFn);
-        wave_append(xn, nl.list, nl.n);
+        stage_append(xf, xs, Fn, nFn);
+        stage_append(xn, ns, nl.list, nl.n);
     }
     unsigned long long s = dev::wave_sum((unsigned long long)cnt);
-    if (lane == 0 && s) atomicAdd(&ctr[1], s);
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctr[0], (unsigned long long)total);
+    if (lane == 0 && s) atomicAdd(&hits, s);
+    stage_flush<256>(xs, Fn, nFn, lds);
+    stage_flush<256>(ns, nl.list, nl.n, lds);
+    if (threadIdx.x == 0) {
+        if (hits) atomicAdd(&ctr[1], hits);
+        if (blockIdx.x == 0) atomicAdd(&ctr[0], (unsigned long long)total);
+    }
 }
 
 // streamed items per wave step of the peeling kernels
@@ -626,14 +772,13 @@ __global__ __launch_bounds__(kSmallRoundThreads) void k_dround_prologue(
     const int32_t *F = c.cur ? buf1 : buf0;
     const int32_t *Fp = c.cur ? buf0 : buf1;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    for (int32_t f = tid; f < nP; f += kSmallRoundThreads) st[Fp[f]] = 2;
-    int32_t l[kSmallRoundPer], run = 0;
+    mark_removed<4>(nP, Fp, st, tid, kSmallRoundThreads);
+    int32_t fi[kSmallRoundPer], l[kSmallRoundPer], run = 0;
 #pragma unroll
-    for (int j = 0; j < kSmallRoundPer; ++j) {
-        const int32_t f = tid * kSmallRoundPer + j;
-        l[j] = f < nF ? prep_edge(f, F, st, truss, label, eu, ev, sup, rst, fy, fx) : 0;
-        run += l[j];
-    }
+    for (int j = 0; j < kSmallRoundPer; ++j) fi[j] = tid * kSmallRoundPer + j;
+    prep_edges<kSmallRoundPer>(fi, nF, F, st, truss, label, eu, ev, sup, rst, fy, fx, l);
+#pragma unroll
+    for (int j = 0; j < kSmallRoundPer; ++j) run += l[j];
     const int32_t incl = dev::wave_incl_scan(run);
     if (lane == 63) wsum[w] = incl;
     __syncthreads();
@@ -706,14 +851,21 @@ __global__ __launch_bounds__(256) void k_big_prologue(
     const int32_t *F = ctl->cur ? buf1 : buf0;
     const int32_t *Fp = ctl->cur ? buf0 : buf1;
     const int32_t stride = gridDim.x * blockDim.x;
-    for (int32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < nP; f += stride) st[Fp[f]] = 2;
+    mark_removed<4>(nP, Fp, st, blockIdx.x * blockDim.x + threadIdx.x, stride);
     const int32_t chunk = (nF + gridDim.x - 1) / gridDim.x;
     const int32_t f0 = min(nF, (int32_t)blockIdx.x * chunk), f1 = min(nF, f0 + chunk);
     int32_t sum = 0;
-    for (int32_t f = f0 + threadIdx.x; f < f1; f += blockDim.x) {
-        const int32_t l = prep_edge(f, F, st, truss, label, eu, ev, sup, rst, fy, fx);
-        len[f] = l;
-        sum += l;
+    for (int32_t b = f0 + threadIdx.x; b < f1; b += 4 * blockDim.x) {
+        int32_t fi[4], l[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) fi[j] = b + j * (int32_t)blockDim.x < f1 ? b + j * (int32_t)blockDim.x : nF;
+        prep_edges<4>(fi, nF, F, st, truss, label, eu, ev, sup, rst, fy, fx, l);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (fi[j] < nF) {
+                len[fi[j]] = l[j];
+                sum += l[j];
+            }
     }
     sum = dev::wave_sum(sum);
     if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = sum;
