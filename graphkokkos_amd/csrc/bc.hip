@@ -145,7 +145,7 @@ __device__ __forceinline__ double backward_finish(int64_t i, uint32_t nd, const 
 }
 
 // Level d -> d+1 for vertices with fewer than kHeavy in-neighbours, one wave
-// per vertex (heavier ones: k_bc_forward_heavy).  cnt accumulates the number
+// per vertex (heavier ones: k_bc_heavy_chunk).  cnt accumulates the number
 // of (vertex, source) pairs reached.
 __global__ __launch_bounds__(256) void k_bc_forward(int64_t n, const int32_t *__restrict__ trst,
                                                     const int32_t *__restrict__ tcol,
@@ -167,35 +167,133 @@ __global__ __launch_bounds__(256) void k_bc_forward(int64_t n, const int32_t *__
     if (lane == 0 && r) atomicAdd(cnt, r);
 }
 
-// The same for the heavy vertices, one workgroup of kHeavyWaves waves each:
-// wave w takes in-neighbours t0 + w, t0 + w + kHeavyWaves, ...; the partial
-// sums are added in wave order through LDS.
-__global__ __launch_bounds__(64 * kHeavyWaves) void k_bc_forward_heavy(
-    int32_t nh, const int32_t *__restrict__ heavy, const int32_t *__restrict__ trst, const int32_t *__restrict__ tcol,
-    const uint64_t *__restrict__ fd, uint64_t *__restrict__ fn, uint64_t *__restrict__ vis, double *__restrict__ S,
-    unsigned long long *__restrict__ cnt) {
-    __shared__ double part[kHeavyWaves][kSrc];
+// Heavy vertices in chunks: a vertex with kHeavy or more neighbours is cut
+// into chunks of kChunk neighbours, one workgroup of kHeavyWaves waves per
+// chunk, so a hub's sum is spread over many workgroups instead of being the
+// long serial tail of one.  chunks[c] = (vertex, first, end, several): a
+// vertex with one chunk is finished by that chunk's workgroup; the chunks
+// of a vertex with several write their 256 partial sums to part[c], and
+// k_bc_heavy_finish adds them in chunk order (deterministic sums).
+// multi[x] = (vertex, first chunk, chunk count) for those vertices.
+constexpr int kChunk = 2048;
+
+// One block: the chunks of every heavy vertex (heavy[0, nh)).
+__global__ __launch_bounds__(1024) void k_heavy_chunks(int32_t nh, const int32_t *__restrict__ heavy,
+                                                       const int32_t *__restrict__ rst, int4 *__restrict__ chunks,
+                                                       int4 *__restrict__ multi, int32_t *__restrict__ counts) {
+    __shared__ int32_t ws[2][1024 / 64];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int32_t per = (nh + 1023) / 1024;
+    const int32_t h0 = min(nh, tid * per), h1 = min(nh, h0 + per);
+    int32_t nc = 0, nm = 0;
+    for (int32_t h = h0; h < h1; ++h) {
+        const int32_t d = rst[heavy[h] + 1] - rst[heavy[h]];
+        const int32_t k = (d + kChunk - 1) / kChunk;
+        nc += k;
+        nm += k > 1;
+    }
+    const int32_t ic = dev::wave_incl_scan(nc), im = dev::wave_incl_scan(nm);
+    if (lane == 63) {
+        ws[0][w] = ic;
+        ws[1][w] = im;
+    }
+    __syncthreads();
+    int32_t oc = ic - nc, om = im - nm;
+    for (int x = 0; x < w; ++x) {
+        oc += ws[0][x];
+        om += ws[1][x];
+    }
+    for (int32_t h = h0; h < h1; ++h) {
+        const int32_t v = heavy[h], b = rst[v], e = rst[v + 1];
+        const int32_t k = (e - b + kChunk - 1) / kChunk;
+        if (k > 1) multi[om++] = make_int4(v, oc, k, 0);
+        for (int32_t j = 0; j < k; ++j) chunks[oc++] = make_int4(v, b + j * kChunk, min(e, b + (j + 1) * kChunk), k > 1);
+    }
+    if (tid == 1023) {
+        counts[0] = oc;
+        counts[1] = om;
+    }
+}
+
+// The sources of vertex i that a sweep works on: forward, those that have
+// not reached i; backward, those for which i is on level d.
+template <bool FWD>
+__device__ __forceinline__ uint32_t sweep_mask(int64_t i, const uint64_t *__restrict__ vis,
+                                               const uint64_t *__restrict__ fd, int lane) {
+    return FWD ? ~nib_of(vis + 4 * i, lane) & 0xFu : nib_of(fd + 4 * i, lane);
+}
+
+// Forward (FWD): sums sigma over in-neighbours on level d (fd) into the new
+// level fn; backward: sums W over out-neighbours on level d+1 (fn).
+template <bool FWD>
+__global__ __launch_bounds__(64 * kHeavyWaves) void k_bc_heavy_chunk(
+    int32_t nc, const int4 *__restrict__ chunks, const int32_t *__restrict__ col, const uint64_t *__restrict__ fd,
+    uint64_t *__restrict__ fn, uint64_t *__restrict__ vis, double *__restrict__ S, double *__restrict__ bc,
+    unsigned long long *__restrict__ cnt, double *__restrict__ part) {
+    __shared__ double psum[kHeavyWaves][kSrc];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    for (int32_t h = blockIdx.x; h < nh; h += gridDim.x) {
-        const int64_t i = heavy[h];
-        const uint32_t unv = ~nib_of(vis + 4 * i, lane) & 0xFu;
+    for (int32_t c = blockIdx.x; c < nc; c += gridDim.x) {
+        const int4 ch = chunks[c];
+        const int64_t i = ch.x;
+        const uint32_t mask = sweep_mask<FWD>(i, vis, fd, lane);
+        if (!__syncthreads_or(mask != 0)) continue;
         double a[4] = {0, 0, 0, 0};
-        if (__ballot(unv != 0)) pull_sum<8>(trst[i] + wv, trst[i + 1], kHeavyWaves, tcol, fd, unv, S, lane, a);
+        pull_sum<8>(ch.y + wv, ch.z, kHeavyWaves, col, FWD ? fd : fn, mask, S, lane, a);
 #pragma unroll
-        for (int b = 0; b < 4; ++b) part[wv][4 * lane + b] = a[b];
+        for (int b = 0; b < 4; ++b) psum[wv][4 * lane + b] = a[b];
         __syncthreads();
         if (wv == 0) {
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
                 double s = 0;
-                for (int x = 0; x < kHeavyWaves; ++x) s += part[x][4 * lane + b];
+                for (int x = 0; x < kHeavyWaves; ++x) s += psum[x][4 * lane + b];
                 a[b] = s;
             }
-            const uint32_t got = forward_finish(i, unv, a, fn, vis, S, lane);
-            const unsigned long long r = dev::wave_sum((unsigned long long)got);
-            if (lane == 0 && r) atomicAdd(cnt, r);
+            if (ch.w) {
+#pragma unroll
+                for (int b = 0; b < 4; ++b) part[(int64_t)c * kSrc + 4 * lane + b] = a[b];
+            } else if (FWD) {
+                const uint32_t got = forward_finish(i, mask, a, fn, vis, S, lane);
+                const unsigned long long r = dev::wave_sum((unsigned long long)got);
+                if (lane == 0 && r) atomicAdd(cnt, r);
+            } else {
+                const double dsum = backward_finish(i, mask, a, S, lane);
+                if (lane == 0) bc[i] += dsum;
+            }
         }
         __syncthreads();
+    }
+}
+
+// The vertices with several chunks: their chunks' partial sums in chunk
+// order, then the level's finish; one wave per vertex.
+template <bool FWD>
+__global__ __launch_bounds__(256) void k_bc_heavy_finish(int32_t nm, const int4 *__restrict__ multi,
+                                                         const double *__restrict__ part,
+                                                         const uint64_t *__restrict__ fd, uint64_t *__restrict__ fn,
+                                                         uint64_t *__restrict__ vis, double *__restrict__ S,
+                                                         double *__restrict__ bc, unsigned long long *__restrict__ cnt) {
+    const int lane = threadIdx.x & 63;
+    const int32_t waves = gridDim.x * (blockDim.x >> 6);
+    for (int32_t x = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); x < nm; x += waves) {
+        const int4 mv = multi[x];
+        const int64_t i = mv.x;
+        const uint32_t mask = sweep_mask<FWD>(i, vis, fd, lane);
+        if (!__ballot(mask != 0)) continue;
+        double a[4] = {0, 0, 0, 0};
+        for (int32_t j = 0; j < mv.z; ++j) {
+            const double *p = part + (int64_t)(mv.y + j) * kSrc + 4 * lane;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) a[b] += p[b];
+        }
+        if (FWD) {
+            const uint32_t got = forward_finish(i, mask, a, fn, vis, S, lane);
+            const unsigned long long r = dev::wave_sum((unsigned long long)got);
+            if (lane == 0 && r) atomicAdd(cnt, r);
+        } else {
+            const double dsum = backward_finish(i, mask, a, S, lane);
+            if (lane == 0) bc[i] += dsum;
+        }
     }
 }
 
@@ -231,35 +329,6 @@ __global__ __launch_bounds__(256) void k_bc_backward(int64_t n, const int32_t *_
         pull_sum<8>(t0, t1, 1, col, fn, nd, S, lane, a);
         const double dsum = backward_finish(i, nd, a, S, lane);
         if (lane == 0) bc[i] += dsum;
-    }
-}
-
-__global__ __launch_bounds__(64 * kHeavyWaves) void k_bc_backward_heavy(
-    int32_t nh, const int32_t *__restrict__ heavy, const int32_t *__restrict__ rst, const int32_t *__restrict__ col,
-    const uint64_t *__restrict__ fd, const uint64_t *__restrict__ fn, double *__restrict__ S,
-    double *__restrict__ bc) {
-    __shared__ double part[kHeavyWaves][kSrc];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    for (int32_t h = blockIdx.x; h < nh; h += gridDim.x) {
-        const int64_t i = heavy[h];
-        const uint32_t nd = nib_of(fd + 4 * i, lane);
-        if (!__syncthreads_or(nd != 0)) continue;
-        double a[4] = {0, 0, 0, 0};
-        pull_sum<8>(rst[i] + wv, rst[i + 1], kHeavyWaves, col, fn, nd, S, lane, a);
-#pragma unroll
-        for (int b = 0; b < 4; ++b) part[wv][4 * lane + b] = a[b];
-        __syncthreads();
-        if (wv == 0) {
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                double s = 0;
-                for (int x = 0; x < kHeavyWaves; ++x) s += part[x][4 * lane + b];
-                a[b] = s;
-            }
-            const double dsum = backward_finish(i, nd, a, S, lane);
-            if (lane == 0) bc[i] += dsum;
-        }
-        __syncthreads();
     }
 }
 
@@ -388,8 +457,6 @@ int sweeps(gk_ctx *ctx, const gk_dcsr *A, const int32_t *trst, const int32_t *tc
     if (dups)
         for (auto &t : *dups) dup_max = std::max(dup_max, t.size());
     if (dup_max) GK_TRY(P.alloc(&ddup, dup_max));
-    uint64_t *f0;
-    GK_TRY(P.level(0, &f0));
     GK_HIP(hipMemsetAsync(nh, 0, 8, ctx->stream));
     {
         const int g = (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, ctx->num_cus * 16));
@@ -398,11 +465,42 @@ int sweeps(gk_ctx *ctx, const gk_dcsr *A, const int32_t *trst, const int32_t *tc
         GK_TRY(launch_check("k_heavy_list"));
     }
     unsigned long long *h;
-    GK_TRY(pinned(ctx, 16, (void **)&h));
+    GK_TRY(pinned(ctx, 32, (void **)&h));
     GK_HIP(hipMemcpyAsync(h, nh, 8, hipMemcpyDeviceToHost, ctx->stream));
     GK_HIP(hipStreamSynchronize(ctx->stream));
     int32_t nheavy[2];
     std::memcpy(nheavy, h, 8);
+    // the heavy vertices' chunks: [0] in-neighbours (forward), [1]
+    // out-neighbours (backward); a vertex's chunks are at most its
+    // neighbours / kChunk + 1, and the heavy vertices' neighbours at most
+    // A's entries
+    int4 *chunks[2] = {nullptr, nullptr}, *multi[2] = {nullptr, nullptr};
+    double *part[2] = {nullptr, nullptr};
+    int32_t nchunk[2] = {0, 0}, nmulti[2] = {0, 0}, *dcounts;
+    GK_TRY(P.alloc(&dcounts, 4));
+    for (int d = 0; d < 2; ++d) {
+        if (!nheavy[d]) continue;
+        const size_t cap = (size_t)(A->nz / kChunk) + (size_t)nheavy[d] + 1;
+        GK_TRY(P.alloc(&chunks[d], cap));
+        GK_TRY(P.alloc(&multi[d], nheavy[d]));
+        GK_TRY(P.alloc(&part[d], cap * kSrc));
+        k_heavy_chunks<<<1, 1024, 0, ctx->stream>>>(nheavy[d], d ? hout : hin, d ? A->rst : trst, chunks[d],
+                                                     multi[d], dcounts + 2 * d);
+        GK_TRY(launch_check("k_heavy_chunks"));
+    }
+    if (nheavy[0] || nheavy[1]) {
+        GK_HIP(hipMemcpyAsync(h, dcounts, 16, hipMemcpyDeviceToHost, ctx->stream));
+        GK_HIP(hipStreamSynchronize(ctx->stream));
+        int32_t c4[4];
+        std::memcpy(c4, h, 16);
+        for (int d = 0; d < 2; ++d)
+            if (nheavy[d]) {
+                nchunk[d] = c4[2 * d];
+                nmulti[d] = c4[2 * d + 1];
+            }
+    }
+    uint64_t *f0;
+    GK_TRY(P.level(0, &f0));
     const int gw = grid_waves(ctx, n);
     const int gi = (int)std::max<int64_t>(1, std::min<int64_t>((4 * n + 255) / 256, ctx->num_cus * 16));
     hipStream_t hs = ctx->aux[0];
@@ -438,8 +536,11 @@ int sweeps(gk_ctx *ctx, const gk_dcsr *A, const int32_t *trst, const int32_t *tc
             GK_HIP(hipMemsetAsync(dcnt, 0, 8, ctx->stream));
             if (nheavy[0]) {
                 GK_TRY(fork());
-                k_bc_forward_heavy<<<std::min(nheavy[0], ctx->num_cus * 2), 64 * kHeavyWaves, 0, hs>>>(
-                    nheavy[0], hin, trst, tcol, fd, fn, vis, S, dcnt);
+                k_bc_heavy_chunk<true><<<std::min(nchunk[0], ctx->num_cus * 2), 64 * kHeavyWaves, 0, hs>>>(
+                    nchunk[0], chunks[0], tcol, fd, fn, vis, S, d_bc, dcnt, part[0]);
+                if (nmulti[0])
+                    k_bc_heavy_finish<true><<<std::max(1, std::min((nmulti[0] + 3) / 4, ctx->num_cus * 4)), 256, 0,
+                                               hs>>>(nmulti[0], multi[0], part[0], fd, fn, vis, S, d_bc, dcnt);
             }
             k_bc_forward<<<gw, 256, 0, ctx->stream>>>(n, trst, tcol, fd, fn, vis, S, dcnt);
             if (nheavy[0]) GK_TRY(join());
@@ -466,8 +567,12 @@ int sweeps(gk_ctx *ctx, const gk_dcsr *A, const int32_t *trst, const int32_t *tc
                 GK_TRY(P.level(l + 1, &fn));
                 if (nheavy[1]) {
                     GK_TRY(fork());
-                    k_bc_backward_heavy<<<std::min(nheavy[1], ctx->num_cus * 2), 64 * kHeavyWaves, 0, hs>>>(
-                        nheavy[1], hout, A->rst, A->col, fd, fn, S, d_bc);
+                    k_bc_heavy_chunk<false><<<std::min(nchunk[1], ctx->num_cus * 2), 64 * kHeavyWaves, 0, hs>>>(
+                        nchunk[1], chunks[1], A->col, fd, fn, nullptr, S, d_bc, nullptr, part[1]);
+                    if (nmulti[1])
+                        k_bc_heavy_finish<false><<<std::max(1, std::min((nmulti[1] + 3) / 4, ctx->num_cus * 4)), 256,
+                                                    0, hs>>>(nmulti[1], multi[1], part[1], fd, fn, nullptr, S, d_bc,
+                                                             nullptr);
                 }
                 k_bc_backward<<<gw, 256, 0, ctx->stream>>>(n, A->rst, A->col, fd, fn, S, d_bc);
                 if (nheavy[1]) GK_TRY(join());

@@ -63,13 +63,14 @@ struct ArenaPool final : bc::Pool {
 };
 
 // Bytes bc::sweeps takes before its levels on a sorted pattern with n
-// vertices: sigma / W (n x 256 FP64), visited bits (n x 4 words), the
-// batch's sources, the level counter, the two heavy-vertex lists and their
-// counts.
+// vertices whatever its degrees: sigma / W (n x 256 FP64), visited bits
+// (n x 4 words), the batch's sources, the level counter, the two
+// heavy-vertex lists and their counts, the chunk counts.  The chunks of
+// heavy vertices (kHeavy neighbours or more) come on top.
 size_t fixed_bytes(int64_t n) {
     const size_t m = (size_t)std::max<int64_t>(n, 1);
     return align256(m * bc::kSrc * 8) + align256(m * 32) + align256(4 * bc::kSrc) + align256(8) +
-           2 * align256(m * 4) + align256(8);
+           2 * align256(m * 4) + align256(8) + align256(16);
 }
 
 constexpr int kMaxDevices = 64;
