@@ -556,9 +556,14 @@ def main() -> int:
         r = run_bc(ctx, G20.A, sources, 1)
         byts = bc_bytes(G20.n, G20.A.nz, r["levels"] // max(a.bc_batches, 1), 256) * a.bc_batches
         gbs = byts / 1e9 / r["wall_s"]
+        traffic = measured_traffic("bc", f"bc-rmat{a.ktruss_scale}") if a.bc_batches == 4 else None
         bc = {"workload": f"batched Brandes BC, R-MAT scale {a.ktruss_scale}, {a.bc_batches} x 256 sources, FP64",
               "value": nsrc * E20 / r["wall_s"], "unit": "source-edges/s", "ms": r["wall_s"] * 1e3,
               "levels_total": r["levels"], "bytes_alg": byts, "achieved_gbs": gbs, "frac": gbs / HBM_PEAK_GBS,
+              # the formula counts 24 B for every (edge, source) pair; the sweeps touch only the pairs
+              # whose frontier bits match, so measured HBM bytes (rocprofv3) are reported beside it
+              "traffic": traffic,
+              "measured_gbs": traffic / 1e9 / r["wall_s"] if traffic else None,
               "forward_ms": r["timing"]["select_ms"], "backward_ms": r["timing"]["kernel_ms"] - r["timing"]["select_ms"]}
         log(f"bc: {r['wall_s'] * 1e3:.1f} ms for {nsrc} sources, {bc['value']:.3g} source-edges/s")
 
