@@ -33,9 +33,11 @@ constexpr int kWindowHashLong = 2048;
 constexpr int kWindowWavesLong = 4;
 constexpr int kWindowBlocksLong = 8;  // per CU
 // streamed entries per lane in flight in the window kernels
-constexpr int kWindowU = 16;
+constexpr int kWindowU = 8;
 // ... in the MCA-wave kernel
 constexpr int kMcaU = 8;
+// rounds (of 64 / 16 tasks) an MCA-wave wave takes per ticket
+constexpr int kMcaRounds = 8;
 constexpr int kNumClass = GK_KERNEL_MSA_LONG + 1;
 constexpr int kClassMsaLarge = GK_NUM_ACC;
 constexpr int kFlopBins = 32;
@@ -88,6 +90,7 @@ struct Plan {
     int32_t *bin_off = nullptr;             // device, kNumBins + 1
     unsigned long long *stats = nullptr;    // device, kNumClass * ST_NUM
     unsigned long long *hits = nullptr;     // device, kNumClass
+    int32_t *ticket = nullptr;              // device, kNumClass
 };
 
 // Per-accumulator work counters gathered while binning (device, u64):
@@ -106,6 +109,9 @@ struct Prob {
     long long *__restrict__ task_cycles;        // optional: clock64 duration of every task
     const Task *__restrict__ tasks;
     const int32_t *__restrict__ bin_off;
+    // per kernel class: tasks taken so far (the window and MCA-wave kernels
+    // take tasks by atomic ticket; zeroed before every launch of the classes)
+    int32_t *ticket;
     // Triangle counting with hub bitmaps: row k < hub_n of B as hub_w
     // words at hub + k * hub_w (bit c set iff column c is in the row).
     const uint32_t *__restrict__ hub;

@@ -289,56 +289,6 @@ __device__ __forceinline__ int32_t build_prefix(const Prob &P, int32_t a0, int32
     return total;
 }
 
-// The same with hub bitmaps: each A entry's B row is streamed, read as
-// bitmap words or bit-tested (hub_choice; mb = the mask row's first entry).
-// pre/bst as build_prefix for the streamed rows (0 for the others);
-// hpre[x] = bitmap words before A entry a0+x, hbst[x] = its row's word
-// offset in P.hub minus hpre[x]; ppre[x] = bit tests before it, pofs[x] =
-// its row's word offset.  Returns (streamed entries, words, tests).
-template <int G>
-__device__ __forceinline__ int3 build_prefix_hub(const Prob &P, int32_t a0, int32_t a1, int32_t mb, int32_t *pre,
-                                                 int32_t *bst, int32_t *hpre, int32_t *hbst, int32_t *ppre,
-                                                 int32_t *pofs, int gl) {
-    int32_t total = 0, htotal = 0, ptotal = 0;
-    for (int32_t base = a0; base < a1; base += G) {
-        const int32_t p = base + gl;
-        int32_t len = 0, bs = 0, hlen = 0, hbs = 0, plen = 0, pbs = 0;
-        if (p < a1) {
-            const int32_t k = P.acol[p];
-            bs = P.brst[k];
-            len = P.brst[k + 1] - bs;
-            hub_choice(P, k, p - mb, len, hlen, hbs, plen, pbs);
-        }
-        int32_t incl = len, hincl = hlen, pincl = plen;
-#pragma unroll
-        for (int o = 1; o < G; o <<= 1) {
-            const int32_t w = __shfl_up(incl, o, G), hw = __shfl_up(hincl, o, G), pw = __shfl_up(pincl, o, G);
-            if (gl >= o) {
-                incl += w;
-                hincl += hw;
-                pincl += pw;
-            }
-        }
-        if (p < a1) {
-            pre[p - a0] = total + incl - len;
-            bst[p - a0] = bs - (total + incl - len);
-            hpre[p - a0] = htotal + hincl - hlen;
-            hbst[p - a0] = hbs - (htotal + hincl - hlen);
-            ppre[p - a0] = ptotal + pincl - plen;
-            pofs[p - a0] = pbs;
-        }
-        total += __shfl(incl, G - 1, G);
-        htotal += __shfl(hincl, G - 1, G);
-        ptotal += __shfl(pincl, G - 1, G);
-    }
-    if (gl == 0) {
-        pre[a1 - a0] = total;
-        hpre[a1 - a0] = htotal;
-        ppre[a1 - a0] = ptotal;
-    }
-    return make_int3(total, htotal, ptotal);
-}
-
 // Hit handling common to every accumulator.  q = mask position (0-based in
 // the row), kk = A entry offset in the task, t = index of the streamed entry
 // in B.col.
@@ -507,7 +457,7 @@ __device__ __forceinline__ uint32_t stream_entries(const Prob &P, const Task &tk
 }
 
 // Triangle counting over the hub rows of a task: the concatenated bitmap
-// words of its hub B rows (hpre / hbst from build_prefix_hub), U words per
+// words of its hub B rows (hpre / hbst from WinPrefix), U words per
 // lane in flight; word w of row k holds columns [32w, 32w + 32), all below
 // k, and is ANDed with word w of the mask row's LDS bitmap.  Consecutive
 // lanes read consecutive words of a row.  Returns the number of hits.
@@ -665,100 +615,159 @@ __global__ __launch_bounds__(64) void k_msa_lds(Prob P, int cls) {
 // window) and the few columns at or above kWindow in a small LDS hash, so
 // almost every probe is one LDS word read and a bit test.  Only the bitmap
 // words a row sets are cleared afterwards.
-// Block-wide B-row prefix for tasks of up to kMaxTaskA A entries, blocks of
-// NT threads (NT >= kMaxTaskA); `wsum` holds NT/64 wave totals.
-template <int NT>
-__device__ __forceinline__ int32_t build_prefix_block(const Prob &P, int32_t a0, int32_t a1, int32_t *pre,
-                                                      int32_t *bst, int32_t *wsum) {
-    static_assert(NT >= kMaxTaskA, "one A entry per thread");
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int32_t p = a0 + tid;
-    int32_t len = 0, bs = 0;
-    if (p < a1) {
-        const int32_t k = P.acol[p];
-        bs = P.brst[k];
-        len = P.brst[k + 1] - bs;
-    }
-    const int32_t incl = dev::wave_incl_scan(len);
-    if (lane == 63) wsum[w] = incl;
-    __syncthreads();
-    int32_t before = 0, total = 0;
-#pragma unroll
-    for (int x = 0; x < NT / 64; ++x) {
-        const int32_t v = wsum[x];
-        before += x < w ? v : 0;
-        total += v;
-    }
-    if (p < a1) {
-        const int32_t ex = before + incl - len;
-        pre[p - a0] = ex;
-        bst[p - a0] = bs - ex;
-    }
-    if (tid == 0) pre[a1 - a0] = total;
-    return total;
-}
 
-// build_prefix_block with hub bitmaps (see build_prefix_hub); wsum holds
-// 3 * NT/64 wave totals.
-template <int NT>
-__device__ __forceinline__ int3 build_prefix_block_hub(const Prob &P, int32_t a0, int32_t a1, int32_t mb,
-                                                       int32_t *pre, int32_t *bst, int32_t *hpre, int32_t *hbst,
-                                                       int32_t *ppre, int32_t *pofs, int32_t *wsum) {
-    static_assert(NT >= kMaxTaskA, "one A entry per thread");
-    constexpr int NW = NT / 64;
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int32_t p = a0 + tid;
-    int32_t len = 0, bs = 0, hlen = 0, hbs = 0, plen = 0, pbs = 0;
-    if (p < a1) {
-        const int32_t k = P.acol[p];
-        bs = P.brst[k];
-        len = P.brst[k + 1] - bs;
-        hub_choice(P, k, p - mb, len, hlen, hbs, plen, pbs);
-    }
-    const int32_t incl = dev::wave_incl_scan(len), hincl = dev::wave_incl_scan(hlen);
-    const int32_t pincl = dev::wave_incl_scan(plen);
-    if (lane == 63) {
-        wsum[w] = incl;
-        wsum[NW + w] = hincl;
-        wsum[2 * NW + w] = pincl;
-    }
-    __syncthreads();
-    int32_t before = 0, total = 0, hbefore = 0, htotal = 0, pbefore = 0, ptotal = 0;
+// The B-row prefix of one window task (up to kMaxTaskA A entries), built in
+// two steps so that every global load of a task's set-up is issued before
+// the first barrier: load() reads the A entries' columns and B row pointers
+// (and picks each hub row's way, hub_choice); a one-wave block then writes
+// the whole prefix (scan_wave), a larger block (one A entry per thread)
+// scans per wave before the barrier (scan_block1) and adds the wave totals
+// after it (scan_block2).  pre/bst as build_prefix; hpre[x] = bitmap words
+// before A entry a0+x, hbst[x] = its row's word offset in P.hub minus
+// hpre[x]; ppre[x] = bit tests before it, pofs[x] = its row's word offset.
+template <int NT, bool HUB>
+struct WinPrefix {
+    static constexpr int R = (kMaxTaskA + NT - 1) / NT;
+    static constexpr int NW = NT / 64;
+    static_assert(NT == 64 || R == 1, "one-wave blocks or one A entry per thread");
+    int32_t len[R], bs[R], hlen[R], hbs[R], plen[R], pbs[R];
+    int32_t incl = 0, hincl = 0, pincl = 0;  // scan_block1: the thread's wave-inclusive sums
+    int32_t total = 0, htotal = 0, ptotal = 0;
+
+    __device__ __forceinline__ void load(const Prob &P, int32_t a0, int32_t a1, int32_t mb) {
+        const int tid = threadIdx.x;
+        int32_t k[R];
 #pragma unroll
-    for (int x = 0; x < NW; ++x) {
-        const int32_t v = wsum[x], hv = wsum[NW + x], pv = wsum[2 * NW + x];
-        before += x < w ? v : 0;
-        total += v;
-        hbefore += x < w ? hv : 0;
-        htotal += hv;
-        pbefore += x < w ? pv : 0;
-        ptotal += pv;
+        for (int r = 0; r < R; ++r) {
+            const int32_t p = a0 + r * NT + tid;
+            k[r] = p < a1 ? P.acol[p] : -1;
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            bs[r] = k[r] >= 0 ? P.brst[k[r]] : 0;
+            len[r] = k[r] >= 0 ? P.brst[k[r] + 1] : 0;
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            len[r] -= bs[r];
+            hlen[r] = hbs[r] = plen[r] = pbs[r] = 0;
+            if (HUB && k[r] >= 0) hub_choice(P, k[r], a0 + r * NT + tid - mb, len[r], hlen[r], hbs[r], plen[r], pbs[r]);
+        }
     }
-    if (p < a1) {
-        const int32_t ex = before + incl - len, hex = hbefore + hincl - hlen;
-        pre[p - a0] = ex;
-        bst[p - a0] = bs - ex;
-        hpre[p - a0] = hex;
-        hbst[p - a0] = hbs - hex;
-        ppre[p - a0] = pbefore + pincl - plen;
-        pofs[p - a0] = pbs;
+
+    __device__ __forceinline__ void scan_wave(int32_t a0, int32_t a1, int32_t *pre, int32_t *bst, int32_t *hpre,
+                                              int32_t *hbst, int32_t *ppre, int32_t *pofs) {
+        static_assert(NT == 64, "one-wave blocks");
+        const int lane = threadIdx.x;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (a0 + r * 64 >= a1) break;
+            const int32_t p = a0 + r * 64 + lane;
+            const int32_t i1 = dev::wave_incl_scan(len[r]);
+            const int32_t i2 = HUB ? dev::wave_incl_scan(hlen[r]) : 0;
+            const int32_t i3 = HUB ? dev::wave_incl_scan(plen[r]) : 0;
+            if (p < a1) {
+                const int32_t ex = total + i1 - len[r];
+                pre[p - a0] = ex;
+                bst[p - a0] = bs[r] - ex;
+                if (HUB) {
+                    const int32_t hex = htotal + i2 - hlen[r];
+                    hpre[p - a0] = hex;
+                    hbst[p - a0] = hbs[r] - hex;
+                    ppre[p - a0] = ptotal + i3 - plen[r];
+                    pofs[p - a0] = pbs[r];
+                }
+            }
+            total += __shfl(i1, 63);
+            if (HUB) {
+                htotal += __shfl(i2, 63);
+                ptotal += __shfl(i3, 63);
+            }
+        }
+        if (lane == 0) {
+            pre[a1 - a0] = total;
+            if (HUB) {
+                hpre[a1 - a0] = htotal;
+                ppre[a1 - a0] = ptotal;
+            }
+        }
     }
-    if (tid == 0) {
-        pre[a1 - a0] = total;
-        hpre[a1 - a0] = htotal;
-        ppre[a1 - a0] = ptotal;
+
+    // wsum: 3 * NW ints
+    __device__ __forceinline__ void scan_block1(int32_t *wsum) {
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        incl = dev::wave_incl_scan(len[0]);
+        if (HUB) {
+            hincl = dev::wave_incl_scan(hlen[0]);
+            pincl = dev::wave_incl_scan(plen[0]);
+        }
+        if (lane == 63) {
+            wsum[w] = incl;
+            if (HUB) {
+                wsum[NW + w] = hincl;
+                wsum[2 * NW + w] = pincl;
+            }
+        }
     }
-    return make_int3(total, htotal, ptotal);
-}
+
+    __device__ __forceinline__ void scan_block2(int32_t a0, int32_t a1, int32_t *pre, int32_t *bst, int32_t *hpre,
+                                                int32_t *hbst, int32_t *ppre, int32_t *pofs, const int32_t *wsum) {
+        const int tid = threadIdx.x, w = tid >> 6;
+        int32_t before = 0, hbefore = 0, pbefore = 0;
+#pragma unroll
+        for (int x = 0; x < NW; ++x) {
+            const int32_t v = wsum[x];
+            before += x < w ? v : 0;
+            total += v;
+            if (HUB) {
+                const int32_t hv = wsum[NW + x], pv = wsum[2 * NW + x];
+                hbefore += x < w ? hv : 0;
+                htotal += hv;
+                pbefore += x < w ? pv : 0;
+                ptotal += pv;
+            }
+        }
+        const int32_t p = a0 + tid;
+        if (p < a1) {
+            const int32_t ex = before + incl - len[0];
+            pre[p - a0] = ex;
+            bst[p - a0] = bs[0] - ex;
+            if (HUB) {
+                const int32_t hex = hbefore + hincl - hlen[0];
+                hpre[p - a0] = hex;
+                hbst[p - a0] = hbs[0] - hex;
+                ppre[p - a0] = pbefore + pincl - plen[0];
+                pofs[p - a0] = pbs[0];
+            }
+        }
+        if (tid == 0) {
+            pre[a1 - a0] = total;
+            if (HUB) {
+                hpre[a1 - a0] = htotal;
+                ppre[a1 - a0] = ptotal;
+            }
+        }
+    }
+};
 
 // TC and SUPPORT modes.  SUPPORT also needs the mask position of a hit:
 // the sorted mask row is copied to LDS and a hit below kWindow finds its
 // position by binary search there (hits are a few percent of the probes);
 // the hash for the columns above the window stores positions directly.
+// A task's set-up is one round of global loads: the mask row (held in
+// registers until the task ends, when its bitmap words are cleared from
+// them), the A columns and B row pointers (WinPrefix), all issued before
+// the first barrier.  Blocks take tasks by ticket (P.ticket), the next one
+// while the current task streams, so a class whose blocks outnumber the
+// slots free beside the other classes spreads onto the CUs they release.
+// The hash starts empty for every
+// task (it is cleared after a task that used it), with a capacity fixed by
+// the mask row's length, so columns go into it as they are read.
 template <int MODE, int NW, int HC, int MAXM, bool HUB>
 __global__ __launch_bounds__(64 * NW) void k_msa_window(Prob P, int cls) {
     static_assert(MODE == MODE_TC || MODE == MODE_SUPPORT, "window kernel modes");
     constexpr int NT = 64 * NW;
+    constexpr int CPT = (MAXM + NT - 1) / NT;  // mask columns per thread
     constexpr bool SUP = MODE == MODE_SUPPORT;
     __shared__ uint32_t bm[kWindow / 32];
     __shared__ int32_t hkey[HC];
@@ -775,65 +784,74 @@ __global__ __launch_bounds__(64 * NW) void k_msa_window(Prob P, int cls) {
     __shared__ int32_t ppre[HUB ? kMaxTaskA + 1 : 1];
     __shared__ int32_t pofs[HUB ? kMaxTaskA : 1];
     __shared__ int32_t wsum[3 * NW];
+    __shared__ int32_t nhigh_s;  // the task's mask columns >= kWindow
+    __shared__ int32_t next_s;   // the block's next task
     const int tid = threadIdx.x;
-    for (int w = tid; w < kWindow / 32; w += NT) bm[w] = 0;
     const int32_t lo = P.bin_off[cls * kFlopBins], hi = P.bin_off[(cls + 1) * kFlopBins];
+    for (int w = tid; w < kWindow / 32; w += NT) bm[w] = 0;
+    for (int s = tid; s < HC; s += NT) hkey[s] = -1;
+    if (tid == 0) {
+        nhigh_s = 0;
+        next_s = lo + atomicAdd(&P.ticket[cls], 1);
+    }
+    __syncthreads();
     uint32_t cnt = 0;
-    for (int32_t ti = lo + blockIdx.x; ti < hi; ti += gridDim.x) {
-        const Task tk = P.tasks[ti];
+    for (int32_t ti = __builtin_amdgcn_readfirstlane(next_s); ti < hi;
+         ti = __builtin_amdgcn_readfirstlane(next_s)) {
         const long long tstart = P.task_cycles ? clock64() : 0;
+        const Task tk = P.tasks[ti];
         const int32_t mb = P.mrst[tk.row], mlen = P.mrst[tk.row + 1] - mb;
-        // mask columns are sorted: the ones >= kWindow form the row's tail
-        int32_t l = 0, h = mlen;
-        while (l < h) {
-            int32_t mid = l + ((h - l) >> 1);
-            if (P.mcol[mb + mid] < kWindow) l = mid + 1; else h = mid;
-        }
-        const int32_t nlow = l, nhigh = mlen - l;
         int lg = 6;
-        while ((1 << lg) < 4 * nhigh && (1 << lg) < HC) ++lg;
+        while ((1 << lg) < 4 * mlen && (1 << lg) < HC) ++lg;
         const uint32_t cmask = (1u << lg) - 1;
-        if (nhigh)
-            for (uint32_t s = tid; s <= cmask; s += NT) hkey[s] = -1;
+        int32_t mc[CPT];
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) {
+            const int32_t q = tid + j * NT;
+            mc[j] = q < mlen ? P.mcol[mb + q] : -1;
+        }
+        WinPrefix<NT, HUB> pf;
+        pf.load(P, tk.a0, tk.a1, mb);
         if (SUP) {
             for (int32_t q = tid; q < mlen; q += NT) acc[q] = 0;
             for (int32_t x = tid; x < tk.a1 - tk.a0; x += NT) acnt[x] = 0;
         }
+        // mask columns are sorted: the ones below kWindow are positions
+        // [0, nlow) and go to the bitmap, the rest to the hash
+        int32_t nh = 0;
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) {
+            const int32_t q = tid + j * NT, c = mc[j];
+            if (c < 0) continue;
+            if (c < kWindow) {
+                atomicOr(&bm[c >> 5], 1u << (c & 31));
+                if (SUP) mk[q] = c;
+            } else {
+                uint32_t s = dev::hash32((uint32_t)c) >> (32 - lg);
+                while (atomicCAS(&hkey[s], -1, c) != -1) s = (s + 1) & cmask;
+                if (SUP) hpos[s] = q;
+                ++nh;
+            }
+        }
+        if (nh) atomicAdd(&nhigh_s, nh);
+        if constexpr (NW == 1) pf.scan_wave(tk.a0, tk.a1, pre, bst, hpre, hbst, ppre, pofs);
+        else pf.scan_block1(wsum);
         __syncthreads();
-        for (int32_t q = tid; q < nlow; q += NT) {
-            const int32_t c = P.mcol[mb + q];
-            atomicOr(&bm[c >> 5], 1u << (c & 31));
-            if (SUP) mk[q] = c;
+        const int32_t nhigh = nhigh_s, nlow = mlen - nhigh;
+        // the next task's ticket, taken while this one streams
+        int32_t tick = 0;
+        if (tid == 0) tick = atomicAdd(&P.ticket[cls], 1);
+        if constexpr (NW > 1) {
+            pf.scan_block2(tk.a0, tk.a1, pre, bst, hpre, hbst, ppre, pofs, wsum);
+            __syncthreads();
         }
-        for (int32_t q = nlow + tid; q < mlen; q += NT) {
-            const int32_t c = P.mcol[mb + q];
-            uint32_t s = dev::hash32((uint32_t)c) >> (32 - lg);
-            while (atomicCAS(&hkey[s], -1, c) != -1) s = (s + 1) & cmask;
-            if (SUP) hpos[s] = q;
-        }
-        int32_t total, htotal = 0, ptotal = 0;
-        if constexpr (HUB) {
-            int3 tt;
-            if constexpr (NW == 1)
-                tt = build_prefix_hub<64>(P, tk.a0, tk.a1, mb, pre, bst, hpre, hbst, ppre, pofs, tid);
-            else
-                tt = build_prefix_block_hub<NT>(P, tk.a0, tk.a1, mb, pre, bst, hpre, hbst, ppre, pofs, wsum);
-            total = tt.x;
-            htotal = tt.y;
-            ptotal = tt.z;
-        } else if constexpr (NW == 1) {
-            total = build_prefix<64>(P, tk.a0, tk.a1, pre, bst, tid);
-        } else {
-            total = build_prefix_block<NT>(P, tk.a0, tk.a1, pre, bst, wsum);
-        }
-        __syncthreads();
         uint32_t c = 0;
         if constexpr (HUB) {
-            if (htotal) c = stream_hub<NT, kWindowU>(P, hpre, hbst, tk.a1 - tk.a0, htotal, tid, bm);
-            if (ptotal) c += stream_probe<NT, kWindowU>(P, ppre, pofs, tk.a1 - tk.a0, ptotal, tid, mb);
+            if (pf.htotal) c = stream_hub<NT, kWindowU>(P, hpre, hbst, tk.a1 - tk.a0, pf.htotal, tid, bm);
+            if (pf.ptotal) c += stream_probe<NT, kWindowU>(P, ppre, pofs, tk.a1 - tk.a0, pf.ptotal, tid, mb);
         }
         // the window kernels without hub bitmaps stream long B rows: clamped
-        c += stream_entries<MODE, NT, kWindowU, !HUB>(P, tk, mb, pre, bst, total, tid, acc, acnt,
+        c += stream_entries<MODE, NT, kWindowU, !HUB>(P, tk, mb, pre, bst, pf.total, tid, acc, acnt,
                                                        [&](int32_t b) -> int32_t {
             if (b < kWindow) {
                 if (!((bm[b >> 5] >> (b & 31)) & 1u)) return -1;
@@ -854,8 +872,14 @@ __global__ __launch_bounds__(64 * NW) void k_msa_window(Prob P, int cls) {
                 s = (s + 1) & cmask;
             }
         });
+        if (tid == 0) next_s = lo + tick;
         __syncthreads();
-        for (int32_t q = tid; q < nlow; q += NT) bm[P.mcol[mb + q] >> 5] = 0;
+#pragma unroll
+        for (int j = 0; j < CPT; ++j)
+            if (mc[j] >= 0 && mc[j] < kWindow) bm[mc[j] >> 5] = 0;
+        if (nhigh)
+            for (uint32_t s = tid; s <= cmask; s += NT) hkey[s] = -1;
+        if (tid == 0) nhigh_s = 0;
         if (SUP) flush<MODE, NT>(P, tk, mb, mlen, acc, acnt, tid);
         if (MODE == MODE_TC && P.rowcount) {
             uint32_t r = dev::wave_sum(c);
@@ -945,12 +969,17 @@ __global__ __launch_bounds__(64) void k_mca_wave(Prob P, int cls) {
     const int lane = threadIdx.x, tile = lane / G, gl = lane % G;
     const int32_t lo = P.bin_off[cls * kFlopBins], hi = P.bin_off[(cls + 1) * kFlopBins];
     uint32_t cnt = 0;
-    // All tiles of a wave run the same number of rounds so that the wave
-    // stays converged at the barriers.
-    const int32_t ntask = hi - lo;
-    const int32_t rounds = (ntask + (int32_t)gridDim.x * T - 1) / ((int32_t)gridDim.x * T);
-    for (int32_t r = 0; r < rounds; ++r) {
-        const int32_t ti = lo + (r * (int32_t)gridDim.x + (int32_t)blockIdx.x) * T + tile;
+    // The wave takes kMcaRounds rounds of T tasks per ticket (P.ticket):
+    // tasks here are small and many, and one ticket per round would
+    // serialise on the shared counter.  The next ticket is taken during a
+    // chunk's last round.  All tiles run the same rounds, so the wave stays
+    // converged at the barriers.
+    constexpr int RB = kMcaRounds;
+    int32_t first = 0, tick = 0, r = 0;
+    if (lane == 0) first = atomicAdd(&P.ticket[cls], T * RB);
+    first = lo + __builtin_amdgcn_readfirstlane(first);
+    while (first + r * T < hi) {
+        const int32_t ti = first + r * T + tile;
         const bool live = ti < hi;
         Task tk{0, 0, 0, 0};
         int32_t mb = 0, mlen = 0;
@@ -979,6 +1008,7 @@ __global__ __launch_bounds__(64) void k_mca_wave(Prob P, int cls) {
             total = build_prefix<G>(P, tk.a0, tk.a1, pre[tile], bst[tile], gl);
         }
         __syncthreads();
+        if (r == RB - 1 && lane == 0) tick = atomicAdd(&P.ticket[cls], T * RB);
         uint32_t cnt_task = 0;
         if (live && mlen > 0 && total > 0) {
             const int32_t *m = mk[tile];
@@ -1069,6 +1099,10 @@ __global__ __launch_bounds__(64) void k_mca_wave(Prob P, int cls) {
             if (P.task_cycles && gl == 0) P.task_cycles[ti] = clock64() - tstart;
         }
         __syncthreads();
+        if (++r == RB) {
+            r = 0;
+            first = lo + __builtin_amdgcn_readfirstlane(tick);
+        }
     }
     finish_kernel<MODE>(P, cnt, cls, lane);
 }
@@ -1277,9 +1311,10 @@ int plan_tasks(gk_ctx *ctx, const gk_dcsr *A, RowStats &st, const SelectParams &
     const int64_t m = A->m;
     void *p;
     // Layout: u64 stats[GK_NUM_ACC * ST_NUM], u64 hits[GK_NUM_ACC], then the
-    // int32 bin counters, offsets, cursors and per-accumulator max mask.
+    // int32 bin counters, offsets, cursors, per-accumulator max mask and the
+    // class kernels' task tickets.
     const size_t u64s = kNumClass * ST_NUM + kNumClass;
-    const size_t words = 3 * kNumBins + 1 + kNumClass;
+    const size_t words = 3 * kNumBins + 1 + 2 * kNumClass;
     const size_t bytes = 8 * u64s + sizeof(int32_t) * words;
     GK_TRY(scratch(ctx, SC_BINS, bytes, &p));
     plan->stats = (unsigned long long *)p;
@@ -1287,6 +1322,7 @@ int plan_tasks(gk_ctx *ctx, const gk_dcsr *A, RowStats &st, const SelectParams &
     int32_t *bin_count = (int32_t *)(plan->hits + kNumClass), *bin_off = bin_count + kNumBins;
     int32_t *bin_cur = bin_off + kNumBins + 1;
     int32_t *acc_maxmask = bin_cur + kNumBins;
+    plan->ticket = acc_maxmask + kNumClass;
     GK_HIP(hipMemsetAsync(p, 0, bytes, ctx->stream));
     int blocks = grid_for(ctx, (m + 255) / 256, 8);
     if (m > 0) {
@@ -1400,6 +1436,10 @@ int launch_classes(gk_ctx *ctx, const Prob &P, const ClassLaunch &L, Timer *tm) 
     sc[GK_ACC_MSA_LDS] = ctx->aux[1];
     sc[GK_ACC_MCA_WAVE] = ctx->aux[2];
     sc[GK_ACC_HASH_GLOBAL] = ctx->aux[3];
+    // the class kernels' task tickets start at 0 (a fill kernel, not a
+    // memset node, so that a captured graph replays it correctly)
+    k_zero_u32<<<1, 64, 0, ctx->stream>>>((uint32_t *)P.ticket, kNumClass);
+    GK_TRY(launch_check("k_zero_u32"));
     GK_HIP(hipEventRecord(ctx->fork, ctx->stream));
     for (int i = 0; i < gk_ctx::kAux; ++i) GK_HIP(hipStreamWaitEvent(ctx->aux[i], ctx->fork, 0));
     if (L.grid[kClassMsaLarge]) {
@@ -1520,7 +1560,7 @@ int masked_product(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcs
     P.mrst = M->rst; P.mcol = M->col;
     P.cval = d_out; P.count = d_count; P.rowcount = d_rowcount; P.hits = plan.hits;
     P.task_cycles = task_cycles;
-    P.tasks = plan.tasks; P.bin_off = plan.bin_off;
+    P.tasks = plan.tasks; P.bin_off = plan.bin_off; P.ticket = plan.ticket;
     double acc_ms[kNumClass] = {};
     if (mode == MODE_TC) GK_TRY(run_mode<MODE_TC>(ctx, P, plan, B->n, o, acc_ms));
     else if (mode == MODE_VALUES) GK_TRY(run_mode<MODE_VALUES>(ctx, P, plan, B->n, o, acc_ms));
@@ -1577,7 +1617,7 @@ int prepare_product(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dc
     P->brst = B->rst; P->bcol = B->col; P->bval = B->val;
     P->mrst = M->rst; P->mcol = M->col;
     P->cval = d_out; P->hits = plan.hits;
-    P->tasks = plan.tasks; P->bin_off = plan.bin_off;
+    P->tasks = plan.tasks; P->bin_off = plan.bin_off; P->ticket = plan.ticket;
     return setup_classes(ctx, plan, B->n, o, L);
 }
 

@@ -124,9 +124,9 @@ def train_from_graphs(ctx: Context, graphs, epochs: int = 40, seed: int = 1, mod
 SEARCH_SPACE = {
     "mca_max_mask": [0, 8, 16, 32, 48, 64],
     "small_max_mask": [32, 64, 96, 128],
-    "blocks_small": [12, 16, 24, 32],
-    "blocks_long": [4, 6, 8],
-    "blocks_mca": [4, 8, 16],
+    "blocks_small": [12, 16, 24, 32, 48],
+    "blocks_long": [4, 8, 16, 24],
+    "blocks_mca": [4, 8, 16, 32],
     "split_flop": [1 << 16, 1 << 18, 1 << 20],
     "hub_window": [0, 4096, 8192, 16384, 32768],
     "hub_probe": [0, 2, 4, 8],
