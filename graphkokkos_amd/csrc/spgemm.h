@@ -38,6 +38,10 @@ constexpr int kWindowU = 8;
 constexpr int kMcaU = 8;
 // rounds (of 64 / 16 tasks) an MCA-wave wave takes per ticket
 constexpr int kMcaRounds = 8;
+// triangle counting: MCA-wave's per-task bit filter of the mask row
+// (2^kMcaFilterLg bits; a mask row of 64 columns sets at most 1/32 of them)
+constexpr int kMcaFilterLg = 11;
+constexpr int kMcaFilterWords = (1 << kMcaFilterLg) / 32;
 constexpr int kNumClass = GK_KERNEL_MSA_LONG + 1;
 constexpr int kClassMsaLarge = GK_NUM_ACC;
 constexpr int kFlopBins = 32;

@@ -949,11 +949,17 @@ __device__ __forceinline__ int2 build_prefix_probe(const Prob &P, int32_t a0, in
     return make_int2(total, htotal);
 }
 
+// Filter slot of column c in an MCA-wave tile's mask filter (multiplicative
+// hash, kMcaFilterBits slots).
+__device__ __forceinline__ uint32_t mca_hash(int32_t c) {
+    return ((uint32_t)c * 2654435761u) >> (32 - kMcaFilterLg);
+}
+
 // Tiles of G lanes, 64/G tasks per wave.  The sorted mask row sits in LDS;
 // every lane binary-searches its streamed B entry in it (after a range
-// check against the row's first and last column).  In TC mode the tile's
-// hits are counted from the wave's ballot of found entries with a popcount
-// of the tile's bits.
+// check against the row's first and last column).  In TC mode a bit filter
+// of the mask row comes first: only entries whose bit is set are searched
+// (one per lane per pass), and each lane counts its own hits.
 template <int MODE, int G, bool HUB>
 __global__ __launch_bounds__(64) void k_mca_wave(Prob P, int cls) {
     constexpr int T = 64 / G;
@@ -966,7 +972,14 @@ __global__ __launch_bounds__(64) void k_mca_wave(Prob P, int cls) {
     constexpr bool TCH = HUB;
     __shared__ int32_t hpre[TCH ? T : 1][kMcaMaxTaskA + 1];
     __shared__ int32_t hbst[TCH ? T : 1][kMcaMaxTaskA];
+    // TC: a bit filter of each tile's mask row (bit mca_hash(c) per column)
+    constexpr bool FLT = MODE == MODE_TC;
+    __shared__ uint32_t flt[FLT ? T : 1][FLT ? kMcaFilterWords : 1];
     const int lane = threadIdx.x, tile = lane / G, gl = lane % G;
+    if constexpr (FLT) {
+        for (int w = gl; w < kMcaFilterWords; w += G) flt[tile][w] = 0;
+        __syncthreads();
+    }
     const int32_t lo = P.bin_off[cls * kFlopBins], hi = P.bin_off[(cls + 1) * kFlopBins];
     uint32_t cnt = 0;
     // The wave takes kMcaRounds rounds of T tasks per ticket (P.ticket):
@@ -990,8 +1003,13 @@ __global__ __launch_bounds__(64) void k_mca_wave(Prob P, int cls) {
             mlen = P.mrst[tk.row + 1] - mb;
         }
         for (int32_t q = gl; q < mlen; q += G) {
-            mk[tile][q] = P.mcol[mb + q];
+            const int32_t c = P.mcol[mb + q];
+            mk[tile][q] = c;
             if (HasAcc<MODE>::v) acc[tile][q] = 0;
+            if constexpr (FLT) {
+                const uint32_t h = mca_hash(c);
+                atomicOr(&flt[tile][h >> 5], 1u << (h & 31));
+            }
         }
         if (MODE == MODE_SUPPORT)
             for (int32_t x = gl; x < tk.a1 - tk.a0; x += G) acnt[tile][x] = 0;
@@ -1034,6 +1052,39 @@ __global__ __launch_bounds__(64) void k_mca_wave(Prob P, int cls) {
                 }
 #pragma unroll
                 for (int u = 0; u < kMcaU; ++u) b[u] = t[u] >= 0 ? P.bcol[t[u]] : -1;
+                if constexpr (FLT) {
+                    // TC: entries whose filter bit is clear are not in the
+                    // mask row (almost all of them: hits are rare); a lane
+                    // binary-searches one remaining candidate per pass, so
+                    // the wave runs about one search per batch of kMcaU
+                    // entries instead of one per entry
+                    uint32_t cand = 0;
+#pragma unroll
+                    for (int u = 0; u < kMcaU; ++u) {
+                        const uint32_t h = mca_hash(b[u]);
+                        const bool in = t[u] >= 0 && b[u] >= mlo && b[u] <= mhi &&
+                                        ((flt[tile][h >> 5] >> (h & 31)) & 1u);
+                        cand |= in ? 1u << u : 0u;
+                    }
+                    while (__any(cand != 0)) {
+                        int32_t bb = -1;
+                        if (cand) {
+                            const int uu = __builtin_ctz(cand);
+                            cand &= cand - 1;
+#pragma unroll
+                            for (int u = 0; u < kMcaU; ++u) bb = u == uu ? b[u] : bb;
+                        }
+                        if (bb >= 0) {
+                            int32_t l = 0, h = mlen;  // lower_bound in m[0, mlen)
+                            while (l < h) {
+                                const int32_t mid = l + ((h - l) >> 1);
+                                if (m[mid] < bb) l = mid + 1; else h = mid;
+                            }
+                            cnt_task += m[l] == bb;
+                        }
+                    }
+                    continue;
+                }
 #pragma unroll
                 for (int u = 0; u < kMcaU; ++u) {
                     int32_t q = -1;
@@ -1087,6 +1138,8 @@ __global__ __launch_bounds__(64) void k_mca_wave(Prob P, int cls) {
             }
         }
         __syncthreads();
+        if constexpr (FLT)
+            for (int32_t q = gl; q < mlen; q += G) flt[tile][mca_hash(mk[tile][q]) >> 5] = 0;
         if (live) {
             flush<MODE, G>(P, tk, mb, mlen, acc[tile], acnt[tile], gl);
             if (MODE == MODE_TC && P.rowcount) {
