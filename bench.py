@@ -240,6 +240,11 @@ def bc_bytes(n: int, nnz: int, levels: int, nsrc: int) -> float:
 PLAIN = {"hub_window": 0, "mca_max_mask": 8}
 
 
+# Triangle-counting options of the committed HBM-traffic profile of the
+# tuned product (profiles/tc_rmat22_tuned_counters.md).
+PROFILED_TC_OPTS = {"mca_max_mask": 48, "small_max_mask": 32, "hub_probe": 2, "blocks_long": 16, "blocks_mca": 16}
+
+
 def tc_variants(tuned_h: dict, tuned_l: dict) -> list:
     from graphkokkos_amd import SEL_LEARNED
     return [("plain", PLAIN), ("heuristic_hub", {}), ("tuned_heuristic", tuned_h),
@@ -437,9 +442,12 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-reps", type=int, default=5,
                     help="CPU baselines: best of this many timed runs after one warm-up")
-    ap.add_argument("--tc-variants", default="all", choices=["all", "plain", "default"],
+    ap.add_argument("--tc-variants", default="all", choices=["all", "plain", "default", "opts"],
                     help="plain: only the fixed-heuristic triangle count (no selector training or autotuning); "
-                         "default: plain and the library default (hub bitmaps)")
+                         "default: plain and the library default (hub bitmaps); opts: plain and the options "
+                         "of --tc-opts (a fixed configuration, for profiling it)")
+    ap.add_argument("--tc-opts", default="{}",
+                    help='options of the "opts" triangle-counting variant, as JSON (gk_opts field names)')
     ap.add_argument("--batch-graphs", type=int, default=4096)
     ap.add_argument("--no-learned-graphs", action="store_true",
                     help="skip the grid and Chung-Lu graphs of the learned-selector evaluation")
@@ -487,6 +495,8 @@ def main() -> int:
                         "roofline": roofline_obj(pd),
                         "bytes_alg_all": plain["timing"]["bytes_alg"],
                         "traffic_all_kernels": measured_traffic("tc-product", f"tc-rmat{a.tc_scale}"),
+                        "hbm_frac_measured": (measured_traffic("tc-product", f"tc-rmat{a.tc_scale}") or 0) / 1e9 /
+                        plain["wall_s"] / HBM_PEAK_GBS or None,
                         "frac_all_kernels": (plain["timing"]["bytes_alg"] / 1e9) /
                         (plain["timing"]["total_ms"] / 1e3) / HBM_PEAK_GBS}}
         log(f"tc plain: {plain['wall_s'] * 1e3:.2f} ms, T={plain['count']}, dominant {pd['kernel']} "
@@ -507,6 +517,16 @@ def main() -> int:
                                "frac_all_kernels": (r["timing"]["bytes_alg"] / 1e9) / r["wall_s"] / HBM_PEAK_GBS,
                                "rows_per_acc": r["timing"]["rows_per_acc"]}
         log(f"tc heuristic_hub: {r['wall_s'] * 1e3:.2f} ms ({plain['wall_s'] / r['wall_s']:.3f}x plain)")
+    if tc is not None and a.tc_variants == "opts":
+        opts = json.loads(a.tc_opts)
+        r = run_tc(ctx, G22.L, a.steps, a.warmup, opts)
+        assert r["count"] == plain["count"], "the given options changed the triangle count"
+        d = dominant(r["timing"])
+        tc["given_opts"] = {"value": G22.edges / r["wall_s"], "ms": r["wall_s"] * 1e3, "opts": opts,
+                            "speedup_vs_plain": plain["wall_s"] / r["wall_s"], "roofline": roofline_obj(d),
+                            "frac_all_kernels": (r["timing"]["bytes_alg"] / 1e9) / r["wall_s"] / HBM_PEAK_GBS,
+                            "rows_per_acc": r["timing"]["rows_per_acc"]}
+        log(f"tc given opts {opts}: {r['wall_s'] * 1e3:.2f} ms ({plain['wall_s'] / r['wall_s']:.3f}x plain)")
     if tc is not None and a.tc_variants == "all":
         # learned selector: trained on timing samples the product generates
         # on other seeded graphs; then the global knobs autotuned on the
@@ -536,6 +556,21 @@ def main() -> int:
                          "frac_all_kernels": (r["timing"]["bytes_alg"] / 1e9) / (r["wall_s"]) / HBM_PEAK_GBS,
                          "rows_per_acc": r["timing"]["rows_per_acc"]}
             log(f"tc {name}: {r['wall_s'] * 1e3:.2f} ms ({plain['wall_s'] / r['wall_s']:.3f}x plain)")
+        # the options whose HBM traffic profiles/tc_rmat22_tuned_counters.md
+        # measured (what the autotuner settles on for this graph): measured
+        # bytes over this run's time, beside the algorithmic fraction
+        r = run_tc(ctx, G22.L, a.steps, a.warmup, PROFILED_TC_OPTS)
+        assert r["count"] == plain["count"], "the profiled options changed the triangle count"
+        tb = measured_traffic("tc-product-tuned", f"tc-rmat{a.tc_scale}")
+        res["profiled_tuned"] = {
+            "value": G22.edges / r["wall_s"], "ms": r["wall_s"] * 1e3, "opts": PROFILED_TC_OPTS,
+            "speedup_vs_plain": plain["wall_s"] / r["wall_s"],
+            "frac_all_kernels": (r["timing"]["bytes_alg"] / 1e9) / r["wall_s"] / HBM_PEAK_GBS,
+            "traffic_all_kernels": tb,
+            "hbm_frac_measured": (tb / 1e9) / r["wall_s"] / HBM_PEAK_GBS if tb else None,
+            "rows_per_acc": r["timing"]["rows_per_acc"]}
+        log(f"tc profiled_tuned: {r['wall_s'] * 1e3:.2f} ms ({plain['wall_s'] / r['wall_s']:.3f}x plain), "
+            f"measured HBM {res['profiled_tuned']['hbm_frac_measured']}")
         # every variant against each fixed-selector baseline: plain (above),
         # the library default (hub bitmaps) and the autotuned heuristic
         for name in res:
