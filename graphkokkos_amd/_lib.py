@@ -40,7 +40,7 @@ class GkOpts(C.Structure):
                 ("small_max_mask", C.c_int32), ("blocks_small", C.c_int32), ("blocks_long", C.c_int32),
                 ("blocks_mca", C.c_int32), ("blocks_global", C.c_int32), ("batch_blocks", C.c_int32),
                 ("batch_order", C.c_int32), ("hub_window", C.c_int32), ("learned_margin", C.c_int32),
-                ("hub_probe", C.c_int32)]
+                ("hub_probe", C.c_int32), ("selector_labels", C.c_int32)]
 
 
 class GkTiming(C.Structure):

@@ -203,6 +203,9 @@ gk_opts resolve_opts(const gk_opts *o) {
     gk_opts r;
     gk_opts_default(&r);
     if (o && o->struct_size >= (int32_t)sizeof(gk_opts)) r = *o;
+    // values outside enum gk_selector (the engine's internal ones included)
+    // select the fixed thresholds, as they always have
+    if (r.selector != GK_SEL_LEARNED && r.selector != GK_SEL_FORCED) r.selector = GK_SEL_HEURISTIC;
     return r;
 }
 
@@ -238,6 +241,7 @@ extern "C" void gk_opts_default(gk_opts *o) {
     o->hub_window = 32768;
     o->learned_margin = 250;
     o->hub_probe = 4;
+    o->selector_labels = 1;
 }
 
 namespace gk { int bc_noalloc_setup(gk_ctx *ctx); }

@@ -18,8 +18,9 @@
 //   k_mlp_grad      masked mean-squared-error loss and its gradient,
 //                   per-block LDS accumulation then one atomic per weight
 //   k_adam          Adam update
-//   k_row_cycles    per-row sums of task durations (labels: shares of the
-//                   class kernel's measured time)
+//   k_row_cycles    per-row sums of task durations (labels: wave-time of
+//                   the row, or its share of the class kernel's time)
+//   k_mixed_assign  MSA-LDS / MCA-wave by turns for the concurrent labels
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -237,10 +238,15 @@ __global__ __launch_bounds__(64 * kMlpWaves) void k_mlp_select(int64_t m, RowSta
         if (lane < kTile && i < m) {
             // cheapest predicted accumulator among those that can hold the
             // row; the fixed-threshold choice stays unless it is predicted
-            // to cost more than 2^margin times that
+            // to cost more than 2^margin times that.  The global
+            // accumulators are candidates only for rows too long for
+            // MSA-LDS: where a row fits LDS they never win in the measured
+            // samples, and their labels (timed alone, HBM-bound) compare
+            // badly with those of the LDS kernels timed together.
             const int32_t mlen = st.mlen[i];
-            const bool can[kOut] = {mlen <= kLdsMaxMask, mlen <= kMcaMaxMask, mlen <= kHashGlobalMaxMask, true};
-            int best = GK_ACC_DENSE_GLOBAL;
+            const bool lds = mlen <= kLdsMaxMask;
+            const bool can[kOut] = {lds, mlen <= kMcaMaxMask, !lds && mlen <= kHashGlobalMaxMask, !lds};
+            int best = lds ? (int)GK_ACC_MSA_LDS : (int)GK_ACC_DENSE_GLOBAL;  // allowed either way
 #pragma unroll
             for (int o = 0; o < kOut; ++o)
                 if (can[o] && y[o] < y[best]) best = o;
@@ -360,7 +366,25 @@ __global__ void k_row_cycles(int32_t lo, int32_t hi, const Task *__restrict__ ta
         atomicAdd(&row[tasks[t].row], (double)cyc[t] / per_cu);
 }
 
+// Concurrent label products: row i of round r in MCA-wave or MSA-LDS by the
+// parity of a hash of i plus r, so that each row runs in both over two
+// rounds and each round runs both kernels on about half of the rows
+// (capacity overrides in the binning still apply).
+__global__ void k_mixed_assign(int64_t m, RowStats st, int round) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t h = dev::hash32((uint32_t)i ^ (uint32_t)(i >> 32)) >> 31;
+        st.acc[i] = (int8_t)(((h + (uint32_t)round) & 1u) ? GK_ACC_MCA_WAVE : GK_ACC_MSA_LDS);
+    }
+}
+
 // ---- engine hook -----------------------------------------------------------
+
+int mixed_assign(gk_ctx *ctx, RowStats &st, int64_t m, int round) {
+    if (m == 0) return GK_OK;
+    const int g = (int)std::max<int64_t>(1, std::min<int64_t>((m + 255) / 256, (int64_t)ctx->num_cus * 4));
+    k_mixed_assign<<<g, 256, 0, ctx->stream>>>(m, st, round);
+    return launch_check("k_mixed_assign");
+}
 
 int model_select(gk_ctx *ctx, RowStats &st, int64_t m, int same_am, const SelectParams &sp) {
     if (!ctx->model || !ctx->model->ready) {
@@ -593,11 +617,67 @@ extern "C" int gk_selector_samples(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr 
     GK_TRY(b.alloc(&d_count, 1));
     std::vector<double> hrow(m);
     std::vector<int8_t> hacc(m);
-    // A row's label is its share of the device time of the class kernel it
-    // ran in: the kernel's measured time split over its rows in proportion
-    // to their tasks' clock64 durations.  (Durations alone over-count
-    // kernels whose tasks wait on each other, such as the four tiles of an
-    // MCA wave.)
+    if (base.selector_labels) {
+        // Concurrent labels: two products in which MSA-LDS and MCA-wave run
+        // side by side on alternating halves of the rows (the engine's real
+        // schedule: the classes share the CUs), then hash-global and
+        // dense-global forced in turn.  A row's label is the wave-time of
+        // its tasks: clock64 cycles times the waves a task holds (four for
+        // a long-row window task, one for a short-row or global one, a
+        // quarter for an MCA-wave tile), in ns at 2.4 GHz, averaged where a
+        // row ran in one accumulator more than once.
+        std::vector<double> sum((size_t)m * GK_NUM_ACC, 0.0);
+        std::vector<int32_t> cnt((size_t)m * GK_NUM_ACC, 0);
+        for (int run = 0; run < 4; ++run) {
+            gk_opts o = base;
+            if (run < 2) {
+                o.selector = kSelMixed;
+                o.forced_accum = run;
+            } else {
+                o.selector = GK_SEL_FORCED;
+                o.forced_accum = run == 2 ? GK_ACC_HASH_GLOBAL : GK_ACC_DENSE_GLOBAL;
+            }
+            if (!cyc) GK_TRY(b.alloc(&cyc, A->nz + m + 1));
+            GK_HIP(hipMemsetAsync(d_count, 0, 8, ctx->stream));
+            Plan plan;
+            ProductInfo pi;
+            GK_TRY(masked_product(ctx, A, B, M, mode == 0 ? MODE_TC : MODE_VALUES, d_out, d_count, nullptr, o, &pi,
+                                  cyc, &plan));
+            GK_HIP(d2h(ctx, hacc.data(), st.acc, m));
+            GK_HIP(hipMemsetAsync(rowc, 0, sizeof(double) * m, ctx->stream));
+            int32_t off[kNumBins + 1];
+            GK_HIP(d2h(ctx, off, plan.bin_off, sizeof off));
+            for (int c = 0; c < kNumClass; ++c) {
+                const int32_t lo = off[c * kFlopBins], hi = off[(c + 1) * kFlopBins];
+                // k_row_cycles divides by its `per_cu` argument: 1 / waves per task
+                const float waves = c == kClassMsaLarge ? (float)kWindowWavesLong
+                                    : c == GK_ACC_MCA_WAVE ? 0.25f : 1.f;
+                if (hi > lo)
+                    k_row_cycles<<<std::min(1024, (hi - lo + 255) / 256), 256, 0, ctx->stream>>>(
+                        lo, hi, plan.tasks, cyc, 1.f / waves, rowc);
+            }
+            GK_TRY(launch_check("k_row_cycles"));
+            GK_HIP(d2h(ctx, hrow.data(), rowc, sizeof(double) * m));
+            for (int64_t i = 0; i < m; ++i) {
+                const int c = hacc[i];
+                if (c < 0 || hrow[i] <= 0) continue;
+                const int a = c == kClassMsaLarge ? (int)GK_ACC_MSA_LDS : c;
+                if (run >= 2 && a != o.forced_accum) continue;  // a capacity fallback
+                sum[(size_t)i * GK_NUM_ACC + a] += hrow[i] / 2.4;
+                cnt[(size_t)i * GK_NUM_ACC + a] += 1;
+            }
+        }
+        for (size_t x = 0; x < (size_t)m * GK_NUM_ACC; ++x) {
+            valid[x] = cnt[x] > 0;
+            T[x] = cnt[x] > 0 ? (float)std::log2(sum[x] / cnt[x]) : 0.f;
+        }
+        return GK_OK;
+    }
+    // Forced labels: a row's label is its share of the device time of the
+    // class kernel it ran in: the kernel's measured time split over its rows
+    // in proportion to their tasks' clock64 durations.  (Durations alone
+    // over-count kernels whose tasks wait on each other, such as the four
+    // tiles of an MCA wave.)
     for (int a = 0; a < GK_NUM_ACC; ++a) {
         gk_opts o = base;
         o.selector = GK_SEL_FORCED;

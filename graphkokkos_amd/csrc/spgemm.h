@@ -217,5 +217,10 @@ int launch_classes(gk_ctx *ctx, const Prob &P, const ClassLaunch &L, Timer *tm);
 
 // Learned selector (selector.hip): writes st.acc for every row.
 int model_select(gk_ctx *ctx, RowStats &st, int64_t m, int same_am, const SelectParams &sp);
+// Selector value (internal, outside enum gk_selector) of the concurrent
+// label products: row i runs in MCA-wave or MSA-LDS by the parity of a hash
+// of i plus `round` (gk_opts.forced_accum); mixed_assign writes st.acc.
+constexpr int kSelMixed = 16;
+int mixed_assign(gk_ctx *ctx, RowStats &st, int64_t m, int round);
 
 }  // namespace gk

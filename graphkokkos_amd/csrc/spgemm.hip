@@ -149,7 +149,7 @@ __global__ __launch_bounds__(256) void k_select_bin(int64_t m, RowStats st, Sele
             alen = st.alen[i];
             if (mlen > 0 && flop > 0) {
                 if (sp.selector == GK_SEL_FORCED) acc = sp.forced;
-                else if (sp.selector == GK_SEL_LEARNED) acc = st.acc[i];
+                else if (sp.selector == GK_SEL_LEARNED || sp.selector == kSelMixed) acc = st.acc[i];
                 else acc = heuristic_accum(sp, mlen);
                 // Capacity overrides: a choice that cannot hold the mask row
                 // falls to the next accumulator that can.
@@ -1601,6 +1601,8 @@ int masked_product(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcs
     sp.hub = hub_n > 0;
     if (sp.selector == GK_SEL_LEARNED)
         GK_TRY(model_select(ctx, st, A->m, A->rst == M->rst && A->col == M->col, sp));
+    else if (sp.selector == kSelMixed)
+        GK_TRY(mixed_assign(ctx, st, A->m, o.forced_accum));
     Plan plan;
     GK_TRY(plan_tasks(ctx, A, st, sp, &plan));
     Prob P{};

@@ -118,6 +118,10 @@ typedef struct gk_opts {
                              * row's columns below it is less than the row's words and
                              * entries (a test is a gathered word, the others are
                              * streamed); 0 = off, default 4                         */
+    int32_t selector_labels;/* gk_selector_samples: 1 = concurrent (MSA-LDS and MCA-wave
+                             * timed together on alternating halves of the rows, the
+                             * engine's real schedule), 0 = each accumulator forced
+                             * alone; default 1                                      */
 } gk_opts;
 void gk_opts_default(gk_opts *o);
 
@@ -379,10 +383,15 @@ int gk_model_train(gk_ctx *ctx, const float *X, const float *T, const uint8_t *v
                    int64_t n, int epochs, int batch, float lr, uint64_t seed,
                    float *final_loss);
 /* Training samples of C<M> = A*B (mode 0: triangle counting, 1: values):
- * X[m x 16] features, T[m x 4] measured log2 ns per accumulator (the row's
- * share of its kernel's time), valid[m x 4].  The rows are timed under opts
- * (NULL: defaults) with the accumulator forced in turn, so the engine knobs
- * the selector will run with (hub_window, thresholds) shape the labels. */
+ * X[m x 16] features, T[m x 4] measured log2 cost per accumulator, valid[m x 4].
+ * The rows are timed under opts (NULL: defaults), so the engine knobs the
+ * selector will run with (hub_window, thresholds) shape the labels.
+ * opts.selector_labels = 1: MSA-LDS and MCA-wave run concurrently in two
+ * products, each row in one of them by turns, and a label is the row's
+ * wave-time (clock64 cycles of its tasks times the waves a task holds, in ns
+ * at 2.4 GHz); hash-global and dense-global are forced in turn, same units.
+ * 0: every accumulator forced in turn, a label being the row's share of its
+ * kernel's measured time. */
 int gk_selector_samples(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B,
                         const gk_dcsr *M, int mode, float *X, float *T,
                         uint8_t *valid, const gk_opts *opts);
