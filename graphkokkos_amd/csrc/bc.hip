@@ -86,23 +86,38 @@ __device__ __forceinline__ void pull_sum(int32_t t0, int32_t t1, int32_t step, c
     if (!mask) return;
     const int word = lane >> 4, sh = (lane & 15) * 4;
     for (int32_t t = t0; t < t1; t += step * U) {
-        int64_t w[U];
+        // Every load of a batch is issued before any is used (the column
+        // loads at clamped positions, then the level bits, then the four
+        // doubles of each neighbour with a matching source, predicated with
+        // no branch on loaded data between them), so U neighbours are in
+        // flight per lane.  The sums add the set sources in neighbour order,
+        // as a loop over the neighbours one by one would.
+        int32_t w[U];
         uint32_t f[U];
 #pragma unroll
+        for (int u = 0; u < U; ++u) w[u] = col[min(t + u * step, t1 - 1)];
+#pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int32_t tt = t + u * step;
-            w[u] = tt < t1 ? (int64_t)col[tt] : -1;
+            const uint32_t b = (uint32_t)(fbits[4 * (int64_t)w[u] + word] >> sh) & mask;
+            f[u] = t + u * step < t1 ? b : 0u;
+        }
+        double2 lo[U], hi[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            lo[u] = make_double2(0.0, 0.0);
+            hi[u] = lo[u];
+            if (f[u]) {
+                const double2 *sw = reinterpret_cast<const double2 *>(S + (int64_t)w[u] * kSrc + 4 * lane);
+                lo[u] = sw[0];
+                hi[u] = sw[1];
+            }
         }
 #pragma unroll
-        for (int u = 0; u < U; ++u) f[u] = w[u] >= 0 ? ((uint32_t)(fbits[4 * w[u] + word] >> sh) & mask) : 0u;
-#pragma unroll
         for (int u = 0; u < U; ++u) {
-            if (!f[u]) continue;
-            const double *sw = S + w[u] * kSrc + 4 * lane;
-            if (f[u] & 1) a[0] += sw[0];
-            if (f[u] & 2) a[1] += sw[1];
-            if (f[u] & 4) a[2] += sw[2];
-            if (f[u] & 8) a[3] += sw[3];
+            if (f[u] & 1) a[0] += lo[u].x;
+            if (f[u] & 2) a[1] += lo[u].y;
+            if (f[u] & 4) a[2] += hi[u].x;
+            if (f[u] & 8) a[3] += hi[u].y;
         }
     }
 }
@@ -160,7 +175,7 @@ __global__ __launch_bounds__(256) void k_bc_forward(int64_t n, const int32_t *__
         if (t1 - t0 >= kHeavy) continue;
         const uint32_t unv = ~nib_of(vis + 4 * i, lane) & 0xFu;
         double a[4] = {0, 0, 0, 0};
-        if (__ballot(unv != 0)) pull_sum<8>(t0, t1, 1, tcol, fd, unv, S, lane, a);
+        if (__ballot(unv != 0)) pull_sum<4>(t0, t1, 1, tcol, fd, unv, S, lane, a);
         reached += forward_finish(i, unv, a, fn, vis, S, lane);
     }
     unsigned long long r = dev::wave_sum((unsigned long long)reached);
@@ -238,7 +253,7 @@ __global__ __launch_bounds__(64 * kHeavyWaves) void k_bc_heavy_chunk(
         const uint32_t mask = sweep_mask<FWD>(i, vis, fd, lane);
         if (!__syncthreads_or(mask != 0)) continue;
         double a[4] = {0, 0, 0, 0};
-        pull_sum<8>(ch.y + wv, ch.z, kHeavyWaves, col, FWD ? fd : fn, mask, S, lane, a);
+        pull_sum<4>(ch.y + wv, ch.z, kHeavyWaves, col, FWD ? fd : fn, mask, S, lane, a);
 #pragma unroll
         for (int b = 0; b < 4; ++b) psum[wv][4 * lane + b] = a[b];
         __syncthreads();
@@ -326,7 +341,7 @@ __global__ __launch_bounds__(256) void k_bc_backward(int64_t n, const int32_t *_
         const int32_t t0 = rst[i], t1 = rst[i + 1];
         if (t1 - t0 >= kHeavy) continue;
         double a[4] = {0, 0, 0, 0};
-        pull_sum<8>(t0, t1, 1, col, fn, nd, S, lane, a);
+        pull_sum<4>(t0, t1, 1, col, fn, nd, S, lane, a);
         const double dsum = backward_finish(i, nd, a, S, lane);
         if (lane == 0) bc[i] += dsum;
     }
