@@ -251,7 +251,14 @@ __global__ __launch_bounds__(64 * kHeavyWaves) void k_bc_heavy_chunk(
         const int4 ch = chunks[c];
         const int64_t i = ch.x;
         const uint32_t mask = sweep_mask<FWD>(i, vis, fd, lane);
-        if (!__syncthreads_or(mask != 0)) continue;
+        if (!__syncthreads_or(mask != 0)) {
+            // every source has reached i: i is on no new level, and its
+            // frontier word is written all the same (the level buffers are
+            // reused from batch to batch); a vertex with several chunks gets
+            // it from k_bc_heavy_finish
+            if (FWD && !ch.w && threadIdx.x < 4) fn[4 * i + threadIdx.x] = 0;
+            continue;
+        }
         double a[4] = {0, 0, 0, 0};
         pull_sum<4>(ch.y + wv, ch.z, kHeavyWaves, col, FWD ? fd : fn, mask, S, lane, a);
 #pragma unroll
@@ -294,7 +301,10 @@ __global__ __launch_bounds__(256) void k_bc_heavy_finish(int32_t nm, const int4 
         const int4 mv = multi[x];
         const int64_t i = mv.x;
         const uint32_t mask = sweep_mask<FWD>(i, vis, fd, lane);
-        if (!__ballot(mask != 0)) continue;
+        if (!__ballot(mask != 0)) {
+            if (FWD && lane < 4) fn[4 * i + lane] = 0;  // on no new level
+            continue;
+        }
         double a[4] = {0, 0, 0, 0};
         for (int32_t j = 0; j < mv.z; ++j) {
             const double *p = part + (int64_t)(mv.y + j) * kSrc + 4 * lane;

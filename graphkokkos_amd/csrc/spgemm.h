@@ -36,8 +36,19 @@ constexpr int kWindowBlocksLong = 8;  // per CU
 constexpr int kWindowU = 8;
 // ... in the MCA-wave kernel
 constexpr int kMcaU = 8;
-// rounds (of 64 / 16 tasks) an MCA-wave wave takes per ticket
+// rounds (of 64 / 16 tasks) an MCA-wave wave takes per ticket, at least
 constexpr int kMcaRounds = 8;
+// Task tickets: a class with many tasks per workgroup takes several
+// consecutive tasks per ticket, so that each workgroup (MCA-wave: each
+// wave) draws about kTicketsPerUnit tickets: every ticket is an atomic on
+// one counter that the class's whole grid shares, and with millions of
+// small tasks those atomics, not the tasks, set the pace.
+constexpr int kTicketsPerUnit = 16;
+constexpr int kTicketChunkMax = 64;
+__host__ __device__ constexpr int32_t ticket_chunk(int32_t ntask, int32_t units, int32_t least) {
+    const int32_t k = ntask / (units * kTicketsPerUnit);
+    return k < least ? least : (k > kTicketChunkMax ? kTicketChunkMax : k);
+}
 // triangle counting: MCA-wave's per-task bit filter of the mask row
 // (2^kMcaFilterLg bits; a mask row of 64 columns sets at most 1/32 of them)
 constexpr int kMcaFilterLg = 11;

@@ -757,9 +757,10 @@ struct WinPrefix {
 // A task's set-up is one round of global loads: the mask row (held in
 // registers until the task ends, when its bitmap words are cleared from
 // them), the A columns and B row pointers (WinPrefix), all issued before
-// the first barrier.  Blocks take tasks by ticket (P.ticket), the next one
-// while the current task streams, so a class whose blocks outnumber the
-// slots free beside the other classes spreads onto the CUs they release.
+// the first barrier.  Blocks take tasks by ticket (P.ticket), K consecutive
+// tasks per ticket (ticket_chunk), the next ticket while the chunk's last
+// task streams, so a class whose blocks outnumber the slots free beside the
+// other classes spreads onto the CUs they release.
 // The hash starts empty for every
 // task (it is cleared after a task that used it), with a capacity fixed by
 // the mask row's length, so columns go into it as they are read.
@@ -788,11 +789,12 @@ __global__ __launch_bounds__(64 * NW) void k_msa_window(Prob P, int cls) {
     __shared__ int32_t next_s;   // the block's next task
     const int tid = threadIdx.x;
     const int32_t lo = P.bin_off[cls * kFlopBins], hi = P.bin_off[(cls + 1) * kFlopBins];
+    const int32_t K = ticket_chunk(hi - lo, (int32_t)gridDim.x, 1);
     for (int w = tid; w < kWindow / 32; w += NT) bm[w] = 0;
     for (int s = tid; s < HC; s += NT) hkey[s] = -1;
     if (tid == 0) {
         nhigh_s = 0;
-        next_s = lo + atomicAdd(&P.ticket[cls], 1);
+        next_s = lo + atomicAdd(&P.ticket[cls], 1) * K;
     }
     __syncthreads();
     uint32_t cnt = 0;
@@ -838,9 +840,10 @@ __global__ __launch_bounds__(64 * NW) void k_msa_window(Prob P, int cls) {
         else pf.scan_block1(wsum);
         __syncthreads();
         const int32_t nhigh = nhigh_s, nlow = mlen - nhigh;
-        // the next task's ticket, taken while this one streams
+        // the next chunk's ticket, taken while the chunk's last task streams
+        const bool chunk_end = (ti - lo) % K == K - 1;
         int32_t tick = 0;
-        if (tid == 0) tick = atomicAdd(&P.ticket[cls], 1);
+        if (tid == 0 && chunk_end) tick = atomicAdd(&P.ticket[cls], 1);
         if constexpr (NW > 1) {
             pf.scan_block2(tk.a0, tk.a1, pre, bst, hpre, hbst, ppre, pofs, wsum);
             __syncthreads();
@@ -872,7 +875,7 @@ __global__ __launch_bounds__(64 * NW) void k_msa_window(Prob P, int cls) {
                 s = (s + 1) & cmask;
             }
         });
-        if (tid == 0) next_s = lo + tick;
+        if (tid == 0) next_s = chunk_end ? lo + tick * K : ti + 1;
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < CPT; ++j)
@@ -982,12 +985,12 @@ __global__ __launch_bounds__(64) void k_mca_wave(Prob P, int cls) {
     }
     const int32_t lo = P.bin_off[cls * kFlopBins], hi = P.bin_off[(cls + 1) * kFlopBins];
     uint32_t cnt = 0;
-    // The wave takes kMcaRounds rounds of T tasks per ticket (P.ticket):
-    // tasks here are small and many, and one ticket per round would
-    // serialise on the shared counter.  The next ticket is taken during a
-    // chunk's last round.  All tiles run the same rounds, so the wave stays
-    // converged at the barriers.
-    constexpr int RB = kMcaRounds;
+    // The wave takes RB >= kMcaRounds rounds of T tasks per ticket
+    // (P.ticket, ticket_chunk): tasks here are small and many, and one
+    // ticket per round would serialise on the shared counter.  The next
+    // ticket is taken during a chunk's last round.  All tiles run the same
+    // rounds, so the wave stays converged at the barriers.
+    const int32_t RB = ticket_chunk((hi - lo) / T, (int32_t)gridDim.x, kMcaRounds);
     int32_t first = 0, tick = 0, r = 0;
     if (lane == 0) first = atomicAdd(&P.ticket[cls], T * RB);
     first = lo + __builtin_amdgcn_readfirstlane(first);

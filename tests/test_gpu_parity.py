@@ -357,3 +357,29 @@ def test_bc_repeated_edges(gpu):
     D = SpMtx(G.n, G.n, rst, e[:, 1].astype(np.int32))
     src = rng.choice(np.flatnonzero(np.diff(rst) > 0), size=300, replace=False).astype(np.int32)
     assert_bc_close(gpu.bc_batch_f64(D, src), O.bc_batch(D, src))
+
+
+def hub_batches_graph():
+    """Two heavy vertices (more neighbours than the one-wave kernels take):
+    A = 0 with leaves 1..2600 (several chunks), B = 2601 with leaves 1..256
+    and 2602..3445 (one chunk); far vertex 3702 + j hangs off leaf j + 1.
+    Sources: the 256 far vertices (A and B on level 2 for every one), then
+    leaves 1..256 (A and B on level 1 for every one).  In the second batch
+    every source has reached both hubs before level 2, and their level-2
+    frontier words must not keep the first batch's bits."""
+    u = list(range(1, 2601)) + list(range(1, 257)) + list(range(2602, 3446)) + [3702 + j for j in range(256)]
+    v = [0] * 2600 + [2601] * (256 + 844) + [j + 1 for j in range(256)]
+    G = HostGraph.from_edges(3958, u, v)
+    src = np.array([3702 + j for j in range(256)] + list(range(1, 257)), np.int32)
+    return G, src
+
+
+def test_bc_hubs_reached_earlier_in_a_later_batch(gpu):
+    say("bc hub levels across batches")
+    G, src = hub_batches_graph()
+    assert np.diff(G.A.Rst)[0] == 2600 and np.diff(G.A.Rst)[2601] == 1100
+    want = O.bc_batch(G.A, src)
+    assert_bc_close(gpu.bc_batch_f64(G.A, src), want)
+    # the batches in the other order, and each alone
+    assert_bc_close(gpu.bc_batch_f64(G.A, src[::-1].copy()), want)
+    assert_bc_close(gpu.bc_batch_f64(G.A, src[:256]) + gpu.bc_batch_f64(G.A, src[256:]), want)

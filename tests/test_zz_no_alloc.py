@@ -11,6 +11,7 @@ import pytest
 from graphkokkos_amd import HostGraph, SpMtx
 from graphkokkos_amd._lib import GraphHipError as GkError, lib
 from oracle import oracle as O
+from test_gpu_parity import hub_batches_graph
 
 pytestmark = pytest.mark.gpu
 
@@ -54,6 +55,16 @@ def test_noalloc_matches_bc_batch_dev_two_calls(gpu):
     got, want = both(gpu, H.A, src2)
     assert np.array_equal(got, want)
     assert close(got, O.bc_batch(H.A, src2)) <= 1e-9
+
+
+def test_noalloc_hubs_reached_earlier_in_a_later_batch(gpu):
+    # heavy vertices that every source of the second batch reaches before
+    # level 2: their frontier words of that level are rewritten, not kept
+    # from the first batch (the scratch array is never cleared as a whole)
+    G, src = hub_batches_graph()
+    got, want = both(gpu, G.A, src)
+    assert np.array_equal(got, want)
+    assert close(got, O.bc_batch(G.A, src)) <= 1e-9
 
 
 def test_noalloc_directed_with_transpose(gpu):
