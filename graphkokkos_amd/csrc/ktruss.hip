@@ -19,6 +19,7 @@
 //        quarter of its entries are dead.
 // Rounds repeat until no edge is left (decomposition) or none is removed
 // (single k).  Truss numbers come back per entry of the symmetric pattern.
+#include <cstddef>
 #include <cstring>
 
 #include <rocprim/device/device_scan.hpp>
@@ -428,12 +429,6 @@ constexpr int kPeelU = 4;
 // finds each item's owner among their prefix values in registers
 constexpr int kOwnerWin = 16;
 
-// Frontiers of at most kSmallRound edges get their prologue and the scan of
-// len in one block (k_dround_prologue).
-constexpr int kSmallRoundThreads = 1024;
-constexpr int kSmallRoundPer = 4;
-constexpr int kSmallRound = kSmallRoundThreads * kSmallRoundPer;
-
 // Support update of one round, as the masked product dS<A> = F * A with the
 // adjacency tables as the mask accumulator: the entries w of every frontier
 // edge's streamed row are flattened over all lanes (pre = exclusive scan of
@@ -441,16 +436,16 @@ constexpr int kSmallRound = kSmallRoundThreads * kSmallRoundPer;
 // the frontier edge with the smallest id among its frontier edges -- takes
 // one support off each of its edges that stays alive.  Edge states: 0 alive,
 // 1 frontier, 2 removed earlier.
-// Control block of device-driven rounds (frontiers small enough for the
-// one-block prologue): the kernels of a batch of rounds read the sizes here
-// instead of from the host.  F lives in buf[cur], the previous frontier and
-// this round's crossings in buf[cur ^ 1].
+// Control block of device-driven rounds: the kernels of a batch of rounds
+// read the sizes here instead of from the host.  F lives in buf[cur], the
+// previous frontier and this round's crossings in buf[cur ^ 1].  Two blocks
+// alternate (k_round_prep); seq numbers their writes.
 struct RoundCtl {
     int32_t nP, nF, nFn, cur;
-    int32_t stop;     // set when a round cannot run on the device path
+    int32_t seq;      // writes so far (the newer of the two blocks is the state)
     int32_t rounds;   // rounds run by the batch
     int32_t removed;  // frontier edges of those rounds
-    int32_t adv;      // a streamed round waits for its advance (one-block path)
+    int32_t adv;      // a streamed round waits for its advance (done by the next prep)
 };
 
 // Moves the control block past a streamed round: its frontier becomes the
@@ -469,7 +464,9 @@ __host__ __device__ __forceinline__ void advance_round(RoundCtl &c) {
 // scans (frontier size, complemented minimum support, near-list length,
 // alive-list compaction).
 struct PeelCtl {
-    RoundCtl rc;
+    RoundCtl rc[2];
+    int32_t stop;  // the batch has stopped: its remaining kernels return
+    int32_t err;   // a prep block's look-back wait gave up (no result)
     int32_t n;
     uint32_t mn;
     int32_t nl;
@@ -743,63 +740,156 @@ __device__ __forceinline__ void k_peel_body(int32_t nF, const int32_t *__restric
 // streamed items per wave step of the peeling kernels
 constexpr int kPeelChunk = 64 * kPeelU;
 
-// Device-driven round: the previous round's advance, then prologue + scan in
-// one block, sizes from the control block.  A frontier that is empty or too
-// big for one block stops the batch; the remaining kernels of the batch
-// return.  The round's own advance is left pending (adv) for the next
-// prologue, or for the host after the batch.
-__global__ __launch_bounds__(kSmallRoundThreads) void k_dround_prologue(
-    RoundCtl *__restrict__ ctl, int32_t *__restrict__ buf0, int32_t *__restrict__ buf1, uint8_t *__restrict__ st,
-    uint32_t *__restrict__ truss, uint32_t label, const int32_t *__restrict__ eu, const int32_t *__restrict__ ev,
-    const int32_t *__restrict__ sup, const int32_t *__restrict__ rst, int32_t *__restrict__ fy,
-    int32_t *__restrict__ fx, int32_t *__restrict__ pre, int32_t *__restrict__ cstart, int32_t ccap) {
-    __shared__ int32_t wsum[kSmallRoundThreads / 64];
-    __shared__ RoundCtl c;
-    if (threadIdx.x == 0) {
-        c = *ctl;
-        if (!c.stop && c.adv) {
-            advance_round(c);
-            *ctl = c;
-        }
-    }
-    __syncthreads();
-    if (c.stop) return;
+// Device-driven rounds, two kernels each: k_round_prep, then
+// k_dround_stream.  Round r's prep reads the control block C[r & 1] (the
+// previous round's state, its advance pending when adv is set) and writes
+// the advanced state to C[(r + 1) & 1], which the round's stream reads and
+// whose nFn counts the stream's crossings; so no block is written while a
+// kernel of the same launch may still read it.  The prep marks the previous
+// frontier removed and the new one frontier (labels, streamed rows, their
+// lengths) and scans the lengths into pre in the same launch: each workgroup
+// owns a contiguous piece of the frontier and finds the items before it by
+// decoupled look-back over the workgroups before it (workgroups are
+// dispatched in index order, and each waits only on lower ones).  An empty
+// frontier (the level is finished, or nothing was removed) sets the stop
+// word: every later kernel of the batch returns at once.
+constexpr int kPrepThreads = 1024;
+// pinned staging of a decomposition: the tables' size, scan counters and a
+// PeelCtl copy
+constexpr int kPinnedBytes = 256;
+constexpr int kPrepPer = 4;        // frontier edges per thread per pass
+constexpr int kPrepBlocksMax = 128;
+// look-back polls before a prep workgroup gives up (about a second; a
+// workgroup that gives up sets the error word, and the host fails the run)
+constexpr uint32_t kLookbackSpins = 1u << 20;
+
+// Look-back word of a prep workgroup: epoch << 33 | inclusive << 32 | items
+// (its own, or with every workgroup before it when inclusive).  The epoch
+// is the launch's number in the decomposition, so words of earlier rounds
+// read as not ready.
+__device__ __forceinline__ unsigned long long lb_word(uint32_t epoch, bool incl, uint32_t v) {
+    return ((unsigned long long)epoch << 33) | ((unsigned long long)incl << 32) | v;
+}
+
+__global__ __launch_bounds__(kPrepThreads) void k_round_prep(
+    const RoundCtl *__restrict__ ci, RoundCtl *__restrict__ co, int32_t *__restrict__ stopw,
+    int32_t *__restrict__ err, unsigned long long *__restrict__ lb, uint32_t epoch, int32_t *__restrict__ buf0,
+    int32_t *__restrict__ buf1, uint8_t *__restrict__ st, uint32_t *__restrict__ truss, uint32_t label,
+    const int32_t *__restrict__ eu, const int32_t *__restrict__ ev, const int32_t *__restrict__ sup,
+    const int32_t *__restrict__ rst, int32_t *__restrict__ fy, int32_t *__restrict__ fx,
+    int32_t *__restrict__ len, int32_t *__restrict__ pre, int32_t *__restrict__ cstart, int32_t ccap) {
+    constexpr int NW = kPrepThreads / 64;
+    __shared__ int32_t ws[NW];
+    __shared__ int32_t excl_s;
+    if (*stopw) return;
+    RoundCtl c = *ci;
+    if (c.adv) advance_round(c);
+    c.seq += 1;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int32_t nF = c.nF, nP = c.nP;
-    if (nF == 0 || nF >= kSmallRound) {
-        if (threadIdx.x == 0) ctl->stop = 1;
+    if (nF == 0) {
+        if (blockIdx.x == 0 && tid == 0) {
+            *co = c;
+            *stopw = 1;
+        }
         return;
     }
     const int32_t *F = c.cur ? buf1 : buf0;
     const int32_t *Fp = c.cur ? buf0 : buf1;
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    mark_removed<4>(nP, Fp, st, tid, kSmallRoundThreads);
-    int32_t fi[kSmallRoundPer], l[kSmallRoundPer], run = 0;
+    mark_removed<4>(nP, Fp, st, blockIdx.x * kPrepThreads + tid, gridDim.x * kPrepThreads);
+    const int32_t chunk = (nF + gridDim.x - 1) / gridDim.x;
+    const int32_t f0 = min(nF, (int32_t)blockIdx.x * chunk), f1 = min(nF, f0 + chunk);
+    // pass 1: marks, labels, streamed rows and their lengths; the piece's sum
+    int32_t sum = 0;
+    for (int32_t b = f0 + tid; b < f1; b += kPrepPer * kPrepThreads) {
+        int32_t fi[kPrepPer], l[kPrepPer];
 #pragma unroll
-    for (int j = 0; j < kSmallRoundPer; ++j) fi[j] = tid * kSmallRoundPer + j;
-    prep_edges<kSmallRoundPer>(fi, nF, F, st, truss, label, eu, ev, sup, rst, fy, fx, l);
+        for (int j = 0; j < kPrepPer; ++j) fi[j] = b + j * kPrepThreads < f1 ? b + j * kPrepThreads : nF;
+        prep_edges<kPrepPer>(fi, nF, F, st, truss, label, eu, ev, sup, rst, fy, fx, l);
 #pragma unroll
-    for (int j = 0; j < kSmallRoundPer; ++j) run += l[j];
-    const int32_t incl = dev::wave_incl_scan(run);
-    if (lane == 63) wsum[w] = incl;
-    __syncthreads();
-    int32_t before = 0;
-    for (int x = 0; x < w; ++x) before += wsum[x];
-    int32_t ex = before + incl - run;
-#pragma unroll
-    for (int j = 0; j < kSmallRoundPer; ++j) {
-        const int32_t f = tid * kSmallRoundPer + j;
-        if (f <= nF) pre[f] = ex;
-        if (f < nF) mark_chunks(f, ex, l[j], kPeelChunk, cstart, ccap);
-        ex += l[j];
+        for (int j = 0; j < kPrepPer; ++j)
+            if (fi[j] < nF) {
+                len[fi[j]] = l[j];
+                sum += l[j];
+            }
     }
-    if (tid == 0) {
-        ctl->nFn = 0;
-        ctl->adv = 1;  // the stream kernel that follows runs this round
+    sum = dev::wave_sum(sum);
+    if (lane == 0) ws[w] = sum;
+    __syncthreads();
+    // look-back (wave 0): items of the workgroups before this one
+    if (w == 0) {
+        int32_t agg = 0;
+#pragma unroll
+        for (int x = 0; x < NW; ++x) agg += ws[x];
+        uint32_t excl = 0;
+        const int32_t bi = blockIdx.x;
+        if (lane == 0) __hip_atomic_store(&lb[bi], lb_word(epoch, bi == 0, (uint32_t)agg), __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+        int32_t look = bi - 1;  // highest workgroup not yet added
+        uint32_t spins = 0;
+        while (look >= 0) {
+            const int32_t p = look - lane;
+            // workgroups below 0 read as an inclusive zero, so a window
+            // always ends at workgroup 0 (whose word is inclusive)
+            const unsigned long long v = p >= 0 ? __hip_atomic_load(&lb[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                                : lb_word(epoch, true, 0);
+            const bool ready = (uint32_t)(v >> 33) == epoch;
+            const uint64_t incl = __ballot(ready && ((v >> 32) & 1ull));
+            const int first = incl ? __builtin_ctzll(incl) : 64;  // nearest inclusive word
+            const uint64_t need = first >= 63 ? ~0ull : ((2ull << first) - 1ull);
+            if (__ballot(!ready) & need) {
+                if (++spins > kLookbackSpins) {
+                    if (lane == 0) atomicExch(err, 1);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                continue;
+            }
+            excl += dev::wave_sum(lane <= first ? (uint32_t)v : 0u);
+            if (first < 64) break;
+            look -= 64;
+        }
+        if (bi > 0 && lane == 0)
+            __hip_atomic_store(&lb[bi], lb_word(epoch, true, excl + (uint32_t)agg), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        if (lane == 0) excl_s = (int32_t)excl;
+    }
+    __syncthreads();
+    // pass 2: the piece's lengths scanned from its offset into pre, and the
+    // owners of the stream chunks that start in it
+    int32_t carry = excl_s;
+    for (int32_t c0 = f0; c0 < f1; c0 += kPrepThreads) {
+        const int32_t f = c0 + tid;
+        const int32_t l = f < f1 ? len[f] : 0;
+        const int32_t incl = dev::wave_incl_scan(l);
+        if (lane == 63) ws[w] = incl;
+        __syncthreads();
+        int32_t before = 0, tot = 0;
+#pragma unroll
+        for (int x = 0; x < NW; ++x) {
+            const int32_t v = ws[x];
+            before += x < w ? v : 0;
+            tot += v;
+        }
+        if (f < f1) {
+            const int32_t ex = carry + before + incl - l;
+            pre[f] = ex;
+            mark_chunks(f, ex, l, kPeelChunk, cstart, ccap);
+        }
+        carry += tot;
+        __syncthreads();
+    }
+    if (f1 == nF && f0 < f1 && tid == 0) pre[nF] = carry;
+    if (blockIdx.x == 0 && tid == 0) {
+        c.nFn = 0;
+        c.adv = 1;  // the stream kernel that follows runs this round
+        *co = c;
     }
 }
 
 template <int U>
-__global__ __launch_bounds__(256) void k_dround_stream(const RoundCtl *__restrict__ ctl, int32_t *__restrict__ buf0,
+__global__ __launch_bounds__(256) void k_dround_stream(const RoundCtl *__restrict__ ctl,
+                                                       const int32_t *__restrict__ stopw, int32_t *__restrict__ buf0,
                                                        int32_t *__restrict__ buf1, const int32_t *__restrict__ pre,
                                                        const int32_t *__restrict__ fy, const int32_t *__restrict__ fx,
                                                        const int32_t *__restrict__ cstart, int32_t ccap,
@@ -809,115 +899,11 @@ __global__ __launch_bounds__(256) void k_dround_stream(const RoundCtl *__restric
                                                        const uint8_t *__restrict__ st, int32_t *__restrict__ sup,
                                                        int32_t thresh, int32_t *__restrict__ nFn,
                                                        const NearList nl, unsigned long long *__restrict__ ctr) {
-    if (ctl->stop) return;
+    if (*stopw) return;
     const int32_t cur = ctl->cur;
     // the crossings go to buf[cur ^ 1], counted in ctl->nFn
     k_peel_body<U>(ctl->nF, pre, cstart, ccap, cur ? buf1 : buf0, fy, fx, col, eid, hoff, tab, st, sup, thresh,
                    cur ? buf0 : buf1, nFn, nl, ctr);
-}
-
-// One thread moves the control block to the next round.  (A device-scope
-// fence in every block of the stream kernel, to let its last block do this,
-// costs more than this launch: on gfx950 such a fence writes back the XCD's
-// L2.)
-__global__ void k_dround_advance(RoundCtl *__restrict__ ctl) {
-    if (threadIdx.x != 0 || ctl->stop) return;
-    RoundCtl c = *ctl;
-    advance_round(c);
-    *ctl = c;
-}
-
-// Device-driven rounds for frontiers of at least kSmallRound edges, in
-// three kernels: k_big_prologue (every block owns a contiguous chunk of the
-// frontier: marks, streamed lengths, the chunk's sum), k_big_scan_top (one
-// block: exclusive scan of the chunk sums) and k_big_scan_apply (every
-// chunk's lengths scanned from its offset into pre).  A frontier below
-// kSmallRound (or empty) stops the batch, which the one-block path of
-// k_dround_prologue then continues.
-constexpr int kBigBlocksMax = 1024;
-
-__global__ __launch_bounds__(256) void k_big_prologue(
-    RoundCtl *__restrict__ ctl, int32_t *__restrict__ buf0, int32_t *__restrict__ buf1, uint8_t *__restrict__ st,
-    uint32_t *__restrict__ truss, uint32_t label, const int32_t *__restrict__ eu, const int32_t *__restrict__ ev,
-    const int32_t *__restrict__ sup, const int32_t *__restrict__ rst, int32_t *__restrict__ fy,
-    int32_t *__restrict__ fx, int32_t *__restrict__ len, int32_t *__restrict__ bsum) {
-    __shared__ int32_t ws[256 / 64];
-    if (ctl->stop) return;
-    const int32_t nF = ctl->nF, nP = ctl->nP;
-    if (nF < kSmallRound) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) ctl->stop = 1;
-        return;
-    }
-    const int32_t *F = ctl->cur ? buf1 : buf0;
-    const int32_t *Fp = ctl->cur ? buf0 : buf1;
-    const int32_t stride = gridDim.x * blockDim.x;
-    mark_removed<4>(nP, Fp, st, blockIdx.x * blockDim.x + threadIdx.x, stride);
-    const int32_t chunk = (nF + gridDim.x - 1) / gridDim.x;
-    const int32_t f0 = min(nF, (int32_t)blockIdx.x * chunk), f1 = min(nF, f0 + chunk);
-    int32_t sum = 0;
-    for (int32_t b = f0 + threadIdx.x; b < f1; b += 4 * blockDim.x) {
-        int32_t fi[4], l[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) fi[j] = b + j * (int32_t)blockDim.x < f1 ? b + j * (int32_t)blockDim.x : nF;
-        prep_edges<4>(fi, nF, F, st, truss, label, eu, ev, sup, rst, fy, fx, l);
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-            if (fi[j] < nF) {
-                len[fi[j]] = l[j];
-                sum += l[j];
-            }
-    }
-    sum = dev::wave_sum(sum);
-    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = sum;
-    __syncthreads();
-    if (threadIdx.x == 0) bsum[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
-}
-
-__global__ __launch_bounds__(kBigBlocksMax) void k_big_scan_top(RoundCtl *__restrict__ ctl, int32_t *__restrict__ bsum,
-                                                                int nb) {
-    __shared__ int32_t ws[kBigBlocksMax / 64];
-    if (ctl->stop) return;
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int32_t v = tid < nb ? bsum[tid] : 0;
-    const int32_t incl = dev::wave_incl_scan(v);
-    if (lane == 63) ws[w] = incl;
-    __syncthreads();
-    int32_t before = 0;
-    for (int x = 0; x < w; ++x) before += ws[x];
-    if (tid < nb) bsum[tid] = before + incl - v;
-    if (tid == 0) ctl->nFn = 0;
-}
-
-__global__ __launch_bounds__(256) void k_big_scan_apply(const RoundCtl *__restrict__ ctl,
-                                                        const int32_t *__restrict__ len,
-                                                        const int32_t *__restrict__ boff, int32_t *__restrict__ pre,
-                                                        int32_t *__restrict__ cstart, int32_t ccap) {
-    __shared__ int32_t ws[256 / 64];
-    if (ctl->stop) return;
-    const int32_t nF = ctl->nF;
-    const int32_t chunk = (nF + gridDim.x - 1) / gridDim.x;
-    const int32_t f0 = min(nF, (int32_t)blockIdx.x * chunk), f1 = min(nF, f0 + chunk);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    int32_t carry = boff[blockIdx.x];
-    for (int32_t c = f0; c < f1; c += blockDim.x) {
-        const int32_t f = c + threadIdx.x;
-        const int32_t l = f < f1 ? len[f] : 0;
-        const int32_t incl = dev::wave_incl_scan(l);
-        if (lane == 63) ws[w] = incl;
-        __syncthreads();
-        int32_t before = 0, tot = 0;
-        for (int x = 0; x < 4; ++x) {
-            before += x < w ? ws[x] : 0;
-            tot += ws[x];
-        }
-        if (f < f1) {
-            pre[f] = carry + before + incl - l;
-            mark_chunks(f, carry + before + incl - l, l, kPeelChunk, cstart, ccap);
-        }
-        carry += tot;
-        __syncthreads();
-    }
-    if (f1 == nF && f0 < f1 && threadIdx.x == 0) pre[nF] = carry;
 }
 
 
@@ -996,9 +982,9 @@ int grid_items(gk_ctx *ctx, int64_t n) {
 //   dS = F * A masked by A (k_dround_stream): one support off each
 //        surviving edge of every triangle destroyed this round;
 //   F -> removed; compact the adjacency once a quarter of it is dead.
-// Rounds run in device-driven batches (sizes in a RoundCtl block): frontiers
-// below kSmallRound get the one-block prologue, larger ones the chunked
-// prologue + scan; the host checks in once per batch.
+// Rounds run in device-driven batches (sizes in the RoundCtl blocks), each
+// round a prep (marks + scan) and a stream launch; a batch stops at the end
+// of a level, and the host checks in once per batch.
 // fixed_k > 0 runs the single-k semantics instead (k_truss, src/main.cpp:93-141):
 // the threshold stays k-2, removed edges are labelled 1, and the run stops at
 // the first round that removes nothing; the returned count then includes that
@@ -1035,7 +1021,6 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
     GK_TRY(B.alloc(&hoff, m + 1));
     GK_TRY(B.alloc(&dctr, 2));
     GK_TRY(B.alloc(&dpc, 1));
-    RoundCtl *dctl = &dpc->rc;
     int32_t *dn = &dpc->n, *dnl = &dpc->nl, *dac = &dpc->ac;
     uint32_t *dmin = &dpc->mn;
     int32_t *NL, *NL2;
@@ -1051,7 +1036,7 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
     k_hash_caps<<<grid_items(ctx, m), 256, 0, ctx->stream>>>(m, A->rst, caps);
     GK_TRY(exclusive_scan_i64(ctx, caps, hoff, m + 1));
     int64_t *h64;
-    GK_TRY(pinned(ctx, 128, (void **)&h64));
+    GK_TRY(pinned(ctx, kPinnedBytes, (void **)&h64));
     GK_HIP(hipMemcpyAsync(h64, hoff + m, 8, hipMemcpyDeviceToHost, ctx->stream));
     GK_HIP(hipStreamSynchronize(ctx->stream));
     const int64_t tab_n = h64[0];
@@ -1098,9 +1083,8 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
     int k = fixed_k > 0 ? fixed_k : 3;
     int32_t kbest = nL ? 2 : 0;
     int32_t *h = (int32_t *)(h64 + 2);
-    static_assert(sizeof(PeelCtl) <= 128 - 32, "control block fits the pinned staging");
+    static_assert(sizeof(PeelCtl) <= kPinnedBytes - 32, "control block fits the pinned staging");
     PeelCtl *hpc = (PeelCtl *)(h64 + 4);
-    RoundCtl *hctl = &hpc->rc;
     // Peel-stream timing: HIP events around every kPeelTimeStride-th stream
     // launch only.  A timed event drains the queue; around every launch
     // (3,677 on R-MAT scale 20) they cost 30 ms of a 233 ms decomposition.
@@ -1142,15 +1126,17 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
     bool started = false;
     // Device-driven rounds per host check, adapted to the cascades: a batch
     // whose rounds all ran doubles the next one; a batch cut short (the
-    // level ended, or the frontier changed size class) sizes the next one
-    // from the rounds it ran, so few launches are wasted on stopped rounds.
-    int batch_small = 8, batch_big = 4;
+    // level ended) sizes the next one from the rounds it ran, so few
+    // launches are wasted on stopped rounds.
+    int batch = 8;
     auto next_batch = [](int size, int32_t ran, int cap) {
         return ran >= size ? std::min(2 * size, cap) : std::max(2, std::min<int>(ran + ran / 2 + 1, cap));
     };
-    const int nbig = (int)std::min<int64_t>(kBigBlocksMax, (int64_t)ctx->num_cus * 2);
-    int32_t *bsum;
-    GK_TRY(B.alloc(&bsum, kBigBlocksMax));
+    const int nprep = (int)std::min<int64_t>(kPrepBlocksMax, (int64_t)ctx->num_cus);
+    unsigned long long *lb;  // look-back words of the prep workgroups
+    GK_TRY(B.alloc(&lb, kPrepBlocksMax));
+    GK_HIP(hipMemsetAsync(lb, 0, 8 * kPrepBlocksMax, ctx->stream));
+    uint32_t epoch = 0;
     // Near list (NearList): alive edges with support below nl_hi, built by a
     // full scan of the alive list and kept complete by the peeling kernels.
     // While a level's threshold stays below nl_hi its frontier comes from
@@ -1217,62 +1203,42 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
         }
         const uint32_t label = fixed_k > 0 ? 1u : (uint32_t)(k - 1);
         kbest = k - 1;
-        int64_t marked = 0;  // edges marked removed by the prologues below
-        if (nF >= kSmallRound) {
-            // a batch of large rounds driven from the device
-            *hctl = RoundCtl{nP, nF, 0, fc, 0, 0, 0, 0};
-            GK_HIP(hipMemcpyAsync(dctl, hctl, sizeof(RoundCtl), hipMemcpyHostToDevice, ctx->stream));
-            for (int r = 0; r < batch_big; ++r) {
-                k_big_prologue<<<nbig, 256, 0, ctx->stream>>>(dctl, fb[0], fb[1], st, tl, label, eu, ev, sup,
-                                                              arst[cur], fy, fx, flen, bsum);
-                k_big_scan_top<<<1, kBigBlocksMax, 0, ctx->stream>>>(dctl, bsum, nbig);
-                k_big_scan_apply<<<nbig, 256, 0, ctx->stream>>>(dctl, flen, bsum, fpre, cstart, ccap);
-                GK_TRY(stream_begin());
-                k_dround_stream<kPeelU><<<gs, 256, 0, ctx->stream>>>(dctl, fb[0], fb[1], fpre, fy, fx, cstart, ccap,
-                                                                acol[cur], aeid[cur], hoff, tab, st, sup, k - 2,
-                                                                &dctl->nFn, near_list(), dctr);
-                GK_TRY(stream_end());
-                k_dround_advance<<<1, 64, 0, ctx->stream>>>(dctl);
-            }
-            GK_TRY(launch_check("device large rounds"));
-            GK_HIP(hipMemcpyAsync(hpc, dpc, sizeof(PeelCtl), hipMemcpyDeviceToHost, ctx->stream));
-            GK_HIP(hipStreamSynchronize(ctx->stream));
-            batch_big = next_batch(batch_big, hctl->rounds, 32);
-            nNL = hpc->nl;
-            rounds += hctl->rounds;
-            alive -= hctl->removed;
-            marked = (int64_t)nP + hctl->removed - hctl->nP;
-            nP = hctl->nP;
-            nF = hctl->nF;
-            fc = hctl->cur;
-        } else {
-            // a batch of small rounds driven from the device
-            *hctl = RoundCtl{nP, nF, 0, fc, 0, 0, 0, 0};
-            GK_HIP(hipMemcpyAsync(dctl, hctl, sizeof(RoundCtl), hipMemcpyHostToDevice, ctx->stream));
-            const int gsd = std::min(gs, 1024);
-            for (int r = 0; r < batch_small; ++r) {
-                k_dround_prologue<<<1, kSmallRoundThreads, 0, ctx->stream>>>(dctl, fb[0], fb[1], st, tl, label, eu,
-                                                                            ev, sup, arst[cur], fy, fx, fpre, cstart,
-                                                                            ccap);
-                GK_TRY(stream_begin());
-                k_dround_stream<kPeelU><<<gsd, 256, 0, ctx->stream>>>(dctl, fb[0], fb[1], fpre, fy, fx, cstart, ccap,
-                                                                 acol[cur], aeid[cur], hoff, tab, st, sup, k - 2,
-                                                                 &dctl->nFn, near_list(), dctr);
-                GK_TRY(stream_end());
-            }
-            GK_TRY(launch_check("device rounds"));
-            GK_HIP(hipMemcpyAsync(hpc, dpc, sizeof(PeelCtl), hipMemcpyDeviceToHost, ctx->stream));
-            GK_HIP(hipStreamSynchronize(ctx->stream));
-            if (hctl->adv) advance_round(*hctl);  // the batch's last round
-            batch_small = next_batch(batch_small, hctl->rounds, 64);
-            nNL = hpc->nl;
-            rounds += hctl->rounds;
-            alive -= hctl->removed;
-            marked = (int64_t)nP + hctl->removed - hctl->nP;
-            nP = hctl->nP;
-            nF = hctl->nF;
-            fc = hctl->cur;
+        // a batch of rounds driven from the device
+        hpc->rc[0] = RoundCtl{nP, nF, 0, fc, 0, 0, 0, 0};
+        hpc->rc[1] = RoundCtl{0, 0, 0, 0, -1, 0, 0, 0};
+        hpc->stop = 0;
+        hpc->err = 0;
+        GK_HIP(hipMemcpyAsync(dpc, hpc, offsetof(PeelCtl, n), hipMemcpyHostToDevice, ctx->stream));
+        RoundCtl *dc = dpc->rc;
+        for (int r = 0; r < batch; ++r) {
+            RoundCtl *ci = dc + (r & 1), *co = dc + ((r + 1) & 1);
+            k_round_prep<<<nprep, kPrepThreads, 0, ctx->stream>>>(ci, co, &dpc->stop, &dpc->err, lb, ++epoch, fb[0],
+                                                                  fb[1], st, tl, label, eu, ev, sup, arst[cur], fy,
+                                                                  fx, flen, fpre, cstart, ccap);
+            GK_TRY(stream_begin());
+            k_dround_stream<kPeelU><<<gs, 256, 0, ctx->stream>>>(co, &dpc->stop, fb[0], fb[1], fpre, fy, fx, cstart,
+                                                                ccap, acol[cur], aeid[cur], hoff, tab, st, sup, k - 2,
+                                                                &co->nFn, near_list(), dctr);
+            GK_TRY(stream_end());
         }
+        GK_TRY(launch_check("device rounds"));
+        GK_HIP(hipMemcpyAsync(hpc, dpc, sizeof(PeelCtl), hipMemcpyDeviceToHost, ctx->stream));
+        GK_HIP(hipStreamSynchronize(ctx->stream));
+        if (hpc->err) {
+            set_error("truss peeling: a round's prefix scan did not complete (look-back wait gave up)");
+            return GK_ESTATE;
+        }
+        RoundCtl rc = hpc->rc[hpc->rc[1].seq > hpc->rc[0].seq ? 1 : 0];
+        if (rc.adv) advance_round(rc);  // the batch's last round
+        batch = next_batch(batch, rc.rounds, 64);
+        nNL = hpc->nl;
+        rounds += rc.rounds;
+        alive -= rc.removed;
+        // edges marked removed by the batch's preps
+        const int64_t marked = (int64_t)nP + rc.removed - rc.nP;
+        nP = rc.nP;
+        nF = rc.nF;
+        fc = rc.cur;
         dead_pending += marked;
         // the alive list is compacted once half of it is gone
         if (alive > 0 && 2 * (alive - nF) < na && !(2 * (dead_pending + nP) * 4 > aal_nz)) {
