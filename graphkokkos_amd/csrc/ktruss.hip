@@ -244,8 +244,24 @@ __global__ void k_iota(int64_t n, int32_t *__restrict__ a) {
 // below keep_hi are appended there too (the near list of the next levels).
 constexpr int kScanIt = 4;   // entries per thread per pass of the list scans
 
+// Edge states, 2 bits per edge (16 per word): 0 alive, 1 frontier (removed
+// this round), 2 removed earlier.  At 2 bits the states of R-MAT scale 20's
+// 15.7M edges take 3.9 MB, about an XCD's L2, where a byte each (16 MB)
+// missed it on almost every random read of the peeling stream.
+__device__ __forceinline__ uint32_t st_get(const uint32_t *__restrict__ st, int32_t e) {
+    return (st[e >> 4] >> ((e & 15) * 2)) & 3u;
+}
+// alive -> frontier (00 -> 01) and frontier -> removed (01 -> 10); other
+// edges of the same word may change at the same time, hence the atomics
+__device__ __forceinline__ void st_to_frontier(uint32_t *st, int32_t e) {
+    atomicOr(&st[e >> 4], 1u << ((e & 15) * 2));
+}
+__device__ __forceinline__ void st_to_removed(uint32_t *st, int32_t e) {
+    atomicXor(&st[e >> 4], 3u << ((e & 15) * 2));
+}
+
 __global__ __launch_bounds__(256) void k_frontier(int64_t na, const int32_t *__restrict__ alist,
-                                                  const uint8_t *__restrict__ st, const int32_t *__restrict__ sup,
+                                                  const uint32_t *__restrict__ st, const int32_t *__restrict__ sup,
                                                   int32_t thresh, int32_t *__restrict__ F, int32_t *__restrict__ nF,
                                                   uint32_t *__restrict__ mn, int32_t *__restrict__ keep,
                                                   int32_t *__restrict__ nkeep, int32_t keep_hi) {
@@ -268,7 +284,7 @@ __global__ __launch_bounds__(256) void k_frontier(int64_t na, const int32_t *__r
 #pragma unroll
         for (int i = 0; i < kScanIt; ++i) {
             const int64_t x = base + i * 256 + threadIdx.x;
-            if (x < na && st[e[i]] == 0) {
+            if (x < na && st_get(st, e[i]) == 0) {
                 const int32_t v = sup[e[i]];
                 lo = min(lo, (uint32_t)v);
                 take[i] = v < thresh;
@@ -354,7 +370,7 @@ __device__ __forceinline__ int32_t adj_probe(const int64_t *__restrict__ hoff, c
 // no triangle this round and its row need not be streamed.
 template <int P>
 __device__ __forceinline__ void prep_edges(const int32_t (&f)[P], int32_t nF, const int32_t *__restrict__ F,
-                                           uint8_t *__restrict__ st, uint32_t *__restrict__ truss, uint32_t label,
+                                           uint32_t *__restrict__ st, uint32_t *__restrict__ truss, uint32_t label,
                                            const int32_t *__restrict__ eu, const int32_t *__restrict__ ev,
                                            const int32_t *__restrict__ sup, const int32_t *__restrict__ rst,
                                            int32_t *__restrict__ fy, int32_t *__restrict__ frs, int32_t (&len)[P]) {
@@ -386,7 +402,7 @@ __device__ __forceinline__ void prep_edges(const int32_t (&f)[P], int32_t nF, co
     for (int j = 0; j < P; ++j) {
         len[j] = 0;
         if (e[j] < 0) continue;
-        st[e[j]] = 1;
+        st_to_frontier(st, e[j]);
         truss[e[j]] = label;
         const int32_t du = ru1[j] - ru[j], dv = rv1[j] - rv[j];
         const bool uy = du > dv || (du == dv && u[j] > v[j]);
@@ -398,7 +414,7 @@ __device__ __forceinline__ void prep_edges(const int32_t (&f)[P], int32_t nF, co
 
 // The previous frontier's edges become removed (state 2); P at a time.
 template <int P>
-__device__ __forceinline__ void mark_removed(int32_t nP, const int32_t *__restrict__ Fp, uint8_t *__restrict__ st,
+__device__ __forceinline__ void mark_removed(int32_t nP, const int32_t *__restrict__ Fp, uint32_t *__restrict__ st,
                                              int32_t first, int32_t stride) {
     for (int32_t f0 = first; f0 < nP; f0 += P * stride) {
         int32_t e[P];
@@ -406,7 +422,7 @@ __device__ __forceinline__ void mark_removed(int32_t nP, const int32_t *__restri
         for (int j = 0; j < P; ++j) e[j] = f0 + j * stride < nP ? Fp[f0 + j * stride] : -1;
 #pragma unroll
         for (int j = 0; j < P; ++j)
-            if (e[j] >= 0) st[e[j]] = 2;
+            if (e[j] >= 0) st_to_removed(st, e[j]);
     }
 }
 
@@ -443,20 +459,27 @@ constexpr int kOwnerWin = 16;
 struct RoundCtl {
     int32_t nP, nF, nFn, cur;
     int32_t seq;      // writes so far (the newer of the two blocks is the state)
-    int32_t rounds;   // rounds run by the batch
+    int32_t rounds;   // rounds run by the batch (level changes not counted)
     int32_t removed;  // frontier edges of those rounds
     int32_t adv;      // a streamed round waits for its advance (done by the next prep)
+    int32_t k;        // the level: the frontier holds the edges below k - 2
+    int32_t total;    // items the round streams (pre[nF])
+    int32_t trans;    // this slot moves to the next level instead of streaming
+    uint32_t tep;     // ... with the minimum support tagged by this prep launch
+    int32_t lab;      // the last truss label given to a frontier (0: none)
 };
 
-// Moves the control block past a streamed round: its frontier becomes the
-// previous one (to be marked removed), its crossings the next frontier.
+// Moves the control block past a streamed round (or a level change): its
+// frontier becomes the previous one (to be marked removed), its crossings
+// (or the next level's first frontier) the next frontier.
 __host__ __device__ __forceinline__ void advance_round(RoundCtl &c) {
-    c.rounds += 1;
+    c.rounds += c.trans ? 0 : 1;
     c.removed += c.nF;
     c.nP = c.nF;
     c.nF = c.nFn;
     c.cur ^= 1;
     c.adv = 0;
+    c.trans = 0;
 }
 
 // Control block of a decomposition in one allocation, so the host reads
@@ -465,6 +488,8 @@ __host__ __device__ __forceinline__ void advance_round(RoundCtl &c) {
 // alive-list compaction).
 struct PeelCtl {
     RoundCtl rc[2];
+    unsigned long long lvmin;  // level change: prep epoch << 32 | ~(minimum support)
+    int32_t knext;             // level change: the next level
     int32_t stop;  // the batch has stopped: its remaining kernels return
     int32_t err;   // a prep block's look-back wait gave up (no result)
     int32_t n;
@@ -560,13 +585,13 @@ __device__ __forceinline__ void wave_append(const int32_t (&v)[N], int32_t *__re
 }
 
 template <int U>
-__device__ __forceinline__ void k_peel_body(int32_t nF, const int32_t *__restrict__ pre,
+__device__ __forceinline__ void k_peel_body(int32_t nF, int32_t total, const int32_t *__restrict__ pre,
                                             const int32_t *__restrict__ cstart, int32_t ccap,
                                             const int32_t *__restrict__ F,
                                             const int32_t *__restrict__ fy, const int32_t *__restrict__ frs,
                                             const int32_t *__restrict__ col, const int32_t *__restrict__ eid,
                                             const int64_t *__restrict__ hoff, const int2 *__restrict__ tab,
-                                            const uint8_t *__restrict__ st, int32_t *__restrict__ sup,
+                                            const uint32_t *__restrict__ st, int32_t *__restrict__ sup,
                                             int32_t thresh, int32_t *__restrict__ Fn, int32_t *__restrict__ nFn,
                                             const NearList nl, unsigned long long *__restrict__ ctr) {
     constexpr int CH = 64 * U;
@@ -583,7 +608,6 @@ __device__ __forceinline__ void k_peel_body(int32_t nF, const int32_t *__restric
     const int lane = threadIdx.x & 63;
     const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
-    const int32_t total = pre[nF];
     uint32_t cnt = 0;
     // Each step resolves U items per lane in stages, every stage's loads
     // issued for all U items before any is used: owner edge -> row entry
@@ -680,7 +704,7 @@ __device__ __forceinline__ void k_peel_body(int32_t nF, const int32_t *__restric
             slot[u] = 0;
             kv[u] = make_int2(-1, -1);
             if (t[u] >= 0) {
-                s2[u] = st[b[u]];
+                s2[u] = (uint8_t)st_get(st, b[u]);
                 const int lg = 31 - __clz((int)cap[u]);
                 slot[u] = dev::hash32((uint32_t)w[u]) >> (32 - lg);
                 kv[u] = tab[h0[u] + slot[u]];
@@ -700,7 +724,7 @@ __device__ __forceinline__ void k_peel_body(int32_t nF, const int32_t *__restric
         }
         uint8_t s1[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) s1[u] = a[u] >= 0 ? st[a[u]] : 2;
+        for (int u = 0; u < U; ++u) s1[u] = a[u] >= 0 ? (uint8_t)st_get(st, a[u]) : 2;
         // edges leaving the threshold / entering the near list (-1: none)
         int32_t xf[2 * U], xn[2 * U];
 #pragma unroll
@@ -750,9 +774,17 @@ constexpr int kPeelChunk = 64 * kPeelU;
 // lengths) and scans the lengths into pre in the same launch: each workgroup
 // owns a contiguous piece of the frontier and finds the items before it by
 // decoupled look-back over the workgroups before it (workgroups are
-// dispatched in index order, and each waits only on lower ones).  An empty
-// frontier (the level is finished, or nothing was removed) sets the stop
-// word: every later kernel of the batch returns at once.
+// dispatched in index order, and each waits only on lower ones).
+//
+// An empty frontier ends a level.  In a decomposition whose near list is
+// valid, the slot changes level on the device: when a level ends every
+// alive edge has support >= k - 2, so the next level k' = max(k + 1, s + 3)
+// (s = the minimum support over the alive edges) starts from the alive
+// edges below k' - 2 = s + 1, i.e. those at support s -- all of them in the
+// near list, whose alive entries are exactly the edges below its bound.  The
+// prep takes s over the near list, the stream collects the edges at s as
+// the next frontier.  Otherwise (single k, no valid near list, or a near
+// list with no alive entry) the stop word ends the batch and the host scans.
 constexpr int kPrepThreads = 1024;
 // pinned staging of a decomposition: the tables' size, scan counters and a
 // PeelCtl copy
@@ -771,32 +803,74 @@ __device__ __forceinline__ unsigned long long lb_word(uint32_t epoch, bool incl,
     return ((unsigned long long)epoch << 33) | ((unsigned long long)incl << 32) | v;
 }
 
+// What a batch's kernels share besides the control blocks.
+struct PeelDev {
+    int32_t *stop, *err;
+    unsigned long long *lvmin;
+    int32_t *knext;
+    unsigned long long *lb;  // look-back words, one per prep workgroup
+    const int32_t *nl;       // near list (entries *nl_n) for level changes
+    const int32_t *nl_n;
+    int32_t levels;          // the device may change level
+    uint32_t fixed_label;    // single k: every removed edge gets this label (else k - 1)
+};
+
 __global__ __launch_bounds__(kPrepThreads) void k_round_prep(
-    const RoundCtl *__restrict__ ci, RoundCtl *__restrict__ co, int32_t *__restrict__ stopw,
-    int32_t *__restrict__ err, unsigned long long *__restrict__ lb, uint32_t epoch, int32_t *__restrict__ buf0,
-    int32_t *__restrict__ buf1, uint8_t *__restrict__ st, uint32_t *__restrict__ truss, uint32_t label,
+    const RoundCtl *__restrict__ ci, RoundCtl *__restrict__ co, const PeelDev D, uint32_t epoch,
+    int32_t *__restrict__ buf0, int32_t *__restrict__ buf1, uint32_t *__restrict__ st, uint32_t *__restrict__ truss,
     const int32_t *__restrict__ eu, const int32_t *__restrict__ ev, const int32_t *__restrict__ sup,
     const int32_t *__restrict__ rst, int32_t *__restrict__ fy, int32_t *__restrict__ fx,
     int32_t *__restrict__ len, int32_t *__restrict__ pre, int32_t *__restrict__ cstart, int32_t ccap) {
     constexpr int NW = kPrepThreads / 64;
     __shared__ int32_t ws[NW];
+    __shared__ uint32_t wlo[NW];
     __shared__ int32_t excl_s;
-    if (*stopw) return;
+    if (*D.stop) return;
     RoundCtl c = *ci;
-    if (c.adv) advance_round(c);
+    if (c.adv) {
+        const bool lev = c.trans && c.nFn > 0;  // the slot before collected a new level's frontier
+        advance_round(c);
+        if (lev) c.k = *D.knext;
+    }
     c.seq += 1;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int32_t nF = c.nF, nP = c.nP;
-    if (nF == 0) {
-        if (blockIdx.x == 0 && tid == 0) {
-            *co = c;
-            *stopw = 1;
-        }
-        return;
-    }
     const int32_t *F = c.cur ? buf1 : buf0;
     const int32_t *Fp = c.cur ? buf0 : buf1;
     mark_removed<4>(nP, Fp, st, blockIdx.x * kPrepThreads + tid, gridDim.x * kPrepThreads);
+    c.nP = 0;
+    if (nF == 0) {
+        if (D.levels) {
+            // minimum support over the near list's alive entries
+            uint32_t lo = 0xffffffffu;
+            const int32_t nn = *D.nl_n;
+            for (int32_t x = blockIdx.x * kPrepThreads + tid; x < nn; x += gridDim.x * kPrepThreads) {
+                const int32_t e = D.nl[x];
+                if (st_get(st, e) == 0) lo = min(lo, (uint32_t)sup[e]);
+            }
+            lo = dev::wave_max(~lo);
+            if (lane == 0) wlo[w] = lo;
+            __syncthreads();
+            if (tid == 0) {
+                uint32_t m = 0;
+                for (int x = 0; x < NW; ++x) m = max(m, wlo[x]);
+                if (m) atomicMax(D.lvmin, ((unsigned long long)epoch << 32) | m);
+            }
+            if (blockIdx.x == 0 && tid == 0) {
+                c.trans = 1;
+                c.tep = epoch;
+                c.nFn = 0;
+                c.adv = 1;
+                c.total = 0;
+                *co = c;
+            }
+        } else if (blockIdx.x == 0 && tid == 0) {
+            *co = c;
+            *D.stop = 1;
+        }
+        return;
+    }
+    const uint32_t label = D.fixed_label ? D.fixed_label : (uint32_t)(c.k - 1);
     const int32_t chunk = (nF + gridDim.x - 1) / gridDim.x;
     const int32_t f0 = min(nF, (int32_t)blockIdx.x * chunk), f1 = min(nF, f0 + chunk);
     // pass 1: marks, labels, streamed rows and their lengths; the piece's sum
@@ -823,7 +897,7 @@ __global__ __launch_bounds__(kPrepThreads) void k_round_prep(
         for (int x = 0; x < NW; ++x) agg += ws[x];
         uint32_t excl = 0;
         const int32_t bi = blockIdx.x;
-        if (lane == 0) __hip_atomic_store(&lb[bi], lb_word(epoch, bi == 0, (uint32_t)agg), __ATOMIC_RELAXED,
+        if (lane == 0) __hip_atomic_store(&D.lb[bi], lb_word(epoch, bi == 0, (uint32_t)agg), __ATOMIC_RELAXED,
                                           __HIP_MEMORY_SCOPE_AGENT);
         int32_t look = bi - 1;  // highest workgroup not yet added
         uint32_t spins = 0;
@@ -831,7 +905,8 @@ __global__ __launch_bounds__(kPrepThreads) void k_round_prep(
             const int32_t p = look - lane;
             // workgroups below 0 read as an inclusive zero, so a window
             // always ends at workgroup 0 (whose word is inclusive)
-            const unsigned long long v = p >= 0 ? __hip_atomic_load(&lb[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+            const unsigned long long v = p >= 0 ? __hip_atomic_load(&D.lb[p], __ATOMIC_RELAXED,
+                                                                    __HIP_MEMORY_SCOPE_AGENT)
                                                 : lb_word(epoch, true, 0);
             const bool ready = (uint32_t)(v >> 33) == epoch;
             const uint64_t incl = __ballot(ready && ((v >> 32) & 1ull));
@@ -839,7 +914,7 @@ __global__ __launch_bounds__(kPrepThreads) void k_round_prep(
             const uint64_t need = first >= 63 ? ~0ull : ((2ull << first) - 1ull);
             if (__ballot(!ready) & need) {
                 if (++spins > kLookbackSpins) {
-                    if (lane == 0) atomicExch(err, 1);
+                    if (lane == 0) atomicExch(D.err, 1);
                     break;
                 }
                 __builtin_amdgcn_s_sleep(1);
@@ -850,7 +925,7 @@ __global__ __launch_bounds__(kPrepThreads) void k_round_prep(
             look -= 64;
         }
         if (bi > 0 && lane == 0)
-            __hip_atomic_store(&lb[bi], lb_word(epoch, true, excl + (uint32_t)agg), __ATOMIC_RELAXED,
+            __hip_atomic_store(&D.lb[bi], lb_word(epoch, true, excl + (uint32_t)agg), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
         if (lane == 0) excl_s = (int32_t)excl;
     }
@@ -879,46 +954,83 @@ __global__ __launch_bounds__(kPrepThreads) void k_round_prep(
         carry += tot;
         __syncthreads();
     }
-    if (f1 == nF && f0 < f1 && tid == 0) pre[nF] = carry;
-    if (blockIdx.x == 0 && tid == 0) {
+    // the workgroup holding the frontier's end knows the round's items and
+    // writes the control block for the stream
+    if (f1 == nF && f0 < f1 && tid == 0) {
+        pre[nF] = carry;
+        c.total = carry;
         c.nFn = 0;
         c.adv = 1;  // the stream kernel that follows runs this round
+        c.lab = (int32_t)label;
         *co = c;
     }
 }
 
 template <int U>
-__global__ __launch_bounds__(256) void k_dround_stream(const RoundCtl *__restrict__ ctl,
-                                                       const int32_t *__restrict__ stopw, int32_t *__restrict__ buf0,
-                                                       int32_t *__restrict__ buf1, const int32_t *__restrict__ pre,
+__global__ __launch_bounds__(256) void k_dround_stream(const RoundCtl *__restrict__ ctl, const PeelDev D,
+                                                       int32_t *__restrict__ buf0, int32_t *__restrict__ buf1,
+                                                       const int32_t *__restrict__ pre,
                                                        const int32_t *__restrict__ fy, const int32_t *__restrict__ fx,
                                                        const int32_t *__restrict__ cstart, int32_t ccap,
                                                        const int32_t *__restrict__ col,
                                                        const int32_t *__restrict__ eid,
                                                        const int64_t *__restrict__ hoff, const int2 *__restrict__ tab,
-                                                       const uint8_t *__restrict__ st, int32_t *__restrict__ sup,
-                                                       int32_t thresh, int32_t *__restrict__ nFn,
-                                                       const NearList nl, unsigned long long *__restrict__ ctr) {
-    if (*stopw) return;
-    const int32_t cur = ctl->cur;
+                                                       const uint32_t *__restrict__ st, int32_t *__restrict__ sup,
+                                                       int32_t *__restrict__ nFn, const NearList nl,
+                                                       unsigned long long *__restrict__ ctr) {
+    if (*D.stop) return;
+    const RoundCtl c = *ctl;
+    int32_t *Fn = c.cur ? buf0 : buf1;
+    if (c.trans) {
+        // level change: the alive near-list edges at the minimum support s
+        // (tagged by the prep of this slot) are the next level's frontier
+        const unsigned long long wv = *D.lvmin;
+        if ((uint32_t)(wv >> 32) != c.tep) {  // no alive entry: the host scans
+            if (blockIdx.x == 0 && threadIdx.x == 0) *D.stop = 1;
+            return;
+        }
+        const int32_t s = (int32_t)~(uint32_t)wv;
+        if (blockIdx.x == 0 && threadIdx.x == 0) *D.knext = max(c.k + 1, s + 3);
+        __shared__ int32_t lds[256 / 64 + 1];
+        const int32_t nn = *D.nl_n;
+        constexpr int CH = 256 * kScanIt;
+        for (int32_t base = blockIdx.x * CH; base < nn; base += gridDim.x * CH) {
+            bool take[kScanIt];
+            int32_t e[kScanIt];
+#pragma unroll
+            for (int i = 0; i < kScanIt; ++i) {
+                const int32_t x = base + i * 256 + (int32_t)threadIdx.x;
+                e[i] = x < nn ? D.nl[x] : 0;
+            }
+            int any = 0;
+#pragma unroll
+            for (int i = 0; i < kScanIt; ++i) {
+                const int32_t x = base + i * 256 + (int32_t)threadIdx.x;
+                take[i] = x < nn && st_get(st, e[i]) == 0 && sup[e[i]] == s;
+                any |= take[i];
+            }
+            if (__syncthreads_or(any)) dev::block_append_n<256, kScanIt>(take, e, Fn, nFn, lds);
+        }
+        return;
+    }
     // the crossings go to buf[cur ^ 1], counted in ctl->nFn
-    k_peel_body<U>(ctl->nF, pre, cstart, ccap, cur ? buf1 : buf0, fy, fx, col, eid, hoff, tab, st, sup, thresh,
-                   cur ? buf0 : buf1, nFn, nl, ctr);
+    k_peel_body<U>(c.nF, c.total, pre, cstart, ccap, c.cur ? buf1 : buf0, fy, fx, col, eid, hoff, tab, st, sup,
+                   c.k - 2, Fn, nFn, nl, ctr);
 }
 
 
-__global__ void k_mark_dead(int32_t nF, const int32_t *__restrict__ F, uint8_t *__restrict__ st) {
-    for (int32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < nF; f += gridDim.x * blockDim.x) st[F[f]] = 2;
+__global__ void k_mark_dead(int32_t nF, const int32_t *__restrict__ F, uint32_t *__restrict__ st) {
+    for (int32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < nF; f += gridDim.x * blockDim.x) st_to_removed(st, F[f]);
 }
 
 // Compaction of the alive adjacency (rows stay sorted).
 __global__ void k_alive_count(int64_t m, const int32_t *__restrict__ rst, const int32_t *__restrict__ eid,
-                              const uint8_t *__restrict__ st, int32_t *__restrict__ cnt) {
+                              const uint32_t *__restrict__ st, int32_t *__restrict__ cnt) {
     const int lane = threadIdx.x & 63;
     const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
     for (int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < m; i += waves) {
         int32_t c = 0;
-        for (int32_t p = rst[i] + lane; p < rst[i + 1]; p += 64) c += st[eid[p]] != 2;
+        for (int32_t p = rst[i] + lane; p < rst[i + 1]; p += 64) c += st_get(st, eid[p]) != 2;
         c = dev::wave_sum(c);
         if (lane == 0) cnt[i] = c;
     }
@@ -926,7 +1038,7 @@ __global__ void k_alive_count(int64_t m, const int32_t *__restrict__ rst, const 
 }
 
 __global__ void k_alive_scatter(int64_t m, const int32_t *__restrict__ rst, const int32_t *__restrict__ col,
-                                const int32_t *__restrict__ eid, const uint8_t *__restrict__ st,
+                                const int32_t *__restrict__ eid, const uint32_t *__restrict__ st,
                                 const int32_t *__restrict__ nrst, int32_t *__restrict__ ncol,
                                 int32_t *__restrict__ neid) {
     const int lane = threadIdx.x & 63;
@@ -935,7 +1047,7 @@ __global__ void k_alive_scatter(int64_t m, const int32_t *__restrict__ rst, cons
         int32_t out = nrst[i];
         for (int32_t base = rst[i]; base < rst[i + 1]; base += 64) {
             int32_t p = base + lane;
-            bool k = p < rst[i + 1] && st[eid[p]] != 2;
+            bool k = p < rst[i + 1] && st_get(st, eid[p]) != 2;
             uint64_t bal = __ballot(k);
             if (k) {
                 int32_t o = out + __popcll(bal & ((1ull << lane) - 1));
@@ -948,7 +1060,7 @@ __global__ void k_alive_scatter(int64_t m, const int32_t *__restrict__ rst, cons
 }
 
 __global__ __launch_bounds__(256) void k_alist_compact(int64_t na, const int32_t *__restrict__ alist,
-                                                       const uint8_t *__restrict__ st, int32_t *__restrict__ out,
+                                                       const uint32_t *__restrict__ st, int32_t *__restrict__ out,
                                                        int32_t *__restrict__ nout) {
     __shared__ int32_t lds[256 / 64 + 1];
     constexpr int CH = 256 * kScanIt;
@@ -962,7 +1074,7 @@ __global__ __launch_bounds__(256) void k_alist_compact(int64_t na, const int32_t
             e[i] = x < na ? alist[x] : 0;
         }
 #pragma unroll
-        for (int i = 0; i < kScanIt; ++i) k[i] = base + i * 256 + threadIdx.x < na && st[e[i]] == 0;
+        for (int i = 0; i < kScanIt; ++i) k[i] = base + i * 256 + threadIdx.x < na && st_get(st, e[i]) == 0;
         dev::block_append_n<256, kScanIt>(k, e, out, nout, lds);
     }
 }
@@ -996,7 +1108,7 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
     int32_t *arst[2], *acol[2], *aeid[2];
     int64_t *caps, *hoff;
     int2 *tab;
-    uint8_t *st;
+    uint32_t *st;  // edge states, 2 bits each
     unsigned long long *dctr;  // [0] items streamed, [1] triangles destroyed
     PeelCtl *dpc;
     GK_TRY(B.alloc(&sup, nL));
@@ -1015,7 +1127,8 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
     const int32_t ccap = (int32_t)std::min<int64_t>(A->nz / kPeelChunk + 2, INT32_MAX);
     int32_t *cstart;
     GK_TRY(B.alloc(&cstart, ccap));
-    GK_TRY(B.alloc(&st, nL));
+    const int64_t st_words = (nL + 15) / 16;
+    GK_TRY(B.alloc(&st, st_words));
     GK_TRY(B.alloc(&cnt, m + 1));
     GK_TRY(B.alloc(&caps, m + 1));
     GK_TRY(B.alloc(&hoff, m + 1));
@@ -1059,7 +1172,7 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
         if (sym_eid_out)
             GK_HIP(hipMemcpyAsync(*sym_eid_out, aeid[0], 4 * (size_t)A->nz, hipMemcpyDeviceToDevice, s));
         k_iota<<<grid_items(ctx, nL), 256, 0, s>>>(nL, alist);
-        GK_HIP(hipMemsetAsync(st, 0, nL, s));
+        GK_HIP(hipMemsetAsync(st, 0, 4 * (size_t)std::max<int64_t>(st_words, 1), s));
         GK_HIP(hipMemsetAsync(dctr, 0, 16, s));
         GK_HIP(hipMemsetAsync(tab, 0xff, sizeof(int2) * (size_t)std::max<int64_t>(tab_n, 1), s));
         k_hash_fill<<<gw, 256, 0, s>>>(m, A->rst, A->col, aeid[0], hoff, tab);
@@ -1201,24 +1314,26 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
             scan_needed = true;
             continue;
         }
-        const uint32_t label = fixed_k > 0 ? 1u : (uint32_t)(k - 1);
-        kbest = k - 1;
         // a batch of rounds driven from the device
-        hpc->rc[0] = RoundCtl{nP, nF, 0, fc, 0, 0, 0, 0};
-        hpc->rc[1] = RoundCtl{0, 0, 0, 0, -1, 0, 0, 0};
+        hpc->rc[0] = RoundCtl{nP, nF, 0, fc, 0, 0, 0, 0, k, 0, 0, 0, 0};
+        hpc->rc[1] = RoundCtl{};
+        hpc->rc[1].seq = -1;
+        hpc->lvmin = 0;
+        hpc->knext = 0;
         hpc->stop = 0;
         hpc->err = 0;
         GK_HIP(hipMemcpyAsync(dpc, hpc, offsetof(PeelCtl, n), hipMemcpyHostToDevice, ctx->stream));
+        const PeelDev D{&dpc->stop, &dpc->err, &dpc->lvmin, &dpc->knext, lb, NL, dnl,
+                        fixed_k == 0 && nl_valid ? 1 : 0, fixed_k > 0 ? 1u : 0u};
         RoundCtl *dc = dpc->rc;
         for (int r = 0; r < batch; ++r) {
             RoundCtl *ci = dc + (r & 1), *co = dc + ((r + 1) & 1);
-            k_round_prep<<<nprep, kPrepThreads, 0, ctx->stream>>>(ci, co, &dpc->stop, &dpc->err, lb, ++epoch, fb[0],
-                                                                  fb[1], st, tl, label, eu, ev, sup, arst[cur], fy,
-                                                                  fx, flen, fpre, cstart, ccap);
+            k_round_prep<<<nprep, kPrepThreads, 0, ctx->stream>>>(ci, co, D, ++epoch, fb[0], fb[1], st, tl, eu, ev,
+                                                                  sup, arst[cur], fy, fx, flen, fpre, cstart, ccap);
             GK_TRY(stream_begin());
-            k_dround_stream<kPeelU><<<gs, 256, 0, ctx->stream>>>(co, &dpc->stop, fb[0], fb[1], fpre, fy, fx, cstart,
-                                                                ccap, acol[cur], aeid[cur], hoff, tab, st, sup, k - 2,
-                                                                &co->nFn, near_list(), dctr);
+            k_dround_stream<kPeelU><<<gs, 256, 0, ctx->stream>>>(co, D, fb[0], fb[1], fpre, fy, fx, cstart, ccap,
+                                                                acol[cur], aeid[cur], hoff, tab, st, sup, &co->nFn,
+                                                                near_list(), dctr);
             GK_TRY(stream_end());
         }
         GK_TRY(launch_check("device rounds"));
@@ -1229,8 +1344,16 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
             return GK_ESTATE;
         }
         RoundCtl rc = hpc->rc[hpc->rc[1].seq > hpc->rc[0].seq ? 1 : 0];
-        if (rc.adv) advance_round(rc);  // the batch's last round
-        batch = next_batch(batch, rc.rounds, 64);
+        // slots (rounds and level changes) that ran to the end
+        batch = next_batch(batch, hpc->stop ? rc.seq - 1 : batch, 64);
+        if (rc.adv) {  // the batch's last slot
+            const bool lev = rc.trans && rc.nFn > 0;
+            advance_round(rc);
+            if (lev) rc.k = hpc->knext;
+        }
+        k = rc.k;
+        if (fixed_k > 0) kbest = k - 1;
+        else if (rc.lab > 0) kbest = rc.lab;
         nNL = hpc->nl;
         rounds += rc.rounds;
         alive -= rc.removed;
