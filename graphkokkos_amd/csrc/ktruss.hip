@@ -1397,7 +1397,7 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
     }
     // peel-stream figures: device time over all rounds, algorithmic bytes
     // (per streamed entry: column and edge id 8 B, a table probe 8 B, two
-    // states 2 B; per destroyed triangle: two 4-byte support updates)
+    // 2-bit states 0.5 B; per destroyed triangle: two 4-byte support updates)
     unsigned long long hc[2];
     GK_HIP(d2h(ctx, hc, dctr, sizeof hc));
     double sms = 0;
@@ -1409,10 +1409,10 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
     if (ev_used) sms *= (double)stream_launches / (double)(ev_used / 2);
     gk_timing &tt = ctx->last;
     tt.acc_ms[GK_KERNEL_PEEL] += sms;
-    tt.acc_bytes[GK_KERNEL_PEEL] += 18.0 * (double)hc[0] + 8.0 * (double)hc[1];
+    tt.acc_bytes[GK_KERNEL_PEEL] += 16.5 * (double)hc[0] + 8.0 * (double)hc[1];
     tt.acc_launches[GK_KERNEL_PEEL] += rounds;
     tt.kernel_ms += sms;
-    tt.bytes_alg += 18.0 * (double)hc[0] + 8.0 * (double)hc[1];
+    tt.bytes_alg += 16.5 * (double)hc[0] + 8.0 * (double)hc[1];
     if (rounds_out) *rounds_out = fixed_k > 0 ? rounds + 1 : rounds;
     if (kmax_out) *kmax_out = kbest;
     return GK_OK;
