@@ -308,17 +308,21 @@ __global__ __launch_bounds__(256) void k_frontier(int64_t na, const int32_t *__r
 
 // ---- static adjacency hash for the peeling rounds ------------------------
 // Vertex v's adjacency gets an open-addressing table of cap(v) = the
-// smallest power of two >= 2 deg(v) slots, (neighbour, edge id) per slot,
-// built once per decomposition.  The adjacency only loses entries while
-// peeling, so the tables stay valid: a removed edge is still found and then
-// rejected by its state.
+// smallest power of two >= max(8, 2 deg(v)) slots, (neighbour, edge id) per
+// slot, built once per decomposition, and in front of it a bit filter of
+// 4 cap(v) bits (a neighbour w sets bit hash(w) >> (32 - lg(4 cap))): most
+// probes of the peeling stream look for a w that is not a neighbour, and a
+// clear bit answers them from a 4-byte word in an array a sixteenth of the
+// tables' size (45 MB against 700 MB at R-MAT scale 20) instead of a table
+// slot.  The adjacency only loses entries while peeling, so the tables stay
+// valid: a removed edge is still found and then rejected by its state.
 
 __global__ void k_hash_caps(int64_t m, const int32_t *__restrict__ rst, int64_t *__restrict__ cap) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
         int32_t d = rst[i + 1] - rst[i];
         int32_t c = 0;
         if (d > 0) {
-            c = 2;
+            c = 8;  // so that every table offset is a multiple of 8 (filter words at offset / 8)
             while (c < 2 * d) c <<= 1;
         }
         cap[i] = c;
@@ -326,9 +330,14 @@ __global__ void k_hash_caps(int64_t m, const int32_t *__restrict__ rst, int64_t 
     if (blockIdx.x == 0 && threadIdx.x == 0) cap[m] = 0;
 }
 
+// Bit of neighbour key hash hh in the filter of a table of 2^lg slots.
+__device__ __forceinline__ uint32_t filter_bit(uint32_t hh, int lg) {
+    return lg + 2 >= 32 ? hh : hh >> (30 - lg);
+}
+
 __global__ void k_hash_fill(int64_t m, const int32_t *__restrict__ rst, const int32_t *__restrict__ col,
                             const int32_t *__restrict__ eid, const int64_t *__restrict__ hoff,
-                            int2 *__restrict__ tab) {
+                            int2 *__restrict__ tab, uint32_t *__restrict__ flt) {
     const int lane = threadIdx.x & 63;
     const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
     for (int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < m; i += waves) {
@@ -338,9 +347,12 @@ __global__ void k_hash_fill(int64_t m, const int32_t *__restrict__ rst, const in
         const int lg = 31 - __clz((int)c);
         for (int32_t p = rst[i] + lane; p < rst[i + 1]; p += 64) {
             const int32_t w = col[p];
-            uint32_t s = dev::hash32((uint32_t)w) >> (32 - lg);
+            const uint32_t hh = dev::hash32((uint32_t)w);
+            uint32_t s = hh >> (32 - lg);
             while (atomicCAS(&tab[h0 + s].x, -1, w) != -1) s = (s + 1) & (c - 1);
             tab[h0 + s].y = eid[p];
+            const uint32_t fb = filter_bit(hh, lg);
+            atomicOr(&flt[(h0 >> 3) + (fb >> 5)], 1u << (fb & 31));
         }
     }
 }
@@ -591,6 +603,7 @@ __device__ __forceinline__ void k_peel_body(int32_t nF, int32_t total, const int
                                             const int32_t *__restrict__ fy, const int32_t *__restrict__ frs,
                                             const int32_t *__restrict__ col, const int32_t *__restrict__ eid,
                                             const int64_t *__restrict__ hoff, const int2 *__restrict__ tab,
+                                            const uint32_t *__restrict__ flt,
                                             const uint32_t *__restrict__ st, int32_t *__restrict__ sup,
                                             int32_t thresh, int32_t *__restrict__ Fn, int32_t *__restrict__ nFn,
                                             const NearList nl, unsigned long long *__restrict__ ctr) {
@@ -696,25 +709,35 @@ __device__ __forceinline__ void k_peel_body(int32_t nF, int32_t total, const int
             }
         }
         uint8_t s2[U];
-        uint32_t slot[U];
-        int2 kv[U];
+        uint32_t slot[U], fw[U], fbit[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             s2[u] = 2;
-            slot[u] = 0;
-            kv[u] = make_int2(-1, -1);
+            slot[u] = fw[u] = fbit[u] = 0;
             if (t[u] >= 0) {
                 s2[u] = (uint8_t)st_get(st, b[u]);
                 const int lg = 31 - __clz((int)cap[u]);
-                slot[u] = dev::hash32((uint32_t)w[u]) >> (32 - lg);
-                kv[u] = tab[h0[u] + slot[u]];
+                const uint32_t hh = dev::hash32((uint32_t)w[u]);
+                slot[u] = hh >> (32 - lg);
+                const uint32_t fb = filter_bit(hh, lg);
+                fbit[u] = fb & 31;
+                fw[u] = flt[(h0[u] >> 3) + (fb >> 5)];
             }
+        }
+        // y's table is read only where the filter bit of w is set (w may be
+        // a neighbour of y) and the streamed edge can close a triangle
+        bool look[U];
+        int2 kv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            look[u] = t[u] >= 0 && s2[u] != 2 && !(s2[u] == 1 && b[u] < fe[u]) && ((fw[u] >> fbit[u]) & 1u);
+            kv[u] = look[u] ? tab[h0[u] + slot[u]] : make_int2(-1, -1);
         }
         int32_t a[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             a[u] = -1;
-            if (t[u] < 0 || s2[u] == 2 || (s2[u] == 1 && b[u] < fe[u])) continue;
+            if (!look[u]) continue;
             for (;;) {  // y's table: (neighbour, edge id), -1 marks an empty slot
                 if (kv[u].x == w[u]) { a[u] = kv[u].y; break; }
                 if (kv[u].x < 0) break;
@@ -975,6 +998,7 @@ __global__ __launch_bounds__(256) void k_dround_stream(const RoundCtl *__restric
                                                        const int32_t *__restrict__ col,
                                                        const int32_t *__restrict__ eid,
                                                        const int64_t *__restrict__ hoff, const int2 *__restrict__ tab,
+                                                       const uint32_t *__restrict__ flt,
                                                        const uint32_t *__restrict__ st, int32_t *__restrict__ sup,
                                                        int32_t *__restrict__ nFn, const NearList nl,
                                                        unsigned long long *__restrict__ ctr) {
@@ -1014,8 +1038,8 @@ __global__ __launch_bounds__(256) void k_dround_stream(const RoundCtl *__restric
         return;
     }
     // the crossings go to buf[cur ^ 1], counted in ctl->nFn
-    k_peel_body<U>(c.nF, c.total, pre, cstart, ccap, c.cur ? buf1 : buf0, fy, fx, col, eid, hoff, tab, st, sup,
-                   c.k - 2, Fn, nFn, nl, ctr);
+    k_peel_body<U>(c.nF, c.total, pre, cstart, ccap, c.cur ? buf1 : buf0, fy, fx, col, eid, hoff, tab, flt, st,
+                   sup, c.k - 2, Fn, nFn, nl, ctr);
 }
 
 
@@ -1154,6 +1178,8 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
     GK_HIP(hipStreamSynchronize(ctx->stream));
     const int64_t tab_n = h64[0];
     GK_TRY(B.alloc(&tab, tab_n));
+    uint32_t *tflt;  // the tables' bit filters: tab_n / 8 words (every cap is a multiple of 8)
+    GK_TRY(B.alloc(&tflt, tab_n / 8));
     if (sym_eid_out) GK_TRY(B.alloc(sym_eid_out, A->nz));
     // The set-up that does not need the supports (edge ends, the adjacency
     // with edge ids, the tables, the lists) runs on the side stream, beside
@@ -1175,7 +1201,8 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
         GK_HIP(hipMemsetAsync(st, 0, 4 * (size_t)std::max<int64_t>(st_words, 1), s));
         GK_HIP(hipMemsetAsync(dctr, 0, 16, s));
         GK_HIP(hipMemsetAsync(tab, 0xff, sizeof(int2) * (size_t)std::max<int64_t>(tab_n, 1), s));
-        k_hash_fill<<<gw, 256, 0, s>>>(m, A->rst, A->col, aeid[0], hoff, tab);
+        GK_HIP(hipMemsetAsync(tflt, 0, 4 * (size_t)std::max<int64_t>(tab_n / 8, 1), s));
+        k_hash_fill<<<gw, 256, 0, s>>>(m, A->rst, A->col, aeid[0], hoff, tab, tflt);
         GK_TRY(launch_check("truss_peel setup"));
         GK_HIP(hipEventRecord(ctx->side_join, s));
     }
@@ -1332,7 +1359,8 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
                                                                   sup, arst[cur], fy, fx, flen, fpre, cstart, ccap);
             GK_TRY(stream_begin());
             k_dround_stream<kPeelU><<<gs, 256, 0, ctx->stream>>>(co, D, fb[0], fb[1], fpre, fy, fx, cstart, ccap,
-                                                                acol[cur], aeid[cur], hoff, tab, st, sup, &co->nFn,
+                                                                acol[cur], aeid[cur], hoff, tab, tflt, st, sup,
+                                                                &co->nFn,
                                                                 near_list(), dctr);
             GK_TRY(stream_end());
         }
