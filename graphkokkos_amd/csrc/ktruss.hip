@@ -848,8 +848,12 @@ __global__ __launch_bounds__(kPrepThreads) void k_round_prep(
     __shared__ int32_t ws[NW];
     __shared__ uint32_t wlo[NW];
     __shared__ int32_t excl_s;
-    if (*D.stop) return;
+    // the stop word and the control block are loaded together (one round
+    // trip; the test reads both, and nF is never negative); a stopped
+    // batch's control block is still a valid one
+    const int32_t stopped = *D.stop;
     RoundCtl c = *ci;
+    if (stopped | (c.nF < 0)) return;
     if (c.adv) {
         const bool lev = c.trans && c.nFn > 0;  // the slot before collected a new level's frontier
         advance_round(c);
@@ -1002,8 +1006,9 @@ __global__ __launch_bounds__(256) void k_dround_stream(const RoundCtl *__restric
                                                        const uint32_t *__restrict__ st, int32_t *__restrict__ sup,
                                                        int32_t *__restrict__ nFn, const NearList nl,
                                                        unsigned long long *__restrict__ ctr) {
-    if (*D.stop) return;
+    const int32_t stopped = *D.stop;
     const RoundCtl c = *ctl;
+    if (stopped | (c.nF < 0)) return;  // both loads at once, as in k_round_prep
     int32_t *Fn = c.cur ? buf0 : buf1;
     if (c.trans) {
         // level change: the alive near-list edges at the minimum support s
