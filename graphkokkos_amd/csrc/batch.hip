@@ -85,8 +85,8 @@ __global__ __launch_bounds__(kPeelThreads) void k_peel_graphs(
     const int32_t *__restrict__ arst, const int32_t *__restrict__ acol, const int32_t *__restrict__ aeid,
     const int32_t *__restrict__ eu, const int32_t *__restrict__ ev, int32_t *__restrict__ sup,
     uint8_t *__restrict__ st, int32_t thresh, int32_t *__restrict__ buf0, int32_t *__restrict__ buf1,
-    int32_t *__restrict__ fy, int32_t *__restrict__ fx, int32_t *__restrict__ pre, int32_t *__restrict__ queue,
-    long long *__restrict__ alive_edges) {
+    int32_t *__restrict__ fy, int32_t *__restrict__ fx, int32_t *__restrict__ fh, int32_t *__restrict__ pre,
+    int32_t *__restrict__ queue, long long *__restrict__ alive_edges) {
     __shared__ int32_t lds[kPeelThreads / 64 + 1];
     __shared__ int32_t s_g, s_n;
     __shared__ int32_t s_alive[kPeelThreads / 64];
@@ -100,7 +100,9 @@ __global__ __launch_bounds__(kPeelThreads) void k_peel_graphs(
         const int32_t g = order[gi];
         const int32_t e0 = eoff[g], e1 = eoff[g + 1];
         int32_t *F = buf0 + e0, *Fn = buf1 + e0;
-        int32_t *gy = fy + e0, *gx = fx + e0, *gp = pre + e0 + g;
+        // per frontier edge f: y's row [gy[f], gh[f]) and gx[f] = the start of
+        // x's row minus the items before f (item it of f is entry gx[f] + it)
+        int32_t *gy = fy + e0, *gx = fx + e0, *gh = fh + e0, *gp = pre + e0 + g;
         if (tid == 0) s_n = 0;
         __syncthreads();
         for (int32_t base = e0; base < e1; base += kPeelThreads) {
@@ -113,20 +115,25 @@ __global__ __launch_bounds__(kPeelThreads) void k_peel_graphs(
             int32_t carry = 0;
             for (int32_t c = 0; c < nF; c += kPeelThreads) {
                 const int32_t f = c + tid;
-                int32_t len = 0;
+                int32_t len = 0, xs = 0;
                 if (f < nF) {
                     const int32_t e = F[f], u = eu[e], v = ev[e];
                     st[e] = 1;
-                    const int32_t du = arst[u + 1] - arst[u], dv = arst[v + 1] - arst[v];
+                    const int32_t ru = arst[u], ru1 = arst[u + 1], rv = arst[v], rv1 = arst[v + 1];
+                    const int32_t du = ru1 - ru, dv = rv1 - rv;
                     const bool uy = du > dv || (du == dv && u > v);
-                    gy[f] = uy ? u : v;
-                    gx[f] = uy ? v : u;
+                    gy[f] = uy ? ru : rv;
+                    gh[f] = uy ? ru1 : rv1;
+                    xs = uy ? rv : ru;
                     // an edge with no intact triangle has nothing to destroy
                     len = sup[e] > 0 ? (uy ? dv : du) : 0;
                 }
                 int32_t tot;
                 const int32_t incl = block_incl_scan<kPeelThreads>(len, lds, &tot);
-                if (f < nF) gp[f] = carry + incl - len;
+                if (f < nF) {
+                    gp[f] = carry + incl - len;
+                    gx[f] = xs - (carry + incl - len);
+                }
                 carry += tot;
             }
             if (tid == 0) {
@@ -137,9 +144,12 @@ __global__ __launch_bounds__(kPeelThreads) void k_peel_graphs(
             const int32_t total = carry;
             // Each thread takes kPeelItems consecutive items per step: one
             // search over the frontier prefix finds the owner of its first
-            // item, a cursor walks to the next owners, and since the items
-            // of one frontier edge visit x's row in increasing column order,
-            // the search for w in y's row gallops on from the last position.
+            // item, a cursor walks to the next owners; then every stage's
+            // loads are issued for all the thread's items before any is
+            // used: the entries (w, edge b) -> b's state -> the lower bounds
+            // of w in y's sorted row, one bisection step of every item at a
+            // time -> the edge a of a found w and its state -> the support
+            // updates.
             for (int32_t base = 0; base < total; base += kPeelThreads * kPeelItems) {
                 const int32_t it0 = base + tid * kPeelItems;
                 bool xk[2 * kPeelItems];
@@ -149,57 +159,94 @@ __global__ __launch_bounds__(kPeelThreads) void k_peel_graphs(
                     xk[j] = false;
                     xv[j] = 0;
                 }
+                int32_t pp[kPeelItems], ee[kPeelItems], lo[kPeelItems], hi[kPeelItems];
+#pragma unroll
+                for (int j = 0; j < kPeelItems; ++j) {
+                    pp[j] = -1;
+                    ee[j] = lo[j] = hi[j] = 0;
+                }
                 if (it0 < total) {
-                    int32_t lo = 0, hi = nF - 1;
-                    while (lo < hi) {  // largest f with gp[f] <= it0
-                        const int32_t mid = lo + ((hi - lo + 1) >> 1);
-                        if (gp[mid] <= it0) lo = mid; else hi = mid - 1;
+                    int32_t l = 0, h = nF - 1;
+                    while (l < h) {  // largest f with gp[f] <= it0
+                        const int32_t mid = l + ((h - l + 1) >> 1);
+                        if (gp[mid] <= it0) l = mid; else h = mid - 1;
                     }
-                    int32_t f = lo, fend = gp[f + 1];
-                    int32_t e = F[f], y = gy[f], p0 = arst[gx[f]] - gp[f];
-                    int32_t yl = arst[y], yh = arst[y + 1];
+                    int32_t f = l, fend = gp[f + 1];
+                    int32_t e = F[f], p0 = gx[f], yl = gy[f], yh = gh[f];
 #pragma unroll
                     for (int j = 0; j < kPeelItems; ++j) {
                         const int32_t it = it0 + j;
-                        if (it >= total) break;
-                        if (it >= fend) {
-                            while (it >= fend) fend = gp[++f + 1];
-                            e = F[f];
-                            y = gy[f];
-                            p0 = arst[gx[f]] - gp[f];
-                            yl = arst[y];
-                            yh = arst[y + 1];
-                        }
-                        const int32_t p = p0 + it;
-                        const int32_t w = acol[p], b = aeid[p];
-                        const uint8_t s2 = st[b];
-                        if (s2 == 2 || (s2 == 1 && b < e)) continue;
-                        // lower bound of w in acol[yl, yh): gallop, then bisect
-                        int32_t step = 1, l = yl;
-                        while (l + step < yh && acol[l + step] < w) {
-                            l += step;
-                            step <<= 1;
-                        }
-                        int32_t h = min(l + step, yh);
-                        while (l < h) {
-                            const int32_t mid = l + ((h - l) >> 1);
-                            if (acol[mid] < w) l = mid + 1; else h = mid;
-                        }
-                        yl = l;
-                        if (l < yh && acol[l] == w) {
-                            const int32_t a = aeid[l];
-                            const uint8_t s1 = st[a];
-                            if (!(s1 == 2 || (s1 == 1 && a < e))) {
-                                if (s1 == 0 && atomicSub(&sup[a], 1) == thresh) {
-                                    xk[2 * j] = true;
-                                    xv[2 * j] = a;
-                                }
-                                if (s2 == 0 && atomicSub(&sup[b], 1) == thresh) {
-                                    xk[2 * j + 1] = true;
-                                    xv[2 * j + 1] = b;
-                                }
+                        if (it < total) {
+                            if (it >= fend) {
+                                while (it >= fend) fend = gp[++f + 1];
+                                e = F[f];
+                                p0 = gx[f];
+                                yl = gy[f];
+                                yh = gh[f];
                             }
+                            pp[j] = p0 + it;
+                            ee[j] = e;
+                            lo[j] = yl;
+                            hi[j] = yh;
                         }
+                    }
+                }
+                int32_t w[kPeelItems], b[kPeelItems];
+#pragma unroll
+                for (int j = 0; j < kPeelItems; ++j) {
+                    w[j] = pp[j] >= 0 ? acol[pp[j]] : 0;
+                    b[j] = pp[j] >= 0 ? aeid[pp[j]] : 0;
+                }
+                uint8_t s2[kPeelItems];
+#pragma unroll
+                for (int j = 0; j < kPeelItems; ++j) s2[j] = pp[j] >= 0 ? st[b[j]] : (uint8_t)2;
+                // items whose edge b is removed (or a frontier edge with a
+                // smaller id, which owns the triangle) search nothing
+#pragma unroll
+                for (int j = 0; j < kPeelItems; ++j)
+                    if (s2[j] == 2 || (s2[j] == 1 && b[j] < ee[j])) hi[j] = lo[j];
+                int32_t pos[kPeelItems];
+#pragma unroll
+                for (int j = 0; j < kPeelItems; ++j) pos[j] = -1;
+                for (;;) {  // lower bound of w in acol[lo, hi): rows hold each column once
+                    bool any = false;
+                    int32_t mid[kPeelItems], v[kPeelItems];
+#pragma unroll
+                    for (int j = 0; j < kPeelItems; ++j) {
+                        mid[j] = lo[j] + ((hi[j] - lo[j]) >> 1);
+                        v[j] = lo[j] < hi[j] ? acol[mid[j]] : 0;
+                    }
+#pragma unroll
+                    for (int j = 0; j < kPeelItems; ++j) {
+                        if (lo[j] >= hi[j]) continue;
+                        if (v[j] == w[j]) {
+                            pos[j] = mid[j];
+                            hi[j] = lo[j];
+                        } else if (v[j] < w[j]) {
+                            lo[j] = mid[j] + 1;
+                        } else {
+                            hi[j] = mid[j];
+                        }
+                        any |= lo[j] < hi[j];
+                    }
+                    if (!any) break;
+                }
+                int32_t a[kPeelItems];
+#pragma unroll
+                for (int j = 0; j < kPeelItems; ++j) a[j] = pos[j] >= 0 ? aeid[pos[j]] : -1;
+                uint8_t s1[kPeelItems];
+#pragma unroll
+                for (int j = 0; j < kPeelItems; ++j) s1[j] = a[j] >= 0 ? st[a[j]] : (uint8_t)2;
+#pragma unroll
+                for (int j = 0; j < kPeelItems; ++j) {
+                    if (a[j] < 0 || s1[j] == 2 || (s1[j] == 1 && a[j] < ee[j])) continue;
+                    if (s1[j] == 0 && atomicSub(&sup[a[j]], 1) == thresh) {
+                        xk[2 * j] = true;
+                        xv[2 * j] = a[j];
+                    }
+                    if (s2[j] == 0 && atomicSub(&sup[b[j]], 1) == thresh) {
+                        xk[2 * j + 1] = true;
+                        xv[2 * j + 1] = b[j];
                     }
                 }
                 dev::block_append_n<kPeelThreads, 2 * kPeelItems>(xk, xv, Fn, &s_n, lds);
@@ -336,7 +383,7 @@ extern "C" int gk_batch_tc_ktruss_dev(gk_ctx *ctx, const gk_dcsr *A, int64_t ngr
             return eoff_h[x + 1] - eoff_h[x] > eoff_h[y + 1] - eoff_h[y];
         });
     DevMem D;
-    int32_t *eoff, *order, *sup, *buf0, *buf1, *fy, *fx, *pre, *queue;
+    int32_t *eoff, *order, *sup, *buf0, *buf1, *fy, *fx, *fh, *pre, *queue;
     uint8_t *st;
     unsigned long long *tri;
     long long *alive;
@@ -348,6 +395,7 @@ extern "C" int gk_batch_tc_ktruss_dev(gk_ctx *ctx, const gk_dcsr *A, int64_t ngr
     GK_TRY(D.alloc(&buf1, nL));
     GK_TRY(D.alloc(&fy, nL));
     GK_TRY(D.alloc(&fx, nL));
+    GK_TRY(D.alloc(&fh, nL));
     GK_TRY(D.alloc(&pre, nL + ngraphs));
     GK_TRY(D.alloc(&queue, 1));
     GK_TRY(D.alloc(&tri, ngraphs));
@@ -372,8 +420,8 @@ extern "C" int gk_batch_tc_ktruss_dev(gk_ctx *ctx, const gk_dcsr *A, int64_t ngr
         GK_TRY(launch_classes<MODE_SUPPORT>(ctx, P, CL, nullptr));
         k_graph_triangles<<<gt, 256, 0, ctx->stream>>>((int32_t)ngraphs, eoff, sup, tri);
         k_peel_graphs<<<nb, kPeelThreads, 0, ctx->stream>>>((int32_t)ngraphs, order, eoff, A->rst, A->col, X.aeid,
-                                                            X.eu, X.ev, sup, st, thresh, buf0, buf1, fy, fx, pre,
-                                                            queue, alive);
+                                                            X.eu, X.ev, sup, st, thresh, buf0, buf1, fy, fx, fh,
+                                                            pre, queue, alive);
         if (d_keep)
             k_keep_flags<<<gz, 256, 0, ctx->stream>>>(A->nz, X.aeid, st, d_keep);
         return launch_check("batch step");
