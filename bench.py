@@ -234,10 +234,11 @@ def bc_bytes(n: int, nnz: int, levels: int, nsrc: int) -> float:
 
 
 # The plain fixed-heuristic kernel: the fixed mask-length thresholds as
-# first set (MCA-wave up to 8) pick each row's accumulator and every B row is
-# streamed (hub bitmaps off).  The library default ("heuristic_hub") adds
-# hub bitmaps with bit tests and sends masks up to 16 to MCA-wave.
-PLAIN = {"hub_window": 0, "mca_max_mask": 8}
+# first set (MCA-wave up to 8, the short-row window kernel up to 128) pick
+# each row's accumulator and every B row is streamed (hub bitmaps off).  The
+# library default ("heuristic_hub") adds hub bitmaps with bit tests, sends
+# masks up to 16 to MCA-wave and up to 32 to the short-row window kernel.
+PLAIN = {"hub_window": 0, "mca_max_mask": 8, "small_max_mask": 128}
 
 
 # Triangle-counting options of the committed HBM-traffic profile of the

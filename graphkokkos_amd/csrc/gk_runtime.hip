@@ -231,7 +231,7 @@ extern "C" void gk_opts_default(gk_opts *o) {
     o->dense_min_mask = 16384;
     o->split_flop = 1 << 18;
     o->block_threads = 64;
-    o->small_max_mask = 128;
+    o->small_max_mask = 32;
     o->blocks_small = 24;
     o->blocks_long = 8;
     o->blocks_mca = 8;

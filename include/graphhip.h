@@ -98,7 +98,7 @@ typedef struct gk_opts {
     int32_t split_flop;     /* rows above this many B entries are split   */
     int32_t block_threads;  /* workgroup size for the row kernels         */
     int32_t verbose;
-    int32_t small_max_mask; /* MSA-LDS rows up to this (<= 128) use the short-row kernel */
+    int32_t small_max_mask; /* MSA-LDS rows up to this (<= 128) use the short-row kernel (default 32) */
     int32_t blocks_small;   /* persistent workgroups per CU, short-row MSA-LDS kernel */
     int32_t blocks_long;    /* ... long-row MSA-LDS kernel                           */
     int32_t blocks_mca;     /* ... MCA-wave kernel                                   */
