@@ -900,6 +900,10 @@ __global__ __launch_bounds__(kPrepThreads) void k_round_prep(
     const uint32_t label = D.fixed_label ? D.fixed_label : (uint32_t)(c.k - 1);
     const int32_t chunk = (nF + gridDim.x - 1) / gridDim.x;
     const int32_t f0 = min(nF, (int32_t)blockIdx.x * chunk), f1 = min(nF, f0 + chunk);
+    // workgroups past the frontier's end have nothing more to do: every
+    // workgroup after them is empty too, so none waits on their look-back
+    // words (a frontier of fewer edges than workgroups ends the launch early)
+    if (f0 >= f1) return;
     // pass 1: marks, labels, streamed rows and their lengths; the piece's sum
     int32_t sum = 0;
     for (int32_t b = f0 + tid; b < f1; b += kPrepPer * kPrepThreads) {
