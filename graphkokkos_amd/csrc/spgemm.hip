@@ -975,8 +975,9 @@ __global__ __launch_bounds__(64) void k_mca_wave(Prob P, int cls) {
     constexpr bool TCH = HUB;
     __shared__ int32_t hpre[TCH ? T : 1][kMcaMaxTaskA + 1];
     __shared__ int32_t hbst[TCH ? T : 1][kMcaMaxTaskA];
-    // TC: a bit filter of each tile's mask row (bit mca_hash(c) per column)
-    constexpr bool FLT = MODE == MODE_TC;
+    // TC and supports: a bit filter of each tile's mask row (bit mca_hash(c)
+    // per column)
+    constexpr bool FLT = MODE == MODE_TC || MODE == MODE_SUPPORT;
     __shared__ uint32_t flt[FLT ? T : 1][FLT ? kMcaFilterWords : 1];
     const int lane = threadIdx.x, tile = lane / G, gl = lane % G;
     if constexpr (FLT) {
@@ -1056,8 +1057,8 @@ __global__ __launch_bounds__(64) void k_mca_wave(Prob P, int cls) {
 #pragma unroll
                 for (int u = 0; u < kMcaU; ++u) b[u] = t[u] >= 0 ? P.bcol[t[u]] : -1;
                 if constexpr (FLT) {
-                    // TC: entries whose filter bit is clear are not in the
-                    // mask row (almost all of them: hits are rare); a lane
+                    // Entries whose filter bit is clear are not in the mask
+                    // row (almost all of them: hits are rare); a lane
                     // binary-searches one remaining candidate per pass, so
                     // the wave runs about one search per batch of kMcaU
                     // entries instead of one per entry
@@ -1070,12 +1071,16 @@ __global__ __launch_bounds__(64) void k_mca_wave(Prob P, int cls) {
                         cand |= in ? 1u << u : 0u;
                     }
                     while (__any(cand != 0)) {
-                        int32_t bb = -1;
+                        int32_t bb = -1, tt = 0, kq = 0;
                         if (cand) {
                             const int uu = __builtin_ctz(cand);
                             cand &= cand - 1;
 #pragma unroll
-                            for (int u = 0; u < kMcaU; ++u) bb = u == uu ? b[u] : bb;
+                            for (int u = 0; u < kMcaU; ++u) {
+                                bb = u == uu ? b[u] : bb;
+                                tt = u == uu ? t[u] : tt;
+                                kq = u == uu ? kx[u] : kq;
+                            }
                         }
                         if (bb >= 0) {
                             int32_t l = 0, h = mlen;  // lower_bound in m[0, mlen)
@@ -1083,7 +1088,10 @@ __global__ __launch_bounds__(64) void k_mca_wave(Prob P, int cls) {
                                 const int32_t mid = l + ((h - l) >> 1);
                                 if (m[mid] < bb) l = mid + 1; else h = mid;
                             }
-                            cnt_task += m[l] == bb;
+                            if (m[l] == bb) {
+                                if (MODE == MODE_TC) ++cnt_task;
+                                else Hit<MODE>::on(P, acc[tile], acnt[tile], l, mb, tk.a0, kq, tt, cnt_task);
+                            }
                         }
                     }
                     continue;
