@@ -649,29 +649,39 @@ __device__ __forceinline__ void k_peel_body(int32_t nF, int32_t total, const int
         }
         int32_t t[U], fe[U], yy[U];
         if (f_hi - f_lo < kOwnerWin) {
-            // The chunk's owners are f_lo..f_hi: their prefix values, the
-            // same for the whole wave, are read at once and each item's
-            // owner (the last f with pre[f] <= g) is counted in registers.
+            // The chunk's owners are f_lo..f_hi: lane k reads owner f_lo + k
+            // whole (prefix value, edge, probed end, row start) in one
+            // stage; the prefix values go to scalar registers, each item's
+            // owner (the last f with pre[f] <= g) is counted against them,
+            // and its record comes over by lane shuffle.
+            const int32_t lf = f_lo + min(lane, f_hi - f_lo);
+            int32_t op = 0, oF = 0, oy = 0, orw = 0;
+            if (lane <= kOwnerWin) {
+                op = pre[lf];
+                oF = F[lf];
+                oy = fy[lf];
+                orw = frs[lf];
+            }
             int32_t win[kOwnerWin + 1];
 #pragma unroll
-            for (int k = 0; k <= kOwnerWin; ++k) win[k] = pre[min(f_lo + k, f_hi)];
+            for (int k = 0; k <= kOwnerWin; ++k) win[k] = __builtin_amdgcn_readlane(op, k);
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int32_t g = (int32_t)g0 + 64 * u + lane;
+                int32_t k = 0, pk = win[0];
+#pragma unroll
+                for (int q = 1; q <= kOwnerWin; ++q) {
+                    const bool in = win[q] <= g && q <= f_hi - f_lo;
+                    k = in ? q : k;
+                    pk = in ? win[q] : pk;
+                }
+                const int32_t e = __shfl(oF, k, 64), y = __shfl(oy, k, 64), r = __shfl(orw, k, 64);
                 t[u] = -1;
                 fe[u] = yy[u] = 0;
                 if (g < total) {
-                    int32_t k = 0, pk = win[0];
-#pragma unroll
-                    for (int q = 1; q <= kOwnerWin; ++q) {
-                        const bool in = win[q] <= g && q <= f_hi - f_lo;
-                        k = in ? q : k;
-                        pk = in ? win[q] : pk;
-                    }
-                    const int32_t l = f_lo + k;
-                    fe[u] = F[l];
-                    yy[u] = fy[l];
-                    t[u] = frs[l] + (g - pk);
+                    fe[u] = e;
+                    yy[u] = y;
+                    t[u] = r + (g - pk);
                 }
             }
         } else {
