@@ -317,13 +317,20 @@ __global__ __launch_bounds__(256) void k_frontier(int64_t na, const int32_t *__r
 // slot.  The adjacency only loses entries while peeling, so the tables stay
 // valid: a removed edge is still found and then rejected by its state.
 
-__global__ void k_hash_caps(int64_t m, const int32_t *__restrict__ rst, int64_t *__restrict__ cap) {
+// Largest table: 2^kTableLgMax slots (vertices of degree up to 2^28); the
+// peeling stream packs lg(cap) and two filter-hash bits beside a 32-bit slot
+// index, which needs lg(cap) + 2 < 32.
+constexpr int kTableLgMax = 29;
+
+__global__ void k_hash_caps(int64_t m, const int32_t *__restrict__ rst, int64_t *__restrict__ cap,
+                            int32_t *__restrict__ too_big) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
-        int32_t d = rst[i + 1] - rst[i];
-        int32_t c = 0;
+        const int64_t d = rst[i + 1] - rst[i];
+        int64_t c = 0;
         if (d > 0) {
             c = 8;  // so that every table offset is a multiple of 8 (filter words at offset / 8)
             while (c < 2 * d) c <<= 1;
+            if (c > (1ll << kTableLgMax)) *too_big = 1;
         }
         cap[i] = c;
     }
@@ -651,10 +658,12 @@ __device__ __forceinline__ void k_peel_body(int32_t nF, int32_t total, const int
         if (f_hi - f_lo < kOwnerWin) {
             // The chunk's owners are f_lo..f_hi: lane k reads owner f_lo + k
             // whole (prefix value, edge, probed end, row start) in one
-            // stage; the prefix values go to scalar registers, each item's
-            // owner (the last f with pre[f] <= g) is counted against them,
-            // and its record comes over by lane shuffle.
-            const int32_t lf = f_lo + min(lane, f_hi - f_lo);
+            // stage; each item's owner (the last f with pre[f] <= g) is
+            // found by binary search over the lanes' prefix values, and its
+            // record comes over by lane shuffle.
+            static_assert(kOwnerWin < 32, "the window's search takes five steps");
+            const int32_t nw = f_hi - f_lo;  // window lanes 0..nw
+            const int32_t lf = f_lo + min(lane, nw);
             int32_t op = 0, oF = 0, oy = 0, orw = 0;
             if (lane <= kOwnerWin) {
                 op = pre[lf];
@@ -662,19 +671,16 @@ __device__ __forceinline__ void k_peel_body(int32_t nF, int32_t total, const int
                 oy = fy[lf];
                 orw = frs[lf];
             }
-            int32_t win[kOwnerWin + 1];
-#pragma unroll
-            for (int k = 0; k <= kOwnerWin; ++k) win[k] = __builtin_amdgcn_readlane(op, k);
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int32_t g = (int32_t)g0 + 64 * u + lane;
-                int32_t k = 0, pk = win[0];
+                int32_t k = 0;
 #pragma unroll
-                for (int q = 1; q <= kOwnerWin; ++q) {
-                    const bool in = win[q] <= g && q <= f_hi - f_lo;
-                    k = in ? q : k;
-                    pk = in ? win[q] : pk;
+                for (int step = 16; step > 0; step >>= 1) {
+                    const int32_t q = min(k + step, nw);
+                    k = __shfl(op, q, 64) <= g ? q : k;
                 }
+                const int32_t pk = __shfl(op, k, 64);
                 const int32_t e = __shfl(oF, k, 64), y = __shfl(oy, k, 64), r = __shfl(orw, k, 64);
                 t[u] = -1;
                 fe[u] = yy[u] = 0;
@@ -703,35 +709,39 @@ __device__ __forceinline__ void k_peel_body(int32_t nF, int32_t total, const int
                 }
             }
         }
+        // Per item two registers locate w in y's table and filter: hs, the
+        // table index of w's first slot (y's offset h0 + slot), and lgr =
+        // lg(cap) | r << 8.  Offsets are multiples of 8 and fit 32 bits, and
+        // lg <= 29 (both checked by the host), so w's filter bit, slot * 4 + r
+        // (r = the two hash bits below the slot's), sits in word hs / 8 at
+        // bit (hs % 8) * 4 + r.
         int32_t w[U], b[U];
-        int64_t h0[U];
-        uint32_t cap[U];
+        uint32_t hs[U], lgr[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             w[u] = b[u] = 0;
-            h0[u] = 0;
-            cap[u] = 1;
+            hs[u] = lgr[u] = 0;
             if (t[u] >= 0) {
                 w[u] = col[t[u]];
                 b[u] = eid[t[u]];
-                h0[u] = hoff[yy[u]];
-                cap[u] = (uint32_t)(hoff[yy[u] + 1] - h0[u]);
+                const int64_t o = hoff[yy[u]];
+                hs[u] = (uint32_t)o;
+                lgr[u] = (uint32_t)(hoff[yy[u] + 1] - o);  // the capacity, until the next stage
             }
         }
         uint8_t s2[U];
-        uint32_t slot[U], fw[U], fbit[U];
+        uint32_t fw[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             s2[u] = 2;
-            slot[u] = fw[u] = fbit[u] = 0;
+            fw[u] = 0;
             if (t[u] >= 0) {
                 s2[u] = (uint8_t)st_get(st, b[u]);
-                const int lg = 31 - __clz((int)cap[u]);
+                const int lg = 31 - __clz((int)lgr[u]);
                 const uint32_t hh = dev::hash32((uint32_t)w[u]);
-                slot[u] = hh >> (32 - lg);
-                const uint32_t fb = filter_bit(hh, lg);
-                fbit[u] = fb & 31;
-                fw[u] = flt[(h0[u] >> 3) + (fb >> 5)];
+                hs[u] += hh >> (32 - lg);
+                lgr[u] = (uint32_t)lg | (((hh >> (30 - lg)) & 3u) << 8);
+                fw[u] = flt[hs[u] >> 3];
             }
         }
         // y's table is read only where the filter bit of w is set (w may be
@@ -740,19 +750,30 @@ __device__ __forceinline__ void k_peel_body(int32_t nF, int32_t total, const int
         int2 kv[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            look[u] = t[u] >= 0 && s2[u] != 2 && !(s2[u] == 1 && b[u] < fe[u]) && ((fw[u] >> fbit[u]) & 1u);
-            kv[u] = look[u] ? tab[h0[u] + slot[u]] : make_int2(-1, -1);
+            look[u] = t[u] >= 0 && s2[u] != 2 && !(s2[u] == 1 && b[u] < fe[u]) &&
+                      ((fw[u] >> (((hs[u] & 7u) << 2) | (lgr[u] >> 8))) & 1u);
+            kv[u] = look[u] ? tab[hs[u]] : make_int2(-1, -1);
         }
         int32_t a[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             a[u] = -1;
             if (!look[u]) continue;
-            for (;;) {  // y's table: (neighbour, edge id), -1 marks an empty slot
-                if (kv[u].x == w[u]) { a[u] = kv[u].y; break; }
-                if (kv[u].x < 0) break;
-                slot[u] = (slot[u] + 1) & (cap[u] - 1);
-                kv[u] = tab[h0[u] + slot[u]];
+            if (kv[u].x == w[u]) {
+                a[u] = kv[u].y;
+                continue;
+            }
+            // further slots (rare): y's table starts at h0 = hs - slot
+            const uint32_t lg = lgr[u] & 0xffu, cap = 1u << lg;
+            uint32_t slot = dev::hash32((uint32_t)w[u]) >> (32 - lg);
+            const uint32_t h0 = hs[u] - slot;
+            while (kv[u].x >= 0) {  // y's table: (neighbour, edge id), -1 marks an empty slot
+                slot = (slot + 1) & (cap - 1);
+                kv[u] = tab[h0 + slot];
+                if (kv[u].x == w[u]) {
+                    a[u] = kv[u].y;
+                    break;
+                }
             }
         }
         uint8_t s1[U];
@@ -1007,8 +1028,12 @@ __global__ __launch_bounds__(kPrepThreads) void k_round_prep(
     }
 }
 
+// Eight waves per SIMD (62 VGPRs; the scalar registers over 100 that would
+// cap it at seven are spilled to vector lanes): most rounds are chains of
+// dependent loads, and the long ones are bound by the waves in flight
+// (R-MAT scale 20: 123.5 -> 117.1 ms against seven waves).
 template <int U>
-__global__ __launch_bounds__(256) void k_dround_stream(const RoundCtl *__restrict__ ctl, const PeelDev D,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_dround_stream(const RoundCtl *__restrict__ ctl, const PeelDev D,
                                                        int32_t *__restrict__ buf0, int32_t *__restrict__ buf1,
                                                        const int32_t *__restrict__ pre,
                                                        const int32_t *__restrict__ fy, const int32_t *__restrict__ fx,
@@ -1189,13 +1214,21 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
     }
     const int gw = waves_grid(ctx, m);
     // adjacency tables: their sizes first (the table is allocated from them)
-    k_hash_caps<<<grid_items(ctx, m), 256, 0, ctx->stream>>>(m, A->rst, caps);
+    GK_HIP(hipMemsetAsync(&dpc->ac, 0, 4, ctx->stream));
+    k_hash_caps<<<grid_items(ctx, m), 256, 0, ctx->stream>>>(m, A->rst, caps, &dpc->ac);
     GK_TRY(exclusive_scan_i64(ctx, caps, hoff, m + 1));
     int64_t *h64;
     GK_TRY(pinned(ctx, kPinnedBytes, (void **)&h64));
     GK_HIP(hipMemcpyAsync(h64, hoff + m, 8, hipMemcpyDeviceToHost, ctx->stream));
+    GK_HIP(hipMemcpyAsync(h64 + 1, &dpc->ac, 4, hipMemcpyDeviceToHost, ctx->stream));
     GK_HIP(hipStreamSynchronize(ctx->stream));
     const int64_t tab_n = h64[0];
+    // the stream indexes the tables with 32-bit slot numbers
+    if (*(const int32_t *)(h64 + 1) || tab_n > (int64_t)UINT32_MAX) {
+        set_error("truss peeling: the adjacency tables need a vertex of degree at most 2^28 and at most 2^32 "
+                  "slots in all");
+        return GK_EINVAL;
+    }
     GK_TRY(B.alloc(&tab, tab_n));
     uint32_t *tflt;  // the tables' bit filters: tab_n / 8 words (every cap is a multiple of 8)
     GK_TRY(B.alloc(&tflt, tab_n / 8));
