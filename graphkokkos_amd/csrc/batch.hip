@@ -32,7 +32,11 @@ namespace gk {
 namespace {
 
 constexpr int kPeelThreads = 1024;
-constexpr int kPeelItems = 8;  // consecutive streamed items per thread and step
+// consecutive streamed items per thread and step: four keep the peeling
+// kernel at 64 VGPRs, so two 1024-thread workgroups share a CU (eight waves
+// per SIMD); eight items took 124 VGPRs and one workgroup per CU (4096-graph
+// step 91.8 -> 80.1 ms)
+constexpr int kPeelItems = 4;
 
 template <int NT>
 __device__ __forceinline__ int32_t block_incl_scan(int32_t v, int32_t *lds, int32_t *total) {
@@ -80,7 +84,7 @@ __global__ __launch_bounds__(256) void k_graph_triangles(int32_t ngraphs, const 
 // an edge whose support drops below thresh joins the next frontier.
 // States: 0 alive, 1 frontier, 2 removed.  Graph g uses the segments
 // [eoff[g], eoff[g+1]) of buf0 / buf1 / fy / fx and eoff[g] + g of pre.
-__global__ __launch_bounds__(kPeelThreads) void k_peel_graphs(
+__global__ __launch_bounds__(kPeelThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_peel_graphs(
     int32_t ngraphs, const int32_t *__restrict__ order, const int32_t *__restrict__ eoff,
     const int32_t *__restrict__ arst, const int32_t *__restrict__ acol, const int32_t *__restrict__ aeid,
     const int32_t *__restrict__ eu, const int32_t *__restrict__ ev, int32_t *__restrict__ sup,
