@@ -162,7 +162,10 @@ __device__ __forceinline__ double backward_finish(int64_t i, uint32_t nd, const 
 // Level d -> d+1 for vertices with fewer than kHeavy in-neighbours, one wave
 // per vertex (heavier ones: k_bc_heavy_chunk).  cnt accumulates the number
 // of (vertex, source) pairs reached.
-__global__ __launch_bounds__(256) void k_bc_forward(int64_t n, const int32_t *__restrict__ trst,
+// Eight waves per SIMD (64 VGPRs, one under the 65 it would take): a level
+// is a chain of dependent loads per vertex.  Together with the heavy-chunk
+// kernel's eight waves: R-MAT scale 20, 4 x 256 sources, 106.5 -> 99.8 ms.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_bc_forward(int64_t n, const int32_t *__restrict__ trst,
                                                     const int32_t *__restrict__ tcol,
                                                     const uint64_t *__restrict__ fd, uint64_t *__restrict__ fn,
                                                     uint64_t *__restrict__ vis, double *__restrict__ S,
@@ -239,9 +242,11 @@ __device__ __forceinline__ uint32_t sweep_mask(int64_t i, const uint64_t *__rest
 }
 
 // Forward (FWD): sums sigma over in-neighbours on level d (fd) into the new
-// level fn; backward: sums W over out-neighbours on level d+1 (fn).
+// level fn; backward: sums W over out-neighbours on level d+1 (fn).  Eight
+// waves per SIMD, so two 16-wave workgroups share a CU (at the cost of a few
+// registers spilled: 12 B per lane backward, 36 B forward).
 template <bool FWD>
-__global__ __launch_bounds__(64 * kHeavyWaves) void k_bc_heavy_chunk(
+__global__ __launch_bounds__(64 * kHeavyWaves) __attribute__((amdgpu_waves_per_eu(8))) void k_bc_heavy_chunk(
     int32_t nc, const int4 *__restrict__ chunks, const int32_t *__restrict__ col, const uint64_t *__restrict__ fd,
     uint64_t *__restrict__ fn, uint64_t *__restrict__ vis, double *__restrict__ S, double *__restrict__ bc,
     unsigned long long *__restrict__ cnt, double *__restrict__ part) {
