@@ -210,24 +210,28 @@ __global__ void k_edge_ends(int64_t m, const int32_t *__restrict__ lrst, const i
         }
 }
 
-// Edge id of every entry of the symmetric pattern.
+// Edge id of every entry of the symmetric pattern (also to eid2 if given:
+// a second copy by the same stores instead of a copy kernel afterwards).
 __global__ void k_sym_eid(int64_t m, const int32_t *__restrict__ rst, const int32_t *__restrict__ col,
-                          const int32_t *__restrict__ lrst, int32_t *__restrict__ eid) {
+                          const int32_t *__restrict__ lrst, int32_t *__restrict__ eid,
+                          int32_t *__restrict__ eid2 = nullptr) {
     const int lane = threadIdx.x & 63;
     const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
     for (int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < m; i += waves) {
         for (int32_t p = rst[i] + lane; p < rst[i + 1]; p += 64) {
-            int32_t j = col[p];
+            int32_t j = col[p], e;
             if (j < i) {
-                eid[p] = lrst[i] + (p - rst[i]);
+                e = lrst[i] + (p - rst[i]);
             } else {
                 int32_t l = rst[j], h = rst[j + 1];
                 while (l < h) {
                     int32_t mid = l + ((h - l) >> 1);
                     if (col[mid] < (int32_t)i) l = mid + 1; else h = mid;
                 }
-                eid[p] = lrst[j] + (l - rst[j]);
+                e = lrst[j] + (l - rst[j]);
             }
+            eid[p] = e;
+            if (eid2) eid2[p] = e;
         }
     }
 }
@@ -1244,11 +1248,7 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
         k_edge_ends<<<gw, 256, 0, s>>>(m, L0.rst, L0.col, eu, ev);
         GK_HIP(hipMemcpyAsync(arst[0], A->rst, 4 * (m + 1), hipMemcpyDeviceToDevice, s));
         GK_HIP(hipMemcpyAsync(acol[0], A->col, 4 * (size_t)A->nz, hipMemcpyDeviceToDevice, s));
-        k_sym_eid<<<gw, 256, 0, s>>>(m, A->rst, A->col, L0.rst, aeid[0]);
-        // the edge id of every entry of A, kept for the final per-entry
-        // output (aeid[0] itself is reused by the adjacency compaction)
-        if (sym_eid_out)
-            GK_HIP(hipMemcpyAsync(*sym_eid_out, aeid[0], 4 * (size_t)A->nz, hipMemcpyDeviceToDevice, s));
+        k_sym_eid<<<gw, 256, 0, s>>>(m, A->rst, A->col, L0.rst, aeid[0], sym_eid_out ? *sym_eid_out : nullptr);
         k_iota<<<grid_items(ctx, nL), 256, 0, s>>>(nL, alist);
         GK_HIP(hipMemsetAsync(st, 0, 4 * (size_t)std::max<int64_t>(st_words, 1), s));
         GK_HIP(hipMemsetAsync(dctr, 0, 16, s));
