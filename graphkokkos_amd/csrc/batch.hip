@@ -32,6 +32,7 @@ namespace gk {
 namespace {
 
 constexpr int kPeelThreads = 1024;
+constexpr int kBatchMcaBlocksPerCu = 32;
 // consecutive streamed items per thread and step: four keep the peeling
 // kernel at 64 VGPRs, so two 1024-thread workgroups share a CU (eight waves
 // per SIMD); eight items took 124 VGPRs and one workgroup per CU (4096-graph
@@ -410,7 +411,12 @@ extern "C" int gk_batch_tc_ktruss_dev(gk_ctx *ctx, const gk_dcsr *A, int64_t ngr
     gk_dcsr Ld = view(ctx, A->m, A->m, nL, X.lrst, X.lcol, nullptr);
     Prob P;
     ClassLaunch CL;
-    GK_TRY(prepare_product(ctx, &Ld, &Ld, &Ld, sup, o, &P, &CL));
+    // the union's rows are mostly short (MCA-wave), so that class gets more
+    // workgroups than in a single graph's product: the CUs the window classes
+    // release when they finish first go to it (4096 graphs: 80.3 -> 72.2 ms)
+    gk_opts ob = o;
+    if (ob.blocks_mca <= 0) ob.blocks_mca = kBatchMcaBlocksPerCu;
+    GK_TRY(prepare_product(ctx, &Ld, &Ld, &Ld, sup, ob, &P, &CL));
     GK_HIP(hipStreamSynchronize(ctx->stream));
     const int thresh = k - 2;
     const int nb = (int)std::max<int64_t>(

@@ -234,7 +234,7 @@ extern "C" void gk_opts_default(gk_opts *o) {
     o->small_max_mask = 32;
     o->blocks_small = 24;
     o->blocks_long = 8;
-    o->blocks_mca = 8;
+    o->blocks_mca = 0;  // the engine's default (8), the batching step's 32
     o->blocks_global = 4;
     o->batch_blocks = 8;
     o->batch_order = 0;

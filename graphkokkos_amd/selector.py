@@ -191,6 +191,7 @@ BATCH_SPACE = {
     "small_max_mask": [32, 64, 128],
     "mca_max_mask": [0, 8, 16],
     "blocks_small": [12, 24, 32],
+    "blocks_mca": [8, 16, 32, 64],
     "split_flop": [1 << 16, 1 << 18],
 }
 

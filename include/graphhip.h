@@ -101,7 +101,8 @@ typedef struct gk_opts {
     int32_t small_max_mask; /* MSA-LDS rows up to this (<= 128) use the short-row kernel (default 32) */
     int32_t blocks_small;   /* persistent workgroups per CU, short-row MSA-LDS kernel */
     int32_t blocks_long;    /* ... long-row MSA-LDS kernel                           */
-    int32_t blocks_mca;     /* ... MCA-wave kernel                                   */
+    int32_t blocks_mca;     /* ... MCA-wave kernel (0: 8; the batching step, whose
+                             * support product is mostly short rows, 32)         */
     int32_t blocks_global;  /* ... hash-global and dense-global kernels              */
     int32_t batch_blocks;   /* batching: peeling workgroups per CU                   */
     int32_t batch_order;    /* batching: 0 = graphs largest first, 1 = given order   */
