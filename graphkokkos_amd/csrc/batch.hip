@@ -8,13 +8,14 @@
 //   1. supports of every edge: one SUPPORT-mode masked product C<L> = L*L
 //      over the union -- the engine's persistent class kernels balance
 //      rows across graphs whatever their sizes;
-//   2. k_graph_triangles: triangles(g) = (sum of g's supports) / 3 (each
-//      triangle adds one to each of its three edges);
-//   3. k_peel_graphs: a persistent kernel; each workgroup takes graphs from
-//      a queue (largest first) and peels its graph to the k-truss inside
-//      the workgroup, rounds separated by workgroup barriers -- the rounds
-//      of src/main.cpp:102-133 for every graph, without a grid-wide sync;
-//   4. k_keep_flags: per entry of A whether it is in its graph's k-truss.
+//   2. k_peel_graphs: a persistent kernel; each workgroup takes graphs from
+//      a queue (largest first), sums the graph's supports as it collects
+//      its first frontier (triangles(g) = that sum / 3: each triangle adds
+//      one to each of its three edges) and peels the graph to the k-truss
+//      inside the workgroup, rounds separated by workgroup barriers -- the
+//      rounds of src/main.cpp:102-133 for every graph, without a grid-wide
+//      sync;
+//   3. k_keep_flags: per entry of A whether it is in its graph's k-truss.
 // Every buffer the step writes is cleared by kernels of the step itself,
 // so replays of the captured graph repeat the first run exactly.
 #include <algorithm>
@@ -62,21 +63,6 @@ __global__ void k_zero_bytes(int64_t n, uint8_t *__restrict__ p) {
         p[i] = 0;
 }
 
-__global__ __launch_bounds__(256) void k_graph_triangles(int32_t ngraphs, const int32_t *__restrict__ eoff,
-                                                         const int32_t *__restrict__ sup,
-                                                         unsigned long long *__restrict__ tri) {
-    __shared__ unsigned long long part[256 / 64];
-    for (int32_t g = blockIdx.x; g < ngraphs; g += gridDim.x) {
-        unsigned long long s = 0;
-        for (int32_t e = eoff[g] + threadIdx.x; e < eoff[g + 1]; e += 256) s += (uint32_t)sup[e];
-        s = dev::wave_sum(s);
-        if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
-        __syncthreads();
-        if (threadIdx.x == 0) tri[g] = (part[0] + part[1] + part[2] + part[3]) / 3;
-        __syncthreads();
-    }
-}
-
 // One workgroup peels one graph at a time: frontier F = alive edges with
 // support < thresh; per round each frontier edge f = (y, x) streams the row
 // of x (y = endpoint with the longer row, searched by bisection); a
@@ -91,8 +77,11 @@ __global__ __launch_bounds__(kPeelThreads) __attribute__((amdgpu_waves_per_eu(8)
     const int32_t *__restrict__ eu, const int32_t *__restrict__ ev, int32_t *__restrict__ sup,
     uint8_t *__restrict__ st, int32_t thresh, int32_t *__restrict__ buf0, int32_t *__restrict__ buf1,
     int32_t *__restrict__ fy, int32_t *__restrict__ fx, int32_t *__restrict__ fh, int32_t *__restrict__ pre,
-    int32_t *__restrict__ queue, long long *__restrict__ alive_edges) {
+    int32_t *__restrict__ queue, long long *__restrict__ per_graph) {
+    // per_graph[g]: the edges of g's k-truss; per_graph[ngraphs + g]: g's triangles
+    long long *alive_edges = per_graph, *tri = per_graph + ngraphs;
     __shared__ int32_t lds[kPeelThreads / 64 + 1];
+    __shared__ unsigned long long s_sum[kPeelThreads / 64];
     __shared__ int32_t s_g, s_n;
     __shared__ int32_t s_alive[kPeelThreads / 64];
     const int tid = threadIdx.x;
@@ -110,9 +99,21 @@ __global__ __launch_bounds__(kPeelThreads) __attribute__((amdgpu_waves_per_eu(8)
         int32_t *gy = fy + e0, *gx = fx + e0, *gh = fh + e0, *gp = pre + e0 + g;
         if (tid == 0) s_n = 0;
         __syncthreads();
+        unsigned long long ssum = 0;  // the graph's supports, three per triangle
         for (int32_t base = e0; base < e1; base += kPeelThreads) {
             const int32_t e = base + tid;
-            dev::block_append<kPeelThreads>(e < e1 && sup[e] < thresh, e, F, &s_n, lds);
+            const int32_t sv = e < e1 ? sup[e] : 0;
+            ssum += (uint32_t)sv;
+            dev::block_append<kPeelThreads>(e < e1 && sv < thresh, e, F, &s_n, lds);
+        }
+        ssum = dev::wave_sum(ssum);
+        if ((tid & 63) == 0) s_sum[tid >> 6] = ssum;
+        __syncthreads();
+        if (tid == 0) {
+            unsigned long long t = 0;
+#pragma unroll
+            for (int x = 0; x < kPeelThreads / 64; ++x) t += s_sum[x];
+            tri[g] = (long long)(t / 3);
         }
         int32_t nF = s_n;
         while (nF > 0) {
@@ -390,8 +391,7 @@ extern "C" int gk_batch_tc_ktruss_dev(gk_ctx *ctx, const gk_dcsr *A, int64_t ngr
     DevMem D;
     int32_t *eoff, *order, *sup, *buf0, *buf1, *fy, *fx, *fh, *pre, *queue;
     uint8_t *st;
-    unsigned long long *tri;
-    long long *alive;
+    long long *alive, *tri;  // one allocation: k-truss edges, then triangles, per graph
     GK_TRY(D.alloc(&eoff, ngraphs + 1));
     GK_TRY(D.alloc(&order, ngraphs));
     GK_TRY(D.alloc(&sup, nL));
@@ -403,8 +403,8 @@ extern "C" int gk_batch_tc_ktruss_dev(gk_ctx *ctx, const gk_dcsr *A, int64_t ngr
     GK_TRY(D.alloc(&fh, nL));
     GK_TRY(D.alloc(&pre, nL + ngraphs));
     GK_TRY(D.alloc(&queue, 1));
-    GK_TRY(D.alloc(&tri, ngraphs));
-    GK_TRY(D.alloc(&alive, ngraphs));
+    GK_TRY(D.alloc(&alive, 2 * (int64_t)ngraphs));
+    tri = alive + ngraphs;
     GK_HIP(hipMemcpyAsync(eoff, eoff_h.data(), 4 * (size_t)(ngraphs + 1), hipMemcpyHostToDevice, ctx->stream));
     GK_HIP(hipMemcpyAsync(order, order_h.data(), 4 * (size_t)ngraphs, hipMemcpyHostToDevice, ctx->stream));
     // plan of the support product C<L> = L*L (also outside the step)
@@ -422,13 +422,11 @@ extern "C" int gk_batch_tc_ktruss_dev(gk_ctx *ctx, const gk_dcsr *A, int64_t ngr
     const int nb = (int)std::max<int64_t>(
         1, std::min<int64_t>(ngraphs, (int64_t)ctx->num_cus * std::max(1, std::min(o.batch_blocks, 32))));
     const int gz = (int)std::max<int64_t>(1, std::min<int64_t>((nL + 255) / 256, ctx->num_cus * 16));
-    const int gt = (int)std::min<int64_t>(ngraphs, (int64_t)ctx->num_cus * 8);
     auto step = [&]() -> int {
         k_zero_bytes<<<gz, 256, 0, ctx->stream>>>((int64_t)nL, st);
         k_zero_bytes<<<1, 64, 0, ctx->stream>>>(4, (uint8_t *)queue);
         GK_TRY(zero_u32(ctx, sup, nL));
         GK_TRY(launch_classes<MODE_SUPPORT>(ctx, P, CL, nullptr));
-        k_graph_triangles<<<gt, 256, 0, ctx->stream>>>((int32_t)ngraphs, eoff, sup, tri);
         k_peel_graphs<<<nb, kPeelThreads, 0, ctx->stream>>>((int32_t)ngraphs, order, eoff, A->rst, A->col, X.aeid,
                                                             X.eu, X.ev, sup, st, thresh, buf0, buf1, fy, fx, fh,
                                                             pre, queue, alive);
