@@ -464,6 +464,18 @@ __device__ __forceinline__ void mark_chunks(int32_t f, int32_t pre_f, int32_t le
 
 // streamed entries per lane in flight in the peeling kernels
 constexpr int kPeelU = 4;
+// XCDs of the MI355X; a grid's workgroups are dispatched to them round-robin
+constexpr int kXcds = 8;
+// A round of at most kSpreadChunks stream chunks puts its chunks on different
+// workgroups (so different CUs) in runs of kSpreadRun per XCD, instead of
+// four consecutive chunks on the four waves of one workgroup: such a round
+// is one step of one dependent-load chain per wave, and four waves of one CU
+// queue their random loads behind each other (R-MAT s20 116.5 -> 115.3 ms,
+// grid 2048^2 87.1 -> 81.1, Chung-Lu 4M 287.3 -> 283.9).  Longer rounds keep
+// the four consecutive chunks together: they share owner records and table
+// lines through the CU's L1 (spreading those cost R-MAT 7 ms).
+constexpr int64_t kSpreadChunks = 256;
+constexpr int64_t kSpreadRun = 8;
 // a stream chunk whose items belong to at most kOwnerWin + 1 frontier edges
 // finds each item's owner among their prefix values in registers
 constexpr int kOwnerWin = 16;
@@ -630,8 +642,16 @@ __device__ __forceinline__ void k_peel_body(int32_t nF, int32_t total, const int
     }
     __syncthreads();
     const int lane = threadIdx.x & 63;
-    const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    const int64_t nbx = gridDim.x / kXcds;  // workgroups per XCD
+    int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if ((total + CH - 1) / CH <= kSpreadChunks && gridDim.x % kXcds == 0 &&
+        (nbx * (blockDim.x >> 6)) % kSpreadRun == 0) {
+        // j-th wave of XCD x (its workgroups' first waves first) -> chunk
+        // (j / R) * kXcds * R + x * R + j % R: a bijection onto [0, nwaves)
+        const int64_t x = blockIdx.x % kXcds, j = (int64_t)(threadIdx.x >> 6) * nbx + blockIdx.x / kXcds;
+        wave = (j / kSpreadRun) * (kXcds * kSpreadRun) + x * kSpreadRun + j % kSpreadRun;
+    }
     uint32_t cnt = 0;
     // Each step resolves U items per lane in stages, every stage's loads
     // issued for all U items before any is used: owner edge -> row entry
