@@ -919,9 +919,16 @@ __global__ __launch_bounds__(kPrepThreads) void k_round_prep(
     const int32_t nF = c.nF, nP = c.nP;
     const int32_t *F = c.cur ? buf1 : buf0;
     const int32_t *Fp = c.cur ? buf0 : buf1;
-    mark_removed<4>(nP, Fp, st, blockIdx.x * kPrepThreads + tid, gridDim.x * kPrepThreads);
+    // The previous frontier becomes removed, workgroup b taking the entries
+    // [p0, p1): by all its threads where it has no frontier edges of its own,
+    // else by waves 1.. while wave 0 waits in the look-back (the new and the
+    // previous frontier are disjoint, and the stream reads the states only
+    // after this launch), so the marking is off the launch's load chain.
+    const int32_t pc = (nP + (int32_t)gridDim.x - 1) / (int32_t)gridDim.x;
+    const int32_t p0 = min(nP, (int32_t)blockIdx.x * pc), p1 = min(nP, p0 + pc);
     c.nP = 0;
     if (nF == 0) {
+        mark_removed<4>(p1, Fp, st, p0 + tid, kPrepThreads);
         if (D.levels) {
             // minimum support over the near list's alive entries
             uint32_t lo = 0xffffffffu;
@@ -958,7 +965,10 @@ __global__ __launch_bounds__(kPrepThreads) void k_round_prep(
     // workgroups past the frontier's end have nothing more to do: every
     // workgroup after them is empty too, so none waits on their look-back
     // words (a frontier of fewer edges than workgroups ends the launch early)
-    if (f0 >= f1) return;
+    if (f0 >= f1) {
+        mark_removed<4>(p1, Fp, st, p0 + tid, kPrepThreads);
+        return;
+    }
     // pass 1: marks, labels, streamed rows and their lengths; the piece's sum
     int32_t sum = 0;
     for (int32_t b = f0 + tid; b < f1; b += kPrepPer * kPrepThreads) {
@@ -1014,6 +1024,8 @@ __global__ __launch_bounds__(kPrepThreads) void k_round_prep(
             __hip_atomic_store(&D.lb[bi], lb_word(epoch, true, excl + (uint32_t)agg), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
         if (lane == 0) excl_s = (int32_t)excl;
+    } else {
+        mark_removed<4>(p1, Fp, st, p0 + tid - 64, kPrepThreads - 64);
     }
     __syncthreads();
     // pass 2: the piece's lengths scanned from its offset into pre, and the
