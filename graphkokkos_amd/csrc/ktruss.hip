@@ -312,16 +312,20 @@ __global__ __launch_bounds__(256) void k_frontier(int64_t na, const int32_t *__r
 
 // ---- static adjacency hash for the peeling rounds ------------------------
 // Vertex v's adjacency gets an open-addressing table of cap(v) = the
-// smallest power of two >= max(8, 2 deg(v)) slots, (neighbour, edge id) per
+// smallest power of two >= max(8, 4 deg(v)) slots, (neighbour, edge id) per
 // slot, built once per decomposition, and in front of it a bit filter of
 // 4 cap(v) bits (a neighbour w sets bit hash(w) >> (32 - lg(4 cap))): most
 // probes of the peeling stream look for a w that is not a neighbour, and a
 // clear bit answers them from a 4-byte word in an array a sixteenth of the
-// tables' size (45 MB against 700 MB at R-MAT scale 20) instead of a table
-// slot.  The adjacency only loses entries while peeling, so the tables stay
-// valid: a removed edge is still found and then rejected by its state.
+// tables' size (90 MB against 1.4 GB at R-MAT scale 20) instead of a table
+// slot.  A load factor of at most a quarter keeps the linear probe chains
+// short -- a wave waits for its longest one -- and the filter at 16 bits or
+// more per neighbour (R-MAT s20 115.5 -> 113.8 ms, Chung-Lu 4M 283.0 ->
+// 276.6 ms against at most a half; at most an eighth: 118.5 / 279.6 ms).
+// The adjacency only loses entries while peeling, so the tables stay valid:
+// a removed edge is still found and then rejected by its state.
 
-// Largest table: 2^kTableLgMax slots (vertices of degree up to 2^28); the
+// Largest table: 2^kTableLgMax slots (vertices of degree up to 2^27); the
 // peeling stream packs lg(cap) and two filter-hash bits beside a 32-bit slot
 // index, which needs lg(cap) + 2 < 32.
 constexpr int kTableLgMax = 29;
@@ -333,7 +337,7 @@ __global__ void k_hash_caps(int64_t m, const int32_t *__restrict__ rst, int64_t 
         int64_t c = 0;
         if (d > 0) {
             c = 8;  // so that every table offset is a multiple of 8 (filter words at offset / 8)
-            while (c < 2 * d) c <<= 1;
+            while (c < 4 * d) c <<= 1;
             if (c > (1ll << kTableLgMax)) *too_big = 1;
         }
         cap[i] = c;
@@ -1261,7 +1265,7 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
     const int64_t tab_n = h64[0];
     // the stream indexes the tables with 32-bit slot numbers
     if (*(const int32_t *)(h64 + 1) || tab_n > (int64_t)UINT32_MAX) {
-        set_error("truss peeling: the adjacency tables need a vertex of degree at most 2^28 and at most 2^32 "
+        set_error("truss peeling: the adjacency tables need a vertex of degree at most 2^27 and at most 2^32 "
                   "slots in all");
         return GK_EINVAL;
     }
