@@ -22,6 +22,7 @@
 #include <cstddef>
 #include <cstring>
 
+#include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 
 #include "gk_device.h"
@@ -350,24 +351,60 @@ __device__ __forceinline__ uint32_t filter_bit(uint32_t hh, int lg) {
     return lg + 2 >= 32 ? hh : hh >> (30 - lg);
 }
 
-__global__ void k_hash_fill(int64_t m, const int32_t *__restrict__ rst, const int32_t *__restrict__ col,
-                            const int32_t *__restrict__ eid, const int64_t *__restrict__ hoff,
-                            int2 *__restrict__ tab, uint32_t *__restrict__ flt) {
-    const int lane = threadIdx.x & 63;
-    const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
-    for (int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < m; i += waves) {
-        const int64_t h0 = hoff[i];
-        const uint32_t c = (uint32_t)(hoff[i + 1] - h0);
-        if (!c) continue;
-        const int lg = 31 - __clz((int)c);
-        for (int32_t p = rst[i] + lane; p < rst[i + 1]; p += 64) {
-            const int32_t w = col[p];
-            const uint32_t hh = dev::hash32((uint32_t)w);
-            uint32_t s = hh >> (32 - lg);
-            while (atomicCAS(&tab[h0 + s].x, -1, w) != -1) s = (s + 1) & (c - 1);
-            tab[h0 + s].y = eid[p];
-            const uint32_t fb = filter_bit(hh, lg);
-            atomicOr(&flt[(h0 >> 3) + (fb >> 5)], 1u << (fb & 31));
+// The tables filled edge by edge: edge e = (eu[e], ev[e]) goes into both
+// ends' tables.  Per vertex (one wave a row) a hub's row of 10^5 entries is
+// one wave's serial chain of claims while the rest of the GPU waits; per
+// edge the claims of a hub's table come from every wave (Chung-Lu 4M:
+// 25.4 ms per vertex).  Where a key lands depends on the order of the claims,
+// what a probe finds does not.
+__device__ __forceinline__ void table_put(const int64_t *__restrict__ hoff, int2 *__restrict__ tab,
+                                          uint32_t *__restrict__ flt, int32_t i, int32_t w, int32_t e) {
+    const int64_t h0 = hoff[i];
+    const uint32_t c = (uint32_t)(hoff[i + 1] - h0);
+    const int lg = 31 - __clz((int)c);
+    const uint32_t hh = dev::hash32((uint32_t)w);
+    uint32_t sl = hh >> (32 - lg);
+    while (atomicCAS(&tab[h0 + sl].x, -1, w) != -1) sl = (sl + 1) & (c - 1);
+    tab[h0 + sl].y = e;
+    const uint32_t fb = filter_bit(hh, lg);
+    atomicOr(&flt[(h0 >> 3) + (fb >> 5)], 1u << (fb & 31));
+}
+
+__global__ void k_hash_fill_edges(int64_t nL, const int32_t *__restrict__ eu, const int32_t *__restrict__ ev,
+                                  const int64_t *__restrict__ hoff, int2 *__restrict__ tab,
+                                  uint32_t *__restrict__ flt) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nL; e += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t u = eu[e], v = ev[e];
+        table_put(hoff, tab, flt, u, v, (int32_t)e);
+        table_put(hoff, tab, flt, v, u, (int32_t)e);
+    }
+}
+
+// Edge id of every entry of the symmetric pattern A, without searching:
+// row i's lower entries (columns j < i, first in the row) are L's row i in
+// order, so edge e of L sits at rst[eu[e]] + e - lrst[eu[e]]; row i's upper
+// entries (columns j > i, ascending) are the L edges (j, i) by ascending j,
+// which is L's edges sorted stably by column (key ev) -- the upper parts of
+// all rows one after another, so the q-th of them (column skey[q]) sits at
+// q + lrst[skey[q] + 1] (the entries before it: rst[i] of the rows before,
+// lrst[i + 1] of them lower).  Both halves are flat over the edges, so a hub
+// row costs no single wave a long chain (Chung-Lu 4M: 30.2 ms with one
+// wave per row and a bisection per upper entry).  eid2, if given, gets the
+// same ids.
+__global__ void k_sym_eid_sorted(int64_t nL, const int32_t *__restrict__ rst, const int32_t *__restrict__ lrst,
+                                 const int32_t *__restrict__ eu, const int32_t *__restrict__ skey,
+                                 const int32_t *__restrict__ sval, int32_t *__restrict__ eid,
+                                 int32_t *__restrict__ eid2) {
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < nL; q += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t i = eu[q];
+        const int64_t pl = (int64_t)rst[i] + (q - lrst[i]);
+        const int64_t pu = q + (int64_t)lrst[skey[q] + 1];
+        const int32_t e = sval[q];
+        eid[pl] = (int32_t)q;
+        eid[pu] = e;
+        if (eid2) {
+            eid2[pl] = (int32_t)q;
+            eid2[pu] = e;
         }
     }
 }
@@ -1247,6 +1284,17 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
     int32_t *NL, *NL2;
     GK_TRY(B.alloc(&NL, nL));
     GK_TRY(B.alloc(&NL2, nL));
+    // L's edges sorted by column, for the edge ids of A's entries
+    int32_t *skey, *sval;
+    uint8_t *sort_tmp;
+    GK_TRY(B.alloc(&skey, nL));
+    GK_TRY(B.alloc(&sval, nL));
+    int sort_end_bit = 1;
+    while (sort_end_bit < 32 && ((int64_t)1 << sort_end_bit) < m) ++sort_end_bit;
+    size_t sort_bytes = 0;
+    GK_HIP(rocprim::radix_sort_pairs(nullptr, sort_bytes, ev, skey, alist, sval, (size_t)std::max<int64_t>(nL, 1), 0,
+                                     sort_end_bit, ctx->side));
+    GK_TRY(B.alloc(&sort_tmp, sort_bytes + 16));
     for (int b = 0; b < 2; ++b) {
         GK_TRY(B.alloc(&arst[b], m + 1));
         GK_TRY(B.alloc(&acol[b], A->nz));
@@ -1281,16 +1329,24 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
     GK_HIP(hipStreamWaitEvent(ctx->side, ctx->side_fork, 0));
     {
         hipStream_t s = ctx->side;
+        const int ge = grid_items(ctx, nL);
         k_edge_ends<<<gw, 256, 0, s>>>(m, L0.rst, L0.col, eu, ev);
         GK_HIP(hipMemcpyAsync(arst[0], A->rst, 4 * (m + 1), hipMemcpyDeviceToDevice, s));
         GK_HIP(hipMemcpyAsync(acol[0], A->col, 4 * (size_t)A->nz, hipMemcpyDeviceToDevice, s));
-        k_sym_eid<<<gw, 256, 0, s>>>(m, A->rst, A->col, L0.rst, aeid[0], sym_eid_out ? *sym_eid_out : nullptr);
-        k_iota<<<grid_items(ctx, nL), 256, 0, s>>>(nL, alist);
+        k_iota<<<ge, 256, 0, s>>>(nL, alist);
         GK_HIP(hipMemsetAsync(st, 0, 4 * (size_t)std::max<int64_t>(st_words, 1), s));
         GK_HIP(hipMemsetAsync(dctr, 0, 16, s));
         GK_HIP(hipMemsetAsync(tab, 0xff, sizeof(int2) * (size_t)std::max<int64_t>(tab_n, 1), s));
         GK_HIP(hipMemsetAsync(tflt, 0, 4 * (size_t)std::max<int64_t>(tab_n / 8, 1), s));
-        k_hash_fill<<<gw, 256, 0, s>>>(m, A->rst, A->col, aeid[0], hoff, tab, tflt);
+        k_hash_fill_edges<<<ge, 256, 0, s>>>(nL, eu, ev, hoff, tab, tflt);
+        // edge ids of A's entries: L's edges sorted stably by column (the
+        // alive list, still 0 .. |L|-1, is the sort's edge ids)
+        if (nL > 0) {
+            GK_HIP(rocprim::radix_sort_pairs(sort_tmp, sort_bytes, ev, skey, alist, sval, (size_t)nL, 0, sort_end_bit,
+                                             s));
+            k_sym_eid_sorted<<<ge, 256, 0, s>>>(nL, A->rst, L0.rst, eu, skey, sval, aeid[0],
+                                                sym_eid_out ? *sym_eid_out : nullptr);
+        }
         GK_TRY(launch_check("truss_peel setup"));
         GK_HIP(hipEventRecord(ctx->side_join, s));
     }
