@@ -1169,11 +1169,18 @@ __global__ void k_mark_dead(int32_t nF, const int32_t *__restrict__ F, uint32_t 
 }
 
 // Compaction of the alive adjacency (rows stay sorted).
+// Adjacency compaction: rows of up to kLongRow entries one wave each, longer
+// ones (a power-law graph's hubs: 10^5 entries would be one wave's chain of
+// thousands of dependent steps) one 1024-thread workgroup each.
+constexpr int32_t kLongRow = 4096;
+constexpr int kLongThreads = 1024;
+
 __global__ void k_alive_count(int64_t m, const int32_t *__restrict__ rst, const int32_t *__restrict__ eid,
                               const uint32_t *__restrict__ st, int32_t *__restrict__ cnt) {
     const int lane = threadIdx.x & 63;
     const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
     for (int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < m; i += waves) {
+        if (rst[i + 1] - rst[i] > kLongRow) continue;
         int32_t c = 0;
         for (int32_t p = rst[i] + lane; p < rst[i + 1]; p += 64) c += st_get(st, eid[p]) != 2;
         c = dev::wave_sum(c);
@@ -1189,6 +1196,7 @@ __global__ void k_alive_scatter(int64_t m, const int32_t *__restrict__ rst, cons
     const int lane = threadIdx.x & 63;
     const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
     for (int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < m; i += waves) {
+        if (rst[i + 1] - rst[i] > kLongRow) continue;
         int32_t out = nrst[i];
         for (int32_t base = rst[i]; base < rst[i + 1]; base += 64) {
             int32_t p = base + lane;
@@ -1201,6 +1209,62 @@ __global__ void k_alive_scatter(int64_t m, const int32_t *__restrict__ rst, cons
             }
             out += __popcll(bal);
         }
+    }
+}
+
+// The rows longer than kLongRow, one workgroup each: workgroup b reads the
+// row pointers of rows b, b + grid, b + 2 grid, ... (with degree-ordered
+// ids the long rows are the first ones, so they spread over the
+// workgroups), lists its long rows in LDS, and counts (SCATTER false) or
+// compacts (SCATTER true, tile by tile in order) them.
+template <bool SCATTER>
+__global__ __launch_bounds__(kLongThreads) void k_alive_long(int64_t m, const int32_t *__restrict__ rst,
+                                                             const int32_t *__restrict__ col,
+                                                             const int32_t *__restrict__ eid,
+                                                             const uint32_t *__restrict__ st,
+                                                             int32_t *__restrict__ cnt,
+                                                             const int32_t *__restrict__ nrst,
+                                                             int32_t *__restrict__ ncol, int32_t *__restrict__ neid) {
+    constexpr int NW = kLongThreads / 64;
+    __shared__ int32_t rows[kLongThreads];
+    __shared__ int32_t nrows;
+    __shared__ int32_t ws[NW];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    for (int64_t k0 = 0; (int64_t)blockIdx.x + k0 * gridDim.x < m; k0 += kLongThreads) {
+        if (tid == 0) nrows = 0;
+        __syncthreads();
+        const int64_t i = (int64_t)blockIdx.x + (k0 + tid) * gridDim.x;
+        if (i < m && rst[i + 1] - rst[i] > kLongRow) rows[atomicAdd(&nrows, 1)] = (int32_t)i;
+        __syncthreads();
+        const int32_t nr = nrows;
+        for (int32_t r = 0; r < nr; ++r) {
+            const int32_t row = rows[r];
+            const int32_t b0 = rst[row], b1 = rst[row + 1];
+            int32_t out = SCATTER ? nrst[row] : 0;
+            for (int32_t base = b0; base < b1; base += kLongThreads) {
+                const int32_t p = base + tid;
+                const bool k = p < b1 && st_get(st, eid[p]) != 2;
+                const uint64_t bal = __ballot(k);
+                if (lane == 0) ws[w] = __popcll(bal);
+                __syncthreads();
+                int32_t before = 0, tot = 0;
+#pragma unroll
+                for (int x = 0; x < NW; ++x) {
+                    const int32_t v = ws[x];
+                    before += x < w ? v : 0;
+                    tot += v;
+                }
+                if (SCATTER && k) {
+                    const int32_t o = out + before + __popcll(bal & ((1ull << lane) - 1ull));
+                    ncol[o] = col[p];
+                    neid[o] = eid[p];
+                }
+                out += tot;
+                __syncthreads();
+            }
+            if (!SCATTER && tid == 0) cnt[row] = out;
+        }
+        __syncthreads();
     }
 }
 
@@ -1551,10 +1615,16 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
             k_mark_dead<<<grid_items(ctx, nP), 256, 0, ctx->stream>>>(nP, fb[fc ^ 1], st);
             nP = 0;
             const int nx = cur ^ 1;
+            const int gl = (int)std::max<int64_t>(1, std::min<int64_t>((m + kLongThreads - 1) / kLongThreads,
+                                                                       (int64_t)ctx->num_cus * 2));
             k_alive_count<<<gw, 256, 0, ctx->stream>>>(m, arst[cur], aeid[cur], st, cnt);
+            k_alive_long<false><<<gl, kLongThreads, 0, ctx->stream>>>(m, arst[cur], acol[cur], aeid[cur], st, cnt,
+                                                                     nullptr, nullptr, nullptr);
             GK_TRY(exclusive_scan_i32(ctx, cnt, arst[nx], m + 1));
             k_alive_scatter<<<gw, 256, 0, ctx->stream>>>(m, arst[cur], acol[cur], aeid[cur], st, arst[nx], acol[nx],
                                                          aeid[nx]);
+            k_alive_long<true><<<gl, kLongThreads, 0, ctx->stream>>>(m, arst[cur], acol[cur], aeid[cur], st, cnt,
+                                                                    arst[nx], acol[nx], aeid[nx]);
             GK_HIP(hipMemsetAsync(dac, 0, 4, ctx->stream));
             k_alist_compact<<<grid_scan(ctx, na), 256, 0, ctx->stream>>>(na, alist, st, alist2, dac);
             GK_TRY(launch_check("compaction"));
