@@ -643,6 +643,47 @@ __device__ __forceinline__ void stage_flush(Stage &S, int32_t *__restrict__ out,
     }
 }
 
+// Both stages of a workgroup copied out together when each holds at most NT
+// entries (the usual case): one pass, the two counters' atomics issued by
+// different waves at once instead of one flush after the other.
+template <int NT>
+__device__ __forceinline__ void stage_flush2(Stage &A, int32_t *__restrict__ outA, int32_t *__restrict__ cntA,
+                                             Stage &B, int32_t *__restrict__ outB, int32_t *__restrict__ cntB,
+                                             int32_t *lds) {
+    constexpr int NW = NT / 64;
+    static_assert(NW >= 2, "two waves issue the two atomics");
+    __syncthreads();
+    const int32_t nA = min(A.n, kStage), nB = min(B.n, kStage);
+    if (nA > NT || nB > NT) {
+        stage_flush<NT>(A, outA, cntA, lds);
+        stage_flush<NT>(B, outB, cntB, lds);
+        return;
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int32_t t = (int32_t)threadIdx.x;
+    const int32_t eA = t < nA ? A.v[t] : -1, eB = t < nB ? B.v[t] : -1;
+    const uint64_t bA = __ballot(eA >= 0), bB = __ballot(eB >= 0);
+    if (lane == 0) {
+        lds[w] = __popcll(bA);
+        lds[NW + 1 + w] = __popcll(bB);
+    }
+    __syncthreads();
+    if (lane == 0 && w < 2) {  // wave 0: A's counter, wave 1: B's
+        int32_t *l = lds + (w ? NW + 1 : 0);
+        int32_t tot = 0;
+#pragma unroll
+        for (int x = 0; x < NW; ++x) {
+            const int32_t c = l[x];
+            l[x] = tot;
+            tot += c;
+        }
+        l[NW] = tot ? atomicAdd(w ? cntB : cntA, tot) : 0;
+    }
+    __syncthreads();
+    if (eA >= 0) outA[lds[NW] + lds[w] + __popcll(bA & ((1ull << lane) - 1ull))] = eA;
+    if (eB >= 0) outB[lds[2 * NW + 1] + lds[NW + 1 + w] + __popcll(bB & ((1ull << lane) - 1ull))] = eB;
+}
+
 // Appends the non-negative entries of every lane's v to out, one atomic on
 // *counter per wave (none when the wave has nothing).
 template <int N>
@@ -678,7 +719,7 @@ __device__ __forceinline__ void k_peel_body(int32_t nF, int32_t total, const int
     // and its count of destroyed triangles
     __shared__ Stage xs, ns;
     __shared__ unsigned long long hits;
-    __shared__ int32_t lds[256 / 64 + 1];
+    __shared__ int32_t lds[2 * (256 / 64 + 1)];
     if (threadIdx.x == 0) {
         xs.n = ns.n = 0;
         hits = 0;
@@ -874,8 +915,7 @@ __device__ __forceinline__ void k_peel_body(int32_t nF, int32_t total, const int
     }
     unsigned long long s = dev::wave_sum((unsigned long long)cnt);
     if (lane == 0 && s) atomicAdd(&hits, s);
-    stage_flush<256>(xs, Fn, nFn, lds);
-    stage_flush<256>(ns, nl.list, nl.n, lds);
+    stage_flush2<256>(xs, Fn, nFn, ns, nl.list, nl.n, lds);
     if (threadIdx.x == 0) {
         if (hits) atomicAdd(&ctr[1], hits);
         if (blockIdx.x == 0) atomicAdd(&ctr[0], (unsigned long long)total);
