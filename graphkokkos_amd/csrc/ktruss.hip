@@ -22,7 +22,6 @@
 #include <cstddef>
 #include <cstring>
 
-#include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 
 #include "gk_device.h"
@@ -380,31 +379,32 @@ __global__ void k_hash_fill_edges(int64_t nL, const int32_t *__restrict__ eu, co
     }
 }
 
-// Edge id of every entry of the symmetric pattern A, without searching:
-// row i's lower entries (columns j < i, first in the row) are L's row i in
-// order, so edge e of L sits at rst[eu[e]] + e - lrst[eu[e]]; row i's upper
-// entries (columns j > i, ascending) are the L edges (j, i) by ascending j,
-// which is L's edges sorted stably by column (key ev) -- the upper parts of
-// all rows one after another, so the q-th of them (column skey[q]) sits at
-// q + lrst[skey[q] + 1] (the entries before it: rst[i] of the rows before,
-// lrst[i + 1] of them lower).  Both halves are flat over the edges, so a hub
-// row costs no single wave a long chain (Chung-Lu 4M: 30.2 ms with one
-// wave per row and a bisection per upper entry).  eid2, if given, gets the
-// same ids.
-__global__ void k_sym_eid_sorted(int64_t nL, const int32_t *__restrict__ rst, const int32_t *__restrict__ lrst,
-                                 const int32_t *__restrict__ eu, const int32_t *__restrict__ skey,
-                                 const int32_t *__restrict__ sval, int32_t *__restrict__ eid,
-                                 int32_t *__restrict__ eid2) {
-    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < nL; q += (int64_t)gridDim.x * blockDim.x) {
-        const int32_t i = eu[q];
-        const int64_t pl = (int64_t)rst[i] + (q - lrst[i]);
-        const int64_t pu = q + (int64_t)lrst[skey[q] + 1];
-        const int32_t e = sval[q];
-        eid[pl] = (int32_t)q;
-        eid[pu] = e;
+// Edge id of every entry of the symmetric pattern A, flat over L's edges:
+// edge e = (u, v), v < u, is entry rst[u] + e - lrst[u] of A (row u's lower
+// entries are L's row u in order) and, in row v, the entry whose column is u
+// (a bisection in row v).  One wave per row of A instead made a hub's row of
+// 10^5 entries one wave's chain of bisections while the GPU idled
+// (Chung-Lu 4M: 30.2 ms); sorting L's edges by column instead (the upper
+// parts in order, no search) stalled its look-back passes beside the support
+// product's kernels (R-MAT s20: first round 5 ms later).  eid2, if given,
+// gets the same ids.
+__global__ void k_sym_eid_edges(int64_t nL, const int32_t *__restrict__ rst, const int32_t *__restrict__ col,
+                                const int32_t *__restrict__ lrst, const int32_t *__restrict__ eu,
+                                const int32_t *__restrict__ ev, int32_t *__restrict__ eid,
+                                int32_t *__restrict__ eid2) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nL; e += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t u = eu[e], v = ev[e];
+        const int64_t pl = (int64_t)rst[u] + (e - lrst[u]);
+        int32_t l = rst[v], h = rst[v + 1];
+        while (l < h) {  // first entry of row v with column >= u
+            const int32_t mid = l + ((h - l) >> 1);
+            if (col[mid] < u) l = mid + 1; else h = mid;
+        }
+        eid[pl] = (int32_t)e;
+        eid[l] = (int32_t)e;
         if (eid2) {
-            eid2[pl] = (int32_t)q;
-            eid2[pu] = e;
+            eid2[pl] = (int32_t)e;
+            eid2[l] = (int32_t)e;
         }
     }
 }
@@ -1390,17 +1390,6 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
     int32_t *NL, *NL2;
     GK_TRY(B.alloc(&NL, nL));
     GK_TRY(B.alloc(&NL2, nL));
-    // L's edges sorted by column, for the edge ids of A's entries
-    int32_t *skey, *sval;
-    uint8_t *sort_tmp;
-    GK_TRY(B.alloc(&skey, nL));
-    GK_TRY(B.alloc(&sval, nL));
-    int sort_end_bit = 1;
-    while (sort_end_bit < 32 && ((int64_t)1 << sort_end_bit) < m) ++sort_end_bit;
-    size_t sort_bytes = 0;
-    GK_HIP(rocprim::radix_sort_pairs(nullptr, sort_bytes, ev, skey, alist, sval, (size_t)std::max<int64_t>(nL, 1), 0,
-                                     sort_end_bit, ctx->side));
-    GK_TRY(B.alloc(&sort_tmp, sort_bytes + 16));
     for (int b = 0; b < 2; ++b) {
         GK_TRY(B.alloc(&arst[b], m + 1));
         GK_TRY(B.alloc(&acol[b], A->nz));
@@ -1445,14 +1434,8 @@ int truss_peel(gk_ctx *ctx, const gk_dcsr *A, TrussBufs &B, const Lgraph &L0, ui
         GK_HIP(hipMemsetAsync(tab, 0xff, sizeof(int2) * (size_t)std::max<int64_t>(tab_n, 1), s));
         GK_HIP(hipMemsetAsync(tflt, 0, 4 * (size_t)std::max<int64_t>(tab_n / 8, 1), s));
         k_hash_fill_edges<<<ge, 256, 0, s>>>(nL, eu, ev, hoff, tab, tflt);
-        // edge ids of A's entries: L's edges sorted stably by column (the
-        // alive list, still 0 .. |L|-1, is the sort's edge ids)
-        if (nL > 0) {
-            GK_HIP(rocprim::radix_sort_pairs(sort_tmp, sort_bytes, ev, skey, alist, sval, (size_t)nL, 0, sort_end_bit,
-                                             s));
-            k_sym_eid_sorted<<<ge, 256, 0, s>>>(nL, A->rst, L0.rst, eu, skey, sval, aeid[0],
-                                                sym_eid_out ? *sym_eid_out : nullptr);
-        }
+        k_sym_eid_edges<<<ge, 256, 0, s>>>(nL, A->rst, A->col, L0.rst, eu, ev, aeid[0],
+                                           sym_eid_out ? *sym_eid_out : nullptr);
         GK_TRY(launch_check("truss_peel setup"));
         GK_HIP(hipEventRecord(ctx->side_join, s));
     }
