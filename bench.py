@@ -237,7 +237,9 @@ def bc_bytes(n: int, nnz: int, levels: int, nsrc: int) -> float:
 # first set (MCA-wave up to 8, the short-row window kernel up to 128) pick
 # each row's accumulator and every B row is streamed (hub bitmaps off).  The
 # library default ("heuristic_hub") adds hub bitmaps with bit tests, sends
-# masks up to 16 to MCA-wave and up to 32 to the short-row window kernel.
+# masks up to 48 to MCA-wave and up to 32 to the short-row window kernel, and
+# runs 16 long-row and 16 MCA workgroups per CU (triangle counting's own
+# defaults, found by the autotuner on this graph).
 PLAIN = {"hub_window": 0, "mca_max_mask": 8, "small_max_mask": 128}
 
 

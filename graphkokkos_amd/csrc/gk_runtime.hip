@@ -226,21 +226,21 @@ extern "C" void gk_opts_default(gk_opts *o) {
     o->struct_size = sizeof(gk_opts);
     o->selector = GK_SEL_HEURISTIC;
     o->forced_accum = GK_ACC_MSA_LDS;
-    o->mca_max_mask = 16;
+    o->mca_max_mask = -1;  // the product mode's default (mode_opts)
     o->lds_max_mask = 1536;
     o->dense_min_mask = 16384;
     o->split_flop = 1 << 18;
     o->block_threads = 64;
     o->small_max_mask = 32;
     o->blocks_small = 24;
-    o->blocks_long = 8;
-    o->blocks_mca = 0;  // the engine's default (8), the batching step's 32
+    o->blocks_long = 0;    // the product mode's default
+    o->blocks_mca = 0;     // the product mode's default; the batching step's 32
     o->blocks_global = 4;
     o->batch_blocks = 8;
     o->batch_order = 0;
     o->hub_window = 32768;
     o->learned_margin = 250;
-    o->hub_probe = 4;
+    o->hub_probe = -1;     // the product mode's default
     o->selector_labels = 1;
 }
 

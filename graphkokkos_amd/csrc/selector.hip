@@ -588,7 +588,7 @@ extern "C" int gk_model_train(gk_ctx *ctx, const float *X, const float *T, const
 extern "C" int gk_selector_samples(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcsr *M, int mode,
                                    float *X, float *T, uint8_t *valid, const gk_opts *opts) {
     GK_CHECK_ARG(ctx && A && B && M && X && T && valid, "null argument");
-    const gk_opts base = resolve_opts(opts);
+    const gk_opts base = mode_opts(resolve_opts(opts), mode == 0 ? MODE_TC : MODE_VALUES);
     GK_CHECK_ARG(mode == 0 || mode == 1, "mode must be 0 (triangle counting) or 1 (values)");
     GK_TRY(require_sorted(M, "M"));
     const int64_t m = A->m;

@@ -206,6 +206,13 @@ int build_hub(gk_ctx *ctx, const gk_dcsr *B, int32_t hub_n, Prob *P);
 // Hub rows for C<M> = A*B in the given mode under the options (0: none).
 int32_t hub_rows(const gk_dcsr *B, int mode, const gk_opts &o);
 int plan_tasks(gk_ctx *ctx, const gk_dcsr *A, RowStats &st, const SelectParams &sp, Plan *plan);
+// The options a product of the given mode runs with: the fields left at -1
+// (mca_max_mask, hub_probe) or 0 (blocks_long, blocks_mca) take the mode's
+// default.  Triangle counting: MCA-wave up to mask length 48, hub probes at
+// weight 2, 16 long-row and 16 MCA workgroups per CU (the set autotuned on
+// R-MAT scale 22, also the fastest on the 2048^2 grid and the 4M-vertex
+// Chung-Lu graph); support and value products: 16, 2, 8, 8.
+gk_opts mode_opts(const gk_opts &o, int mode);
 SelectParams select_params(const gk_opts &o);
 int zero_u32(gk_ctx *ctx, void *p, int64_t n);
 

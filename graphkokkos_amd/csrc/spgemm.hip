@@ -1579,6 +1579,16 @@ static int run_mode(gk_ctx *ctx, const Prob &P, const Plan &plan, int64_t bn, co
     return GK_OK;
 }
 
+gk_opts mode_opts(const gk_opts &o, int mode) {
+    const bool tc = mode == MODE_TC;
+    gk_opts r = o;
+    if (r.mca_max_mask < 0) r.mca_max_mask = tc ? 48 : 16;
+    if (r.hub_probe < 0) r.hub_probe = 2;
+    if (r.blocks_long <= 0) r.blocks_long = tc ? 16 : kWindowBlocksLong;
+    if (r.blocks_mca <= 0) r.blocks_mca = tc ? 16 : kMcaBlocksPerCu;
+    return r;
+}
+
 SelectParams select_params(const gk_opts &o) {
     SelectParams sp;
     sp.selector = o.selector;
@@ -1595,8 +1605,9 @@ SelectParams select_params(const gk_opts &o) {
 }
 
 int masked_product(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcsr *M, int mode, int32_t *d_out,
-                   unsigned long long *d_count, unsigned long long *d_rowcount, const gk_opts &o,
+                   unsigned long long *d_count, unsigned long long *d_rowcount, const gk_opts &o_in,
                    ProductInfo *info, long long *task_cycles, Plan *plan_out) {
+    const gk_opts o = mode_opts(o_in, mode);
     if (A->n != B->m || A->m != M->m || B->n != M->n) {
         set_error("shape mismatch: A %lldx%lld, B %lldx%lld, M %lldx%lld", (long long)A->m, (long long)A->n,
                   (long long)B->m, (long long)B->n, (long long)M->m, (long long)M->n);
@@ -1666,7 +1677,8 @@ int masked_product(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcs
 }
 
 int prepare_product(gk_ctx *ctx, const gk_dcsr *A, const gk_dcsr *B, const gk_dcsr *M, int32_t *d_out,
-                    const gk_opts &o, Prob *P, ClassLaunch *L) {
+                    const gk_opts &o_in, Prob *P, ClassLaunch *L) {
+    const gk_opts o = mode_opts(o_in, MODE_SUPPORT);
     if (A->n != B->m || A->m != M->m || B->n != M->n) {
         set_error("shape mismatch in a prepared product");
         return GK_EINVAL;

@@ -92,7 +92,9 @@ typedef struct gk_opts {
     int32_t struct_size;    /* sizeof(gk_opts); 0 or a NULL opts means defaults */
     int32_t selector;       /* enum gk_selector                           */
     int32_t forced_accum;   /* enum gk_accum, used with GK_SEL_FORCED     */
-    int32_t mca_max_mask;   /* heuristic: mask rows up to this use MCA (default 16) */
+    int32_t mca_max_mask;   /* heuristic: mask rows up to this use MCA; -1 (default) =
+                             * the product's mode: 48 for triangle counting, 16 for
+                             * support and value products                            */
     int32_t lds_max_mask;   /* heuristic: mask rows up to this use MSA-LDS*/
     int32_t dense_min_mask; /* heuristic: mask rows from this use dense-global */
     int32_t split_flop;     /* rows above this many B entries are split   */
@@ -100,8 +102,10 @@ typedef struct gk_opts {
     int32_t verbose;
     int32_t small_max_mask; /* MSA-LDS rows up to this (<= 128) use the short-row kernel (default 32) */
     int32_t blocks_small;   /* persistent workgroups per CU, short-row MSA-LDS kernel */
-    int32_t blocks_long;    /* ... long-row MSA-LDS kernel                           */
-    int32_t blocks_mca;     /* ... MCA-wave kernel (0: 8; the batching step, whose
+    int32_t blocks_long;    /* ... long-row MSA-LDS kernel (0: 16 for triangle
+                             * counting, 8 for support and value products)       */
+    int32_t blocks_mca;     /* ... MCA-wave kernel (0: 16 for triangle counting, 8 for
+                             * support and value products; the batching step, whose
                              * support product is mostly short rows, 32)         */
     int32_t blocks_global;  /* ... hash-global and dense-global kernels              */
     int32_t batch_blocks;   /* batching: peeling workgroups per CU                   */
@@ -118,7 +122,7 @@ typedef struct gk_opts {
                              * single bits of a hub row where hub_probe times the mask
                              * row's columns below it is less than the row's words and
                              * entries (a test is a gathered word, the others are
-                             * streamed); 0 = off, default 4                         */
+                             * streamed); 0 = off, -1 (default) = 2                  */
     int32_t selector_labels;/* gk_selector_samples: 1 = concurrent (MSA-LDS and MCA-wave
                              * timed together on alternating halves of the rows, the
                              * engine's real schedule), 0 = each accumulator forced

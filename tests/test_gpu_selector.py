@@ -66,19 +66,21 @@ def test_samples_learned_selection_and_gain(gpu):
     rows_l = gpu.last_timing()["rows_per_acc"]
     gpu.tc_count(d)
     assert rows_l == gpu.last_timing()["rows_per_acc"]
-    # The measured gain of the trained network over the fixed thresholds (the
-    # library default), on a graph large enough to be throughput-bound: the
-    # labels are wave-time, a throughput cost, and products of half a
-    # millisecond are bound by latency instead.  Trained on these small
-    # graphs, the network moves the mask rows of 17-48 columns from MSA-LDS
-    # to MCA-wave, as the autotuner's search over thresholds does.
+    # The measured gain of the trained network over fixed thresholds that send
+    # mask rows up to 16 columns to MCA-wave (the support product's default),
+    # on a graph large enough to be throughput-bound: the labels are
+    # wave-time, a throughput cost, and products of half a millisecond are
+    # bound by latency instead.  Trained on these small graphs, the network
+    # moves the mask rows of 17-48 columns from MSA-LDS to MCA-wave, as the
+    # autotuner's search over thresholds does (triangle counting's default
+    # threshold, 48, came from that search).
     G = HostGraph.rmat(19, 16, seed=9)
     d = gpu.upload(G.L)
     want = gpu.tc_count(d)
     assert gpu.tc_count(d, {"selector": SEL_LEARNED}) == want
-    t_h = selector.time_tc(gpu, d, None, reps=7)
+    t_h = selector.time_tc(gpu, d, {"mca_max_mask": 16}, reps=7)
     rows_h = gpu.last_timing()["rows_per_acc"]
-    t_l = selector.time_tc(gpu, d, {"selector": SEL_LEARNED}, reps=7)
+    t_l = selector.time_tc(gpu, d, {"selector": SEL_LEARNED, "mca_max_mask": 16}, reps=7)
     rows_l = gpu.last_timing()["rows_per_acc"]
     print(f"R-MAT s19 triangle counting: heuristic {t_h * 1e3:.3f} ms (rows MSA-LDS/MCA-wave {rows_h[:2]}), "
           f"learned {t_l * 1e3:.3f} ms ({rows_l[:2]}), learned/heuristic speedup {t_h / t_l:.3f}", flush=True)
@@ -90,8 +92,10 @@ def _heuristic_choice(X, V):
     """The fixed-threshold rule on the features' mask length, at the
     library's default thresholds (MCA-wave, MSA-LDS, dense-global)."""
     o = default_opts()
+    # -1: the product mode's default, 48 for triangle counting
+    mca = o.mca_max_mask if o.mca_max_mask >= 0 else 48
     mlen = np.round(2 ** X[:, 0] - 1).astype(np.int64)
-    c = np.where(mlen <= o.mca_max_mask, 1,
+    c = np.where(mlen <= mca, 1,
                  np.where(mlen <= o.lds_max_mask, 0, np.where(mlen >= o.dense_min_mask, 3, 2)))
     c = np.where(V[np.arange(len(c)), c].astype(bool), c, 0)  # capacity fallback
     return c
